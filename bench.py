@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Benchmark: member-steps/s of the forced modified-Euler spectral step, T42 x 256 members.
+
+Workload (synthetic, seeded): T42 (J=64, I=128), 256 ensemble members, forced
+modified Euler, dt=120 s, a=8.2e7, omega=3.2e-5, Phibar=3e5, K6=1.24e33,
+diffusion and modal filter on; per-member taurad, taudrag log-uniform in
+[1e4, 1e6] s and DPhieq uniform in [1e5, 5e5] (seed 7); rest + seeded Phi
+noise initial condition (member k: seed 1000+k).  Transforms in FP32, spectral
+state update and diffusion in FP64.  Inputs are resident in HBM before the
+timed region; each step replays the HIP graph driver.
+
+Protocol (scripts/benchmark_scan.py:116-141): W untimed warmup steps, then K
+timed steps bracketed by device synchronization; one JSON line on stdout.
+Beside it: the same batch in FP64, the plain FP64 path at one member, the
+per-kernel roofline from HIP events, and the FP64 CPU oracle timed on the host.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "member-steps/s at T42×256 ens; fwd+adjoint time vs plain FP64 GPU, same GPU"
+PEAK = {"fp32": 157.3, "fp64": 78.6}  # TFLOP/s, MI355X dense matrix peaks
+HBM_PEAK = 8000.0  # GB/s
+
+
+def member_params(B: int, seed: int = 7):
+    from my_swampe_amd.model import param_row
+
+    rng = np.random.default_rng(seed)
+    taurad = 10.0 ** rng.uniform(4.0, 6.0, B)
+    taudrag = 10.0 ** rng.uniform(4.0, 6.0, B)
+    dphieq = rng.uniform(1e5, 5e5, B)
+    return np.stack([param_row(dt=120.0, a=8.2e7, omega=3.2e-5, g=9.8, Phibar=3e5, taurad=taurad[b],
+                               taudrag=taudrag[b], DPhieq=dphieq[b], K6=1.24e33, K6Phi=None, alpha=0.01)
+                     for b in range(B)])
+
+
+def member_ics(M: int, B: int, omega: float = 3.2e-5):
+    """Rest state + seeded Phi noise on n <= 10 (std 100), member k uses seed 1000 + k."""
+    from oracle.swampe_oracle import rest_noise_ic  # IC synthesis only (host numpy, outside the timed region)
+
+    fields = [rest_noise_ic(M, 1000 + k, omega) for k in range(B)]
+    return [np.stack([f[i] for f in fields]) for i in range(5)]
+
+
+def flops_per_member_step(M: int, J: int, I: int):
+    """Algorithmic work per member-step (fixed formula; FP64-carry forward step).
+
+    Legendre: 18 triangular contractions (11 forward, 7 inverse), DFT: 13 dense
+    real contractions (8 forward, 5 inverse), complex x real = 4 flops per
+    multiply-add site.
+    """
+    L = (M + 1) * (M + 2) // 2
+    leg = 4 * J * 18 * L
+    dft = 4 * J * 13 * I * (M + 1)
+    return leg, dft
+
+
+def run_batch(M, B, precision, steps, warmup, graph_steps=32):
+    import torch
+    from my_swampe_amd.plan import Plan
+
+    params = member_params(B)
+    ics = member_ics(M, B)
+    plan = Plan(M, params, precision=precision, diagnostics=False)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        plan.init(*ics, stream=stream)
+        plan.run_graph(2, warmup, graph_steps=graph_steps, stream=stream)
+        stream.synchronize()
+        t0 = time.perf_counter()
+        plan.run_graph(2 + warmup, steps, graph_steps=graph_steps, stream=stream)
+        stream.synchronize()
+        wall = time.perf_counter() - t0
+        ms_leg, ms_grid = plan.time_kernels(2 + warmup + steps, 20, stream=stream)
+        ex = plan.export(fields=("Phi_curr",), stream=stream)
+        stream.synchronize()
+    finite = bool(torch.isfinite(ex["Phi_curr"]).all())
+    plan.close()
+    return dict(wall=wall, ms_leg=ms_leg, ms_grid=ms_grid, finite=finite)
+
+
+def cpu_baseline(seconds_target: float = 12.0):
+    """FP64 oracle (numpy, one core) on config 0: T42, 1 member, forced modEuler, dt=120."""
+    from oracle import swampe_oracle as orc
+
+    eta0, delta0, Phi0, U0, V0 = orc.rest_noise_ic(42, 0, 3.2e-5)
+    kw = dict(M=42, dt=120.0, Phibar=3e5, omega=3.2e-5, a=8.2e7, taurad=86400.0, taudrag=86400.0, DPhieq=3e5,
+              eta0_init=eta0, delta0_init=delta0, Phi0_init=Phi0, U0_init=U0, V0_init=V0, diagnostics=False,
+              return_history=False)
+    orc.run_model_scan(tmax=4, **kw)  # warm caches (geometry)
+    steps, t = 0, 0.0
+    while t < seconds_target:
+        t0 = time.perf_counter()
+        orc.run_model_scan(tmax=242, **kw)
+        t += time.perf_counter() - t0
+        steps += 240
+    return dict(value=steps / t, unit="member-steps/s", cores=1, kind="port",
+                sample=f"FP64 numpy oracle, T42 forced modEuler, 1 member, {steps} steps (config 0 x {steps // 240})")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--members", type=int, default=256)
+    ap.add_argument("--precision", default="fp32", choices=("fp32", "fp64"))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; members shard across GPUs with no per-step "
+                                   "exchange (see DESIGN.md), multi-GPU runs are not provided"}))
+        sys.exit(2)
+    import torch
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no ROCm GPU visible"}))
+        sys.exit(1)
+    M, J, I, B = 42, 64, 128, args.members
+    main_run = run_batch(M, B, args.precision, args.steps, args.warmup)
+    value = args.steps * B / main_run["wall"]
+    leg_f, dft_f = flops_per_member_step(M, J, I)
+    grid_tflops = dft_f * B / (main_run["ms_grid"] * 1e-3) / 1e12
+    leg_tflops = leg_f * B / (main_run["ms_leg"] * 1e-3) / 1e12
+    peak = PEAK[args.precision]
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "member-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": main_run["wall"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": f"{args.precision} transforms, fp64 state",
+        "data": "synthetic (seeded rest+noise ICs, per-member forcing params)",
+        "config": {"workload": f"T42x{B} forced modified-Euler forward, dt=120 s, graph-replayed steps",
+                   "members": B, "M": M, "J": J, "I": I, "precision": args.precision},
+        "roofline": {"bound": "mfma", "achieved": grid_tflops, "peak": peak, "unit": "TFLOP/s",
+                     "frac": grid_tflops / peak, "traffic": None,
+                     "kernel": "k_grid (inverse DFT + grid physics + forward DFT)",
+                     "kernel_ms": main_run["ms_grid"], "leg_kernel_ms": main_run["ms_leg"],
+                     "leg_tflops": leg_tflops},
+        "finite": main_run["finite"],
+    }
+    if not args.no_extra:
+        fp64 = run_batch(M, B, "fp64", max(args.steps // 4, 50), 10)
+        plain = run_batch(M, 1, "fp64", max(args.steps // 2, 200), 20)
+        fp64_v = max(args.steps // 4, 50) * B / fp64["wall"]
+        plain_v = max(args.steps // 2, 200) / plain["wall"]
+        out["extra"] = {
+            "fp64_batched_member_steps_per_s": fp64_v,
+            "plain_fp64_1member_member_steps_per_s": plain_v,
+            "ratio_vs_plain_fp64_1member": value / plain_v,
+            "ratio_vs_fp64_same_batch": value / fp64_v,
+            "fp64_batched_kernel_ms": {"leg": fp64["ms_leg"], "grid": fp64["ms_grid"]},
+            "plain_kernel_ms": {"leg": plain["ms_leg"], "grid": plain["ms_grid"]},
+        }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,168 @@
+/*
+ * swampe_capi.h -- C ABI of the MI355X spectral shallow-water time-step core.
+ *
+ * The reference (SWAMPE-JAX, package my_swampe) is pure Python/JAX with no FFI;
+ * its drop-in boundary is the Python API in src/my_swampe/model.py.  This
+ * library is what that boundary calls: the Python host module
+ * my_swampe_amd.model (run_model_scan / run_model_scan_final / run_model)
+ * binds these entry points with ctypes.  Each entry point cites the reference
+ * function whose device work it replaces.
+ *
+ * Conventions
+ *   - extern "C", plain pointers and sizes; status 0 = ok, < 0 = error (see
+ *     swx_last_error()).  Nothing throws across the ABI.
+ *   - Device arrays are C-order, FP64 unless stated.  Complex arrays are
+ *     complex128 stored as interleaved (re, im) doubles.
+ *   - Grid (J, I): j = Gauss node, mu ascending; i = longitude from -pi.
+ *     Fourier (J, M+1); spectral (M+1, N+1) [m][n], zero for n < m.
+ *   - A plan owns its device buffers and is bound to the device that was
+ *     current when it was created.  It is not re-entrant: calls on one plan
+ *     must be serialized by the caller (one stream at a time).
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ */
+#ifndef SWAMPE_CAPI_H
+#define SWAMPE_CAPI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWX_OK 0
+#define SWX_ERR_ARG (-1)
+#define SWX_ERR_HIP (-2)
+#define SWX_ERR_STATE (-3)
+
+/* Transform precision.  The spectral state update and the diffusion filter
+ * are FP64 in every mode; only the DFT/Legendre contractions change. */
+#define SWX_PREC_FP64 0
+#define SWX_PREC_FP32 1
+#define SWX_PREC_BF16X3 2
+
+/* Per-member physical parameters: SWX_NPARAM doubles per member, in this
+ * order.  Mirrors the scalar arguments of run_model_scan
+ * (src/my_swampe/model.py:1345-1388); K6PHI < 0 means "same as K6"
+ * (model.py:513). */
+enum {
+  SWX_P_DT = 0,
+  SWX_P_A,
+  SWX_P_OMEGA,
+  SWX_P_G,
+  SWX_P_PHIBAR,
+  SWX_P_TAURAD,
+  SWX_P_TAUDRAG,
+  SWX_P_DPHIEQ,
+  SWX_P_K6,
+  SWX_P_K6PHI,
+  SWX_P_ALPHA,
+  SWX_P_BLOWUP,
+  SWX_NPARAM
+};
+
+typedef struct swx_plan swx_plan;
+
+typedef struct swx_plan_desc {
+  int M;           /* truncation, 42, 63 or 106 (N = M); initial_conditions.py:50-85 */
+  int n_members;   /* ensemble members sharing this truncation (>= 1) */
+  int precision;   /* SWX_PREC_* */
+  int expflag;     /* 0 = modified Euler, 1 = explicit (time_stepping.py:384) */
+  int forcflag;    /* model.py:177-181 RunFlags */
+  int diffflag;
+  int modalflag;
+  int diagnostics; /* per-step RMS/min/max reductions and blow-up gate */
+  int test;        /* -1 = None (forced), 1 or 2 (Williamson tests) */
+  int exact_basis; /* 0 = reference normalization (default; see swx_geometry),
+                      1 = exactly orthonormal basis at every (m, n) */
+} swx_plan_desc;
+
+/* Pointers (device) receiving the reference's 27-field scan carry
+ * (src/my_swampe/model.py:380-425), each for all members.  Any pointer may
+ * be NULL to skip that field. */
+typedef struct swx_state_out {
+  /* complex128 [B][J][M+1] */
+  double *etam_prev, *etam_curr, *deltam_prev, *deltam_curr, *Phim_prev, *Phim_curr;
+  /* float64 [B][J][I] */
+  double *eta_prev, *eta_curr, *delta_prev, *delta_curr, *Phi_prev, *Phi_curr, *U_curr, *V_curr;
+  /* complex128 [B][J][M+1] */
+  double *Um_curr, *Vm_curr, *Am_curr, *Bm_curr, *Cm_curr, *Dm_curr, *Em_curr;
+  double *PhiFm_curr, *Fm_curr, *Gm_curr, *Rum_lag, *Rvm_lag;
+  /* uint8 [B] */
+  unsigned char *dead;
+  /* complex128 [B][3][M+1][N+1]: spectral eta, delta, Phi of the current
+   * level.  No reference counterpart (the reference never keeps them). */
+  double *spec;
+} swx_state_out;
+
+/* Host-only FP64 geometry for truncation M (no GPU needed).
+ * Replaces spectral_params + gauss_legendre + PmnHmn
+ * (initial_conditions.py:50-85; spectral_transform.py:54-114, :219-287).
+ * Outputs (any may be NULL): mus[J], w[J], lambdas[I], Pmn[J][M+1][N+1],
+ * Hmn[J][M+1][N+1].  With exact_basis = 0 the basis reproduces the
+ * reference's factorial normalization, which rounds to a subnormal or zero
+ * scale for n + m >= ~170 (only reachable at M = 106); exact_basis = 1 gives
+ * the exactly orthonormal basis everywhere.  SWX_ERR_ARG for an unsupported M. */
+int swx_geometry(int M, int exact_basis, double *mus, double *w, double *lambdas, double *Pmn, double *Hmn);
+
+/* Grid sizes for truncation M: *J, *I (either may be NULL). */
+int swx_grid_size(int M, int *J, int *I);
+
+/* Create a plan: precompute the FP64 geometry, upload the basis and twiddle
+ * tables in the requested precision, and allocate the device carry.
+ * params: host [n_members][SWX_NPARAM].  Replaces build_static
+ * (model.py:457-557). */
+int swx_plan_create(const swx_plan_desc *desc, const double *params, swx_plan **out);
+
+/* Replace the per-member parameters (host [n_members][SWX_NPARAM]). */
+int swx_plan_set_params(swx_plan *plan, const double *params, void *stream);
+
+void swx_plan_destroy(swx_plan *plan);
+
+/* Two-level initialization from physical fields, device [B][J][I] each.
+ * U0 and V0 must both be given or both NULL; NULL diagnoses the winds
+ * spectrally from eta0/delta0.  Replaces _init_state_from_fields and
+ * _diagnose_winds (model.py:694-772, :644-666). */
+int swx_init(swx_plan *plan, const double *eta0, const double *delta0, const double *Phi0,
+             const double *U0, const double *V0, void *stream);
+
+/* Advance every member by nsteps steps with step indices t0, t0+1, ...
+ * Replaces the lax.scan of simulate_scan / simulate_scan_last
+ * (model.py:1237-1342) over _step_once (model.py:795-1194).
+ *
+ * History (optional, all device, may be NULL):
+ *   hist_fields float64 [B][5][hist_T][J][I]: eta, delta, Phi, U, V after
+ *     each step, written at time index hist_k0 + s for step s.
+ *   hist_scalars float64 [B][hist_T][4]: rms, spin_min, 0, 0 (diagnostics).
+ *   hist_phiext float64 [B][hist_T][J][2]: per-latitude min/max of the
+ *     Robert-Asselin filtered Phi (reduce over J for phi_min/phi_max).
+ *   hist_dead uint8 [B][hist_T].
+ * Steps run on `stream`; the call returns without synchronizing. */
+int swx_run(swx_plan *plan, int t0, int nsteps, double *hist_fields, double *hist_scalars,
+            double *hist_phiext, unsigned char *hist_dead, int hist_T, int hist_k0, void *stream);
+
+/* Copy the scan carry out to caller-provided device buffers. */
+int swx_export_state(swx_plan *plan, const swx_state_out *out, void *stream);
+
+/* Forward-only benchmark driver: run nsteps steps with no history and no
+ * export, replaying a captured HIP graph of `graph_steps` steps where
+ * possible (graph_steps <= 0 disables graphs).  Same semantics as swx_run
+ * with all history pointers NULL.  No reference counterpart beyond the
+ * jitted lax.scan it replaces (model.py:1281-1342). */
+int swx_run_graph(swx_plan *plan, int t0, int nsteps, int graph_steps, void *stream);
+
+/* Measurement hook: run nsteps steps eagerly with HIP events recorded on
+ * `stream` around each launch and return the average duration (ms) of the
+ * Legendre kernel and of the grid kernel.  Synchronizes the stream.  No
+ * reference counterpart (the reference times whole scans,
+ * scripts/benchmark_scan.py:116-141). */
+int swx_time_kernels(swx_plan *plan, int t0, int nsteps, float *ms_leg, float *ms_grid, void *stream);
+
+/* Last error message of the calling thread. */
+const char *swx_last_error(void);
+
+/* Library version string. */
+const char *swx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SWAMPE_CAPI_H */

@@ -1,0 +1,26 @@
+"""MI355X-native spectral time-step core of SWAMPE-JAX (drop-in for my_swampe.model).
+
+Public API mirrors my_swampe/__init__.py:64-71 (run_model, run_model_gpu,
+run_model_scan, run_model_scan_final, assert_finite_state) plus the batched
+ensemble driver.  All time stepping runs in the HIP kernels of
+my_swampe_amd/_lib/libswampe_hip.so.
+"""
+
+from .model import (  # noqa: F401
+    RunFlags,
+    State,
+    Static,
+    assert_finite_state,
+    build_static,
+    run_model,
+    run_model_gpu,
+    run_model_scan,
+    run_model_scan_final,
+)
+
+__version__ = "0.1.0"
+
+__all__ = [
+    "RunFlags", "State", "Static", "assert_finite_state", "build_static", "run_model", "run_model_gpu",
+    "run_model_scan", "run_model_scan_final", "__version__",
+]

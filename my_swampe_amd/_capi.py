@@ -1,0 +1,123 @@
+"""ctypes bindings of libswampe_hip.so (declarations in include/swampe_capi.h).
+
+Loading fails loudly: there is no CPU fallback for any entry point.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_char_p, c_double, c_int, c_ubyte, c_void_p
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libswampe_hip.so"
+
+SWX_OK = 0
+SWX_ERR_ARG = -1
+SWX_ERR_HIP = -2
+SWX_ERR_STATE = -3
+
+PREC_FP64 = 0
+PREC_FP32 = 1
+PREC_BF16X3 = 2
+PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
+
+PARAM_NAMES = ("dt", "a", "omega", "g", "Phibar", "taurad", "taudrag", "DPhieq", "K6", "K6Phi", "alpha", "blowup_rms")
+NPARAM = len(PARAM_NAMES)
+
+# Every extern "C" symbol declared in include/swampe_capi.h.
+EXPORTED_SYMBOLS = (
+    "swx_geometry", "swx_grid_size", "swx_plan_create", "swx_plan_set_params", "swx_plan_destroy",
+    "swx_init", "swx_run", "swx_export_state", "swx_run_graph", "swx_time_kernels", "swx_last_error",
+    "swx_version",
+)
+
+STATE_FIELDS = (
+    "etam_prev", "etam_curr", "deltam_prev", "deltam_curr", "Phim_prev", "Phim_curr",
+    "eta_prev", "eta_curr", "delta_prev", "delta_curr", "Phi_prev", "Phi_curr", "U_curr", "V_curr",
+    "Um_curr", "Vm_curr", "Am_curr", "Bm_curr", "Cm_curr", "Dm_curr", "Em_curr",
+    "PhiFm_curr", "Fm_curr", "Gm_curr", "Rum_lag", "Rvm_lag",
+)
+
+
+class PlanDesc(ctypes.Structure):
+    _fields_ = [
+        ("M", c_int), ("n_members", c_int), ("precision", c_int), ("expflag", c_int), ("forcflag", c_int),
+        ("diffflag", c_int), ("modalflag", c_int), ("diagnostics", c_int), ("test", c_int),
+        ("exact_basis", c_int),
+    ]
+
+
+class StateOut(ctypes.Structure):
+    _fields_ = [(name, c_void_p) for name in STATE_FIELDS] + [("dead", c_void_p), ("spec", c_void_p)]
+
+
+class SwxError(RuntimeError):
+    """Error reported by the native library (message from swx_last_error)."""
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load (once) and return the native library; raise if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m my_swampe_amd.build` "
+            "(or __graft_entry__.build()). There is no CPU fallback."
+        )
+    lib = ctypes.CDLL(str(LIB_PATH))
+    lib.swx_last_error.restype = c_char_p
+    lib.swx_version.restype = c_char_p
+    lib.swx_geometry.argtypes = [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
+    lib.swx_grid_size.argtypes = [c_int, POINTER(c_int), POINTER(c_int)]
+    lib.swx_plan_create.argtypes = [POINTER(PlanDesc), c_void_p, POINTER(c_void_p)]
+    lib.swx_plan_set_params.argtypes = [c_void_p, c_void_p, c_void_p]
+    lib.swx_plan_destroy.argtypes = [c_void_p]
+    lib.swx_plan_destroy.restype = None
+    lib.swx_init.argtypes = [c_void_p] + [c_void_p] * 5 + [c_void_p]
+    lib.swx_run.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]
+    lib.swx_export_state.argtypes = [c_void_p, POINTER(StateOut), c_void_p]
+    lib.swx_run_graph.argtypes = [c_void_p, c_int, c_int, c_int, c_void_p]
+    lib.swx_time_kernels.argtypes = [c_void_p, c_int, c_int, POINTER(ctypes.c_float), POINTER(ctypes.c_float), c_void_p]
+    for name in EXPORTED_SYMBOLS:
+        getattr(lib, name)  # AttributeError when a declared symbol is missing
+    for name in ("swx_geometry", "swx_grid_size", "swx_plan_create", "swx_plan_set_params", "swx_init",
+                 "swx_run", "swx_export_state", "swx_run_graph", "swx_time_kernels"):
+        getattr(lib, name).restype = c_int
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str) -> None:
+    """Raise SwxError (ValueError for argument errors) when rc is not SWX_OK."""
+    if rc == SWX_OK:
+        return
+    msg = load().swx_last_error().decode(errors="replace")
+    if rc == SWX_ERR_ARG:
+        raise ValueError(f"{what}: {msg}")
+    raise SwxError(f"{what} failed ({rc}): {msg}")
+
+
+def grid_size(M: int):
+    """(J, I) of truncation M; ValueError for an unsupported M."""
+    J, I = c_int(), c_int()
+    check(load().swx_grid_size(int(M), ctypes.byref(J), ctypes.byref(I)), "swx_grid_size")
+    return J.value, I.value
+
+
+def geometry(M: int, exact_basis: bool = False):
+    """Host FP64 geometry (mus, w, lambdas, Pmn, Hmn) as numpy arrays (no GPU needed)."""
+    import numpy as np
+
+    J, I = grid_size(M)
+    mus = np.zeros(J)
+    w = np.zeros(J)
+    lam = np.zeros(I)
+    P = np.zeros((J, M + 1, M + 1))
+    H = np.zeros((J, M + 1, M + 1))
+    ptr = lambda a: a.ctypes.data_as(c_void_p)  # noqa: E731
+    check(load().swx_geometry(int(M), int(bool(exact_basis)), ptr(mus), ptr(w), ptr(lam), ptr(P), ptr(H)), "swx_geometry")
+    return mus, w, lam, P, H

@@ -1,0 +1,136 @@
+// Kernel argument blocks and launchers shared by the plan code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace swx {
+
+// Per-member coefficient table (FP64), layout [B][ncoef]:
+//   scalars [0, kNScal), then sigma[N+1], sigmaPhi[N+1], tcmn[N+1], tc[J], areac[J].
+enum : int {
+  C_DT = 0,
+  C_A,
+  C_OMEGA,
+  C_PHIBAR,
+  C_TAURAD,
+  C_TAUDRAG,
+  C_DPHIEQ,
+  C_ALPHA,
+  C_BLOWUP,
+  C_INVTD,      // 0 if taudrag == -1 else 1/taudrag
+  C_AREAPLANET, // 4 pi a^2
+  C_FMN01,      // omega / sqrt(0.375)
+  C_INVTAURAD,  // 1 / taurad
+  kNScal = 16
+};
+
+struct CoefLayout {
+  int n_sigma, n_sigmaphi, n_tcmn, n_tc, n_areac, ncoef;
+  __host__ __device__ static CoefLayout make(int N, int J) {
+    CoefLayout c;
+    c.n_sigma = kNScal;
+    c.n_sigmaphi = c.n_sigma + (N + 1);
+    c.n_tcmn = c.n_sigmaphi + (N + 1);
+    c.n_tc = c.n_tcmn + (N + 1);
+    c.n_areac = c.n_tc + J;
+    c.ncoef = c.n_areac + J;
+    return c;
+  }
+};
+
+// Shared geometry table (FP64): mus[J], w[J], s1m[J], cosphi[J], denomE[J],
+// cosl[I], day[I] (1.0 on the strict dayside, else 0.0).
+struct GeoLayout {
+  int mus, w, s1m, cosphi, denomE, cosl, day, n;
+  __host__ __device__ static GeoLayout make(int J, int I) {
+    GeoLayout g;
+    g.mus = 0;
+    g.w = J;
+    g.s1m = 2 * J;
+    g.cosphi = 3 * J;
+    g.denomE = 4 * J;
+    g.cosl = 5 * J;
+    g.day = 5 * J + I;
+    g.n = 5 * J + 2 * I;
+    return g;
+  }
+};
+
+// Number of Legendre-input slots written by the grid kernel per (member, m, j):
+// P-basis slots 0..4, H-basis slots 5..8.
+constexpr int kNP = 5;
+constexpr int kNH = 4;
+constexpr int kNT = kNP + kNH;
+// Fourier fields produced by the Legendre kernel per (member, j, m):
+// eta, delta, Phi, U, V.
+constexpr int kNG = 5;
+// Raw forward-DFT products per row: A, B, C, D, E, PhiF, F, G.
+constexpr int kNRaw = 8;
+
+// Offset (in elements of one J-row block) of the triangular basis block of m:
+// the block holds J * (N + 1 - m) values.
+__host__ __device__ inline long long tri_off(int m, int N, int J) {
+  return (long long)J * ((long long)m * (N + 1) - (long long)m * (m - 1) / 2);
+}
+
+enum LegMode : int { LEG_INIT = 0, LEG_STEP = 1 };
+enum GridMode : int { GRID_DFT = 0, GRID_INIT = 1, GRID_STEP = 2 };
+
+template <typename R>
+struct LegArgs {
+  int M, N, J, B;
+  int mode, expflag, diffflag, diagnostics, diag_winds;
+  const R* Pf;  // forward layout  [m-block][j][n-m]
+  const R* Hf;
+  const R* Pi;  // inverse layout  [m-block][n-m][j]
+  const R* Hi;
+  const double* coef;
+  int ncoef;
+  double* S;        // [B][2][3][M+1][N+1] complex, level 0 = prev, 1 = curr
+  const R* T;       // [B][M+1][kNT][J] complex
+  R* G;             // [B][kNG][J][M+1] complex
+  const double* part_uv;  // [B][J][2]
+  unsigned char* dead;    // [B]
+  double* hist_scal;      // [B][hist_T][4] or null
+  unsigned char* hist_dead;
+  int hist_T, hist_k;
+};
+
+template <typename R>
+struct GridArgs {
+  int M, N, J, I, B;
+  int mode, expflag, forcflag, test, diagnostics, do_ra, phys_on, export_raw, diag_winds, have_uv;
+  const R* tw;         // [2][I]: cos(2 pi k / I), sin(2 pi k / I)
+  const double* geo;
+  const double* coef;
+  int ncoef;
+  const R* G;          // [B][kNG][J][M+1] (GRID_STEP, GRID_INIT with diag_winds)
+  R* Gout;             // GRID_DFT / GRID_INIT write targets
+  R* T;                // [B][M+1][kNT][J]
+  const double* in_eta;  // GRID_DFT / GRID_INIT inputs [B][J][I]
+  const double* in_delta;
+  const double* in_Phi;
+  const double* in_U;
+  const double* in_V;
+  double* phys;        // [B][8][J][I]: eta p,c; delta p,c; Phi p,c; U; V
+  double* uvic;        // [B][2][J][I] pinned winds (test == 1)
+  double* part_uv;     // [B][J][2]
+  const unsigned char* dead;
+  double* hist_fields; // [B][5][hist_T][J][I]
+  double* hist_phiext; // [B][hist_T][J][2]
+  int hist_T, hist_k;
+  R* raw;              // [B][kNRaw][J][M+1]
+  R* lag;              // [B][2][J][M+1]
+  R* g0uv;             // [B][2][J][M+1]
+};
+
+template <typename R>
+hipError_t launch_leg(const LegArgs<R>& a, hipStream_t s);
+template <typename R>
+hipError_t launch_grid(const GridArgs<R>& a, hipStream_t s);
+template <typename R>
+hipError_t launch_copy_real(double* dst, long long dstride, const R* src, long long sstride, long long n, int B,
+                            hipStream_t s);
+
+}  // namespace swx

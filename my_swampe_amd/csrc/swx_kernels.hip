@@ -1,0 +1,695 @@
+// Per-step kernels of the spectral shallow-water core (plain path, FP64 or
+// FP32 transforms; the spectral state update and diffusion are FP64).
+//
+// One step is two launches:
+//   k_leg  (grid = (M+1, members)):  forward Legendre contraction of the
+//          Fourier tendency inputs for zonal wavenumber m, FP64 update of the
+//          spectral state (modified-Euler or explicit, diffusion), then the
+//          inverse Legendre contraction with the wind inversion fused in.
+//          Reference: modEuler_tdiff.py:30-268, explicit_tdiff.py:20-233,
+//          spectral_transform.py:370-596, filters.py:11-27.
+//   k_grid (grid = (J, members)):  inverse longitude DFT of one latitude row,
+//          grid-space physics (Robert-Asselin filter, forcing Q/F/G, nonlinear
+//          products A..E, diagnostics), forward truncated DFT and assembly of
+//          the w-weighted Legendre inputs for the next step.
+//          Reference: spectral_transform.py:290-348, forcing.py:14-155,
+//          initial_conditions.py:242-292, model.py:795-1194.
+//
+// The linear carry terms (Pw(etam1), Pw(deltam1), Pw(Phim1), Pw(*m0)) of the
+// reference are applied directly to the FP64 spectral state: Gauss quadrature
+// makes Pw(invrs_leg(X)) == X exactly, so only the nonlinear and forcing
+// tendencies pass through the transforms.
+#include "swx_kernels.h"
+
+#include <cfloat>
+
+namespace swx {
+
+namespace {
+
+__device__ inline double dmin(double a, double b) { return a < b ? a : b; }
+__device__ inline double dmax(double a, double b) { return a > b ? a : b; }
+
+// Block-wide reduction of NV doubles (sum / min / max per op code) through LDS.
+// op: 0 = sum, 1 = min, 2 = max.  All threads must call it.
+template <int NV>
+__device__ void block_reduce(double (&v)[NV], const int (&op)[NV], double* scratch) {
+  const int tid = threadIdx.x;
+  const int nt = blockDim.x;
+  for (int k = 0; k < NV; ++k) scratch[k * nt + tid] = v[k];
+  __syncthreads();
+  for (int s = nt / 2; s > 0; s >>= 1) {
+    if (tid < s) {
+      for (int k = 0; k < NV; ++k) {
+        double a = scratch[k * nt + tid], b = scratch[k * nt + tid + s];
+        scratch[k * nt + tid] = op[k] == 0 ? a + b : (op[k] == 1 ? dmin(a, b) : dmax(a, b));
+      }
+    }
+    __syncthreads();
+  }
+  for (int k = 0; k < NV; ++k) v[k] = scratch[k * nt];
+  __syncthreads();
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+// Legendre kernel
+// ----------------------------------------------------------------------------
+
+template <typename R>
+__global__ void __launch_bounds__(128) k_leg(LegArgs<R> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int m = blockIdx.x;
+  const int b = blockIdx.y;
+  const int tid = threadIdx.x;
+  const int nt = blockDim.x;
+  const int M = a.M, N = a.N, J = a.J;
+  const int K = N + 1 - m;  // retained degrees n = m..N
+  const double* cf = a.coef + (long long)b * a.ncoef;
+  const CoefLayout cl = CoefLayout::make(N, J);
+
+  R* sT = reinterpret_cast<R*>(smem_raw);         // [kNT][J][2]
+  R* sX = sT + kNT * J * 2;                       // [5][N+1][2]
+  double* sRed = reinterpret_cast<double*>(sX + 5 * (N + 1) * 2);
+
+  if (a.mode == LEG_STEP && a.diagnostics) {
+    // Blow-up gate on the current winds (model.py:838-845, :1192-1194).
+    double v[2] = {0.0, DBL_MAX};
+    const int op[2] = {0, 1};
+    for (int j = tid; j < J; j += nt) {
+      v[0] += a.part_uv[((long long)b * J + j) * 2 + 0];
+      v[1] = dmin(v[1], a.part_uv[((long long)b * J + j) * 2 + 1]);
+    }
+    block_reduce<2>(v, op, sRed);
+    const double rms = sqrt(v[0]);
+    const bool dead_next = (a.dead[b] != 0) || (rms > cf[C_BLOWUP]);
+    if (m == 0 && tid == 0) {
+      a.dead[b] = dead_next ? 1 : 0;
+      if (a.hist_scal) {
+        double* h = a.hist_scal + ((long long)b * a.hist_T + a.hist_k) * 4;
+        h[0] = rms;
+        h[1] = v[1];
+      }
+      if (a.hist_dead) a.hist_dead[(long long)b * a.hist_T + a.hist_k] = dead_next ? 1 : 0;
+    }
+    if (dead_next) return;
+  }
+
+  // Stage the Legendre inputs of (b, m).
+  const long long toff = (((long long)b * (M + 1) + m) * kNT) * J * 2;
+  const int nslots = (a.mode == LEG_STEP) ? kNT : 3;
+  for (int e = tid; e < nslots * J * 2; e += nt) sT[e] = a.T[toff + e];
+  __syncthreads();
+
+  const long long boff = tri_off(m, N, J);
+  const R* Pf = a.Pf + boff;
+  const R* Hf = a.Hf + boff;
+  double* S = a.S + (long long)b * 2 * 3 * (M + 1) * (N + 1) * 2;
+  double* Sprev = S;
+  double* Scur = S + 3 * (M + 1) * (N + 1) * 2;
+  const long long fstride = (long long)(M + 1) * (N + 1) * 2;
+
+  for (int np = tid; np < K; np += nt) {
+    const int n = m + np;
+    const long long so = ((long long)m * (N + 1) + n) * 2;
+    double eta_n[2], delta_n[2], Phi_n[2];
+    if (a.mode == LEG_INIT) {
+      R acc[6] = {0, 0, 0, 0, 0, 0};
+      for (int j = 0; j < J; ++j) {
+        const R p = Pf[(long long)j * K + np];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          acc[2 * k] += sT[(k * J + j) * 2] * p;
+          acc[2 * k + 1] += sT[(k * J + j) * 2 + 1] * p;
+        }
+      }
+      eta_n[0] = acc[0]; eta_n[1] = acc[1];
+      delta_n[0] = acc[2]; delta_n[1] = acc[3];
+      Phi_n[0] = acc[4]; Phi_n[1] = acc[5];
+      for (int c = 0; c < 2; ++c) {
+        Sprev[0 * fstride + so + c] = Scur[0 * fstride + so + c] = eta_n[c];
+        Sprev[1 * fstride + so + c] = Scur[1 * fstride + so + c] = delta_n[c];
+        Sprev[2 * fstride + so + c] = Scur[2 * fstride + so + c] = Phi_n[c];
+      }
+    } else {
+      R acc[2 * kNT];
+#pragma unroll
+      for (int k = 0; k < 2 * kNT; ++k) acc[k] = 0;
+      for (int j = 0; j < J; ++j) {
+        const R p = Pf[(long long)j * K + np];
+        const R h = Hf[(long long)j * K + np];
+#pragma unroll
+        for (int k = 0; k < kNP; ++k) {
+          acc[2 * k] += sT[(k * J + j) * 2] * p;
+          acc[2 * k + 1] += sT[(k * J + j) * 2 + 1] * p;
+        }
+#pragma unroll
+        for (int k = kNP; k < kNT; ++k) {
+          acc[2 * k] += sT[(k * J + j) * 2] * h;
+          acc[2 * k + 1] += sT[(k * J + j) * 2 + 1] * h;
+        }
+      }
+      const double dt = cf[C_DT], aa = cf[C_A], Phibar = cf[C_PHIBAR];
+      const double nn = (double)n * (double)(n + 1);
+      for (int c = 0; c < 2; ++c) {
+        const double fp0 = acc[0 + c], fp1 = acc[2 + c], fp2 = acc[4 + c], fp3 = acc[6 + c], fp4 = acc[8 + c];
+        const double fh0 = acc[10 + c], fh1 = acc[12 + c], fh2 = acc[14 + c], fh3 = acc[16 + c];
+        const double eo = Scur[0 * fstride + so + c];
+        const double d_o = Scur[1 * fstride + so + c];
+        const double po = Scur[2 * fstride + so + c];
+        if (!a.expflag) {
+          // modEuler_tdiff.py:95-103, :185-193, :261 with the carry terms in spectral space.
+          const double t2 = (2.0 * dt / (aa * aa)) / 4.0;
+          Phi_n[c] = po - dt * Phibar * d_o - Phibar * 0.5 * ((1.0 / (aa * aa)) * (nn * po)) + fp0 + nn * fp1 + fh0;
+          delta_n[c] = d_o + nn * (t2 * po) - Phibar * (nn * d_o) / (aa * aa) + fp2 + nn * fp3 + fh1 + nn * fh2;
+          eta_n[c] = eo + fp4 + fh3;
+        } else {
+          // explicit_tdiff.py:71, :138, :212 (prev-level carries).
+          const double ep = Sprev[0 * fstride + so + c];
+          const double dp = Sprev[1 * fstride + so + c];
+          const double pp = Sprev[2 * fstride + so + c];
+          Phi_n[c] = pp - 2.0 * dt * Phibar * d_o + fp0 + fh0;
+          delta_n[c] = dp + fp2 + fh1;
+          eta_n[c] = ep + fp4 + fh3;
+        }
+      }
+      if (a.diffflag) {
+        const double sg = cf[cl.n_sigma + n], sp = cf[cl.n_sigmaphi + n];
+        for (int c = 0; c < 2; ++c) {
+          Phi_n[c] *= sp;
+          delta_n[c] *= sg;
+          eta_n[c] *= sg;
+        }
+      }
+      for (int c = 0; c < 2; ++c) {
+        if (a.expflag) {
+          Sprev[0 * fstride + so + c] = Scur[0 * fstride + so + c];
+          Sprev[1 * fstride + so + c] = Scur[1 * fstride + so + c];
+          Sprev[2 * fstride + so + c] = Scur[2 * fstride + so + c];
+        }
+        Scur[0 * fstride + so + c] = eta_n[c];
+        Scur[1 * fstride + so + c] = delta_n[c];
+        Scur[2 * fstride + so + c] = Phi_n[c];
+      }
+    }
+    // Inputs of the inverse transform: eta, delta, Phi and the wind
+    // potentials (spectral_transform.py:577-582).
+    const double tcmn = cf[cl.n_tcmn + n];
+    const double f = (m == 0 && n == 1) ? cf[C_FMN01] : 0.0;
+    for (int c = 0; c < 2; ++c) {
+      const double dh = (n == 0) ? 0.0 : delta_n[c] * tcmn;
+      const double eh = (n == 0) ? 0.0 : (eta_n[c] - (c == 0 ? f : 0.0)) * tcmn;
+      sX[(0 * (N + 1) + np) * 2 + c] = (R)eta_n[c];
+      sX[(1 * (N + 1) + np) * 2 + c] = (R)delta_n[c];
+      sX[(2 * (N + 1) + np) * 2 + c] = (R)Phi_n[c];
+      sX[(3 * (N + 1) + np) * 2 + c] = (R)dh;
+      sX[(4 * (N + 1) + np) * 2 + c] = (R)eh;
+    }
+  }
+  if (a.mode == LEG_INIT && !a.diag_winds) return;
+  __syncthreads();
+
+  const R* Pi = a.Pi + boff;
+  const R* Hi = a.Hi + boff;
+  R* G = a.G + (long long)b * kNG * J * (M + 1) * 2;
+  const R mr = (R)m;
+  for (int j = tid; j < J; j += nt) {
+    R p[10];
+    R h[4];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) p[k] = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) h[k] = 0;
+    for (int np = 0; np < K; ++np) {
+      const R pv = Pi[(long long)np * J + j];
+      const R hv = Hi[(long long)np * J + j];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        p[2 * k] += pv * sX[(k * (N + 1) + np) * 2];
+        p[2 * k + 1] += pv * sX[(k * (N + 1) + np) * 2 + 1];
+      }
+      // H applied to eta_hat (slot 4) and delta_hat (slot 3).
+      h[0] += hv * sX[(4 * (N + 1) + np) * 2];
+      h[1] += hv * sX[(4 * (N + 1) + np) * 2 + 1];
+      h[2] += hv * sX[(3 * (N + 1) + np) * 2];
+      h[3] += hv * sX[(3 * (N + 1) + np) * 2 + 1];
+    }
+    const long long ro = ((long long)j * (M + 1) + m) * 2;
+    const long long fs = (long long)J * (M + 1) * 2;
+    if (a.mode == LEG_STEP) {
+      G[0 * fs + ro] = p[0];
+      G[0 * fs + ro + 1] = p[1];
+      G[1 * fs + ro] = p[2];
+      G[1 * fs + ro + 1] = p[3];
+      G[2 * fs + ro] = p[4];
+      G[2 * fs + ro + 1] = p[5];
+    }
+    // U_m = -i m P(delta_hat) + H(eta_hat);  V_m = -i m P(eta_hat) - H(delta_hat)
+    // (spectral_transform.py:584-594).
+    G[3 * fs + ro] = mr * p[7] + h[0];
+    G[3 * fs + ro + 1] = -mr * p[6] + h[1];
+    G[4 * fs + ro] = mr * p[9] - h[2];
+    G[4 * fs + ro + 1] = -mr * p[8] - h[3];
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Grid kernel
+// ----------------------------------------------------------------------------
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_grid(GridArgs<R> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int j = blockIdx.x;
+  const int b = blockIdx.y;
+  const int tid = threadIdx.x;
+  const int nt = blockDim.x;
+  const int M = a.M, N = a.N, J = a.J, I = a.I;
+  const int Mp = M + 1;
+  const double* cf = a.coef + (long long)b * a.ncoef;
+  const CoefLayout cl = CoefLayout::make(N, J);
+  const GeoLayout gl = GeoLayout::make(J, I);
+  const double* geo = a.geo;
+
+  constexpr int kMaxF = 10;  // forward-DFT fields per row
+  R* sTc = reinterpret_cast<R*>(smem_raw);  // [I] cos table
+  R* sTs = sTc + I;                          // [I] sin table
+  R* sG = sTs + I;                           // [kNG][Mp][2]
+  R* sX = sG + kNG * Mp * 2;                 // [kMaxF][I]
+  R* sF = sX + kMaxF * I;                    // [kMaxF][Mp][2]
+  double* sRed = reinterpret_cast<double*>(sF + kMaxF * Mp * 2);
+
+  const long long plane = (long long)J * I;
+  const long long row = (long long)j * I;
+  double* phys = a.phys + (long long)b * 8 * plane;
+
+  if (a.mode == GRID_STEP && a.diagnostics && a.dead[b]) {
+    // Frozen member: emit the current level (model.py:847-864).
+    double v[2] = {DBL_MAX, -DBL_MAX};
+    const int op[2] = {1, 2};
+    for (int i = tid; i < I; i += nt) {
+      const double phi = phys[5 * plane + row + i];
+      v[0] = dmin(v[0], phi);
+      v[1] = dmax(v[1], phi);
+      if (a.hist_fields) {
+        const int src[5] = {1, 3, 5, 6, 7};
+        for (int f = 0; f < 5; ++f)
+          a.hist_fields[(((long long)b * 5 + f) * a.hist_T + a.hist_k) * plane + row + i] = phys[src[f] * plane + row + i];
+      }
+    }
+    block_reduce<2>(v, op, sRed);
+    if (a.hist_phiext && tid == 0) {
+      double* h = a.hist_phiext + (((long long)b * a.hist_T + a.hist_k) * J + j) * 2;
+      h[0] = v[0];
+      h[1] = v[1];
+    }
+    return;
+  }
+
+  for (int i = tid; i < I; i += nt) {
+    sTc[i] = a.tw[i];
+    sTs[i] = a.tw[I + i];
+  }
+  const bool need_g = (a.mode == GRID_STEP) || (a.mode == GRID_INIT && a.diag_winds);
+  if (need_g) {
+    const R* Gb = a.G + (long long)b * kNG * J * Mp * 2;
+    for (int e = tid; e < kNG * Mp * 2; e += nt) {
+      const int f = e / (Mp * 2), r = e % (Mp * 2);
+      sG[e] = Gb[((long long)f * J + j) * Mp * 2 + r];
+    }
+  }
+  __syncthreads();
+
+  const double mu_s1m = geo[gl.s1m + j];
+  const double cosphi = geo[gl.cosphi + j];
+  const double denomE = geo[gl.denomE + j];
+  const double Phibar = cf[C_PHIBAR];
+  const double alpha = cf[C_ALPHA];
+
+  int nfwd = 0;  // number of forward-DFT fields staged in sX
+  double red[4] = {0.0, DBL_MAX, DBL_MAX, -DBL_MAX};
+
+  if (a.mode == GRID_DFT) {
+    nfwd = a.have_uv ? 5 : 3;
+    const double* src[5] = {a.in_eta, a.in_delta, a.in_Phi, a.in_U, a.in_V};
+    for (int i = tid; i < I; i += nt)
+      for (int f = 0; f < nfwd; ++f) sX[f * I + i] = (R)src[f][(long long)b * plane + row + i];
+  } else {
+    // Fields 0..7: A, B, C, D, E, PhiF, F, G; 8, 9: U, V when their DFT is needed.
+    const bool dft_uv = (a.mode == GRID_INIT) || (a.test == 1 && a.expflag);
+    nfwd = dft_uv ? 10 : 8;
+    for (int i = tid; i < I; i += nt) {
+      double eta, delta, Phi, U, V;
+      if (a.mode == GRID_INIT) {
+        eta = a.in_eta[(long long)b * plane + row + i];
+        delta = a.in_delta[(long long)b * plane + row + i];
+        Phi = a.in_Phi[(long long)b * plane + row + i];
+      }
+      if (a.mode == GRID_STEP || (a.mode == GRID_INIT && a.diag_winds)) {
+        // Inverse DFT of the Hermitian-symmetric row: x = Re c0 + 2 sum_m Re(c_m e^{+i theta})
+        // (spectral_transform.py:331-348 with the real part taken, model.py:973-977).
+        const int f0 = (a.mode == GRID_STEP) ? 0 : 3;
+        R acc[kNG];
+#pragma unroll
+        for (int f = 0; f < kNG; ++f) acc[f] = 0;
+        int idx = 0;
+        for (int m = 1; m < Mp; ++m) {
+          idx += i;
+          if (idx >= I) idx -= I;
+          const R c = sTc[idx], s = sTs[idx];
+#pragma unroll
+          for (int f = 0; f < kNG; ++f) {
+            if (f < f0) continue;
+            acc[f] += sG[(f * Mp + m) * 2] * c - sG[(f * Mp + m) * 2 + 1] * s;
+          }
+        }
+        double x[kNG];
+#pragma unroll
+        for (int f = 0; f < kNG; ++f) x[f] = (double)sG[(f * Mp) * 2] + 2.0 * (double)acc[f];
+        if (a.mode == GRID_STEP) {
+          eta = x[0];
+          delta = x[1];
+          Phi = x[2];
+        }
+        U = x[3];
+        V = x[4];
+      }
+      if (a.mode == GRID_INIT && !a.diag_winds) {
+        U = a.in_U[(long long)b * plane + row + i];
+        V = a.in_V[(long long)b * plane + row + i];
+      }
+      if (a.test == 1) {
+        if (a.mode == GRID_INIT) {
+          a.uvic[((long long)b * 2 + 0) * plane + row + i] = U;
+          a.uvic[((long long)b * 2 + 1) * plane + row + i] = V;
+        } else {
+          U = a.uvic[((long long)b * 2 + 0) * plane + row + i];
+          V = a.uvic[((long long)b * 2 + 1) * plane + row + i];
+        }
+      }
+      // Physical carry and the Robert-Asselin filter on the prev level only
+      // (model.py:988-1006).
+      double Phi_mid = Phi;
+      if (a.mode == GRID_INIT) {
+        phys[0 * plane + row + i] = eta;
+        phys[1 * plane + row + i] = eta;
+        phys[2 * plane + row + i] = delta;
+        phys[3 * plane + row + i] = delta;
+        phys[4 * plane + row + i] = Phi;
+        phys[5 * plane + row + i] = Phi;
+        phys[6 * plane + row + i] = U;
+        phys[7 * plane + row + i] = V;
+      } else if (a.phys_on) {
+        const double ep = phys[0 * plane + row + i], ec = phys[1 * plane + row + i];
+        const double dp = phys[2 * plane + row + i], dc = phys[3 * plane + row + i];
+        const double pp = phys[4 * plane + row + i], pc = phys[5 * plane + row + i];
+        double em = ec, dm = dc, pm = pc;
+        if (a.do_ra) {
+          em = ec + alpha * (ep - 2.0 * ec + eta);
+          dm = dc + alpha * (dp - 2.0 * dc + delta);
+          pm = pc + alpha * (pp - 2.0 * pc + Phi);
+        }
+        Phi_mid = pm;
+        phys[0 * plane + row + i] = em;
+        phys[1 * plane + row + i] = eta;
+        phys[2 * plane + row + i] = dm;
+        phys[3 * plane + row + i] = delta;
+        phys[4 * plane + row + i] = pm;
+        phys[5 * plane + row + i] = Phi;
+        phys[6 * plane + row + i] = U;
+        phys[7 * plane + row + i] = V;
+      }
+      if (a.hist_fields) {
+        const double outv[5] = {eta, delta, Phi, U, V};
+        for (int f = 0; f < 5; ++f)
+          a.hist_fields[(((long long)b * 5 + f) * a.hist_T + a.hist_k) * plane + row + i] = outv[f];
+      }
+      // Forcing (forcing.py:65-155 via model.py:560-603): computed whenever test is None.
+      double PhiF = 0.0, F = 0.0, G = 0.0;
+      if (a.test < 0) {
+        const double Phieq = geo[gl.day + i] != 0.0 ? Phibar + (cf[C_DPHIEQ] * geo[gl.cosl + i]) * mu_s1m : Phibar;
+        const double Q = cf[C_INVTAURAD] * (Phieq - (Phi + Phibar));
+        const double Qc = Q < 0.0 ? 0.0 : Q;
+        const double tot = Phi + Phibar;
+        const double safe = fabs(tot) > 0.0 ? tot : DBL_MIN;
+        const double Ru = -U * Qc / safe;
+        const double Rv = -V * Qc / safe;
+        const double td = cf[C_TAUDRAG];
+        PhiF = Q;
+        if (td == -1.0) {
+          F = Ru;
+          G = Rv;
+        } else {
+          F = Ru - (U / td);
+          G = Rv - (V / td);
+        }
+      }
+      // Nonlinear products (initial_conditions.py:283-292).
+      sX[0 * I + i] = (R)(U * eta);
+      sX[1 * I + i] = (R)(V * eta);
+      sX[2 * I + i] = (R)(U * Phi);
+      sX[3 * I + i] = (R)(V * Phi);
+      sX[4 * I + i] = (R)((U * U + V * V) / denomE);
+      sX[5 * I + i] = (R)PhiF;
+      sX[6 * I + i] = (R)F;
+      sX[7 * I + i] = (R)G;
+      if (nfwd == 10) {
+        sX[8 * I + i] = (R)U;
+        sX[9 * I + i] = (R)V;
+      }
+      // Diagnostics partials: RMS integrand, min |wind|, min/max of filtered Phi
+      // (time_stepping.py:537-576; model.py:838-845, :1112-1117).
+      const double uc = U / cosphi, vc = V / cosphi;
+      red[0] += cf[cl.n_areac + j] * (uc * uc + vc * vc) / cf[C_AREAPLANET];
+      red[1] = dmin(red[1], sqrt(U * U + V * V));
+      red[2] = dmin(red[2], Phi_mid);
+      red[3] = dmax(red[3], Phi_mid);
+    }
+  }
+  if (a.mode != GRID_DFT && (a.diagnostics || a.hist_phiext)) {
+    const int op[4] = {0, 1, 1, 2};
+    block_reduce<4>(red, op, sRed);
+    if (tid == 0) {
+      if (a.diagnostics) {
+        a.part_uv[((long long)b * J + j) * 2 + 0] = red[0];
+        a.part_uv[((long long)b * J + j) * 2 + 1] = red[1];
+      }
+      if (a.hist_phiext && a.mode == GRID_STEP) {
+        double* h = a.hist_phiext + (((long long)b * a.hist_T + a.hist_k) * J + j) * 2;
+        h[0] = red[2];
+        h[1] = red[3];
+      }
+    }
+  }
+  __syncthreads();
+
+  // Forward truncated DFT: X_m = (1/I) sum_i x_i e^{-2 pi i m i / I} with the
+  // index-based phase (spectral_transform.py:290-328).
+  const double invI = 1.0 / (double)I;
+  for (int p = tid; p < nfwd * Mp; p += nt) {
+    const int f = p / Mp, m = p % Mp;
+    R re = 0, im = 0;
+    int idx = 0;
+    const R* xf = sX + f * I;
+    for (int i = 0; i < I; ++i) {
+      const R x = xf[i];
+      re += x * sTc[idx];
+      im -= x * sTs[idx];
+      idx += m;
+      if (idx >= I) idx -= I;
+    }
+    sF[(f * Mp + m) * 2] = (R)((double)re * invI);
+    sF[(f * Mp + m) * 2 + 1] = (R)((double)im * invI);
+  }
+  __syncthreads();
+
+  if (a.mode == GRID_DFT) {
+    // Raw Fourier of the initial fields and the w-weighted Legendre inputs
+    // of the spectral state initialization.
+    R* Gb = a.Gout + (long long)b * kNG * J * Mp * 2;
+    const double wj = geo[gl.w + j];
+    for (int e = tid; e < nfwd * Mp; e += nt) {
+      const int f = e / Mp, m = e % Mp;
+      Gb[((long long)f * J + j) * Mp * 2 + m * 2] = sF[(f * Mp + m) * 2];
+      Gb[((long long)f * J + j) * Mp * 2 + m * 2 + 1] = sF[(f * Mp + m) * 2 + 1];
+      if (f < 3) {
+        R* Tm = a.T + (((long long)b * Mp + m) * kNT + f) * J * 2;
+        Tm[j * 2] = (R)((double)sF[(f * Mp + m) * 2] * wj);
+        Tm[j * 2 + 1] = (R)((double)sF[(f * Mp + m) * 2 + 1] * wj);
+      }
+    }
+    return;
+  }
+
+  if (a.export_raw || a.mode == GRID_INIT) {
+    R* rb = a.raw + (long long)b * kNRaw * J * Mp * 2;
+    for (int e = tid; e < kNRaw * Mp; e += nt) {
+      const int f = e / Mp, m = e % Mp;
+      rb[((long long)f * J + j) * Mp * 2 + m * 2] = sF[(f * Mp + m) * 2];
+      rb[((long long)f * J + j) * Mp * 2 + m * 2 + 1] = sF[(f * Mp + m) * 2 + 1];
+    }
+  }
+  if (a.mode == GRID_INIT) {
+    // Um0/Vm0 (model.py:727-729), the pinned copy for test 1, and the lagged
+    // momentum-forcing remainder Fm + Um/taudrag (model.py:775-792).
+    R* Gb = a.Gout + (long long)b * kNG * J * Mp * 2;
+    R* g0 = a.g0uv + (long long)b * 2 * J * Mp * 2;
+    R* lg = a.lag + (long long)b * 2 * J * Mp * 2;
+    const double itd = cf[C_INVTD];
+    for (int e = tid; e < 2 * Mp * 2; e += nt) {
+      const int f = e / (Mp * 2), r = e % (Mp * 2);
+      const R v = sF[(8 + f) * Mp * 2 + r];
+      Gb[((long long)(3 + f) * J + j) * Mp * 2 + r] = v;
+      g0[((long long)f * J + j) * Mp * 2 + r] = v;
+      lg[((long long)f * J + j) * Mp * 2 + r] = (R)((double)sF[(6 + f) * Mp * 2 + r] + (double)v * itd);
+    }
+  }
+
+  // Assemble the w-weighted Legendre inputs (all slots complex, FP64 math).
+  const double wj = geo[gl.w + j];
+  const double tc = cf[cl.n_tc + j];
+  const double dt = cf[C_DT], aa = cf[C_A];
+  for (int m = tid; m < Mp; m += nt) {
+    auto ld = [&](int f, double* z) {
+      z[0] = (double)sF[(f * Mp + m) * 2];
+      z[1] = (double)sF[(f * Mp + m) * 2 + 1];
+    };
+    double Am[2], Bm[2], Cm[2], Dm[2], Em[2], PFm[2], Fm[2], Gm[2];
+    ld(0, Am); ld(1, Bm); ld(2, Cm); ld(3, Dm); ld(4, Em); ld(5, PFm); ld(6, Fm); ld(7, Gm);
+    const double mm = (double)m;
+    double out[kNT][2];
+    for (int k = 0; k < kNT; ++k) out[k][0] = out[k][1] = 0.0;
+    // i m z = (-m z.im, m z.re)
+    if (!a.expflag) {
+      // modEuler_tdiff.py:72-110, :162-200, :253-259 regrouped by basis and n-weight.
+      const double t1 = tc / 4.0;
+      const double t2 = (2.0 * dt / (aa * aa)) / 4.0;
+      const double te = a.forcflag ? tc : tc / 2.0;
+      const double ia2 = 1.0 / (aa * aa);
+      double Bs[2], As[2], Bf[2], Af[2];
+      for (int c = 0; c < 2; ++c) {
+        Bf[c] = Bm[c] + Fm[c];
+        Af[c] = Am[c] - Gm[c];
+        Bs[c] = a.forcflag ? Bf[c] : Bm[c];
+        As[c] = a.forcflag ? Af[c] : Am[c];
+      }
+      const double fPF = a.forcflag ? 1.0 : 0.0;
+      // P slot 0 (Phi): -t1 i m C - 0.5 Phibar t1 i m B* + dt PhiF
+      // P slot 1 (Phi, x n(n+1)): -0.5 Phibar t2 E
+      // P slot 2 (delta): t1 i m Bf
+      // P slot 3 (delta, x n(n+1)): t2 E + t1 i m C / (2 a^2) + (dt/2) PhiF / a^2
+      // P slot 4 (eta): -te i m A*
+      // H slot 5 (Phi): t1 D - 0.5 Phibar t1 A*
+      // H slot 6 (delta): t1 Af
+      // H slot 7 (delta, x n(n+1)): t1 D / (2 a^2)
+      // H slot 8 (eta): te B*
+      const double imC[2] = {-mm * Cm[1], mm * Cm[0]};
+      const double imBs[2] = {-mm * Bs[1], mm * Bs[0]};
+      const double imBf[2] = {-mm * Bf[1], mm * Bf[0]};
+      const double imAs[2] = {-mm * As[1], mm * As[0]};
+      for (int c = 0; c < 2; ++c) {
+        out[0][c] = -(t1 * imC[c]) - 0.5 * Phibar * (t1 * imBs[c]) + fPF * (dt * PFm[c]);
+        out[1][c] = -0.5 * Phibar * (t2 * Em[c]);
+        out[2][c] = t1 * imBf[c];
+        out[3][c] = t2 * Em[c] + (t1 * imC[c]) / (2.0 * aa * aa) + fPF * ((dt / 2.0) * PFm[c]) * ia2;
+        out[4][c] = -(te * imAs[c]);
+        out[5][c] = t1 * Dm[c] - 0.5 * Phibar * (t1 * As[c]);
+        out[6][c] = t1 * Af[c];
+        out[7][c] = (t1 * Dm[c]) / (2.0 * aa * aa);
+        out[8][c] = te * Bs[c];
+      }
+    } else {
+      // explicit_tdiff.py:63-75, :134-153, :207-224 regrouped by basis.
+      double Um[2], Vm[2];
+      if (a.test == 1) {
+        Um[0] = (double)sF[(8 * Mp + m) * 2];
+        Um[1] = (double)sF[(8 * Mp + m) * 2 + 1];
+        Vm[0] = (double)sF[(9 * Mp + m) * 2];
+        Vm[1] = (double)sF[(9 * Mp + m) * 2 + 1];
+      } else if (a.mode == GRID_INIT) {
+        Um[0] = (double)sF[(8 * Mp + m) * 2];
+        Um[1] = (double)sF[(8 * Mp + m) * 2 + 1];
+        Vm[0] = (double)sF[(9 * Mp + m) * 2];
+        Vm[1] = (double)sF[(9 * Mp + m) * 2 + 1];
+      } else {
+        Um[0] = (double)sG[(3 * Mp + m) * 2];
+        Um[1] = (double)sG[(3 * Mp + m) * 2 + 1];
+        Vm[0] = (double)sG[(4 * Mp + m) * 2];
+        Vm[1] = (double)sG[(4 * Mp + m) * 2 + 1];
+      }
+      const double t1 = tc;
+      const double tau = cf[C_TAUDRAG];
+      const double imC[2] = {-mm * Cm[1], mm * Cm[0]};
+      const double imA[2] = {-mm * Am[1], mm * Am[0]};
+      const double imU[2] = {-mm * Um[1], mm * Um[0]};
+      const double imV[2] = {-mm * Vm[1], mm * Vm[0]};
+      const double imF[2] = {-mm * Fm[1], mm * Fm[0]};
+      const double imG[2] = {-mm * Gm[1], mm * Gm[0]};
+      for (int c = 0; c < 2; ++c) {
+        out[0][c] = -(t1 * imC[c]) + (a.forcflag ? (2.0 * dt) * PFm[c] : 0.0);
+        out[5][c] = t1 * Dm[c];
+        out[4][c] = -(t1 * imA[c]);
+        out[8][c] = t1 * Bm[c];
+        if (a.forcflag) {
+          out[2][c] = -((t1 * imU[c]) / tau) + t1 * imF[c];
+          out[6][c] = (t1 * Vm[c]) / tau - t1 * Gm[c];
+          out[4][c] += -((t1 * imV[c]) / tau) + t1 * imG[c];
+          out[8][c] += (t1 * Um[c]) / tau + t1 * Fm[c];
+        }
+      }
+    }
+    R* Tm = a.T + (((long long)b * Mp + m) * kNT) * J * 2;
+    for (int k = 0; k < kNT; ++k) {
+      Tm[(k * J + j) * 2] = (R)(out[k][0] * wj);
+      Tm[(k * J + j) * 2 + 1] = (R)(out[k][1] * wj);
+    }
+  }
+}
+
+template <typename R>
+__global__ void k_copy_real(double* dst, long long dstride, const R* src, long long sstride, long long n) {
+  const int b = blockIdx.y;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x)
+    dst[b * dstride + e] = (double)src[b * sstride + e];
+}
+
+// ----------------------------------------------------------------------------
+// Launchers
+// ----------------------------------------------------------------------------
+
+template <typename R>
+hipError_t launch_leg(const LegArgs<R>& a, hipStream_t s) {
+  const size_t smem = sizeof(R) * (kNT * a.J * 2 + 5 * (a.N + 1) * 2) + sizeof(double) * 2 * 128 + 16;
+  dim3 grid(a.M + 1, a.B);
+  hipLaunchKernelGGL(k_leg<R>, grid, dim3(128), smem, s, a);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_grid(const GridArgs<R>& a, hipStream_t s) {
+  const int Mp = a.M + 1;
+  const size_t smem = sizeof(R) * (2 * a.I + kNG * Mp * 2 + 10 * a.I + 10 * Mp * 2) + sizeof(double) * 4 * 256 + 16;
+  dim3 grid(a.J, a.B);
+  hipLaunchKernelGGL(k_grid<R>, grid, dim3(256), smem, s, a);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_copy_real(double* dst, long long dstride, const R* src, long long sstride, long long n, int B,
+                            hipStream_t s) {
+  if (n <= 0 || B <= 0) return hipSuccess;
+  int gx = (int)((n + 255) / 256);
+  if (gx > 1024) gx = 1024;
+  hipLaunchKernelGGL(k_copy_real<R>, dim3(gx, B), dim3(256), 0, s, dst, dstride, src, sstride, n);
+  return hipGetLastError();
+}
+
+template hipError_t launch_leg<double>(const LegArgs<double>&, hipStream_t);
+template hipError_t launch_leg<float>(const LegArgs<float>&, hipStream_t);
+template hipError_t launch_grid<double>(const GridArgs<double>&, hipStream_t);
+template hipError_t launch_grid<float>(const GridArgs<float>&, hipStream_t);
+template hipError_t launch_copy_real<double>(double*, long long, const double*, long long, long long, int, hipStream_t);
+template hipError_t launch_copy_real<float>(double*, long long, const float*, long long, long long, int, hipStream_t);
+
+}  // namespace swx

@@ -1,0 +1,604 @@
+// Plan management and the extern "C" entry points of libswampe_hip.so.
+//
+// A plan holds one truncation group (M, J, I) with B ensemble members: the
+// FP64 geometry (host), the basis/twiddle tables on the device in the
+// transform precision, per-member coefficient tables, and the device carry.
+// Stepping is two kernel launches per step (k_leg, k_grid); the forward
+// benchmark driver replays a captured HIP graph of many steps.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/swampe_capi.h"
+#include "geometry.h"
+#include "swx_kernels.h"
+
+using namespace swx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define SWX_HIP(call)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (call);                                                                        \
+    if (e_ != hipSuccess)                                                                          \
+      return fail(SWX_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));                 \
+  } while (0)
+
+struct Geometry {
+  int M = 0, N = 0, J = 0, I = 0;
+  std::vector<double> mus, w, lam, P, H;
+};
+
+bool build_geometry(int M, bool exact_basis, Geometry& g) {
+  int J, I;
+  if (!resolution(M, &J, &I, nullptr)) return false;
+  g.M = M;
+  g.N = M;
+  g.J = J;
+  g.I = I;
+  g.mus.resize(J);
+  g.w.resize(J);
+  g.lam.resize(I);
+  gauss_legendre(J, g.mus.data(), g.w.data());
+  lambdas(I, g.lam.data());
+  g.P.resize((size_t)J * (M + 1) * (M + 1));
+  g.H.resize(g.P.size());
+  legendre_basis(M, M, J, g.mus.data(), g.P.data(), g.H.data(), !exact_basis);
+  return true;
+}
+
+}  // namespace
+
+struct swx_plan {
+  swx_plan_desc d{};
+  Geometry geo;
+  int device = 0;
+  size_t rsz = 8;  // bytes per transform real
+  CoefLayout cl{};
+  GeoLayout gl{};
+  std::vector<double> params;  // [B][SWX_NPARAM]
+  // device buffers
+  void* Pf = nullptr;
+  void* Hf = nullptr;
+  void* Pi = nullptr;
+  void* Hi = nullptr;
+  void* tw = nullptr;
+  double* geo_d = nullptr;
+  double* coef = nullptr;
+  double* S = nullptr;
+  void* T = nullptr;
+  void* G = nullptr;
+  void* Gprev = nullptr;
+  void* raw = nullptr;
+  void* lag = nullptr;
+  void* g0uv = nullptr;
+  double* phys = nullptr;
+  double* uvic = nullptr;
+  double* part_uv = nullptr;
+  unsigned char* dead = nullptr;
+  bool initialized = false;
+  // captured forward graph
+  hipGraphExec_t gexec = nullptr;
+  int gexec_steps = 0;
+  hipStream_t gexec_stream = nullptr;
+};
+
+namespace {
+
+template <typename T>
+int dalloc(T** p, size_t bytes) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, bytes < 16 ? 16 : bytes);
+  if (e != hipSuccess) return fail(SWX_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  *p = static_cast<T*>(q);
+  return SWX_OK;
+}
+
+// Upload a host double array in the transform precision.
+int upload_real(void* dst, const std::vector<double>& src, size_t rsz) {
+  if (rsz == 8) {
+    hipError_t e = hipMemcpy(dst, src.data(), src.size() * 8, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(SWX_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  } else {
+    std::vector<float> f(src.size());
+    for (size_t k = 0; k < src.size(); ++k) f[k] = (float)src[k];
+    hipError_t e = hipMemcpy(dst, f.data(), f.size() * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(SWX_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  }
+  return SWX_OK;
+}
+
+// Per-member coefficient tables (model.py:498-518; time_stepping.py:491-522;
+// filters.py:95-151; forcing.py:94; time_stepping.py:568-573).
+void fill_coef(const swx_plan* p, std::vector<double>& out) {
+  const int N = p->geo.N, J = p->geo.J, B = p->d.n_members;
+  const CoefLayout& cl = p->cl;
+  out.assign((size_t)B * cl.ncoef, 0.0);
+  const double pi = 3.14159265358979323846;
+  const std::vector<double>& mus = p->geo.mus;
+  const std::vector<double>& lam = p->geo.lam;
+  for (int b = 0; b < B; ++b) {
+    const double* pr = &p->params[(size_t)b * SWX_NPARAM];
+    double* c = &out[(size_t)b * cl.ncoef];
+    const double dt = pr[SWX_P_DT], a = pr[SWX_P_A], K6 = pr[SWX_P_K6];
+    const double K6Phi = pr[SWX_P_K6PHI] < 0 ? K6 : pr[SWX_P_K6PHI];
+    c[C_DT] = dt;
+    c[C_A] = a;
+    c[C_OMEGA] = pr[SWX_P_OMEGA];
+    c[C_PHIBAR] = pr[SWX_P_PHIBAR];
+    c[C_TAURAD] = pr[SWX_P_TAURAD];
+    c[C_TAUDRAG] = pr[SWX_P_TAUDRAG];
+    c[C_DPHIEQ] = pr[SWX_P_DPHIEQ];
+    c[C_ALPHA] = pr[SWX_P_ALPHA];
+    c[C_BLOWUP] = pr[SWX_P_BLOWUP];
+    c[C_INVTD] = pr[SWX_P_TAUDRAG] == -1.0 ? 0.0 : 1.0 / pr[SWX_P_TAUDRAG];
+    c[C_AREAPLANET] = 4.0 * pi * a * a;
+    c[C_FMN01] = pr[SWX_P_OMEGA] / std::sqrt(0.375);
+    c[C_INVTAURAD] = 1.0 / pr[SWX_P_TAURAD];
+    const double a3 = a * a * a, a6 = a3 * a3;
+    for (int n = 0; n <= N; ++n) {
+      const double nv = n, n1 = n + 1.0;
+      const double ncoeff = ((nv * nv * nv) / a3) * ((n1 * n1 * n1) / a3);
+      c[cl.n_sigma + n] = 1.0 / (1.0 + (2.0 * dt * K6) * (ncoeff - 8.0 / a6));
+      c[cl.n_sigmaphi + n] = 1.0 / (1.0 + (2.0 * dt * K6Phi) * ncoeff);
+      c[cl.n_tcmn + n] = n == 0 ? 0.0 : a / (nv * n1);
+    }
+    const double dl = lam[2] - lam[1];
+    const double dp = std::asin(mus[2]) - std::asin(mus[1]);
+    for (int j = 0; j < J; ++j) {
+      c[cl.n_tc + j] = (2.0 * dt) / (a * (1.0 - mus[j] * mus[j]));
+      const double sp = std::sin(std::asin(mus[j]) + pi / 2.0);
+      c[cl.n_areac + j] = (a * a) * (sp * sp) * dp * dl;
+    }
+  }
+}
+
+template <typename R>
+LegArgs<R> leg_args(swx_plan* p, int mode) {
+  LegArgs<R> a{};
+  a.M = p->geo.M;
+  a.N = p->geo.N;
+  a.J = p->geo.J;
+  a.B = p->d.n_members;
+  a.mode = mode;
+  a.expflag = p->d.expflag;
+  a.diffflag = p->d.diffflag;
+  a.diagnostics = p->d.diagnostics;
+  a.Pf = (const R*)p->Pf;
+  a.Hf = (const R*)p->Hf;
+  a.Pi = (const R*)p->Pi;
+  a.Hi = (const R*)p->Hi;
+  a.coef = p->coef;
+  a.ncoef = p->cl.ncoef;
+  a.S = p->S;
+  a.T = (const R*)p->T;
+  a.G = (R*)p->G;
+  a.part_uv = p->part_uv;
+  a.dead = p->dead;
+  return a;
+}
+
+template <typename R>
+GridArgs<R> grid_args(swx_plan* p, int mode) {
+  GridArgs<R> a{};
+  a.M = p->geo.M;
+  a.N = p->geo.N;
+  a.J = p->geo.J;
+  a.I = p->geo.I;
+  a.B = p->d.n_members;
+  a.mode = mode;
+  a.expflag = p->d.expflag;
+  a.forcflag = p->d.forcflag;
+  a.test = p->d.test;
+  a.diagnostics = p->d.diagnostics;
+  a.phys_on = 1;
+  a.tw = (const R*)p->tw;
+  a.geo = p->geo_d;
+  a.coef = p->coef;
+  a.ncoef = p->cl.ncoef;
+  a.G = (const R*)p->G;
+  a.Gout = (R*)p->G;
+  a.T = (R*)p->T;
+  a.phys = p->phys;
+  a.uvic = p->uvic;
+  a.part_uv = p->part_uv;
+  a.dead = p->dead;
+  a.raw = (R*)p->raw;
+  a.lag = (R*)p->lag;
+  a.g0uv = (R*)p->g0uv;
+  return a;
+}
+
+template <typename R>
+int do_init(swx_plan* p, const double* eta0, const double* delta0, const double* Phi0, const double* U0,
+            const double* V0, hipStream_t s) {
+  const int have_uv = (U0 != nullptr);
+  const int B = p->d.n_members;
+  SWX_HIP(hipMemsetAsync(p->dead, 0, (size_t)B, s));
+  // 1) DFT of the initial fields -> raw Fourier (G) and Legendre inputs (T slots 0..2).
+  GridArgs<R> g = grid_args<R>(p, GRID_DFT);
+  g.in_eta = eta0;
+  g.in_delta = delta0;
+  g.in_Phi = Phi0;
+  g.in_U = U0;
+  g.in_V = V0;
+  g.have_uv = have_uv;
+  SWX_HIP(launch_grid<R>(g, s));
+  // 2) Spectral state (and, when diagnosing, the wind Fourier coefficients).
+  LegArgs<R> l = leg_args<R>(p, LEG_INIT);
+  l.diag_winds = !have_uv;
+  SWX_HIP(launch_leg<R>(l, s));
+  // 3) Physical carry, forcing, nonlinear products, first Legendre inputs.
+  GridArgs<R> gi = grid_args<R>(p, GRID_INIT);
+  gi.in_eta = eta0;
+  gi.in_delta = delta0;
+  gi.in_Phi = Phi0;
+  gi.in_U = U0;
+  gi.in_V = V0;
+  gi.have_uv = have_uv;
+  gi.diag_winds = !have_uv;
+  SWX_HIP(launch_grid<R>(gi, s));
+  const size_t gbytes = (size_t)B * kNG * p->geo.J * (p->geo.M + 1) * 2 * p->rsz;
+  SWX_HIP(hipMemcpyAsync(p->Gprev, p->G, gbytes, hipMemcpyDeviceToDevice, s));
+  p->initialized = true;
+  return SWX_OK;
+}
+
+template <typename R>
+int do_steps(swx_plan* p, int t0, int nsteps, double* hist_fields, double* hist_scal, double* hist_phiext,
+             unsigned char* hist_dead, int hist_T, int hist_k0, bool save_last, hipStream_t s) {
+  const int B = p->d.n_members;
+  const size_t gbytes = (size_t)B * kNG * p->geo.J * (p->geo.M + 1) * 2 * p->rsz;
+  for (int k = 0; k < nsteps; ++k) {
+    const int t = t0 + k;
+    const bool last = save_last && (k == nsteps - 1);
+    if (last) SWX_HIP(hipMemcpyAsync(p->Gprev, p->G, gbytes, hipMemcpyDeviceToDevice, s));
+    LegArgs<R> l = leg_args<R>(p, LEG_STEP);
+    l.hist_scal = hist_scal;
+    l.hist_dead = hist_dead;
+    l.hist_T = hist_T;
+    l.hist_k = hist_k0 + k;
+    SWX_HIP(launch_leg<R>(l, s));
+    GridArgs<R> g = grid_args<R>(p, GRID_STEP);
+    g.do_ra = (p->d.modalflag && t > 2) ? 1 : 0;
+    g.export_raw = last ? 1 : 0;
+    g.hist_fields = hist_fields;
+    g.hist_phiext = hist_phiext;
+    g.hist_T = hist_T;
+    g.hist_k = hist_k0 + k;
+    SWX_HIP(launch_grid<R>(g, s));
+  }
+  return SWX_OK;
+}
+
+int check_plan(swx_plan* p) {
+  if (!p) return fail(SWX_ERR_ARG, "null plan");
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(SWX_ERR_HIP, "hipGetDevice failed");
+  if (dev != p->device) return fail(SWX_ERR_STATE, "plan used on a different device than the one it was created on");
+  return SWX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* swx_last_error(void) { return g_err.c_str(); }
+
+const char* swx_version(void) { return "swampe-hip 0.1.0 (gfx950)"; }
+
+int swx_grid_size(int M, int* J, int* I) {
+  if (!resolution(M, J, I, nullptr))
+    return fail(SWX_ERR_ARG, "Unsupported value of M=" + std::to_string(M) + ". Only 42, 63, and 106 are supported.");
+  return SWX_OK;
+}
+
+int swx_geometry(int M, int exact_basis, double* mus, double* w, double* lambdas_out, double* Pmn, double* Hmn) {
+  Geometry g;
+  if (!build_geometry(M, exact_basis != 0, g))
+    return fail(SWX_ERR_ARG, "Unsupported value of M=" + std::to_string(M) + ". Only 42, 63, and 106 are supported.");
+  if (mus) std::memcpy(mus, g.mus.data(), g.J * sizeof(double));
+  if (w) std::memcpy(w, g.w.data(), g.J * sizeof(double));
+  if (lambdas_out) std::memcpy(lambdas_out, g.lam.data(), g.I * sizeof(double));
+  if (Pmn) std::memcpy(Pmn, g.P.data(), g.P.size() * sizeof(double));
+  if (Hmn) std::memcpy(Hmn, g.H.data(), g.H.size() * sizeof(double));
+  return SWX_OK;
+}
+
+int swx_plan_create(const swx_plan_desc* desc, const double* params, swx_plan** out) {
+  if (!desc || !params || !out) return fail(SWX_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (desc->n_members < 1) return fail(SWX_ERR_ARG, "n_members must be >= 1");
+  if (desc->precision != SWX_PREC_FP64 && desc->precision != SWX_PREC_FP32)
+    return fail(SWX_ERR_ARG, "unsupported precision " + std::to_string(desc->precision));
+  if (!(desc->test == -1 || desc->test == 1 || desc->test == 2))
+    return fail(SWX_ERR_ARG, "test must be -1 (None), 1 or 2");
+  swx_plan* p = new swx_plan();
+  p->d = *desc;
+  if (!build_geometry(desc->M, desc->exact_basis != 0, p->geo)) {
+    delete p;
+    return fail(SWX_ERR_ARG, "Unsupported value of M=" + std::to_string(desc->M) + ". Only 42, 63, and 106 are supported.");
+  }
+  if (hipGetDevice(&p->device) != hipSuccess) {
+    delete p;
+    return fail(SWX_ERR_HIP, "hipGetDevice failed");
+  }
+  const int M = p->geo.M, N = p->geo.N, J = p->geo.J, I = p->geo.I, B = desc->n_members;
+  const int Mp = M + 1;
+  p->rsz = desc->precision == SWX_PREC_FP64 ? 8 : 4;
+  p->cl = CoefLayout::make(N, J);
+  p->gl = GeoLayout::make(J, I);
+  p->params.assign(params, params + (size_t)B * SWX_NPARAM);
+
+  // Triangular basis blocks in both layouts, w folded into the Legendre inputs.
+  const long long tri = tri_off(M + 1, N, J);
+  std::vector<double> Pf(tri), Hf(tri), Pi(tri), Hi(tri);
+  for (int m = 0; m <= M; ++m) {
+    const long long off = tri_off(m, N, J);
+    const int K = N + 1 - m;
+    for (int j = 0; j < J; ++j)
+      for (int np = 0; np < K; ++np) {
+        const size_t src = ((size_t)j * Mp + m) * (N + 1) + (m + np);
+        Pf[off + (long long)j * K + np] = p->geo.P[src];
+        Hf[off + (long long)j * K + np] = p->geo.H[src];
+        Pi[off + (long long)np * J + j] = p->geo.P[src];
+        Hi[off + (long long)np * J + j] = p->geo.H[src];
+      }
+  }
+  std::vector<double> tw(2 * I);
+  const double pi = 3.14159265358979323846;
+  for (int k = 0; k < I; ++k) {
+    tw[k] = std::cos(2.0 * pi * k / I);
+    tw[I + k] = std::sin(2.0 * pi * k / I);
+  }
+  std::vector<double> geo(p->gl.n);
+  for (int j = 0; j < J; ++j) {
+    const double mu = p->geo.mus[j];
+    geo[p->gl.mus + j] = mu;
+    geo[p->gl.w + j] = p->geo.w[j];
+    geo[p->gl.s1m + j] = std::sqrt(1 - mu * mu);
+    geo[p->gl.cosphi + j] = std::cos(std::asin(mu));
+    geo[p->gl.denomE + j] = 2.0 * (1.0 - mu * mu);
+  }
+  for (int i = 0; i < I; ++i) {
+    const double l = p->geo.lam[i];
+    geo[p->gl.cosl + i] = std::cos(l);
+    geo[p->gl.day + i] = (l > -pi / 2 && l < pi / 2) ? 1.0 : 0.0;
+  }
+  std::vector<double> coef;
+  fill_coef(p, coef);
+
+  int rc = SWX_OK;
+  const size_t rs = p->rsz;
+  const size_t cplxJM = (size_t)J * Mp * 2;
+  if ((rc = dalloc(&p->Pf, tri * rs)) || (rc = dalloc(&p->Hf, tri * rs)) || (rc = dalloc(&p->Pi, tri * rs)) ||
+      (rc = dalloc(&p->Hi, tri * rs)) || (rc = dalloc(&p->tw, 2 * I * rs)) ||
+      (rc = dalloc(&p->geo_d, geo.size() * 8)) || (rc = dalloc(&p->coef, coef.size() * 8)) ||
+      (rc = dalloc(&p->S, (size_t)B * 2 * 3 * Mp * (N + 1) * 2 * 8)) ||
+      (rc = dalloc(&p->T, (size_t)B * Mp * kNT * J * 2 * rs)) || (rc = dalloc(&p->G, (size_t)B * kNG * cplxJM * rs)) ||
+      (rc = dalloc(&p->Gprev, (size_t)B * kNG * cplxJM * rs)) || (rc = dalloc(&p->raw, (size_t)B * kNRaw * cplxJM * rs)) ||
+      (rc = dalloc(&p->lag, (size_t)B * 2 * cplxJM * rs)) || (rc = dalloc(&p->g0uv, (size_t)B * 2 * cplxJM * rs)) ||
+      (rc = dalloc(&p->phys, (size_t)B * 8 * J * I * 8)) || (rc = dalloc(&p->uvic, (size_t)B * 2 * J * I * 8)) ||
+      (rc = dalloc(&p->part_uv, (size_t)B * J * 2 * 8)) || (rc = dalloc(&p->dead, (size_t)B))) {
+    swx_plan_destroy(p);
+    return rc;
+  }
+  if ((rc = upload_real(p->Pf, Pf, rs)) || (rc = upload_real(p->Hf, Hf, rs)) || (rc = upload_real(p->Pi, Pi, rs)) ||
+      (rc = upload_real(p->Hi, Hi, rs)) || (rc = upload_real(p->tw, tw, rs)) ||
+      (rc = upload_real(p->geo_d, geo, 8)) || (rc = upload_real(p->coef, coef, 8))) {
+    swx_plan_destroy(p);
+    return rc;
+  }
+  if (hipMemset(p->dead, 0, (size_t)B) != hipSuccess || hipMemset(p->part_uv, 0, (size_t)B * J * 2 * 8) != hipSuccess) {
+    swx_plan_destroy(p);
+    return fail(SWX_ERR_HIP, "hipMemset failed");
+  }
+  *out = p;
+  return SWX_OK;
+}
+
+int swx_plan_set_params(swx_plan* p, const double* params, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!params) return fail(SWX_ERR_ARG, "null params");
+  p->params.assign(params, params + (size_t)p->d.n_members * SWX_NPARAM);
+  std::vector<double> coef;
+  fill_coef(p, coef);
+  hipStream_t s = (hipStream_t)stream;
+  SWX_HIP(hipMemcpyAsync(p->coef, coef.data(), coef.size() * 8, hipMemcpyHostToDevice, s));
+  SWX_HIP(hipStreamSynchronize(s));
+  return SWX_OK;
+}
+
+void swx_plan_destroy(swx_plan* p) {
+  if (!p) return;
+  if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  void* bufs[] = {p->Pf, p->Hf, p->Pi, p->Hi, p->tw, p->geo_d, p->coef, p->S, p->T, p->G, p->Gprev,
+                  p->raw, p->lag, p->g0uv, p->phys, p->uvic, p->part_uv, p->dead};
+  for (void* q : bufs)
+    if (q) (void)hipFree(q);
+  delete p;
+}
+
+int swx_init(swx_plan* p, const double* eta0, const double* delta0, const double* Phi0, const double* U0,
+             const double* V0, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!eta0 || !delta0 || !Phi0) return fail(SWX_ERR_ARG, "eta0, delta0 and Phi0 are required");
+  if ((U0 == nullptr) != (V0 == nullptr)) return fail(SWX_ERR_ARG, "Provide both U0 and V0, or neither.");
+  hipStream_t s = (hipStream_t)stream;
+  if (p->rsz == 8) return do_init<double>(p, eta0, delta0, Phi0, U0, V0, s);
+  return do_init<float>(p, eta0, delta0, Phi0, U0, V0, s);
+}
+
+int swx_run(swx_plan* p, int t0, int nsteps, double* hist_fields, double* hist_scalars, double* hist_phiext,
+            unsigned char* hist_dead, int hist_T, int hist_k0, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!p->initialized) return fail(SWX_ERR_STATE, "swx_run before swx_init");
+  if (nsteps < 0) return fail(SWX_ERR_ARG, "nsteps must be >= 0");
+  const bool any_hist = hist_fields || hist_scalars || hist_phiext || hist_dead;
+  if (any_hist && (hist_T <= 0 || hist_k0 < 0 || hist_k0 + nsteps > hist_T))
+    return fail(SWX_ERR_ARG, "history window out of range");
+  hipStream_t s = (hipStream_t)stream;
+  if (p->rsz == 8)
+    return do_steps<double>(p, t0, nsteps, hist_fields, hist_scalars, hist_phiext, hist_dead, hist_T, hist_k0, true, s);
+  return do_steps<float>(p, t0, nsteps, hist_fields, hist_scalars, hist_phiext, hist_dead, hist_T, hist_k0, true, s);
+}
+
+int swx_run_graph(swx_plan* p, int t0, int nsteps, int graph_steps, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!p->initialized) return fail(SWX_ERR_STATE, "swx_run_graph before swx_init");
+  if (nsteps <= 0) return SWX_OK;
+  hipStream_t s = (hipStream_t)stream;
+  auto steps = [&](int ts, int n, bool save_last) {
+    if (p->rsz == 8) return do_steps<double>(p, ts, n, nullptr, nullptr, nullptr, nullptr, 0, 0, save_last, s);
+    return do_steps<float>(p, ts, n, nullptr, nullptr, nullptr, nullptr, 0, 0, save_last, s);
+  };
+  // Steps with t <= 2 differ (no Robert-Asselin filter) and the last step
+  // saves the previous Fourier level; both run eagerly.
+  int t = t0, left = nsteps;
+  while (left > 1 && t <= 2) {
+    if ((rc = steps(t, 1, false))) return rc;
+    ++t;
+    --left;
+  }
+  if (graph_steps > 0 && left - 1 >= graph_steps) {
+    if (!p->gexec || p->gexec_steps != graph_steps || p->gexec_stream != s) {
+      if (p->gexec) {
+        (void)hipGraphExecDestroy(p->gexec);
+        p->gexec = nullptr;
+      }
+      hipGraph_t graph = nullptr;
+      SWX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      rc = steps(t, graph_steps, false);
+      hipError_t ec = hipStreamEndCapture(s, &graph);
+      if (rc) return rc;
+      if (ec != hipSuccess) return fail(SWX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+      ec = hipGraphInstantiate(&p->gexec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ec != hipSuccess) return fail(SWX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ec));
+      p->gexec_steps = graph_steps;
+      p->gexec_stream = s;
+    }
+    while (left - 1 >= graph_steps) {
+      SWX_HIP(hipGraphLaunch(p->gexec, s));
+      t += graph_steps;
+      left -= graph_steps;
+    }
+  }
+  if (left > 1) {
+    if ((rc = steps(t, left - 1, false))) return rc;
+    t += left - 1;
+    left = 1;
+  }
+  return steps(t, left, true);
+}
+
+int swx_time_kernels(swx_plan* p, int t0, int nsteps, float* ms_leg, float* ms_grid, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!p->initialized) return fail(SWX_ERR_STATE, "swx_time_kernels before swx_init");
+  if (nsteps <= 0 || !ms_leg || !ms_grid) return fail(SWX_ERR_ARG, "nsteps must be > 0 and outputs non-null");
+  hipStream_t s = (hipStream_t)stream;
+  hipEvent_t ev[3];
+  for (auto& e : ev) SWX_HIP(hipEventCreate(&e));
+  double tl = 0.0, tg = 0.0;
+  for (int k = 0; k < nsteps; ++k) {
+    const int t = t0 + k;
+    SWX_HIP(hipEventRecord(ev[0], s));
+    if (p->rsz == 8) {
+      LegArgs<double> l = leg_args<double>(p, LEG_STEP);
+      SWX_HIP(launch_leg<double>(l, s));
+    } else {
+      LegArgs<float> l = leg_args<float>(p, LEG_STEP);
+      SWX_HIP(launch_leg<float>(l, s));
+    }
+    SWX_HIP(hipEventRecord(ev[1], s));
+    if (p->rsz == 8) {
+      GridArgs<double> g = grid_args<double>(p, GRID_STEP);
+      g.do_ra = (p->d.modalflag && t > 2) ? 1 : 0;
+      SWX_HIP(launch_grid<double>(g, s));
+    } else {
+      GridArgs<float> g = grid_args<float>(p, GRID_STEP);
+      g.do_ra = (p->d.modalflag && t > 2) ? 1 : 0;
+      SWX_HIP(launch_grid<float>(g, s));
+    }
+    SWX_HIP(hipEventRecord(ev[2], s));
+    SWX_HIP(hipEventSynchronize(ev[2]));
+    float a = 0.f, b = 0.f;
+    SWX_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+    SWX_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
+    tl += a;
+    tg += b;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  *ms_leg = (float)(tl / nsteps);
+  *ms_grid = (float)(tg / nsteps);
+  return SWX_OK;
+}
+
+int swx_export_state(swx_plan* p, const swx_state_out* o, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!o) return fail(SWX_ERR_ARG, "null output struct");
+  if (!p->initialized) return fail(SWX_ERR_STATE, "export before swx_init");
+  hipStream_t s = (hipStream_t)stream;
+  const int B = p->d.n_members, J = p->geo.J, I = p->geo.I, Mp = p->geo.M + 1, Np = p->geo.N + 1;
+  const long long cjm = (long long)J * Mp * 2;  // doubles per complex (J, M+1) field
+  const long long pl = (long long)J * I;
+  auto cp = [&](double* dst, const void* base, int nfield, int f, long long per) -> int {
+    if (!dst) return SWX_OK;
+    hipError_t e;
+    if (p->rsz == 8)
+      e = launch_copy_real<double>(dst, per, (const double*)base + f * per, nfield * per, per, B, s);
+    else
+      e = launch_copy_real<float>(dst, per, (const float*)base + f * per, nfield * per, per, B, s);
+    if (e != hipSuccess) return fail(SWX_ERR_HIP, std::string("export copy: ") + hipGetErrorString(e));
+    return SWX_OK;
+  };
+  auto cpd = [&](double* dst, const double* base, int nfield, int f, long long per) -> int {
+    if (!dst) return SWX_OK;
+    hipError_t e = launch_copy_real<double>(dst, per, base + f * per, nfield * per, per, B, s);
+    if (e != hipSuccess) return fail(SWX_ERR_HIP, std::string("export copy: ") + hipGetErrorString(e));
+    return SWX_OK;
+  };
+  const bool pinned = p->d.test == 1;
+  if ((rc = cp(o->etam_curr, p->G, kNG, 0, cjm)) || (rc = cp(o->deltam_curr, p->G, kNG, 1, cjm)) ||
+      (rc = cp(o->Phim_curr, p->G, kNG, 2, cjm)) || (rc = cp(o->etam_prev, p->Gprev, kNG, 0, cjm)) ||
+      (rc = cp(o->deltam_prev, p->Gprev, kNG, 1, cjm)) || (rc = cp(o->Phim_prev, p->Gprev, kNG, 2, cjm)) ||
+      (rc = cp(o->Um_curr, pinned ? p->g0uv : p->G, pinned ? 2 : kNG, pinned ? 0 : 3, cjm)) ||
+      (rc = cp(o->Vm_curr, pinned ? p->g0uv : p->G, pinned ? 2 : kNG, pinned ? 1 : 4, cjm)) ||
+      (rc = cp(o->Am_curr, p->raw, kNRaw, 0, cjm)) || (rc = cp(o->Bm_curr, p->raw, kNRaw, 1, cjm)) ||
+      (rc = cp(o->Cm_curr, p->raw, kNRaw, 2, cjm)) || (rc = cp(o->Dm_curr, p->raw, kNRaw, 3, cjm)) ||
+      (rc = cp(o->Em_curr, p->raw, kNRaw, 4, cjm)) || (rc = cp(o->PhiFm_curr, p->raw, kNRaw, 5, cjm)) ||
+      (rc = cp(o->Fm_curr, p->raw, kNRaw, 6, cjm)) || (rc = cp(o->Gm_curr, p->raw, kNRaw, 7, cjm)) ||
+      (rc = cp(o->Rum_lag, p->lag, 2, 0, cjm)) || (rc = cp(o->Rvm_lag, p->lag, 2, 1, cjm)))
+    return rc;
+  if ((rc = cpd(o->eta_prev, p->phys, 8, 0, pl)) || (rc = cpd(o->eta_curr, p->phys, 8, 1, pl)) ||
+      (rc = cpd(o->delta_prev, p->phys, 8, 2, pl)) || (rc = cpd(o->delta_curr, p->phys, 8, 3, pl)) ||
+      (rc = cpd(o->Phi_prev, p->phys, 8, 4, pl)) || (rc = cpd(o->Phi_curr, p->phys, 8, 5, pl)) ||
+      (rc = cpd(o->U_curr, p->phys, 8, 6, pl)) || (rc = cpd(o->V_curr, p->phys, 8, 7, pl)))
+    return rc;
+  if (o->spec) {
+    // Current spectral level: [B][2][3][M+1][N+1] -> [B][3][M+1][N+1].
+    const long long per = 3LL * Mp * Np * 2;
+    if ((rc = cpd(o->spec, p->S + per, 2, 0, per))) return rc;
+  }
+  if (o->dead) SWX_HIP(hipMemcpyAsync(o->dead, p->dead, (size_t)B, hipMemcpyDeviceToDevice, s));
+  return SWX_OK;
+}
+
+}  // extern "C"

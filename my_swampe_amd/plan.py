@@ -1,0 +1,161 @@
+"""Torch-facing wrapper of a native plan (one truncation group, B members).
+
+PyTorch provides device memory and the stream; all arithmetic happens in the
+HIP kernels of libswampe_hip.so.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Dict, Optional
+
+import numpy as np
+
+from . import _capi
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def require_gpu():
+    """Return torch after checking that a ROCm GPU is usable; raise otherwise."""
+    torch = _torch()
+    if not torch.cuda.is_available():
+        raise RuntimeError("my_swampe_amd needs a ROCm GPU (torch.cuda.is_available() is False); "
+                           "there is no CPU fallback.")
+    return torch
+
+
+def _ptr(t) -> Optional[int]:
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+class Plan:
+    """One native plan: geometry, basis tables and the device carry of B members."""
+
+    def __init__(self, M: int, params: np.ndarray, *, precision: str = "fp64", expflag: bool = False,
+                 forcflag: bool = True, diffflag: bool = True, modalflag: bool = True,
+                 diagnostics: bool = False, test: Optional[int] = None, exact_basis: bool = False):
+        torch = require_gpu()
+        lib = _capi.load()
+        params = np.ascontiguousarray(np.asarray(params, dtype=np.float64))
+        if params.ndim == 1:
+            params = params[None, :]
+        if params.shape[1] != _capi.NPARAM:
+            raise ValueError(f"params must have {_capi.NPARAM} columns {_capi.PARAM_NAMES}")
+        if precision not in _capi.PRECISIONS:
+            raise ValueError(f"precision must be one of {tuple(_capi.PRECISIONS)}")
+        self.M = int(M)
+        self.J, self.I = _capi.grid_size(self.M)
+        self.B = params.shape[0]
+        self.precision = precision
+        self.test = test
+        self.device = torch.device("cuda", torch.cuda.current_device())
+        desc = _capi.PlanDesc(M=self.M, n_members=self.B, precision=_capi.PRECISIONS[precision],
+                              expflag=int(bool(expflag)), forcflag=int(bool(forcflag)),
+                              diffflag=int(bool(diffflag)), modalflag=int(bool(modalflag)),
+                              diagnostics=int(bool(diagnostics)), test=-1 if test is None else int(test),
+                              exact_basis=int(bool(exact_basis)))
+        handle = ctypes.c_void_p()
+        _capi.check(lib.swx_plan_create(ctypes.byref(desc), params.ctypes.data_as(ctypes.c_void_p),
+                                        ctypes.byref(handle)), "swx_plan_create")
+        self._h = handle
+        self._lib = lib
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.swx_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover - best effort
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _stream(self, stream=None):
+        torch = _torch()
+        s = stream if stream is not None else torch.cuda.current_stream(self.device)
+        return ctypes.c_void_p(s.cuda_stream)
+
+    def _field(self, x, name):
+        torch = _torch()
+        if x is None:
+            return None
+        t = torch.as_tensor(x, dtype=torch.float64, device=self.device)
+        if t.dim() == 2:
+            t = t.expand(self.B, -1, -1)
+        if tuple(t.shape) != (self.B, self.J, self.I):
+            raise ValueError(f"{name} must have shape {(self.J, self.I)} or {(self.B, self.J, self.I)}, "
+                             f"got {tuple(t.shape)}.")
+        return t.contiguous()
+
+    # -- operations -------------------------------------------------------
+    def set_params(self, params: np.ndarray, stream=None) -> None:
+        params = np.ascontiguousarray(np.asarray(params, dtype=np.float64).reshape(self.B, _capi.NPARAM))
+        _capi.check(self._lib.swx_plan_set_params(self._h, params.ctypes.data_as(ctypes.c_void_p),
+                                                  self._stream(stream)), "swx_plan_set_params")
+
+    def init(self, eta0, delta0, Phi0, U0=None, V0=None, stream=None) -> None:
+        if (U0 is None) != (V0 is None):
+            raise ValueError("Provide both U0_init and V0_init, or neither.")
+        e, d, p = self._field(eta0, "eta0"), self._field(delta0, "delta0"), self._field(Phi0, "Phi0")
+        u, v = self._field(U0, "U0"), self._field(V0, "V0")
+        self._keep = (e, d, p, u, v)  # inputs must outlive the asynchronous launches
+        _capi.check(self._lib.swx_init(self._h, _ptr(e), _ptr(d), _ptr(p), _ptr(u), _ptr(v),
+                                       self._stream(stream)), "swx_init")
+
+    def run(self, t0: int, nsteps: int, *, history: bool = False, stream=None) -> Dict[str, object]:
+        """Advance nsteps steps; with history, return per-step fields and diagnostics (device tensors)."""
+        torch = _torch()
+        out: Dict[str, object] = {}
+        hf = hs = hp = hd = None
+        if history and nsteps > 0:
+            hf = torch.empty((self.B, 5, nsteps, self.J, self.I), dtype=torch.float64, device=self.device)
+            hs = torch.zeros((self.B, nsteps, 4), dtype=torch.float64, device=self.device)
+            hp = torch.empty((self.B, nsteps, self.J, 2), dtype=torch.float64, device=self.device)
+            hd = torch.zeros((self.B, nsteps), dtype=torch.uint8, device=self.device)
+            out = dict(fields=hf, scalars=hs, phiext=hp, dead=hd)
+        _capi.check(self._lib.swx_run(self._h, int(t0), int(nsteps), _ptr(hf), _ptr(hs), _ptr(hp), _ptr(hd),
+                                      int(nsteps) if history else 0, 0, self._stream(stream)), "swx_run")
+        return out
+
+    def run_graph(self, t0: int, nsteps: int, graph_steps: int = 32, stream=None) -> None:
+        """Forward-only stepping that replays a captured HIP graph (needs a non-default stream)."""
+        _capi.check(self._lib.swx_run_graph(self._h, int(t0), int(nsteps), int(graph_steps),
+                                            self._stream(stream)), "swx_run_graph")
+
+    def time_kernels(self, t0: int, nsteps: int, stream=None):
+        """Average (ms) of the Legendre and grid kernels over nsteps eager steps (HIP events)."""
+        a, b = ctypes.c_float(), ctypes.c_float()
+        _capi.check(self._lib.swx_time_kernels(self._h, int(t0), int(nsteps), ctypes.byref(a), ctypes.byref(b),
+                                               self._stream(stream)), "swx_time_kernels")
+        return a.value, b.value
+
+    def export(self, fields=None, spec: bool = False, stream=None) -> Dict[str, object]:
+        """Copy the carry out as complex128/float64 device tensors with leading member axis."""
+        torch = _torch()
+        names = _capi.STATE_FIELDS if fields is None else tuple(fields)
+        out = {}
+        st = _capi.StateOut()
+        Mp = self.M + 1
+        for name in names:
+            if name.endswith("m_prev") or name.endswith("m_curr") or name.endswith("_lag"):
+                t = torch.empty((self.B, self.J, Mp), dtype=torch.complex128, device=self.device)
+            else:
+                t = torch.empty((self.B, self.J, self.I), dtype=torch.float64, device=self.device)
+            out[name] = t
+            setattr(st, name, t.data_ptr())
+        dead = torch.zeros((self.B,), dtype=torch.uint8, device=self.device)
+        st.dead = dead.data_ptr()
+        out["dead"] = dead
+        if spec:
+            sp = torch.empty((self.B, 3, Mp, Mp), dtype=torch.complex128, device=self.device)
+            st.spec = sp.data_ptr()
+            out["spec"] = sp
+        _capi.check(self._lib.swx_export_state(self._h, ctypes.byref(st), self._stream(stream)), "swx_export_state")
+        return out

@@ -1,0 +1,247 @@
+"""HIP path vs the upstream-SWAMPE fixtures and the FP64 oracle (GPU).
+
+Tolerances: the reference's parity contract (eta/delta 1e-10, Phi 5e-8,
+U/V 1e-9 atol; unit_tests/test_parity_reference_regression.py:58-64) for the
+FP64 path on short runs; relative L2 bounds, written in each test, for the
+FP32 transform path.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import ATOL, FIXTURE_CASES, GOLDEN, phase_curve_from_phi
+from oracle import swampe_oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("eta", "delta", "Phi", "U", "V")
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _print_case(name, **kw):
+    print(f"[gpu-case] {name} {kw}", flush=True)
+
+
+@pytest.mark.parametrize("case", list(FIXTURE_CASES))
+def test_fixture_terminal_and_snapshots(case, torch_gpu):
+    """unit_tests/test_parity_reference_regression.py:96-130 on the HIP FP64 path."""
+    from my_swampe_amd.model import run_model_scan
+
+    _print_case("fixture", case=case)
+    ref = np.load(GOLDEN / f"{case}.npz")
+    out = run_model_scan(**FIXTURE_CASES[case], diagnostics=True, jit_scan=False)
+    last = out["last_state"]
+    for f in FIELDS:
+        np.testing.assert_allclose(_np(getattr(last, f + "_curr")), ref[f"final_{f}"], rtol=0.0, atol=ATOL[f])
+    t_seq = _np(out["t_seq"])
+    for i, step in enumerate(ref["snapshot_steps"]):
+        idx = int(np.where(t_seq == step)[0][0])
+        for f in FIELDS:
+            np.testing.assert_allclose(_np(out["outs"][f][idx]), ref[f"snapshot_{f}"][i], rtol=0.0, atol=ATOL[f])
+
+
+@pytest.mark.parametrize("case", list(FIXTURE_CASES))
+def test_fixture_phase_curve(case, torch_gpu):
+    """unit_tests/test_parity_reference_regression.py:133-148 on the HIP FP64 path."""
+    from my_swampe_amd.model import run_model_scan_final
+
+    _print_case("phase", case=case)
+    ref = np.load(GOLDEN / f"{case}.npz")
+    out = run_model_scan_final(**FIXTURE_CASES[case], diagnostics=False, jit_scan=False)
+    curve = phase_curve_from_phi(_np(out["last_state"].Phi_curr), ref["phase_angles"])
+    np.testing.assert_allclose(curve, ref["phase_curve"], rtol=0.0, atol=1.0e-10)
+
+
+@pytest.mark.parametrize("case", list(FIXTURE_CASES))
+def test_fixture_legacy_diagnostics(case, torch_gpu):
+    """Legacy (tmax, 2) spinup/geopot arrays of run_model (model.py:2105-2158) vs the fixtures."""
+    from my_swampe_amd.model import run_model_gpu
+
+    _print_case("legacy", case=case)
+    ref = np.load(GOLDEN / f"{case}.npz")
+    kw = dict(FIXTURE_CASES[case])
+    res = run_model_gpu(kw.pop("M"), kw.pop("dt"), kw.pop("tmax"), kw.pop("Phibar"), kw.pop("omega"), kw.pop("a"),
+                        verbose=False, **kw)
+    np.testing.assert_allclose(_np(res["spinup"]), ref["spinup"], rtol=1e-11, atol=1e-9)
+    np.testing.assert_allclose(_np(res["geopot"]), ref["geopot"], rtol=1e-11, atol=5e-8)
+
+
+BASE = dict(M=42, dt=1200.0, tmax=9, Phibar=3.0e5, omega=3.2e-5, a=8.2e7, test=None, g=9.8, forcflag=True,
+            taurad=36000.0, taudrag=21600.0, DPhieq=1.0e6, diffflag=True, modalflag=True, alpha=0.01,
+            expflag=False, K6=1.24e33)
+VARIANTS = {
+    "modeuler_forced": dict(),
+    "modeuler_unforced": dict(forcflag=False),
+    "explicit_forced": dict(expflag=True, dt=120.0),
+    "explicit_unforced": dict(expflag=True, forcflag=False, dt=120.0),
+    "no_diffusion_no_modal": dict(diffflag=False, modalflag=False),
+    "taudrag_sentinel": dict(taudrag=-1.0),
+    "test2": dict(test=2, dt=30.0, Phibar=3e3 * 9.8, omega=7.2921159e-5, a=6.37122e6),
+    "test1_explicit": dict(test=1, expflag=True, forcflag=False, diffflag=False, dt=30.0, Phibar=3e3,
+                           omega=7.2921159e-5, a=6.37122e6),
+}
+
+
+def _scale_atol(ref_arr, atol_rel):
+    return atol_rel * max(1.0, float(np.max(np.abs(ref_arr))))
+
+
+@pytest.mark.parametrize("name", list(VARIANTS))
+def test_scheme_variants_vs_oracle(name, torch_gpu):
+    """Flag/scheme variants vs the oracle: every physical and Fourier carry field, 7 steps."""
+    from my_swampe_amd.model import run_model_scan
+
+    kw = dict(BASE)
+    kw.update(VARIANTS[name])
+    _print_case("variant", name=name)
+    ref = orc.run_model_scan(**kw)
+    got = run_model_scan(**kw, diagnostics=True)
+    last, rl = got["last_state"], ref["last_state"]
+    # Relative-to-magnitude bound 1e-12 (the FP64 parity contract is 5e-8 on Phi ~1e5, i.e. 5e-13).
+    for fld in ("eta_curr", "delta_curr", "Phi_curr", "U_curr", "V_curr", "eta_prev", "delta_prev", "Phi_prev",
+                "etam_curr", "deltam_curr", "Phim_curr", "etam_prev", "Phim_prev", "Um_curr", "Vm_curr",
+                "Am_curr", "Bm_curr", "Cm_curr", "Dm_curr", "Em_curr", "PhiFm_curr", "Fm_curr", "Gm_curr",
+                "Rum_lag", "Rvm_lag"):
+        r = getattr(rl, fld)
+        np.testing.assert_allclose(_np(getattr(last, fld)), r, rtol=0, atol=_scale_atol(r, 1e-12), err_msg=fld)
+    for f in FIELDS:
+        r = ref["outs"][f]
+        np.testing.assert_allclose(_np(got["outs"][f]), r, rtol=0, atol=_scale_atol(r, 1e-12), err_msg=f)
+    for f in ("rms", "spin_min", "phi_min", "phi_max"):
+        np.testing.assert_allclose(_np(got["outs"][f]), ref["outs"][f], rtol=1e-10, atol=1e-10, err_msg=f)
+
+
+@pytest.mark.parametrize("M", [63, 106])
+def test_higher_truncations_vs_oracle(M, torch_gpu):
+    """T63 / T106 forced modified-Euler, 4 steps, every physical field vs the oracle."""
+    from my_swampe_amd.model import run_model_scan
+
+    kw = dict(BASE, M=M, tmax=6, dt=600.0)
+    _print_case("truncation", M=M)
+    ref = orc.run_model_scan(**kw, diagnostics=False, return_history=False)
+    got = run_model_scan(**kw, diagnostics=False, return_history=False)
+    for fld in ("eta_curr", "delta_curr", "Phi_curr", "U_curr", "V_curr", "Phim_curr"):
+        r = getattr(ref["last_state"], fld)
+        np.testing.assert_allclose(_np(getattr(got["last_state"], fld)), r, rtol=0, atol=_scale_atol(r, 1e-12),
+                                   err_msg=fld)
+
+
+def test_diagnosed_winds_vs_oracle(torch_gpu):
+    """Explicit eta/delta/Phi ICs without winds: spectral wind diagnosis (model.py:644-666)."""
+    from my_swampe_amd.model import run_model_scan
+
+    s = orc.build_static(M=42, dt=600.0, a=6.37122e6, omega=7.2921159e-5, g=9.8, Phibar=3e3, taurad=86400.0,
+                         taudrag=86400.0, DPhieq=4e6, K6=1.24e33, K6Phi=None, test=2)
+    eta0, delta0, Phi0, _, _ = orc.analytic_ic(s, 2, np.pi / 4)
+    rng = np.random.default_rng(3)
+    delta0 = delta0 + 1e-7 * rng.standard_normal(delta0.shape)
+    kw = dict(M=42, dt=600.0, tmax=4, Phibar=3e3, omega=7.2921159e-5, a=6.37122e6, test=2,
+              eta0_init=eta0, delta0_init=delta0, Phi0_init=Phi0)
+    _print_case("diag-winds")
+    ref = orc.run_model_scan(**kw)
+    got = run_model_scan(**kw)
+    for f in FIELDS:
+        r = ref["outs"][f]
+        np.testing.assert_allclose(_np(got["outs"][f]), r, rtol=0, atol=_scale_atol(r, 1e-12), err_msg=f)
+
+
+def test_blowup_gate_freezes_member(torch_gpu):
+    """Huge initial winds trip rms > blowup_rms at the first step (model.py:838-864)."""
+    from my_swampe_amd.model import run_model_scan
+
+    s = orc.build_static(**{k: BASE[k] for k in ("M", "dt", "a", "omega", "g", "Phibar", "taurad", "taudrag",
+                                                  "DPhieq", "K6")}, K6Phi=None, test=None)
+    eta0, delta0, Phi0, U0, V0 = orc.analytic_ic(s, None, 0.05)
+    U0 = U0 + 2.0e4
+    kw = dict(BASE, tmax=6, eta0_init=eta0, delta0_init=delta0, Phi0_init=Phi0, U0_init=U0, V0_init=V0)
+    _print_case("blowup")
+    ref = orc.run_model_scan(**kw)
+    got = run_model_scan(**kw)
+    assert int(got["dead_first_idx"]) == ref["dead_first_idx"] == 0
+    assert bool(got["outs"]["dead"].all())
+    np.testing.assert_allclose(_np(got["outs"]["Phi"][-1]), Phi0, rtol=0, atol=0)
+    np.testing.assert_allclose(_np(got["outs"]["rms"]), ref["outs"]["rms"], rtol=1e-12)
+
+
+def test_ensemble_members_equal_direct_runs(torch_gpu):
+    """Batched members (per-member DPhieq/taurad/taudrag) == direct single runs (test_vmap_smoke.py:84-93)."""
+    from my_swampe_amd.model import param_row
+    from my_swampe_amd.plan import Plan
+
+    torch = torch_gpu
+    base = dict(dt=600.0, a=6.37122e6, omega=7.292e-5, g=9.8, Phibar=3.0e5, K6=1.24e33, K6Phi=None, alpha=0.01)
+    members = [dict(DPhieq=2e6, taurad=86400.0, taudrag=86400.0), dict(DPhieq=4e6, taurad=5e4, taudrag=2e5),
+               dict(DPhieq=6e6, taurad=1e5, taudrag=-1.0)]
+    params = np.stack([param_row(**base, **m) for m in members])
+    _print_case("ensemble", B=len(members))
+    s = orc.build_static(M=42, test=None, **{k: v for k, v in base.items() if k not in ("alpha",)},
+                         taurad=86400.0, taudrag=86400.0, DPhieq=4e6)
+    eta0, delta0, Phi0, U0, V0 = orc.analytic_ic(s, None, 0.05)
+    plan = Plan(42, params)
+    plan.init(eta0, delta0, Phi0, U0, V0)
+    plan.run(2, 5)
+    ens = _np(plan.export(fields=("Phi_curr",))["Phi_curr"])
+    plan.close()
+    assert np.min([np.max(np.abs(ens[i] - ens[j])) for i in range(3) for j in range(i + 1, 3)]) > 0.0
+    for i, m in enumerate(members):
+        ref = orc.run_model_scan(M=42, tmax=7, test=None, diagnostics=False, return_history=False,
+                                 eta0_init=eta0, delta0_init=delta0, Phi0_init=Phi0, U0_init=U0, V0_init=V0,
+                                 **{k: v for k, v in base.items()}, **m)
+        np.testing.assert_allclose(ens[i], ref["last_state"].Phi_curr, rtol=0, atol=1e-7)
+        single = Plan(42, params[i:i + 1])
+        single.init(eta0, delta0, Phi0, U0, V0)
+        single.run(2, 5)
+        np.testing.assert_allclose(ens[i], _np(single.export(fields=("Phi_curr",))["Phi_curr"][0]), rtol=0, atol=1e-10)
+        single.close()
+    torch.cuda.synchronize()
+
+
+def test_graph_replay_matches_eager(torch_gpu):
+    """The HIP-graph forward driver gives the same state as eager stepping (bitwise)."""
+    from my_swampe_amd.plan import Plan
+    from my_swampe_amd.model import param_row
+
+    torch = torch_gpu
+    p = param_row(dt=120.0, a=8.2e7, omega=3.2e-5, g=9.8, Phibar=3e5, taurad=86400.0, taudrag=86400.0,
+                  DPhieq=3e5, K6=1.24e33, K6Phi=None, alpha=0.01)
+    eta0, delta0, Phi0, U0, V0 = orc.rest_noise_ic(42, 0, 3.2e-5)
+    _print_case("graph")
+    outs = []
+    for use_graph in (False, True):
+        plan = Plan(42, np.stack([p, p]))
+        plan.init(eta0, delta0, Phi0, U0, V0)
+        s = torch.cuda.Stream()
+        with torch.cuda.stream(s):
+            if use_graph:
+                plan.run_graph(2, 40, graph_steps=8, stream=s)
+            else:
+                plan.run(2, 40, stream=s)
+            ex = plan.export(fields=("Phi_curr", "Phi_prev", "U_curr"), spec=True, stream=s)
+        s.synchronize()
+        outs.append({k: _np(v) for k, v in ex.items()})
+        plan.close()
+    for k in outs[0]:
+        np.testing.assert_array_equal(outs[0][k], outs[1][k], err_msg=k)
+
+
+def test_fp32_transforms_close_to_fp64(torch_gpu):
+    """FP32 transforms (FP64 state update): rel L2 of eta/delta/Phi vs the FP64 oracle <= 1e-6 over 60 steps."""
+    from my_swampe_amd.model import run_model_scan_final
+
+    eta0, delta0, Phi0, U0, V0 = orc.rest_noise_ic(42, 0, 3.2e-5)
+    kw = dict(M=42, dt=120.0, tmax=62, Phibar=3e5, omega=3.2e-5, a=8.2e7, taurad=86400.0, taudrag=86400.0,
+              DPhieq=3e5, eta0_init=eta0, delta0_init=delta0, Phi0_init=Phi0, U0_init=U0, V0_init=V0)
+    _print_case("fp32")
+    ref = orc.run_model_scan(**kw, diagnostics=False, return_history=False)["last_state"]
+    got = run_model_scan_final(**kw, precision="fp32")["last_state"]
+    for f in ("eta_curr", "delta_curr", "Phi_curr"):
+        r = getattr(ref, f)
+        g = _np(getattr(got, f))
+        rel = np.linalg.norm(g - r) / np.linalg.norm(r - (np.mean(r) if f == "eta_curr" else 0.0))
+        print(f, "relL2", rel)
+        assert rel < 1e-5, (f, rel)
