@@ -16,6 +16,10 @@ void gauss_legendre(int J, double* x, double* w);
 // reference are scaled the same way (zeroed or precision-reduced).
 void legendre_basis(int M, int N, int J, const double* mus, double* P, double* H, bool reference_scaling);
 
+// sqrt(r_double) / sqrt(r_exact) of the reference normalization of (m, n):
+// 1 where it is exact, < 1 or 0 where it under/overflows.
+double reference_scale_factor(int m, int n);
+
 // Longitudes -pi + 2 pi i / I.
 void lambdas(int I, double* lam);
 

@@ -87,6 +87,9 @@ struct LegArgs {
   const R* Hi;
   const double* coef;
   int ncoef;
+  // [M+1][N+1]: Pw(invrs_leg(X)) = f2 * X for the basis in use (1 for an
+  // orthonormal basis; < 1 or 0 where the reference normalization underflows).
+  const double* f2;
   double* S;        // [B][2][3][M+1][N+1] complex, level 0 = prev, 1 = curr
   const R* T;       // [B][M+1][kNT][J] complex
   R* G;             // [B][kNG][J][M+1] complex

@@ -127,10 +127,14 @@ __global__ void __launch_bounds__(128) k_leg(LegArgs<R> a) {
       eta_n[0] = acc[0]; eta_n[1] = acc[1];
       delta_n[0] = acc[2]; delta_n[1] = acc[3];
       Phi_n[0] = acc[4]; Phi_n[1] = acc[5];
+      // The stored level is divided by f2 so that the carry f2 * S of the
+      // first step equals Pw(DFT(field0)) exactly as in the reference.
+      const double f2 = a.f2[(long long)m * (N + 1) + n];
+      const double inv = f2 == 0.0 ? 0.0 : 1.0 / f2;
       for (int c = 0; c < 2; ++c) {
-        Sprev[0 * fstride + so + c] = Scur[0 * fstride + so + c] = eta_n[c];
-        Sprev[1 * fstride + so + c] = Scur[1 * fstride + so + c] = delta_n[c];
-        Sprev[2 * fstride + so + c] = Scur[2 * fstride + so + c] = Phi_n[c];
+        Sprev[0 * fstride + so + c] = Scur[0 * fstride + so + c] = eta_n[c] * inv;
+        Sprev[1 * fstride + so + c] = Scur[1 * fstride + so + c] = delta_n[c] * inv;
+        Sprev[2 * fstride + so + c] = Scur[2 * fstride + so + c] = Phi_n[c] * inv;
       }
     } else {
       R acc[2 * kNT];
@@ -152,12 +156,14 @@ __global__ void __launch_bounds__(128) k_leg(LegArgs<R> a) {
       }
       const double dt = cf[C_DT], aa = cf[C_A], Phibar = cf[C_PHIBAR];
       const double nn = (double)n * (double)(n + 1);
+      // Carry terms Pw(Xm1) of the reference = f2 * S (Gauss quadrature).
+      const double f2 = a.f2[(long long)m * (N + 1) + n];
       for (int c = 0; c < 2; ++c) {
         const double fp0 = acc[0 + c], fp1 = acc[2 + c], fp2 = acc[4 + c], fp3 = acc[6 + c], fp4 = acc[8 + c];
         const double fh0 = acc[10 + c], fh1 = acc[12 + c], fh2 = acc[14 + c], fh3 = acc[16 + c];
-        const double eo = Scur[0 * fstride + so + c];
-        const double d_o = Scur[1 * fstride + so + c];
-        const double po = Scur[2 * fstride + so + c];
+        const double eo = f2 * Scur[0 * fstride + so + c];
+        const double d_o = f2 * Scur[1 * fstride + so + c];
+        const double po = f2 * Scur[2 * fstride + so + c];
         if (!a.expflag) {
           // modEuler_tdiff.py:95-103, :185-193, :261 with the carry terms in spectral space.
           const double t2 = (2.0 * dt / (aa * aa)) / 4.0;
@@ -166,9 +172,9 @@ __global__ void __launch_bounds__(128) k_leg(LegArgs<R> a) {
           eta_n[c] = eo + fp4 + fh3;
         } else {
           // explicit_tdiff.py:71, :138, :212 (prev-level carries).
-          const double ep = Sprev[0 * fstride + so + c];
-          const double dp = Sprev[1 * fstride + so + c];
-          const double pp = Sprev[2 * fstride + so + c];
+          const double ep = f2 * Sprev[0 * fstride + so + c];
+          const double dp = f2 * Sprev[1 * fstride + so + c];
+          const double pp = f2 * Sprev[2 * fstride + so + c];
           Phi_n[c] = pp - 2.0 * dt * Phibar * d_o + fp0 + fh0;
           delta_n[c] = dp + fp2 + fh1;
           eta_n[c] = ep + fp4 + fh3;

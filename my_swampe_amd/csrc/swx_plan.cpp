@@ -75,6 +75,7 @@ struct swx_plan {
   void* Hi = nullptr;
   void* tw = nullptr;
   double* geo_d = nullptr;
+  double* f2 = nullptr;
   double* coef = nullptr;
   double* S = nullptr;
   void* T = nullptr;
@@ -181,6 +182,7 @@ LegArgs<R> leg_args(swx_plan* p, int mode) {
   a.Hi = (const R*)p->Hi;
   a.coef = p->coef;
   a.ncoef = p->cl.ncoef;
+  a.f2 = p->f2;
   a.S = p->S;
   a.T = (const R*)p->T;
   a.G = (R*)p->G;
@@ -378,6 +380,13 @@ int swx_plan_create(const swx_plan_desc* desc, const double* params, swx_plan** 
   }
   std::vector<double> coef;
   fill_coef(p, coef);
+  std::vector<double> f2((size_t)Mp * (N + 1), 1.0);
+  if (!desc->exact_basis)
+    for (int m = 0; m <= M; ++m)
+      for (int n = m; n <= N; ++n) {
+        const double f = reference_scale_factor(m, n);
+        f2[(size_t)m * (N + 1) + n] = f * f;
+      }
 
   int rc = SWX_OK;
   const size_t rs = p->rsz;
@@ -385,6 +394,7 @@ int swx_plan_create(const swx_plan_desc* desc, const double* params, swx_plan** 
   if ((rc = dalloc(&p->Pf, tri * rs)) || (rc = dalloc(&p->Hf, tri * rs)) || (rc = dalloc(&p->Pi, tri * rs)) ||
       (rc = dalloc(&p->Hi, tri * rs)) || (rc = dalloc(&p->tw, 2 * I * rs)) ||
       (rc = dalloc(&p->geo_d, geo.size() * 8)) || (rc = dalloc(&p->coef, coef.size() * 8)) ||
+      (rc = dalloc(&p->f2, f2.size() * 8)) ||
       (rc = dalloc(&p->S, (size_t)B * 2 * 3 * Mp * (N + 1) * 2 * 8)) ||
       (rc = dalloc(&p->T, (size_t)B * Mp * kNT * J * 2 * rs)) || (rc = dalloc(&p->G, (size_t)B * kNG * cplxJM * rs)) ||
       (rc = dalloc(&p->Gprev, (size_t)B * kNG * cplxJM * rs)) || (rc = dalloc(&p->raw, (size_t)B * kNRaw * cplxJM * rs)) ||
@@ -396,11 +406,13 @@ int swx_plan_create(const swx_plan_desc* desc, const double* params, swx_plan** 
   }
   if ((rc = upload_real(p->Pf, Pf, rs)) || (rc = upload_real(p->Hf, Hf, rs)) || (rc = upload_real(p->Pi, Pi, rs)) ||
       (rc = upload_real(p->Hi, Hi, rs)) || (rc = upload_real(p->tw, tw, rs)) ||
-      (rc = upload_real(p->geo_d, geo, 8)) || (rc = upload_real(p->coef, coef, 8))) {
+      (rc = upload_real(p->geo_d, geo, 8)) || (rc = upload_real(p->coef, coef, 8)) ||
+      (rc = upload_real(p->f2, f2, 8))) {
     swx_plan_destroy(p);
     return rc;
   }
-  if (hipMemset(p->dead, 0, (size_t)B) != hipSuccess || hipMemset(p->part_uv, 0, (size_t)B * J * 2 * 8) != hipSuccess) {
+  if (hipMemset(p->dead, 0, (size_t)B) != hipSuccess || hipMemset(p->part_uv, 0, (size_t)B * J * 2 * 8) != hipSuccess ||
+      hipMemset(p->S, 0, (size_t)B * 2 * 3 * Mp * (N + 1) * 2 * 8) != hipSuccess) {
     swx_plan_destroy(p);
     return fail(SWX_ERR_HIP, "hipMemset failed");
   }
@@ -424,7 +436,7 @@ int swx_plan_set_params(swx_plan* p, const double* params, void* stream) {
 void swx_plan_destroy(swx_plan* p) {
   if (!p) return;
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
-  void* bufs[] = {p->Pf, p->Hf, p->Pi, p->Hi, p->tw, p->geo_d, p->coef, p->S, p->T, p->G, p->Gprev,
+  void* bufs[] = {p->Pf, p->Hf, p->Pi, p->Hi, p->tw, p->geo_d, p->f2, p->coef, p->S, p->T, p->G, p->Gprev,
                   p->raw, p->lag, p->g0uv, p->phys, p->uvic, p->part_uv, p->dead};
   for (void* q : bufs)
     if (q) (void)hipFree(q);
