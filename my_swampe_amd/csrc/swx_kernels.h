@@ -132,6 +132,13 @@ template <typename R>
 hipError_t launch_leg(const LegArgs<R>& a, hipStream_t s);
 template <typename R>
 hipError_t launch_grid(const GridArgs<R>& a, hipStream_t s);
+// MFMA grid kernel (FP32, GRID_STEP without history/diagnostics); tables:
+// TinvT [I][k1p], TfwdT [k2p][I].
+hipError_t launch_grid_mfma(const GridArgs<float>& a, const float* TinvT, const float* TfwdT, hipStream_t s);
+size_t grid_mfma_lds_bytes(int M);
+int grid_mfma_k1p(int M);
+int grid_mfma_k2p(int M);
+
 template <typename R>
 hipError_t launch_copy_real(double* dst, long long dstride, const R* src, long long sstride, long long n, int B,
                             hipStream_t s);

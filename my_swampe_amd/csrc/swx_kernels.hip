@@ -23,6 +23,8 @@
 
 #include <cfloat>
 
+#include "swx_physics.h"
+
 namespace swx {
 
 namespace {
@@ -327,10 +329,7 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs<R> a) {
   }
   __syncthreads();
 
-  const double mu_s1m = geo[gl.s1m + j];
   const double cosphi = geo[gl.cosphi + j];
-  const double denomE = geo[gl.denomE + j];
-  const double Phibar = cf[C_PHIBAR];
   const double alpha = cf[C_ALPHA];
 
   int nfwd = 0;  // number of forward-DFT fields staged in sX
@@ -431,35 +430,10 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs<R> a) {
         for (int f = 0; f < 5; ++f)
           a.hist_fields[(((long long)b * 5 + f) * a.hist_T + a.hist_k) * plane + row + i] = outv[f];
       }
-      // Forcing (forcing.py:65-155 via model.py:560-603): computed whenever test is None.
-      double PhiF = 0.0, F = 0.0, G = 0.0;
-      if (a.test < 0) {
-        const double Phieq = geo[gl.day + i] != 0.0 ? Phibar + (cf[C_DPHIEQ] * geo[gl.cosl + i]) * mu_s1m : Phibar;
-        const double Q = cf[C_INVTAURAD] * (Phieq - (Phi + Phibar));
-        const double Qc = Q < 0.0 ? 0.0 : Q;
-        const double tot = Phi + Phibar;
-        const double safe = fabs(tot) > 0.0 ? tot : DBL_MIN;
-        const double Ru = -U * Qc / safe;
-        const double Rv = -V * Qc / safe;
-        const double td = cf[C_TAUDRAG];
-        PhiF = Q;
-        if (td == -1.0) {
-          F = Ru;
-          G = Rv;
-        } else {
-          F = Ru - (U / td);
-          G = Rv - (V / td);
-        }
-      }
-      // Nonlinear products (initial_conditions.py:283-292).
-      sX[0 * I + i] = (R)(U * eta);
-      sX[1 * I + i] = (R)(V * eta);
-      sX[2 * I + i] = (R)(U * Phi);
-      sX[3 * I + i] = (R)(V * Phi);
-      sX[4 * I + i] = (R)((U * U + V * V) / denomE);
-      sX[5 * I + i] = (R)PhiF;
-      sX[6 * I + i] = (R)F;
-      sX[7 * I + i] = (R)G;
+      // Forcing and nonlinear products (shared with the MFMA kernel).
+      double prod[8];
+      grid_products(cf, geo, gl, j, i, a.test, eta, Phi, U, V, prod);
+      for (int f = 0; f < 8; ++f) sX[f * I + i] = (R)prod[f];
       if (nfwd == 10) {
         sX[8 * I + i] = (R)U;
         sX[9 * I + i] = (R)V;
@@ -552,103 +526,30 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs<R> a) {
     }
   }
 
-  // Assemble the w-weighted Legendre inputs (all slots complex, FP64 math).
+  // Assemble the w-weighted Legendre inputs (FP64 math).
   const double wj = geo[gl.w + j];
   const double tc = cf[cl.n_tc + j];
-  const double dt = cf[C_DT], aa = cf[C_A];
   for (int m = tid; m < Mp; m += nt) {
-    auto ld = [&](int f, double* z) {
-      z[0] = (double)sF[(f * Mp + m) * 2];
-      z[1] = (double)sF[(f * Mp + m) * 2 + 1];
-    };
-    double Am[2], Bm[2], Cm[2], Dm[2], Em[2], PFm[2], Fm[2], Gm[2];
-    ld(0, Am); ld(1, Bm); ld(2, Cm); ld(3, Dm); ld(4, Em); ld(5, PFm); ld(6, Fm); ld(7, Gm);
-    const double mm = (double)m;
-    double out[kNT][2];
-    for (int k = 0; k < kNT; ++k) out[k][0] = out[k][1] = 0.0;
-    // i m z = (-m z.im, m z.re)
-    if (!a.expflag) {
-      // modEuler_tdiff.py:72-110, :162-200, :253-259 regrouped by basis and n-weight.
-      const double t1 = tc / 4.0;
-      const double t2 = (2.0 * dt / (aa * aa)) / 4.0;
-      const double te = a.forcflag ? tc : tc / 2.0;
-      const double ia2 = 1.0 / (aa * aa);
-      double Bs[2], As[2], Bf[2], Af[2];
-      for (int c = 0; c < 2; ++c) {
-        Bf[c] = Bm[c] + Fm[c];
-        Af[c] = Am[c] - Gm[c];
-        Bs[c] = a.forcflag ? Bf[c] : Bm[c];
-        As[c] = a.forcflag ? Af[c] : Am[c];
-      }
-      const double fPF = a.forcflag ? 1.0 : 0.0;
-      // P slot 0 (Phi): -t1 i m C - 0.5 Phibar t1 i m B* + dt PhiF
-      // P slot 1 (Phi, x n(n+1)): -0.5 Phibar t2 E
-      // P slot 2 (delta): t1 i m Bf
-      // P slot 3 (delta, x n(n+1)): t2 E + t1 i m C / (2 a^2) + (dt/2) PhiF / a^2
-      // P slot 4 (eta): -te i m A*
-      // H slot 5 (Phi): t1 D - 0.5 Phibar t1 A*
-      // H slot 6 (delta): t1 Af
-      // H slot 7 (delta, x n(n+1)): t1 D / (2 a^2)
-      // H slot 8 (eta): te B*
-      const double imC[2] = {-mm * Cm[1], mm * Cm[0]};
-      const double imBs[2] = {-mm * Bs[1], mm * Bs[0]};
-      const double imBf[2] = {-mm * Bf[1], mm * Bf[0]};
-      const double imAs[2] = {-mm * As[1], mm * As[0]};
-      for (int c = 0; c < 2; ++c) {
-        out[0][c] = -(t1 * imC[c]) - 0.5 * Phibar * (t1 * imBs[c]) + fPF * (dt * PFm[c]);
-        out[1][c] = -0.5 * Phibar * (t2 * Em[c]);
-        out[2][c] = t1 * imBf[c];
-        out[3][c] = t2 * Em[c] + (t1 * imC[c]) / (2.0 * aa * aa) + fPF * ((dt / 2.0) * PFm[c]) * ia2;
-        out[4][c] = -(te * imAs[c]);
-        out[5][c] = t1 * Dm[c] - 0.5 * Phibar * (t1 * As[c]);
-        out[6][c] = t1 * Af[c];
-        out[7][c] = (t1 * Dm[c]) / (2.0 * aa * aa);
-        out[8][c] = te * Bs[c];
-      }
-    } else {
-      // explicit_tdiff.py:63-75, :134-153, :207-224 regrouped by basis.
-      double Um[2], Vm[2];
-      if (a.test == 1) {
-        Um[0] = (double)sF[(8 * Mp + m) * 2];
-        Um[1] = (double)sF[(8 * Mp + m) * 2 + 1];
-        Vm[0] = (double)sF[(9 * Mp + m) * 2];
-        Vm[1] = (double)sF[(9 * Mp + m) * 2 + 1];
-      } else if (a.mode == GRID_INIT) {
-        Um[0] = (double)sF[(8 * Mp + m) * 2];
-        Um[1] = (double)sF[(8 * Mp + m) * 2 + 1];
-        Vm[0] = (double)sF[(9 * Mp + m) * 2];
-        Vm[1] = (double)sF[(9 * Mp + m) * 2 + 1];
-      } else {
-        Um[0] = (double)sG[(3 * Mp + m) * 2];
-        Um[1] = (double)sG[(3 * Mp + m) * 2 + 1];
-        Vm[0] = (double)sG[(4 * Mp + m) * 2];
-        Vm[1] = (double)sG[(4 * Mp + m) * 2 + 1];
-      }
-      const double t1 = tc;
-      const double tau = cf[C_TAUDRAG];
-      const double imC[2] = {-mm * Cm[1], mm * Cm[0]};
-      const double imA[2] = {-mm * Am[1], mm * Am[0]};
-      const double imU[2] = {-mm * Um[1], mm * Um[0]};
-      const double imV[2] = {-mm * Vm[1], mm * Vm[0]};
-      const double imF[2] = {-mm * Fm[1], mm * Fm[0]};
-      const double imG[2] = {-mm * Gm[1], mm * Gm[0]};
-      for (int c = 0; c < 2; ++c) {
-        out[0][c] = -(t1 * imC[c]) + (a.forcflag ? (2.0 * dt) * PFm[c] : 0.0);
-        out[5][c] = t1 * Dm[c];
-        out[4][c] = -(t1 * imA[c]);
-        out[8][c] = t1 * Bm[c];
-        if (a.forcflag) {
-          out[2][c] = -((t1 * imU[c]) / tau) + t1 * imF[c];
-          out[6][c] = (t1 * Vm[c]) / tau - t1 * Gm[c];
-          out[4][c] += -((t1 * imV[c]) / tau) + t1 * imG[c];
-          out[8][c] += (t1 * Um[c]) / tau + t1 * Fm[c];
-        }
-      }
+    double fm[8][2];
+    for (int f = 0; f < 8; ++f) {
+      fm[f][0] = (double)sF[(f * Mp + m) * 2];
+      fm[f][1] = (double)sF[(f * Mp + m) * 2 + 1];
     }
+    double Um[2] = {0.0, 0.0}, Vm[2] = {0.0, 0.0};
+    if (a.expflag) {
+      // Winds: pinned (test 1) or initial DFT (slots 8, 9), else the invrsUV output.
+      const bool from_dft = (a.test == 1) || (a.mode == GRID_INIT);
+      const R* su = from_dft ? sF + (8 * Mp + m) * 2 : sG + (3 * Mp + m) * 2;
+      const R* sv = from_dft ? sF + (9 * Mp + m) * 2 : sG + (4 * Mp + m) * 2;
+      Um[0] = su[0]; Um[1] = su[1];
+      Vm[0] = sv[0]; Vm[1] = sv[1];
+    }
+    double out[kNT][2];
+    assemble_slots(cf, tc, wj, m, a.expflag, a.forcflag, fm, Um, Vm, out);
     R* Tm = a.T + (((long long)b * Mp + m) * kNT) * J * 2;
     for (int k = 0; k < kNT; ++k) {
-      Tm[(k * J + j) * 2] = (R)(out[k][0] * wj);
-      Tm[(k * J + j) * 2 + 1] = (R)(out[k][1] * wj);
+      Tm[(k * J + j) * 2] = (R)out[k][0];
+      Tm[(k * J + j) * 2 + 1] = (R)out[k][1];
     }
   }
 }

@@ -9,6 +9,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -74,6 +75,9 @@ struct swx_plan {
   void* Pi = nullptr;
   void* Hi = nullptr;
   void* tw = nullptr;
+  float* TinvT = nullptr;  // MFMA grid kernel tables (FP32 plans)
+  float* TfwdT = nullptr;
+  bool use_mfma = false;
   double* geo_d = nullptr;
   double* f2 = nullptr;
   double* coef = nullptr;
@@ -222,6 +226,18 @@ GridArgs<R> grid_args(swx_plan* p, int mode) {
   return a;
 }
 
+// The MFMA grid kernel covers the forward stepping of FP32 plans without
+// per-step history or diagnostics; everything else uses the plain kernel.
+template <typename R>
+hipError_t launch_grid_any(swx_plan* p, const GridArgs<R>& g, hipStream_t s) {
+  if constexpr (sizeof(R) == 4) {
+    const bool mfma_ok = p->use_mfma && g.mode == GRID_STEP && !g.diagnostics && !g.hist_fields && !g.hist_phiext &&
+                         !(g.test == 1 && g.expflag);
+    if (mfma_ok) return launch_grid_mfma(g, p->TinvT, p->TfwdT, s);
+  }
+  return launch_grid<R>(g, s);
+}
+
 template <typename R>
 int do_init(swx_plan* p, const double* eta0, const double* delta0, const double* Phi0, const double* U0,
             const double* V0, hipStream_t s) {
@@ -279,7 +295,7 @@ int do_steps(swx_plan* p, int t0, int nsteps, double* hist_fields, double* hist_
     g.hist_phiext = hist_phiext;
     g.hist_T = hist_T;
     g.hist_k = hist_k0 + k;
-    SWX_HIP(launch_grid<R>(g, s));
+    SWX_HIP(launch_grid_any<R>(p, g, s));
   }
   return SWX_OK;
 }
@@ -404,6 +420,31 @@ int swx_plan_create(const swx_plan_desc* desc, const double* params, swx_plan** 
     swx_plan_destroy(p);
     return rc;
   }
+  if (desc->precision == SWX_PREC_FP32) {
+    int maxlds = 0;
+    (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device);
+    const char* off = std::getenv("SWX_DISABLE_MFMA");
+    p->use_mfma = grid_mfma_lds_bytes(M) <= (size_t)maxlds && !(off && off[0] == '1');
+    if (p->use_mfma) {
+      const int k1p = grid_mfma_k1p(M), k2p = grid_mfma_k2p(M);
+      std::vector<double> ti((size_t)I * k1p, 0.0), tf((size_t)k2p * I, 0.0);
+      for (int i = 0; i < I; ++i)
+        for (int m = 0; m < Mp; ++m) {
+          const int kk = (int)(((long long)m * i) % I);
+          const double c = std::cos(2.0 * pi * kk / I), sn = std::sin(2.0 * pi * kk / I);
+          const double wm = m == 0 ? 1.0 : 2.0;
+          ti[(size_t)i * k1p + 2 * m] = wm * c;
+          ti[(size_t)i * k1p + 2 * m + 1] = -wm * sn;
+          tf[(size_t)(2 * m) * I + i] = c / I;
+          tf[(size_t)(2 * m + 1) * I + i] = -sn / I;
+        }
+      if ((rc = dalloc(&p->TinvT, ti.size() * 4)) || (rc = dalloc(&p->TfwdT, tf.size() * 4)) ||
+          (rc = upload_real(p->TinvT, ti, 4)) || (rc = upload_real(p->TfwdT, tf, 4))) {
+        swx_plan_destroy(p);
+        return rc;
+      }
+    }
+  }
   if ((rc = upload_real(p->Pf, Pf, rs)) || (rc = upload_real(p->Hf, Hf, rs)) || (rc = upload_real(p->Pi, Pi, rs)) ||
       (rc = upload_real(p->Hi, Hi, rs)) || (rc = upload_real(p->tw, tw, rs)) ||
       (rc = upload_real(p->geo_d, geo, 8)) || (rc = upload_real(p->coef, coef, 8)) ||
@@ -436,7 +477,7 @@ int swx_plan_set_params(swx_plan* p, const double* params, void* stream) {
 void swx_plan_destroy(swx_plan* p) {
   if (!p) return;
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
-  void* bufs[] = {p->Pf, p->Hf, p->Pi, p->Hi, p->tw, p->geo_d, p->f2, p->coef, p->S, p->T, p->G, p->Gprev,
+  void* bufs[] = {p->Pf, p->Hf, p->Pi, p->Hi, p->tw, p->TinvT, p->TfwdT, p->geo_d, p->f2, p->coef, p->S, p->T, p->G, p->Gprev,
                   p->raw, p->lag, p->g0uv, p->phys, p->uvic, p->part_uv, p->dead};
   for (void* q : bufs)
     if (q) (void)hipFree(q);
@@ -546,7 +587,7 @@ int swx_time_kernels(swx_plan* p, int t0, int nsteps, float* ms_leg, float* ms_g
     } else {
       GridArgs<float> g = grid_args<float>(p, GRID_STEP);
       g.do_ra = (p->d.modalflag && t > 2) ? 1 : 0;
-      SWX_HIP(launch_grid<float>(g, s));
+      SWX_HIP(launch_grid_any<float>(p, g, s));
     }
     SWX_HIP(hipEventRecord(ev[2], s));
     SWX_HIP(hipEventSynchronize(ev[2]));

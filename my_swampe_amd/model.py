@@ -276,6 +276,24 @@ def _validate(*, tmax, dt, test, semi_implicit, expflag, raw_filter, contflag, c
     return test, start
 
 
+def _check_ic_args(shape, eta0_init, delta0_init, Phi0_init, U0_init, V0_init):
+    """Presence and shape checks of explicit initial fields, on the host (model.py:1616-1640)."""
+    have = (eta0_init is not None) or (delta0_init is not None) or (Phi0_init is not None)
+    if not have:
+        return
+    if eta0_init is None or delta0_init is None or Phi0_init is None:
+        raise ValueError("If providing explicit ICs, eta0_init, delta0_init, and Phi0_init must all be provided.")
+    for name, arr in (("eta0_init", eta0_init), ("delta0_init", delta0_init), ("Phi0_init", Phi0_init)):
+        if tuple(np.shape(arr)) != tuple(shape):
+            raise ValueError(f"{name} must have shape {tuple(shape)}, got {tuple(np.shape(arr))}.")
+    if (U0_init is None) != (V0_init is None):
+        raise ValueError("Provide both U0_init and V0_init, or neither.")
+    if U0_init is not None:
+        for name, arr in (("U0_init", U0_init), ("V0_init", V0_init)):
+            if tuple(np.shape(arr)) != tuple(shape):
+                raise ValueError(f"{name} must have shape {tuple(shape)}, got {tuple(np.shape(arr))}.")
+
+
 def _initial_fields(static: Static, test, a1, eta0_init, delta0_init, Phi0_init, U0_init, V0_init):
     """Explicit or analytic initial fields; U0/V0 None means 'diagnose spectrally' (model.py:1616-1682)."""
     torch = require_gpu()
@@ -334,7 +352,8 @@ def run_model_scan(*, M: int, dt: Scalar, tmax: int, Phibar: Scalar, omega: Scal
     """
     test, start = _validate(tmax=tmax, dt=dt, test=test, semi_implicit=semi_implicit, expflag=expflag,
                             raw_filter=raw_filter, contflag=contflag, contTime=contTime, starttime=starttime)
-    _capi.grid_size(int(M))  # ValueError for an unsupported truncation (initial_conditions.py:80-81)
+    J, I = _capi.grid_size(int(M))  # ValueError for an unsupported truncation (initial_conditions.py:80-81)
+    _check_ic_args((J, I), eta0_init, delta0_init, Phi0_init, U0_init, V0_init)
     if precision not in _capi.PRECISIONS:
         raise ValueError(f"precision must be one of {tuple(_capi.PRECISIONS)}, got {precision!r}")
     torch = require_gpu()

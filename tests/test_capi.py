@@ -1,0 +1,122 @@
+"""C-ABI library on the CPU: load, exported symbols, host geometry, API validation (no GPU compute)."""
+
+from __future__ import annotations
+
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from oracle import swampe_oracle as orc
+
+
+def _declared_symbols():
+    text = (ROOT / "include" / "swampe_capi.h").read_text()
+    return sorted(set(re.findall(r"\b(swx_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from my_swampe_amd import _capi
+
+    lib = _capi.load()
+    declared = _declared_symbols()
+    assert len(declared) >= 10
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert set(declared) == set(_capi.EXPORTED_SYMBOLS)
+    assert lib.swx_version().startswith(b"swampe-hip")
+
+
+def test_grid_size_and_unsupported_truncation():
+    from my_swampe_amd import _capi
+
+    assert _capi.grid_size(42) == (64, 128)
+    assert _capi.grid_size(63) == (96, 192)
+    assert _capi.grid_size(106) == (160, 320)
+    with pytest.raises(ValueError, match="M=50"):
+        _capi.grid_size(50)
+
+
+@pytest.mark.parametrize("M", [42, 63, 106])
+def test_host_geometry_matches_oracle(M):
+    """C++ Gauss-Legendre + stable-recurrence basis with the reference normalization vs the scipy-based oracle."""
+    from my_swampe_amd import _capi
+
+    mus, w, lam, P, H = _capi.geometry(M)
+    N, I, J, lo, mo, wo, Po, Ho = orc.cached_geometry(M)
+    np.testing.assert_allclose(mus, mo, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(w, wo, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(lam, lo, rtol=0, atol=0)
+    np.testing.assert_allclose(P, Po, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(H, Ho, rtol=0, atol=1e-11)
+
+
+def test_exact_basis_is_orthonormal_at_T106():
+    from my_swampe_amd import _capi
+
+    mus, w, lam, P, H = _capi.geometry(106, exact_basis=True)
+    for m in (0, 53, 90, 106):
+        G = np.einsum("j,jn,jk->nk", w, P[:, m, m:], P[:, m, m:])
+        np.testing.assert_allclose(G, np.eye(G.shape[0]), atol=1e-13)
+    Pr = _capi.geometry(106)[3]
+    assert np.all(Pr[:, 106, 106] == 0.0) and not np.any(P[:, 106, 106] == 0.0)
+
+
+BASE = dict(M=42, dt=600.0, Phibar=3.0e5, omega=7.292e-5, a=6.37122e6, test=None, g=9.8, forcflag=True,
+            taurad=86400.0, taudrag=86400.0, DPhieq=4.0e6, diffflag=True, modalflag=True, alpha=0.01,
+            expflag=False, K6=1.24e33, diagnostics=False, jit_scan=False)
+
+
+def _run(**kw):
+    from my_swampe_amd.model import run_model_scan
+
+    return run_model_scan(**{**BASE, **kw})
+
+
+def test_validation_errors_raise_before_device_work():
+    """unit_tests/test_invalid_input.py: same exception types and messages, raised on the host."""
+    with pytest.raises(ValueError, match="tmax"):
+        _run(tmax=1)
+    with pytest.raises(ValueError, match="dt"):
+        _run(tmax=4, dt=0.0)
+    with pytest.raises(ValueError, match="starttime"):
+        _run(tmax=4, starttime=10)
+    with pytest.raises(ValueError, match="contTime"):
+        _run(tmax=4, contflag=True)
+    with pytest.raises(ValueError, match="numeric"):
+        _run(tmax=4, contflag=True, contTime="not-a-number")
+    with pytest.raises(ValueError, match="semi_implicit"):
+        _run(tmax=4, semi_implicit=True, expflag=True)
+    with pytest.raises(ValueError, match="raw_filter"):
+        _run(tmax=4, raw_filter=True, expflag=True)
+    with pytest.raises(ValueError, match="M=50"):
+        _run(tmax=4, M=50)
+    with pytest.raises(ValueError, match="precision"):
+        _run(tmax=4, precision="fp16")
+    z = np.zeros((64, 128))
+    with pytest.raises(ValueError, match=r"eta0_init.*delta0_init.*Phi0_init"):
+        _run(tmax=4, eta0_init=z)
+    with pytest.raises(ValueError, match=r"U0_init.*V0_init"):
+        _run(tmax=4, eta0_init=z, delta0_init=z, Phi0_init=z, U0_init=z)
+    with pytest.raises(ValueError, match="Phi0_init"):
+        _run(tmax=4, eta0_init=z, delta0_init=z, Phi0_init=np.zeros((65, 128)))
+
+
+def test_out_of_scope_options_raise_not_implemented():
+    with pytest.raises(NotImplementedError):
+        _run(tmax=4, semi_implicit=True)
+    with pytest.raises(NotImplementedError):
+        _run(tmax=4, raw_filter=True)
+    with pytest.raises(NotImplementedError):
+        _run(tmax=4, test=99)
+
+
+def test_product_path_fails_loudly_without_gpu():
+    """No CPU fallback: on a host without a ROCm GPU the run raises instead of computing."""
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        _run(tmax=4)

@@ -261,3 +261,33 @@ def test_fp32_transforms_close_to_fp64(torch_gpu):
         rel[f] = float(np.linalg.norm(g - r) / np.linalg.norm(r))
     print("relL2", rel)
     assert rel["Phi_curr"] <= 1e-5 and rel["eta_curr"] <= 1e-5 and rel["delta_curr"] <= 1e-3, rel
+
+
+@pytest.mark.parametrize("M", [42, 63, 106])
+def test_mfma_grid_kernel_matches_plain_fp32(M, torch_gpu, monkeypatch):
+    """The MFMA grid kernel (FP32) vs the plain FP32 grid kernel and the FP64 oracle, 12 steps."""
+    from my_swampe_amd.model import param_row
+    from my_swampe_amd.plan import Plan
+
+    p = param_row(dt=120.0 if M == 42 else 60.0, a=8.2e7, omega=3.2e-5, g=9.8, Phibar=3e5, taurad=86400.0,
+                  taudrag=86400.0, DPhieq=3e5, K6=1.24e33, K6Phi=None, alpha=0.01)
+    ics = orc.rest_noise_ic(M, 0, 3.2e-5)
+    _print_case("mfma-vs-plain", M=M)
+    res = {}
+    for kind in ("plain", "mfma"):
+        monkeypatch.setenv("SWX_DISABLE_MFMA", "1" if kind == "plain" else "0")
+        plan = Plan(M, p, precision="fp32")
+        plan.init(*ics)
+        plan.run(2, 12)
+        res[kind] = {k: _np(v[0]) for k, v in plan.export(fields=("Phi_curr", "U_curr", "eta_curr")).items()
+                     if k != "dead"}
+        plan.close()
+    ref = orc.run_model_scan(M=M, dt=float(p[0]), tmax=14, Phibar=3e5, omega=3.2e-5, a=8.2e7, taurad=86400.0,
+                             taudrag=86400.0, DPhieq=3e5, eta0_init=ics[0], delta0_init=ics[1], Phi0_init=ics[2],
+                             U0_init=ics[3], V0_init=ics[4], diagnostics=False, return_history=False)["last_state"]
+    for f in ("Phi_curr", "U_curr", "eta_curr"):
+        r = getattr(ref, f)
+        for kind in ("plain", "mfma"):
+            rel = np.linalg.norm(res[kind][f] - r) / np.linalg.norm(r)
+            print(M, kind, f, "relL2 vs oracle", rel)
+            assert rel < (2e-6 if f != "U_curr" else 1e-3), (M, kind, f, rel)
