@@ -66,13 +66,16 @@ def flops_per_member_step(M: int, J: int, I: int):
     return leg, dft
 
 
-def run_batch(M, B, precision, steps, warmup, graph_steps=32):
+def run_batch(M, B, precision, steps, warmup, graph_steps=32, full_phys=False):
     import torch
+    from my_swampe_amd import _capi
     from my_swampe_amd.plan import Plan
 
     params = member_params(B)
     ics = member_ics(M, B)
     plan = Plan(M, params, precision=precision, diagnostics=False)
+    if full_phys:
+        plan.set_option(_capi.OPT_FULL_PHYS_CARRY, 1)
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
         plan.init(*ics, stream=stream)
@@ -158,11 +161,14 @@ def main():
         "finite": main_run["finite"],
     }
     if not args.no_extra:
+        full = run_batch(M, B, args.precision, max(args.steps // 2, 100), 10, full_phys=True)
         fp64 = run_batch(M, B, "fp64", max(args.steps // 4, 50), 10)
         plain = run_batch(M, 1, "fp64", max(args.steps // 2, 200), 20)
         fp64_v = max(args.steps // 4, 50) * B / fp64["wall"]
         plain_v = max(args.steps // 2, 200) / plain["wall"]
         out["extra"] = {
+            "full_phys_carry_member_steps_per_s": max(args.steps // 2, 100) * B / full["wall"],
+            "full_phys_carry_kernel_ms": {"leg": full["ms_leg"], "grid": full["ms_grid"]},
             "fp64_batched_member_steps_per_s": fp64_v,
             "plain_fp64_1member_member_steps_per_s": plain_v,
             "ratio_vs_plain_fp64_1member": value / plain_v,

@@ -155,6 +155,16 @@ int swx_run_graph(swx_plan *plan, int t0, int nsteps, int graph_steps, void *str
  * scripts/benchmark_scan.py:116-141). */
 int swx_time_kernels(swx_plan *plan, int t0, int nsteps, float *ms_leg, float *ms_grid, void *stream);
 
+/* Plan options (swx_plan_set_option).
+ *   SWX_OPT_GRID_KERNEL: 0 = automatic (MFMA grid kernel for FP32 forward
+ *     stepping when it fits), 1 = always the plain grid kernel.
+ *   SWX_OPT_FULL_PHYS_CARRY: 1 = write the physical (grid) carry at every
+ *     step; 0 (default) = only in the final steps whose Robert-Asselin
+ *     contribution can reach the exported state (identical results). */
+#define SWX_OPT_GRID_KERNEL 1
+#define SWX_OPT_FULL_PHYS_CARRY 2
+int swx_plan_set_option(swx_plan *plan, int option, int value);
+
 /* Last error message of the calling thread. */
 const char *swx_last_error(void);
 

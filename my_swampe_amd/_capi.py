@@ -19,6 +19,9 @@ SWX_ERR_STATE = -3
 PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_BF16X3 = 2
+
+OPT_GRID_KERNEL = 1
+OPT_FULL_PHYS_CARRY = 2
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
 
 PARAM_NAMES = ("dt", "a", "omega", "g", "Phibar", "taurad", "taudrag", "DPhieq", "K6", "K6Phi", "alpha", "blowup_rms")
@@ -27,8 +30,8 @@ NPARAM = len(PARAM_NAMES)
 # Every extern "C" symbol declared in include/swampe_capi.h.
 EXPORTED_SYMBOLS = (
     "swx_geometry", "swx_grid_size", "swx_plan_create", "swx_plan_set_params", "swx_plan_destroy",
-    "swx_init", "swx_run", "swx_export_state", "swx_run_graph", "swx_time_kernels", "swx_last_error",
-    "swx_version",
+    "swx_init", "swx_run", "swx_export_state", "swx_run_graph", "swx_time_kernels", "swx_plan_set_option",
+    "swx_last_error", "swx_version",
 )
 
 STATE_FIELDS = (
@@ -81,11 +84,12 @@ def load() -> ctypes.CDLL:
     lib.swx_run.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]
     lib.swx_export_state.argtypes = [c_void_p, POINTER(StateOut), c_void_p]
     lib.swx_run_graph.argtypes = [c_void_p, c_int, c_int, c_int, c_void_p]
+    lib.swx_plan_set_option.argtypes = [c_void_p, c_int, c_int]
     lib.swx_time_kernels.argtypes = [c_void_p, c_int, c_int, POINTER(ctypes.c_float), POINTER(ctypes.c_float), c_void_p]
     for name in EXPORTED_SYMBOLS:
         getattr(lib, name)  # AttributeError when a declared symbol is missing
     for name in ("swx_geometry", "swx_grid_size", "swx_plan_create", "swx_plan_set_params", "swx_init",
-                 "swx_run", "swx_export_state", "swx_run_graph", "swx_time_kernels"):
+                 "swx_run", "swx_export_state", "swx_run_graph", "swx_time_kernels", "swx_plan_set_option"):
         getattr(lib, name).restype = c_int
     _lib = lib
     return lib

@@ -41,7 +41,7 @@ struct MfmaShape {
   static constexpr int ICH = 64;                     // longitudes per chunk
   static constexpr int ROWS = 16;
   // LDS floats: X^T [5][K1p][16] + Y^T [8][ICH][16]; the final exchange
-  // buffer [8][16][16] reuses the Y^T region.
+  // buffer [2][8][16][16] reuses the Y^T region.
   static constexpr int LDS_FLOATS = 5 * K1p * 16 + 8 * ICH * 16;
 };
 
@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(256) k_grid_mfma(GridArgs<float> a, const floa
   extern __shared__ __align__(16) float lds[];
   float* sXT = lds;                    // [5][K1p][16]
   float* sYT = lds + 5 * K1p * 16;     // [8][ICH][16]
-  float* sC = sYT;                     // [8][16][16] after the last chunk
+  float* sC = sYT;                     // [2][8][16][16] after the last chunk
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -88,18 +88,24 @@ __global__ void __launch_bounds__(256) k_grid_mfma(GridArgs<float> a, const floa
   const int n = ln;
   const int j = j0 + n;
   for (int ic = 0; ic < I; ic += ICH) {
-    // ---- Phase 1: inverse DFT of one 16-wide i-tile, all five fields.
+    // ---- Phase 1: inverse DFT of one 16-wide i-tile, all five fields.  The
+    // wave's A fragments (its i-tile of Tinv) are loaded up front.
     const int ib = ic + 16 * wave;
     v4f c1[5];
 #pragma unroll
     for (int f = 0; f < 5; ++f) c1[f] = v4f{0.f, 0.f, 0.f, 0.f};
-    const float* Ta = TinvT + (long long)(ib + ln) * K1p + lg;
-#pragma unroll 2
+    float afr[K1p / 4];
+    {
+      const float* Ta = TinvT + (long long)(ib + ln) * K1p + lg;
+#pragma unroll
+      for (int ks = 0; ks < K1p / 4; ++ks) afr[ks] = Ta[4 * ks];
+    }
+#pragma unroll
     for (int ks = 0; ks < K1p / 4; ++ks) {
-      const float av = Ta[4 * ks];
       const float* xb = sXT + (4 * ks + lg) * 16 + ln;
 #pragma unroll
-      for (int f = 0; f < 5; ++f) c1[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, xb[f * K1p * 16], c1[f], 0, 0, 0);
+      for (int f = 0; f < 5; ++f)
+        c1[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(afr[ks], xb[f * K1p * 16], c1[f], 0, 0, 0);
     }
     // ---- Grid physics on this lane's 4 points (row n, i = ib + 4 lg + q).
 #pragma unroll
@@ -138,42 +144,59 @@ __global__ void __launch_bounds__(256) k_grid_mfma(GridArgs<float> a, const floa
       for (int f = 0; f < 8; ++f) sYT[(f * ICH + il) * 16 + n] = (float)prod[f];
     }
     __syncthreads();
-    // ---- Phase 2: forward DFT partial sums over this chunk.
+    // ---- Phase 2: forward DFT partial sums over this chunk; the A fragments
+    // of step ks+1 are fetched while step ks runs.
+    float an[NT2];
+#pragma unroll
+    for (int t = 0; t < NT2; ++t) an[t] = TfwdT[(long long)(16 * t + ln) * I + ic + lg];
 #pragma unroll 2
     for (int ks = 0; ks < ICH / 4; ++ks) {
       const int il = 4 * ks + lg;
+      float ac[NT2];
+#pragma unroll
+      for (int t = 0; t < NT2; ++t) ac[t] = an[t];
+      if (ks + 1 < ICH / 4) {
+#pragma unroll
+        for (int t = 0; t < NT2; ++t) an[t] = TfwdT[(long long)(16 * t + ln) * I + ic + il + 4];
+      }
       const float b0 = sYT[((2 * wave) * ICH + il) * 16 + ln];
       const float b1 = sYT[((2 * wave + 1) * ICH + il) * 16 + ln];
 #pragma unroll
       for (int t = 0; t < NT2; ++t) {
-        const float av = TfwdT[(long long)(16 * t + ln) * I + ic + il];
-        acc[2 * t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b0, acc[2 * t], 0, 0, 0);
-        acc[2 * t + 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b1, acc[2 * t + 1], 0, 0, 0);
+        acc[2 * t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[t], b0, acc[2 * t], 0, 0, 0);
+        acc[2 * t + 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[t], b1, acc[2 * t + 1], 0, 0, 0);
       }
     }
     __syncthreads();
   }
 
-  // ---- Phase 3: assemble the Legendre inputs, one k'-tile (8 wavenumbers) at a time.
+  // ---- Phase 3: assemble the Legendre inputs, two k'-tiles (16 wavenumbers)
+  // per round through LDS, one (m, row) pair per thread.
 #pragma unroll
-  for (int t = 0; t < NT2; ++t) {
+  for (int t0 = 0; t0 < NT2; t0 += 2) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int f = 2 * wave + h;
+    for (int dt2 = 0; dt2 < 2; ++dt2) {
+      const int t = t0 + dt2;
+      if (t < NT2) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) sC[(f * 16 + 4 * lg + q) * 16 + ln] = acc[2 * t + h][q];
+        for (int h = 0; h < 2; ++h) {
+          const int f = 2 * wave + h;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) sC[((dt2 * 8 + f) * 16 + 4 * lg + q) * 16 + ln] = acc[2 * t + h][q];
+        }
+      }
     }
     __syncthreads();
-    if (tid < 128) {
-      const int ml = tid >> 4, nn = tid & 15;
-      const int m = 8 * t + ml;
-      if (m < Mp) {
+    {
+      const int dt2 = tid >> 7, ml = (tid >> 4) & 7, nn = tid & 15;
+      const int m = 8 * (t0 + dt2) + ml;
+      if (t0 + dt2 < NT2 && m < Mp) {
         const int jj = j0 + nn;
         double fm[8][2];
 #pragma unroll
         for (int f = 0; f < 8; ++f) {
-          fm[f][0] = sC[(f * 16 + 2 * ml) * 16 + nn];
-          fm[f][1] = sC[(f * 16 + 2 * ml + 1) * 16 + nn];
+          fm[f][0] = sC[((dt2 * 8 + f) * 16 + 2 * ml) * 16 + nn];
+          fm[f][1] = sC[((dt2 * 8 + f) * 16 + 2 * ml + 1) * 16 + nn];
         }
         double Um[2], Vm[2];
         Um[0] = sXT[(3 * K1p + 2 * m) * 16 + nn];

@@ -7,6 +7,7 @@
 // benchmark driver replays a captured HIP graph of many steps.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -77,7 +78,9 @@ struct swx_plan {
   void* tw = nullptr;
   float* TinvT = nullptr;  // MFMA grid kernel tables (FP32 plans)
   float* TfwdT = nullptr;
-  bool use_mfma = false;
+  bool use_mfma = false;       // MFMA grid kernel available for this plan
+  bool opt_plain_grid = false;  // SWX_OPT_GRID_KERNEL = 1
+  bool opt_full_phys = false;   // SWX_OPT_FULL_PHYS_CARRY = 1
   double* geo_d = nullptr;
   double* f2 = nullptr;
   double* coef = nullptr;
@@ -231,7 +234,7 @@ GridArgs<R> grid_args(swx_plan* p, int mode) {
 template <typename R>
 hipError_t launch_grid_any(swx_plan* p, const GridArgs<R>& g, hipStream_t s) {
   if constexpr (sizeof(R) == 4) {
-    const bool mfma_ok = p->use_mfma && g.mode == GRID_STEP && !g.diagnostics && !g.hist_fields && !g.hist_phiext &&
+    const bool mfma_ok = p->use_mfma && !p->opt_plain_grid && g.mode == GRID_STEP && !g.diagnostics && !g.hist_fields && !g.hist_phiext &&
                          !(g.test == 1 && g.expflag);
     if (mfma_ok) return launch_grid_mfma(g, p->TinvT, p->TfwdT, s);
   }
@@ -273,9 +276,24 @@ int do_init(swx_plan* p, const double* eta0, const double* delta0, const double*
   return SWX_OK;
 }
 
+// Number of final steps whose physical (grid) carry can still reach the
+// exported state: the Robert-Asselin recurrence prev' = a prev + (1-2a) curr
+// + a new contracts earlier levels by |a| per step, so older contributions
+// fall below 2^-64 relative after ceil(64 ln2 / ln(1/|a|)) steps.  Steps
+// before the window do not need to write the physical carry.
+int phys_window(const swx_plan* p) {
+  if (p->opt_full_phys || p->d.diagnostics) return 1 << 30;
+  double amax = 0.0;
+  for (int b = 0; b < p->d.n_members; ++b) amax = std::max(amax, std::fabs(p->params[(size_t)b * SWX_NPARAM + SWX_P_ALPHA]));
+  if (amax >= 0.5) return 1 << 30;
+  if (amax == 0.0) return 16;
+  const int k = (int)std::ceil(64.0 * std::log(2.0) / std::log(1.0 / amax)) + 2;
+  return std::max(16, k);
+}
+
 template <typename R>
 int do_steps(swx_plan* p, int t0, int nsteps, double* hist_fields, double* hist_scal, double* hist_phiext,
-             unsigned char* hist_dead, int hist_T, int hist_k0, bool save_last, hipStream_t s) {
+             unsigned char* hist_dead, int hist_T, int hist_k0, bool save_last, hipStream_t s, int phys_from = 0) {
   const int B = p->d.n_members;
   const size_t gbytes = (size_t)B * kNG * p->geo.J * (p->geo.M + 1) * 2 * p->rsz;
   for (int k = 0; k < nsteps; ++k) {
@@ -290,6 +308,7 @@ int do_steps(swx_plan* p, int t0, int nsteps, double* hist_fields, double* hist_
     SWX_HIP(launch_leg<R>(l, s));
     GridArgs<R> g = grid_args<R>(p, GRID_STEP);
     g.do_ra = (p->d.modalflag && t > 2) ? 1 : 0;
+    g.phys_on = (k >= phys_from || hist_fields || hist_phiext) ? 1 : 0;
     g.export_raw = last ? 1 : 0;
     g.hist_fields = hist_fields;
     g.hist_phiext = hist_phiext;
@@ -423,8 +442,7 @@ int swx_plan_create(const swx_plan_desc* desc, const double* params, swx_plan** 
   if (desc->precision == SWX_PREC_FP32) {
     int maxlds = 0;
     (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device);
-    const char* off = std::getenv("SWX_DISABLE_MFMA");
-    p->use_mfma = grid_mfma_lds_bytes(M) <= (size_t)maxlds && !(off && off[0] == '1');
+    p->use_mfma = grid_mfma_lds_bytes(M) <= (size_t)maxlds;
     if (p->use_mfma) {
       const int k1p = grid_mfma_k1p(M), k2p = grid_mfma_k2p(M);
       std::vector<double> ti((size_t)I * k1p, 0.0), tf((size_t)k2p * I, 0.0);
@@ -505,9 +523,12 @@ int swx_run(swx_plan* p, int t0, int nsteps, double* hist_fields, double* hist_s
   if (any_hist && (hist_T <= 0 || hist_k0 < 0 || hist_k0 + nsteps > hist_T))
     return fail(SWX_ERR_ARG, "history window out of range");
   hipStream_t s = (hipStream_t)stream;
+  const int phys_from = any_hist ? 0 : std::max(0, nsteps - phys_window(p));
   if (p->rsz == 8)
-    return do_steps<double>(p, t0, nsteps, hist_fields, hist_scalars, hist_phiext, hist_dead, hist_T, hist_k0, true, s);
-  return do_steps<float>(p, t0, nsteps, hist_fields, hist_scalars, hist_phiext, hist_dead, hist_T, hist_k0, true, s);
+    return do_steps<double>(p, t0, nsteps, hist_fields, hist_scalars, hist_phiext, hist_dead, hist_T, hist_k0, true, s,
+                            phys_from);
+  return do_steps<float>(p, t0, nsteps, hist_fields, hist_scalars, hist_phiext, hist_dead, hist_T, hist_k0, true, s,
+                         phys_from);
 }
 
 int swx_run_graph(swx_plan* p, int t0, int nsteps, int graph_steps, void* stream) {
@@ -516,19 +537,26 @@ int swx_run_graph(swx_plan* p, int t0, int nsteps, int graph_steps, void* stream
   if (!p->initialized) return fail(SWX_ERR_STATE, "swx_run_graph before swx_init");
   if (nsteps <= 0) return SWX_OK;
   hipStream_t s = (hipStream_t)stream;
-  auto steps = [&](int ts, int n, bool save_last) {
-    if (p->rsz == 8) return do_steps<double>(p, ts, n, nullptr, nullptr, nullptr, nullptr, 0, 0, save_last, s);
-    return do_steps<float>(p, ts, n, nullptr, nullptr, nullptr, nullptr, 0, 0, save_last, s);
+  if (graph_steps > 0 && s == nullptr) return fail(SWX_ERR_ARG, "graph replay needs a non-default stream");
+  // phys_on: from step phys_from of this call (the window that reaches the exported state).
+  const int window = phys_window(p);
+  const int phys_from = std::max(0, nsteps - window);
+  auto steps = [&](int k0, int n, bool save_last) {
+    const int pf = std::max(0, phys_from - k0);
+    if (p->rsz == 8)
+      return do_steps<double>(p, t0 + k0, n, nullptr, nullptr, nullptr, nullptr, 0, 0, save_last, s, pf);
+    return do_steps<float>(p, t0 + k0, n, nullptr, nullptr, nullptr, nullptr, 0, 0, save_last, s, pf);
   };
-  // Steps with t <= 2 differ (no Robert-Asselin filter) and the last step
-  // saves the previous Fourier level; both run eagerly.
-  int t = t0, left = nsteps;
-  while (left > 1 && t <= 2) {
-    if ((rc = steps(t, 1, false))) return rc;
-    ++t;
-    --left;
+  // Steps with t <= 2 differ (no Robert-Asselin filter) and run eagerly, as do
+  // the steps of the physical-carry window and the last step (which saves the
+  // previous Fourier level).
+  int k = 0;
+  while (k < nsteps && t0 + k <= 2) {
+    if ((rc = steps(k, 1, k == nsteps - 1))) return rc;
+    ++k;
   }
-  if (graph_steps > 0 && left - 1 >= graph_steps) {
+  const int graph_end = std::min(phys_from, nsteps - 1);
+  if (graph_steps > 0 && graph_end - k >= graph_steps) {
     if (!p->gexec || p->gexec_steps != graph_steps || p->gexec_stream != s) {
       if (p->gexec) {
         (void)hipGraphExecDestroy(p->gexec);
@@ -536,7 +564,11 @@ int swx_run_graph(swx_plan* p, int t0, int nsteps, int graph_steps, void* stream
       }
       hipGraph_t graph = nullptr;
       SWX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      rc = steps(t, graph_steps, false);
+      // Captured steps have t > 2 and lie before the window (phys_on = 0).
+      if (p->rsz == 8)
+        rc = do_steps<double>(p, 3, graph_steps, nullptr, nullptr, nullptr, nullptr, 0, 0, false, s, 1 << 30);
+      else
+        rc = do_steps<float>(p, 3, graph_steps, nullptr, nullptr, nullptr, nullptr, 0, 0, false, s, 1 << 30);
       hipError_t ec = hipStreamEndCapture(s, &graph);
       if (rc) return rc;
       if (ec != hipSuccess) return fail(SWX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
@@ -546,18 +578,34 @@ int swx_run_graph(swx_plan* p, int t0, int nsteps, int graph_steps, void* stream
       p->gexec_steps = graph_steps;
       p->gexec_stream = s;
     }
-    while (left - 1 >= graph_steps) {
+    while (graph_end - k >= graph_steps) {
       SWX_HIP(hipGraphLaunch(p->gexec, s));
-      t += graph_steps;
-      left -= graph_steps;
+      k += graph_steps;
     }
   }
-  if (left > 1) {
-    if ((rc = steps(t, left - 1, false))) return rc;
-    t += left - 1;
-    left = 1;
+  if (k < nsteps) return steps(k, nsteps - k, true);
+  return SWX_OK;
+}
+
+int swx_plan_set_option(swx_plan* p, int option, int value) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  switch (option) {
+    case SWX_OPT_GRID_KERNEL:
+      if (value != 0 && value != 1) return fail(SWX_ERR_ARG, "SWX_OPT_GRID_KERNEL must be 0 (auto) or 1 (plain)");
+      p->opt_plain_grid = value == 1;
+      break;
+    case SWX_OPT_FULL_PHYS_CARRY:
+      p->opt_full_phys = value != 0;
+      break;
+    default:
+      return fail(SWX_ERR_ARG, "unknown option " + std::to_string(option));
   }
-  return steps(t, left, true);
+  if (p->gexec) {
+    (void)hipGraphExecDestroy(p->gexec);
+    p->gexec = nullptr;
+  }
+  return SWX_OK;
 }
 
 int swx_time_kernels(swx_plan* p, int t0, int nsteps, float* ms_leg, float* ms_grid, void* stream) {

@@ -100,6 +100,10 @@ class Plan:
         _capi.check(self._lib.swx_plan_set_params(self._h, params.ctypes.data_as(ctypes.c_void_p),
                                                   self._stream(stream)), "swx_plan_set_params")
 
+    def set_option(self, option: int, value: int) -> None:
+        """Plan option (see SWX_OPT_* in include/swampe_capi.h)."""
+        _capi.check(self._lib.swx_plan_set_option(self._h, int(option), int(value)), "swx_plan_set_option")
+
     def init(self, eta0, delta0, Phi0, U0=None, V0=None, stream=None) -> None:
         if (U0 is None) != (V0 is None):
             raise ValueError("Provide both U0_init and V0_init, or neither.")
