@@ -628,13 +628,17 @@ int swx_time_kernels(swx_plan* p, int t0, int nsteps, float* ms_leg, float* ms_g
       SWX_HIP(launch_leg<float>(l, s));
     }
     SWX_HIP(hipEventRecord(ev[1], s));
+    // Same physical-carry mode as the steady steps of swx_run_graph.
+    const int phys_on = phys_window(p) >= (1 << 30) ? 1 : 0;
     if (p->rsz == 8) {
       GridArgs<double> g = grid_args<double>(p, GRID_STEP);
       g.do_ra = (p->d.modalflag && t > 2) ? 1 : 0;
+      g.phys_on = phys_on;
       SWX_HIP(launch_grid<double>(g, s));
     } else {
       GridArgs<float> g = grid_args<float>(p, GRID_STEP);
       g.do_ra = (p->d.modalflag && t > 2) ? 1 : 0;
+      g.phys_on = phys_on;
       SWX_HIP(launch_grid_any<float>(p, g, s));
     }
     SWX_HIP(hipEventRecord(ev[2], s));
