@@ -155,6 +155,37 @@ int swx_run_graph(swx_plan *plan, int t0, int nsteps, int graph_steps, void *str
  * scripts/benchmark_scan.py:116-141). */
 int swx_time_kernels(swx_plan *plan, int t0, int nsteps, float *ms_leg, float *ms_grid, void *stream);
 
+/* ---- Spectral initial state and checkpoints ------------------------------
+ * State arrays are device complex128 [B][3][M+1][N+1] (eta, delta, Phi
+ * spectral coefficients, n >= m used). */
+
+/* Two-level start from spectral coefficients c: the grid fields are their
+ * inverse transforms (winds by the spectral inversion), so the first step is
+ * the same map as every later step.  The differentiable entry point of the
+ * adjoint (the reference differentiates the scan through jax.grad,
+ * unit_tests/test_autodiff.py:53-185). */
+int swx_init_spectral(swx_plan *plan, const double *c, void *stream);
+
+/* Save / restore the current spectral level (checkpointing).  state_set also
+ * recomputes the Fourier rows and Legendre inputs of the restored level. */
+int swx_state_get(swx_plan *plan, double *out, void *stream);
+int swx_state_set(swx_plan *plan, const double *in, void *stream);
+
+/* ---- Hand-written adjoint (reverse mode) of the modified-Euler step -----
+ * lam = dL/dS is held by the plan.  Usage: begin; terminal (cotangents of
+ * the final grid fields, device float64 [B][J][I], any may be NULL); then for
+ * t = T-1 .. 0: step(S_t) with the forward state S_t before step t; result
+ * gives dL/dS_0 (device complex128 [B][3][M+1][N+1]) and the accumulated
+ * parameter gradients (device float64 [B][3]: taurad, taudrag, DPhieq).
+ * Replaces the reverse-mode transpose of the lax.scan that jax.grad builds
+ * (model.py:1281-1342; SURVEY call stack D).  The forward carry of the plan
+ * (Fourier rows) is used as scratch. */
+int swx_adjoint_begin(swx_plan *plan, void *stream);
+int swx_adjoint_terminal(swx_plan *plan, const double *g_eta, const double *g_delta, const double *g_Phi,
+                         const double *g_U, const double *g_V, void *stream);
+int swx_adjoint_step(swx_plan *plan, const double *state_t, void *stream);
+int swx_adjoint_result(swx_plan *plan, double *lam_out, double *pgrad_out, void *stream);
+
 /* Plan options (swx_plan_set_option).
  *   SWX_OPT_GRID_KERNEL: 0 = automatic (MFMA grid kernel for FP32 forward
  *     stepping when it fits), 1 = always the plain grid kernel.

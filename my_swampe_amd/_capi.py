@@ -31,7 +31,8 @@ NPARAM = len(PARAM_NAMES)
 EXPORTED_SYMBOLS = (
     "swx_geometry", "swx_grid_size", "swx_plan_create", "swx_plan_set_params", "swx_plan_destroy",
     "swx_init", "swx_run", "swx_export_state", "swx_run_graph", "swx_time_kernels", "swx_plan_set_option",
-    "swx_last_error", "swx_version",
+    "swx_init_spectral", "swx_state_get", "swx_state_set", "swx_adjoint_begin", "swx_adjoint_terminal",
+    "swx_adjoint_step", "swx_adjoint_result", "swx_last_error", "swx_version",
 )
 
 STATE_FIELDS = (
@@ -85,11 +86,20 @@ def load() -> ctypes.CDLL:
     lib.swx_export_state.argtypes = [c_void_p, POINTER(StateOut), c_void_p]
     lib.swx_run_graph.argtypes = [c_void_p, c_int, c_int, c_int, c_void_p]
     lib.swx_plan_set_option.argtypes = [c_void_p, c_int, c_int]
+    lib.swx_init_spectral.argtypes = [c_void_p, c_void_p, c_void_p]
+    lib.swx_state_get.argtypes = [c_void_p, c_void_p, c_void_p]
+    lib.swx_state_set.argtypes = [c_void_p, c_void_p, c_void_p]
+    lib.swx_adjoint_begin.argtypes = [c_void_p, c_void_p]
+    lib.swx_adjoint_terminal.argtypes = [c_void_p] + [c_void_p] * 5 + [c_void_p]
+    lib.swx_adjoint_step.argtypes = [c_void_p, c_void_p, c_void_p]
+    lib.swx_adjoint_result.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p]
     lib.swx_time_kernels.argtypes = [c_void_p, c_int, c_int, POINTER(ctypes.c_float), POINTER(ctypes.c_float), c_void_p]
     for name in EXPORTED_SYMBOLS:
         getattr(lib, name)  # AttributeError when a declared symbol is missing
     for name in ("swx_geometry", "swx_grid_size", "swx_plan_create", "swx_plan_set_params", "swx_init",
-                 "swx_run", "swx_export_state", "swx_run_graph", "swx_time_kernels", "swx_plan_set_option"):
+                 "swx_run", "swx_export_state", "swx_run_graph", "swx_time_kernels", "swx_plan_set_option",
+                 "swx_init_spectral", "swx_state_get", "swx_state_set", "swx_adjoint_begin",
+                 "swx_adjoint_terminal", "swx_adjoint_step", "swx_adjoint_result"):
         getattr(lib, name).restype = c_int
     _lib = lib
     return lib

@@ -1,0 +1,174 @@
+"""Differentiable rollout: hand-written adjoint behind torch.autograd.Function.
+
+``rollout_phi(c, taurad, taudrag, DPhieq, ...)`` advances spectral initial
+coefficients ``c`` (B, 3, M+1, N+1) of eta, delta, Phi by ``nsteps`` forced
+modified-Euler steps on the GPU and returns the final geopotential grid
+(B, J, I).  Its backward runs the native reverse-mode kernels
+(swx_adjoint_*): the forward pass keeps only checkpoints of the spectral state
+every K steps (K = ceil(sqrt(nsteps)) by default), and the backward pass
+recomputes each segment from its checkpoint, keeping that segment's states,
+then steps the adjoint back through it.  Memory: (nsteps/K + K) spectral
+states per member instead of the full trajectory.
+
+Gradients are returned for c (complex, d/dRe + i d/dIm) and for the
+per-member taurad, taudrag and DPhieq; the other physical constants are fixed.
+This replaces reverse-mode differentiation of the reference's scan
+(model.py:1281-1342 under jax.grad, unit_tests/test_autodiff.py:53-185).
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from .model import param_row
+from .plan import Plan, require_gpu
+
+
+@dataclass
+class RolloutConfig:
+    M: int
+    nsteps: int
+    dt: float
+    a: float
+    omega: float
+    Phibar: float
+    K6: float = 1.24e33
+    K6Phi: Optional[float] = None
+    alpha: float = 0.01
+    g: float = 9.8
+    precision: str = "fp64"
+    checkpoint_every: Optional[int] = None
+    diffflag: bool = True
+    modalflag: bool = True
+    forcflag: bool = True
+    t0: int = 2
+    stats: dict = field(default_factory=dict)
+
+    def every(self) -> int:
+        if self.checkpoint_every:
+            return max(1, int(self.checkpoint_every))
+        return max(1, int(math.ceil(math.sqrt(max(self.nsteps, 1)))))
+
+
+def _params(cfg: RolloutConfig, taurad, taudrag, DPhieq) -> np.ndarray:
+    tr = np.atleast_1d(np.asarray(taurad, dtype=np.float64))
+    td = np.atleast_1d(np.asarray(taudrag, dtype=np.float64))
+    dp = np.atleast_1d(np.asarray(DPhieq, dtype=np.float64))
+    B = max(tr.size, td.size, dp.size)
+    tr, td, dp = (np.broadcast_to(x, (B,)) for x in (tr, td, dp))
+    return np.stack([param_row(dt=cfg.dt, a=cfg.a, omega=cfg.omega, g=cfg.g, Phibar=cfg.Phibar, taurad=tr[b],
+                               taudrag=td[b], DPhieq=dp[b], K6=cfg.K6, K6Phi=cfg.K6Phi, alpha=cfg.alpha)
+                     for b in range(B)])
+
+
+def _make_plan(cfg: RolloutConfig, params) -> Plan:
+    return Plan(cfg.M, params, precision=cfg.precision, forcflag=cfg.forcflag, diffflag=cfg.diffflag,
+                modalflag=cfg.modalflag, diagnostics=False)
+
+
+def forward_with_checkpoints(plan: Plan, cfg: RolloutConfig, c):
+    """Run the rollout from c, returning the checkpoints (spectral states before each segment)."""
+    K = cfg.every()
+    plan.init_spectral(c)
+    ckpts: List = []
+    t = cfg.t0
+    done = 0
+    while done < cfg.nsteps:
+        n = min(K, cfg.nsteps - done)
+        ckpts.append(plan.state_get())
+        plan.run(t, n)
+        t += n
+        done += n
+    return ckpts
+
+
+def adjoint_through_checkpoints(plan: Plan, cfg: RolloutConfig, ckpts, gPhi):
+    """Reverse sweep: terminal cotangent gPhi (B, J, I) -> (dL/dc, dL/d(taurad, taudrag, DPhieq))."""
+    K = cfg.every()
+    plan.adjoint_begin()
+    plan.adjoint_terminal(g_Phi=gPhi)
+    peak = 0
+    for seg in reversed(range(len(ckpts))):
+        start = seg * K
+        n = min(K, cfg.nsteps - start)
+        plan.state_set(ckpts[seg])
+        tape = [ckpts[seg]]
+        t = cfg.t0 + start
+        for s in range(n - 1):
+            plan.run(t + s, 1)
+            tape.append(plan.state_get())
+        peak = max(peak, len(tape))
+        for s in reversed(range(n)):
+            plan.adjoint_step(tape[s])
+    cfg.stats.update(checkpoints=len(ckpts), segment_states=peak)
+    return plan.adjoint_result()
+
+
+def _fold(g, shape):
+    """Per-member gradient g (B,) folded onto the parameter's shape (a shared scalar sums members)."""
+    if len(shape) == 0:
+        return g.sum()
+    if shape[0] == 1 and g.numel() > 1:
+        return g.sum().reshape(shape)
+    return g.reshape(shape)
+
+
+class _Rollout:
+    """torch.autograd.Function factory (torch imported lazily)."""
+
+    _cls = None
+
+    @classmethod
+    def get(cls):
+        if cls._cls is not None:
+            return cls._cls
+        torch = require_gpu()
+
+        class Fn(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, c, taurad, taudrag, DPhieq, cfg: RolloutConfig):
+                params = _params(cfg, taurad.detach().cpu().numpy(), taudrag.detach().cpu().numpy(),
+                                 DPhieq.detach().cpu().numpy())
+                plan = _make_plan(cfg, params)
+                ckpts = forward_with_checkpoints(plan, cfg, c.detach())
+                phi = plan.export(fields=("Phi_curr",))["Phi_curr"]
+                torch.cuda.current_stream(plan.device).synchronize()
+                ctx.plan, ctx.cfg, ctx.ckpts = plan, cfg, ckpts
+                ctx.shapes = (taurad.shape, taudrag.shape, DPhieq.shape)
+                return phi
+
+            @staticmethod
+            def backward(ctx, gPhi):
+                plan, cfg = ctx.plan, ctx.cfg
+                lam, pg = adjoint_through_checkpoints(plan, cfg, ctx.ckpts, gPhi.contiguous())
+                torch.cuda.current_stream(plan.device).synchronize()
+                plan.close()
+                ctx.plan = None
+                ctx.ckpts = None
+                sr, sd, sp = ctx.shapes
+                return lam, _fold(pg[:, 0], sr), _fold(pg[:, 1], sd), _fold(pg[:, 2], sp), None
+
+        cls._cls = Fn
+        return Fn
+
+
+def rollout_phi(c, taurad, taudrag, DPhieq, *, M: int, nsteps: int, dt: float, a: float, omega: float,
+                Phibar: float, K6: float = 1.24e33, K6Phi: Optional[float] = None, alpha: float = 0.01,
+                precision: str = "fp64", checkpoint_every: Optional[int] = None, diffflag: bool = True,
+                modalflag: bool = True, forcflag: bool = True, stats: Optional[dict] = None):
+    """Final Phi grid (B, J, I) of a differentiable rollout (see module docstring).
+
+    c: complex128 tensor (B, 3, M+1, N+1); taurad, taudrag, DPhieq: float64 tensors of
+    shape (B,) or scalars (shared by all members).
+    """
+    torch = require_gpu()
+    dev = torch.device("cuda", torch.cuda.current_device())
+    cfg = RolloutConfig(M=M, nsteps=nsteps, dt=dt, a=a, omega=omega, Phibar=Phibar, K6=K6, K6Phi=K6Phi,
+                        alpha=alpha, precision=precision, checkpoint_every=checkpoint_every, diffflag=diffflag,
+                        modalflag=modalflag, forcflag=forcflag, stats=stats if stats is not None else {})
+    conv = lambda x: x if torch.is_tensor(x) else torch.tensor(x, dtype=torch.float64, device=dev)  # noqa: E731
+    return _Rollout.get().apply(c, conv(taurad), conv(taudrag), conv(DPhieq), cfg)

@@ -1,0 +1,55 @@
+// Argument blocks and launchers of the adjoint kernels (swx_adjoint.hip).
+#pragma once
+
+#include "swx_kernels.h"
+
+namespace swx {
+
+enum LegAdjMode : int { LADJ_A = 0, LADJ_B = 1, LADJ_TERM = 2 };
+enum GridAdjMode : int { GADJ_STEP = 0, GADJ_TERM = 1 };
+
+template <typename R>
+struct LegAdjArgs {
+  int M, N, J, B;
+  int mode, diffflag;
+  const R* Pf;
+  const R* Hf;
+  const R* Pi;
+  const R* Hi;
+  const double* coef;
+  int ncoef;
+  const double* f2;
+  double* Lam;        // [B][3][M+1][N+1] complex: dL/dS
+  const double* St;   // [B][3][M+1][N+1] complex: forward state S_t (mode A)
+  R* Tbar;            // [B][M+1][kNT][J] complex slot cotangents (written by mode A)
+  R* G;               // [B][kNG][J][M+1] complex Fourier rows of S_t (written by mode A)
+  const R* Gbar;      // [B][kNG][J][M+1] complex Fourier cotangents (modes B, TERM)
+};
+
+template <typename R>
+struct GridAdjArgs {
+  int M, N, J, I, B;
+  int mode, forcflag, test;
+  const R* tw;
+  const double* geo;
+  const double* coef;
+  int ncoef;
+  const R* G;         // recomputed Fourier rows of S_t
+  const R* Tbar;      // slot cotangents
+  R* Gbar;            // out: Fourier cotangents
+  const double* uvic; // pinned winds (test 1)
+  double* pgrad;      // [B][J][3] accumulated d/d(taurad, taudrag, DPhieq)
+  const double* g_eta;  // GADJ_TERM grid cotangents [B][J][I] (any may be null)
+  const double* g_delta;
+  const double* g_Phi;
+  const double* g_U;
+  const double* g_V;
+};
+
+template <typename R>
+hipError_t launch_leg_adj(const LegAdjArgs<R>& a, hipStream_t s);
+template <typename R>
+hipError_t launch_grid_adj(const GridAdjArgs<R>& a, hipStream_t s);
+hipError_t launch_pgrad_reduce(const double* pg, double* out, int B, int J, hipStream_t s);
+
+}  // namespace swx

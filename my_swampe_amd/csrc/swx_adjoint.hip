@@ -1,0 +1,377 @@
+// Hand-written adjoint (reverse mode) of the modified-Euler step.
+//
+// The forward step maps the spectral state S_t to
+//   S_{t+1} = sigma * [ C(f2 * S_t) + Leg(T(S_t)) ],
+//   T(S) = assemble(DFT(products(invDFT(invLeg(S))))),
+// (see swx_kernels.hip).  One reverse step takes lam = dL/dS_{t+1} to
+// dL/dS_t = f2 * C^T(sigma lam) + invLeg^T(invDFT^T(products'^T(DFT^T(assemble^T(Leg^T(sigma lam))))))
+// with the grid fields of S_t recomputed (invLeg then invDFT) instead of
+// stored.  Three launches:
+//   k_leg_adj  (mode A, grid (M+1, B)): mu = sigma lam, carry adjoint, slot
+//              cotangents tau = Leg^T(mu) (inverse-shaped contraction over n),
+//              and G_t = invLeg(S_t) for the recompute.
+//   k_grid_adj (grid (J, B)): invDFT of G_t, assemble^T, DFT^T (synthesis),
+//              pointwise Jacobian^T incl. the Q<0 clamp and the parameter
+//              partials (taurad, taudrag, DPhieq), invDFT^T (analysis).
+//   k_leg_adj  (mode B): invLeg^T with the wind-inversion adjoint
+//              (spectral_transform.py:577-594 transposed), added into lam.
+// Reference functions differentiated: modEuler_tdiff.py:30-268,
+// forcing.py:65-155, initial_conditions.py:242-292, spectral_transform.py:290-596.
+#include <cfloat>
+
+#include "swx_adjoint.h"
+#include "swx_physics.h"
+
+namespace swx {
+
+namespace {
+
+__device__ inline double dmin2(double a, double b) { return a < b ? a : b; }
+
+template <int NV>
+__device__ void block_sum(double (&v)[NV], double* scratch) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int k = 0; k < NV; ++k) scratch[k * nt + tid] = v[k];
+  __syncthreads();
+  for (int s = nt / 2; s > 0; s >>= 1) {
+    if (tid < s)
+      for (int k = 0; k < NV; ++k) scratch[k * nt + tid] += scratch[k * nt + tid + s];
+    __syncthreads();
+  }
+  for (int k = 0; k < NV; ++k) v[k] = scratch[k * nt];
+  __syncthreads();
+}
+
+}  // namespace
+
+template <typename R>
+__global__ void __launch_bounds__(128) k_leg_adj(LegAdjArgs<R> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int m = blockIdx.x, b = blockIdx.y;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int M = a.M, N = a.N, J = a.J, Mp = M + 1, Np = N + 1;
+  const int K = Np - m;
+  const double* cf = a.coef + (long long)b * a.ncoef;
+  const CoefLayout cl = CoefLayout::make(N, J);
+  const long long boff = tri_off(m, N, J);
+  const long long fst = (long long)Mp * Np * 2;  // doubles per spectral field
+  double* Lam = a.Lam + (long long)b * 3 * fst;
+  const double mm = (double)m;
+
+  if (a.mode == LADJ_A) {
+    R* sL = reinterpret_cast<R*>(smem_raw);  // [kNT][Np][2] slot cotangents over n
+    R* sX = sL + kNT * Np * 2;               // [5][Np][2] inputs of the recomputed inverse
+    const double* St = a.St + (long long)b * 3 * fst;
+    const double dt = cf[C_DT], aa = cf[C_A], Phibar = cf[C_PHIBAR];
+    const double t2 = (2.0 * dt / (aa * aa)) / 4.0;
+    for (int np = tid; np < K; np += nt) {
+      const int n = m + np;
+      const long long so = ((long long)m * Np + n) * 2;
+      const double nn = (double)n * (double)(n + 1);
+      const double sg = a.diffflag ? cf[cl.n_sigma + n] : 1.0;
+      const double sp = a.diffflag ? cf[cl.n_sigmaphi + n] : 1.0;
+      const double f2 = a.f2[(long long)m * Np + n];
+      for (int c = 0; c < 2; ++c) {
+        const double me = sg * Lam[0 * fst + so + c];
+        const double md = sg * Lam[1 * fst + so + c];
+        const double mp = sp * Lam[2 * fst + so + c];
+        // Carry adjoint (transpose of the FP64 spectral update in k_leg).
+        const double xp = mp * (1.0 - Phibar * 0.5 * (nn / (aa * aa))) + nn * t2 * md;
+        const double xd = -dt * Phibar * mp + md * (1.0 - Phibar * nn / (aa * aa));
+        Lam[0 * fst + so + c] = f2 * me;
+        Lam[1 * fst + so + c] = f2 * xd;
+        Lam[2 * fst + so + c] = f2 * xp;
+        // Slot cotangents before the transposed Legendre contraction.
+        sL[(0 * Np + np) * 2 + c] = (R)mp;
+        sL[(1 * Np + np) * 2 + c] = (R)(nn * mp);
+        sL[(2 * Np + np) * 2 + c] = (R)md;
+        sL[(3 * Np + np) * 2 + c] = (R)(nn * md);
+        sL[(4 * Np + np) * 2 + c] = (R)me;
+        sL[(5 * Np + np) * 2 + c] = (R)mp;
+        sL[(6 * Np + np) * 2 + c] = (R)md;
+        sL[(7 * Np + np) * 2 + c] = (R)(nn * md);
+        sL[(8 * Np + np) * 2 + c] = (R)me;
+      }
+      // Recompute inputs: eta, delta, Phi and the wind potentials of S_t.
+      const double tcmn = cf[cl.n_tcmn + n];
+      const double f = (m == 0 && n == 1) ? cf[C_FMN01] : 0.0;
+      for (int c = 0; c < 2; ++c) {
+        const double e = St[0 * fst + so + c], d = St[1 * fst + so + c], p = St[2 * fst + so + c];
+        sX[(0 * Np + np) * 2 + c] = (R)e;
+        sX[(1 * Np + np) * 2 + c] = (R)d;
+        sX[(2 * Np + np) * 2 + c] = (R)p;
+        sX[(3 * Np + np) * 2 + c] = (R)(n == 0 ? 0.0 : d * tcmn);
+        sX[(4 * Np + np) * 2 + c] = (R)(n == 0 ? 0.0 : (e - (c == 0 ? f : 0.0)) * tcmn);
+      }
+    }
+    __syncthreads();
+    const R* Pi = a.Pi + boff;
+    const R* Hi = a.Hi + boff;
+    R* Tb = a.Tbar + (((long long)b * Mp + m) * kNT) * J * 2;
+    R* G = a.G + (long long)b * kNG * J * Mp * 2;
+    const long long fs = (long long)J * Mp * 2;
+    const R mr = (R)m;
+    for (int j = tid; j < J; j += nt) {
+      R tp[2 * kNT];
+      R p[10], h[4];
+#pragma unroll
+      for (int k = 0; k < 2 * kNT; ++k) tp[k] = 0;
+#pragma unroll
+      for (int k = 0; k < 10; ++k) p[k] = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) h[k] = 0;
+      for (int np = 0; np < K; ++np) {
+        const R pv = Pi[(long long)np * J + j];
+        const R hv = Hi[(long long)np * J + j];
+#pragma unroll
+        for (int k = 0; k < kNP; ++k) {
+          tp[2 * k] += pv * sL[(k * Np + np) * 2];
+          tp[2 * k + 1] += pv * sL[(k * Np + np) * 2 + 1];
+        }
+#pragma unroll
+        for (int k = kNP; k < kNT; ++k) {
+          tp[2 * k] += hv * sL[(k * Np + np) * 2];
+          tp[2 * k + 1] += hv * sL[(k * Np + np) * 2 + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+          p[2 * k] += pv * sX[(k * Np + np) * 2];
+          p[2 * k + 1] += pv * sX[(k * Np + np) * 2 + 1];
+        }
+        h[0] += hv * sX[(4 * Np + np) * 2];
+        h[1] += hv * sX[(4 * Np + np) * 2 + 1];
+        h[2] += hv * sX[(3 * Np + np) * 2];
+        h[3] += hv * sX[(3 * Np + np) * 2 + 1];
+      }
+#pragma unroll
+      for (int k = 0; k < kNT; ++k) {
+        Tb[(k * J + j) * 2] = tp[2 * k];
+        Tb[(k * J + j) * 2 + 1] = tp[2 * k + 1];
+      }
+      const long long ro = ((long long)j * Mp + m) * 2;
+      G[0 * fs + ro] = p[0];
+      G[0 * fs + ro + 1] = p[1];
+      G[1 * fs + ro] = p[2];
+      G[1 * fs + ro + 1] = p[3];
+      G[2 * fs + ro] = p[4];
+      G[2 * fs + ro + 1] = p[5];
+      G[3 * fs + ro] = mr * p[7] + h[0];
+      G[3 * fs + ro + 1] = -mr * p[6] + h[1];
+      G[4 * fs + ro] = mr * p[9] - h[2];
+      G[4 * fs + ro + 1] = -mr * p[8] - h[3];
+    }
+    return;
+  }
+
+  // Modes B and TERM: lam (+)= invLeg^T(Gbar) with the wind-inversion adjoint.
+  R* sGb = reinterpret_cast<R*>(smem_raw);  // [5][J][2]
+  const R* Gb = a.Gbar + (long long)b * kNG * J * Mp * 2;
+  for (int e = tid; e < kNG * J * 2; e += nt) {
+    const int f = e / (J * 2), r = e % (J * 2);
+    const int j = r / 2, c = r % 2;
+    sGb[e] = Gb[((long long)f * J + j) * Mp * 2 + m * 2 + c];
+  }
+  __syncthreads();
+  const R* Pf = a.Pf + boff;
+  const R* Hf = a.Hf + boff;
+  for (int np = tid; np < K; np += nt) {
+    const int n = m + np;
+    R p[10], h[4];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) p[k] = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) h[k] = 0;
+    for (int j = 0; j < J; ++j) {
+      const R pv = Pf[(long long)j * K + np];
+      const R hv = Hf[(long long)j * K + np];
+#pragma unroll
+      for (int f = 0; f < 5; ++f) {
+        p[2 * f] += pv * sGb[(f * J + j) * 2];
+        p[2 * f + 1] += pv * sGb[(f * J + j) * 2 + 1];
+      }
+      h[0] += hv * sGb[(3 * J + j) * 2];
+      h[1] += hv * sGb[(3 * J + j) * 2 + 1];
+      h[2] += hv * sGb[(4 * J + j) * 2];
+      h[3] += hv * sGb[(4 * J + j) * 2 + 1];
+    }
+    // dhat = i m P^T(Ubar) - H^T(Vbar);  ehat = H^T(Ubar) + i m P^T(Vbar);  i m z = (-m z.im, m z.re)
+    const double dh[2] = {-mm * (double)p[7] - (double)h[2], mm * (double)p[6] - (double)h[3]};
+    const double eh[2] = {(double)h[0] - mm * (double)p[9], (double)h[1] + mm * (double)p[8]};
+    const double tcmn = n == 0 ? 0.0 : cf[cl.n_tcmn + n];
+    const long long so = ((long long)m * Np + n) * 2;
+    for (int c = 0; c < 2; ++c) {
+      const double le = (double)p[0 + c] + tcmn * eh[c];
+      const double ld = (double)p[2 + c] + tcmn * dh[c];
+      const double lp = (double)p[4 + c];
+      if (a.mode == LADJ_TERM) {
+        Lam[0 * fst + so + c] = le;
+        Lam[1 * fst + so + c] = ld;
+        Lam[2 * fst + so + c] = lp;
+      } else {
+        Lam[0 * fst + so + c] += le;
+        Lam[1 * fst + so + c] += ld;
+        Lam[2 * fst + so + c] += lp;
+      }
+    }
+  }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_grid_adj(GridAdjArgs<R> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int j = blockIdx.x, b = blockIdx.y;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int M = a.M, N = a.N, J = a.J, I = a.I, Mp = M + 1;
+  const double* cf = a.coef + (long long)b * a.ncoef;
+  const CoefLayout cl = CoefLayout::make(N, J);
+  const GeoLayout gl = GeoLayout::make(J, I);
+  const double* geo = a.geo;
+  const long long plane = (long long)J * I;
+
+  R* sTc = reinterpret_cast<R*>(smem_raw);  // [I]
+  R* sTs = sTc + I;                          // [I]
+  R* sG = sTs + I;                           // [5][Mp][2] forward Fourier row
+  R* sFb = sG + kNG * Mp * 2;                // [8][Mp][2] Fourier cotangents
+  R* sGg = sFb + 8 * Mp * 2;                 // [5][I] grid cotangents
+  double* sRed = reinterpret_cast<double*>(sGg + 5 * I);
+
+  for (int i = tid; i < I; i += nt) {
+    sTc[i] = a.tw[i];
+    sTs[i] = a.tw[I + i];
+  }
+  if (a.mode == GADJ_STEP) {
+    const R* Gr = a.G + (long long)b * kNG * J * Mp * 2;
+    for (int e = tid; e < kNG * Mp * 2; e += nt) {
+      const int f = e / (Mp * 2), r = e % (Mp * 2);
+      sG[e] = Gr[((long long)f * J + j) * Mp * 2 + r];
+    }
+    // assemble^T for every m of this row.
+    const double wj = geo[gl.w + j];
+    const double tc = cf[cl.n_tc + j];
+    for (int m = tid; m < Mp; m += nt) {
+      const R* Tb = a.Tbar + (((long long)b * Mp + m) * kNT) * J * 2;
+      double tau[kNT][2];
+      for (int k = 0; k < kNT; ++k) {
+        tau[k][0] = Tb[(k * J + j) * 2];
+        tau[k][1] = Tb[(k * J + j) * 2 + 1];
+      }
+      double fb[8][2];
+      assemble_slots_adjoint(cf, tc, wj, m, a.forcflag, tau, fb);
+      for (int f = 0; f < 8; ++f) {
+        sFb[(f * Mp + m) * 2] = (R)fb[f][0];
+        sFb[(f * Mp + m) * 2 + 1] = (R)fb[f][1];
+      }
+    }
+  }
+  __syncthreads();
+
+  double dpar[3] = {0.0, 0.0, 0.0};
+  const double invI = 1.0 / (double)I;
+  for (int i = tid; i < I; i += nt) {
+    double gb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // cotangents of eta, delta, Phi, U, V at (j, i)
+    if (a.mode == GADJ_TERM) {
+      const double* src[5] = {a.g_eta, a.g_delta, a.g_Phi, a.g_U, a.g_V};
+      for (int f = 0; f < 5; ++f)
+        if (src[f]) gb[f] = src[f][(long long)b * plane + (long long)j * I + i];
+    } else {
+      // Grid fields of S_t (inverse DFT of the recomputed Fourier row).
+      R acc[kNG];
+      for (int f = 0; f < kNG; ++f) acc[f] = 0;
+      R pb[8];
+      for (int f = 0; f < 8; ++f) pb[f] = (R)sFb[(f * Mp) * 2];  // m = 0 term of DFT^T (cos 0 = 1)
+      int idx = 0;
+      for (int m = 1; m < Mp; ++m) {
+        idx += i;
+        if (idx >= I) idx -= I;
+        const R c = sTc[idx], s = sTs[idx];
+        for (int f = 0; f < kNG; ++f) acc[f] += sG[(f * Mp + m) * 2] * c - sG[(f * Mp + m) * 2 + 1] * s;
+        // DFT^T: xbar_i = (1/I) sum_m (Xbar.re cos - Xbar.im sin)
+        for (int f = 0; f < 8; ++f) pb[f] += sFb[(f * Mp + m) * 2] * c - sFb[(f * Mp + m) * 2 + 1] * s;
+      }
+      double x[kNG];
+      for (int f = 0; f < kNG; ++f) x[f] = (double)sG[(f * Mp) * 2] + 2.0 * (double)acc[f];
+      double pbar[8];
+      for (int f = 0; f < 8; ++f) pbar[f] = (double)pb[f] * invI;
+      double U = x[3], V = x[4];
+      if (a.test == 1) {
+        U = a.uvic[((long long)b * 2 + 0) * plane + (long long)j * I + i];
+        V = a.uvic[((long long)b * 2 + 1) * plane + (long long)j * I + i];
+      }
+      double o[4];
+      grid_products_adjoint(cf, geo, gl, j, i, a.test, x[0], x[2], U, V, pbar, o, dpar);
+      gb[0] = o[0];
+      gb[2] = o[1];
+      if (a.test != 1) {  // pinned winds of test 1 do not depend on the state
+        gb[3] = o[2];
+        gb[4] = o[3];
+      }
+    }
+    for (int f = 0; f < 5; ++f) sGg[f * I + i] = (R)gb[f];
+  }
+  if (a.mode == GADJ_STEP) {
+    block_sum<3>(dpar, sRed);
+    if (tid == 0) {
+      double* pg = a.pgrad + ((long long)b * J + j) * 3;
+      pg[0] += dpar[0];
+      pg[1] += dpar[1];
+      pg[2] += dpar[2];
+    }
+  }
+  __syncthreads();
+
+  // invDFT^T: cbar_m = w_m sum_i xbar_i (cos theta, -sin theta), w_0 = 1, w_m = 2.
+  R* Gb = a.Gbar + (long long)b * kNG * J * Mp * 2;
+  for (int p = tid; p < kNG * Mp; p += nt) {
+    const int f = p / Mp, m = p % Mp;
+    R re = 0, im = 0;
+    int idx = 0;
+    const R* xf = sGg + f * I;
+    for (int i = 0; i < I; ++i) {
+      const R xv = xf[i];
+      re += xv * sTc[idx];
+      im -= xv * sTs[idx];
+      idx += m;
+      if (idx >= I) idx -= I;
+    }
+    const R wm = m == 0 ? (R)1 : (R)2;
+    Gb[((long long)f * J + j) * Mp * 2 + m * 2] = wm * re;
+    Gb[((long long)f * J + j) * Mp * 2 + m * 2 + 1] = m == 0 ? (R)0 : wm * im;
+  }
+}
+
+__global__ void k_pgrad_reduce(const double* pg, double* out, int J) {
+  const int b = blockIdx.x;
+  if (threadIdx.x < 3) {
+    double s = 0.0;
+    for (int j = 0; j < J; ++j) s += pg[((long long)b * J + j) * 3 + threadIdx.x];
+    out[b * 3 + threadIdx.x] = s;
+  }
+}
+
+template <typename R>
+hipError_t launch_leg_adj(const LegAdjArgs<R>& a, hipStream_t s) {
+  const int Np = a.N + 1;
+  size_t smem = a.mode == LADJ_A ? sizeof(R) * (kNT * Np * 2 + 5 * Np * 2) : sizeof(R) * (kNG * a.J * 2);
+  hipLaunchKernelGGL(k_leg_adj<R>, dim3(a.M + 1, a.B), dim3(128), smem + 16, s, a);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_grid_adj(const GridAdjArgs<R>& a, hipStream_t s) {
+  const int Mp = a.M + 1;
+  const size_t smem = sizeof(R) * (2 * a.I + kNG * Mp * 2 + 8 * Mp * 2 + 5 * a.I) + sizeof(double) * 3 * 256 + 16;
+  hipLaunchKernelGGL(k_grid_adj<R>, dim3(a.J, a.B), dim3(256), smem, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_pgrad_reduce(const double* pg, double* out, int B, int J, hipStream_t s) {
+  hipLaunchKernelGGL(k_pgrad_reduce, dim3(B), dim3(64), 0, s, pg, out, J);
+  return hipGetLastError();
+}
+
+template hipError_t launch_leg_adj<double>(const LegAdjArgs<double>&, hipStream_t);
+template hipError_t launch_leg_adj<float>(const LegAdjArgs<float>&, hipStream_t);
+template hipError_t launch_grid_adj<double>(const GridAdjArgs<double>&, hipStream_t);
+template hipError_t launch_grid_adj<float>(const GridAdjArgs<float>&, hipStream_t);
+
+}  // namespace swx

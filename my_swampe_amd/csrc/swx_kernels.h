@@ -74,7 +74,7 @@ __host__ __device__ inline long long tri_off(int m, int N, int J) {
   return (long long)J * ((long long)m * (N + 1) - (long long)m * (m - 1) / 2);
 }
 
-enum LegMode : int { LEG_INIT = 0, LEG_STEP = 1 };
+enum LegMode : int { LEG_INIT = 0, LEG_STEP = 1, LEG_PRIME = 2 };
 enum GridMode : int { GRID_DFT = 0, GRID_INIT = 1, GRID_STEP = 2 };
 
 template <typename R>
@@ -103,7 +103,7 @@ struct LegArgs {
 template <typename R>
 struct GridArgs {
   int M, N, J, I, B;
-  int mode, expflag, forcflag, test, diagnostics, do_ra, phys_on, export_raw, diag_winds, have_uv;
+  int mode, expflag, forcflag, test, diagnostics, do_ra, phys_on, export_raw, diag_winds, have_uv, init_from_g;
   const R* tw;         // [2][I]: cos(2 pi k / I), sin(2 pi k / I)
   const double* geo;
   const double* coef;

@@ -100,7 +100,7 @@ __global__ void __launch_bounds__(128) k_leg(LegArgs<R> a) {
 
   // Stage the Legendre inputs of (b, m).
   const long long toff = (((long long)b * (M + 1) + m) * kNT) * J * 2;
-  const int nslots = (a.mode == LEG_STEP) ? kNT : 3;
+  const int nslots = (a.mode == LEG_STEP) ? kNT : (a.mode == LEG_INIT ? 3 : 0);
   for (int e = tid; e < nslots * J * 2; e += nt) sT[e] = a.T[toff + e];
   __syncthreads();
 
@@ -116,7 +116,14 @@ __global__ void __launch_bounds__(128) k_leg(LegArgs<R> a) {
     const int n = m + np;
     const long long so = ((long long)m * (N + 1) + n) * 2;
     double eta_n[2], delta_n[2], Phi_n[2];
-    if (a.mode == LEG_INIT) {
+    if (a.mode == LEG_PRIME) {
+      // Fourier rows of the stored current level (restart from a saved state).
+      for (int c = 0; c < 2; ++c) {
+        eta_n[c] = Scur[0 * fstride + so + c];
+        delta_n[c] = Scur[1 * fstride + so + c];
+        Phi_n[c] = Scur[2 * fstride + so + c];
+      }
+    } else if (a.mode == LEG_INIT) {
       R acc[6] = {0, 0, 0, 0, 0, 0};
       for (int j = 0; j < J; ++j) {
         const R p = Pf[(long long)j * K + np];
@@ -245,7 +252,7 @@ __global__ void __launch_bounds__(128) k_leg(LegArgs<R> a) {
     }
     const long long ro = ((long long)j * (M + 1) + m) * 2;
     const long long fs = (long long)J * (M + 1) * 2;
-    if (a.mode == LEG_STEP) {
+    if (a.mode != LEG_INIT) {
       G[0 * fs + ro] = p[0];
       G[0 * fs + ro + 1] = p[1];
       G[1 * fs + ro] = p[2];
@@ -319,7 +326,7 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs<R> a) {
     sTc[i] = a.tw[i];
     sTs[i] = a.tw[I + i];
   }
-  const bool need_g = (a.mode == GRID_STEP) || (a.mode == GRID_INIT && a.diag_winds);
+  const bool need_g = (a.mode == GRID_STEP) || (a.mode == GRID_INIT && (a.diag_winds || a.init_from_g));
   if (need_g) {
     const R* Gb = a.G + (long long)b * kNG * J * Mp * 2;
     for (int e = tid; e < kNG * Mp * 2; e += nt) {
@@ -346,15 +353,16 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs<R> a) {
     nfwd = dft_uv ? 10 : 8;
     for (int i = tid; i < I; i += nt) {
       double eta, delta, Phi, U, V;
-      if (a.mode == GRID_INIT) {
+      const bool all_from_g = (a.mode == GRID_STEP) || a.init_from_g;
+      if (a.mode == GRID_INIT && !a.init_from_g) {
         eta = a.in_eta[(long long)b * plane + row + i];
         delta = a.in_delta[(long long)b * plane + row + i];
         Phi = a.in_Phi[(long long)b * plane + row + i];
       }
-      if (a.mode == GRID_STEP || (a.mode == GRID_INIT && a.diag_winds)) {
+      if (all_from_g || (a.mode == GRID_INIT && a.diag_winds)) {
         // Inverse DFT of the Hermitian-symmetric row: x = Re c0 + 2 sum_m Re(c_m e^{+i theta})
         // (spectral_transform.py:331-348 with the real part taken, model.py:973-977).
-        const int f0 = (a.mode == GRID_STEP) ? 0 : 3;
+        const int f0 = all_from_g ? 0 : 3;
         R acc[kNG];
 #pragma unroll
         for (int f = 0; f < kNG; ++f) acc[f] = 0;
@@ -372,7 +380,7 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs<R> a) {
         double x[kNG];
 #pragma unroll
         for (int f = 0; f < kNG; ++f) x[f] = (double)sG[(f * Mp) * 2] + 2.0 * (double)acc[f];
-        if (a.mode == GRID_STEP) {
+        if (all_from_g) {
           eta = x[0];
           delta = x[1];
           Phi = x[2];
@@ -380,7 +388,7 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs<R> a) {
         U = x[3];
         V = x[4];
       }
-      if (a.mode == GRID_INIT && !a.diag_winds) {
+      if (a.mode == GRID_INIT && !a.diag_winds && !a.init_from_g) {
         U = a.in_U[(long long)b * plane + row + i];
         V = a.in_V[(long long)b * plane + row + i];
       }

@@ -135,4 +135,96 @@ __device__ inline void assemble_slots(const double* cf, double tc, double wj, in
   }
 }
 
+// Adjoint of grid_products at one grid point (reverse mode, the Jacobian of
+// forcing.py:65-155 and initial_conditions.py:283-292 transposed).
+// pbar: cotangents of A, B, C, D, E, PhiF, F, G.  Returns the cotangents of
+// eta, Phi, U, V and accumulates d/d(taurad, taudrag, DPhieq) into dpar[3].
+// The Q < 0 clamp passes the gradient where Q >= 0 (jnp.where(Q < 0, 0, Q)).
+__device__ inline void grid_products_adjoint(const double* cf, const double* geo, const GeoLayout& gl, int j, int i,
+                                             int test, double eta, double Phi, double U, double V,
+                                             const double pbar[8], double out[4], double dpar[3]) {
+  const double Ab = pbar[0], Bb = pbar[1], Cb = pbar[2], Db = pbar[3], Eb = pbar[4];
+  const double den = geo[gl.denomE + j];
+  double etab = Ab * U + Bb * V;
+  double Phib = Cb * U + Db * V;
+  double Ub = Ab * eta + Cb * Phi + Eb * (2.0 * U / den);
+  double Vb = Bb * eta + Db * Phi + Eb * (2.0 * V / den);
+  if (test < 0) {
+    const double Phibar = cf[C_PHIBAR];
+    const double day = geo[gl.day + i];
+    const double dphi_term = (geo[gl.cosl + i]) * geo[gl.s1m + j];
+    const double Phieq = day != 0.0 ? Phibar + (cf[C_DPHIEQ] * geo[gl.cosl + i]) * geo[gl.s1m + j] : Phibar;
+    const double itr = cf[C_INVTAURAD];
+    const double Q = itr * (Phieq - (Phi + Phibar));
+    const double pass = Q < 0.0 ? 0.0 : 1.0;
+    const double Qc = Q < 0.0 ? 0.0 : Q;
+    const double tot = Phi + Phibar;
+    const double safe = fabs(tot) > 0.0 ? tot : DBL_MIN;
+    const double Fb = pbar[6], Gb = pbar[7];
+    const double td = cf[C_TAUDRAG];
+    const double inv_td = td == -1.0 ? 0.0 : 1.0 / td;
+    // F = -U Qc / tot - U / td, G = -V Qc / tot - V / td
+    Ub += Fb * (-Qc / safe - inv_td);
+    Vb += Gb * (-Qc / safe - inv_td);
+    const double Qcb = Fb * (-U / safe) + Gb * (-V / safe);
+    Phib += (Fb * U + Gb * V) * Qc / (safe * safe);
+    const double Qb = pbar[5] + Qcb * pass;
+    Phib += Qb * (-itr);
+    dpar[0] += Qb * (-Q * itr);                                   // d/dtaurad
+    if (td != -1.0) dpar[1] += (Fb * U + Gb * V) * (inv_td * inv_td);  // d/dtaudrag
+    if (day != 0.0) dpar[2] += Qb * itr * dphi_term;              // d/dDPhieq
+  }
+  out[0] = etab;
+  out[1] = Phib;
+  out[2] = Ub;
+  out[3] = Vb;
+}
+
+// Adjoint of assemble_slots for the modified-Euler scheme: slot cotangents
+// tau[9][2] (without the w_j factor applied by the caller) -> cotangents of
+// the Fourier coefficients of A, B, C, D, E, PhiF, F, G.  For a term
+// c * (i m) * Z the cotangent of Z is c * (-i m) * tau.
+__device__ inline void assemble_slots_adjoint(const double* cf, double tc, double wj, int m, int forcflag,
+                                              const double tau_in[kNT][2], double fb[8][2]) {
+  const double dt = cf[C_DT], aa = cf[C_A], Phibar = cf[C_PHIBAR];
+  const double mm = (double)m;
+  const double t1 = tc / 4.0;
+  const double t2 = (2.0 * dt / (aa * aa)) / 4.0;
+  const double te = forcflag ? tc : tc / 2.0;
+  const double fPF = forcflag ? 1.0 : 0.0;
+  double tau[kNT][2];
+  for (int k = 0; k < kNT; ++k) {
+    tau[k][0] = tau_in[k][0] * wj;
+    tau[k][1] = tau_in[k][1] * wj;
+  }
+  // J(z) = (-i m) z = (m z.im, -m z.re)
+  auto Jre = [&](int k) { return mm * tau[k][1]; };
+  auto Jim = [&](int k) { return -mm * tau[k][0]; };
+  double Cb[2], Bsb[2], PFb[2], Eb[2], Bfb[2], Asb[2], Db[2], Afb[2];
+  Cb[0] = -t1 * Jre(0) + (t1 / (2.0 * aa * aa)) * Jre(3);
+  Cb[1] = -t1 * Jim(0) + (t1 / (2.0 * aa * aa)) * Jim(3);
+  Bsb[0] = -0.5 * Phibar * t1 * Jre(0) + te * tau[8][0];
+  Bsb[1] = -0.5 * Phibar * t1 * Jim(0) + te * tau[8][1];
+  Bfb[0] = t1 * Jre(2);
+  Bfb[1] = t1 * Jim(2);
+  Asb[0] = -te * Jre(4) - 0.5 * Phibar * t1 * tau[5][0];
+  Asb[1] = -te * Jim(4) - 0.5 * Phibar * t1 * tau[5][1];
+  Afb[0] = t1 * tau[6][0];
+  Afb[1] = t1 * tau[6][1];
+  for (int c = 0; c < 2; ++c) {
+    PFb[c] = fPF * dt * tau[0][c] + fPF * ((dt / 2.0) / (aa * aa)) * tau[3][c];
+    Eb[c] = -0.5 * Phibar * t2 * tau[1][c] + t2 * tau[3][c];
+    Db[c] = t1 * tau[5][c] + (t1 / (2.0 * aa * aa)) * tau[7][c];
+    // Bs = forcflag ? B + F : B ; Bf = B + F ; As = forcflag ? A - G : A ; Af = A - G
+    fb[1][c] = Bsb[c] + Bfb[c];                         // B
+    fb[6][c] = (forcflag ? Bsb[c] : 0.0) + Bfb[c];      // F
+    fb[0][c] = Asb[c] + Afb[c];                         // A
+    fb[7][c] = -((forcflag ? Asb[c] : 0.0) + Afb[c]);   // G
+    fb[2][c] = Cb[c];
+    fb[3][c] = Db[c];
+    fb[4][c] = Eb[c];
+    fb[5][c] = PFb[c];
+  }
+}
+
 }  // namespace swx

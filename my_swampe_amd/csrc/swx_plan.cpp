@@ -17,6 +17,7 @@
 
 #include "../../include/swampe_capi.h"
 #include "geometry.h"
+#include "swx_adjoint.h"
 #include "swx_kernels.h"
 
 using namespace swx;
@@ -96,6 +97,11 @@ struct swx_plan {
   double* part_uv = nullptr;
   unsigned char* dead = nullptr;
   bool initialized = false;
+  // adjoint workspace (allocated on first use)
+  double* Lam = nullptr;
+  void* Tbar = nullptr;
+  void* Gbar = nullptr;
+  double* pgrad = nullptr;
   // captured forward graph
   hipGraphExec_t gexec = nullptr;
   int gexec_steps = 0;
@@ -327,6 +333,121 @@ int check_plan(swx_plan* p) {
   return SWX_OK;
 }
 
+// Copy between the current level of the plan's spectral state ([B][2][3][Mp][Np]
+// complex) and a compact [B][3][Mp][Np] complex array.
+int copy_state(swx_plan* p, double* dst, const double* src, bool to_plan, bool both_levels, hipStream_t s) {
+  const size_t per = (size_t)3 * (p->geo.M + 1) * (p->geo.N + 1) * 2 * sizeof(double);
+  const int B = p->d.n_members;
+  if (to_plan) {
+    for (int lvl = both_levels ? 0 : 1; lvl < 2; ++lvl)
+      SWX_HIP(hipMemcpy2DAsync((char*)p->S + lvl * per, 2 * per, src, per, per, B, hipMemcpyDeviceToDevice, s));
+  } else {
+    SWX_HIP(hipMemcpy2DAsync(dst, per, (const char*)p->S + per, 2 * per, per, B, hipMemcpyDeviceToDevice, s));
+  }
+  return SWX_OK;
+}
+
+// Fourier rows and Legendre inputs of the stored current level: invLeg then
+// the grid kernel (GRID_INIT from the rows when `init`, else GRID_STEP
+// without the physical carry).
+template <typename R>
+int prime(swx_plan* p, bool init, hipStream_t s) {
+  LegArgs<R> l = leg_args<R>(p, LEG_PRIME);
+  l.diagnostics = 0;
+  SWX_HIP(launch_leg<R>(l, s));
+  GridArgs<R> g = grid_args<R>(p, init ? GRID_INIT : GRID_STEP);
+  g.diagnostics = init ? p->d.diagnostics : 0;
+  g.init_from_g = init ? 1 : 0;
+  g.phys_on = init ? 1 : 0;
+  SWX_HIP(launch_grid<R>(g, s));
+  if (init) {
+    const size_t gbytes = (size_t)p->d.n_members * kNG * p->geo.J * (p->geo.M + 1) * 2 * p->rsz;
+    SWX_HIP(hipMemcpyAsync(p->Gprev, p->G, gbytes, hipMemcpyDeviceToDevice, s));
+  }
+  return SWX_OK;
+}
+
+int ensure_adjoint(swx_plan* p) {
+  if (p->Lam) return SWX_OK;
+  const int B = p->d.n_members, J = p->geo.J, Mp = p->geo.M + 1, Np = p->geo.N + 1;
+  int rc;
+  if ((rc = dalloc(&p->Lam, (size_t)B * 3 * Mp * Np * 2 * 8)) ||
+      (rc = dalloc(&p->Tbar, (size_t)B * Mp * kNT * J * 2 * p->rsz)) ||
+      (rc = dalloc(&p->Gbar, (size_t)B * kNG * J * Mp * 2 * p->rsz)) || (rc = dalloc(&p->pgrad, (size_t)B * J * 3 * 8)))
+    return rc;
+  return SWX_OK;
+}
+
+template <typename R>
+LegAdjArgs<R> leg_adj_args(swx_plan* p, int mode) {
+  LegAdjArgs<R> a{};
+  a.M = p->geo.M;
+  a.N = p->geo.N;
+  a.J = p->geo.J;
+  a.B = p->d.n_members;
+  a.mode = mode;
+  a.diffflag = p->d.diffflag;
+  a.Pf = (const R*)p->Pf;
+  a.Hf = (const R*)p->Hf;
+  a.Pi = (const R*)p->Pi;
+  a.Hi = (const R*)p->Hi;
+  a.coef = p->coef;
+  a.ncoef = p->cl.ncoef;
+  a.f2 = p->f2;
+  a.Lam = p->Lam;
+  a.Tbar = (R*)p->Tbar;
+  a.G = (R*)p->G;
+  a.Gbar = (const R*)p->Gbar;
+  return a;
+}
+
+template <typename R>
+GridAdjArgs<R> grid_adj_args(swx_plan* p, int mode) {
+  GridAdjArgs<R> a{};
+  a.M = p->geo.M;
+  a.N = p->geo.N;
+  a.J = p->geo.J;
+  a.I = p->geo.I;
+  a.B = p->d.n_members;
+  a.mode = mode;
+  a.forcflag = p->d.forcflag;
+  a.test = p->d.test;
+  a.tw = (const R*)p->tw;
+  a.geo = p->geo_d;
+  a.coef = p->coef;
+  a.ncoef = p->cl.ncoef;
+  a.G = (const R*)p->G;
+  a.Tbar = (const R*)p->Tbar;
+  a.Gbar = (R*)p->Gbar;
+  a.uvic = p->uvic;
+  a.pgrad = p->pgrad;
+  return a;
+}
+
+template <typename R>
+int adjoint_terminal(swx_plan* p, const double* ge, const double* gd, const double* gp, const double* gu,
+                     const double* gv, hipStream_t s) {
+  GridAdjArgs<R> g = grid_adj_args<R>(p, GADJ_TERM);
+  g.g_eta = ge;
+  g.g_delta = gd;
+  g.g_Phi = gp;
+  g.g_U = gu;
+  g.g_V = gv;
+  SWX_HIP(launch_grid_adj<R>(g, s));
+  SWX_HIP(launch_leg_adj<R>(leg_adj_args<R>(p, LADJ_TERM), s));
+  return SWX_OK;
+}
+
+template <typename R>
+int adjoint_step(swx_plan* p, const double* St, hipStream_t s) {
+  LegAdjArgs<R> a = leg_adj_args<R>(p, LADJ_A);
+  a.St = St;
+  SWX_HIP(launch_leg_adj<R>(a, s));
+  SWX_HIP(launch_grid_adj<R>(grid_adj_args<R>(p, GADJ_STEP), s));
+  SWX_HIP(launch_leg_adj<R>(leg_adj_args<R>(p, LADJ_B), s));
+  return SWX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -495,7 +616,7 @@ int swx_plan_set_params(swx_plan* p, const double* params, void* stream) {
 void swx_plan_destroy(swx_plan* p) {
   if (!p) return;
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
-  void* bufs[] = {p->Pf, p->Hf, p->Pi, p->Hi, p->tw, p->TinvT, p->TfwdT, p->geo_d, p->f2, p->coef, p->S, p->T, p->G, p->Gprev,
+  void* bufs[] = {p->Lam, p->Tbar, p->Gbar, p->pgrad, p->Pf, p->Hf, p->Pi, p->Hi, p->tw, p->TinvT, p->TfwdT, p->geo_d, p->f2, p->coef, p->S, p->T, p->G, p->Gprev,
                   p->raw, p->lag, p->g0uv, p->phys, p->uvic, p->part_uv, p->dead};
   for (void* q : bufs)
     if (q) (void)hipFree(q);
@@ -652,6 +773,81 @@ int swx_time_kernels(swx_plan* p, int t0, int nsteps, float* ms_leg, float* ms_g
   for (auto& e : ev) (void)hipEventDestroy(e);
   *ms_leg = (float)(tl / nsteps);
   *ms_grid = (float)(tg / nsteps);
+  return SWX_OK;
+}
+
+int swx_init_spectral(swx_plan* p, const double* c, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!c) return fail(SWX_ERR_ARG, "null spectral state");
+  hipStream_t s = (hipStream_t)stream;
+  const int B = p->d.n_members;
+  SWX_HIP(hipMemsetAsync(p->dead, 0, (size_t)B, s));
+  if ((rc = copy_state(p, nullptr, c, true, true, s))) return rc;
+  rc = p->rsz == 8 ? prime<double>(p, true, s) : prime<float>(p, true, s);
+  if (rc) return rc;
+  p->initialized = true;
+  return SWX_OK;
+}
+
+int swx_state_get(swx_plan* p, double* out, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!out) return fail(SWX_ERR_ARG, "null output");
+  if (!p->initialized) return fail(SWX_ERR_STATE, "swx_state_get before initialization");
+  return copy_state(p, out, nullptr, false, false, (hipStream_t)stream);
+}
+
+int swx_state_set(swx_plan* p, const double* in, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!in) return fail(SWX_ERR_ARG, "null state");
+  if (!p->initialized) return fail(SWX_ERR_STATE, "swx_state_set before initialization");
+  hipStream_t s = (hipStream_t)stream;
+  if ((rc = copy_state(p, nullptr, in, true, true, s))) return rc;
+  return p->rsz == 8 ? prime<double>(p, false, s) : prime<float>(p, false, s);
+}
+
+int swx_adjoint_begin(swx_plan* p, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (p->d.expflag) return fail(SWX_ERR_ARG, "the adjoint covers the modified-Euler scheme (expflag = 0)");
+  if ((rc = ensure_adjoint(p))) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const int B = p->d.n_members, J = p->geo.J, Mp = p->geo.M + 1, Np = p->geo.N + 1;
+  SWX_HIP(hipMemsetAsync(p->Lam, 0, (size_t)B * 3 * Mp * Np * 2 * 8, s));
+  SWX_HIP(hipMemsetAsync(p->pgrad, 0, (size_t)B * J * 3 * 8, s));
+  return SWX_OK;
+}
+
+int swx_adjoint_terminal(swx_plan* p, const double* g_eta, const double* g_delta, const double* g_Phi,
+                         const double* g_U, const double* g_V, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!p->Lam) return fail(SWX_ERR_STATE, "swx_adjoint_terminal before swx_adjoint_begin");
+  hipStream_t s = (hipStream_t)stream;
+  return p->rsz == 8 ? adjoint_terminal<double>(p, g_eta, g_delta, g_Phi, g_U, g_V, s)
+                     : adjoint_terminal<float>(p, g_eta, g_delta, g_Phi, g_U, g_V, s);
+}
+
+int swx_adjoint_step(swx_plan* p, const double* state_t, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!p->Lam) return fail(SWX_ERR_STATE, "swx_adjoint_step before swx_adjoint_begin");
+  if (!state_t) return fail(SWX_ERR_ARG, "null state");
+  hipStream_t s = (hipStream_t)stream;
+  return p->rsz == 8 ? adjoint_step<double>(p, state_t, s) : adjoint_step<float>(p, state_t, s);
+}
+
+int swx_adjoint_result(swx_plan* p, double* lam_out, double* pgrad_out, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!p->Lam) return fail(SWX_ERR_STATE, "swx_adjoint_result before swx_adjoint_begin");
+  hipStream_t s = (hipStream_t)stream;
+  const int B = p->d.n_members, J = p->geo.J, Mp = p->geo.M + 1, Np = p->geo.N + 1;
+  if (lam_out)
+    SWX_HIP(hipMemcpyAsync(lam_out, p->Lam, (size_t)B * 3 * Mp * Np * 2 * 8, hipMemcpyDeviceToDevice, s));
+  if (pgrad_out) SWX_HIP(launch_pgrad_reduce(p->pgrad, pgrad_out, B, J, s));
   return SWX_OK;
 }
 

@@ -133,6 +133,58 @@ class Plan:
         _capi.check(self._lib.swx_run_graph(self._h, int(t0), int(nsteps), int(graph_steps),
                                             self._stream(stream)), "swx_run_graph")
 
+    # -- spectral states, checkpoints and the adjoint ------------------------
+    def _spec(self, c, name="state"):
+        torch = _torch()
+        t = torch.as_tensor(c, dtype=torch.complex128, device=self.device)
+        Mp = self.M + 1
+        if t.dim() == 3:
+            t = t.expand(self.B, -1, -1, -1)
+        if tuple(t.shape) != (self.B, 3, Mp, Mp):
+            raise ValueError(f"{name} must have shape {(self.B, 3, Mp, Mp)}, got {tuple(t.shape)}")
+        return t.contiguous()
+
+    def init_spectral(self, c, stream=None) -> None:
+        """Start from spectral coefficients c (B, 3, M+1, N+1) of eta, delta, Phi."""
+        t = self._spec(c, "c")
+        self._keep = (t,)
+        _capi.check(self._lib.swx_init_spectral(self._h, _ptr(t), self._stream(stream)), "swx_init_spectral")
+
+    def state_get(self, stream=None):
+        torch = _torch()
+        out = torch.empty((self.B, 3, self.M + 1, self.M + 1), dtype=torch.complex128, device=self.device)
+        _capi.check(self._lib.swx_state_get(self._h, _ptr(out), self._stream(stream)), "swx_state_get")
+        return out
+
+    def state_set(self, S, stream=None) -> None:
+        t = self._spec(S)
+        self._keep = (t,)
+        _capi.check(self._lib.swx_state_set(self._h, _ptr(t), self._stream(stream)), "swx_state_set")
+
+    def adjoint_begin(self, stream=None) -> None:
+        _capi.check(self._lib.swx_adjoint_begin(self._h, self._stream(stream)), "swx_adjoint_begin")
+
+    def adjoint_terminal(self, g_eta=None, g_delta=None, g_Phi=None, g_U=None, g_V=None, stream=None) -> None:
+        fields = [self._field(g, n) for g, n in ((g_eta, "g_eta"), (g_delta, "g_delta"), (g_Phi, "g_Phi"),
+                                                  (g_U, "g_U"), (g_V, "g_V"))]
+        self._keep_adj = fields
+        _capi.check(self._lib.swx_adjoint_terminal(self._h, *[_ptr(f) for f in fields], self._stream(stream)),
+                    "swx_adjoint_terminal")
+
+    def adjoint_step(self, state_t, stream=None) -> None:
+        t = self._spec(state_t)
+        self._keep_adj_state = t
+        _capi.check(self._lib.swx_adjoint_step(self._h, _ptr(t), self._stream(stream)), "swx_adjoint_step")
+
+    def adjoint_result(self, stream=None):
+        """(dL/dS_0 as complex128 (B, 3, M+1, N+1), dL/d(taurad, taudrag, DPhieq) as float64 (B, 3))."""
+        torch = _torch()
+        lam = torch.empty((self.B, 3, self.M + 1, self.M + 1), dtype=torch.complex128, device=self.device)
+        pg = torch.empty((self.B, 3), dtype=torch.float64, device=self.device)
+        _capi.check(self._lib.swx_adjoint_result(self._h, _ptr(lam), _ptr(pg), self._stream(stream)),
+                    "swx_adjoint_result")
+        return lam, pg
+
     def time_kernels(self, t0: int, nsteps: int, stream=None):
         """Average (ms) of the Legendre and grid kernels over nsteps eager steps (HIP events)."""
         a, b = ctypes.c_float(), ctypes.c_float()

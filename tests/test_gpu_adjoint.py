@@ -1,0 +1,159 @@
+"""Hand-written adjoint (GPU, FP64) vs central finite differences of the HIP forward and of the oracle.
+
+Bar (north_star): gradients within 1e-6 relative in FP64.  Configuration: T42, two
+members with different forcing parameters, 6 forced modified-Euler steps,
+loss = mean((Phi_T - Phi_target)^2) per member, summed.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import swampe_oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+CFG = dict(M=42, nsteps=6, dt=600.0, a=6.37122e6, omega=7.292e-5, Phibar=3.0e5)
+TAURAD = np.array([86400.0, 5.0e4])
+TAUDRAG = np.array([86400.0, 2.0e5])
+DPHIEQ = np.array([4.0e6, 3.0e6])
+
+
+def _ic(M=42, seed=5):
+    """Spectral eta/delta/Phi: solid-body Coriolis vorticity plus low-degree noise (m=0 real)."""
+    rng = np.random.default_rng(seed)
+    c = np.zeros((3, M + 1, M + 1), complex)
+    c[0, 0, 1] = 2 * CFG["omega"] / np.sqrt(0.375) * 0.5  # eta: partially balanced vorticity
+    for f, amp in ((0, 2e-6), (1, 5e-7), (2, 5e2)):
+        for m in range(0, 9):
+            for n in range(m, 9):
+                c[f, m, n] += amp * (rng.standard_normal() + (1j * rng.standard_normal() if m else 0))
+    return c
+
+
+def _target(J=64, I=128):
+    rng = np.random.default_rng(11)
+    return 2e3 * rng.standard_normal((J, I))
+
+
+def _loss(phi, tgt):
+    return ((phi - tgt) ** 2).mean(dim=(1, 2)).sum()
+
+
+def _forward(torch, c, tr, td, dp, **kw):
+    from my_swampe_amd.autograd import rollout_phi
+
+    with torch.no_grad():
+        return rollout_phi(c, tr, td, dp, **CFG, **kw)
+
+
+@pytest.fixture(scope="module")
+def grads():
+    import torch
+
+    from my_swampe_amd.autograd import rollout_phi
+
+    print("[gpu-case] adjoint grads", flush=True)
+    dev = torch.device("cuda")
+    c = torch.tensor(np.stack([_ic(seed=5), _ic(seed=6)]), device=dev, requires_grad=True)
+    tr = torch.tensor(TAURAD, device=dev, requires_grad=True)
+    td = torch.tensor(TAUDRAG, device=dev, requires_grad=True)
+    dp = torch.tensor(DPHIEQ, device=dev, requires_grad=True)
+    tgt = torch.tensor(_target(), device=dev)
+    stats = {}
+    phi = rollout_phi(c, tr, td, dp, **CFG, stats=stats)
+    L = _loss(phi, tgt)
+    L.backward()
+    return dict(c=c.detach(), tgt=tgt, L=float(L), gc=c.grad.detach(), gtr=tr.grad.detach().cpu().numpy(),
+                gtd=td.grad.detach().cpu().numpy(), gdp=dp.grad.detach().cpu().numpy(), stats=stats)
+
+
+def test_parameter_gradients_vs_hip_central_differences(grads):
+    import torch
+
+    dev = torch.device("cuda")
+    base = {"tr": TAURAD, "td": TAUDRAG, "dp": DPHIEQ}
+    got = {"tr": grads["gtr"], "td": grads["gtd"], "dp": grads["gdp"]}
+    for name in ("tr", "td", "dp"):
+        for b in range(2):
+            h = 1e-6 * abs(base[name][b])
+            vals = []
+            for sgn in (1, -1):
+                p = {k: v.copy() for k, v in base.items()}
+                p[name][b] += sgn * h
+                phi = _forward(torch, grads["c"], *(torch.tensor(p[k], device=dev) for k in ("tr", "td", "dp")))
+                vals.append(float(_loss(phi, grads["tgt"])))
+            fd = (vals[0] - vals[1]) / (2 * h)
+            rel = abs(got[name][b] - fd) / max(abs(fd), 1e-300)
+            print(name, b, "adjoint", got[name][b], "fd", fd, "rel", rel)
+            assert rel < 1e-6, (name, b, got[name][b], fd, rel)
+
+
+def test_initial_state_gradient_vs_directional_difference(grads):
+    import torch
+
+    dev = torch.device("cuda")
+    rng = np.random.default_rng(21)
+    for trial in range(3):
+        v = np.zeros((2, 3, 43, 43), complex)
+        for f, amp in ((0, 1e-6), (1, 1e-7), (2, 1e2)):
+            v[:, f] = amp * (rng.standard_normal((2, 43, 43)) + 1j * rng.standard_normal((2, 43, 43)))
+        v[:, :, 0, :] = v[:, :, 0, :].real
+        v *= np.triu(np.ones((43, 43)))[None, None]
+        vt = torch.tensor(v, device=dev)
+        h = 1e-5
+        tr, td, dp = (torch.tensor(x, device=dev) for x in (TAURAD, TAUDRAG, DPHIEQ))
+        Lp = float(_loss(_forward(torch, grads["c"] + h * vt, tr, td, dp), grads["tgt"]))
+        Lm = float(_loss(_forward(torch, grads["c"] - h * vt, tr, td, dp), grads["tgt"]))
+        fd = (Lp - Lm) / (2 * h)
+        g = grads["gc"]
+        ad = float((g.real * vt.real + g.imag * vt.imag).sum())
+        rel = abs(ad - fd) / abs(fd)
+        print("direction", trial, "adjoint", ad, "fd", fd, "rel", rel)
+        assert rel < 1e-6, (trial, ad, fd, rel)
+
+
+def test_parameter_gradients_vs_oracle_central_differences(grads):
+    """Same gradients against finite differences of the FP64 NumPy oracle run from the grid fields of c."""
+    N, I, J, lam, mus, w, P, H = orc.cached_geometry(42)
+    c = grads["c"].cpu().numpy()
+    tgt = grads["tgt"].cpu().numpy()
+
+    def oracle_loss(b, tr, td, dp):
+        fields = [np.real(orc.invrs_fft(orc.invrs_leg(c[b, f], I, J, 42, 42, P), I)) for f in range(3)]
+        out = orc.run_model_scan(M=42, dt=CFG["dt"], tmax=2 + CFG["nsteps"], Phibar=CFG["Phibar"],
+                                 omega=CFG["omega"], a=CFG["a"], taurad=tr, taudrag=td, DPhieq=dp,
+                                 eta0_init=fields[0], delta0_init=fields[1], Phi0_init=fields[2],
+                                 diagnostics=False, return_history=False)
+        return float(np.mean((out["last_state"].Phi_curr - tgt) ** 2))
+
+    got = {"tr": grads["gtr"], "td": grads["gtd"], "dp": grads["gdp"]}
+    for b in range(2):
+        base = {"tr": TAURAD[b], "td": TAUDRAG[b], "dp": DPHIEQ[b]}
+        for name in ("tr", "td", "dp"):
+            h = 1e-6 * abs(base[name])
+            lp = oracle_loss(b, **{k: v + (h if k == name else 0.0) for k, v in base.items()})
+            lm = oracle_loss(b, **{k: v - (h if k == name else 0.0) for k, v in base.items()})
+            fd = (lp - lm) / (2 * h)
+            rel = abs(got[name][b] - fd) / max(abs(fd), 1e-300)
+            print("oracle", name, b, got[name][b], fd, rel)
+            assert rel < 1e-6, (name, b, got[name][b], fd, rel)
+
+
+def test_checkpoint_interval_does_not_change_gradients(grads):
+    import torch
+
+    from my_swampe_amd.autograd import rollout_phi
+
+    dev = torch.device("cuda")
+    for every in (1, 4, 6):
+        c = grads["c"].clone().requires_grad_(True)
+        tr = torch.tensor(TAURAD, device=dev, requires_grad=True)
+        td = torch.tensor(TAUDRAG, device=dev)
+        dp = torch.tensor(DPHIEQ, device=dev)
+        L = _loss(rollout_phi(c, tr, td, dp, **CFG, checkpoint_every=every), grads["tgt"])
+        L.backward()
+        np.testing.assert_array_equal(c.grad.cpu().numpy(), grads["gc"].cpu().numpy())
+        np.testing.assert_array_equal(tr.grad.cpu().numpy(), grads["gtr"])
+    assert grads["stats"]["checkpoints"] == 3 and grads["stats"]["segment_states"] == 3
