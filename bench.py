@@ -93,6 +93,47 @@ def run_batch(M, B, precision, steps, warmup, graph_steps=32, full_phys=False):
     return dict(wall=wall, ms_leg=ms_leg, ms_grid=ms_grid, finite=finite)
 
 
+def spectral_ics(M: int, B: int, omega: float = 3.2e-5):
+    """Spectral (eta, delta, Phi) of the rest + noise state: Coriolis vorticity f_mn plus seeded Phi noise."""
+    rng = np.random.default_rng(1000)
+    c = np.zeros((B, 3, M + 1, M + 1), complex)
+    c[:, 0, 0, 1] = omega / np.sqrt(0.375)
+    for m in range(11):
+        for n in range(m, 11):
+            c[:, 2, m, n] = 30.0 * (rng.standard_normal(B) + (1j * rng.standard_normal(B) if m else 0))
+    return c
+
+
+def run_fwd_adjoint(M, B, precision, steps, reps=2):
+    """Wall time of forward + hand-written adjoint (sqrt(N) checkpointing) through the autograd rollout."""
+    import torch
+    from my_swampe_amd.autograd import rollout_phi
+
+    dev = torch.device("cuda")
+    params = member_params(B)
+    c0 = torch.tensor(spectral_ics(M, B), device=dev)
+    tr0 = torch.tensor(params[:, 5], device=dev)
+    td0 = torch.tensor(params[:, 6], device=dev)
+    dp0 = torch.tensor(params[:, 7], device=dev)
+    J, I = (64, 128) if M == 42 else ((96, 192) if M == 63 else (160, 320))
+    tgt = torch.zeros((J, I), dtype=torch.float64, device=dev)
+    times = []
+    for r in range(reps + 1):
+        c = c0.clone().requires_grad_(True)
+        tr, td, dp = (x.clone().requires_grad_(True) for x in (tr0, td0, dp0))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        phi = rollout_phi(c, tr, td, dp, M=M, nsteps=steps, dt=120.0, a=8.2e7, omega=3.2e-5, Phibar=3e5,
+                          precision=precision)
+        loss = ((phi - tgt) ** 2).mean(dim=(1, 2)).sum()
+        loss.backward()
+        torch.cuda.synchronize()
+        if r > 0:  # first repetition warms up plan creation and caches
+            times.append(time.perf_counter() - t0)
+    finite = bool(torch.isfinite(c.grad).all()) and bool(torch.isfinite(tr.grad).all())
+    return dict(wall=float(np.median(times)), finite=finite)
+
+
 def cpu_baseline(seconds_target: float = 12.0):
     """FP64 oracle (numpy, one core) on config 0: T42, 1 member, forced modEuler, dt=120."""
     from oracle import swampe_oracle as orc
@@ -121,6 +162,8 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=("fp32", "fp64"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; members shard across GPUs with no per-step "
@@ -175,6 +218,18 @@ def main():
             "ratio_vs_fp64_same_batch": value / fp64_v,
             "fp64_batched_kernel_ms": {"leg": fp64["ms_leg"], "grid": fp64["ms_grid"]},
             "plain_kernel_ms": {"leg": plain["ms_leg"], "grid": plain["ms_grid"]},
+        }
+    if not args.no_adjoint:
+        na = args.adjoint_steps
+        fa = run_fwd_adjoint(M, B, args.precision, na)
+        pa = run_fwd_adjoint(M, 1, "fp64", na)
+        fa_v = na * B / fa["wall"]
+        pa_v = na / pa["wall"]
+        out["fwd_adjoint"] = {
+            "steps": na, "members": B, "checkpoint_every": int(np.ceil(np.sqrt(na))),
+            "wall_s": fa["wall"], "member_steps_per_s": fa_v, "finite": fa["finite"],
+            "plain_fp64_1member_wall_s": pa["wall"], "plain_fp64_1member_member_steps_per_s": pa_v,
+            "ratio_vs_plain_fp64_1member": fa_v / pa_v,
         }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()

@@ -108,8 +108,8 @@ __global__ void __launch_bounds__(128) k_leg_adj(LegAdjArgs<R> a) {
     const R* Pi = a.Pi + boff;
     const R* Hi = a.Hi + boff;
     R* Tb = a.Tbar + (((long long)b * Mp + m) * kNT) * J * 2;
-    R* G = a.G + (long long)b * kNG * J * Mp * 2;
-    const long long fs = (long long)J * Mp * 2;
+    R* G = a.G + fidx(b, 0, m, 0, Mp, J);
+    const long long fs = (long long)Mp * J * 2;  // field stride of the m-major rows
     const R mr = (R)m;
     for (int j = tid; j < J; j += nt) {
       R tp[2 * kNT];
@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(128) k_leg_adj(LegAdjArgs<R> a) {
         Tb[(k * J + j) * 2] = tp[2 * k];
         Tb[(k * J + j) * 2 + 1] = tp[2 * k + 1];
       }
-      const long long ro = ((long long)j * Mp + m) * 2;
+      const long long ro = (long long)j * 2;
       G[0 * fs + ro] = p[0];
       G[0 * fs + ro + 1] = p[1];
       G[1 * fs + ro] = p[2];
@@ -165,11 +165,9 @@ __global__ void __launch_bounds__(128) k_leg_adj(LegAdjArgs<R> a) {
 
   // Modes B and TERM: lam (+)= invLeg^T(Gbar) with the wind-inversion adjoint.
   R* sGb = reinterpret_cast<R*>(smem_raw);  // [5][J][2]
-  const R* Gb = a.Gbar + (long long)b * kNG * J * Mp * 2;
   for (int e = tid; e < kNG * J * 2; e += nt) {
     const int f = e / (J * 2), r = e % (J * 2);
-    const int j = r / 2, c = r % 2;
-    sGb[e] = Gb[((long long)f * J + j) * Mp * 2 + m * 2 + c];
+    sGb[e] = a.Gbar[fidx(b, f, m, 0, Mp, J) + r];
   }
   __syncthreads();
   const R* Pf = a.Pf + boff;
@@ -240,10 +238,9 @@ __global__ void __launch_bounds__(256) k_grid_adj(GridAdjArgs<R> a) {
     sTs[i] = a.tw[I + i];
   }
   if (a.mode == GADJ_STEP) {
-    const R* Gr = a.G + (long long)b * kNG * J * Mp * 2;
     for (int e = tid; e < kNG * Mp * 2; e += nt) {
       const int f = e / (Mp * 2), r = e % (Mp * 2);
-      sG[e] = Gr[((long long)f * J + j) * Mp * 2 + r];
+      sG[e] = a.G[fidx(b, f, r >> 1, j, Mp, J) + (r & 1)];
     }
     // assemble^T for every m of this row.
     const double wj = geo[gl.w + j];
@@ -320,7 +317,6 @@ __global__ void __launch_bounds__(256) k_grid_adj(GridAdjArgs<R> a) {
   __syncthreads();
 
   // invDFT^T: cbar_m = w_m sum_i xbar_i (cos theta, -sin theta), w_0 = 1, w_m = 2.
-  R* Gb = a.Gbar + (long long)b * kNG * J * Mp * 2;
   for (int p = tid; p < kNG * Mp; p += nt) {
     const int f = p / Mp, m = p % Mp;
     R re = 0, im = 0;
@@ -334,8 +330,8 @@ __global__ void __launch_bounds__(256) k_grid_adj(GridAdjArgs<R> a) {
       if (idx >= I) idx -= I;
     }
     const R wm = m == 0 ? (R)1 : (R)2;
-    Gb[((long long)f * J + j) * Mp * 2 + m * 2] = wm * re;
-    Gb[((long long)f * J + j) * Mp * 2 + m * 2 + 1] = m == 0 ? (R)0 : wm * im;
+    a.Gbar[fidx(b, f, m, j, Mp, J)] = wm * re;
+    a.Gbar[fidx(b, f, m, j, Mp, J) + 1] = m == 0 ? (R)0 : wm * im;
   }
 }
 

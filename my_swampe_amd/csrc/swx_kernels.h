@@ -68,6 +68,13 @@ constexpr int kNG = 5;
 // Raw forward-DFT products per row: A, B, C, D, E, PhiF, F, G.
 constexpr int kNRaw = 8;
 
+// Fourier rows G / Gprev / Gbar are stored m-major: [member][field][m][j]
+// complex, so a Legendre block (one m) reads and writes contiguous j.  Offset
+// in reals of (b, f, m, j):
+__host__ __device__ inline long long fidx(int b, int f, int m, int j, int Mp, int J) {
+  return ((((long long)b * kNG + f) * Mp + m) * J + j) * 2;
+}
+
 // Offset (in elements of one J-row block) of the triangular basis block of m:
 // the block holds J * (N + 1 - m) values.
 __host__ __device__ inline long long tri_off(int m, int N, int J) {
@@ -142,5 +149,8 @@ int grid_mfma_k2p(int M);
 template <typename R>
 hipError_t launch_copy_real(double* dst, long long dstride, const R* src, long long sstride, long long n, int B,
                             hipStream_t s);
+// Field f of an m-major Fourier-row buffer -> complex128 [B][J][M+1].
+template <typename R>
+hipError_t launch_copy_fourier(double* dst, const R* src, int f, int B, int J, int Mp, hipStream_t s);
 
 }  // namespace swx

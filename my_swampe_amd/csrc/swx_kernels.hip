@@ -227,7 +227,8 @@ __global__ void __launch_bounds__(128) k_leg(LegArgs<R> a) {
 
   const R* Pi = a.Pi + boff;
   const R* Hi = a.Hi + boff;
-  R* G = a.G + (long long)b * kNG * J * (M + 1) * 2;
+  R* G = a.G + fidx(b, 0, m, 0, M + 1, J);
+  const long long fs = (long long)(M + 1) * J * 2;  // field stride of the m-major rows
   const R mr = (R)m;
   for (int j = tid; j < J; j += nt) {
     R p[10];
@@ -250,8 +251,7 @@ __global__ void __launch_bounds__(128) k_leg(LegArgs<R> a) {
       h[2] += hv * sX[(3 * (N + 1) + np) * 2];
       h[3] += hv * sX[(3 * (N + 1) + np) * 2 + 1];
     }
-    const long long ro = ((long long)j * (M + 1) + m) * 2;
-    const long long fs = (long long)J * (M + 1) * 2;
+    const long long ro = (long long)j * 2;
     if (a.mode != LEG_INIT) {
       G[0 * fs + ro] = p[0];
       G[0 * fs + ro + 1] = p[1];
@@ -328,10 +328,9 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs<R> a) {
   }
   const bool need_g = (a.mode == GRID_STEP) || (a.mode == GRID_INIT && (a.diag_winds || a.init_from_g));
   if (need_g) {
-    const R* Gb = a.G + (long long)b * kNG * J * Mp * 2;
     for (int e = tid; e < kNG * Mp * 2; e += nt) {
       const int f = e / (Mp * 2), r = e % (Mp * 2);
-      sG[e] = Gb[((long long)f * J + j) * Mp * 2 + r];
+      sG[e] = a.G[fidx(b, f, r >> 1, j, Mp, J) + (r & 1)];
     }
   }
   __syncthreads();
@@ -495,12 +494,11 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs<R> a) {
   if (a.mode == GRID_DFT) {
     // Raw Fourier of the initial fields and the w-weighted Legendre inputs
     // of the spectral state initialization.
-    R* Gb = a.Gout + (long long)b * kNG * J * Mp * 2;
     const double wj = geo[gl.w + j];
     for (int e = tid; e < nfwd * Mp; e += nt) {
       const int f = e / Mp, m = e % Mp;
-      Gb[((long long)f * J + j) * Mp * 2 + m * 2] = sF[(f * Mp + m) * 2];
-      Gb[((long long)f * J + j) * Mp * 2 + m * 2 + 1] = sF[(f * Mp + m) * 2 + 1];
+      a.Gout[fidx(b, f, m, j, Mp, J)] = sF[(f * Mp + m) * 2];
+      a.Gout[fidx(b, f, m, j, Mp, J) + 1] = sF[(f * Mp + m) * 2 + 1];
       if (f < 3) {
         R* Tm = a.T + (((long long)b * Mp + m) * kNT + f) * J * 2;
         Tm[j * 2] = (R)((double)sF[(f * Mp + m) * 2] * wj);
@@ -521,14 +519,13 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs<R> a) {
   if (a.mode == GRID_INIT) {
     // Um0/Vm0 (model.py:727-729), the pinned copy for test 1, and the lagged
     // momentum-forcing remainder Fm + Um/taudrag (model.py:775-792).
-    R* Gb = a.Gout + (long long)b * kNG * J * Mp * 2;
     R* g0 = a.g0uv + (long long)b * 2 * J * Mp * 2;
     R* lg = a.lag + (long long)b * 2 * J * Mp * 2;
     const double itd = cf[C_INVTD];
     for (int e = tid; e < 2 * Mp * 2; e += nt) {
       const int f = e / (Mp * 2), r = e % (Mp * 2);
       const R v = sF[(8 + f) * Mp * 2 + r];
-      Gb[((long long)(3 + f) * J + j) * Mp * 2 + r] = v;
+      a.Gout[fidx(b, 3 + f, r >> 1, j, Mp, J) + (r & 1)] = v;
       g0[((long long)f * J + j) * Mp * 2 + r] = v;
       lg[((long long)f * J + j) * Mp * 2 + r] = (R)((double)sF[(6 + f) * Mp * 2 + r] + (double)v * itd);
     }
@@ -569,6 +566,16 @@ __global__ void k_copy_real(double* dst, long long dstride, const R* src, long l
     dst[b * dstride + e] = (double)src[b * sstride + e];
 }
 
+template <typename R>
+__global__ void k_copy_fourier(double* dst, const R* src, int f, int J, int Mp) {
+  const int b = blockIdx.y;
+  const long long n = (long long)J * Mp * 2;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(e & 1), m = (int)((e >> 1) % Mp), j = (int)((e >> 1) / Mp);
+    dst[(long long)b * n + e] = (double)src[fidx(b, f, m, j, Mp, J) + c];
+  }
+}
+
 // ----------------------------------------------------------------------------
 // Launchers
 // ----------------------------------------------------------------------------
@@ -600,6 +607,16 @@ hipError_t launch_copy_real(double* dst, long long dstride, const R* src, long l
   return hipGetLastError();
 }
 
+template <typename R>
+hipError_t launch_copy_fourier(double* dst, const R* src, int f, int B, int J, int Mp, hipStream_t s) {
+  int gx = (int)(((long long)J * Mp * 2 + 255) / 256);
+  if (gx > 1024) gx = 1024;
+  hipLaunchKernelGGL(k_copy_fourier<R>, dim3(gx, B), dim3(256), 0, s, dst, src, f, J, Mp);
+  return hipGetLastError();
+}
+
+template hipError_t launch_copy_fourier<double>(double*, const double*, int, int, int, int, hipStream_t);
+template hipError_t launch_copy_fourier<float>(double*, const float*, int, int, int, int, hipStream_t);
 template hipError_t launch_leg<double>(const LegArgs<double>&, hipStream_t);
 template hipError_t launch_leg<float>(const LegArgs<float>&, hipStream_t);
 template hipError_t launch_grid<double>(const GridArgs<double>&, hipStream_t);

@@ -69,15 +69,12 @@ __global__ void __launch_bounds__(256) k_grid_mfma(GridArgs<float> a, const floa
   double* phys = a.phys + (long long)b * 8 * plane;
   const double alpha = cf[C_ALPHA];
 
-  // Phase 0: Fourier rows -> X^T (k-major, zero padded).
-  {
-    const float* Gb = a.G + (long long)b * kNG * J * Mp * 2;
-    for (int e = tid; e < 5 * 16 * K1p; e += 256) {
-      const int f = e / (16 * K1p), r = e % (16 * K1p);
-      const int n = r / K1p, k = r % K1p;
-      const float v = k < K1 ? Gb[((long long)f * J + j0 + n) * (Mp * 2) + k] : 0.0f;
-      sXT[(f * K1p + k) * 16 + n] = v;
-    }
+  // Phase 0: Fourier rows (m-major, 16 consecutive j per (f, m)) -> X^T (k-major, zero padded).
+  for (int e = tid; e < 5 * (K1p / 2) * 32; e += 256) {
+    const int c = e & 1, n = (e >> 1) & 15, fm = e >> 5;
+    const int f = fm / (K1p / 2), m = fm % (K1p / 2);
+    const float v = m < Mp ? a.G[fidx(b, f, m, j0 + n, Mp, J) + c] : 0.0f;
+    sXT[(f * K1p + 2 * m + c) * 16 + n] = v;
   }
   __syncthreads();
 

@@ -876,12 +876,22 @@ int swx_export_state(swx_plan* p, const swx_state_out* o, void* stream) {
     if (e != hipSuccess) return fail(SWX_ERR_HIP, std::string("export copy: ") + hipGetErrorString(e));
     return SWX_OK;
   };
+  auto cf = [&](double* dst, const void* base, int f) -> int {
+    if (!dst) return SWX_OK;
+    hipError_t e;
+    if (p->rsz == 8)
+      e = launch_copy_fourier<double>(dst, (const double*)base, f, B, J, Mp, s);
+    else
+      e = launch_copy_fourier<float>(dst, (const float*)base, f, B, J, Mp, s);
+    if (e != hipSuccess) return fail(SWX_ERR_HIP, std::string("export copy: ") + hipGetErrorString(e));
+    return SWX_OK;
+  };
   const bool pinned = p->d.test == 1;
-  if ((rc = cp(o->etam_curr, p->G, kNG, 0, cjm)) || (rc = cp(o->deltam_curr, p->G, kNG, 1, cjm)) ||
-      (rc = cp(o->Phim_curr, p->G, kNG, 2, cjm)) || (rc = cp(o->etam_prev, p->Gprev, kNG, 0, cjm)) ||
-      (rc = cp(o->deltam_prev, p->Gprev, kNG, 1, cjm)) || (rc = cp(o->Phim_prev, p->Gprev, kNG, 2, cjm)) ||
-      (rc = cp(o->Um_curr, pinned ? p->g0uv : p->G, pinned ? 2 : kNG, pinned ? 0 : 3, cjm)) ||
-      (rc = cp(o->Vm_curr, pinned ? p->g0uv : p->G, pinned ? 2 : kNG, pinned ? 1 : 4, cjm)) ||
+  if ((rc = cf(o->etam_curr, p->G, 0)) || (rc = cf(o->deltam_curr, p->G, 1)) || (rc = cf(o->Phim_curr, p->G, 2)) ||
+      (rc = cf(o->etam_prev, p->Gprev, 0)) || (rc = cf(o->deltam_prev, p->Gprev, 1)) ||
+      (rc = cf(o->Phim_prev, p->Gprev, 2)) ||
+      (rc = pinned ? cp(o->Um_curr, p->g0uv, 2, 0, cjm) : cf(o->Um_curr, p->G, 3)) ||
+      (rc = pinned ? cp(o->Vm_curr, p->g0uv, 2, 1, cjm) : cf(o->Vm_curr, p->G, 4)) ||
       (rc = cp(o->Am_curr, p->raw, kNRaw, 0, cjm)) || (rc = cp(o->Bm_curr, p->raw, kNRaw, 1, cjm)) ||
       (rc = cp(o->Cm_curr, p->raw, kNRaw, 2, cjm)) || (rc = cp(o->Dm_curr, p->raw, kNRaw, 3, cjm)) ||
       (rc = cp(o->Em_curr, p->raw, kNRaw, 4, cjm)) || (rc = cp(o->PhiFm_curr, p->raw, kNRaw, 5, cjm)) ||
