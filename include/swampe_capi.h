@@ -191,9 +191,12 @@ int swx_adjoint_result(swx_plan *plan, double *lam_out, double *pgrad_out, void 
  *     stepping when it fits), 1 = always the plain grid kernel.
  *   SWX_OPT_FULL_PHYS_CARRY: 1 = write the physical (grid) carry at every
  *     step; 0 (default) = only in the final steps whose Robert-Asselin
- *     contribution can reach the exported state (identical results). */
+ *     contribution can reach the exported state (identical results).
+ *   SWX_OPT_LEG_KERNEL: 0 = automatic (MFMA Legendre kernel for FP32 forward
+ *     stepping), 1 = always the plain Legendre kernel. */
 #define SWX_OPT_GRID_KERNEL 1
 #define SWX_OPT_FULL_PHYS_CARRY 2
+#define SWX_OPT_LEG_KERNEL 3
 int swx_plan_set_option(swx_plan *plan, int option, int value);
 
 /* Last error message of the calling thread. */

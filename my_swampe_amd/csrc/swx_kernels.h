@@ -143,6 +143,9 @@ hipError_t launch_grid(const GridArgs<R>& a, hipStream_t s);
 // TinvT [I][k1p], TfwdT [k2p][I].
 hipError_t launch_grid_mfma(const GridArgs<float>& a, const float* TinvT, const float* TfwdT, hipStream_t s);
 size_t grid_mfma_lds_bytes(int M);
+// MFMA Legendre kernel (FP32, LEG_STEP without diagnostics).
+hipError_t launch_leg_mfma(const LegArgs<float>& a, hipStream_t s);
+size_t leg_mfma_lds_bytes(int M);
 int grid_mfma_k1p(int M);
 int grid_mfma_k2p(int M);
 

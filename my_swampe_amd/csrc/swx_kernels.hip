@@ -163,63 +163,27 @@ __global__ void __launch_bounds__(128) k_leg(LegArgs<R> a) {
           acc[2 * k + 1] += sT[(k * J + j) * 2 + 1] * h;
         }
       }
-      const double dt = cf[C_DT], aa = cf[C_A], Phibar = cf[C_PHIBAR];
-      const double nn = (double)n * (double)(n + 1);
-      // Carry terms Pw(Xm1) of the reference = f2 * S (Gauss quadrature).
-      const double f2 = a.f2[(long long)m * (N + 1) + n];
-      for (int c = 0; c < 2; ++c) {
-        const double fp0 = acc[0 + c], fp1 = acc[2 + c], fp2 = acc[4 + c], fp3 = acc[6 + c], fp4 = acc[8 + c];
-        const double fh0 = acc[10 + c], fh1 = acc[12 + c], fh2 = acc[14 + c], fh3 = acc[16 + c];
-        const double eo = f2 * Scur[0 * fstride + so + c];
-        const double d_o = f2 * Scur[1 * fstride + so + c];
-        const double po = f2 * Scur[2 * fstride + so + c];
-        if (!a.expflag) {
-          // modEuler_tdiff.py:95-103, :185-193, :261 with the carry terms in spectral space.
-          const double t2 = (2.0 * dt / (aa * aa)) / 4.0;
-          Phi_n[c] = po - dt * Phibar * d_o - Phibar * 0.5 * ((1.0 / (aa * aa)) * (nn * po)) + fp0 + nn * fp1 + fh0;
-          delta_n[c] = d_o + nn * (t2 * po) - Phibar * (nn * d_o) / (aa * aa) + fp2 + nn * fp3 + fh1 + nn * fh2;
-          eta_n[c] = eo + fp4 + fh3;
-        } else {
-          // explicit_tdiff.py:71, :138, :212 (prev-level carries).
-          const double ep = f2 * Sprev[0 * fstride + so + c];
-          const double dp = f2 * Sprev[1 * fstride + so + c];
-          const double pp = f2 * Sprev[2 * fstride + so + c];
-          Phi_n[c] = pp - 2.0 * dt * Phibar * d_o + fp0 + fh0;
-          delta_n[c] = dp + fp2 + fh1;
-          eta_n[c] = ep + fp4 + fh3;
-        }
+      double fp[5][2], fh[4][2];
+      for (int k = 0; k < 5; ++k) {
+        fp[k][0] = acc[2 * k];
+        fp[k][1] = acc[2 * k + 1];
       }
-      if (a.diffflag) {
-        const double sg = cf[cl.n_sigma + n], sp = cf[cl.n_sigmaphi + n];
-        for (int c = 0; c < 2; ++c) {
-          Phi_n[c] *= sp;
-          delta_n[c] *= sg;
-          eta_n[c] *= sg;
-        }
+      for (int k = 0; k < 4; ++k) {
+        fh[k][0] = acc[2 * (kNP + k)];
+        fh[k][1] = acc[2 * (kNP + k) + 1];
       }
-      for (int c = 0; c < 2; ++c) {
-        if (a.expflag) {
-          Sprev[0 * fstride + so + c] = Scur[0 * fstride + so + c];
-          Sprev[1 * fstride + so + c] = Scur[1 * fstride + so + c];
-          Sprev[2 * fstride + so + c] = Scur[2 * fstride + so + c];
-        }
-        Scur[0 * fstride + so + c] = eta_n[c];
-        Scur[1 * fstride + so + c] = delta_n[c];
-        Scur[2 * fstride + so + c] = Phi_n[c];
-      }
+      spectral_update(cf, cl, a.f2[(long long)m * (N + 1) + n], n, a.expflag, a.diffflag, fp, fh, Scur, Sprev, so,
+                      fstride, eta_n, delta_n, Phi_n);
     }
-    // Inputs of the inverse transform: eta, delta, Phi and the wind
-    // potentials (spectral_transform.py:577-582).
-    const double tcmn = cf[cl.n_tcmn + n];
-    const double f = (m == 0 && n == 1) ? cf[C_FMN01] : 0.0;
+    // Inputs of the inverse transform: eta, delta, Phi and the wind potentials.
+    double dh[2], eh[2];
+    wind_potentials(cf, cl, m, n, eta_n, delta_n, dh, eh);
     for (int c = 0; c < 2; ++c) {
-      const double dh = (n == 0) ? 0.0 : delta_n[c] * tcmn;
-      const double eh = (n == 0) ? 0.0 : (eta_n[c] - (c == 0 ? f : 0.0)) * tcmn;
       sX[(0 * (N + 1) + np) * 2 + c] = (R)eta_n[c];
       sX[(1 * (N + 1) + np) * 2 + c] = (R)delta_n[c];
       sX[(2 * (N + 1) + np) * 2 + c] = (R)Phi_n[c];
-      sX[(3 * (N + 1) + np) * 2 + c] = (R)dh;
-      sX[(4 * (N + 1) + np) * 2 + c] = (R)eh;
+      sX[(3 * (N + 1) + np) * 2 + c] = (R)dh[c];
+      sX[(4 * (N + 1) + np) * 2 + c] = (R)eh[c];
     }
   }
   if (a.mode == LEG_INIT && !a.diag_winds) return;

@@ -267,4 +267,218 @@ hipError_t launch_grid_mfma(const GridArgs<float>& a, const float* TinvT, const 
   }
 }
 
+
+
+// ----------------------------------------------------------------------------
+// MFMA Legendre kernel (FP32 transforms, FP64 spectral update).
+//
+// One workgroup = one zonal wavenumber m and MB members.  With K = N + 1 - m
+// retained degrees:
+//   phase F  forward contraction as GEMMs: C[n][r] = sum_j P^T[n][j] T[j][r]
+//            (P slots) and H^T for the H slots, r = (member, slot, re/im);
+//            B operands staged in LDS as [j][r]; A = basis rows read from the
+//            [n][j] layout.  (modEuler_tdiff.py fwd_leg_w calls, regrouped.)
+//   phase U  FP64 spectral update per (member, n) (swx_physics.h).
+//   phase I  inverse contraction: D[j][r'] = sum_n P[j][n] X[n][r'] for
+//            eta, delta, Phi and the two wind potentials, H[j][n] for the wind
+//            potentials (spectral_transform.py:438-596).
+//   phase W  U_m = -i m P d^ + H e^, V_m = -i m P e^ - H d^ and coalesced
+//            writes of the m-major Fourier rows.
+// Column groups are padded to 16 so that every MFMA tile is all-P or all-H.
+// ----------------------------------------------------------------------------
+
+namespace {
+constexpr int ceil16(int x) { return (x + 15) / 16 * 16; }
+constexpr int ceil4(int x) { return (x + 3) / 4 * 4; }
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+}  // namespace
+
+template <int MT, int MB>
+struct LegShape {
+  static constexpr int Np = MT + 1;
+  static constexpr int J = MT == 42 ? 64 : (MT == 63 ? 96 : 160);
+  static constexpr int RP = ceil16(10 * MB), RH = ceil16(8 * MB), RT = RP + RH;  // forward columns
+  static constexpr int XP = ceil16(10 * MB), XH = ceil16(4 * MB), XT = XP + XH;  // inverse columns
+  static constexpr int KNP = ceil16(Np);  // padded degree rows
+  static constexpr int SA = cmax(cmax(J * RT, KNP * RT), J * XT);
+  static constexpr int SX = KNP * XT;
+  static constexpr int LDS_FLOATS = SA + SX;
+};
+
+template <int MT, int MB>
+__global__ void __launch_bounds__(256) k_leg_mfma(LegArgs<float> a) {
+  using S = LegShape<MT, MB>;
+  constexpr int Np = S::Np, Mp = MT + 1, J = S::J, RP = S::RP, RT = S::RT, XP = S::XP, XT = S::XT;
+  extern __shared__ __align__(16) float lds[];
+  float* sA = lds;           // [J][RT] inputs, then [K][RT] forward results, then [J][XT] inverse results
+  float* sX = lds + S::SA;   // [KNP][XT] inverse-transform inputs
+  const int m = blockIdx.x;
+  const int b0 = blockIdx.y * MB;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, ln = lane & 15, lg = lane >> 4;
+  const int B = a.B, N = a.N;
+  const int K = Np - m;
+  const int Kt = (K + 15) / 16;
+  const int K4 = ceil4(K);
+  const long long boff = tri_off(m, N, J);
+  const float* Pi = a.Pi + boff;
+  const float* Hi = a.Hi + boff;
+  const float* Pf = a.Pf + boff;
+  const float* Hf = a.Hf + boff;
+  const CoefLayout cl = CoefLayout::make(N, J);
+
+  // Phase 0: stage the Legendre inputs [j][member, slot, re/im]; zero sX.
+  for (int e = tid; e < MB * kNT * J * 2; e += 256) {
+    const int c = e & 1, j = (e >> 1) % J, ks = (e >> 1) / J;
+    const int k = ks % kNT, mb = ks / kNT;
+    const int b = b0 + mb;
+    const int col = k < kNP ? mb * 10 + 2 * k + c : RP + mb * 8 + 2 * (k - kNP) + c;
+    sA[j * RT + col] = b < B ? a.T[(((long long)b * Mp + m) * kNT + k) * J * 2 + 2 * j + c] : 0.0f;
+  }
+  for (int e = tid; e < S::SX; e += 256) sX[e] = 0.0f;
+  constexpr int padH = RT - RP - MB * 8, padP = RP - MB * 10;
+  if constexpr (padH > 0)
+    for (int e = tid; e < J * padH; e += 256) sA[(e / padH) * RT + RP + MB * 8 + e % padH] = 0.0f;
+  if constexpr (padP > 0)
+    for (int e = tid; e < J * padP; e += 256) sA[(e / padP) * RT + MB * 10 + e % padP] = 0.0f;
+  __syncthreads();
+
+  // Phase F: forward contraction tiles (Kt row tiles x RT/16 column tiles).
+  constexpr int CT = RT / 16;
+  constexpr int MAXT = (ceil16(Np) / 16 * CT + 3) / 4;
+  v4f acc[MAXT];
+#pragma unroll
+  for (int q = 0; q < MAXT; ++q) acc[q] = v4f{0.f, 0.f, 0.f, 0.f};
+  const int ntile = Kt * CT;
+#pragma unroll
+  for (int q = 0; q < MAXT; ++q) {
+    const int t = wave + 4 * q;
+    if (t < ntile) {
+      const int nt16 = t / CT, ct = t % CT;
+      const bool isP = ct * 16 < RP;
+      const float* Bi = isP ? Pi : Hi;
+      const int np = nt16 * 16 + ln;
+      const bool rowok = np < K;
+      for (int ks = 0; ks < J / 4; ++ks) {
+        const int j = 4 * ks + lg;
+        const float av = rowok ? Bi[(long long)np * J + j] : 0.0f;
+        const float bv = sA[j * RT + ct * 16 + ln];
+        acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[q], 0, 0, 0);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MAXT; ++q) {
+    const int t = wave + 4 * q;
+    if (t < ntile) {
+      const int nt16 = t / CT, ct = t % CT;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sA[(nt16 * 16 + 4 * lg + r) * RT + ct * 16 + ln] = acc[q][r];
+    }
+  }
+  __syncthreads();
+
+  // Phase U: FP64 update per (member, n) and the inverse-transform inputs.
+  for (int p = tid; p < MB * K; p += 256) {
+    const int mb = p / K, np = p % K;
+    const int b = b0 + mb;
+    if (b >= B) continue;
+    const int n = m + np;
+    const double* cf = a.coef + (long long)b * a.ncoef;
+    double fp[5][2], fh[4][2];
+    for (int k = 0; k < 5; ++k)
+      for (int c = 0; c < 2; ++c) fp[k][c] = sA[np * RT + mb * 10 + 2 * k + c];
+    for (int k = 0; k < 4; ++k)
+      for (int c = 0; c < 2; ++c) fh[k][c] = sA[np * RT + RP + mb * 8 + 2 * k + c];
+    double* Sb = a.S + (long long)b * 2 * 3 * Mp * Np * 2;
+    const long long fstride = (long long)Mp * Np * 2;
+    const long long so = ((long long)m * Np + n) * 2;
+    double eta_n[2], delta_n[2], Phi_n[2], dh[2], eh[2];
+    spectral_update(cf, cl, a.f2[(long long)m * Np + n], n, a.expflag, a.diffflag, fp, fh, Sb + 3 * fstride, Sb,
+                    so, fstride, eta_n, delta_n, Phi_n);
+    wind_potentials(cf, cl, m, n, eta_n, delta_n, dh, eh);
+    float* xr = sX + np * XT;
+    for (int c = 0; c < 2; ++c) {
+      xr[mb * 10 + 0 + c] = (float)eta_n[c];
+      xr[mb * 10 + 2 + c] = (float)delta_n[c];
+      xr[mb * 10 + 4 + c] = (float)Phi_n[c];
+      xr[mb * 10 + 6 + c] = (float)dh[c];
+      xr[mb * 10 + 8 + c] = (float)eh[c];
+      xr[XP + mb * 4 + 0 + c] = (float)eh[c];
+      xr[XP + mb * 4 + 2 + c] = (float)dh[c];
+    }
+  }
+  __syncthreads();
+
+  // Phase I: inverse contraction tiles (J/16 row tiles x XT/16 column tiles), results to sA.
+  constexpr int CI = XT / 16;
+  constexpr int NTI = (J / 16) * CI;
+  for (int t = wave; t < NTI; t += 4) {
+    const int jt = t / CI, ct = t % CI;
+    const bool isP = ct * 16 < XP;
+    const float* Bf = isP ? Pf : Hf;
+    const int j = jt * 16 + ln;
+    v4f d = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int ks = 0; ks < K4 / 4; ++ks) {
+      const int np = 4 * ks + lg;
+      const float av = np < K ? Bf[(long long)j * K + np] : 0.0f;
+      const float bv = sX[np * XT + ct * 16 + ln];
+      d = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, d, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) sA[(jt * 16 + 4 * lg + r) * XT + ct * 16 + ln] = d[r];
+  }
+  __syncthreads();
+
+  // Phase W: Fourier rows of eta, delta, Phi, U, V (m-major, coalesced in j).
+  const float mr = (float)m;
+  for (int p = tid; p < MB * J; p += 256) {
+    const int mb = p / J, j = p % J;
+    const int b = b0 + mb;
+    if (b >= B) continue;
+    const float* dr = sA + j * XT;
+    const float* pr = dr + mb * 10;
+    const float* hr = dr + XP + mb * 4;
+    float* G = a.G + fidx(b, 0, m, j, Mp, J);
+    const long long fs = (long long)Mp * J * 2;
+    G[0] = pr[0];
+    G[1] = pr[1];
+    G[fs] = pr[2];
+    G[fs + 1] = pr[3];
+    G[2 * fs] = pr[4];
+    G[2 * fs + 1] = pr[5];
+    G[3 * fs] = mr * pr[7] + hr[0];
+    G[3 * fs + 1] = -mr * pr[6] + hr[1];
+    G[4 * fs] = mr * pr[9] - hr[2];
+    G[4 * fs + 1] = -mr * pr[8] - hr[3];
+  }
+}
+
+template <int MT, int MB>
+static hipError_t launch_leg_mfma_t(const LegArgs<float>& a, hipStream_t s) {
+  using S = LegShape<MT, MB>;
+  const size_t smem = sizeof(float) * S::LDS_FLOATS;
+  dim3 grid(MT + 1, (a.B + MB - 1) / MB);
+  hipLaunchKernelGGL((k_leg_mfma<MT, MB>), grid, dim3(256), smem, s, a);
+  return hipGetLastError();
+}
+
+size_t leg_mfma_lds_bytes(int M) {
+  switch (M) {
+    case 42: return sizeof(float) * LegShape<42, 8>::LDS_FLOATS;
+    case 63: return sizeof(float) * LegShape<63, 4>::LDS_FLOATS;
+    case 106: return sizeof(float) * LegShape<106, 4>::LDS_FLOATS;
+    default: return 0;
+  }
+}
+
+hipError_t launch_leg_mfma(const LegArgs<float>& a, hipStream_t s) {
+  switch (a.M) {
+    case 42: return launch_leg_mfma_t<42, 8>(a, s);
+    case 63: return launch_leg_mfma_t<63, 4>(a, s);
+    case 106: return launch_leg_mfma_t<106, 4>(a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 }  // namespace swx

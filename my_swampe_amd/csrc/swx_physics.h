@@ -135,6 +135,70 @@ __device__ inline void assemble_slots(const double* cf, double tc, double wj, in
   }
 }
 
+// FP64 spectral update of one (m, n) (modEuler_tdiff.py:95-103, :185-193,
+// :261; explicit_tdiff.py:71, :138, :212) from the transformed slots
+// fp[5][2] (P basis) and fh[4][2] (H basis) and the stored levels.  The carry
+// terms Pw(X) of the reference are f2 * S (Gauss quadrature).  Writes the new
+// current level (the explicit scheme first moves current -> previous) and
+// returns it in eta/delta/Phi.
+__device__ inline void spectral_update(const double* cf, const CoefLayout& cl, double f2, int n, int expflag,
+                                       int diffflag, const double fp[5][2], const double fh[4][2], double* Scur,
+                                       double* Sprev, long long so, long long fstride, double eta_n[2],
+                                       double delta_n[2], double Phi_n[2]) {
+  const double dt = cf[C_DT], aa = cf[C_A], Phibar = cf[C_PHIBAR];
+  const double nn = (double)n * (double)(n + 1);
+  for (int c = 0; c < 2; ++c) {
+    const double eo = f2 * Scur[0 * fstride + so + c];
+    const double d_o = f2 * Scur[1 * fstride + so + c];
+    const double po = f2 * Scur[2 * fstride + so + c];
+    if (!expflag) {
+      const double t2 = (2.0 * dt / (aa * aa)) / 4.0;
+      Phi_n[c] = po - dt * Phibar * d_o - Phibar * 0.5 * ((1.0 / (aa * aa)) * (nn * po)) + fp[0][c] + nn * fp[1][c] +
+                 fh[0][c];
+      delta_n[c] = d_o + nn * (t2 * po) - Phibar * (nn * d_o) / (aa * aa) + fp[2][c] + nn * fp[3][c] + fh[1][c] +
+                   nn * fh[2][c];
+      eta_n[c] = eo + fp[4][c] + fh[3][c];
+    } else {
+      const double ep = f2 * Sprev[0 * fstride + so + c];
+      const double dp = f2 * Sprev[1 * fstride + so + c];
+      const double pp = f2 * Sprev[2 * fstride + so + c];
+      Phi_n[c] = pp - 2.0 * dt * Phibar * d_o + fp[0][c] + fh[0][c];
+      delta_n[c] = dp + fp[2][c] + fh[1][c];
+      eta_n[c] = ep + fp[4][c] + fh[3][c];
+    }
+  }
+  if (diffflag) {
+    const double sg = cf[cl.n_sigma + n], sp = cf[cl.n_sigmaphi + n];
+    for (int c = 0; c < 2; ++c) {
+      Phi_n[c] *= sp;
+      delta_n[c] *= sg;
+      eta_n[c] *= sg;
+    }
+  }
+  for (int c = 0; c < 2; ++c) {
+    if (expflag) {
+      Sprev[0 * fstride + so + c] = Scur[0 * fstride + so + c];
+      Sprev[1 * fstride + so + c] = Scur[1 * fstride + so + c];
+      Sprev[2 * fstride + so + c] = Scur[2 * fstride + so + c];
+    }
+    Scur[0 * fstride + so + c] = eta_n[c];
+    Scur[1 * fstride + so + c] = delta_n[c];
+    Scur[2 * fstride + so + c] = Phi_n[c];
+  }
+}
+
+// Wind potentials of the inverse transform (spectral_transform.py:577-582):
+// delta_hat = delta a/(n(n+1)), eta_hat = (eta - f) a/(n(n+1)), zero at n = 0.
+__device__ inline void wind_potentials(const double* cf, const CoefLayout& cl, int m, int n, const double eta[2],
+                                       const double delta[2], double dh[2], double eh[2]) {
+  const double tcmn = cf[cl.n_tcmn + n];
+  const double f = (m == 0 && n == 1) ? cf[C_FMN01] : 0.0;
+  for (int c = 0; c < 2; ++c) {
+    dh[c] = (n == 0) ? 0.0 : delta[c] * tcmn;
+    eh[c] = (n == 0) ? 0.0 : (eta[c] - (c == 0 ? f : 0.0)) * tcmn;
+  }
+}
+
 // Adjoint of grid_products at one grid point (reverse mode, the Jacobian of
 // forcing.py:65-155 and initial_conditions.py:283-292 transposed).
 // pbar: cotangents of A, B, C, D, E, PhiF, F, G.  Returns the cotangents of

@@ -82,6 +82,8 @@ struct swx_plan {
   bool use_mfma = false;       // MFMA grid kernel available for this plan
   bool opt_plain_grid = false;  // SWX_OPT_GRID_KERNEL = 1
   bool opt_full_phys = false;   // SWX_OPT_FULL_PHYS_CARRY = 1
+  bool use_leg_mfma = false;    // MFMA Legendre kernel available
+  bool opt_plain_leg = false;   // SWX_OPT_LEG_KERNEL = 1
   double* geo_d = nullptr;
   double* f2 = nullptr;
   double* coef = nullptr;
@@ -248,6 +250,14 @@ hipError_t launch_grid_any(swx_plan* p, const GridArgs<R>& g, hipStream_t s) {
 }
 
 template <typename R>
+hipError_t launch_leg_any(swx_plan* p, const LegArgs<R>& l, hipStream_t s) {
+  if constexpr (sizeof(R) == 4) {
+    if (p->use_leg_mfma && !p->opt_plain_leg && l.mode == LEG_STEP && !l.diagnostics) return launch_leg_mfma(l, s);
+  }
+  return launch_leg<R>(l, s);
+}
+
+template <typename R>
 int do_init(swx_plan* p, const double* eta0, const double* delta0, const double* Phi0, const double* U0,
             const double* V0, hipStream_t s) {
   const int have_uv = (U0 != nullptr);
@@ -311,7 +321,7 @@ int do_steps(swx_plan* p, int t0, int nsteps, double* hist_fields, double* hist_
     l.hist_dead = hist_dead;
     l.hist_T = hist_T;
     l.hist_k = hist_k0 + k;
-    SWX_HIP(launch_leg<R>(l, s));
+    SWX_HIP(launch_leg_any<R>(p, l, s));
     GridArgs<R> g = grid_args<R>(p, GRID_STEP);
     g.do_ra = (p->d.modalflag && t > 2) ? 1 : 0;
     g.phys_on = (k >= phys_from || hist_fields || hist_phiext) ? 1 : 0;
@@ -564,6 +574,7 @@ int swx_plan_create(const swx_plan_desc* desc, const double* params, swx_plan** 
     int maxlds = 0;
     (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device);
     p->use_mfma = grid_mfma_lds_bytes(M) <= (size_t)maxlds;
+    p->use_leg_mfma = leg_mfma_lds_bytes(M) <= (size_t)maxlds;
     if (p->use_mfma) {
       const int k1p = grid_mfma_k1p(M), k2p = grid_mfma_k2p(M);
       std::vector<double> ti((size_t)I * k1p, 0.0), tf((size_t)k2p * I, 0.0);
@@ -719,6 +730,10 @@ int swx_plan_set_option(swx_plan* p, int option, int value) {
     case SWX_OPT_FULL_PHYS_CARRY:
       p->opt_full_phys = value != 0;
       break;
+    case SWX_OPT_LEG_KERNEL:
+      if (value != 0 && value != 1) return fail(SWX_ERR_ARG, "SWX_OPT_LEG_KERNEL must be 0 (auto) or 1 (plain)");
+      p->opt_plain_leg = value == 1;
+      break;
     default:
       return fail(SWX_ERR_ARG, "unknown option " + std::to_string(option));
   }
@@ -746,7 +761,7 @@ int swx_time_kernels(swx_plan* p, int t0, int nsteps, float* ms_leg, float* ms_g
       SWX_HIP(launch_leg<double>(l, s));
     } else {
       LegArgs<float> l = leg_args<float>(p, LEG_STEP);
-      SWX_HIP(launch_leg<float>(l, s));
+      SWX_HIP(launch_leg_any<float>(p, l, s));
     }
     SWX_HIP(hipEventRecord(ev[1], s));
     // Same physical-carry mode as the steady steps of swx_run_graph.

@@ -265,7 +265,7 @@ def test_fp32_transforms_close_to_fp64(torch_gpu):
 
 @pytest.mark.parametrize("M", [42, 63, 106])
 def test_mfma_grid_kernel_matches_plain_fp32(M, torch_gpu):
-    """The MFMA grid kernel (FP32) vs the plain FP32 grid kernel and the FP64 oracle, 12 forced steps
+    """The MFMA grid and Legendre kernels (FP32) vs the plain FP32 kernels and the FP64 oracle, 12 forced steps
     from a balanced zonal flow (Williamson test 2 fields, U ~ 40 m/s).  Relative L2 <= 5e-6 for
     Phi, eta and U."""
     from my_swampe_amd import _capi
@@ -282,8 +282,9 @@ def test_mfma_grid_kernel_matches_plain_fp32(M, torch_gpu):
     _print_case("mfma-vs-plain", M=M)
     res = {}
     for kind in ("plain", "mfma"):
-        plan = Plan(M, p, precision="fp32")
+        plan = Plan(M, np.stack([p] * 3), precision="fp32")  # 3 members: a partial member group
         plan.set_option(_capi.OPT_GRID_KERNEL, 1 if kind == "plain" else 0)
+        plan.set_option(_capi.OPT_LEG_KERNEL, 1 if kind == "plain" else 0)
         plan.init(*ics)
         plan.run(2, 12)
         res[kind] = {k: _np(v[0]) for k, v in plan.export(fields=("Phi_curr", "U_curr", "eta_curr")).items()
