@@ -4,8 +4,8 @@
 Workload (synthetic, seeded): T42 (J=64, I=128), 256 ensemble members, forced
 modified Euler, dt=120 s, a=8.2e7, omega=3.2e-5, Phibar=3e5, K6=1.24e33,
 diffusion and modal filter on; per-member taurad, taudrag log-uniform in
-[1e4, 1e6] s and DPhieq uniform in [1e5, 5e5] (seed 7); rest + seeded Phi
-noise initial condition (member k: seed 1000+k).  Transforms in FP32, spectral
+[1e4, 1e6] s and DPhieq uniform in [1e5, 5e5] (seed 7); rest + seeded
+spectral Phi noise initial condition (n <= 10), set through swx_init_spectral.  Transforms in FP32, spectral
 state update and diffusion in FP64.  Inputs are resident in HBM before the
 timed region; each step replays the HIP graph driver.
 
@@ -45,14 +45,6 @@ def member_params(B: int, seed: int = 7):
                      for b in range(B)])
 
 
-def member_ics(M: int, B: int, omega: float = 3.2e-5):
-    """Rest state + seeded Phi noise on n <= 10 (std 100), member k uses seed 1000 + k."""
-    from oracle.swampe_oracle import rest_noise_ic  # IC synthesis only (host numpy, outside the timed region)
-
-    fields = [rest_noise_ic(M, 1000 + k, omega) for k in range(B)]
-    return [np.stack([f[i] for f in fields]) for i in range(5)]
-
-
 def flops_per_member_step(M: int, J: int, I: int):
     """Algorithmic work per member-step (fixed formula; FP64-carry forward step).
 
@@ -72,13 +64,14 @@ def run_batch(M, B, precision, steps, warmup, graph_steps=32, full_phys=False):
     from my_swampe_amd.plan import Plan
 
     params = member_params(B)
-    ics = member_ics(M, B)
+    c0 = torch.tensor(spectral_ics(M, B), device="cuda")
     plan = Plan(M, params, precision=precision, diagnostics=False)
     if full_phys:
         plan.set_option(_capi.OPT_FULL_PHYS_CARRY, 1)
     stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
     with torch.cuda.stream(stream):
-        plan.init(*ics, stream=stream)
+        plan.init_spectral(c0, stream=stream)
         plan.run_graph(2, warmup, graph_steps=graph_steps, stream=stream)
         stream.synchronize()
         t0 = time.perf_counter()
@@ -94,7 +87,8 @@ def run_batch(M, B, precision, steps, warmup, graph_steps=32, full_phys=False):
 
 
 def spectral_ics(M: int, B: int, omega: float = 3.2e-5):
-    """Spectral (eta, delta, Phi) of the rest + noise state: Coriolis vorticity f_mn plus seeded Phi noise."""
+    """Spectral (eta, delta, Phi) of the rest + noise state: the planetary vorticity (m=0, n=1 coefficient of
+    2 omega mu) plus Phi noise of std 30 per real component on n <= 10 (one generator, seed 1000)."""
     rng = np.random.default_rng(1000)
     c = np.zeros((B, 3, M + 1, M + 1), complex)
     c[:, 0, 0, 1] = omega / np.sqrt(0.375)
@@ -135,7 +129,7 @@ def run_fwd_adjoint(M, B, precision, steps, reps=2):
 
 
 def cpu_baseline(seconds_target: float = 12.0):
-    """FP64 oracle (numpy, one core) on config 0: T42, 1 member, forced modEuler, dt=120."""
+    """FP64 oracle (numpy, one core): T42, 1 member, forced modified Euler, dt=120 s, runs of 240 steps."""
     from oracle import swampe_oracle as orc
 
     eta0, delta0, Phi0, U0, V0 = orc.rest_noise_ic(42, 0, 3.2e-5)
@@ -150,7 +144,7 @@ def cpu_baseline(seconds_target: float = 12.0):
         t += time.perf_counter() - t0
         steps += 240
     return dict(value=steps / t, unit="member-steps/s", cores=1, kind="port",
-                sample=f"FP64 numpy oracle, T42 forced modEuler, 1 member, {steps} steps (config 0 x {steps // 240})")
+                sample=f"FP64 numpy oracle, T42 forced modEuler, 1 member, {steps} steps ({steps // 240} runs of 240)")
 
 
 def main():
@@ -167,7 +161,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; members shard across GPUs with no per-step "
-                                   "exchange (see DESIGN.md), multi-GPU runs are not provided"}))
+                                   "exchange, multi-GPU runs are not provided"}))
         sys.exit(2)
     import torch
 

@@ -178,7 +178,7 @@ int swx_state_set(swx_plan *plan, const double *in, void *stream);
  * gives dL/dS_0 (device complex128 [B][3][M+1][N+1]) and the accumulated
  * parameter gradients (device float64 [B][3]: taurad, taudrag, DPhieq).
  * Replaces the reverse-mode transpose of the lax.scan that jax.grad builds
- * (model.py:1281-1342; SURVEY call stack D).  The forward carry of the plan
+ * (model.py:1281-1342).  The forward carry of the plan
  * (Fourier rows) is used as scratch. */
 int swx_adjoint_begin(swx_plan *plan, void *stream);
 int swx_adjoint_terminal(swx_plan *plan, const double *g_eta, const double *g_delta, const double *g_Phi,

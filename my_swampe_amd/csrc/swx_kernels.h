@@ -68,6 +68,9 @@ constexpr int kNG = 5;
 // Raw forward-DFT products per row: A, B, C, D, E, PhiF, F, G.
 constexpr int kNRaw = 8;
 
+// Degree rows padded to whole 16-row MFMA tiles.
+__host__ __device__ constexpr int leg_knp(int N) { return (N + 1 + 15) / 16 * 16; }
+
 // Fourier rows G / Gprev / Gbar are stored m-major: [member][field][m][j]
 // complex, so a Legendre block (one m) reads and writes contiguous j.  Offset
 // in reals of (b, f, m, j):
@@ -92,6 +95,14 @@ struct LegArgs {
   const R* Hf;
   const R* Pi;  // inverse layout  [m-block][n-m][j]
   const R* Hi;
+  // MFMA-fragment tables of the same blocks (FP32 plans; KNP = N+1 rounded up to 16):
+  //   fwd[m][nt][lane][ks] = basis(j = 4 ks + lane/16, n = m + 16 nt + lane%16)   ks < J/4
+  //   inv[m][jt][lane][ks] = basis(j = 16 jt + lane%16, n = m + 4 ks + lane/16)   ks < KNP/4
+  // zero where n > N; each lane's ks run is contiguous (float4 loads).
+  const float* fwdP;
+  const float* fwdH;
+  const float* invP;
+  const float* invH;
   const double* coef;
   int ncoef;
   // [M+1][N+1]: Pw(invrs_leg(X)) = f2 * X for the basis in use (1 for an

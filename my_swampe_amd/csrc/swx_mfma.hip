@@ -300,7 +300,8 @@ struct LegShape {
   static constexpr int RP = ceil16(10 * MB), RH = ceil16(8 * MB), RT = RP + RH;  // forward columns
   static constexpr int XP = ceil16(10 * MB), XH = ceil16(4 * MB), XT = XP + XH;  // inverse columns
   static constexpr int KNP = ceil16(Np);  // padded degree rows
-  static constexpr int SA = cmax(cmax(J * RT, KNP * RT), J * XT);
+  static constexpr int JO = J + 4;         // column stride of the inverse results (bank spread)
+  static constexpr int SA = cmax(cmax(J * RT, KNP * RT), XT * JO);
   static constexpr int SX = KNP * XT;
   static constexpr int LDS_FLOATS = SA + SX;
 };
@@ -308,9 +309,11 @@ struct LegShape {
 template <int MT, int MB>
 __global__ void __launch_bounds__(256) k_leg_mfma(LegArgs<float> a) {
   using S = LegShape<MT, MB>;
-  constexpr int Np = S::Np, Mp = MT + 1, J = S::J, RP = S::RP, RT = S::RT, XP = S::XP, XT = S::XT;
+  constexpr int Np = S::Np, Mp = MT + 1, J = S::J, RP = S::RP, RT = S::RT, XP = S::XP, XT = S::XT, JO = S::JO;
+  constexpr int KNP = S::KNP;
+  static_assert(KNP == leg_knp(MT), "fragment tables use leg_knp rows");
   extern __shared__ __align__(16) float lds[];
-  float* sA = lds;           // [J][RT] inputs, then [K][RT] forward results, then [J][XT] inverse results
+  float* sA = lds;           // [J][RT] inputs, then [K][RT] forward results, then [XT][JO] inverse results
   float* sX = lds + S::SA;   // [KNP][XT] inverse-transform inputs
   const int m = blockIdx.x;
   const int b0 = blockIdx.y * MB;
@@ -319,20 +322,33 @@ __global__ void __launch_bounds__(256) k_leg_mfma(LegArgs<float> a) {
   const int K = Np - m;
   const int Kt = (K + 15) / 16;
   const int K4 = ceil4(K);
-  const long long boff = tri_off(m, N, J);
-  const float* Pi = a.Pi + boff;
-  const float* Hi = a.Hi + boff;
-  const float* Pf = a.Pf + boff;
-  const float* Hf = a.Hf + boff;
   const CoefLayout cl = CoefLayout::make(N, J);
 
-  // Phase 0: stage the Legendre inputs [j][member, slot, re/im]; zero sX.
-  for (int e = tid; e < MB * kNT * J * 2; e += 256) {
-    const int c = e & 1, j = (e >> 1) % J, ks = (e >> 1) / J;
-    const int k = ks % kNT, mb = ks / kNT;
-    const int b = b0 + mb;
-    const int col = k < kNP ? mb * 10 + 2 * k + c : RP + mb * 8 + 2 * (k - kNP) + c;
-    sA[j * RT + col] = b < B ? a.T[(((long long)b * Mp + m) * kNT + k) * J * 2 + 2 * j + c] : 0.0f;
+  // Phase 0: stage the Legendre inputs [j][member, slot, re/im]; zero sX.  Each member's
+  // [slot][j][re/im] block is contiguous in T: float4 loads (two latitudes), all issued first.
+  {
+    constexpr int QPM = kNT * J / 2;  // float4 per member
+    constexpr int NQ = (MB * QPM + 255) / 256;
+    v4f tv[NQ];
+#pragma unroll
+    for (int u = 0; u < NQ; ++u) {
+      const int q = tid + 256 * u;
+      const int mb = q / QPM < MB ? q / QPM : MB - 1;
+      const int b = b0 + mb < B ? b0 + mb : B - 1;  // clamped address, zeroed below
+      const v4f v = *reinterpret_cast<const v4f*>(a.T + (((long long)b * Mp + m) * kNT) * J * 2 + 4 * (q % QPM));
+      tv[u] = (q < MB * QPM && b0 + q / QPM < B) ? v : v4f{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int u = 0; u < NQ; ++u) {
+      const int q = tid + 256 * u;
+      if (q < MB * QPM) {
+        const int mb = q / QPM, r = q % QPM;
+        const int k = r / (J / 2), j = 2 * (r % (J / 2));
+        const int col = k < kNP ? mb * 10 + 2 * k : RP + mb * 8 + 2 * (k - kNP);
+        *reinterpret_cast<float2*>(sA + j * RT + col) = make_float2(tv[u][0], tv[u][1]);
+        *reinterpret_cast<float2*>(sA + (j + 1) * RT + col) = make_float2(tv[u][2], tv[u][3]);
+      }
+    }
   }
   for (int e = tid; e < S::SX; e += 256) sX[e] = 0.0f;
   constexpr int padH = RT - RP - MB * 8, padP = RP - MB * 10;
@@ -355,15 +371,16 @@ __global__ void __launch_bounds__(256) k_leg_mfma(LegArgs<float> a) {
     if (t < ntile) {
       const int nt16 = t / CT, ct = t % CT;
       const bool isP = ct * 16 < RP;
-      const float* Bi = isP ? Pi : Hi;
-      const int np = nt16 * 16 + ln;
-      const bool rowok = np < K;
-      for (int ks = 0; ks < J / 4; ++ks) {
-        const int j = 4 * ks + lg;
-        const float av = rowok ? Bi[(long long)np * J + j] : 0.0f;
-        const float bv = sA[j * RT + ct * 16 + ln];
-        acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[q], 0, 0, 0);
-      }
+      // The tile's A fragments (fragment-ordered table, contiguous per lane) first, then the MFMAs.
+      const v4f* ar = reinterpret_cast<const v4f*>((isP ? a.fwdP : a.fwdH) +
+                                                   ((long long)(m * (KNP / 16) + nt16) * 64 + lane) * (J / 4));
+      v4f av[J / 16];
+#pragma unroll
+      for (int q4 = 0; q4 < J / 16; ++q4) av[q4] = ar[q4];
+#pragma unroll
+      for (int ks = 0; ks < J / 4; ++ks)
+        acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[ks / 4][ks % 4], sA[(4 * ks + lg) * RT + ct * 16 + ln],
+                                                      acc[q], 0, 0, 0);
     }
   }
   __syncthreads();
@@ -410,23 +427,46 @@ __global__ void __launch_bounds__(256) k_leg_mfma(LegArgs<float> a) {
   }
   __syncthreads();
 
-  // Phase I: inverse contraction tiles (J/16 row tiles x XT/16 column tiles), results to sA.
-  constexpr int CI = XT / 16;
-  constexpr int NTI = (J / 16) * CI;
-  for (int t = wave; t < NTI; t += 4) {
-    const int jt = t / CI, ct = t % CI;
-    const bool isP = ct * 16 < XP;
-    const float* Bf = isP ? Pf : Hf;
+  // Phase I: inverse contraction; wave w owns latitude tiles jt = w, w+4, ... and every
+  // column tile, so its P and H fragments are loaded once and reused across them.
+  constexpr int CIP = XP / 16, CIH = (XT - XP) / 16;
+  constexpr int KSM = KNP / 4;  // k-steps at m = 0
+  const int ksn = K4 / 4;
+  for (int jt = wave; jt < J / 16; jt += 4) {
     const int j = jt * 16 + ln;
-    v4f d = v4f{0.f, 0.f, 0.f, 0.f};
-    for (int ks = 0; ks < K4 / 4; ++ks) {
-      const int np = 4 * ks + lg;
-      const float av = np < K ? Bf[(long long)j * K + np] : 0.0f;
-      const float bv = sX[np * XT + ct * 16 + ln];
-      d = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, d, 0, 0, 0);
-    }
+    const long long fo = ((long long)(m * (J / 16) + jt) * 64 + lane) * KSM;
+    const v4f* pr = reinterpret_cast<const v4f*>(a.invP + fo);
+    const v4f* hr = reinterpret_cast<const v4f*>(a.invH + fo);
+    v4f pa[KSM / 4], ha[KSM / 4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) sA[(jt * 16 + 4 * lg + r) * XT + ct * 16 + ln] = d[r];
+    for (int q4 = 0; q4 < KSM / 4; ++q4) {
+      const bool need = 4 * q4 < ksn;
+      pa[q4] = need ? pr[q4] : v4f{0.f, 0.f, 0.f, 0.f};
+      ha[q4] = need ? hr[q4] : v4f{0.f, 0.f, 0.f, 0.f};
+    }
+    v4f dP[CIP], dH[CIH];
+#pragma unroll
+    for (int c = 0; c < CIP; ++c) dP[c] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < CIH; ++c) dH[c] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < KSM; ++ks) {
+      if (ks < ksn) {
+        const float* xr = sX + (4 * ks + lg) * XT + ln;
+#pragma unroll
+        for (int c = 0; c < CIP; ++c)
+          dP[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(pa[ks / 4][ks % 4], xr[c * 16], dP[c], 0, 0, 0);
+#pragma unroll
+        for (int c = 0; c < CIH; ++c)
+          dH[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(ha[ks / 4][ks % 4], xr[XP + c * 16], dH[c], 0, 0, 0);
+      }
+    }
+    // Column-major [col][JO] output: the 4 rows a lane holds are contiguous (one b128 store).
+    float* out = sA + ln * JO + jt * 16 + 4 * lg;
+#pragma unroll
+    for (int c = 0; c < CIP; ++c) *reinterpret_cast<v4f*>(out + c * 16 * JO) = dP[c];
+#pragma unroll
+    for (int c = 0; c < CIH; ++c) *reinterpret_cast<v4f*>(out + (XP + c * 16) * JO) = dH[c];
   }
   __syncthreads();
 
@@ -436,21 +476,20 @@ __global__ void __launch_bounds__(256) k_leg_mfma(LegArgs<float> a) {
     const int mb = p / J, j = p % J;
     const int b = b0 + mb;
     if (b >= B) continue;
-    const float* dr = sA + j * XT;
-    const float* pr = dr + mb * 10;
-    const float* hr = dr + XP + mb * 4;
+    const float* pr = sA + mb * 10 * JO + j;
+    const float* hr = sA + (XP + mb * 4) * JO + j;
     float* G = a.G + fidx(b, 0, m, j, Mp, J);
     const long long fs = (long long)Mp * J * 2;
     G[0] = pr[0];
-    G[1] = pr[1];
-    G[fs] = pr[2];
-    G[fs + 1] = pr[3];
-    G[2 * fs] = pr[4];
-    G[2 * fs + 1] = pr[5];
-    G[3 * fs] = mr * pr[7] + hr[0];
-    G[3 * fs + 1] = -mr * pr[6] + hr[1];
-    G[4 * fs] = mr * pr[9] - hr[2];
-    G[4 * fs + 1] = -mr * pr[8] - hr[3];
+    G[1] = pr[JO];
+    G[fs] = pr[2 * JO];
+    G[fs + 1] = pr[3 * JO];
+    G[2 * fs] = pr[4 * JO];
+    G[2 * fs + 1] = pr[5 * JO];
+    G[3 * fs] = mr * pr[7 * JO] + hr[0];
+    G[3 * fs + 1] = -mr * pr[6 * JO] + hr[JO];
+    G[4 * fs] = mr * pr[9 * JO] - hr[2 * JO];
+    G[4 * fs + 1] = -mr * pr[8 * JO] - hr[3 * JO];
   }
 }
 

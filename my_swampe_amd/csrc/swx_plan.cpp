@@ -76,6 +76,7 @@ struct swx_plan {
   void* Hf = nullptr;
   void* Pi = nullptr;
   void* Hi = nullptr;
+  float* fragP[4] = {nullptr, nullptr, nullptr, nullptr};  // MFMA Legendre tables: fwd P, fwd H, inv P, inv H
   void* tw = nullptr;
   float* TinvT = nullptr;  // MFMA grid kernel tables (FP32 plans)
   float* TfwdT = nullptr;
@@ -195,6 +196,10 @@ LegArgs<R> leg_args(swx_plan* p, int mode) {
   a.Hf = (const R*)p->Hf;
   a.Pi = (const R*)p->Pi;
   a.Hi = (const R*)p->Hi;
+  a.fwdP = p->fragP[0];
+  a.fwdH = p->fragP[1];
+  a.invP = p->fragP[2];
+  a.invH = p->fragP[3];
   a.coef = p->coef;
   a.ncoef = p->cl.ncoef;
   a.f2 = p->f2;
@@ -575,6 +580,42 @@ int swx_plan_create(const swx_plan_desc* desc, const double* params, swx_plan** 
     (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device);
     p->use_mfma = grid_mfma_lds_bytes(M) <= (size_t)maxlds;
     p->use_leg_mfma = leg_mfma_lds_bytes(M) <= (size_t)maxlds;
+    if (p->use_leg_mfma) {
+      // Fragment-ordered copies of the basis blocks (layouts in LegArgs).
+      const int KNP = leg_knp(N);
+      const size_t per = (size_t)Mp * KNP * J;
+      std::vector<std::vector<double>> fr(4, std::vector<double>(per, 0.0));
+      for (int m = 0; m <= M; ++m) {
+        const int K = N + 1 - m;
+        for (int lane = 0; lane < 64; ++lane) {
+          const int ln = lane & 15, lg = lane >> 4;
+          for (int nt = 0; nt < KNP / 16; ++nt)
+            for (int ks = 0; ks < J / 4; ++ks) {
+              const int np = 16 * nt + ln, j = 4 * ks + lg;
+              if (np >= K) continue;
+              const size_t src = ((size_t)j * Mp + m) * (N + 1) + (m + np);
+              const size_t dst = (((size_t)m * (KNP / 16) + nt) * 64 + lane) * (J / 4) + ks;
+              fr[0][dst] = p->geo.P[src];
+              fr[1][dst] = p->geo.H[src];
+            }
+          for (int jt = 0; jt < J / 16; ++jt)
+            for (int ks = 0; ks < KNP / 4; ++ks) {
+              const int np = 4 * ks + lg, j = 16 * jt + ln;
+              if (np >= K) continue;
+              const size_t src = ((size_t)j * Mp + m) * (N + 1) + (m + np);
+              const size_t dst = (((size_t)m * (J / 16) + jt) * 64 + lane) * (KNP / 4) + ks;
+              fr[2][dst] = p->geo.P[src];
+              fr[3][dst] = p->geo.H[src];
+            }
+        }
+      }
+      for (int q = 0; q < 4 && rc == SWX_OK; ++q)
+        if ((rc = dalloc(&p->fragP[q], per * 4)) == SWX_OK) rc = upload_real(p->fragP[q], fr[q], 4);
+      if (rc) {
+        swx_plan_destroy(p);
+        return rc;
+      }
+    }
     if (p->use_mfma) {
       const int k1p = grid_mfma_k1p(M), k2p = grid_mfma_k2p(M);
       std::vector<double> ti((size_t)I * k1p, 0.0), tf((size_t)k2p * I, 0.0);
@@ -627,7 +668,8 @@ int swx_plan_set_params(swx_plan* p, const double* params, void* stream) {
 void swx_plan_destroy(swx_plan* p) {
   if (!p) return;
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
-  void* bufs[] = {p->Lam, p->Tbar, p->Gbar, p->pgrad, p->Pf, p->Hf, p->Pi, p->Hi, p->tw, p->TinvT, p->TfwdT, p->geo_d, p->f2, p->coef, p->S, p->T, p->G, p->Gprev,
+  void* bufs[] = {p->Lam, p->Tbar, p->Gbar, p->pgrad, p->Pf, p->Hf, p->Pi, p->Hi, p->fragP[0], p->fragP[1],
+                  p->fragP[2], p->fragP[3], p->tw, p->TinvT, p->TfwdT, p->geo_d, p->f2, p->coef, p->S, p->T, p->G, p->Gprev,
                   p->raw, p->lag, p->g0uv, p->phys, p->uvic, p->part_uv, p->dead};
   for (void* q : bufs)
     if (q) (void)hipFree(q);

@@ -1,6 +1,6 @@
 """Hand-written adjoint (GPU, FP64) vs central finite differences of the HIP forward and of the oracle.
 
-Bar (north_star): gradients within 1e-6 relative in FP64.  Configuration: T42, two
+Bar: gradients within 1e-6 relative in FP64.  Configuration: T42, two
 members with different forcing parameters, 6 forced modified-Euler steps,
 loss = mean((Phi_T - Phi_target)^2) per member, summed.
 """
