@@ -150,14 +150,19 @@ template <typename R>
 hipError_t launch_leg(const LegArgs<R>& a, hipStream_t s);
 template <typename R>
 hipError_t launch_grid(const GridArgs<R>& a, hipStream_t s);
-// MFMA grid kernel (FP32, GRID_STEP without history/diagnostics); tables:
-// TinvT [I][k1p], TfwdT [k2p][I].
-hipError_t launch_grid_mfma(const GridArgs<float>& a, const float* TinvT, const float* TfwdT, hipStream_t s);
+// MFMA grid kernel (FP32, GRID_STEP without history/diagnostics).  DFT tables
+// in MFMA-fragment order (lane = 16 * (lane / 16) + lane % 16, ln = lane % 16,
+// lg = lane / 16), zero outside the matrices:
+//   inv[it][lane][ks] = Tinv(i = 16 it + ln, k = 4 ks + lg),       ks < kq1
+//   fwd[t][g][lane][q] = Tfwd(k' = 16 t + ln, i = 16 g + 4 q + lg), q < 4
+// with Tinv(i, 2m) = w_m cos(2 pi m i / I), Tinv(i, 2m+1) = -w_m sin(...),
+// Tfwd(2m, i) = cos(2 pi m i / I) / I, Tfwd(2m+1, i) = -sin(...) / I.
+hipError_t launch_grid_mfma(const GridArgs<float>& a, const float* TinvF, const float* TfwdF, hipStream_t s);
+int grid_mfma_kq1(int M);
 size_t grid_mfma_lds_bytes(int M);
 // MFMA Legendre kernel (FP32, LEG_STEP without diagnostics).
 hipError_t launch_leg_mfma(const LegArgs<float>& a, hipStream_t s);
 size_t leg_mfma_lds_bytes(int M);
-int grid_mfma_k1p(int M);
 int grid_mfma_k2p(int M);
 
 template <typename R>

@@ -35,6 +35,7 @@ struct MfmaShape {
   static constexpr int I = MT == 42 ? 128 : (MT == 63 ? 192 : 320);
   static constexpr int K1 = 2 * Mp;                  // inverse-DFT inner dimension
   static constexpr int K1p = (K1 + 3) / 4 * 4;       // padded to the MFMA k-step
+  static constexpr int KQ1 = (K1p / 4 + 3) / 4 * 4;  // per-lane inverse-table run (float4 loads)
   static constexpr int K2 = 2 * Mp;                  // forward-DFT outputs (re, im)
   static constexpr int K2p = (K2 + 15) / 16 * 16;    // padded to 16-row tiles
   static constexpr int NT2 = K2p / 16;
@@ -46,10 +47,11 @@ struct MfmaShape {
 };
 
 template <int MT>
-__global__ void __launch_bounds__(256) k_grid_mfma(GridArgs<float> a, const float* __restrict__ TinvT,
-                                                   const float* __restrict__ TfwdT) {
+__global__ void __launch_bounds__(256) k_grid_mfma(GridArgs<float> a, const float* __restrict__ TinvF,
+                                                   const float* __restrict__ TfwdF) {
   using S = MfmaShape<MT>;
-  constexpr int Mp = S::Mp, I = S::I, K1 = S::K1, K1p = S::K1p, K2p = S::K2p, NT2 = S::NT2, ICH = S::ICH;
+  constexpr int Mp = S::Mp, I = S::I, K1p = S::K1p, NT2 = S::NT2, ICH = S::ICH, KQ1 = S::KQ1;
+  static_assert(ICH % 16 == 0 && I % ICH == 0, "chunking");
   extern __shared__ __align__(16) float lds[];
   float* sXT = lds;                    // [5][K1p][16]
   float* sYT = lds + 5 * K1p * 16;     // [8][ICH][16]
@@ -70,11 +72,30 @@ __global__ void __launch_bounds__(256) k_grid_mfma(GridArgs<float> a, const floa
   const double alpha = cf[C_ALPHA];
 
   // Phase 0: Fourier rows (m-major, 16 consecutive j per (f, m)) -> X^T (k-major, zero padded).
-  for (int e = tid; e < 5 * (K1p / 2) * 32; e += 256) {
-    const int c = e & 1, n = (e >> 1) & 15, fm = e >> 5;
-    const int f = fm / (K1p / 2), m = fm % (K1p / 2);
-    const float v = m < Mp ? a.G[fidx(b, f, m, j0 + n, Mp, J) + c] : 0.0f;
-    sXT[(f * K1p + 2 * m + c) * 16 + n] = v;
+  // One float4 = two latitudes x (re, im); all loads are issued before the LDS writes.
+  {
+    constexpr int NE = 5 * (K1p / 2) * 8;
+    constexpr int NQ = (NE + 255) / 256;
+    v4f gv[NQ];
+#pragma unroll
+    for (int u = 0; u < NQ; ++u) {
+      const int e = tid + 256 * u;
+      const int jq = e & 7, fm = e >> 3;
+      const int f = fm / (K1p / 2) < 5 ? fm / (K1p / 2) : 4, m = fm % (K1p / 2);
+      const v4f v = *reinterpret_cast<const v4f*>(a.G + fidx(b, f, m < Mp ? m : Mp - 1, j0 + 2 * jq, Mp, J));
+      gv[u] = (e < NE && m < Mp) ? v : v4f{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int u = 0; u < NQ; ++u) {
+      const int e = tid + 256 * u;
+      if (e < NE) {
+        const int jq = e & 7, fm = e >> 3;
+        const int f = fm / (K1p / 2), m = fm % (K1p / 2);
+        float* xr = sXT + (f * K1p + 2 * m) * 16 + 2 * jq;
+        *reinterpret_cast<float2*>(xr) = make_float2(gv[u][0], gv[u][2]);
+        *reinterpret_cast<float2*>(xr + 16) = make_float2(gv[u][1], gv[u][3]);
+      }
+    }
   }
   __syncthreads();
 
@@ -91,18 +112,23 @@ __global__ void __launch_bounds__(256) k_grid_mfma(GridArgs<float> a, const floa
     v4f c1[5];
 #pragma unroll
     for (int f = 0; f < 5; ++f) c1[f] = v4f{0.f, 0.f, 0.f, 0.f};
-    float afr[K1p / 4];
+    v4f afr[KQ1 / 4];
     {
-      const float* Ta = TinvT + (long long)(ib + ln) * K1p + lg;
+      const v4f* Ta = reinterpret_cast<const v4f*>(TinvF + ((long long)(ib / 16) * 64 + lane) * KQ1);
 #pragma unroll
-      for (int ks = 0; ks < K1p / 4; ++ks) afr[ks] = Ta[4 * ks];
+      for (int q4 = 0; q4 < KQ1 / 4; ++q4) afr[q4] = Ta[q4];
     }
+    // First k-group of this chunk's forward-DFT fragments, in flight during GEMM 1 and the physics.
+    const v4f* Tf = reinterpret_cast<const v4f*>(TfwdF) + lane;
+    v4f an[NT2];
+#pragma unroll
+    for (int t = 0; t < NT2; ++t) an[t] = Tf[((long long)t * (I / 16) + ic / 16) * 64];
 #pragma unroll
     for (int ks = 0; ks < K1p / 4; ++ks) {
       const float* xb = sXT + (4 * ks + lg) * 16 + ln;
 #pragma unroll
       for (int f = 0; f < 5; ++f)
-        c1[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(afr[ks], xb[f * K1p * 16], c1[f], 0, 0, 0);
+        c1[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(afr[ks / 4][ks % 4], xb[f * K1p * 16], c1[f], 0, 0, 0);
     }
     // ---- Grid physics on this lane's 4 points (row n, i = ib + 4 lg + q).
 #pragma unroll
@@ -141,27 +167,28 @@ __global__ void __launch_bounds__(256) k_grid_mfma(GridArgs<float> a, const floa
       for (int f = 0; f < 8; ++f) sYT[(f * ICH + il) * 16 + n] = (float)prod[f];
     }
     __syncthreads();
-    // ---- Phase 2: forward DFT partial sums over this chunk; the A fragments
-    // of step ks+1 are fetched while step ks runs.
-    float an[NT2];
-#pragma unroll
-    for (int t = 0; t < NT2; ++t) an[t] = TfwdT[(long long)(16 * t + ln) * I + ic + lg];
-#pragma unroll 2
-    for (int ks = 0; ks < ICH / 4; ++ks) {
-      const int il = 4 * ks + lg;
-      float ac[NT2];
+    // ---- Phase 2: forward DFT partial sums over this chunk, one k-group (16
+    // longitudes, one float4 of A per 16-row tile) at a time; the next group's
+    // fragments are fetched while the current group's MFMAs run.
+#pragma unroll 1
+    for (int gg = 0; gg < ICH / 16; ++gg) {
+      v4f ac[NT2];
 #pragma unroll
       for (int t = 0; t < NT2; ++t) ac[t] = an[t];
-      if (ks + 1 < ICH / 4) {
+      if (gg + 1 < ICH / 16) {
 #pragma unroll
-        for (int t = 0; t < NT2; ++t) an[t] = TfwdT[(long long)(16 * t + ln) * I + ic + il + 4];
+        for (int t = 0; t < NT2; ++t) an[t] = Tf[((long long)t * (I / 16) + ic / 16 + gg + 1) * 64];
       }
-      const float b0 = sYT[((2 * wave) * ICH + il) * 16 + ln];
-      const float b1 = sYT[((2 * wave + 1) * ICH + il) * 16 + ln];
 #pragma unroll
-      for (int t = 0; t < NT2; ++t) {
-        acc[2 * t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[t], b0, acc[2 * t], 0, 0, 0);
-        acc[2 * t + 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[t], b1, acc[2 * t + 1], 0, 0, 0);
+      for (int q = 0; q < 4; ++q) {
+        const int il = 16 * gg + 4 * q + lg;
+        const float b0 = sYT[((2 * wave) * ICH + il) * 16 + ln];
+        const float b1 = sYT[((2 * wave + 1) * ICH + il) * 16 + ln];
+#pragma unroll
+        for (int t = 0; t < NT2; ++t) {
+          acc[2 * t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[t][q], b0, acc[2 * t], 0, 0, 0);
+          acc[2 * t + 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[t][q], b1, acc[2 * t + 1], 0, 0, 0);
+        }
       }
     }
     __syncthreads();
@@ -223,11 +250,11 @@ __global__ void __launch_bounds__(256) k_grid_mfma(GridArgs<float> a, const floa
 }
 
 template <int MT>
-static hipError_t launch_mfma_t(const GridArgs<float>& a, const float* TinvT, const float* TfwdT, hipStream_t s) {
+static hipError_t launch_mfma_t(const GridArgs<float>& a, const float* TinvF, const float* TfwdF, hipStream_t s) {
   using S = MfmaShape<MT>;
   const size_t smem = sizeof(float) * S::LDS_FLOATS;
   dim3 grid((unsigned)((long long)a.B * a.J / S::ROWS));
-  hipLaunchKernelGGL(k_grid_mfma<MT>, grid, dim3(256), smem, s, a, TinvT, TfwdT);
+  hipLaunchKernelGGL(k_grid_mfma<MT>, grid, dim3(256), smem, s, a, TinvF, TfwdF);
   return hipGetLastError();
 }
 
@@ -240,11 +267,11 @@ size_t grid_mfma_lds_bytes(int M) {
   }
 }
 
-int grid_mfma_k1p(int M) {
+int grid_mfma_kq1(int M) {
   switch (M) {
-    case 42: return MfmaShape<42>::K1p;
-    case 63: return MfmaShape<63>::K1p;
-    case 106: return MfmaShape<106>::K1p;
+    case 42: return MfmaShape<42>::KQ1;
+    case 63: return MfmaShape<63>::KQ1;
+    case 106: return MfmaShape<106>::KQ1;
     default: return 0;
   }
 }
@@ -258,11 +285,11 @@ int grid_mfma_k2p(int M) {
   }
 }
 
-hipError_t launch_grid_mfma(const GridArgs<float>& a, const float* TinvT, const float* TfwdT, hipStream_t s) {
+hipError_t launch_grid_mfma(const GridArgs<float>& a, const float* TinvF, const float* TfwdF, hipStream_t s) {
   switch (a.M) {
-    case 42: return launch_mfma_t<42>(a, TinvT, TfwdT, s);
-    case 63: return launch_mfma_t<63>(a, TinvT, TfwdT, s);
-    case 106: return launch_mfma_t<106>(a, TinvT, TfwdT, s);
+    case 42: return launch_mfma_t<42>(a, TinvF, TfwdF, s);
+    case 63: return launch_mfma_t<63>(a, TinvF, TfwdF, s);
+    case 106: return launch_mfma_t<106>(a, TinvF, TfwdF, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -78,8 +78,8 @@ struct swx_plan {
   void* Hi = nullptr;
   float* fragP[4] = {nullptr, nullptr, nullptr, nullptr};  // MFMA Legendre tables: fwd P, fwd H, inv P, inv H
   void* tw = nullptr;
-  float* TinvT = nullptr;  // MFMA grid kernel tables (FP32 plans)
-  float* TfwdT = nullptr;
+  float* TinvF = nullptr;  // MFMA grid kernel DFT tables, fragment order (FP32 plans)
+  float* TfwdF = nullptr;
   bool use_mfma = false;       // MFMA grid kernel available for this plan
   bool opt_plain_grid = false;  // SWX_OPT_GRID_KERNEL = 1
   bool opt_full_phys = false;   // SWX_OPT_FULL_PHYS_CARRY = 1
@@ -249,7 +249,7 @@ hipError_t launch_grid_any(swx_plan* p, const GridArgs<R>& g, hipStream_t s) {
   if constexpr (sizeof(R) == 4) {
     const bool mfma_ok = p->use_mfma && !p->opt_plain_grid && g.mode == GRID_STEP && !g.diagnostics && !g.hist_fields && !g.hist_phiext &&
                          !(g.test == 1 && g.expflag);
-    if (mfma_ok) return launch_grid_mfma(g, p->TinvT, p->TfwdT, s);
+    if (mfma_ok) return launch_grid_mfma(g, p->TinvF, p->TfwdF, s);
   }
   return launch_grid<R>(g, s);
 }
@@ -617,20 +617,31 @@ int swx_plan_create(const swx_plan_desc* desc, const double* params, swx_plan** 
       }
     }
     if (p->use_mfma) {
-      const int k1p = grid_mfma_k1p(M), k2p = grid_mfma_k2p(M);
-      std::vector<double> ti((size_t)I * k1p, 0.0), tf((size_t)k2p * I, 0.0);
-      for (int i = 0; i < I; ++i)
-        for (int m = 0; m < Mp; ++m) {
-          const int kk = (int)(((long long)m * i) % I);
-          const double c = std::cos(2.0 * pi * kk / I), sn = std::sin(2.0 * pi * kk / I);
-          const double wm = m == 0 ? 1.0 : 2.0;
-          ti[(size_t)i * k1p + 2 * m] = wm * c;
-          ti[(size_t)i * k1p + 2 * m + 1] = -wm * sn;
-          tf[(size_t)(2 * m) * I + i] = c / I;
-          tf[(size_t)(2 * m + 1) * I + i] = -sn / I;
-        }
-      if ((rc = dalloc(&p->TinvT, ti.size() * 4)) || (rc = dalloc(&p->TfwdT, tf.size() * 4)) ||
-          (rc = upload_real(p->TinvT, ti, 4)) || (rc = upload_real(p->TfwdT, tf, 4))) {
+      const int k1 = 2 * Mp, k2p = grid_mfma_k2p(M), kq1 = grid_mfma_kq1(M);
+      auto tinv = [&](int i, int k) -> double {
+        if (k >= k1) return 0.0;
+        const int m = k / 2;
+        const double ph = 2.0 * pi * (double)(((long long)m * i) % I) / I, wm = m == 0 ? 1.0 : 2.0;
+        return (k & 1) ? -wm * std::sin(ph) : wm * std::cos(ph);
+      };
+      auto tfwd = [&](int kp, int i) -> double {
+        if (kp >= k1) return 0.0;
+        const int m = kp / 2;
+        const double ph = 2.0 * pi * (double)(((long long)m * i) % I) / I;
+        return (kp & 1) ? -std::sin(ph) / I : std::cos(ph) / I;
+      };
+      std::vector<double> ti((size_t)(I / 16) * 64 * kq1), tf((size_t)k2p * I);
+      for (int lane = 0; lane < 64; ++lane) {
+        const int ln = lane & 15, lg = lane >> 4;
+        for (int it = 0; it < I / 16; ++it)
+          for (int ks = 0; ks < kq1; ++ks) ti[((size_t)it * 64 + lane) * kq1 + ks] = tinv(16 * it + ln, 4 * ks + lg);
+        for (int t = 0; t < k2p / 16; ++t)
+          for (int g = 0; g < I / 16; ++g)
+            for (int q = 0; q < 4; ++q)
+              tf[(((size_t)t * (I / 16) + g) * 64 + lane) * 4 + q] = tfwd(16 * t + ln, 16 * g + 4 * q + lg);
+      }
+      if ((rc = dalloc(&p->TinvF, ti.size() * 4)) || (rc = dalloc(&p->TfwdF, tf.size() * 4)) ||
+          (rc = upload_real(p->TinvF, ti, 4)) || (rc = upload_real(p->TfwdF, tf, 4))) {
         swx_plan_destroy(p);
         return rc;
       }
@@ -669,7 +680,7 @@ void swx_plan_destroy(swx_plan* p) {
   if (!p) return;
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   void* bufs[] = {p->Lam, p->Tbar, p->Gbar, p->pgrad, p->Pf, p->Hf, p->Pi, p->Hi, p->fragP[0], p->fragP[1],
-                  p->fragP[2], p->fragP[3], p->tw, p->TinvT, p->TfwdT, p->geo_d, p->f2, p->coef, p->S, p->T, p->G, p->Gprev,
+                  p->fragP[2], p->fragP[3], p->tw, p->TinvF, p->TfwdF, p->geo_d, p->f2, p->coef, p->S, p->T, p->G, p->Gprev,
                   p->raw, p->lag, p->g0uv, p->phys, p->uvic, p->part_uv, p->dead};
   for (void* q : bufs)
     if (q) (void)hipFree(q);
