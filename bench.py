@@ -147,6 +147,32 @@ def cpu_baseline(seconds_target: float = 12.0):
                 sample=f"FP64 numpy oracle, T42 forced modEuler, 1 member, {steps} steps ({steps // 240} runs of 240)")
 
 
+def measured_traffic(kernel: str, members: int, precision: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter passes (FETCH_SIZE,
+    WRITE_SIZE, each scaled by the copy-kernel calibration), or None for another workload."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_T42x{members}_{precision}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    k = rec["kernels"].get(kernel)
+    return None if k is None else int(k["hbm_bytes"])
+
+
+def copy_calibration(mib: int):
+    """A plain copy of known size: FETCH_SIZE / WRITE_SIZE counters read on it give the scale
+    factors applied to the step kernels' counter values."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4, "copies": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,7 +184,12 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=100)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
+                    help="only run a plain device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; members shard across GPUs with no per-step "
                                    "exchange, multi-GPU runs are not provided"}))
@@ -191,8 +222,10 @@ def main():
         "config": {"workload": f"T42x{B} forced modified-Euler forward, dt=120 s, graph-replayed steps",
                    "members": B, "M": M, "J": J, "I": I, "precision": args.precision},
         "roofline": {"bound": "mfma", "achieved": grid_tflops, "peak": peak, "unit": "TFLOP/s",
-                     "frac": grid_tflops / peak, "traffic": None,
-                     "kernel": "k_grid (inverse DFT + grid physics + forward DFT)",
+                     "frac": grid_tflops / peak,
+                     "traffic": measured_traffic(f"k_grid_mfma<{M}>", B, args.precision),
+                     "kernel": f"k_grid_mfma<{M}> (inverse DFT + grid physics + forward DFT)",
+                     "algorithmic_flops_per_launch": dft_f * B,
                      "kernel_ms": main_run["ms_grid"], "leg_kernel_ms": main_run["ms_leg"],
                      "leg_tflops": leg_tflops},
         "finite": main_run["finite"],

@@ -148,9 +148,14 @@ int swx_export_state(swx_plan *plan, const swx_state_out *out, void *stream);
  * jitted lax.scan it replaces (model.py:1281-1342). */
 int swx_run_graph(swx_plan *plan, int t0, int nsteps, int graph_steps, void *stream);
 
-/* Measurement hook: run nsteps steps eagerly with HIP events recorded on
- * `stream` around each launch and return the average duration (ms) of the
- * Legendre kernel and of the grid kernel.  Synchronizes the stream.  No
+/* Measurement hook (HIP events on `stream`): run nsteps steps eagerly
+ * (t0 .. t0+nsteps-1, the steady physical-carry mode of swx_run_graph) timed
+ * as one block, then nsteps back-to-back launches of the grid kernel on the
+ * final state (it reads the Fourier rows and rewrites the same Legendre
+ * inputs, so the state is unchanged).  *ms_grid = grid block / nsteps,
+ * *ms_leg = step block / nsteps - *ms_grid.  When the physical carry is
+ * written every step (|alpha| >= 0.5, diagnostics, or the full-carry option)
+ * each launch is bracketed by events instead.  Synchronizes the stream.  No
  * reference counterpart (the reference times whole scans,
  * scripts/benchmark_scan.py:116-141). */
 int swx_time_kernels(swx_plan *plan, int t0, int nsteps, float *ms_leg, float *ms_grid, void *stream);

@@ -803,22 +803,9 @@ int swx_time_kernels(swx_plan* p, int t0, int nsteps, float* ms_leg, float* ms_g
   if (!p->initialized) return fail(SWX_ERR_STATE, "swx_time_kernels before swx_init");
   if (nsteps <= 0 || !ms_leg || !ms_grid) return fail(SWX_ERR_ARG, "nsteps must be > 0 and outputs non-null");
   hipStream_t s = (hipStream_t)stream;
-  hipEvent_t ev[3];
-  for (auto& e : ev) SWX_HIP(hipEventCreate(&e));
-  double tl = 0.0, tg = 0.0;
-  for (int k = 0; k < nsteps; ++k) {
-    const int t = t0 + k;
-    SWX_HIP(hipEventRecord(ev[0], s));
-    if (p->rsz == 8) {
-      LegArgs<double> l = leg_args<double>(p, LEG_STEP);
-      SWX_HIP(launch_leg<double>(l, s));
-    } else {
-      LegArgs<float> l = leg_args<float>(p, LEG_STEP);
-      SWX_HIP(launch_leg_any<float>(p, l, s));
-    }
-    SWX_HIP(hipEventRecord(ev[1], s));
-    // Same physical-carry mode as the steady steps of swx_run_graph.
-    const int phys_on = phys_window(p) >= (1 << 30) ? 1 : 0;
+  // Same physical-carry mode as the steady steps of swx_run_graph.
+  const int phys_on = phys_window(p) >= (1 << 30) ? 1 : 0;
+  auto launch_grid_step = [&](int t) -> int {
     if (p->rsz == 8) {
       GridArgs<double> g = grid_args<double>(p, GRID_STEP);
       g.do_ra = (p->d.modalflag && t > 2) ? 1 : 0;
@@ -830,17 +817,56 @@ int swx_time_kernels(swx_plan* p, int t0, int nsteps, float* ms_leg, float* ms_g
       g.phys_on = phys_on;
       SWX_HIP(launch_grid_any<float>(p, g, s));
     }
+    return SWX_OK;
+  };
+  auto launch_leg_step = [&]() -> int {
+    if (p->rsz == 8) {
+      LegArgs<double> l = leg_args<double>(p, LEG_STEP);
+      SWX_HIP(launch_leg<double>(l, s));
+    } else {
+      LegArgs<float> l = leg_args<float>(p, LEG_STEP);
+      SWX_HIP(launch_leg_any<float>(p, l, s));
+    }
+    return SWX_OK;
+  };
+  hipEvent_t ev[4];
+  for (auto& e : ev) SWX_HIP(hipEventCreate(&e));
+  double step_ms = 0.0, grid_ms = 0.0;
+  if (!phys_on) {
+    SWX_HIP(hipEventRecord(ev[0], s));
+    for (int k = 0; k < nsteps && rc == SWX_OK; ++k)
+      if ((rc = launch_leg_step()) == SWX_OK) rc = launch_grid_step(t0 + k);
+    SWX_HIP(hipEventRecord(ev[1], s));
+    // With the physical carry off the grid launch reads G and rewrites the same T:
+    // repeating it leaves the state as it is.
+    for (int k = 0; k < nsteps && rc == SWX_OK; ++k) rc = launch_grid_step(t0 + nsteps - 1);
     SWX_HIP(hipEventRecord(ev[2], s));
     SWX_HIP(hipEventSynchronize(ev[2]));
     float a = 0.f, b = 0.f;
     SWX_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
     SWX_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
-    tl += a;
-    tg += b;
+    step_ms = a / nsteps;
+    grid_ms = b / nsteps;
+  } else {
+    // Carry written every step (repeats would advance the RA filter): events around each launch.
+    for (int k = 0; k < nsteps && rc == SWX_OK; ++k) {
+      SWX_HIP(hipEventRecord(ev[0], s));
+      if ((rc = launch_leg_step()) != SWX_OK) break;
+      SWX_HIP(hipEventRecord(ev[1], s));
+      if ((rc = launch_grid_step(t0 + k)) != SWX_OK) break;
+      SWX_HIP(hipEventRecord(ev[2], s));
+      SWX_HIP(hipEventSynchronize(ev[2]));
+      float a = 0.f, b = 0.f;
+      SWX_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+      SWX_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
+      step_ms += (double)(a + b) / nsteps;
+      grid_ms += (double)b / nsteps;
+    }
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
-  *ms_leg = (float)(tl / nsteps);
-  *ms_grid = (float)(tg / nsteps);
+  if (rc) return rc;
+  *ms_grid = (float)grid_ms;
+  *ms_leg = (float)(step_ms - grid_ms > 0.0 ? step_ms - grid_ms : 0.0);
   return SWX_OK;
 }
 

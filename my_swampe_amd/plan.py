@@ -186,7 +186,9 @@ class Plan:
         return lam, pg
 
     def time_kernels(self, t0: int, nsteps: int, stream=None):
-        """Average (ms) of the Legendre and grid kernels over nsteps eager steps (HIP events)."""
+        """Average (ms) of the Legendre and grid kernels: nsteps eager steps timed as one block
+        (HIP events), then nsteps back-to-back grid launches on the final state (state unchanged);
+        the Legendre figure is the step average minus the grid average."""
         a, b = ctypes.c_float(), ctypes.c_float()
         _capi.check(self._lib.swx_time_kernels(self._h, int(t0), int(nsteps), ctypes.byref(a), ctypes.byref(b),
                                                self._stream(stream)), "swx_time_kernels")
