@@ -33,14 +33,14 @@ PEAK = {"fp32": 157.3, "fp64": 78.6}  # TFLOP/s, MI355X dense matrix peaks
 HBM_PEAK = 8000.0  # GB/s
 
 
-def member_params(B: int, seed: int = 7):
+def member_params(B: int, seed: int = 7, dt: float = 120.0):
     from my_swampe_amd.model import param_row
 
     rng = np.random.default_rng(seed)
     taurad = 10.0 ** rng.uniform(4.0, 6.0, B)
     taudrag = 10.0 ** rng.uniform(4.0, 6.0, B)
     dphieq = rng.uniform(1e5, 5e5, B)
-    return np.stack([param_row(dt=120.0, a=8.2e7, omega=3.2e-5, g=9.8, Phibar=3e5, taurad=taurad[b],
+    return np.stack([param_row(dt=dt, a=8.2e7, omega=3.2e-5, g=9.8, Phibar=3e5, taurad=taurad[b],
                                taudrag=taudrag[b], DPhieq=dphieq[b], K6=1.24e33, K6Phi=None, alpha=0.01)
                      for b in range(B)])
 
@@ -128,6 +128,38 @@ def run_fwd_adjoint(M, B, precision, steps, reps=2):
     return dict(wall=float(np.median(times)), finite=finite)
 
 
+def run_t106_fwd_adjoint(B=64, steps=5000, every=70, precision="fp32"):
+    """T106 x 64 members, dt = 60 s, 5000 steps forward + adjoint with a spectral checkpoint every
+    70 steps: wall time and peak device memory (torch-held checkpoints and segment states at their
+    peak plus the plan's own buffers)."""
+    import torch
+    from my_swampe_amd.autograd import rollout_phi
+
+    dev = torch.device("cuda")
+    params = member_params(B, dt=60.0)
+    c = torch.tensor(spectral_ics(106, B), device=dev).requires_grad_(True)
+    tr, td, dp = (torch.tensor(params[:, k], device=dev).requires_grad_(True) for k in (5, 6, 7))
+    tgt = torch.zeros((160, 320), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats()
+    base = torch.cuda.memory_allocated()
+    stats = {}
+    t0 = time.perf_counter()
+    phi = rollout_phi(c, tr, td, dp, M=106, nsteps=steps, dt=60.0, a=8.2e7, omega=3.2e-5, Phibar=3e5,
+                      precision=precision, checkpoint_every=every, stats=stats)
+    loss = ((phi - tgt) ** 2).mean(dim=(1, 2)).sum()
+    loss.backward()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    torch_peak = torch.cuda.max_memory_allocated() - base
+    finite = bool(torch.isfinite(c.grad).all()) and bool(torch.isfinite(tr.grad).all())
+    return dict(members=B, steps=steps, dt=60.0, checkpoint_every=every, precision=precision, wall_s=wall,
+                member_steps_per_s=steps * B / wall, checkpoints=stats.get("checkpoints"),
+                segment_states=stats.get("segment_states"), torch_peak_bytes=int(torch_peak),
+                plan_device_bytes=int(stats.get("plan_device_bytes", 0)),
+                peak_hbm_bytes=int(torch_peak + stats.get("plan_device_bytes", 0)), finite=finite)
+
+
 def cpu_baseline(seconds_target: float = 12.0):
     """FP64 oracle (numpy, one core): T42, 1 member, forced modified Euler, dt=120 s, runs of 240 steps."""
     from oracle import swampe_oracle as orc
@@ -184,6 +216,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=100)
+    ap.add_argument("--no-t106", action="store_true", help="skip the T106 x 64, 5000-step fwd+adjoint run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only run a plain device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -258,6 +291,8 @@ def main():
             "plain_fp64_1member_wall_s": pa["wall"], "plain_fp64_1member_member_steps_per_s": pa_v,
             "ratio_vs_plain_fp64_1member": fa_v / pa_v,
         }
+    if not args.no_t106:
+        out["t106_fwd_adjoint"] = run_t106_fwd_adjoint(precision=args.precision)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

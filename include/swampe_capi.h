@@ -148,6 +148,11 @@ int swx_export_state(swx_plan *plan, const swx_state_out *out, void *stream);
  * jitted lax.scan it replaces (model.py:1281-1342). */
 int swx_run_graph(swx_plan *plan, int t0, int nsteps, int graph_steps, void *stream);
 
+/* Device memory (bytes) the plan holds: basis and DFT tables, carry, work
+ * buffers, and the adjoint workspace once allocated.  Caller-owned tensors
+ * (checkpoints, outputs) are not included.  No reference counterpart. */
+int swx_plan_device_bytes(const swx_plan *plan, size_t *bytes);
+
 /* Measurement hook (HIP events on `stream`): run nsteps steps eagerly
  * (t0 .. t0+nsteps-1, the steady physical-carry mode of swx_run_graph) timed
  * as one block, then nsteps back-to-back launches of the grid kernel on the

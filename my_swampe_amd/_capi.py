@@ -33,7 +33,7 @@ EXPORTED_SYMBOLS = (
     "swx_geometry", "swx_grid_size", "swx_plan_create", "swx_plan_set_params", "swx_plan_destroy",
     "swx_init", "swx_run", "swx_export_state", "swx_run_graph", "swx_time_kernels", "swx_plan_set_option",
     "swx_init_spectral", "swx_state_get", "swx_state_set", "swx_adjoint_begin", "swx_adjoint_terminal",
-    "swx_adjoint_step", "swx_adjoint_result", "swx_last_error", "swx_version",
+    "swx_adjoint_step", "swx_adjoint_result", "swx_last_error", "swx_version", "swx_plan_device_bytes",
 )
 
 STATE_FIELDS = (
@@ -73,6 +73,11 @@ def load() -> ctypes.CDLL:
             f"{LIB_PATH} is missing: build it with `python -m my_swampe_amd.build` "
             "(or __graft_entry__.build()). There is no CPU fallback."
         )
+    # The library needs libamdhip64.so.7 by soname.  torch ships its own copy under that soname:
+    # loading torch first makes the one runtime torch uses serve the library too (loading the
+    # library first would put a second HIP runtime beside torch's, and device calls then fail).
+    import torch  # noqa: F401
+
     lib = ctypes.CDLL(str(LIB_PATH))
     lib.swx_last_error.restype = c_char_p
     lib.swx_version.restype = c_char_p
@@ -95,12 +100,13 @@ def load() -> ctypes.CDLL:
     lib.swx_adjoint_step.argtypes = [c_void_p, c_void_p, c_void_p]
     lib.swx_adjoint_result.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p]
     lib.swx_time_kernels.argtypes = [c_void_p, c_int, c_int, POINTER(ctypes.c_float), POINTER(ctypes.c_float), c_void_p]
+    lib.swx_plan_device_bytes.argtypes = [c_void_p, POINTER(ctypes.c_size_t)]
     for name in EXPORTED_SYMBOLS:
         getattr(lib, name)  # AttributeError when a declared symbol is missing
     for name in ("swx_geometry", "swx_grid_size", "swx_plan_create", "swx_plan_set_params", "swx_init",
                  "swx_run", "swx_export_state", "swx_run_graph", "swx_time_kernels", "swx_plan_set_option",
                  "swx_init_spectral", "swx_state_get", "swx_state_set", "swx_adjoint_begin",
-                 "swx_adjoint_terminal", "swx_adjoint_step", "swx_adjoint_result"):
+                 "swx_adjoint_terminal", "swx_adjoint_step", "swx_adjoint_result", "swx_plan_device_bytes"):
         getattr(lib, name).restype = c_int
     _lib = lib
     return lib

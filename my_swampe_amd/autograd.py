@@ -146,6 +146,7 @@ class _Rollout:
                 plan, cfg = ctx.plan, ctx.cfg
                 lam, pg = adjoint_through_checkpoints(plan, cfg, ctx.ckpts, gPhi.contiguous())
                 torch.cuda.current_stream(plan.device).synchronize()
+                cfg.stats["plan_device_bytes"] = plan.device_bytes()
                 plan.close()
                 ctx.plan = None
                 ctx.ckpts = None

@@ -109,16 +109,20 @@ struct swx_plan {
   hipGraphExec_t gexec = nullptr;
   int gexec_steps = 0;
   hipStream_t gexec_stream = nullptr;
+  size_t dev_bytes = 0;  // device memory held by the plan
 };
 
 namespace {
 
+// Device allocation owned by `owner` (counted in owner->dev_bytes).
 template <typename T>
-int dalloc(T** p, size_t bytes) {
+int dalloc(swx_plan* owner, T** p, size_t bytes) {
   void* q = nullptr;
-  hipError_t e = hipMalloc(&q, bytes < 16 ? 16 : bytes);
+  const size_t nb = bytes < 16 ? 16 : bytes;
+  hipError_t e = hipMalloc(&q, nb);
   if (e != hipSuccess) return fail(SWX_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
   *p = static_cast<T*>(q);
+  owner->dev_bytes += nb;
   return SWX_OK;
 }
 
@@ -386,9 +390,9 @@ int ensure_adjoint(swx_plan* p) {
   if (p->Lam) return SWX_OK;
   const int B = p->d.n_members, J = p->geo.J, Mp = p->geo.M + 1, Np = p->geo.N + 1;
   int rc;
-  if ((rc = dalloc(&p->Lam, (size_t)B * 3 * Mp * Np * 2 * 8)) ||
-      (rc = dalloc(&p->Tbar, (size_t)B * Mp * kNT * J * 2 * p->rsz)) ||
-      (rc = dalloc(&p->Gbar, (size_t)B * kNG * J * Mp * 2 * p->rsz)) || (rc = dalloc(&p->pgrad, (size_t)B * J * 3 * 8)))
+  if ((rc = dalloc(p, &p->Lam, (size_t)B * 3 * Mp * Np * 2 * 8)) ||
+      (rc = dalloc(p, &p->Tbar, (size_t)B * Mp * kNT * J * 2 * p->rsz)) ||
+      (rc = dalloc(p, &p->Gbar, (size_t)B * kNG * J * Mp * 2 * p->rsz)) || (rc = dalloc(p, &p->pgrad, (size_t)B * J * 3 * 8)))
     return rc;
   return SWX_OK;
 }
@@ -562,16 +566,16 @@ int swx_plan_create(const swx_plan_desc* desc, const double* params, swx_plan** 
   int rc = SWX_OK;
   const size_t rs = p->rsz;
   const size_t cplxJM = (size_t)J * Mp * 2;
-  if ((rc = dalloc(&p->Pf, tri * rs)) || (rc = dalloc(&p->Hf, tri * rs)) || (rc = dalloc(&p->Pi, tri * rs)) ||
-      (rc = dalloc(&p->Hi, tri * rs)) || (rc = dalloc(&p->tw, 2 * I * rs)) ||
-      (rc = dalloc(&p->geo_d, geo.size() * 8)) || (rc = dalloc(&p->coef, coef.size() * 8)) ||
-      (rc = dalloc(&p->f2, f2.size() * 8)) ||
-      (rc = dalloc(&p->S, (size_t)B * 2 * 3 * Mp * (N + 1) * 2 * 8)) ||
-      (rc = dalloc(&p->T, (size_t)B * Mp * kNT * J * 2 * rs)) || (rc = dalloc(&p->G, (size_t)B * kNG * cplxJM * rs)) ||
-      (rc = dalloc(&p->Gprev, (size_t)B * kNG * cplxJM * rs)) || (rc = dalloc(&p->raw, (size_t)B * kNRaw * cplxJM * rs)) ||
-      (rc = dalloc(&p->lag, (size_t)B * 2 * cplxJM * rs)) || (rc = dalloc(&p->g0uv, (size_t)B * 2 * cplxJM * rs)) ||
-      (rc = dalloc(&p->phys, (size_t)B * 8 * J * I * 8)) || (rc = dalloc(&p->uvic, (size_t)B * 2 * J * I * 8)) ||
-      (rc = dalloc(&p->part_uv, (size_t)B * J * 2 * 8)) || (rc = dalloc(&p->dead, (size_t)B))) {
+  if ((rc = dalloc(p, &p->Pf, tri * rs)) || (rc = dalloc(p, &p->Hf, tri * rs)) || (rc = dalloc(p, &p->Pi, tri * rs)) ||
+      (rc = dalloc(p, &p->Hi, tri * rs)) || (rc = dalloc(p, &p->tw, 2 * I * rs)) ||
+      (rc = dalloc(p, &p->geo_d, geo.size() * 8)) || (rc = dalloc(p, &p->coef, coef.size() * 8)) ||
+      (rc = dalloc(p, &p->f2, f2.size() * 8)) ||
+      (rc = dalloc(p, &p->S, (size_t)B * 2 * 3 * Mp * (N + 1) * 2 * 8)) ||
+      (rc = dalloc(p, &p->T, (size_t)B * Mp * kNT * J * 2 * rs)) || (rc = dalloc(p, &p->G, (size_t)B * kNG * cplxJM * rs)) ||
+      (rc = dalloc(p, &p->Gprev, (size_t)B * kNG * cplxJM * rs)) || (rc = dalloc(p, &p->raw, (size_t)B * kNRaw * cplxJM * rs)) ||
+      (rc = dalloc(p, &p->lag, (size_t)B * 2 * cplxJM * rs)) || (rc = dalloc(p, &p->g0uv, (size_t)B * 2 * cplxJM * rs)) ||
+      (rc = dalloc(p, &p->phys, (size_t)B * 8 * J * I * 8)) || (rc = dalloc(p, &p->uvic, (size_t)B * 2 * J * I * 8)) ||
+      (rc = dalloc(p, &p->part_uv, (size_t)B * J * 2 * 8)) || (rc = dalloc(p, &p->dead, (size_t)B))) {
     swx_plan_destroy(p);
     return rc;
   }
@@ -610,7 +614,7 @@ int swx_plan_create(const swx_plan_desc* desc, const double* params, swx_plan** 
         }
       }
       for (int q = 0; q < 4 && rc == SWX_OK; ++q)
-        if ((rc = dalloc(&p->fragP[q], per * 4)) == SWX_OK) rc = upload_real(p->fragP[q], fr[q], 4);
+        if ((rc = dalloc(p, &p->fragP[q], per * 4)) == SWX_OK) rc = upload_real(p->fragP[q], fr[q], 4);
       if (rc) {
         swx_plan_destroy(p);
         return rc;
@@ -640,7 +644,7 @@ int swx_plan_create(const swx_plan_desc* desc, const double* params, swx_plan** 
             for (int q = 0; q < 4; ++q)
               tf[(((size_t)t * (I / 16) + g) * 64 + lane) * 4 + q] = tfwd(16 * t + ln, 16 * g + 4 * q + lg);
       }
-      if ((rc = dalloc(&p->TinvF, ti.size() * 4)) || (rc = dalloc(&p->TfwdF, tf.size() * 4)) ||
+      if ((rc = dalloc(p, &p->TinvF, ti.size() * 4)) || (rc = dalloc(p, &p->TfwdF, tf.size() * 4)) ||
           (rc = upload_real(p->TinvF, ti, 4)) || (rc = upload_real(p->TfwdF, tf, 4))) {
         swx_plan_destroy(p);
         return rc;
@@ -867,6 +871,12 @@ int swx_time_kernels(swx_plan* p, int t0, int nsteps, float* ms_leg, float* ms_g
   if (rc) return rc;
   *ms_grid = (float)grid_ms;
   *ms_leg = (float)(step_ms - grid_ms > 0.0 ? step_ms - grid_ms : 0.0);
+  return SWX_OK;
+}
+
+int swx_plan_device_bytes(const swx_plan* p, size_t* bytes) {
+  if (!p || !bytes) return fail(SWX_ERR_ARG, "null plan or output");
+  *bytes = p->dev_bytes;
   return SWX_OK;
 }
 

@@ -185,6 +185,12 @@ class Plan:
                     "swx_adjoint_result")
         return lam, pg
 
+    def device_bytes(self) -> int:
+        """Device memory held by the plan (tables, carry, work buffers, adjoint workspace)."""
+        n = ctypes.c_size_t()
+        _capi.check(self._lib.swx_plan_device_bytes(self._h, ctypes.byref(n)), "swx_plan_device_bytes")
+        return int(n.value)
+
     def time_kernels(self, t0: int, nsteps: int, stream=None):
         """Average (ms) of the Legendre and grid kernels: nsteps eager steps timed as one block
         (HIP events), then nsteps back-to-back grid launches on the final state (state unchanged);

@@ -1,0 +1,194 @@
+"""Long runs on the GPU against the FP64 oracle and against finite differences.
+
+* T42, one member, 2000 forced modified-Euler steps (tmax = 2002, dt = 120 s, a = 8.2e7,
+  omega = 3.2e-5, Phibar = 3e5, DPhieq = 3e5, taurad = taudrag = 86400 s, K6 = 1.24e33,
+  rest + seeded Phi noise, seed 0): the FP64 HIP path and the FP32-transform path against
+  the FP64 oracle, relative L2 and max-abs of the terminal eta, delta, Phi.
+* T63, one member, 2000 steps, FP64: gradients of mean((Phi_T - Phi_target)^2) w.r.t.
+  taurad, taudrag, DPhieq (at 86400, 86400, 3e5) and the spectral initial state, target from
+  a run with taurad = 5e4, taudrag = 2e5; checked against central differences (parameters:
+  h = 1e-6 |theta|; initial-state direction: fourth order, h = 1e-4), relative 1e-6.
+* T42, 4 members, 100 steps: gradients through the FP32-transform path within 1e-4 (relative
+  L2) of the FP64 gradients.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import swampe_oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+COMMON = dict(a=8.2e7, omega=3.2e-5, Phibar=3e5, g=9.8, K6=1.24e33, alpha=0.01, forcflag=True, diffflag=True,
+              modalflag=True, expflag=False)
+
+
+def _print_case(tag, **kw):
+    print(f"[gpu-case] {tag} {kw}", flush=True)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def t42_2000(torch_gpu):
+    ics = orc.rest_noise_ic(42, 0, COMMON["omega"])
+    kw = dict(M=42, dt=120.0, tmax=2002, Phibar=COMMON["Phibar"], omega=COMMON["omega"], a=COMMON["a"],
+              taurad=86400.0, taudrag=86400.0, DPhieq=3e5, K6=COMMON["K6"], alpha=COMMON["alpha"],
+              eta0_init=ics[0], delta0_init=ics[1], Phi0_init=ics[2], U0_init=ics[3], V0_init=ics[4])
+    ref = orc.run_model_scan(**kw, diagnostics=False, return_history=False)["last_state"]
+    return kw, ref
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_t42_2000_steps_vs_oracle(t42_2000, precision):
+    """Relative L2 of the terminal fields after 2000 steps (the short-run atols do not apply: Phi
+    reaches 2e5 here).  FP64: <= 1e-9 (FP64 rounding of two summation orders, amplified over
+    2000 steps).  FP32 transforms: <= 1e-5 for Phi and eta, <= 1e-3 for delta (a small, noisy
+    field)."""
+    from my_swampe_amd.model import run_model_scan_final
+
+    kw, ref = t42_2000
+    _print_case("t42-2000", precision=precision)
+    got = run_model_scan_final(**kw, precision=precision)["last_state"]
+    res = {}
+    for f in ("eta_curr", "delta_curr", "Phi_curr"):
+        g = getattr(got, f).cpu().numpy()
+        r = getattr(ref, f)
+        res[f] = (_rel(g, r), float(np.abs(g - r).max()), bool(np.isfinite(g).all()))
+        print(precision, f, "relL2", res[f][0], "maxabs", res[f][1], "scale", float(np.abs(r).max()))
+    for f, (rel, mx, finite) in res.items():
+        assert finite, f
+        if precision == "fp64":
+            assert rel <= 1e-9, (f, rel, mx)
+        else:
+            assert rel <= (1e-3 if f == "delta_curr" else 1e-5), (f, rel)
+
+
+# ---------------------------------------------------------------------------------------------
+# T63 2000-step FP64 adjoint
+
+T63 = dict(M=63, nsteps=2000, dt=120.0, a=8.2e7, omega=3.2e-5, Phibar=3e5)
+THETA = {"tr": 86400.0, "td": 86400.0, "dp": 3e5}
+
+
+def _spectral_rest_noise(M, seed, omega, amp=30.0, nmax=10):
+    """eta = planetary vorticity (the m=0, n=1 coefficient of 2 omega mu), delta = 0, Phi = seeded
+    noise on n <= nmax (m = 0 real)."""
+    rng = np.random.default_rng(seed)
+    c = np.zeros((1, 3, M + 1, M + 1), complex)
+    c[0, 0, 0, 1] = omega / np.sqrt(0.375)
+    for m in range(nmax + 1):
+        for n in range(m, nmax + 1):
+            c[0, 2, m, n] = amp * (rng.standard_normal() + (1j * rng.standard_normal() if m else 0.0))
+    return c
+
+
+@pytest.fixture(scope="module")
+def t63_adjoint(torch_gpu):
+    import torch
+
+    from my_swampe_amd.autograd import rollout_phi
+
+    _print_case("t63-2000-adjoint", **T63)
+    dev = torch.device("cuda")
+    c0 = torch.tensor(_spectral_rest_noise(63, 0, T63["omega"]), device=dev)
+    with torch.no_grad():
+        tgt = rollout_phi(c0, 5e4, 2e5, 3e5, **T63)
+    c = c0.clone().requires_grad_(True)
+    tr, td, dp = (torch.tensor([THETA[k]], dtype=torch.float64, device=dev, requires_grad=True)
+                  for k in ("tr", "td", "dp"))
+    phi = rollout_phi(c, tr, td, dp, **T63)
+    L = ((phi - tgt) ** 2).mean()
+    L.backward()
+    return dict(c0=c0, tgt=tgt, L=float(L.detach()), gc=c.grad.detach(), g={"tr": float(tr.grad), "td": float(td.grad),
+                                                                            "dp": float(dp.grad)})
+
+
+def _loss63(torch, res, c, tr, td, dp):
+    from my_swampe_amd.autograd import rollout_phi
+
+    with torch.no_grad():
+        phi = rollout_phi(c, tr, td, dp, **T63)
+        return float(((phi - res["tgt"]) ** 2).mean())
+
+
+def test_t63_2000_parameter_gradients_vs_central_differences(t63_adjoint):
+    import torch
+
+    res = t63_adjoint
+    for name in ("tr", "td", "dp"):
+        h = 1e-6 * abs(THETA[name])
+
+        def at(dh):
+            p = dict(THETA)
+            p[name] += dh
+            return _loss63(torch, res, res["c0"], p["tr"], p["td"], p["dp"])
+
+        fd = (at(h) - at(-h)) / (2 * h)
+        ad = res["g"][name]
+        rel = abs(ad - fd) / abs(fd)
+        print(name, "loss", res["L"], "adjoint", ad, "fd", fd, "rel", rel)
+        assert rel < 1e-6, (name, ad, fd, rel)
+
+
+def test_t63_2000_initial_state_gradient_vs_directional_difference(t63_adjoint):
+    import torch
+
+    res = t63_adjoint
+    rng = np.random.default_rng(3)
+    M = 63
+    v = np.zeros((1, 3, M + 1, M + 1), complex)
+    for f, amp in ((0, 1e-6), (1, 1e-7), (2, 30.0)):
+        v[:, f] = amp * (rng.standard_normal((1, M + 1, M + 1)) + 1j * rng.standard_normal((1, M + 1, M + 1)))
+    v[:, :, 0, :] = v[:, :, 0, :].real
+    v *= np.triu(np.ones((M + 1, M + 1)))[None, None]
+    vt = torch.tensor(v, device=res["c0"].device)
+
+    def at(h):
+        return _loss63(torch, res, res["c0"] + h * vt, THETA["tr"], THETA["td"], THETA["dp"])
+
+    # Fourth-order central difference: over 2000 steps the rounding noise of the loss,
+    # divided by a 1e-6 step, reaches 1e-6 relative; h = 1e-4 keeps it below 1e-8.
+    h = 1e-4
+    fd = (8.0 * (at(h) - at(-h)) - (at(2 * h) - at(-2 * h))) / (12.0 * h)
+    g = res["gc"]
+    ad = float((g.real * vt.real + g.imag * vt.imag).sum())
+    rel = abs(ad - fd) / abs(fd)
+    print("direction adjoint", ad, "fd", fd, "rel", rel)
+    assert rel < 1e-6, (ad, fd, rel)
+
+
+# ---------------------------------------------------------------------------------------------
+# FP32-transform adjoint vs the FP64 adjoint
+
+def test_fp32_gradients_within_1e4_of_fp64(torch_gpu):
+    """T42, 4 members with different forcing parameters, 100 steps (dt = 120 s, a = 8.2e7):
+    gradients of sum_b mean((Phi_T,b)^2) through the FP32-transform path vs the FP64 path,
+    relative L2 per gradient <= 1e-4."""
+    import torch
+
+    from my_swampe_amd.autograd import rollout_phi
+
+    _print_case("fp32-vs-fp64-gradients", M=42, nsteps=100, members=4)
+    dev = torch.device("cuda")
+    rng = np.random.default_rng(9)
+    B = 4
+    c0 = np.concatenate([_spectral_rest_noise(42, 100 + b, 3.2e-5) for b in range(B)])
+    tr0, td0 = 10.0 ** rng.uniform(4.5, 5.5, B), 10.0 ** rng.uniform(4.5, 5.5, B)
+    dp0 = rng.uniform(1e5, 5e5, B)
+    out = {}
+    for precision in ("fp64", "fp32"):
+        c = torch.tensor(c0, device=dev, requires_grad=True)
+        tr, td, dp = (torch.tensor(x, device=dev, requires_grad=True) for x in (tr0, td0, dp0))
+        phi = rollout_phi(c, tr, td, dp, M=42, nsteps=100, dt=120.0, a=8.2e7, omega=3.2e-5, Phibar=3e5,
+                          precision=precision)
+        (phi ** 2).mean(dim=(1, 2)).sum().backward()
+        out[precision] = [x.grad.detach().cpu().numpy() for x in (c, tr, td, dp)]
+    for name, g32, g64 in zip(("c", "taurad", "taudrag", "DPhieq"), out["fp32"], out["fp64"]):
+        rel = float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64))
+        print(name, "fp32 vs fp64 gradient relL2", rel)
+        assert rel <= 1e-4, (name, rel)
