@@ -128,6 +128,50 @@ def run_fwd_adjoint(M, B, precision, steps, reps=2):
     return dict(wall=float(np.median(times)), finite=finite)
 
 
+def run_mixed(steps, warmup, B42=256, B63=64, graph_steps=32):
+    """256 x T42 + 64 x T63 members, FP32 transforms, dt = 120 s, per-member forcing parameters
+    (seeds 7 and 8): both truncations in one Legendre and one grid launch per step (Ensemble),
+    and the same two plans stepped one after the other for comparison."""
+    import torch
+    from my_swampe_amd.plan import Ensemble, Plan
+
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    walls = {}
+    with torch.cuda.stream(stream):
+        plans = [Plan(42, member_params(B42, seed=7), precision="fp32"),
+                 Plan(63, member_params(B63, seed=8), precision="fp32")]
+        for pl, (M, B) in zip(plans, ((42, B42), (63, B63))):
+            pl.init_spectral(torch.tensor(spectral_ics(M, B), device="cuda"), stream=stream)
+        ens = Ensemble(plans[0], plans[1])
+        ens.run_graph(2, warmup, graph_steps=graph_steps, stream=stream)
+        stream.synchronize()
+        t0 = time.perf_counter()
+        ens.run_graph(2 + warmup, steps, graph_steps=graph_steps, stream=stream)
+        stream.synchronize()
+        walls["one_launch"] = time.perf_counter() - t0
+        t1 = 2 + warmup + steps
+        for pl in plans:
+            pl.run_graph(t1, warmup, graph_steps=graph_steps, stream=stream)
+        stream.synchronize()
+        t0 = time.perf_counter()
+        for pl in plans:
+            pl.run_graph(t1 + warmup, steps, graph_steps=graph_steps, stream=stream)
+        stream.synchronize()
+        walls["separate"] = time.perf_counter() - t0
+        ex = [pl.export(fields=("Phi_curr",), stream=stream)["Phi_curr"] for pl in plans]
+        stream.synchronize()
+        finite = all(bool(torch.isfinite(x).all()) for x in ex)
+        ens.close()
+        for pl in plans:
+            pl.close()
+    n = B42 + B63
+    return dict(members={"T42": B42, "T63": B63}, steps=steps, ms_per_step=walls["one_launch"] / steps * 1e3,
+                member_steps_per_s=steps * n / walls["one_launch"],
+                separate_ms_per_step=walls["separate"] / steps * 1e3,
+                separate_member_steps_per_s=steps * n / walls["separate"], finite=finite)
+
+
 def run_t106_fwd_adjoint(B=64, steps=5000, every=70, precision="fp32"):
     """T106 x 64 members, dt = 60 s, 5000 steps forward + adjoint with a spectral checkpoint every
     70 steps: wall time and peak device memory (torch-held checkpoints and segment states at their
@@ -217,6 +261,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=100)
     ap.add_argument("--no-t106", action="store_true", help="skip the T106 x 64, 5000-step fwd+adjoint run")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the 256 x T42 + 64 x T63 one-launch run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only run a plain device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -291,6 +336,8 @@ def main():
             "plain_fp64_1member_wall_s": pa["wall"], "plain_fp64_1member_member_steps_per_s": pa_v,
             "ratio_vs_plain_fp64_1member": fa_v / pa_v,
         }
+    if not args.no_mixed and args.precision == "fp32":
+        out["mixed_T42x256_T63x64"] = run_mixed(args.steps, args.warmup)
     if not args.no_t106:
         out["t106_fwd_adjoint"] = run_t106_fwd_adjoint(precision=args.precision)
     if not args.no_cpu_baseline:

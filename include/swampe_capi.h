@@ -148,6 +148,20 @@ int swx_export_state(swx_plan *plan, const swx_state_out *out, void *stream);
  * jitted lax.scan it replaces (model.py:1281-1342). */
 int swx_run_graph(swx_plan *plan, int t0, int nsteps, int graph_steps, void *stream);
 
+/* Mixed truncations in one launch: an ensemble of two initialized FP32 plans of
+ * different truncations (42/63, 42/106 or 63/106; plans keep their own state,
+ * parameters and tables).  swx_ensemble_run_graph advances both plans by
+ * nsteps steps from t0 exactly as swx_run_graph does for each, with one
+ * Legendre launch and one grid launch per step covering both plans' members,
+ * replayed from a captured HIP graph of `graph_steps` steps.  The plans must
+ * outlive the ensemble and are not modified otherwise.  Replaces batching
+ * members of different truncations, which the reference cannot do in one
+ * vmapped scan (one Static per jit, model.py:1281-1342). */
+typedef struct swx_ensemble swx_ensemble;
+int swx_ensemble_create(swx_plan *a, swx_plan *b, swx_ensemble **out);
+int swx_ensemble_run_graph(swx_ensemble *ens, int t0, int nsteps, int graph_steps, void *stream);
+void swx_ensemble_destroy(swx_ensemble *ens);
+
 /* Device memory (bytes) the plan holds: basis and DFT tables, carry, work
  * buffers, and the adjoint workspace once allocated.  Caller-owned tensors
  * (checkpoints, outputs) are not included.  No reference counterpart. */

@@ -34,6 +34,7 @@ EXPORTED_SYMBOLS = (
     "swx_init", "swx_run", "swx_export_state", "swx_run_graph", "swx_time_kernels", "swx_plan_set_option",
     "swx_init_spectral", "swx_state_get", "swx_state_set", "swx_adjoint_begin", "swx_adjoint_terminal",
     "swx_adjoint_step", "swx_adjoint_result", "swx_last_error", "swx_version", "swx_plan_device_bytes",
+    "swx_ensemble_create", "swx_ensemble_run_graph", "swx_ensemble_destroy",
 )
 
 STATE_FIELDS = (
@@ -101,12 +102,17 @@ def load() -> ctypes.CDLL:
     lib.swx_adjoint_result.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p]
     lib.swx_time_kernels.argtypes = [c_void_p, c_int, c_int, POINTER(ctypes.c_float), POINTER(ctypes.c_float), c_void_p]
     lib.swx_plan_device_bytes.argtypes = [c_void_p, POINTER(ctypes.c_size_t)]
+    lib.swx_ensemble_create.argtypes = [c_void_p, c_void_p, POINTER(c_void_p)]
+    lib.swx_ensemble_run_graph.argtypes = [c_void_p, c_int, c_int, c_int, c_void_p]
+    lib.swx_ensemble_destroy.argtypes = [c_void_p]
+    lib.swx_ensemble_destroy.restype = None
     for name in EXPORTED_SYMBOLS:
         getattr(lib, name)  # AttributeError when a declared symbol is missing
     for name in ("swx_geometry", "swx_grid_size", "swx_plan_create", "swx_plan_set_params", "swx_init",
                  "swx_run", "swx_export_state", "swx_run_graph", "swx_time_kernels", "swx_plan_set_option",
                  "swx_init_spectral", "swx_state_get", "swx_state_set", "swx_adjoint_begin",
-                 "swx_adjoint_terminal", "swx_adjoint_step", "swx_adjoint_result", "swx_plan_device_bytes"):
+                 "swx_adjoint_terminal", "swx_adjoint_step", "swx_adjoint_result", "swx_plan_device_bytes",
+                 "swx_ensemble_create", "swx_ensemble_run_graph"):
         getattr(lib, name).restype = c_int
     _lib = lib
     return lib

@@ -160,6 +160,11 @@ hipError_t launch_grid(const GridArgs<R>& a, hipStream_t s);
 hipError_t launch_grid_mfma(const GridArgs<float>& a, const float* TinvF, const float* TfwdF, hipStream_t s);
 int grid_mfma_kq1(int M);
 size_t grid_mfma_lds_bytes(int M);
+// Two FP32 plans of different truncations (M0 < M1: 42/63, 42/106, 63/106) in one launch.
+bool mfma_pair_supported(int M0, int M1);
+hipError_t launch_grid_mfma_pair(const GridArgs<float>& a0, const float* ti0, const float* tf0,
+                                 const GridArgs<float>& a1, const float* ti1, const float* tf1, hipStream_t s);
+hipError_t launch_leg_mfma_pair(const LegArgs<float>& a0, const LegArgs<float>& a1, hipStream_t s);
 // MFMA Legendre kernel (FP32, LEG_STEP without diagnostics).
 hipError_t launch_leg_mfma(const LegArgs<float>& a, hipStream_t s);
 size_t leg_mfma_lds_bytes(int M);

@@ -46,13 +46,13 @@ struct MfmaShape {
   static constexpr int LDS_FLOATS = 5 * K1p * 16 + 8 * ICH * 16;
 };
 
+// One workgroup's work; `blk` = its 16-row block (member * J / 16 + latitude block).
 template <int MT>
-__global__ void __launch_bounds__(256) k_grid_mfma(GridArgs<float> a, const float* __restrict__ TinvF,
-                                                   const float* __restrict__ TfwdF) {
+__device__ __forceinline__ void grid_mfma_body(const GridArgs<float>& a, const float* __restrict__ TinvF,
+                                               const float* __restrict__ TfwdF, const int blk, float* lds) {
   using S = MfmaShape<MT>;
   constexpr int Mp = S::Mp, I = S::I, K1p = S::K1p, NT2 = S::NT2, ICH = S::ICH, KQ1 = S::KQ1;
   static_assert(ICH % 16 == 0 && I % ICH == 0, "chunking");
-  extern __shared__ __align__(16) float lds[];
   float* sXT = lds;                    // [5][K1p][16]
   float* sYT = lds + 5 * K1p * 16;     // [8][ICH][16]
   float* sC = sYT;                     // [2][8][16][16] after the last chunk
@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(256) k_grid_mfma(GridArgs<float> a, const floa
   const int wave = tid >> 6, lane = tid & 63;
   const int ln = lane & 15, lg = lane >> 4;
   const int J = a.J, N = a.N;
-  const int row0 = blockIdx.x * S::ROWS;
+  const int row0 = blk * S::ROWS;
   const int b = row0 / J, j0 = row0 % J;
   const double* cf = a.coef + (long long)b * a.ncoef;
   const CoefLayout cl = CoefLayout::make(N, J);
@@ -250,6 +250,26 @@ __global__ void __launch_bounds__(256) k_grid_mfma(GridArgs<float> a, const floa
 }
 
 template <int MT>
+__global__ void __launch_bounds__(256) k_grid_mfma(GridArgs<float> a, const float* __restrict__ TinvF,
+                                                   const float* __restrict__ TfwdF) {
+  extern __shared__ __align__(16) float lds[];
+  grid_mfma_body<MT>(a, TinvF, TfwdF, blockIdx.x, lds);
+}
+
+// Two plans of different truncations in one launch: blocks [0, nblk0) are plan 0's.
+template <int MA, int MB>
+__global__ void __launch_bounds__(256) k_grid_mfma_pair(GridArgs<float> a0, const float* __restrict__ ti0,
+                                                        const float* __restrict__ tf0, GridArgs<float> a1,
+                                                        const float* __restrict__ ti1, const float* __restrict__ tf1,
+                                                        int nblk0) {
+  extern __shared__ __align__(16) float lds[];
+  if ((int)blockIdx.x < nblk0)
+    grid_mfma_body<MA>(a0, ti0, tf0, blockIdx.x, lds);
+  else
+    grid_mfma_body<MB>(a1, ti1, tf1, blockIdx.x - nblk0, lds);
+}
+
+template <int MT>
 static hipError_t launch_mfma_t(const GridArgs<float>& a, const float* TinvF, const float* TfwdF, hipStream_t s) {
   using S = MfmaShape<MT>;
   const size_t smem = sizeof(float) * S::LDS_FLOATS;
@@ -333,17 +353,16 @@ struct LegShape {
   static constexpr int LDS_FLOATS = SA + SX;
 };
 
+// One workgroup's work: zonal wavenumber m, member group `grp` (members grp*MB ...).
 template <int MT, int MB>
-__global__ void __launch_bounds__(256) k_leg_mfma(LegArgs<float> a) {
+__device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int m, const int grp, float* lds) {
   using S = LegShape<MT, MB>;
   constexpr int Np = S::Np, Mp = MT + 1, J = S::J, RP = S::RP, RT = S::RT, XP = S::XP, XT = S::XT, JO = S::JO;
   constexpr int KNP = S::KNP;
   static_assert(KNP == leg_knp(MT), "fragment tables use leg_knp rows");
-  extern __shared__ __align__(16) float lds[];
   float* sA = lds;           // [J][RT] inputs, then [K][RT] forward results, then [XT][JO] inverse results
   float* sX = lds + S::SA;   // [KNP][XT] inverse-transform inputs
-  const int m = blockIdx.x;
-  const int b0 = blockIdx.y * MB;
+  const int b0 = grp * MB;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, ln = lane & 15, lg = lane >> 4;
   const int B = a.B, N = a.N;
   const int K = Np - m;
@@ -521,6 +540,23 @@ __global__ void __launch_bounds__(256) k_leg_mfma(LegArgs<float> a) {
 }
 
 template <int MT, int MB>
+__global__ void __launch_bounds__(256) k_leg_mfma(LegArgs<float> a) {
+  extern __shared__ __align__(16) float lds[];
+  leg_mfma_body<MT, MB>(a, blockIdx.x, blockIdx.y, lds);
+}
+
+// Two plans of different truncations in one launch: block = grp * (M+1) + m, plan 0's first.
+template <int MA, int MBA, int MB, int MBB>
+__global__ void __launch_bounds__(256) k_leg_mfma_pair(LegArgs<float> a0, LegArgs<float> a1, int nblk0) {
+  extern __shared__ __align__(16) float lds[];
+  const int blk = blockIdx.x;
+  if (blk < nblk0)
+    leg_mfma_body<MA, MBA>(a0, blk % (MA + 1), blk / (MA + 1), lds);
+  else
+    leg_mfma_body<MB, MBB>(a1, (blk - nblk0) % (MB + 1), (blk - nblk0) / (MB + 1), lds);
+}
+
+template <int MT, int MB>
 static hipError_t launch_leg_mfma_t(const LegArgs<float>& a, hipStream_t s) {
   using S = LegShape<MT, MB>;
   const size_t smem = sizeof(float) * S::LDS_FLOATS;
@@ -545,6 +581,44 @@ hipError_t launch_leg_mfma(const LegArgs<float>& a, hipStream_t s) {
     case 106: return launch_leg_mfma_t<106, 4>(a, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+template <int MA, int MB>
+static hipError_t launch_grid_pair_t(const GridArgs<float>& a0, const float* ti0, const float* tf0,
+                                     const GridArgs<float>& a1, const float* ti1, const float* tf1, hipStream_t s) {
+  const size_t smem = sizeof(float) * (size_t)cmax(MfmaShape<MA>::LDS_FLOATS, MfmaShape<MB>::LDS_FLOATS);
+  const int n0 = (int)((long long)a0.B * a0.J / MfmaShape<MA>::ROWS);
+  const int n1 = (int)((long long)a1.B * a1.J / MfmaShape<MB>::ROWS);
+  hipLaunchKernelGGL((k_grid_mfma_pair<MA, MB>), dim3(n0 + n1), dim3(256), smem, s, a0, ti0, tf0, a1, ti1, tf1, n0);
+  return hipGetLastError();
+}
+
+template <int MA, int MBA, int MB, int MBB>
+static hipError_t launch_leg_pair_t(const LegArgs<float>& a0, const LegArgs<float>& a1, hipStream_t s) {
+  const size_t smem = sizeof(float) * (size_t)cmax(LegShape<MA, MBA>::LDS_FLOATS, LegShape<MB, MBB>::LDS_FLOATS);
+  const int n0 = (MA + 1) * ((a0.B + MBA - 1) / MBA);
+  const int n1 = (MB + 1) * ((a1.B + MBB - 1) / MBB);
+  hipLaunchKernelGGL((k_leg_mfma_pair<MA, MBA, MB, MBB>), dim3(n0 + n1), dim3(256), smem, s, a0, a1, n0);
+  return hipGetLastError();
+}
+
+bool mfma_pair_supported(int M0, int M1) {
+  return (M0 == 42 && M1 == 63) || (M0 == 42 && M1 == 106) || (M0 == 63 && M1 == 106);
+}
+
+hipError_t launch_grid_mfma_pair(const GridArgs<float>& a0, const float* ti0, const float* tf0,
+                                 const GridArgs<float>& a1, const float* ti1, const float* tf1, hipStream_t s) {
+  if (a0.M == 42 && a1.M == 63) return launch_grid_pair_t<42, 63>(a0, ti0, tf0, a1, ti1, tf1, s);
+  if (a0.M == 42 && a1.M == 106) return launch_grid_pair_t<42, 106>(a0, ti0, tf0, a1, ti1, tf1, s);
+  if (a0.M == 63 && a1.M == 106) return launch_grid_pair_t<63, 106>(a0, ti0, tf0, a1, ti1, tf1, s);
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_leg_mfma_pair(const LegArgs<float>& a0, const LegArgs<float>& a1, hipStream_t s) {
+  if (a0.M == 42 && a1.M == 63) return launch_leg_pair_t<42, 8, 63, 4>(a0, a1, s);
+  if (a0.M == 42 && a1.M == 106) return launch_leg_pair_t<42, 8, 106, 4>(a0, a1, s);
+  if (a0.M == 63 && a1.M == 106) return launch_leg_pair_t<63, 4, 106, 4>(a0, a1, s);
+  return hipErrorInvalidValue;
 }
 
 }  // namespace swx

@@ -249,11 +249,20 @@ GridArgs<R> grid_args(swx_plan* p, int mode) {
 // The MFMA grid kernel covers the forward stepping of FP32 plans without
 // per-step history or diagnostics; everything else uses the plain kernel.
 template <typename R>
+bool grid_uses_mfma(const swx_plan* p, const GridArgs<R>& g) {
+  return sizeof(R) == 4 && p->use_mfma && !p->opt_plain_grid && g.mode == GRID_STEP && !g.diagnostics &&
+         !g.hist_fields && !g.hist_phiext && !(g.test == 1 && g.expflag);
+}
+
+template <typename R>
+bool leg_uses_mfma(const swx_plan* p, const LegArgs<R>& l) {
+  return sizeof(R) == 4 && p->use_leg_mfma && !p->opt_plain_leg && l.mode == LEG_STEP && !l.diagnostics;
+}
+
+template <typename R>
 hipError_t launch_grid_any(swx_plan* p, const GridArgs<R>& g, hipStream_t s) {
   if constexpr (sizeof(R) == 4) {
-    const bool mfma_ok = p->use_mfma && !p->opt_plain_grid && g.mode == GRID_STEP && !g.diagnostics && !g.hist_fields && !g.hist_phiext &&
-                         !(g.test == 1 && g.expflag);
-    if (mfma_ok) return launch_grid_mfma(g, p->TinvF, p->TfwdF, s);
+    if (grid_uses_mfma(p, g)) return launch_grid_mfma(g, p->TinvF, p->TfwdF, s);
   }
   return launch_grid<R>(g, s);
 }
@@ -261,7 +270,7 @@ hipError_t launch_grid_any(swx_plan* p, const GridArgs<R>& g, hipStream_t s) {
 template <typename R>
 hipError_t launch_leg_any(swx_plan* p, const LegArgs<R>& l, hipStream_t s) {
   if constexpr (sizeof(R) == 4) {
-    if (p->use_leg_mfma && !p->opt_plain_leg && l.mode == LEG_STEP && !l.diagnostics) return launch_leg_mfma(l, s);
+    if (leg_uses_mfma(p, l)) return launch_leg_mfma(l, s);
   }
   return launch_leg<R>(l, s);
 }
@@ -464,6 +473,52 @@ int adjoint_step(swx_plan* p, const double* St, hipStream_t s) {
   SWX_HIP(launch_leg_adj<R>(a, s));
   SWX_HIP(launch_grid_adj<R>(grid_adj_args<R>(p, GADJ_STEP), s));
   SWX_HIP(launch_leg_adj<R>(leg_adj_args<R>(p, LADJ_B), s));
+  return SWX_OK;
+}
+
+}  // namespace
+
+// Two FP32 plans of different truncations stepped together: each step is one Legendre launch
+// and one grid launch covering both plans' workgroups (MFMA pair kernels); a step where either
+// plan needs a plain kernel falls back to one launch per plan.
+struct swx_ensemble {
+  swx_plan* p[2] = {nullptr, nullptr};  // p[0]->geo.M < p[1]->geo.M
+  hipGraphExec_t gexec = nullptr;
+  int gexec_steps = 0;
+  hipStream_t gexec_stream = nullptr;
+};
+
+namespace {
+
+int ens_steps(swx_ensemble* e, int t0, int nsteps, bool save_last, hipStream_t s, const int* phys_from) {
+  for (int k = 0; k < nsteps; ++k) {
+    const int t = t0 + k;
+    const bool last = save_last && (k == nsteps - 1);
+    LegArgs<float> l[2];
+    GridArgs<float> g[2];
+    for (int q = 0; q < 2; ++q) {
+      swx_plan* p = e->p[q];
+      if (last) {
+        const size_t gbytes = (size_t)p->d.n_members * kNG * p->geo.J * (p->geo.M + 1) * 2 * p->rsz;
+        SWX_HIP(hipMemcpyAsync(p->Gprev, p->G, gbytes, hipMemcpyDeviceToDevice, s));
+      }
+      l[q] = leg_args<float>(p, LEG_STEP);
+      g[q] = grid_args<float>(p, GRID_STEP);
+      g[q].do_ra = (p->d.modalflag && t > 2) ? 1 : 0;
+      g[q].phys_on = k >= phys_from[q] ? 1 : 0;
+      g[q].export_raw = last ? 1 : 0;
+    }
+    if (leg_uses_mfma(e->p[0], l[0]) && leg_uses_mfma(e->p[1], l[1])) {
+      SWX_HIP(launch_leg_mfma_pair(l[0], l[1], s));
+    } else {
+      for (int q = 0; q < 2; ++q) SWX_HIP(launch_leg_any<float>(e->p[q], l[q], s));
+    }
+    if (grid_uses_mfma(e->p[0], g[0]) && grid_uses_mfma(e->p[1], g[1])) {
+      SWX_HIP(launch_grid_mfma_pair(g[0], e->p[0]->TinvF, e->p[0]->TfwdF, g[1], e->p[1]->TinvF, e->p[1]->TfwdF, s));
+    } else {
+      for (int q = 0; q < 2; ++q) SWX_HIP(launch_grid_any<float>(e->p[q], g[q], s));
+    }
+  }
   return SWX_OK;
 }
 
@@ -774,6 +829,82 @@ int swx_run_graph(swx_plan* p, int t0, int nsteps, int graph_steps, void* stream
   }
   if (k < nsteps) return steps(k, nsteps - k, true);
   return SWX_OK;
+}
+
+int swx_ensemble_create(swx_plan* a, swx_plan* b, swx_ensemble** out) {
+  if (!out) return fail(SWX_ERR_ARG, "null output");
+  *out = nullptr;
+  int rc = check_plan(a);
+  if (rc == SWX_OK) rc = check_plan(b);
+  if (rc) return rc;
+  if (a->rsz != 4 || b->rsz != 4) return fail(SWX_ERR_ARG, "an ensemble needs FP32-transform plans");
+  swx_plan* p0 = a->geo.M < b->geo.M ? a : b;
+  swx_plan* p1 = a->geo.M < b->geo.M ? b : a;
+  if (!mfma_pair_supported(p0->geo.M, p1->geo.M))
+    return fail(SWX_ERR_ARG, "an ensemble needs two plans of different truncations among 42, 63, 106");
+  if (!p0->use_mfma || !p0->use_leg_mfma || !p1->use_mfma || !p1->use_leg_mfma)
+    return fail(SWX_ERR_ARG, "the MFMA kernels are unavailable for one of the plans on this device");
+  swx_ensemble* e = new swx_ensemble();
+  e->p[0] = p0;
+  e->p[1] = p1;
+  *out = e;
+  return SWX_OK;
+}
+
+int swx_ensemble_run_graph(swx_ensemble* e, int t0, int nsteps, int graph_steps, void* stream) {
+  if (!e) return fail(SWX_ERR_ARG, "null ensemble");
+  int rc = check_plan(e->p[0]);
+  if (rc == SWX_OK) rc = check_plan(e->p[1]);
+  if (rc) return rc;
+  if (!e->p[0]->initialized || !e->p[1]->initialized) return fail(SWX_ERR_STATE, "swx_ensemble_run_graph before init");
+  if (nsteps <= 0) return SWX_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (graph_steps > 0 && s == nullptr) return fail(SWX_ERR_ARG, "graph replay needs a non-default stream");
+  int pf[2];
+  for (int q = 0; q < 2; ++q) pf[q] = std::max(0, nsteps - phys_window(e->p[q]));
+  auto steps = [&](int k0, int n, bool save_last) {
+    const int pk[2] = {std::max(0, pf[0] - k0), std::max(0, pf[1] - k0)};
+    return ens_steps(e, t0 + k0, n, save_last, s, pk);
+  };
+  // As swx_run_graph: t <= 2, the physical-carry windows and the last step run eagerly.
+  int k = 0;
+  while (k < nsteps && t0 + k <= 2) {
+    if ((rc = steps(k, 1, k == nsteps - 1))) return rc;
+    ++k;
+  }
+  const int graph_end = std::min(std::min(pf[0], pf[1]), nsteps - 1);
+  if (graph_steps > 0 && graph_end - k >= graph_steps) {
+    if (!e->gexec || e->gexec_steps != graph_steps || e->gexec_stream != s) {
+      if (e->gexec) {
+        (void)hipGraphExecDestroy(e->gexec);
+        e->gexec = nullptr;
+      }
+      hipGraph_t graph = nullptr;
+      SWX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      const int off[2] = {1 << 30, 1 << 30};
+      rc = ens_steps(e, 3, graph_steps, false, s, off);
+      hipError_t ec = hipStreamEndCapture(s, &graph);
+      if (rc) return rc;
+      if (ec != hipSuccess) return fail(SWX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+      ec = hipGraphInstantiate(&e->gexec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ec != hipSuccess) return fail(SWX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ec));
+      e->gexec_steps = graph_steps;
+      e->gexec_stream = s;
+    }
+    while (graph_end - k >= graph_steps) {
+      SWX_HIP(hipGraphLaunch(e->gexec, s));
+      k += graph_steps;
+    }
+  }
+  if (k < nsteps) return steps(k, nsteps - k, true);
+  return SWX_OK;
+}
+
+void swx_ensemble_destroy(swx_ensemble* e) {
+  if (!e) return;
+  if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+  delete e;
 }
 
 int swx_plan_set_option(swx_plan* p, int option, int value) {

@@ -223,3 +223,35 @@ class Plan:
             out["spec"] = sp
         _capi.check(self._lib.swx_export_state(self._h, ctypes.byref(st), self._stream(stream)), "swx_export_state")
         return out
+
+
+class Ensemble:
+    """Two FP32 plans of different truncations (42/63, 42/106, 63/106) stepped together: one
+    Legendre launch and one grid launch per step cover both plans' members.  Each plan keeps
+    its own state, parameters and tables; export them with Plan.export as usual."""
+
+    def __init__(self, a: Plan, b: Plan):
+        self._lib = _capi.load()
+        self.plans = (a, b)  # keep both alive while the ensemble exists
+        if a.device != b.device:
+            raise ValueError("ensemble plans must be on the same device")
+        h = ctypes.c_void_p()
+        _capi.check(self._lib.swx_ensemble_create(a._h, b._h, ctypes.byref(h)), "swx_ensemble_create")
+        self._h = h
+        self.device = a.device
+
+    def run_graph(self, t0: int, nsteps: int, graph_steps: int = 32, stream=None) -> None:
+        """Advance both plans by nsteps steps from t0 (as Plan.run_graph does for each)."""
+        _capi.check(self._lib.swx_ensemble_run_graph(self._h, int(t0), int(nsteps), int(graph_steps),
+                                                     self.plans[0]._stream(stream)), "swx_ensemble_run_graph")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.swx_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover - best effort
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -328,3 +328,63 @@ def test_physical_carry_window_matches_full_carry(precision, torch_gpu):
         plan.close()
     for k in outs[0]:
         np.testing.assert_allclose(outs[1][k], outs[0][k], rtol=1e-15, atol=0, err_msg=k)
+
+
+def _noise_spectral(M, B, seed, omega):
+    rng = np.random.default_rng(seed)
+    c = np.zeros((B, 3, M + 1, M + 1), complex)
+    c[:, 0, 0, 1] = omega / np.sqrt(0.375)
+    for m in range(9):
+        for n in range(m, 9):
+            c[:, 2, m, n] = 40.0 * (rng.standard_normal(B) + (1j * rng.standard_normal(B) if m else 0.0))
+    return c
+
+
+@pytest.mark.parametrize("pair", [(42, 63), (42, 106)])
+def test_mixed_truncation_ensemble_matches_separate_plans(pair, torch_gpu):
+    """Two FP32 plans of different truncations stepped together (one Legendre and one grid launch
+    per step for both) reach exactly the state each plan reaches alone: bitwise, since every
+    workgroup runs the same device code.  40 steps: eager start, graph replays, the physical
+    carry window and the last step; partial member groups (3 and 2 members)."""
+    from my_swampe_amd.model import param_row
+    from my_swampe_amd.plan import Ensemble, Plan
+
+    torch = torch_gpu
+    _print_case("mixed-ensemble", pair=pair)
+    omega = 3.2e-5
+    specs = []
+    for M, B in zip(pair, (3, 2)):
+        dt = 60.0 if M == 106 else 120.0
+        rng = np.random.default_rng(M)
+        params = np.stack([param_row(dt=dt, a=8.2e7, omega=omega, g=9.8, Phibar=3e5, taurad=10 ** rng.uniform(4, 6),
+                                     taudrag=10 ** rng.uniform(4, 6), DPhieq=rng.uniform(1e5, 5e5), K6=1.24e33,
+                                     K6Phi=None, alpha=0.01) for _ in range(B)])
+        specs.append((M, params, _noise_spectral(M, B, M + 1, omega)))
+    stream = torch.cuda.Stream()
+    out = {}
+    for mode in ("separate", "ensemble"):
+        with torch.cuda.stream(stream):
+            plans = []
+            for M, params, c in specs:
+                pl = Plan(M, params, precision="fp32")
+                pl.init_spectral(torch.tensor(c, device="cuda"), stream=stream)
+                plans.append(pl)
+            if mode == "separate":
+                for pl in plans:
+                    pl.run_graph(2, 40, graph_steps=8, stream=stream)
+            else:
+                ens = Ensemble(plans[0], plans[1])
+                ens.run_graph(2, 40, graph_steps=8, stream=stream)
+            res = [pl.export(fields=("Phi_curr", "U_curr", "etam_curr", "Phi_prev"), spec=True, stream=stream)
+                   for pl in plans]
+            stream.synchronize()
+            if mode == "ensemble":
+                ens.close()
+            for pl in plans:
+                pl.close()
+        out[mode] = res
+    for q, M in enumerate(pair):
+        for k in ("Phi_curr", "U_curr", "etam_curr", "Phi_prev", "spec"):
+            a, b = out["separate"][q][k], out["ensemble"][q][k]
+            assert bool(torch.isfinite(torch.view_as_real(a) if a.is_complex() else a).all()), (M, k)
+            assert torch.equal(a, b), (M, k, float((a - b).abs().max()))
