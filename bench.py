@@ -29,7 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "member-steps/s at T42×256 ens; fwd+adjoint time vs plain FP64 GPU, same GPU"
-PEAK = {"fp32": 157.3, "fp64": 78.6}  # TFLOP/s, MI355X dense matrix peaks
+PEAK = {"fp32": 157.3, "fp64": 78.6, "bf16x3": 2500.0 / 3}  # TFLOP/s, MI355X dense matrix peaks (bf16x3: 3 bf16 products)
 HBM_PEAK = 8000.0  # GB/s
 
 
@@ -255,7 +255,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--members", type=int, default=256)
-    ap.add_argument("--precision", default="fp32", choices=("fp32", "fp64"))
+    ap.add_argument("--precision", default="fp32", choices=("fp32", "fp64", "bf16x3"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -284,6 +284,7 @@ def main():
     grid_tflops = dft_f * B / (main_run["ms_grid"] * 1e-3) / 1e12
     leg_tflops = leg_f * B / (main_run["ms_leg"] * 1e-3) / 1e12
     peak = PEAK[args.precision]
+    grid_kernel = {"fp32": f"k_grid_mfma<{M}>", "bf16x3": f"k_grid_bf16x3<{M}>", "fp64": "k_grid<double>"}[args.precision]
     out = {
         "metric": METRIC,
         "value": value,
@@ -301,8 +302,8 @@ def main():
                    "members": B, "M": M, "J": J, "I": I, "precision": args.precision},
         "roofline": {"bound": "mfma", "achieved": grid_tflops, "peak": peak, "unit": "TFLOP/s",
                      "frac": grid_tflops / peak,
-                     "traffic": measured_traffic(f"k_grid_mfma<{M}>", B, args.precision),
-                     "kernel": f"k_grid_mfma<{M}> (inverse DFT + grid physics + forward DFT)",
+                     "traffic": measured_traffic(grid_kernel, B, args.precision),
+                     "kernel": f"{grid_kernel} (inverse DFT + grid physics + forward DFT)",
                      "algorithmic_flops_per_launch": dft_f * B,
                      "kernel_ms": main_run["ms_grid"], "leg_kernel_ms": main_run["ms_leg"],
                      "leg_tflops": leg_tflops},
@@ -312,6 +313,7 @@ def main():
         full = run_batch(M, B, args.precision, max(args.steps // 2, 100), 10, full_phys=True)
         fp64 = run_batch(M, B, "fp64", max(args.steps // 4, 50), 10)
         plain = run_batch(M, 1, "fp64", max(args.steps // 2, 200), 20)
+        bf = run_batch(M, B, "bf16x3", max(args.steps // 2, 100), 10)
         fp64_v = max(args.steps // 4, 50) * B / fp64["wall"]
         plain_v = max(args.steps // 2, 200) / plain["wall"]
         out["extra"] = {
@@ -323,6 +325,9 @@ def main():
             "ratio_vs_fp64_same_batch": value / fp64_v,
             "fp64_batched_kernel_ms": {"leg": fp64["ms_leg"], "grid": fp64["ms_grid"]},
             "plain_kernel_ms": {"leg": plain["ms_leg"], "grid": plain["ms_grid"]},
+            "bf16x3_member_steps_per_s": max(args.steps // 2, 100) * B / bf["wall"],
+            "bf16x3_kernel_ms": {"leg": bf["ms_leg"], "grid": bf["ms_grid"]},
+            "bf16x3_finite": bf["finite"],
         }
     if not args.no_adjoint:
         na = args.adjoint_steps

@@ -1,0 +1,111 @@
+// Pieces shared by the matrix-core grid kernels (FP32 MFMA and split-BF16x3):
+// the grid-space work of one point and the assembly of the Legendre inputs from
+// the forward-DFT accumulators.
+#pragma once
+
+#include "swx_kernels.h"
+#include "swx_physics.h"
+
+namespace swx {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// One grid point (row j, longitude i) of member b: pinned winds (test 1), the
+// Robert-Asselin filter of the physical prev level and the physical carry (when
+// a.phys_on; model.py:988-1006), then the 8 products of the forward transform
+// (forcing.py, initial_conditions.py:283-292).
+__device__ __forceinline__ void grid_point_step(const GridArgs<float>& a, const double* cf, const double* geo,
+                                                const GeoLayout& gl, double* phys, long long plane, double alpha,
+                                                int b, int j, int i, double eta, double delta, double Phi, double U,
+                                                double V, double prod[8]) {
+  const long long pix = (long long)j * a.I + i;
+  if (a.test == 1) {
+    U = a.uvic[((long long)b * 2 + 0) * plane + pix];
+    V = a.uvic[((long long)b * 2 + 1) * plane + pix];
+  }
+  if (a.phys_on) {
+    const double ep = phys[0 * plane + pix], ec = phys[1 * plane + pix];
+    const double dp = phys[2 * plane + pix], dc = phys[3 * plane + pix];
+    const double pp = phys[4 * plane + pix], pc = phys[5 * plane + pix];
+    double em = ec, dm = dc, pm = pc;
+    if (a.do_ra) {
+      em = ec + alpha * (ep - 2.0 * ec + eta);
+      dm = dc + alpha * (dp - 2.0 * dc + delta);
+      pm = pc + alpha * (pp - 2.0 * pc + Phi);
+    }
+    phys[0 * plane + pix] = em;
+    phys[1 * plane + pix] = eta;
+    phys[2 * plane + pix] = dm;
+    phys[3 * plane + pix] = delta;
+    phys[4 * plane + pix] = pm;
+    phys[5 * plane + pix] = Phi;
+    phys[6 * plane + pix] = U;
+    phys[7 * plane + pix] = V;
+  }
+  grid_products(cf, geo, gl, j, i, a.test, eta, Phi, U, V, prod);
+}
+
+// Assembly of the 9 Legendre inputs of 16 rows (member b, latitudes j0..j0+15)
+// from the forward-DFT accumulators: wave w holds acc[2t + h] = fields 2w + h,
+// wavenumber rows 16t + 4 (lane/16) + q, latitude column lane % 16.  Two
+// 16-row k' tiles go through LDS (sC, 2*8*16*16 floats) per round, then one
+// (m, row) pair per thread assembles (swx_physics.h:assemble_slots) and writes T.
+// U_m, V_m of the step are read from the Fourier rows G.
+template <int Mp, int NT2>
+__device__ __forceinline__ void grid_assemble(const GridArgs<float>& a, const v4f* acc, float* sC, int b, int j0,
+                                              const double* cf, const CoefLayout& cl, const double* geo,
+                                              const GeoLayout& gl) {
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, ln = lane & 15, lg = lane >> 4;
+  const int J = a.J;
+#pragma unroll
+  for (int t0 = 0; t0 < NT2; t0 += 2) {
+#pragma unroll
+    for (int dt2 = 0; dt2 < 2; ++dt2) {
+      const int t = t0 + dt2;
+      if (t < NT2) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int f = 2 * wave + h;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) sC[((dt2 * 8 + f) * 16 + 4 * lg + q) * 16 + ln] = acc[2 * t + h][q];
+        }
+      }
+    }
+    __syncthreads();
+    {
+      const int dt2 = tid >> 7, ml = (tid >> 4) & 7, nn = tid & 15;
+      const int m = 8 * (t0 + dt2) + ml;
+      if (t0 + dt2 < NT2 && m < Mp) {
+        const int jj = j0 + nn;
+        double fm[8][2];
+#pragma unroll
+        for (int f = 0; f < 8; ++f) {
+          fm[f][0] = sC[((dt2 * 8 + f) * 16 + 2 * ml) * 16 + nn];
+          fm[f][1] = sC[((dt2 * 8 + f) * 16 + 2 * ml + 1) * 16 + nn];
+        }
+        const float* gu = a.G + fidx(b, 3, m, jj, Mp, J);
+        const float* gv = a.G + fidx(b, 4, m, jj, Mp, J);
+        double Um[2] = {gu[0], gu[1]}, Vm[2] = {gv[0], gv[1]};
+        if (a.export_raw) {
+          float* rb = a.raw + (long long)b * kNRaw * J * Mp * 2;
+#pragma unroll
+          for (int f = 0; f < kNRaw; ++f) {
+            rb[((long long)f * J + jj) * Mp * 2 + 2 * m] = (float)fm[f][0];
+            rb[((long long)f * J + jj) * Mp * 2 + 2 * m + 1] = (float)fm[f][1];
+          }
+        }
+        double out[kNT][2];
+        assemble_slots(cf, cf[cl.n_tc + jj], geo[gl.w + jj], m, a.expflag, a.forcflag, fm, Um, Vm, out);
+        float* Tm = a.T + (((long long)b * Mp + m) * kNT) * J * 2;
+#pragma unroll
+        for (int k = 0; k < kNT; ++k) {
+          Tm[(k * J + jj) * 2] = (float)out[k][0];
+          Tm[(k * J + jj) * 2 + 1] = (float)out[k][1];
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace swx

@@ -160,6 +160,11 @@ hipError_t launch_grid(const GridArgs<R>& a, hipStream_t s);
 hipError_t launch_grid_mfma(const GridArgs<float>& a, const float* TinvF, const float* TfwdF, hipStream_t s);
 int grid_mfma_kq1(int M);
 size_t grid_mfma_lds_bytes(int M);
+// Split-BF16x3 grid kernel (same eligibility as the MFMA grid kernel); tables
+// of bf16 hi / lo pairs in its fragment order (swx_bf16x3.hip).
+hipError_t launch_grid_bf16x3(const GridArgs<float>& a, const void* TinvB, const void* TfwdB, hipStream_t s);
+size_t grid_bf16x3_lds_bytes(int M);
+int grid_bf16x3_ks1(int M);
 // Two FP32 plans of different truncations (M0 < M1: 42/63, 42/106, 63/106) in one launch.
 bool mfma_pair_supported(int M0, int M1);
 hipError_t launch_grid_mfma_pair(const GridArgs<float>& a0, const float* ti0, const float* tf0,

@@ -22,12 +22,9 @@
 //   threads that assemble the Legendre inputs.
 #include <cfloat>
 
-#include "swx_kernels.h"
-#include "swx_physics.h"
+#include "swx_grid_common.h"
 
 namespace swx {
-
-typedef float v4f __attribute__((ext_vector_type(4)));
 
 template <int MT>
 struct MfmaShape {
@@ -134,35 +131,9 @@ __device__ __forceinline__ void grid_mfma_body(const GridArgs<float>& a, const f
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int il = 16 * wave + 4 * lg + q;
-      const int i = ic + il;
-      const double eta = c1[0][q], delta = c1[1][q], Phi = c1[2][q];
-      double U = c1[3][q], V = c1[4][q];
-      const long long pix = (long long)j * I + i;
-      if (a.test == 1) {
-        U = a.uvic[((long long)b * 2 + 0) * plane + pix];
-        V = a.uvic[((long long)b * 2 + 1) * plane + pix];
-      }
-      if (a.phys_on) {
-        const double ep = phys[0 * plane + pix], ec = phys[1 * plane + pix];
-        const double dp = phys[2 * plane + pix], dc = phys[3 * plane + pix];
-        const double pp = phys[4 * plane + pix], pc = phys[5 * plane + pix];
-        double em = ec, dm = dc, pm = pc;
-        if (a.do_ra) {
-          em = ec + alpha * (ep - 2.0 * ec + eta);
-          dm = dc + alpha * (dp - 2.0 * dc + delta);
-          pm = pc + alpha * (pp - 2.0 * pc + Phi);
-        }
-        phys[0 * plane + pix] = em;
-        phys[1 * plane + pix] = eta;
-        phys[2 * plane + pix] = dm;
-        phys[3 * plane + pix] = delta;
-        phys[4 * plane + pix] = pm;
-        phys[5 * plane + pix] = Phi;
-        phys[6 * plane + pix] = U;
-        phys[7 * plane + pix] = V;
-      }
       double prod[8];
-      grid_products(cf, geo, gl, j, i, a.test, eta, Phi, U, V, prod);
+      grid_point_step(a, cf, geo, gl, phys, plane, alpha, b, j, ic + il, c1[0][q], c1[1][q], c1[2][q], c1[3][q],
+                      c1[4][q], prod);
 #pragma unroll
       for (int f = 0; f < 8; ++f) sYT[(f * ICH + il) * 16 + n] = (float)prod[f];
     }
@@ -194,59 +165,8 @@ __device__ __forceinline__ void grid_mfma_body(const GridArgs<float>& a, const f
     __syncthreads();
   }
 
-  // ---- Phase 3: assemble the Legendre inputs, two k'-tiles (16 wavenumbers)
-  // per round through LDS, one (m, row) pair per thread.
-#pragma unroll
-  for (int t0 = 0; t0 < NT2; t0 += 2) {
-#pragma unroll
-    for (int dt2 = 0; dt2 < 2; ++dt2) {
-      const int t = t0 + dt2;
-      if (t < NT2) {
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int f = 2 * wave + h;
-#pragma unroll
-          for (int q = 0; q < 4; ++q) sC[((dt2 * 8 + f) * 16 + 4 * lg + q) * 16 + ln] = acc[2 * t + h][q];
-        }
-      }
-    }
-    __syncthreads();
-    {
-      const int dt2 = tid >> 7, ml = (tid >> 4) & 7, nn = tid & 15;
-      const int m = 8 * (t0 + dt2) + ml;
-      if (t0 + dt2 < NT2 && m < Mp) {
-        const int jj = j0 + nn;
-        double fm[8][2];
-#pragma unroll
-        for (int f = 0; f < 8; ++f) {
-          fm[f][0] = sC[((dt2 * 8 + f) * 16 + 2 * ml) * 16 + nn];
-          fm[f][1] = sC[((dt2 * 8 + f) * 16 + 2 * ml + 1) * 16 + nn];
-        }
-        double Um[2], Vm[2];
-        Um[0] = sXT[(3 * K1p + 2 * m) * 16 + nn];
-        Um[1] = sXT[(3 * K1p + 2 * m + 1) * 16 + nn];
-        Vm[0] = sXT[(4 * K1p + 2 * m) * 16 + nn];
-        Vm[1] = sXT[(4 * K1p + 2 * m + 1) * 16 + nn];
-        if (a.export_raw) {
-          float* rb = a.raw + (long long)b * kNRaw * J * Mp * 2;
-#pragma unroll
-          for (int f = 0; f < kNRaw; ++f) {
-            rb[((long long)f * J + jj) * Mp * 2 + 2 * m] = (float)fm[f][0];
-            rb[((long long)f * J + jj) * Mp * 2 + 2 * m + 1] = (float)fm[f][1];
-          }
-        }
-        double out[kNT][2];
-        assemble_slots(cf, cf[cl.n_tc + jj], geo[gl.w + jj], m, a.expflag, a.forcflag, fm, Um, Vm, out);
-        float* Tm = a.T + (((long long)b * Mp + m) * kNT) * J * 2;
-#pragma unroll
-        for (int k = 0; k < kNT; ++k) {
-          Tm[(k * J + jj) * 2] = (float)out[k][0];
-          Tm[(k * J + jj) * 2 + 1] = (float)out[k][1];
-        }
-      }
-    }
-    __syncthreads();
-  }
+  // ---- Phase 3: assemble the Legendre inputs (swx_grid_common.h).
+  grid_assemble<Mp, NT2>(a, acc, sC, b, j0, cf, cl, geo, gl);
 }
 
 template <int MT>

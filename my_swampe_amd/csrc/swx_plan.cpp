@@ -81,6 +81,9 @@ struct swx_plan {
   float* TinvF = nullptr;  // MFMA grid kernel DFT tables, fragment order (FP32 plans)
   float* TfwdF = nullptr;
   bool use_mfma = false;       // MFMA grid kernel available for this plan
+  bool use_bf16x3 = false;     // split-BF16x3 grid kernel (precision SWX_PREC_BF16X3)
+  void* TinvB = nullptr;       // its DFT tables: bf16 hi / lo pairs in fragment order
+  void* TfwdB = nullptr;
   bool opt_plain_grid = false;  // SWX_OPT_GRID_KERNEL = 1
   bool opt_full_phys = false;   // SWX_OPT_FULL_PHYS_CARRY = 1
   bool use_leg_mfma = false;    // MFMA Legendre kernel available
@@ -124,6 +127,27 @@ int dalloc(swx_plan* owner, T** p, size_t bytes) {
   *p = static_cast<T*>(q);
   owner->dev_bytes += nb;
   return SWX_OK;
+}
+
+// bf16 (round to nearest even) bit pattern of a float, and its value.
+uint16_t bf16_bits(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+float bf16_value(uint16_t h) {
+  const uint32_t u = (uint32_t)h << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+// x = hi + lo + O(2^-16 |x|): hi = bf16(x), lo = bf16(x - hi).
+void bf16_split(double x, uint16_t& hi, uint16_t& lo) {
+  hi = bf16_bits((float)x);
+  lo = bf16_bits((float)(x - (double)bf16_value(hi)));
 }
 
 // Upload a host double array in the transform precision.
@@ -262,7 +286,10 @@ bool leg_uses_mfma(const swx_plan* p, const LegArgs<R>& l) {
 template <typename R>
 hipError_t launch_grid_any(swx_plan* p, const GridArgs<R>& g, hipStream_t s) {
   if constexpr (sizeof(R) == 4) {
-    if (grid_uses_mfma(p, g)) return launch_grid_mfma(g, p->TinvF, p->TfwdF, s);
+    if (grid_uses_mfma(p, g)) {
+      if (p->use_bf16x3) return launch_grid_bf16x3(g, p->TinvB, p->TfwdB, s);
+      return launch_grid_mfma(g, p->TinvF, p->TfwdF, s);
+    }
   }
   return launch_grid<R>(g, s);
 }
@@ -552,7 +579,7 @@ int swx_plan_create(const swx_plan_desc* desc, const double* params, swx_plan** 
   if (!desc || !params || !out) return fail(SWX_ERR_ARG, "null argument");
   *out = nullptr;
   if (desc->n_members < 1) return fail(SWX_ERR_ARG, "n_members must be >= 1");
-  if (desc->precision != SWX_PREC_FP64 && desc->precision != SWX_PREC_FP32)
+  if (desc->precision != SWX_PREC_FP64 && desc->precision != SWX_PREC_FP32 && desc->precision != SWX_PREC_BF16X3)
     return fail(SWX_ERR_ARG, "unsupported precision " + std::to_string(desc->precision));
   if (!(desc->test == -1 || desc->test == 1 || desc->test == 2))
     return fail(SWX_ERR_ARG, "test must be -1 (None), 1 or 2");
@@ -634,7 +661,7 @@ int swx_plan_create(const swx_plan_desc* desc, const double* params, swx_plan** 
     swx_plan_destroy(p);
     return rc;
   }
-  if (desc->precision == SWX_PREC_FP32) {
+  if (desc->precision == SWX_PREC_FP32 || desc->precision == SWX_PREC_BF16X3) {
     int maxlds = 0;
     (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device);
     p->use_mfma = grid_mfma_lds_bytes(M) <= (size_t)maxlds;
@@ -704,6 +731,37 @@ int swx_plan_create(const swx_plan_desc* desc, const double* params, swx_plan** 
         swx_plan_destroy(p);
         return rc;
       }
+      if (desc->precision == SWX_PREC_BF16X3 && grid_bf16x3_lds_bytes(M) <= (size_t)maxlds) {
+        // Fragment order of the split-BF16x3 kernel: per (tile, k-step, lane) 8 hi then 8 lo
+        // values, k (or i) = 32 ks + 8 (lane / 16) + e.
+        const int ks1 = grid_bf16x3_ks1(M);
+        std::vector<uint16_t> bi((size_t)(I / 16) * ks1 * 64 * 16), bf((size_t)(k2p / 16) * (I / 32) * 64 * 16);
+        for (int lane = 0; lane < 64; ++lane) {
+          const int ln = lane & 15, lg = lane >> 4;
+          for (int e = 0; e < 8; ++e) {
+            for (int it = 0; it < I / 16; ++it)
+              for (int ks = 0; ks < ks1; ++ks) {
+                const size_t o = (((size_t)it * ks1 + ks) * 64 + lane) * 16;
+                bf16_split(tinv(16 * it + ln, 32 * ks + 8 * lg + e), bi[o + e], bi[o + 8 + e]);
+              }
+            for (int t = 0; t < k2p / 16; ++t)
+              for (int g = 0; g < I / 32; ++g) {
+                const size_t o = (((size_t)t * (I / 32) + g) * 64 + lane) * 16;
+                bf16_split(tfwd(16 * t + ln, 32 * g + 8 * lg + e), bf[o + e], bf[o + 8 + e]);
+              }
+          }
+        }
+        if ((rc = dalloc(p, &p->TinvB, bi.size() * 2)) || (rc = dalloc(p, &p->TfwdB, bf.size() * 2))) {
+          swx_plan_destroy(p);
+          return rc;
+        }
+        if (hipMemcpy(p->TinvB, bi.data(), bi.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(p->TfwdB, bf.data(), bf.size() * 2, hipMemcpyHostToDevice) != hipSuccess) {
+          swx_plan_destroy(p);
+          return fail(SWX_ERR_HIP, "hipMemcpy of the BF16x3 tables failed");
+        }
+        p->use_bf16x3 = true;
+      }
     }
   }
   if ((rc = upload_real(p->Pf, Pf, rs)) || (rc = upload_real(p->Hf, Hf, rs)) || (rc = upload_real(p->Pi, Pi, rs)) ||
@@ -739,7 +797,7 @@ void swx_plan_destroy(swx_plan* p) {
   if (!p) return;
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   void* bufs[] = {p->Lam, p->Tbar, p->Gbar, p->pgrad, p->Pf, p->Hf, p->Pi, p->Hi, p->fragP[0], p->fragP[1],
-                  p->fragP[2], p->fragP[3], p->tw, p->TinvF, p->TfwdF, p->geo_d, p->f2, p->coef, p->S, p->T, p->G, p->Gprev,
+                  p->fragP[2], p->fragP[3], p->tw, p->TinvF, p->TfwdF, p->TinvB, p->TfwdB, p->geo_d, p->f2, p->coef, p->S, p->T, p->G, p->Gprev,
                   p->raw, p->lag, p->g0uv, p->phys, p->uvic, p->part_uv, p->dead};
   for (void* q : bufs)
     if (q) (void)hipFree(q);
@@ -837,7 +895,8 @@ int swx_ensemble_create(swx_plan* a, swx_plan* b, swx_ensemble** out) {
   int rc = check_plan(a);
   if (rc == SWX_OK) rc = check_plan(b);
   if (rc) return rc;
-  if (a->rsz != 4 || b->rsz != 4) return fail(SWX_ERR_ARG, "an ensemble needs FP32-transform plans");
+  if (a->d.precision != SWX_PREC_FP32 || b->d.precision != SWX_PREC_FP32)
+    return fail(SWX_ERR_ARG, "an ensemble needs FP32-transform plans");
   swx_plan* p0 = a->geo.M < b->geo.M ? a : b;
   swx_plan* p1 = a->geo.M < b->geo.M ? b : a;
   if (!mfma_pair_supported(p0->geo.M, p1->geo.M))

@@ -43,7 +43,7 @@ def t42_2000(torch_gpu):
     return kw, ref
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3"])
 def test_t42_2000_steps_vs_oracle(t42_2000, precision):
     """Relative L2 of the terminal fields after 2000 steps (the short-run atols do not apply: Phi
     reaches 2e5 here).  FP64: <= 1e-9 (FP64 rounding of two summation orders, amplified over
@@ -64,6 +64,8 @@ def test_t42_2000_steps_vs_oracle(t42_2000, precision):
         assert finite, f
         if precision == "fp64":
             assert rel <= 1e-9, (f, rel, mx)
+        elif precision == "bf16x3":
+            assert rel <= (1e-4 if f == "delta_curr" else 1e-5), (f, rel)
         else:
             assert rel <= (1e-3 if f == "delta_curr" else 1e-5), (f, rel)
 

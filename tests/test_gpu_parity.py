@@ -240,7 +240,8 @@ def test_graph_replay_matches_eager(torch_gpu):
 
 
 def test_fp32_transforms_close_to_fp64(torch_gpu):
-    """FP32 transforms with the FP64 state update vs the FP64 oracle, 60 steps of config-1 physics.
+    """FP32 transforms with the FP64 state update vs the FP64 oracle, 60 forced steps at T42
+    (dt = 120 s, a = 8.2e7, omega = 3.2e-5, Phibar = 3e5, DPhieq = 3e5, rest + noise).
 
     Bounds (relative L2): Phi and eta <= 1e-5 (the stated target for the
     reduced-precision path); delta, a small residual of nearly balanced
@@ -388,3 +389,38 @@ def test_mixed_truncation_ensemble_matches_separate_plans(pair, torch_gpu):
             a, b = out["separate"][q][k], out["ensemble"][q][k]
             assert bool(torch.isfinite(torch.view_as_real(a) if a.is_complex() else a).all()), (M, k)
             assert torch.equal(a, b), (M, k, float((a - b).abs().max()))
+
+
+@pytest.mark.parametrize("M", [42, 63, 106])
+def test_bf16x3_grid_kernel_vs_oracle(M, torch_gpu):
+    """Split-BF16x3 grid kernel (with the FP32 MFMA Legendre kernel and FP64 update) vs the FP64
+    oracle and vs the FP32 MFMA path, 12 forced steps from the balanced zonal flow of
+    test_mfma_grid_kernel_matches_plain_fp32 (3 members).  Relative L2 <= 2e-5 for Phi, U, eta
+    (the dropped lo*lo term and the 16-bit split leave ~2^-17 per product; FP32 reaches 2e-7)."""
+    from my_swampe_amd.model import param_row
+    from my_swampe_amd.plan import Plan
+
+    dt = 120.0 if M == 42 else 60.0
+    a, omega, Phibar = 6.37122e6, 7.2921159e-5, 3e4
+    p = param_row(dt=dt, a=a, omega=omega, g=9.8, Phibar=Phibar, taurad=86400.0, taudrag=86400.0,
+                  DPhieq=3e4, K6=1.24e33, K6Phi=None, alpha=0.01)
+    s = orc.build_static(M=M, dt=dt, a=a, omega=omega, g=9.8, Phibar=Phibar, taurad=86400.0, taudrag=86400.0,
+                         DPhieq=3e4, K6=1.24e33, K6Phi=None, test=2)
+    ics = orc.analytic_ic(s, 2, 0.3)
+    _print_case("bf16x3-vs-oracle", M=M)
+    res = {}
+    for prec in ("fp32", "bf16x3"):
+        plan = Plan(M, np.stack([p] * 3), precision=prec)
+        plan.init(*ics)
+        plan.run(2, 12)
+        res[prec] = {k: _np(v[0]) for k, v in plan.export(fields=("Phi_curr", "U_curr", "eta_curr")).items()
+                     if k != "dead"}
+        plan.close()
+    ref = orc.run_model_scan(M=M, dt=dt, tmax=14, Phibar=Phibar, omega=omega, a=a, taurad=86400.0,
+                             taudrag=86400.0, DPhieq=3e4, eta0_init=ics[0], delta0_init=ics[1], Phi0_init=ics[2],
+                             U0_init=ics[3], V0_init=ics[4], diagnostics=False, return_history=False)["last_state"]
+    for f in ("Phi_curr", "U_curr", "eta_curr"):
+        r = getattr(ref, f)
+        rel = {k: float(np.linalg.norm(res[k][f] - r) / np.linalg.norm(r)) for k in res}
+        print(M, f, "relL2 vs oracle", rel)
+        assert rel["bf16x3"] < 2e-5, (M, f, rel)
