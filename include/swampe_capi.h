@@ -217,10 +217,19 @@ int swx_adjoint_result(swx_plan *plan, double *lam_out, double *pgrad_out, void 
  *     step; 0 (default) = only in the final steps whose Robert-Asselin
  *     contribution can reach the exported state (identical results).
  *   SWX_OPT_LEG_KERNEL: 0 = automatic (MFMA Legendre kernel for FP32 forward
- *     stepping), 1 = always the plain Legendre kernel. */
+ *     stepping), 1 = always the plain Legendre kernel.
+ *   SWX_OPT_PHYS_STEPS: k > 0 = write the physical carry only in the last k
+ *     steps of each swx_run / swx_run_graph call without history; 0 (default)
+ *     = automatic as above.  The current-level grid fields (eta, delta, Phi,
+ *     U, V _curr) are exact for any k >= 1; the physical prev level, which
+ *     carries the Robert-Asselin filter's memory, is exact only when k covers
+ *     that memory.  k = -1: never (the exported grid fields are not updated).
+ *     For callers that need only the spectral state and the final
+ *     current-level fields (the differentiable rollout and its recompute). */
 #define SWX_OPT_GRID_KERNEL 1
 #define SWX_OPT_FULL_PHYS_CARRY 2
 #define SWX_OPT_LEG_KERNEL 3
+#define SWX_OPT_PHYS_STEPS 4
 int swx_plan_set_option(swx_plan *plan, int option, int value);
 
 /* Last error message of the calling thread. */

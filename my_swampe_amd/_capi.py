@@ -23,6 +23,7 @@ PREC_BF16X3 = 2
 OPT_GRID_KERNEL = 1
 OPT_FULL_PHYS_CARRY = 2
 OPT_LEG_KERNEL = 3
+OPT_PHYS_STEPS = 4
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
 
 PARAM_NAMES = ("dt", "a", "omega", "g", "Phibar", "taurad", "taudrag", "DPhieq", "K6", "K6Phi", "alpha", "blowup_rms")

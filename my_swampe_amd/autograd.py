@@ -24,6 +24,7 @@ from typing import List, Optional
 
 import numpy as np
 
+from . import _capi
 from .model import param_row
 from .plan import Plan, require_gpu
 
@@ -66,8 +67,12 @@ def _params(cfg: RolloutConfig, taurad, taudrag, DPhieq) -> np.ndarray:
 
 
 def _make_plan(cfg: RolloutConfig, params) -> Plan:
-    return Plan(cfg.M, params, precision=cfg.precision, forcflag=cfg.forcflag, diffflag=cfg.diffflag,
+    plan = Plan(cfg.M, params, precision=cfg.precision, forcflag=cfg.forcflag, diffflag=cfg.diffflag,
                 modalflag=cfg.modalflag, diagnostics=False)
+    # The rollout reads the spectral state and the final Phi_curr only: the physical grid carry
+    # is written in the last step of each run call.
+    plan.set_option(_capi.OPT_PHYS_STEPS, 1)
+    return plan
 
 
 def forward_with_checkpoints(plan: Plan, cfg: RolloutConfig, c):
@@ -91,6 +96,7 @@ def adjoint_through_checkpoints(plan: Plan, cfg: RolloutConfig, ckpts, gPhi):
     K = cfg.every()
     plan.adjoint_begin()
     plan.adjoint_terminal(g_Phi=gPhi)
+    plan.set_option(_capi.OPT_PHYS_STEPS, -1)  # the recompute needs the spectral states only
     peak = 0
     for seg in reversed(range(len(ckpts))):
         start = seg * K
@@ -104,6 +110,7 @@ def adjoint_through_checkpoints(plan: Plan, cfg: RolloutConfig, ckpts, gPhi):
         peak = max(peak, len(tape))
         for s in reversed(range(n)):
             plan.adjoint_step(tape[s])
+    plan.set_option(_capi.OPT_PHYS_STEPS, 1)
     cfg.stats.update(checkpoints=len(ckpts), segment_states=peak)
     return plan.adjoint_result()
 

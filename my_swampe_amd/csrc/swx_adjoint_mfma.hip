@@ -1,0 +1,257 @@
+// MFMA grid adjoint kernel (FP32): the reverse of k_grid_mfma for one step,
+// replacing k_grid_adj in GADJ_STEP mode where its LDS fits (T42, T63).
+//
+// Per 16 rows (member, latitude), with the forward kernel's DFT tables:
+//   phase 0  X^T [12][k][row]: the Fourier rows of S_t (eta, Phi, U, V) and the
+//            Fourier cotangents Fbar = assemble^T(Tbar) (swx_physics.h), the
+//            latter scaled by 1 / (I w_m) so that DFT^T = Tinv X~:
+//            xbar_i = (1/I) sum_m (Fbar.re cos - Fbar.im sin)
+//                   = sum_k Tinv[i][k] Fbar[k] / (I w_m);
+//   phase 1  GEMM 1 (v_mfma_f32_16x16x4f32): the grid fields of S_t and the
+//            product cotangents on one 16-wide longitude tile per wave;
+//   pointwise  grid_products_adjoint (the Q < 0 clamp included) and the
+//            per-row partials of taurad, taudrag, DPhieq;
+//   phase 2  GEMM 2: invDFT^T = Tinv^T = diag(I w_m) Tfwd, wave w owning the
+//            cotangent of eta, Phi, U or V over all k' tiles;
+//   output   Gbar (delta's cotangent is zero) and pgrad.
+// Reference functions differentiated: spectral_transform.py:290-348,
+// forcing.py:65-155, initial_conditions.py:242-292.
+#include "swx_adjoint.h"
+#include "swx_grid_common.h"
+
+namespace swx {
+
+namespace {
+
+template <int MT>
+struct AdjShape {
+  static constexpr int Mp = MT + 1;
+  static constexpr int I = MT == 42 ? 128 : (MT == 63 ? 192 : 320);
+  static constexpr int K1 = 2 * Mp;
+  static constexpr int K1p = (K1 + 3) / 4 * 4;
+  static constexpr int KQ1 = (K1p / 4 + 3) / 4 * 4;  // per-lane run of the inverse table (its layout)
+  static constexpr int K2p = (K1 + 15) / 16 * 16;
+  static constexpr int NT2 = K2p / 16;
+  static constexpr int ICH = 64;
+  static constexpr int NX = 12;  // GEMM-1 fields: eta, Phi, U, V of S_t, then the 8 Fourier cotangents
+  // LDS floats: X^T [NX][K1p][16], grid cotangents [4][ICH][16], row partials [4 waves][16][3] doubles.
+  static constexpr int LDS_FLOATS = NX * K1p * 16 + 4 * ICH * 16 + 2 * 4 * 16 * 3;
+};
+
+}  // namespace
+
+template <int MT>
+__global__ void __launch_bounds__(256) k_grid_adj_mfma(GridAdjArgs<float> a, const float* __restrict__ TinvF,
+                                                       const float* __restrict__ TfwdF) {
+  using S = AdjShape<MT>;
+  constexpr int Mp = S::Mp, I = S::I, K1p = S::K1p, KQ1 = S::KQ1, NT2 = S::NT2, ICH = S::ICH, NX = S::NX;
+  extern __shared__ __align__(16) float lds[];
+  float* sXT = lds;                                                 // [NX][K1p][16]
+  float* sYT = lds + NX * K1p * 16;                                 // [4][ICH][16]
+  double* sRed = reinterpret_cast<double*>(sYT + 4 * ICH * 16);     // [4][16][3]
+
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, ln = lane & 15, lg = lane >> 4;
+  const int J = a.J, N = a.N;
+  const int row0 = blockIdx.x * 16;
+  const int b = row0 / J, j0 = row0 % J;
+  const double* cf = a.coef + (long long)b * a.ncoef;
+  const CoefLayout cl = CoefLayout::make(N, J);
+  const GeoLayout gl = GeoLayout::make(J, I);
+  const double* geo = a.geo;
+  const long long plane = (long long)J * I;
+
+  // Phase 0a: Fourier rows of S_t, G fields 0, 2, 3, 4 -> X^T fields 0..3 (float4 loads first).
+  {
+    constexpr int NE = 4 * (K1p / 2) * 8;
+    constexpr int NQ = (NE + 255) / 256;
+    v4f gv[NQ];
+#pragma unroll
+    for (int u = 0; u < NQ; ++u) {
+      const int e = tid + 256 * u;
+      const int jq = e & 7, fm = e >> 3;
+      const int fx = fm / (K1p / 2) < 4 ? fm / (K1p / 2) : 3, m = fm % (K1p / 2);
+      const int fg = fx == 0 ? 0 : fx + 1;
+      const v4f v = *reinterpret_cast<const v4f*>(a.G + fidx(b, fg, m < Mp ? m : Mp - 1, j0 + 2 * jq, Mp, J));
+      gv[u] = (e < NE && m < Mp) ? v : v4f{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int u = 0; u < NQ; ++u) {
+      const int e = tid + 256 * u;
+      if (e < NE) {
+        const int jq = e & 7, fm = e >> 3;
+        const int fx = fm / (K1p / 2), m = fm % (K1p / 2);
+        float* xr = sXT + (fx * K1p + 2 * m) * 16 + 2 * jq;
+        *reinterpret_cast<float2*>(xr) = make_float2(gv[u][0], gv[u][2]);
+        *reinterpret_cast<float2*>(xr + 16) = make_float2(gv[u][1], gv[u][3]);
+      }
+    }
+  }
+  // Phase 0b: Fourier cotangents per (m, row) -> X^T fields 4..11, scaled by 1 / (I w_m).
+  for (int p = tid; p < (K1p / 2) * 16; p += 256) {
+    const int m = p >> 4, r = p & 15;
+    float* xr = sXT + (4 * K1p + 2 * m) * 16 + r;
+    if (m < Mp) {
+      const int jj = j0 + r;
+      const float* Tb = a.Tbar + (((long long)b * Mp + m) * kNT) * J * 2;
+      double tau[kNT][2];
+#pragma unroll
+      for (int k = 0; k < kNT; ++k) {
+        tau[k][0] = Tb[(k * J + jj) * 2];
+        tau[k][1] = Tb[(k * J + jj) * 2 + 1];
+      }
+      double fb[8][2];
+      assemble_slots_adjoint(cf, cf[cl.n_tc + jj], geo[gl.w + jj], m, a.forcflag, tau, fb);
+      const double sc = 1.0 / ((double)I * (m == 0 ? 1.0 : 2.0));
+#pragma unroll
+      for (int f = 0; f < 8; ++f) {
+        xr[(f * K1p) * 16] = (float)(fb[f][0] * sc);
+        xr[(f * K1p + 1) * 16] = (float)(fb[f][1] * sc);
+      }
+    } else {
+#pragma unroll
+      for (int f = 0; f < 8; ++f) {
+        xr[(f * K1p) * 16] = 0.0f;
+        xr[(f * K1p + 1) * 16] = 0.0f;
+      }
+    }
+  }
+  __syncthreads();
+
+  v4f acc[NT2];
+#pragma unroll
+  for (int t = 0; t < NT2; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+  double dpar[3] = {0.0, 0.0, 0.0};
+  const int j = j0 + ln;
+  const v4f* Tf = reinterpret_cast<const v4f*>(TfwdF) + lane;
+
+  for (int ic = 0; ic < I; ic += ICH) {
+    // ---- Phase 1: GEMM 1 on this wave's longitude tile, 12 fields.
+    const int ib = ic + 16 * wave;
+    v4f afr[KQ1 / 4];
+    {
+      const v4f* Ta = reinterpret_cast<const v4f*>(TinvF + ((long long)(ib / 16) * 64 + lane) * KQ1);
+#pragma unroll
+      for (int q4 = 0; q4 < KQ1 / 4; ++q4) afr[q4] = Ta[q4];
+    }
+    v4f an[NT2];
+#pragma unroll
+    for (int t = 0; t < NT2; ++t) an[t] = Tf[((long long)t * (I / 16) + ic / 16) * 64];
+    v4f c1[NX];
+#pragma unroll
+    for (int f = 0; f < NX; ++f) c1[f] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < K1p / 4; ++ks) {
+      const float* xb = sXT + (4 * ks + lg) * 16 + ln;
+#pragma unroll
+      for (int f = 0; f < NX; ++f)
+        c1[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(afr[ks / 4][ks % 4], xb[f * K1p * 16], c1[f], 0, 0, 0);
+    }
+    // ---- Pointwise adjoint on this lane's 4 points (row ln, i = ib + 4 lg + q).
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int il = 16 * wave + 4 * lg + q;
+      const int i = ic + il;
+      double U = c1[2][q], V = c1[3][q];
+      if (a.test == 1) {
+        U = a.uvic[((long long)b * 2 + 0) * plane + (long long)j * I + i];
+        V = a.uvic[((long long)b * 2 + 1) * plane + (long long)j * I + i];
+      }
+      double pbar[8];
+#pragma unroll
+      for (int f = 0; f < 8; ++f) pbar[f] = c1[4 + f][q];
+      double o[4];
+      grid_products_adjoint(cf, geo, gl, j, i, a.test, c1[0][q], c1[1][q], U, V, pbar, o, dpar);
+      sYT[(0 * ICH + il) * 16 + ln] = (float)o[0];
+      sYT[(1 * ICH + il) * 16 + ln] = (float)o[1];
+      // The pinned winds of test 1 do not depend on the state.
+      sYT[(2 * ICH + il) * 16 + ln] = a.test == 1 ? 0.0f : (float)o[2];
+      sYT[(3 * ICH + il) * 16 + ln] = a.test == 1 ? 0.0f : (float)o[3];
+    }
+    __syncthreads();
+    // ---- Phase 2: GEMM 2 partial sums, wave w owning cotangent field w; the next
+    // 16-longitude group's A fragments are in flight while the current one runs.
+#pragma unroll 1
+    for (int gg = 0; gg < ICH / 16; ++gg) {
+      v4f ac[NT2];
+#pragma unroll
+      for (int t = 0; t < NT2; ++t) ac[t] = an[t];
+      if (gg + 1 < ICH / 16) {
+#pragma unroll
+        for (int t = 0; t < NT2; ++t) an[t] = Tf[((long long)t * (I / 16) + ic / 16 + gg + 1) * 64];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float bv = sYT[(wave * ICH + 16 * gg + 4 * q + lg) * 16 + ln];
+#pragma unroll
+        for (int t = 0; t < NT2; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[t][q], bv, acc[t], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+
+  // Per-row parameter partials: row ln is held by lanes ln + 16 g of every wave.
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double v = dpar[c];
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    dpar[c] = v;
+  }
+  if (lg == 0)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) sRed[(wave * 16 + ln) * 3 + c] = dpar[c];
+  __syncthreads();
+  if (tid < 16 * 3) {
+    const int r = tid / 3, c = tid % 3;
+    const double sum = sRed[(0 * 16 + r) * 3 + c] + sRed[(1 * 16 + r) * 3 + c] + sRed[(2 * 16 + r) * 3 + c] +
+                       sRed[(3 * 16 + r) * 3 + c];
+    a.pgrad[((long long)b * J + j0 + r) * 3 + c] += sum;
+  }
+
+  // Gbar = I w_m (Tfwd xbar): wave w -> field eta, Phi, U, V (G index 0, 2, 3, 4).
+  const int fo = wave == 0 ? 0 : wave + 1;
+#pragma unroll
+  for (int t = 0; t < NT2; ++t) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int kp = 16 * t + 4 * lg + q;
+      const int m = kp >> 1, c = kp & 1;
+      if (m < Mp) {
+        const float v = (m == 0 && c == 1) ? 0.0f : (float)I * (m == 0 ? 1.0f : 2.0f) * acc[t][q];
+        a.Gbar[fidx(b, fo, m, j, Mp, J) + c] = v;
+      }
+    }
+  }
+  // delta's cotangent is zero (the products do not depend on delta).
+  for (int p = tid; p < Mp * 16 * 2; p += 256) {
+    const int m = p >> 5, r = (p >> 1) & 15, c = p & 1;
+    a.Gbar[fidx(b, 1, m, j0 + r, Mp, J) + c] = 0.0f;
+  }
+}
+
+template <int MT>
+static hipError_t launch_adj_mfma_t(const GridAdjArgs<float>& a, const float* TinvF, const float* TfwdF,
+                                    hipStream_t s) {
+  const size_t smem = sizeof(float) * AdjShape<MT>::LDS_FLOATS;
+  hipLaunchKernelGGL(k_grid_adj_mfma<MT>, dim3((unsigned)((long long)a.B * a.J / 16)), dim3(256), smem, s, a, TinvF,
+                     TfwdF);
+  return hipGetLastError();
+}
+
+size_t grid_adj_mfma_lds_bytes(int M) {
+  switch (M) {
+    case 42: return sizeof(float) * AdjShape<42>::LDS_FLOATS;
+    case 63: return sizeof(float) * AdjShape<63>::LDS_FLOATS;
+    default: return 0;  // T106: the 12-field staging exceeds the workgroup's LDS
+  }
+}
+
+hipError_t launch_grid_adj_mfma(const GridAdjArgs<float>& a, const float* TinvF, const float* TfwdF, hipStream_t s) {
+  switch (a.M) {
+    case 42: return launch_adj_mfma_t<42>(a, TinvF, TfwdF, s);
+    case 63: return launch_adj_mfma_t<63>(a, TinvF, TfwdF, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace swx

@@ -82,12 +82,14 @@ struct swx_plan {
   float* TfwdF = nullptr;
   bool use_mfma = false;       // MFMA grid kernel available for this plan
   bool use_bf16x3 = false;     // split-BF16x3 grid kernel (precision SWX_PREC_BF16X3)
+  bool use_adj_mfma = false;   // MFMA grid adjoint kernel (FP32 transforms, where its LDS fits)
   void* TinvB = nullptr;       // its DFT tables: bf16 hi / lo pairs in fragment order
   void* TfwdB = nullptr;
   bool opt_plain_grid = false;  // SWX_OPT_GRID_KERNEL = 1
   bool opt_full_phys = false;   // SWX_OPT_FULL_PHYS_CARRY = 1
   bool use_leg_mfma = false;    // MFMA Legendre kernel available
   bool opt_plain_leg = false;   // SWX_OPT_LEG_KERNEL = 1
+  int opt_phys_steps = 0;       // SWX_OPT_PHYS_STEPS (0 = automatic, -1 = never)
   double* geo_d = nullptr;
   double* f2 = nullptr;
   double* coef = nullptr;
@@ -344,6 +346,8 @@ int do_init(swx_plan* p, const double* eta0, const double* delta0, const double*
 // before the window do not need to write the physical carry.
 int phys_window(const swx_plan* p) {
   if (p->opt_full_phys || p->d.diagnostics) return 1 << 30;
+  if (p->opt_phys_steps > 0) return p->opt_phys_steps;
+  if (p->opt_phys_steps < 0) return 0;
   double amax = 0.0;
   for (int b = 0; b < p->d.n_members; ++b) amax = std::max(amax, std::fabs(p->params[(size_t)b * SWX_NPARAM + SWX_P_ALPHA]));
   if (amax >= 0.5) return 1 << 30;
@@ -414,7 +418,8 @@ int prime(swx_plan* p, bool init, hipStream_t s) {
   g.diagnostics = init ? p->d.diagnostics : 0;
   g.init_from_g = init ? 1 : 0;
   g.phys_on = init ? 1 : 0;
-  SWX_HIP(launch_grid<R>(g, s));
+  // A restored state needs only the Legendre inputs: the step's grid kernel (MFMA when eligible).
+  SWX_HIP(init ? launch_grid<R>(g, s) : launch_grid_any<R>(p, g, s));
   if (init) {
     const size_t gbytes = (size_t)p->d.n_members * kNG * p->geo.J * (p->geo.M + 1) * 2 * p->rsz;
     SWX_HIP(hipMemcpyAsync(p->Gprev, p->G, gbytes, hipMemcpyDeviceToDevice, s));
@@ -498,7 +503,16 @@ int adjoint_step(swx_plan* p, const double* St, hipStream_t s) {
   LegAdjArgs<R> a = leg_adj_args<R>(p, LADJ_A);
   a.St = St;
   SWX_HIP(launch_leg_adj<R>(a, s));
-  SWX_HIP(launch_grid_adj<R>(grid_adj_args<R>(p, GADJ_STEP), s));
+  const GridAdjArgs<R> g = grid_adj_args<R>(p, GADJ_STEP);
+  if constexpr (sizeof(R) == 4) {
+    if (p->use_adj_mfma && !p->opt_plain_grid) {
+      SWX_HIP(launch_grid_adj_mfma(g, p->TinvF, p->TfwdF, s));
+    } else {
+      SWX_HIP(launch_grid_adj<R>(g, s));
+    }
+  } else {
+    SWX_HIP(launch_grid_adj<R>(g, s));
+  }
   SWX_HIP(launch_leg_adj<R>(leg_adj_args<R>(p, LADJ_B), s));
   return SWX_OK;
 }
@@ -665,6 +679,8 @@ int swx_plan_create(const swx_plan_desc* desc, const double* params, swx_plan** 
     int maxlds = 0;
     (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device);
     p->use_mfma = grid_mfma_lds_bytes(M) <= (size_t)maxlds;
+    const size_t adj_lds = grid_adj_mfma_lds_bytes(M);
+    p->use_adj_mfma = p->use_mfma && adj_lds > 0 && adj_lds <= (size_t)maxlds;
     p->use_leg_mfma = leg_mfma_lds_bytes(M) <= (size_t)maxlds;
     if (p->use_leg_mfma) {
       // Fragment-ordered copies of the basis blocks (layouts in LegArgs).
@@ -980,6 +996,10 @@ int swx_plan_set_option(swx_plan* p, int option, int value) {
     case SWX_OPT_LEG_KERNEL:
       if (value != 0 && value != 1) return fail(SWX_ERR_ARG, "SWX_OPT_LEG_KERNEL must be 0 (auto) or 1 (plain)");
       p->opt_plain_leg = value == 1;
+      break;
+    case SWX_OPT_PHYS_STEPS:
+      if (value < -1) return fail(SWX_ERR_ARG, "SWX_OPT_PHYS_STEPS must be >= -1");
+      p->opt_phys_steps = value;
       break;
     default:
       return fail(SWX_ERR_ARG, "unknown option " + std::to_string(option));
