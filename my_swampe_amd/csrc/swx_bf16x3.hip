@@ -124,6 +124,7 @@ __global__ void __launch_bounds__(256) k_grid_bf16x3(GridArgs<float> a, const v8
 
   const int n = ln;
   const int j = j0 + n;
+  const double inv_denE = 1.0 / geo[gl.denomE + j];
   for (int ic = 0; ic < I; ic += ICH) {
     // ---- Phase 1: inverse DFT of this wave's 16-wide longitude tile, all five fields.
     const int it = ic / 16 + wave;
@@ -162,7 +163,7 @@ __global__ void __launch_bounds__(256) k_grid_bf16x3(GridArgs<float> a, const v8
       for (int q = 0; q < 4; ++q) {
         double prod[8];
         grid_point_step(a, cf, geo, gl, phys, plane, alpha, b, j, ic + 16 * wave + 4 * lg + q, c1[0][q], c1[1][q],
-                        c1[2][q], c1[3][q], c1[4][q], prod);
+                        c1[2][q], c1[3][q], c1[4][q], inv_denE, prod);
 #pragma unroll
         for (int f = 0; f < 8; ++f) pr[f][q] = (float)prod[f];
       }

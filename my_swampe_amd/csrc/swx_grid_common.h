@@ -13,11 +13,12 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 // One grid point (row j, longitude i) of member b: pinned winds (test 1), the
 // Robert-Asselin filter of the physical prev level and the physical carry (when
 // a.phys_on; model.py:988-1006), then the 8 products of the forward transform
-// (forcing.py, initial_conditions.py:283-292).
+// (forcing.py, initial_conditions.py:283-292) in reciprocal form (inv_denE =
+// 1 / (2 (1 - mu_j^2))).
 __device__ __forceinline__ void grid_point_step(const GridArgs<float>& a, const double* cf, const double* geo,
                                                 const GeoLayout& gl, double* phys, long long plane, double alpha,
                                                 int b, int j, int i, double eta, double delta, double Phi, double U,
-                                                double V, double prod[8]) {
+                                                double V, double inv_denE, double prod[8]) {
   const long long pix = (long long)j * a.I + i;
   if (a.test == 1) {
     U = a.uvic[((long long)b * 2 + 0) * plane + pix];
@@ -42,7 +43,7 @@ __device__ __forceinline__ void grid_point_step(const GridArgs<float>& a, const 
     phys[6 * plane + pix] = U;
     phys[7 * plane + pix] = V;
   }
-  grid_products(cf, geo, gl, j, i, a.test, eta, Phi, U, V, prod);
+  grid_products_recip(cf, geo, gl, j, i, a.test, eta, Phi, U, V, inv_denE, prod);
 }
 
 // Assembly of the 9 Legendre inputs of 16 rows (member b, latitudes j0..j0+15)

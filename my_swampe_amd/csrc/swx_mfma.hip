@@ -102,6 +102,7 @@ __device__ __forceinline__ void grid_mfma_body(const GridArgs<float>& a, const f
 
   const int n = ln;
   const int j = j0 + n;
+  const double inv_denE = 1.0 / geo[gl.denomE + j];
   for (int ic = 0; ic < I; ic += ICH) {
     // ---- Phase 1: inverse DFT of one 16-wide i-tile, all five fields.  The
     // wave's A fragments (its i-tile of Tinv) are loaded up front.
@@ -133,7 +134,7 @@ __device__ __forceinline__ void grid_mfma_body(const GridArgs<float>& a, const f
       const int il = 16 * wave + 4 * lg + q;
       double prod[8];
       grid_point_step(a, cf, geo, gl, phys, plane, alpha, b, j, ic + il, c1[0][q], c1[1][q], c1[2][q], c1[3][q],
-                      c1[4][q], prod);
+                      c1[4][q], inv_denE, prod);
 #pragma unroll
       for (int f = 0; f < 8; ++f) sYT[(f * ICH + il) * 16 + n] = (float)prod[f];
     }
@@ -268,8 +269,11 @@ struct LegShape {
   static constexpr int XP = ceil16(10 * MB), XH = ceil16(4 * MB), XT = XP + XH;  // inverse columns
   static constexpr int KNP = ceil16(Np);  // padded degree rows
   static constexpr int JO = J + 4;         // column stride of the inverse results (bank spread)
-  static constexpr int SA = cmax(cmax(J * RT, KNP * RT), XT * JO);
-  static constexpr int SX = KNP * XT;
+  // Odd row strides (one unused column): consecutive rows start in different LDS banks, so
+  // the per-row accesses of the staging and update phases do not conflict.
+  static constexpr int RS = RT + 1, XS = XT + 1;
+  static constexpr int SA = cmax(cmax(J * RS, KNP * RS), XT * JO);
+  static constexpr int SX = KNP * XS;
   static constexpr int LDS_FLOATS = SA + SX;
 };
 
@@ -278,10 +282,11 @@ template <int MT, int MB>
 __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int m, const int grp, float* lds) {
   using S = LegShape<MT, MB>;
   constexpr int Np = S::Np, Mp = MT + 1, J = S::J, RP = S::RP, RT = S::RT, XP = S::XP, XT = S::XT, JO = S::JO;
+  constexpr int RS = S::RS, XS = S::XS;
   constexpr int KNP = S::KNP;
   static_assert(KNP == leg_knp(MT), "fragment tables use leg_knp rows");
-  float* sA = lds;           // [J][RT] inputs, then [K][RT] forward results, then [XT][JO] inverse results
-  float* sX = lds + S::SA;   // [KNP][XT] inverse-transform inputs
+  float* sA = lds;           // [J][RS] inputs, then [K][RS] forward results, then [XT][JO] inverse results
+  float* sX = lds + S::SA;   // [KNP][XS] inverse-transform inputs
   const int b0 = grp * MB;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, ln = lane & 15, lg = lane >> 4;
   const int B = a.B, N = a.N;
@@ -311,17 +316,20 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
         const int mb = q / QPM, r = q % QPM;
         const int k = r / (J / 2), j = 2 * (r % (J / 2));
         const int col = k < kNP ? mb * 10 + 2 * k : RP + mb * 8 + 2 * (k - kNP);
-        *reinterpret_cast<float2*>(sA + j * RT + col) = make_float2(tv[u][0], tv[u][1]);
-        *reinterpret_cast<float2*>(sA + (j + 1) * RT + col) = make_float2(tv[u][2], tv[u][3]);
+        float* d0 = sA + j * RS + col;
+        d0[0] = tv[u][0];
+        d0[1] = tv[u][1];
+        d0[RS] = tv[u][2];
+        d0[RS + 1] = tv[u][3];
       }
     }
   }
   for (int e = tid; e < S::SX; e += 256) sX[e] = 0.0f;
   constexpr int padH = RT - RP - MB * 8, padP = RP - MB * 10;
   if constexpr (padH > 0)
-    for (int e = tid; e < J * padH; e += 256) sA[(e / padH) * RT + RP + MB * 8 + e % padH] = 0.0f;
+    for (int e = tid; e < J * padH; e += 256) sA[(e / padH) * RS + RP + MB * 8 + e % padH] = 0.0f;
   if constexpr (padP > 0)
-    for (int e = tid; e < J * padP; e += 256) sA[(e / padP) * RT + MB * 10 + e % padP] = 0.0f;
+    for (int e = tid; e < J * padP; e += 256) sA[(e / padP) * RS + MB * 10 + e % padP] = 0.0f;
   __syncthreads();
 
   // Phase F: forward contraction tiles (Kt row tiles x RT/16 column tiles).
@@ -345,7 +353,7 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
       for (int q4 = 0; q4 < J / 16; ++q4) av[q4] = ar[q4];
 #pragma unroll
       for (int ks = 0; ks < J / 4; ++ks)
-        acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[ks / 4][ks % 4], sA[(4 * ks + lg) * RT + ct * 16 + ln],
+        acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[ks / 4][ks % 4], sA[(4 * ks + lg) * RS + ct * 16 + ln],
                                                       acc[q], 0, 0, 0);
     }
   }
@@ -356,7 +364,7 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
     if (t < ntile) {
       const int nt16 = t / CT, ct = t % CT;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) sA[(nt16 * 16 + 4 * lg + r) * RT + ct * 16 + ln] = acc[q][r];
+      for (int r = 0; r < 4; ++r) sA[(nt16 * 16 + 4 * lg + r) * RS + ct * 16 + ln] = acc[q][r];
     }
   }
   __syncthreads();
@@ -370,9 +378,9 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
     const double* cf = a.coef + (long long)b * a.ncoef;
     double fp[5][2], fh[4][2];
     for (int k = 0; k < 5; ++k)
-      for (int c = 0; c < 2; ++c) fp[k][c] = sA[np * RT + mb * 10 + 2 * k + c];
+      for (int c = 0; c < 2; ++c) fp[k][c] = sA[np * RS + mb * 10 + 2 * k + c];
     for (int k = 0; k < 4; ++k)
-      for (int c = 0; c < 2; ++c) fh[k][c] = sA[np * RT + RP + mb * 8 + 2 * k + c];
+      for (int c = 0; c < 2; ++c) fh[k][c] = sA[np * RS + RP + mb * 8 + 2 * k + c];
     double* Sb = a.S + (long long)b * 2 * 3 * Mp * Np * 2;
     const long long fstride = (long long)Mp * Np * 2;
     const long long so = ((long long)m * Np + n) * 2;
@@ -380,7 +388,7 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
     spectral_update(cf, cl, a.f2[(long long)m * Np + n], n, a.expflag, a.diffflag, fp, fh, Sb + 3 * fstride, Sb,
                     so, fstride, eta_n, delta_n, Phi_n);
     wind_potentials(cf, cl, m, n, eta_n, delta_n, dh, eh);
-    float* xr = sX + np * XT;
+    float* xr = sX + np * XS;
     for (int c = 0; c < 2; ++c) {
       xr[mb * 10 + 0 + c] = (float)eta_n[c];
       xr[mb * 10 + 2 + c] = (float)delta_n[c];
@@ -418,7 +426,7 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
 #pragma unroll
     for (int ks = 0; ks < KSM; ++ks) {
       if (ks < ksn) {
-        const float* xr = sX + (4 * ks + lg) * XT + ln;
+        const float* xr = sX + (4 * ks + lg) * XS + ln;
 #pragma unroll
         for (int c = 0; c < CIP; ++c)
           dP[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(pa[ks / 4][ks % 4], xr[c * 16], dP[c], 0, 0, 0);

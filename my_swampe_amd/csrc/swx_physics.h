@@ -45,6 +45,37 @@ __device__ inline void grid_products(const double* cf, const double* geo, const 
   out[7] = G;
 }
 
+// grid_products with reciprocals in place of the divisions (1/(Phi + Phibar) once per
+// point, 1/taudrag from the coefficient table, 1/(2(1-mu^2)) passed in by the caller, who
+// holds one latitude per lane): for the FP32-transform kernels, whose products are rounded
+// to FP32, where the last-bit differences of the FP64 intermediates do not show.
+__device__ inline void grid_products_recip(const double* cf, const double* geo, const GeoLayout& gl, int j, int i,
+                                           int test, double eta, double Phi, double U, double V, double inv_denE,
+                                           double out[8]) {
+  const double Phibar = cf[C_PHIBAR];
+  double PhiF = 0.0, F = 0.0, G = 0.0;
+  if (test < 0) {
+    const double Phieq =
+        geo[gl.day + i] != 0.0 ? Phibar + (cf[C_DPHIEQ] * geo[gl.cosl + i]) * geo[gl.s1m + j] : Phibar;
+    const double Q = cf[C_INVTAURAD] * (Phieq - (Phi + Phibar));
+    const double Qc = Q < 0.0 ? 0.0 : Q;
+    const double tot = Phi + Phibar;
+    const double r = -Qc / (fabs(tot) > 0.0 ? tot : DBL_MIN);
+    const double itd = cf[C_INVTD];  // 0 for the taudrag == -1 sentinel
+    PhiF = Q;
+    F = U * r - U * itd;
+    G = V * r - V * itd;
+  }
+  out[0] = U * eta;
+  out[1] = V * eta;
+  out[2] = U * Phi;
+  out[3] = V * Phi;
+  out[4] = (U * U + V * V) * inv_denE;
+  out[5] = PhiF;
+  out[6] = F;
+  out[7] = G;
+}
+
 // w-weighted Legendre inputs of one (member, j, m) from the truncated Fourier
 // coefficients f[8][2] (A, B, C, D, E, PhiF, F, G) and, for the explicit
 // scheme, the wind coefficients Um/Vm.  P-basis slots 0..4, H-basis 5..8:
