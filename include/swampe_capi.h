@@ -230,7 +230,14 @@ int swx_adjoint_result(swx_plan *plan, double *lam_out, double *pgrad_out, void 
 #define SWX_OPT_FULL_PHYS_CARRY 2
 #define SWX_OPT_LEG_KERNEL 3
 #define SWX_OPT_PHYS_STEPS 4
+/*   SWX_OPT_LEG_MB: members per workgroup of the MFMA Legendre kernel; set at
+ *     plan creation to the fastest of the instantiated sizes (4, 8 at T42;
+ *     2, 4, 8 at T63 and T106), timed on the plan's own buffers; setting it
+ *     forces another available size.  Results do not depend on it. */
+#define SWX_OPT_LEG_MB 5
 int swx_plan_set_option(swx_plan *plan, int option, int value);
+/* Current value of an option (for SWX_OPT_LEG_MB: the size in use). */
+int swx_plan_get_option(const swx_plan *plan, int option, int *value);
 
 /* Last error message of the calling thread. */
 const char *swx_last_error(void);

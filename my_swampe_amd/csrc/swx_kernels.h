@@ -171,8 +171,11 @@ hipError_t launch_grid_mfma_pair(const GridArgs<float>& a0, const float* ti0, co
                                  const GridArgs<float>& a1, const float* ti1, const float* tf1, hipStream_t s);
 hipError_t launch_leg_mfma_pair(const LegArgs<float>& a0, const LegArgs<float>& a1, hipStream_t s);
 // MFMA Legendre kernel (FP32, LEG_STEP without diagnostics).
-hipError_t launch_leg_mfma(const LegArgs<float>& a, hipStream_t s);
-size_t leg_mfma_lds_bytes(int M);
+// mb = members per workgroup, one of leg_mfma_candidates(M) (the plan picks the fastest).
+hipError_t launch_leg_mfma(const LegArgs<float>& a, int mb, hipStream_t s);
+size_t leg_mfma_lds_bytes(int M, int mb);
+int leg_mfma_candidates(int M, int* mbs);  // up to 3 values
+int leg_mfma_default_mb(int M);
 int grid_mfma_k2p(int M);
 
 template <typename R>

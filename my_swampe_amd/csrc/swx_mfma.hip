@@ -493,20 +493,43 @@ static hipError_t launch_leg_mfma_t(const LegArgs<float>& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-size_t leg_mfma_lds_bytes(int M) {
+// Member-group sizes instantiated per truncation (the plan times them at creation).
+int leg_mfma_candidates(int M, int* mbs) {
   switch (M) {
-    case 42: return sizeof(float) * LegShape<42, 8>::LDS_FLOATS;
-    case 63: return sizeof(float) * LegShape<63, 4>::LDS_FLOATS;
-    case 106: return sizeof(float) * LegShape<106, 4>::LDS_FLOATS;
+    case 42: mbs[0] = 4; mbs[1] = 8; return 2;
+    case 63:
+    case 106: mbs[0] = 2; mbs[1] = 4; mbs[2] = 8; return 3;
     default: return 0;
   }
 }
 
-hipError_t launch_leg_mfma(const LegArgs<float>& a, hipStream_t s) {
-  switch (a.M) {
-    case 42: return launch_leg_mfma_t<42, 8>(a, s);
-    case 63: return launch_leg_mfma_t<63, 4>(a, s);
-    case 106: return launch_leg_mfma_t<106, 4>(a, s);
+int leg_mfma_default_mb(int M) { return M == 42 ? 8 : 4; }
+
+size_t leg_mfma_lds_bytes(int M, int mb) {
+  const int key = M * 100 + mb;
+  switch (key) {
+    case 4204: return sizeof(float) * LegShape<42, 4>::LDS_FLOATS;
+    case 4208: return sizeof(float) * LegShape<42, 8>::LDS_FLOATS;
+    case 6302: return sizeof(float) * LegShape<63, 2>::LDS_FLOATS;
+    case 6304: return sizeof(float) * LegShape<63, 4>::LDS_FLOATS;
+    case 6308: return sizeof(float) * LegShape<63, 8>::LDS_FLOATS;
+    case 10602: return sizeof(float) * LegShape<106, 2>::LDS_FLOATS;
+    case 10604: return sizeof(float) * LegShape<106, 4>::LDS_FLOATS;
+    case 10608: return sizeof(float) * LegShape<106, 8>::LDS_FLOATS;
+    default: return 0;
+  }
+}
+
+hipError_t launch_leg_mfma(const LegArgs<float>& a, int mb, hipStream_t s) {
+  switch (a.M * 100 + mb) {
+    case 4204: return launch_leg_mfma_t<42, 4>(a, s);
+    case 4208: return launch_leg_mfma_t<42, 8>(a, s);
+    case 6302: return launch_leg_mfma_t<63, 2>(a, s);
+    case 6304: return launch_leg_mfma_t<63, 4>(a, s);
+    case 6308: return launch_leg_mfma_t<63, 8>(a, s);
+    case 10602: return launch_leg_mfma_t<106, 2>(a, s);
+    case 10604: return launch_leg_mfma_t<106, 4>(a, s);
+    case 10608: return launch_leg_mfma_t<106, 8>(a, s);
     default: return hipErrorInvalidValue;
   }
 }

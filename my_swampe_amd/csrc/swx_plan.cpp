@@ -88,6 +88,8 @@ struct swx_plan {
   bool opt_plain_grid = false;  // SWX_OPT_GRID_KERNEL = 1
   bool opt_full_phys = false;   // SWX_OPT_FULL_PHYS_CARRY = 1
   bool use_leg_mfma = false;    // MFMA Legendre kernel available
+  int leg_mb = 0;               // its members per workgroup (timed at plan creation)
+  float leg_mb_ms[3] = {0, 0, 0};  // the candidates' measured launch times
   bool opt_plain_leg = false;   // SWX_OPT_LEG_KERNEL = 1
   int opt_phys_steps = 0;       // SWX_OPT_PHYS_STEPS (0 = automatic, -1 = never)
   double* geo_d = nullptr;
@@ -299,7 +301,7 @@ hipError_t launch_grid_any(swx_plan* p, const GridArgs<R>& g, hipStream_t s) {
 template <typename R>
 hipError_t launch_leg_any(swx_plan* p, const LegArgs<R>& l, hipStream_t s) {
   if constexpr (sizeof(R) == 4) {
-    if (leg_uses_mfma(p, l)) return launch_leg_mfma(l, s);
+    if (leg_uses_mfma(p, l)) return launch_leg_mfma(l, p->leg_mb, s);
   }
   return launch_leg<R>(l, s);
 }
@@ -519,6 +521,44 @@ int adjoint_step(swx_plan* p, const double* St, hipStream_t s) {
 
 }  // namespace
 
+// Legendre tile autotuner: time each instantiated member-group size on this plan's own
+// buffers (zeroed Legendre inputs and state, 2 warm-up + 5 timed launches on the null
+// stream) and keep the fastest.  The state is zeroed again afterwards.
+int autotune_leg(swx_plan* p) {
+  int mbs[3];
+  const int nc = leg_mfma_candidates(p->geo.M, mbs);
+  int maxlds = 0;
+  (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device);
+  const int B = p->d.n_members, J = p->geo.J, Mp = p->geo.M + 1, Np = p->geo.N + 1;
+  const size_t sbytes = (size_t)B * 2 * 3 * Mp * Np * 2 * 8;
+  SWX_HIP(hipMemset(p->T, 0, (size_t)B * Mp * kNT * J * 2 * p->rsz));
+  hipEvent_t e0, e1;
+  SWX_HIP(hipEventCreate(&e0));
+  SWX_HIP(hipEventCreate(&e1));
+  const LegArgs<float> l = leg_args<float>(p, LEG_STEP);
+  float best = 0.f;
+  for (int c = 0; c < nc; ++c) {
+    p->leg_mb_ms[c] = 0.f;
+    if (leg_mfma_lds_bytes(p->geo.M, mbs[c]) > (size_t)maxlds) continue;
+    for (int w = 0; w < 2; ++w) SWX_HIP(launch_leg_mfma(l, mbs[c], nullptr));
+    SWX_HIP(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < 5; ++r) SWX_HIP(launch_leg_mfma(l, mbs[c], nullptr));
+    SWX_HIP(hipEventRecord(e1, nullptr));
+    SWX_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    SWX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    p->leg_mb_ms[c] = ms / 5;
+    if (best == 0.f || ms / 5 < best) {
+      best = ms / 5;
+      p->leg_mb = mbs[c];
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  SWX_HIP(hipMemset(p->S, 0, sbytes));
+  return SWX_OK;
+}
+
 // Two FP32 plans of different truncations stepped together: each step is one Legendre launch
 // and one grid launch covering both plans' workgroups (MFMA pair kernels); a step where either
 // plan needs a plain kernel falls back to one launch per plan.
@@ -681,7 +721,8 @@ int swx_plan_create(const swx_plan_desc* desc, const double* params, swx_plan** 
     p->use_mfma = grid_mfma_lds_bytes(M) <= (size_t)maxlds;
     const size_t adj_lds = grid_adj_mfma_lds_bytes(M);
     p->use_adj_mfma = p->use_mfma && adj_lds > 0 && adj_lds <= (size_t)maxlds;
-    p->use_leg_mfma = leg_mfma_lds_bytes(M) <= (size_t)maxlds;
+    p->leg_mb = leg_mfma_default_mb(M);
+    p->use_leg_mfma = leg_mfma_lds_bytes(M, p->leg_mb) <= (size_t)maxlds;
     if (p->use_leg_mfma) {
       // Fragment-ordered copies of the basis blocks (layouts in LegArgs).
       const int KNP = leg_knp(N);
@@ -791,6 +832,10 @@ int swx_plan_create(const swx_plan_desc* desc, const double* params, swx_plan** 
       hipMemset(p->S, 0, (size_t)B * 2 * 3 * Mp * (N + 1) * 2 * 8) != hipSuccess) {
     swx_plan_destroy(p);
     return fail(SWX_ERR_HIP, "hipMemset failed");
+  }
+  if (p->use_leg_mfma && (rc = autotune_leg(p)) != SWX_OK) {
+    swx_plan_destroy(p);
+    return rc;
   }
   *out = p;
   return SWX_OK;
@@ -997,6 +1042,18 @@ int swx_plan_set_option(swx_plan* p, int option, int value) {
       if (value != 0 && value != 1) return fail(SWX_ERR_ARG, "SWX_OPT_LEG_KERNEL must be 0 (auto) or 1 (plain)");
       p->opt_plain_leg = value == 1;
       break;
+    case SWX_OPT_LEG_MB: {
+      int mbs[3];
+      const int nc = leg_mfma_candidates(p->geo.M, mbs);
+      bool ok = false;
+      for (int c = 0; c < nc; ++c) ok = ok || mbs[c] == value;
+      int maxlds = 0;
+      (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device);
+      if (!ok || leg_mfma_lds_bytes(p->geo.M, value) > (size_t)maxlds)
+        return fail(SWX_ERR_ARG, "SWX_OPT_LEG_MB: not an available member-group size for this truncation");
+      p->leg_mb = value;
+      break;
+    }
     case SWX_OPT_PHYS_STEPS:
       if (value < -1) return fail(SWX_ERR_ARG, "SWX_OPT_PHYS_STEPS must be >= -1");
       p->opt_phys_steps = value;
@@ -1081,6 +1138,19 @@ int swx_time_kernels(swx_plan* p, int t0, int nsteps, float* ms_leg, float* ms_g
   if (rc) return rc;
   *ms_grid = (float)grid_ms;
   *ms_leg = (float)(step_ms - grid_ms > 0.0 ? step_ms - grid_ms : 0.0);
+  return SWX_OK;
+}
+
+int swx_plan_get_option(const swx_plan* p, int option, int* value) {
+  if (!p || !value) return fail(SWX_ERR_ARG, "null plan or output");
+  switch (option) {
+    case SWX_OPT_GRID_KERNEL: *value = p->opt_plain_grid ? 1 : 0; break;
+    case SWX_OPT_FULL_PHYS_CARRY: *value = p->opt_full_phys ? 1 : 0; break;
+    case SWX_OPT_LEG_KERNEL: *value = p->opt_plain_leg ? 1 : 0; break;
+    case SWX_OPT_PHYS_STEPS: *value = p->opt_phys_steps; break;
+    case SWX_OPT_LEG_MB: *value = p->leg_mb; break;
+    default: return fail(SWX_ERR_ARG, "unknown option " + std::to_string(option));
+  }
   return SWX_OK;
 }
 

@@ -185,6 +185,11 @@ class Plan:
                     "swx_adjoint_result")
         return lam, pg
 
+    def get_option(self, option: int) -> int:
+        v = ctypes.c_int()
+        _capi.check(self._lib.swx_plan_get_option(self._h, int(option), ctypes.byref(v)), "swx_plan_get_option")
+        return int(v.value)
+
     def device_bytes(self) -> int:
         """Device memory held by the plan (tables, carry, work buffers, adjoint workspace)."""
         n = ctypes.c_size_t()

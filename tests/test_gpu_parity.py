@@ -424,3 +424,37 @@ def test_bf16x3_grid_kernel_vs_oracle(M, torch_gpu):
         rel = {k: float(np.linalg.norm(res[k][f] - r) / np.linalg.norm(r)) for k in res}
         print(M, f, "relL2 vs oracle", rel)
         assert rel["bf16x3"] < 2e-5, (M, f, rel)
+
+
+@pytest.mark.parametrize("M", [42, 63, 106])
+def test_legendre_member_group_sizes_agree_bitwise(M, torch_gpu):
+    """Every member-group size of the MFMA Legendre kernel (the autotuner's candidates) gives the
+    same state bitwise: a member's arithmetic does not depend on which members share its tile.
+    5 members (partial groups for every size), 6 forced steps, FP32 transforms."""
+    from my_swampe_amd import _capi
+    from my_swampe_amd.model import param_row
+    from my_swampe_amd.plan import Plan
+
+    torch = torch_gpu
+    _print_case("leg-mb", M=M)
+    dt = 60.0 if M == 106 else 120.0
+    rng = np.random.default_rng(5)
+    params = np.stack([param_row(dt=dt, a=8.2e7, omega=3.2e-5, g=9.8, Phibar=3e5, taurad=10 ** rng.uniform(4, 6),
+                                 taudrag=10 ** rng.uniform(4, 6), DPhieq=rng.uniform(1e5, 5e5), K6=1.24e33,
+                                 K6Phi=None, alpha=0.01) for _ in range(5)])
+    c = torch.tensor(_noise_spectral(M, 5, 11, 3.2e-5), device="cuda")
+    states = {}
+    tuned = None
+    for mb in ((4, 8) if M == 42 else (2, 4, 8)):
+        plan = Plan(M, params, precision="fp32")
+        tuned = plan.get_option(_capi.OPT_LEG_MB)
+        plan.set_option(_capi.OPT_LEG_MB, mb)
+        plan.init_spectral(c)
+        plan.run(2, 6)
+        states[mb] = plan.state_get()
+        torch.cuda.synchronize()
+        plan.close()
+    print(M, "autotuned member-group size", tuned)
+    ref = next(iter(states.values()))
+    for mb, st in states.items():
+        assert torch.equal(st, ref), (M, mb)
