@@ -226,6 +226,23 @@ def sigma6Phi(M, N, K6, a, dt):
     return np.broadcast_to(s[None, :], (M + 1, N + 1)).copy()
 
 
+def sigma6_exponential(M, N, K6, a, dt):
+    """Exponential (integrating-factor) 6th-order hyperdiffusion for eta/delta over a leapfrog
+    step, offset clamped at zero (filters.py:154-182)."""
+    n = np.arange(N + 1, dtype=np.float64)
+    ncoeff = ((n * n * n) / a ** 3) * (((n + 1) * (n + 1) * (n + 1)) / a ** 3)
+    s = np.exp(-(2.0 * dt * K6) * np.maximum(ncoeff - 8.0 / a ** 6, 0.0))
+    return np.broadcast_to(s[None, :], (M + 1, N + 1)).copy()
+
+
+def sigma6Phi_exponential(M, N, K6, a, dt):
+    """Exponential hyperdiffusion for Phi, no offset (filters.py:185-213)."""
+    n = np.arange(N + 1, dtype=np.float64)
+    ncoeff = ((n * n * n) / a ** 3) * (((n + 1) * (n + 1) * (n + 1)) / a ** 3)
+    s = np.exp(-(2.0 * dt * K6) * ncoeff)
+    return np.broadcast_to(s[None, :], (M + 1, N + 1)).copy()
+
+
 def coriolismn(M, omega):
     """f_mn with only [0, 1] = omega/sqrt(0.375) (initial_conditions.py:295-316)."""
     f = np.zeros((M + 1, M + 1))
@@ -373,6 +390,8 @@ class Static:
     sigma: np.ndarray
     sigmaPhi: np.ndarray
     Phieq: np.ndarray
+    sigma_exp: np.ndarray = None
+    sigmaPhi_exp: np.ndarray = None
 
 
 def build_static(*, M, dt, a, omega, g, Phibar, taurad, taudrag, DPhieq, K6, K6Phi, test) -> Static:
@@ -389,6 +408,7 @@ def build_static(*, M, dt, a, omega, g, Phibar, taurad, taudrag, DPhieq, K6, K6P
         tstepcoeff=tstepcoeff(J, M, dt, mus, a), tstepcoeff2=tstepcoeff2(J, M, dt, a),
         tstepcoeffmn=tstepcoeffmn(M, N, a), marray=marray(M, N), mJarray=mJarray(J, M), narray=narray(M, N),
         sigma=sigma6(M, N, K6, a, dt), sigmaPhi=sigma6Phi(M, N, K6Phi_eff, a, dt), Phieq=Phieq,
+        sigma_exp=sigma6_exponential(M, N, K6, a, dt), sigmaPhi_exp=sigma6Phi_exponential(M, N, K6Phi_eff, a, dt),
     )
 
 
@@ -557,9 +577,59 @@ def _explicit(st: OracleState, s: Static, forcflag: bool, diffflag: bool):
     return etamn, deltamn, Phimn
 
 
-def tstepping(st: OracleState, s: Static, forcflag: bool, diffflag: bool, expflag: bool):
-    """13-tuple scheme dispatch (time_stepping.py:16-384)."""
-    etamn, deltamn, Phimn = (_explicit if expflag else _modeuler)(st, s, forcflag, diffflag)
+def si_solve(delta_star, Phi_star, b, c, kd, kr):
+    """Closed-form per-degree implicit solve of (1 + kd) delta - b Phi = delta*, c delta + (1 + kr) Phi = Phi*
+    (semi_implicit_tdiff.py:232-236)."""
+    delta = (delta_star + b * Phi_star / (1.0 + kr)) / ((1.0 + kd) + b * c / (1.0 + kr))
+    return delta, (Phi_star - c * delta) / (1.0 + kr)
+
+
+def _semi_implicit(st: OracleState, s: Static, forcflag: bool, diffflag: bool, relax_implicit: bool,
+                   si_alpha: float):
+    """Semi-implicit leapfrog with implicit gravity waves and linear relaxation/drag, closed-form
+    per-degree solve, exponential hyperdiffusion (semi_implicit_tdiff.py:79-264)."""
+    Pw, Hw, mJ, fmn = s.Pmnw, s.Hmnw, s.mJarray, s.fmn
+    dt, a, Phibar = s.dt, s.a, s.Phibar
+    two_dt = 2.0 * dt
+    lam = s.narray / a ** 2
+    if relax_implicit:
+        inv_td = 0.0 if s.taudrag == -1.0 else 1.0 / s.taudrag
+        inv_tr = 1.0 / s.taurad
+        Fe, Ge = st.Rum_lag, st.Rvm_lag
+        PhiFe = st.PhiFm_curr + st.Phim_curr * inv_tr
+    else:
+        inv_td = inv_tr = 0.0
+        Fe, Ge, PhiFe = st.Fm_curr, st.Gm_curr, st.PhiFm_curr
+    Aeff = st.Am_curr - Ge if forcflag else st.Am_curr
+    Beff = st.Bm_curr + Fe if forcflag else st.Bm_curr
+    tc = s.tstepcoeff
+    etamn0, deltamn0, Phimn0, imA, imB, imC, Emn = fwd_leg(np.stack((
+        st.etam_prev, st.deltam_prev, st.Phim_prev, tc * 1j * mJ * Aeff, tc * 1j * mJ * Beff,
+        tc * 1j * mJ * st.Cm_curr, st.Em_curr)), Pw)
+    HA, HB, HD = fwd_leg(np.stack((tc * Aeff, tc * Beff, tc * st.Dm_curr)), Hw)
+    xi = two_dt * si_alpha
+    expl = two_dt * (1.0 - si_alpha)
+    kd, kr = xi * inv_td, xi * inv_tr
+    gd, gr = expl * inv_td, expl * inv_tr
+    eta_star = etamn0 - imA + HB - gd * (etamn0 - fmn)
+    etamn = (eta_star + kd * fmn) / (1.0 + kd)
+    delta_star = deltamn0 + imB + HA + two_dt * lam * Emn + expl * lam * Phimn0 - gd * deltamn0
+    Phi_star = Phimn0 - imC + HD - expl * Phibar * deltamn0 - gr * Phimn0
+    if forcflag:
+        Phi_star = Phi_star + fwd_leg(two_dt * PhiFe, Pw)
+    deltamn, Phimn = si_solve(delta_star, Phi_star, xi * lam, xi * Phibar, kd, kr)
+    if diffflag:
+        etamn, deltamn, Phimn = etamn * s.sigma_exp, deltamn * s.sigma_exp, Phimn * s.sigmaPhi_exp
+    return etamn, deltamn, Phimn
+
+
+def tstepping(st: OracleState, s: Static, forcflag: bool, diffflag: bool, expflag: bool, semi_implicit=False,
+              relax_implicit=False, si_alpha=0.5):
+    """13-tuple scheme dispatch (time_stepping.py:16-384, :387-462)."""
+    if semi_implicit:
+        etamn, deltamn, Phimn = _semi_implicit(st, s, forcflag, diffflag, relax_implicit, si_alpha)
+    else:
+        etamn, deltamn, Phimn = (_explicit if expflag else _modeuler)(st, s, forcflag, diffflag)
     I, J, M, N = s.I, s.J, s.M, s.N
     out = {}
     for name, mn in (("eta", etamn), ("delta", deltamn), ("Phi", Phimn)):
@@ -578,8 +648,11 @@ def tstepping(st: OracleState, s: Static, forcflag: bool, diffflag: bool, expfla
 
 
 def step_once(st: OracleState, t: int, s: Static, flags: dict, test, Uic, Vic):
-    """One step with the RA filter, blow-up gate and next-step forcing (model.py:795-1194)."""
+    """One step with the RA / RAW filter, blow-up gate, lagged-forcing pipeline and next-step
+    forcing (model.py:795-1194)."""
     diag = flags["diagnostics"]
+    si = flags.get("semi_implicit", False)
+    raw = flags.get("raw_filter", False)
     if diag:
         rms = rms_winds(s.a, s.I, s.J, s.lambdas, s.mus, st.U_curr, st.V_curr)
         spin_min = float(np.min(np.sqrt(st.U_curr * st.U_curr + st.V_curr * st.V_curr)))
@@ -591,33 +664,63 @@ def step_once(st: OracleState, t: int, s: Static, flags: dict, test, Uic, Vic):
                    V=st.V_curr, rms=rms, spin_min=spin_min, phi_min=float(np.min(st.Phi_curr)),
                    phi_max=float(np.max(st.Phi_curr)))
         return replace(st, dead=True), out
-    r = tstepping(st, s, flags["forcflag"], flags["diffflag"], flags["expflag"])
+    r = tstepping(st, s, flags["forcflag"], flags["diffflag"], flags["expflag"], semi_implicit=si,
+                  relax_implicit=bool(flags["forcflag"] and test is None), si_alpha=flags.get("si_alpha", 0.5))
     neweta, newdelta, newPhi = np.real(r["eta"]), np.real(r["delta"]), np.real(r["Phi"])
     newU, newV, newUm, newVm = np.real(r["U"]), np.real(r["V"]), r["Um"], r["Vm"]
+    newetam, newdeltam, newPhim = r["etam"], r["deltam"], r["Phim"]
     if test == 1:
         newU, newV, newUm, newVm = Uic, Vic, st.Um_curr, st.Vm_curr
     alpha = flags["alpha"]
-    if flags["modalflag"] and t > 2:
-        eta_mid = st.eta_curr + alpha * (st.eta_prev - 2.0 * st.eta_curr + neweta)
-        delta_mid = st.delta_curr + alpha * (st.delta_prev - 2.0 * st.delta_curr + newdelta)
-        Phi_mid = st.Phi_curr + alpha * (st.Phi_prev - 2.0 * st.Phi_curr + newPhi)
+    do_ra = flags["modalflag"] and t > 2
+    levels = (("eta", neweta, newetam), ("delta", newdelta, newdeltam), ("Phi", newPhi, newPhim))
+    mid, eff, mid_m, eff_m = {}, {}, {}, {}
+    if not raw:
+        # Classic Robert-Asselin on the physical prev level (model.py:991-1006).
+        for name, new, newm in levels:
+            cur, prv = getattr(st, name + "_curr"), getattr(st, name + "_prev")
+            mid[name] = cur + alpha * (prv - 2.0 * cur + new) if do_ra else cur
+            eff[name], eff_m[name] = new, newm
+            curm, prvm = getattr(st, name + "m_curr"), getattr(st, name + "m_prev")
+            if si:
+                # The leapfrog reads the Fourier prev carry: filter its lineage too (model.py:1089-1105).
+                mid_m[name] = curm + alpha * (prvm - 2.0 * curm + newm) if do_ra else curm
+            else:
+                mid_m[name] = curm  # locked default: unfiltered current level (model.py:1107-1110)
     else:
-        eta_mid, delta_mid, Phi_mid = st.eta_curr, st.delta_curr, st.Phi_curr
-    phi_min = float(np.min(Phi_mid)) if diag else 0.0
-    phi_max = float(np.max(Phi_mid)) if diag else 0.0
-    PhiF2, F2, G2 = forcing_phys(s, test, newPhi, newU, newV)
+        # Robert-Asselin-Williams on the physical and Fourier lineages (model.py:1007-1087).
+        w = flags.get("williams_alpha", 0.53)
+        for name, new, newm in levels:
+            cur, prv = getattr(st, name + "_curr"), getattr(st, name + "_prev")
+            curm, prvm = getattr(st, name + "m_curr"), getattr(st, name + "m_prev")
+            if do_ra:
+                d = prv - 2.0 * cur + new
+                dm = prvm - 2.0 * curm + newm
+                mid[name], eff[name] = cur + alpha * w * d, new - alpha * (1.0 - w) * d
+                mid_m[name], eff_m[name] = curm + alpha * w * dm, newm - alpha * (1.0 - w) * dm
+            else:
+                mid[name], eff[name], mid_m[name], eff_m[name] = cur, new, curm, newm
+    phi_min = float(np.min(mid["Phi"])) if diag else 0.0
+    phi_max = float(np.max(mid["Phi"])) if diag else 0.0
+    PhiF2, F2, G2 = forcing_phys(s, test, eff["Phi"], newU, newV)
     PhiFm2, Fm2, Gm2 = fwd_fft_trunc(np.stack((PhiF2, F2, G2)), s.I, s.M)
-    Am2, Bm2, Cm2, Dm2, Em2 = nonlinear_spectral(s, neweta, newPhi, newU, newV)
+    Am2, Bm2, Cm2, Dm2, Em2 = nonlinear_spectral(s, eff["eta"], eff["Phi"], newU, newV)
+    if si:
+        # Lagged momentum-forcing remainder of the incoming level (model.py:775-792, :1135-1143).
+        inv_td = 0.0 if s.taudrag == -1.0 else 1.0 / s.taudrag
+        Rum_next, Rvm_next = st.Fm_curr + st.Um_curr * inv_td, st.Gm_curr + st.Vm_curr * inv_td
+    else:
+        Rum_next, Rvm_next = st.Rum_lag, st.Rvm_lag
     new = OracleState(
-        etam_prev=st.etam_curr, etam_curr=r["etam"], deltam_prev=st.deltam_curr, deltam_curr=r["deltam"],
-        Phim_prev=st.Phim_curr, Phim_curr=r["Phim"],
-        eta_prev=eta_mid, eta_curr=neweta, delta_prev=delta_mid, delta_curr=newdelta,
-        Phi_prev=Phi_mid, Phi_curr=newPhi, U_curr=newU, V_curr=newV, Um_curr=newUm, Vm_curr=newVm,
+        etam_prev=mid_m["eta"], etam_curr=eff_m["eta"], deltam_prev=mid_m["delta"], deltam_curr=eff_m["delta"],
+        Phim_prev=mid_m["Phi"], Phim_curr=eff_m["Phi"],
+        eta_prev=mid["eta"], eta_curr=eff["eta"], delta_prev=mid["delta"], delta_curr=eff["delta"],
+        Phi_prev=mid["Phi"], Phi_curr=eff["Phi"], U_curr=newU, V_curr=newV, Um_curr=newUm, Vm_curr=newVm,
         Am_curr=Am2, Bm_curr=Bm2, Cm_curr=Cm2, Dm_curr=Dm2, Em_curr=Em2,
-        PhiFm_curr=PhiFm2, Fm_curr=Fm2, Gm_curr=Gm2, Rum_lag=st.Rum_lag, Rvm_lag=st.Rvm_lag,
+        PhiFm_curr=PhiFm2, Fm_curr=Fm2, Gm_curr=Gm2, Rum_lag=Rum_next, Rvm_lag=Rvm_next,
         dead=dead_next, spec=dict(eta=r["etamn"], delta=r["deltamn"], Phi=r["Phimn"]),
     )
-    out = dict(t=t, dead=dead_next, eta=neweta, delta=newdelta, Phi=newPhi, U=newU, V=newV,
+    out = dict(t=t, dead=dead_next, eta=eff["eta"], delta=eff["delta"], Phi=eff["Phi"], U=newU, V=newV,
                rms=rms, spin_min=spin_min, phi_min=phi_min, phi_max=phi_max)
     return new, out
 
@@ -626,12 +729,16 @@ def run_model_scan(*, M, dt, tmax, Phibar, omega, a, test=None, g=9.8, forcflag=
                    taudrag=86400.0, DPhieq=4e6, a1=0.05, diffflag=True, modalflag=True, alpha=0.01,
                    expflag=False, K6=1.24e33, K6Phi=None, starttime=None, eta0_init=None, delta0_init=None,
                    Phi0_init=None, U0_init=None, V0_init=None, diagnostics=True, return_history=True,
-                   blowup_rms=8000.0):
+                   blowup_rms=8000.0, semi_implicit=False, si_alpha=0.5, raw_filter=False, williams_alpha=0.53):
     """Scan driver: validation, IC selection, time loop, result dict (model.py:1345-1765)."""
     if tmax < 2:
         raise ValueError("tmax must be >= 2 (SWAMPE uses a 2-level initialization).")
     if float(dt) <= 0.0:
         raise ValueError("dt must be positive.")
+    if semi_implicit and expflag:
+        raise ValueError("semi_implicit=True is incompatible with expflag=True (pick one scheme).")
+    if raw_filter and expflag:
+        raise ValueError("raw_filter=True is incompatible with expflag=True.")
     s = build_static(M=M, dt=dt, a=a, omega=omega, g=g, Phibar=Phibar, taurad=taurad, taudrag=taudrag,
                      DPhieq=DPhieq, K6=K6, K6Phi=K6Phi, test=test)
     start = 2 if starttime is None else int(starttime)
@@ -650,7 +757,8 @@ def run_model_scan(*, M, dt, tmax, Phibar, omega, a, test=None, g=9.8, forcflag=
     else:
         eta0, delta0, Phi0, U0, V0 = analytic_ic(s, test, a1)
     flags = dict(forcflag=forcflag, diffflag=diffflag, expflag=expflag, modalflag=modalflag,
-                 diagnostics=diagnostics, alpha=alpha, blowup_rms=blowup_rms)
+                 diagnostics=diagnostics, alpha=alpha, blowup_rms=blowup_rms, semi_implicit=bool(semi_implicit),
+                 si_alpha=float(si_alpha), raw_filter=bool(raw_filter), williams_alpha=float(williams_alpha))
     st = init_state_from_fields(s, test, eta0, delta0, Phi0, U0, V0)
     t_seq = np.arange(start, tmax, dtype=np.int32)
     outs = {k: [] for k in ("t", "dead", "eta", "delta", "Phi", "U", "V", "rms", "spin_min", "phi_min", "phi_max")}

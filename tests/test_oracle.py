@@ -279,3 +279,81 @@ def test_members_are_independent():
     b = orc.run_model_scan(DPhieq=6e6, **kw)["last_state"].Phi_curr
     a2 = orc.run_model_scan(DPhieq=2e6, **kw)["last_state"].Phi_curr
     assert np.max(np.abs(a - b)) > 0 and np.array_equal(a, a2)
+
+
+# ---------------------------------------------------------------------------------------------
+# Semi-implicit scheme, exponential hyperdiffusion and the RAW filter (semi_implicit_tdiff.py,
+# filters.py:154-213, model.py:1007-1143); properties of unit_tests/test_semi_implicit.py and
+# unit_tests/test_raw_filter.py.
+
+def test_si_solve_satisfies_the_coupled_system():
+    """The closed form solves (1 + kd) delta - b Phi = delta*, c delta + (1 + kr) Phi = Phi* for every mode."""
+    rng = np.random.default_rng(4)
+    ds = rng.standard_normal(50) + 1j * rng.standard_normal(50)
+    ps = 1e4 * (rng.standard_normal(50) + 1j * rng.standard_normal(50))
+    b, c = 10 ** rng.uniform(-8, -2, 50), 10 ** rng.uniform(3, 8, 50)
+    kd, kr = 10 ** rng.uniform(-4, 1, 50), 10 ** rng.uniform(-4, 1, 50)
+    d, p = orc.si_solve(ds, ps, b, c, kd, kr)
+    np.testing.assert_allclose((1 + kd) * d - b * p, ds, rtol=1e-12, atol=1e-12 * np.abs(ds).max())
+    np.testing.assert_allclose(c * d + (1 + kr) * p, ps, rtol=1e-12, atol=1e-12 * np.abs(ps).max())
+
+
+def test_exponential_hyperdiffusion_is_the_exact_decay():
+    """sigma6_exponential is exp(-2 dt K6 k_n): a semigroup in dt, neutral at n = 0 and 1, and equal to
+    the implicit factor 1/(1 + 2 dt K6 k_n) to first order in the damping."""
+    M, K6, a = 42, 1.24e33, 8.2e7
+    s1 = orc.sigma6_exponential(M, M, K6, a, 100.0)[0]
+    s2 = orc.sigma6_exponential(M, M, K6, a, 200.0)[0]
+    np.testing.assert_allclose(s1 * s1, s2, rtol=1e-14)
+    assert s1[0] == 1.0 and abs(s1[1] - 1.0) < 1e-15
+    tiny = orc.sigma6_exponential(M, M, K6, a, 1e-6)[0]
+    impl = orc.sigma6(M, M, K6, a, 1e-6)[0]
+    eps = 1.0 - impl
+    np.testing.assert_allclose(tiny, impl, atol=2.0 * float(np.max(eps * eps)) + 1e-16)
+    sp = orc.sigma6Phi_exponential(M, M, K6, a, 100.0)[0]
+    assert sp[0] == 1.0 and sp[1] < 1.0  # no 8/a^6 offset for Phi
+
+
+_RAW_COMMON = dict(M=42, dt=240.0, tmax=40, Phibar=3.0e5, omega=3.2e-5, a=8.2e7, taurad=36000.0, taudrag=21600.0,
+                   DPhieq=1.0e6, test=None, diagnostics=False)
+
+
+def test_raw_filter_alpha_one_is_bit_identical_to_classic_ra():
+    """unit_tests/test_raw_filter.py:37-46: williams_alpha = 1 reproduces the classic filter bit for bit."""
+    base = orc.run_model_scan(**_RAW_COMMON)
+    raw = orc.run_model_scan(**_RAW_COMMON, raw_filter=True, williams_alpha=1.0)
+    for k in ("eta", "delta", "Phi", "U", "V"):
+        np.testing.assert_array_equal(base["outs"][k], raw["outs"][k], err_msg=k)
+
+
+def test_raw_filter_williams_optimum_changes_trajectory():
+    """unit_tests/test_raw_filter.py:49-56."""
+    base = orc.run_model_scan(**_RAW_COMMON)
+    raw = orc.run_model_scan(**_RAW_COMMON, raw_filter=True, williams_alpha=0.53)
+    d = np.max(np.abs(base["outs"]["Phi"][-1] - raw["outs"]["Phi"][-1]))
+    assert np.isfinite(d) and d > 0.0
+
+
+def test_semi_implicit_runs_finite_default_regime():
+    """unit_tests/test_semi_implicit.py:52-57 (dt = 240 s, 118 steps)."""
+    out = orc.run_model_scan(**dict(_RAW_COMMON, tmax=120), semi_implicit=True)
+    assert out["dead_first_idx"] == -1
+    for k in ("eta", "delta", "Phi", "U", "V"):
+        assert np.all(np.isfinite(out["outs"][k][-1])), k
+
+
+def test_semi_implicit_preserves_steady_advection():
+    """unit_tests/test_semi_implicit.py:108-119: Williamson test 1 (solid-body advection) keeps its
+    steady vorticity to roundoff over 6 hours at dt = 600 s."""
+    out = orc.run_model_scan(M=42, dt=600.0, tmax=2 + 36, Phibar=3.0e3, omega=7.2921159e-5, a=6.37122e6, test=1,
+                             forcflag=False, semi_implicit=True)
+    eta = out["outs"]["eta"]
+    drift = float(np.sqrt(np.mean((eta[-1] - eta[0]) ** 2))) / float(np.std(eta[0]))
+    assert drift < 1e-8
+
+
+def test_semi_implicit_and_raw_reject_the_explicit_scheme():
+    with pytest.raises(ValueError):
+        orc.run_model_scan(**_RAW_COMMON, semi_implicit=True, expflag=True)
+    with pytest.raises(ValueError):
+        orc.run_model_scan(**_RAW_COMMON, raw_filter=True, expflag=True)
