@@ -41,7 +41,9 @@ extern "C" {
 /* Per-member physical parameters: SWX_NPARAM doubles per member, in this
  * order.  Mirrors the scalar arguments of run_model_scan
  * (src/my_swampe/model.py:1345-1388); K6PHI < 0 means "same as K6"
- * (model.py:513). */
+ * (model.py:513).  SI_ALPHA (semi-implicit off-centering) and WILLIAMS_ALPHA
+ * (RAW filter) are read only by plans with semi_implicit / raw_filter set
+ * (model.py:156-187 RunFlags.si_alpha, williams_alpha). */
 enum {
   SWX_P_DT = 0,
   SWX_P_A,
@@ -55,6 +57,8 @@ enum {
   SWX_P_K6PHI,
   SWX_P_ALPHA,
   SWX_P_BLOWUP,
+  SWX_P_SI_ALPHA,
+  SWX_P_WILLIAMS_ALPHA,
   SWX_NPARAM
 };
 
@@ -72,6 +76,18 @@ typedef struct swx_plan_desc {
   int test;        /* -1 = None (forced), 1 or 2 (Williamson tests) */
   int exact_basis; /* 0 = reference normalization (default; see swx_geometry),
                       1 = exactly orthonormal basis at every (m, n) */
+  int semi_implicit; /* 1 = semi-implicit leapfrog + exponential hyperdiffusion
+                        (semi_implicit_tdiff.py:79-264); not with expflag */
+  int raw_filter;    /* 1 = Robert-Asselin-Williams filter (model.py:1007-1087);
+                        not with expflag */
+  /* Optional host arrays [J] (NULL = built-in): Gauss nodes mu (ascending, in
+   * (-1, 1)) and weights to use instead of the built-in quadrature, e.g. the
+   * caller's scipy.special.roots_legendre(J) as the reference takes it
+   * (spectral_transform.py:80-114).  Those weights carry rounding of up to
+   * ~3e-11 relative at J = 160; passing them reproduces the reference's
+   * quadrature exactly.  The basis is evaluated at the given nodes. */
+  const double *quad_mus;
+  const double *quad_w;
 } swx_plan_desc;
 
 /* Pointers (device) receiving the reference's 27-field scan carry

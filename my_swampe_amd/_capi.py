@@ -27,7 +27,8 @@ OPT_PHYS_STEPS = 4
 OPT_LEG_MB = 5
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
 
-PARAM_NAMES = ("dt", "a", "omega", "g", "Phibar", "taurad", "taudrag", "DPhieq", "K6", "K6Phi", "alpha", "blowup_rms")
+PARAM_NAMES = ("dt", "a", "omega", "g", "Phibar", "taurad", "taudrag", "DPhieq", "K6", "K6Phi", "alpha", "blowup_rms",
+               "si_alpha", "williams_alpha")
 NPARAM = len(PARAM_NAMES)
 
 # Every extern "C" symbol declared in include/swampe_capi.h.
@@ -51,7 +52,8 @@ class PlanDesc(ctypes.Structure):
     _fields_ = [
         ("M", c_int), ("n_members", c_int), ("precision", c_int), ("expflag", c_int), ("forcflag", c_int),
         ("diffflag", c_int), ("modalflag", c_int), ("diagnostics", c_int), ("test", c_int),
-        ("exact_basis", c_int),
+        ("exact_basis", c_int), ("semi_implicit", c_int), ("raw_filter", c_int),
+        ("quad_mus", c_void_p), ("quad_w", c_void_p),
     ]
 
 
@@ -136,6 +138,22 @@ def grid_size(M: int):
     J, I = c_int(), c_int()
     check(load().swx_grid_size(int(M), ctypes.byref(J), ctypes.byref(I)), "swx_grid_size")
     return J.value, I.value
+
+
+def reference_quadrature(J: int):
+    """Gauss nodes (ascending) and weights as the reference takes them: scipy.special.roots_legendre,
+    or numpy's leggauss where SciPy is missing (spectral_transform.py:80-114)."""
+    import numpy as np
+
+    try:
+        from scipy import special as sp
+
+        mus, w = sp.roots_legendre(int(J))
+    except ImportError:  # pragma: no cover - SciPy is part of the image
+        from numpy.polynomial.legendre import leggauss
+
+        mus, w = leggauss(int(J))
+    return np.ascontiguousarray(mus, dtype=np.float64), np.ascontiguousarray(w, dtype=np.float64)
 
 
 def geometry(M: int, exact_basis: bool = False):

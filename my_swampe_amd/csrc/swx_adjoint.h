@@ -24,6 +24,11 @@ struct LegAdjArgs {
   R* Tbar;            // [B][M+1][kNT][J] complex slot cotangents (written by mode A)
   R* G;               // [B][kNG][J][M+1] complex Fourier rows of S_t (written by mode A)
   const R* Gbar;      // [B][kNG][J][M+1] complex Fourier cotangents (modes B, TERM)
+  // Fragment-ordered basis tables of the matrix-core kernels (LegArgs; FP32 plans).
+  const float* fwdP;
+  const float* fwdH;
+  const float* invP;
+  const float* invH;
 };
 
 template <typename R>
@@ -55,5 +60,9 @@ hipError_t launch_pgrad_reduce(const double* pg, double* out, int B, int J, hipS
 // grid_adj_mfma_lds_bytes is 0 where it is not provided (T106).
 hipError_t launch_grid_adj_mfma(const GridAdjArgs<float>& a, const float* TinvF, const float* TfwdF, hipStream_t s);
 size_t grid_adj_mfma_lds_bytes(int M);
+// Matrix-core Legendre adjoint (all modes, FP32; swx_adjoint_leg_mfma.hip) and its largest
+// per-workgroup LDS.
+hipError_t launch_leg_adj_mfma(const LegAdjArgs<float>& a, hipStream_t s);
+size_t leg_adj_mfma_lds_bytes(int M);
 
 }  // namespace swx

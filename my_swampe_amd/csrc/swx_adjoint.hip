@@ -20,6 +20,7 @@
 #include <cfloat>
 
 #include "swx_adjoint.h"
+#include "swx_adjoint_common.h"
 #include "swx_physics.h"
 
 namespace swx {
@@ -56,52 +57,20 @@ __global__ void __launch_bounds__(128) k_leg_adj(LegAdjArgs<R> a) {
   const long long boff = tri_off(m, N, J);
   const long long fst = (long long)Mp * Np * 2;  // doubles per spectral field
   double* Lam = a.Lam + (long long)b * 3 * fst;
-  const double mm = (double)m;
 
   if (a.mode == LADJ_A) {
     R* sL = reinterpret_cast<R*>(smem_raw);  // [kNT][Np][2] slot cotangents over n
     R* sX = sL + kNT * Np * 2;               // [5][Np][2] inputs of the recomputed inverse
     const double* St = a.St + (long long)b * 3 * fst;
-    const double dt = cf[C_DT], aa = cf[C_A], Phibar = cf[C_PHIBAR];
-    const double t2 = (2.0 * dt / (aa * aa)) / 4.0;
     for (int np = tid; np < K; np += nt) {
       const int n = m + np;
       const long long so = ((long long)m * Np + n) * 2;
-      const double nn = (double)n * (double)(n + 1);
-      const double sg = a.diffflag ? cf[cl.n_sigma + n] : 1.0;
-      const double sp = a.diffflag ? cf[cl.n_sigmaphi + n] : 1.0;
-      const double f2 = a.f2[(long long)m * Np + n];
+      double slot[kNT][2], x[5][2];
+      leg_adj_carry(cf, cl, a.f2[(long long)m * Np + n], n, a.diffflag, Lam, so, fst, slot);
+      leg_adj_recompute_inputs(cf, cl, m, n, St, so, fst, x);
       for (int c = 0; c < 2; ++c) {
-        const double me = sg * Lam[0 * fst + so + c];
-        const double md = sg * Lam[1 * fst + so + c];
-        const double mp = sp * Lam[2 * fst + so + c];
-        // Carry adjoint (transpose of the FP64 spectral update in k_leg).
-        const double xp = mp * (1.0 - Phibar * 0.5 * (nn / (aa * aa))) + nn * t2 * md;
-        const double xd = -dt * Phibar * mp + md * (1.0 - Phibar * nn / (aa * aa));
-        Lam[0 * fst + so + c] = f2 * me;
-        Lam[1 * fst + so + c] = f2 * xd;
-        Lam[2 * fst + so + c] = f2 * xp;
-        // Slot cotangents before the transposed Legendre contraction.
-        sL[(0 * Np + np) * 2 + c] = (R)mp;
-        sL[(1 * Np + np) * 2 + c] = (R)(nn * mp);
-        sL[(2 * Np + np) * 2 + c] = (R)md;
-        sL[(3 * Np + np) * 2 + c] = (R)(nn * md);
-        sL[(4 * Np + np) * 2 + c] = (R)me;
-        sL[(5 * Np + np) * 2 + c] = (R)mp;
-        sL[(6 * Np + np) * 2 + c] = (R)md;
-        sL[(7 * Np + np) * 2 + c] = (R)(nn * md);
-        sL[(8 * Np + np) * 2 + c] = (R)me;
-      }
-      // Recompute inputs: eta, delta, Phi and the wind potentials of S_t.
-      const double tcmn = cf[cl.n_tcmn + n];
-      const double f = (m == 0 && n == 1) ? cf[C_FMN01] : 0.0;
-      for (int c = 0; c < 2; ++c) {
-        const double e = St[0 * fst + so + c], d = St[1 * fst + so + c], p = St[2 * fst + so + c];
-        sX[(0 * Np + np) * 2 + c] = (R)e;
-        sX[(1 * Np + np) * 2 + c] = (R)d;
-        sX[(2 * Np + np) * 2 + c] = (R)p;
-        sX[(3 * Np + np) * 2 + c] = (R)(n == 0 ? 0.0 : d * tcmn);
-        sX[(4 * Np + np) * 2 + c] = (R)(n == 0 ? 0.0 : (e - (c == 0 ? f : 0.0)) * tcmn);
+        for (int k = 0; k < kNT; ++k) sL[(k * Np + np) * 2 + c] = (R)slot[k][c];
+        for (int f = 0; f < 5; ++f) sX[(f * Np + np) * 2 + c] = (R)x[f][c];
       }
     }
     __syncthreads();
@@ -192,25 +161,10 @@ __global__ void __launch_bounds__(128) k_leg_adj(LegAdjArgs<R> a) {
       h[2] += hv * sGb[(4 * J + j) * 2];
       h[3] += hv * sGb[(4 * J + j) * 2 + 1];
     }
-    // dhat = i m P^T(Ubar) - H^T(Vbar);  ehat = H^T(Ubar) + i m P^T(Vbar);  i m z = (-m z.im, m z.re)
-    const double dh[2] = {-mm * (double)p[7] - (double)h[2], mm * (double)p[6] - (double)h[3]};
-    const double eh[2] = {(double)h[0] - mm * (double)p[9], (double)h[1] + mm * (double)p[8]};
-    const double tcmn = n == 0 ? 0.0 : cf[cl.n_tcmn + n];
-    const long long so = ((long long)m * Np + n) * 2;
-    for (int c = 0; c < 2; ++c) {
-      const double le = (double)p[0 + c] + tcmn * eh[c];
-      const double ld = (double)p[2 + c] + tcmn * dh[c];
-      const double lp = (double)p[4 + c];
-      if (a.mode == LADJ_TERM) {
-        Lam[0 * fst + so + c] = le;
-        Lam[1 * fst + so + c] = ld;
-        Lam[2 * fst + so + c] = lp;
-      } else {
-        Lam[0 * fst + so + c] += le;
-        Lam[1 * fst + so + c] += ld;
-        Lam[2 * fst + so + c] += lp;
-      }
-    }
+    double pd[10], hd[4];
+    for (int k = 0; k < 10; ++k) pd[k] = (double)p[k];
+    for (int k = 0; k < 4; ++k) hd[k] = (double)h[k];
+    leg_adj_lambda(cf, cl, m, n, pd, hd, a.mode == LADJ_TERM, Lam, ((long long)m * Np + n) * 2, fst);
   }
 }
 

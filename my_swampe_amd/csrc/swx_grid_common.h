@@ -10,38 +10,24 @@ namespace swx {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-// One grid point (row j, longitude i) of member b: pinned winds (test 1), the
-// Robert-Asselin filter of the physical prev level and the physical carry (when
-// a.phys_on; model.py:988-1006), then the 8 products of the forward transform
-// (forcing.py, initial_conditions.py:283-292) in reciprocal form (inv_denE =
-// 1 / (2 (1 - mu_j^2))).
+// One grid point (row j, longitude i) of member b: the initial fields' residue
+// (RAW), pinned winds (test 1), the time filter of the physical prev level and
+// the physical carry (when a.phys_on; swx_physics.h:physical_carry), then the 8
+// products of the forward transform (forcing.py, initial_conditions.py:283-292)
+// in reciprocal form (inv_denE = 1 / (2 (1 - mu_j^2))).
 __device__ __forceinline__ void grid_point_step(const GridArgs<float>& a, const double* cf, const double* geo,
                                                 const GeoLayout& gl, double* phys, long long plane, double alpha,
                                                 int b, int j, int i, double eta, double delta, double Phi, double U,
                                                 double V, double inv_denE, double prod[8]) {
   const long long pix = (long long)j * a.I + i;
+  if (a.raw_filter) add_residue(a.resid_c, a.R0p, b, plane, pix, eta, delta, Phi);
   if (a.test == 1) {
     U = a.uvic[((long long)b * 2 + 0) * plane + pix];
     V = a.uvic[((long long)b * 2 + 1) * plane + pix];
   }
   if (a.phys_on) {
-    const double ep = phys[0 * plane + pix], ec = phys[1 * plane + pix];
-    const double dp = phys[2 * plane + pix], dc = phys[3 * plane + pix];
-    const double pp = phys[4 * plane + pix], pc = phys[5 * plane + pix];
-    double em = ec, dm = dc, pm = pc;
-    if (a.do_ra) {
-      em = ec + alpha * (ep - 2.0 * ec + eta);
-      dm = dc + alpha * (dp - 2.0 * dc + delta);
-      pm = pc + alpha * (pp - 2.0 * pc + Phi);
-    }
-    phys[0 * plane + pix] = em;
-    phys[1 * plane + pix] = eta;
-    phys[2 * plane + pix] = dm;
-    phys[3 * plane + pix] = delta;
-    phys[4 * plane + pix] = pm;
-    phys[5 * plane + pix] = Phi;
-    phys[6 * plane + pix] = U;
-    phys[7 * plane + pix] = V;
+    double Phi_mid;
+    physical_carry(cf, a.do_ra, a.raw_filter, alpha, phys, plane, pix, eta, delta, Phi, U, V, Phi_mid);
   }
   grid_products_recip(cf, geo, gl, j, i, a.test, eta, Phi, U, V, inv_denE, prod);
 }
@@ -96,7 +82,13 @@ __device__ __forceinline__ void grid_assemble(const GridArgs<float>& a, const v4
           }
         }
         double out[kNT][2];
-        assemble_slots(cf, cf[cl.n_tc + jj], geo[gl.w + jj], m, a.expflag, a.forcflag, fm, Um, Vm, out);
+        if (a.si) {
+          double lagR[2][2];
+          si_lag_exchange(a, cf, b, jj, m, fm, Um, Vm, false, lagR);
+          assemble_slots_si(cf, cf[cl.n_tc + jj], geo[gl.w + jj], m, a.forcflag, a.relax, fm, lagR, out);
+        } else {
+          assemble_slots(cf, cf[cl.n_tc + jj], geo[gl.w + jj], m, a.expflag, a.forcflag, fm, Um, Vm, out);
+        }
         float* Tm = a.T + (((long long)b * Mp + m) * kNT) * J * 2;
 #pragma unroll
         for (int k = 0; k < kNT; ++k) {

@@ -7,7 +7,8 @@
 namespace swx {
 
 // Per-member coefficient table (FP64), layout [B][ncoef]:
-//   scalars [0, kNScal), then sigma[N+1], sigmaPhi[N+1], tcmn[N+1], tc[J], areac[J].
+//   scalars [0, kNScal), then sigma[N+1], sigmaPhi[N+1], tcmn[N+1], tc[J], areac[J],
+//   sigma_exp[N+1], sigmaPhi_exp[N+1].
 enum : int {
   C_DT = 0,
   C_A,
@@ -22,11 +23,13 @@ enum : int {
   C_AREAPLANET, // 4 pi a^2
   C_FMN01,      // omega / sqrt(0.375)
   C_INVTAURAD,  // 1 / taurad
+  C_SIALPHA,    // semi-implicit off-centering
+  C_WILLIAMS,   // RAW filter weight
   kNScal = 16
 };
 
 struct CoefLayout {
-  int n_sigma, n_sigmaphi, n_tcmn, n_tc, n_areac, ncoef;
+  int n_sigma, n_sigmaphi, n_tcmn, n_tc, n_areac, n_sigexp, n_sigexpphi, ncoef;
   __host__ __device__ static CoefLayout make(int N, int J) {
     CoefLayout c;
     c.n_sigma = kNScal;
@@ -34,9 +37,20 @@ struct CoefLayout {
     c.n_tcmn = c.n_sigmaphi + (N + 1);
     c.n_tc = c.n_tcmn + (N + 1);
     c.n_areac = c.n_tc + J;
-    c.ncoef = c.n_areac + J;
+    c.n_sigexp = c.n_areac + J;          // exponential hyperdiffusion (semi-implicit scheme)
+    c.n_sigexpphi = c.n_sigexp + (N + 1);
+    c.ncoef = c.n_sigexpphi + (N + 1);
     return c;
   }
+};
+
+// Scheme flags of one step of the spectral update (host fills them per launch).
+struct StepFlags {
+  int expflag, diffflag;
+  int si;      // semi-implicit leapfrog
+  int raw;     // Robert-Asselin-Williams filter
+  int relax;   // semi-implicit: implicit relaxation/drag (forcflag and test None)
+  int do_ra;   // time filter active this step (modalflag and t > 2)
 };
 
 // Shared geometry table (FP64): mus[J], w[J], s1m[J], cosphi[J], denomE[J],
@@ -116,6 +130,14 @@ struct LegArgs {
   double* hist_scal;      // [B][hist_T][4] or null
   unsigned char* hist_dead;
   int hist_T, hist_k;
+  // Semi-implicit / RAW (LEG_STEP): scheme flags of this step and the per-member
+  // coefficients (c_prev, c_curr) of the initial fields' non-band-limited residue
+  // in the filtered lineages, [B][2] (advanced by the m = 0 workgroups).
+  int si, raw_filter, relax, do_ra;
+  double* resid_c;
+  // LEG_PRIME: 1 = transform the prev level instead of the current one.
+  int prime_prev;
+  __host__ __device__ StepFlags flags() const { return StepFlags{expflag, diffflag, si, raw_filter, relax, do_ra}; }
 };
 
 template <typename R>
@@ -144,6 +166,15 @@ struct GridArgs {
   R* raw;              // [B][kNRaw][J][M+1]
   R* lag;              // [B][2][J][M+1]
   R* g0uv;             // [B][2][J][M+1]
+  // Semi-implicit: the Legendre inputs use the lagged momentum remainder `lag`
+  // (read, then overwritten with the current one; the value read is copied to
+  // lagx on export_raw steps).  RAW: the physical fields gain c_curr * R0p (the
+  // initial fields' residue, [B][3][J][I]) and the physical carry follows the
+  // Williams filter.
+  int si, raw_filter, relax;
+  R* lagx;
+  const double* resid_c;  // [B][2]
+  const double* R0p;
 };
 
 template <typename R>
@@ -181,8 +212,15 @@ int grid_mfma_k2p(int M);
 template <typename R>
 hipError_t launch_copy_real(double* dst, long long dstride, const R* src, long long sstride, long long n, int B,
                             hipStream_t s);
-// Field f of an m-major Fourier-row buffer -> complex128 [B][J][M+1].
+// Field f of an m-major Fourier-row buffer -> complex128 [B][J][M+1]; with `res`, plus
+// rc[2 b + lvl] times the same field of `res` (the initial fields' residue).
 template <typename R>
-hipError_t launch_copy_fourier(double* dst, const R* src, int f, int B, int J, int Mp, hipStream_t s);
+hipError_t launch_copy_fourier(double* dst, const R* src, int f, int B, int J, int Mp, hipStream_t s,
+                               const R* res = nullptr, const double* rc = nullptr, int lvl = 0);
+// Initial fields' residue of semi-implicit / RAW plans (swx_kernels.hip:k_residue_init).
+template <typename R>
+hipError_t launch_residue_init(const R* G, R* R0f, double* R0p, const double* in_eta, const double* in_delta,
+                               const double* in_Phi, const R* tw, double* resid_c, int B, int J, int I, int Mp,
+                               hipStream_t s);
 
 }  // namespace swx

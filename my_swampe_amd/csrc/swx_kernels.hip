@@ -97,6 +97,7 @@ __global__ void __launch_bounds__(128) k_leg(LegArgs<R> a) {
     }
     if (dead_next) return;
   }
+  if (a.mode == LEG_STEP && (a.si || a.raw_filter) && m == 0 && tid == 0) residue_step(cf, a.flags(), a.resid_c + 2 * b);
 
   // Stage the Legendre inputs of (b, m).
   const long long toff = (((long long)b * (M + 1) + m) * kNT) * J * 2;
@@ -116,12 +117,15 @@ __global__ void __launch_bounds__(128) k_leg(LegArgs<R> a) {
     const int n = m + np;
     const long long so = ((long long)m * (N + 1) + n) * 2;
     double eta_n[2], delta_n[2], Phi_n[2];
+    // Wind potentials come from the new level (differs from the stored one under RAW).
+    double eta_w[2], delta_w[2];
     if (a.mode == LEG_PRIME) {
-      // Fourier rows of the stored current level (restart from a saved state).
+      // Fourier rows of a stored level (restart from a saved state, exports).
+      const double* Sl = a.prime_prev ? Sprev : Scur;
       for (int c = 0; c < 2; ++c) {
-        eta_n[c] = Scur[0 * fstride + so + c];
-        delta_n[c] = Scur[1 * fstride + so + c];
-        Phi_n[c] = Scur[2 * fstride + so + c];
+        eta_n[c] = Sl[0 * fstride + so + c];
+        delta_n[c] = Sl[1 * fstride + so + c];
+        Phi_n[c] = Sl[2 * fstride + so + c];
       }
     } else if (a.mode == LEG_INIT) {
       R acc[6] = {0, 0, 0, 0, 0, 0};
@@ -172,12 +176,25 @@ __global__ void __launch_bounds__(128) k_leg(LegArgs<R> a) {
         fh[k][0] = acc[2 * (kNP + k)];
         fh[k][1] = acc[2 * (kNP + k) + 1];
       }
-      spectral_update(cf, cl, a.f2[(long long)m * (N + 1) + n], n, a.expflag, a.diffflag, fp, fh, Scur, Sprev, so,
-                      fstride, eta_n, delta_n, Phi_n);
+      double eff[3][2], nw[3][2];
+      spectral_update(cf, cl, a.f2[(long long)m * (N + 1) + n], m, n, a.flags(), fp, fh, Scur, Sprev, so, fstride,
+                      eff, nw);
+      for (int c = 0; c < 2; ++c) {
+        eta_n[c] = eff[0][c];
+        delta_n[c] = eff[1][c];
+        Phi_n[c] = eff[2][c];
+        eta_w[c] = nw[0][c];
+        delta_w[c] = nw[1][c];
+      }
     }
+    if (a.mode != LEG_STEP)
+      for (int c = 0; c < 2; ++c) {
+        eta_w[c] = eta_n[c];
+        delta_w[c] = delta_n[c];
+      }
     // Inputs of the inverse transform: eta, delta, Phi and the wind potentials.
     double dh[2], eh[2];
-    wind_potentials(cf, cl, m, n, eta_n, delta_n, dh, eh);
+    wind_potentials(cf, cl, m, n, eta_w, delta_w, dh, eh);
     for (int c = 0; c < 2; ++c) {
       sX[(0 * (N + 1) + np) * 2 + c] = (R)eta_n[c];
       sX[(1 * (N + 1) + np) * 2 + c] = (R)delta_n[c];
@@ -347,6 +364,7 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs<R> a) {
           eta = x[0];
           delta = x[1];
           Phi = x[2];
+          if (a.raw_filter && a.mode == GRID_STEP) add_residue(a.resid_c, a.R0p, b, plane, row + i, eta, delta, Phi);
         }
         U = x[3];
         V = x[4];
@@ -364,8 +382,7 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs<R> a) {
           V = a.uvic[((long long)b * 2 + 1) * plane + row + i];
         }
       }
-      // Physical carry and the Robert-Asselin filter on the prev level only
-      // (model.py:988-1006).
+      // Physical carry and the time filter of the prev level (model.py:988-1087).
       double Phi_mid = Phi;
       if (a.mode == GRID_INIT) {
         phys[0 * plane + row + i] = eta;
@@ -377,24 +394,7 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs<R> a) {
         phys[6 * plane + row + i] = U;
         phys[7 * plane + row + i] = V;
       } else if (a.phys_on) {
-        const double ep = phys[0 * plane + row + i], ec = phys[1 * plane + row + i];
-        const double dp = phys[2 * plane + row + i], dc = phys[3 * plane + row + i];
-        const double pp = phys[4 * plane + row + i], pc = phys[5 * plane + row + i];
-        double em = ec, dm = dc, pm = pc;
-        if (a.do_ra) {
-          em = ec + alpha * (ep - 2.0 * ec + eta);
-          dm = dc + alpha * (dp - 2.0 * dc + delta);
-          pm = pc + alpha * (pp - 2.0 * pc + Phi);
-        }
-        Phi_mid = pm;
-        phys[0 * plane + row + i] = em;
-        phys[1 * plane + row + i] = eta;
-        phys[2 * plane + row + i] = dm;
-        phys[3 * plane + row + i] = delta;
-        phys[4 * plane + row + i] = pm;
-        phys[5 * plane + row + i] = Phi;
-        phys[6 * plane + row + i] = U;
-        phys[7 * plane + row + i] = V;
+        physical_carry(cf, a.do_ra, a.raw_filter, alpha, phys, plane, row + i, eta, delta, Phi, U, V, Phi_mid);
       }
       if (a.hist_fields) {
         const double outv[5] = {eta, delta, Phi, U, V};
@@ -482,7 +482,8 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs<R> a) {
   }
   if (a.mode == GRID_INIT) {
     // Um0/Vm0 (model.py:727-729), the pinned copy for test 1, and the lagged
-    // momentum-forcing remainder Fm + Um/taudrag (model.py:775-792).
+    // momentum-forcing remainder Fm + Um/taudrag (model.py:775-792; the
+    // semi-implicit assembly below stores its own).
     R* g0 = a.g0uv + (long long)b * 2 * J * Mp * 2;
     R* lg = a.lag + (long long)b * 2 * J * Mp * 2;
     const double itd = cf[C_INVTD];
@@ -491,7 +492,7 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs<R> a) {
       const R v = sF[(8 + f) * Mp * 2 + r];
       a.Gout[fidx(b, 3 + f, r >> 1, j, Mp, J) + (r & 1)] = v;
       g0[((long long)f * J + j) * Mp * 2 + r] = v;
-      lg[((long long)f * J + j) * Mp * 2 + r] = (R)((double)sF[(6 + f) * Mp * 2 + r] + (double)v * itd);
+      if (!a.si) lg[((long long)f * J + j) * Mp * 2 + r] = (R)((double)sF[(6 + f) * Mp * 2 + r] + (double)v * itd);
     }
   }
 
@@ -514,7 +515,18 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs<R> a) {
       Vm[0] = sv[0]; Vm[1] = sv[1];
     }
     double out[kNT][2];
-    assemble_slots(cf, tc, wj, m, a.expflag, a.forcflag, fm, Um, Vm, out);
+    if (a.si) {
+      // Winds of the remainder: the initial DFT at initialization, else the step's.
+      const bool init = a.mode == GRID_INIT;
+      const R* su = init ? sF + (8 * Mp + m) * 2 : sG + (3 * Mp + m) * 2;
+      const R* sv = init ? sF + (9 * Mp + m) * 2 : sG + (4 * Mp + m) * 2;
+      const double u2[2] = {(double)su[0], (double)su[1]}, v2[2] = {(double)sv[0], (double)sv[1]};
+      double lagR[2][2];
+      si_lag_exchange(a, cf, b, j, m, fm, u2, v2, init, lagR);
+      assemble_slots_si(cf, tc, wj, m, a.forcflag, a.relax, fm, lagR, out);
+    } else {
+      assemble_slots(cf, tc, wj, m, a.expflag, a.forcflag, fm, Um, Vm, out);
+    }
     R* Tm = a.T + (((long long)b * Mp + m) * kNT) * J * 2;
     for (int k = 0; k < kNT; ++k) {
       Tm[(k * J + j) * 2] = (R)out[k][0];
@@ -531,12 +543,55 @@ __global__ void k_copy_real(double* dst, long long dstride, const R* src, long l
 }
 
 template <typename R>
-__global__ void k_copy_fourier(double* dst, const R* src, int f, int J, int Mp) {
+__global__ void k_copy_fourier(double* dst, const R* src, int f, int J, int Mp, const R* res, const double* rc,
+                               int lvl) {
   const int b = blockIdx.y;
   const long long n = (long long)J * Mp * 2;
+  const double cr = res ? rc[2 * b + lvl] : 0.0;
   for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
     const int c = (int)(e & 1), m = (int)((e >> 1) % Mp), j = (int)((e >> 1) / Mp);
-    dst[(long long)b * n + e] = (double)src[fidx(b, f, m, j, Mp, J) + c];
+    const long long o = fidx(b, f, m, j, Mp, J) + c;
+    dst[(long long)b * n + e] = res ? (double)src[o] + cr * (double)res[o] : (double)src[o];
+  }
+}
+
+// Initial fields' non-band-limited residue of a semi-implicit / RAW plan, one block per
+// (row j, member b).  On entry G holds the raw Fourier rows of the initial fields and R0f
+// the rows of their band-limited part (the inverse Legendre transform of the initial
+// spectral state).  Writes R0p = field0 - invDFT(band) (when R0p is non-null), then
+// R0f = raw - band (fields eta, delta, Phi), and sets the residue coefficients to 1.
+template <typename R>
+__global__ void __launch_bounds__(256) k_residue_init(const R* G, R* R0f, double* R0p, const double* in_eta,
+                                                      const double* in_delta, const double* in_Phi, const R* tw,
+                                                      double* resid_c, int J, int I, int Mp) {
+  const int j = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+  const long long plane = (long long)J * I;
+  if (R0p) {
+    const double* src[3] = {in_eta, in_delta, in_Phi};
+    for (int e = tid; e < 3 * I; e += blockDim.x) {
+      const int f = e / I, i = e % I;
+      // x_i = Re c_0 + 2 sum_m Re(c_m e^{+2 pi i m i / I}) (spectral_transform.py:331-348).
+      double acc = 0.0;
+      int idx = 0;
+      for (int m = 1; m < Mp; ++m) {
+        idx += i;
+        if (idx >= I) idx -= I;
+        const long long o = fidx(b, f, m, j, Mp, J);
+        acc += (double)R0f[o] * (double)tw[idx] - (double)R0f[o + 1] * (double)tw[I + idx];
+      }
+      const double band = (double)R0f[fidx(b, f, 0, j, Mp, J)] + 2.0 * acc;
+      R0p[((long long)b * 3 + f) * plane + (long long)j * I + i] = src[f][(long long)b * plane + (long long)j * I + i] - band;
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < 3 * Mp * 2; e += blockDim.x) {
+    const int f = e / (Mp * 2), r = e % (Mp * 2);
+    const long long o = fidx(b, f, r >> 1, j, Mp, J) + (r & 1);
+    R0f[o] = (R)((double)G[o] - (double)R0f[o]);
+  }
+  if (j == 0 && tid == 0) {
+    resid_c[2 * b] = 1.0;
+    resid_c[2 * b + 1] = 1.0;
   }
 }
 
@@ -572,15 +627,33 @@ hipError_t launch_copy_real(double* dst, long long dstride, const R* src, long l
 }
 
 template <typename R>
-hipError_t launch_copy_fourier(double* dst, const R* src, int f, int B, int J, int Mp, hipStream_t s) {
+hipError_t launch_copy_fourier(double* dst, const R* src, int f, int B, int J, int Mp, hipStream_t s, const R* res,
+                               const double* rc, int lvl) {
   int gx = (int)(((long long)J * Mp * 2 + 255) / 256);
   if (gx > 1024) gx = 1024;
-  hipLaunchKernelGGL(k_copy_fourier<R>, dim3(gx, B), dim3(256), 0, s, dst, src, f, J, Mp);
+  hipLaunchKernelGGL(k_copy_fourier<R>, dim3(gx, B), dim3(256), 0, s, dst, src, f, J, Mp, res, rc, lvl);
   return hipGetLastError();
 }
 
-template hipError_t launch_copy_fourier<double>(double*, const double*, int, int, int, int, hipStream_t);
-template hipError_t launch_copy_fourier<float>(double*, const float*, int, int, int, int, hipStream_t);
+template <typename R>
+hipError_t launch_residue_init(const R* G, R* R0f, double* R0p, const double* in_eta, const double* in_delta,
+                               const double* in_Phi, const R* tw, double* resid_c, int B, int J, int I, int Mp,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(k_residue_init<R>, dim3(J, B), dim3(256), 0, s, G, R0f, R0p, in_eta, in_delta, in_Phi, tw,
+                     resid_c, J, I, Mp);
+  return hipGetLastError();
+}
+
+template hipError_t launch_copy_fourier<double>(double*, const double*, int, int, int, int, hipStream_t,
+                                                const double*, const double*, int);
+template hipError_t launch_copy_fourier<float>(double*, const float*, int, int, int, int, hipStream_t, const float*,
+                                               const double*, int);
+template hipError_t launch_residue_init<double>(const double*, double*, double*, const double*, const double*,
+                                                const double*, const double*, double*, int, int, int, int,
+                                                hipStream_t);
+template hipError_t launch_residue_init<float>(const float*, float*, double*, const double*, const double*,
+                                               const double*, const float*, double*, int, int, int, int,
+                                               hipStream_t);
 template hipError_t launch_leg<double>(const LegArgs<double>&, hipStream_t);
 template hipError_t launch_leg<float>(const LegArgs<float>&, hipStream_t);
 template hipError_t launch_grid<double>(const GridArgs<double>&, hipStream_t);

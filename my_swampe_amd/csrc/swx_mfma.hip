@@ -370,6 +370,8 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
   __syncthreads();
 
   // Phase U: FP64 update per (member, n) and the inverse-transform inputs.
+  if ((a.si || a.raw_filter) && m == 0 && tid < MB && b0 + tid < B)
+    residue_step(a.coef + (long long)(b0 + tid) * a.ncoef, a.flags(), a.resid_c + 2 * (b0 + tid));
   for (int p = tid; p < MB * K; p += 256) {
     const int mb = p / K, np = p % K;
     const int b = b0 + mb;
@@ -384,15 +386,15 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
     double* Sb = a.S + (long long)b * 2 * 3 * Mp * Np * 2;
     const long long fstride = (long long)Mp * Np * 2;
     const long long so = ((long long)m * Np + n) * 2;
-    double eta_n[2], delta_n[2], Phi_n[2], dh[2], eh[2];
-    spectral_update(cf, cl, a.f2[(long long)m * Np + n], n, a.expflag, a.diffflag, fp, fh, Sb + 3 * fstride, Sb,
-                    so, fstride, eta_n, delta_n, Phi_n);
-    wind_potentials(cf, cl, m, n, eta_n, delta_n, dh, eh);
+    double eff[3][2], nw[3][2], dh[2], eh[2];
+    spectral_update(cf, cl, a.f2[(long long)m * Np + n], m, n, a.flags(), fp, fh, Sb + 3 * fstride, Sb, so,
+                    fstride, eff, nw);
+    wind_potentials(cf, cl, m, n, nw[0], nw[1], dh, eh);
     float* xr = sX + np * XS;
     for (int c = 0; c < 2; ++c) {
-      xr[mb * 10 + 0 + c] = (float)eta_n[c];
-      xr[mb * 10 + 2 + c] = (float)delta_n[c];
-      xr[mb * 10 + 4 + c] = (float)Phi_n[c];
+      xr[mb * 10 + 0 + c] = (float)eff[0][c];
+      xr[mb * 10 + 2 + c] = (float)eff[1][c];
+      xr[mb * 10 + 4 + c] = (float)eff[2][c];
       xr[mb * 10 + 6 + c] = (float)dh[c];
       xr[mb * 10 + 8 + c] = (float)eh[c];
       xr[XP + mb * 4 + 0 + c] = (float)eh[c];

@@ -76,6 +76,101 @@ __device__ inline void grid_products_recip(const double* cf, const double* geo, 
   out[7] = G;
 }
 
+// Initial fields' non-band-limited residue in the current physical level (RAW): eta, delta,
+// Phi += c_curr R0p.  c_curr shrinks by about alpha per step; once it is below 1e-20 the
+// term is under the FP64 rounding of the fields and is skipped (no R0p read).
+__device__ inline void add_residue(const double* resid_c, const double* R0p, int b, long long plane, long long pix,
+                                   double& eta, double& delta, double& Phi) {
+  const double c = resid_c[2 * b + 1];
+  if (fabs(c) > 1e-20) {
+    const double* r = R0p + (long long)b * 3 * plane + pix;
+    eta += c * r[0];
+    delta += c * r[plane];
+    Phi += c * r[2 * plane];
+  }
+}
+
+// Physical carry of one grid point (phys: eta p,c; delta p,c; Phi p,c; U; V planes) given
+// the current level eta, delta, Phi of this step.  Filter of the prev level (model.py:991-1087):
+//   classic RA  prev' = curr + alpha (prev - 2 curr + new), new = the current level;
+//   RAW         prev' = curr + alpha w d with d = prev - 2 curr + new; the current level is
+//               eff = new - alpha (1 - w) d, so d = (prev - 2 curr + eff) / (1 - alpha (1 - w)).
+// Phi_mid = the filtered prev level of Phi (the diagnostics' Phi extrema).
+__device__ inline void physical_carry(const double* cf, int do_ra, int raw, double alpha, double* phys,
+                                      long long plane, long long pix, double eta, double delta, double Phi,
+                                      double U, double V, double& Phi_mid) {
+  const double ep = phys[0 * plane + pix], ec = phys[1 * plane + pix];
+  const double dp = phys[2 * plane + pix], dc = phys[3 * plane + pix];
+  const double pp = phys[4 * plane + pix], pc = phys[5 * plane + pix];
+  double em = ec, dm = dc, pm = pc;
+  if (do_ra) {
+    if (raw) {
+      const double w = cf[C_WILLIAMS];
+      const double g = alpha * w / (1.0 - alpha * (1.0 - w));
+      em = ec + g * (ep - 2.0 * ec + eta);
+      dm = dc + g * (dp - 2.0 * dc + delta);
+      pm = pc + g * (pp - 2.0 * pc + Phi);
+    } else {
+      em = ec + alpha * (ep - 2.0 * ec + eta);
+      dm = dc + alpha * (dp - 2.0 * dc + delta);
+      pm = pc + alpha * (pp - 2.0 * pc + Phi);
+    }
+  }
+  Phi_mid = pm;
+  phys[0 * plane + pix] = em;
+  phys[1 * plane + pix] = eta;
+  phys[2 * plane + pix] = dm;
+  phys[3 * plane + pix] = delta;
+  phys[4 * plane + pix] = pm;
+  phys[5 * plane + pix] = Phi;
+  phys[6 * plane + pix] = U;
+  phys[7 * plane + pix] = V;
+}
+
+// Semi-implicit lagged momentum remainder of (member b, row j, wavenumber m)
+// (model.py:775-792, :1135-1143).  lagR receives the remainder the next step's Legendre
+// inputs use, that of the previous level (kept in a.lag), and a.lag receives the current
+// one, F_m + U_m / taudrag (U_m, V_m: the step's winds, the pinned initial ones in test 1).
+// At initialization (`init`) the current remainder is both used and stored.  a.lagx (the
+// exported carry) receives the value used on initialization and export_raw steps.
+template <typename R>
+__device__ inline void si_lag_exchange(const GridArgs<R>& a, const double* cf, int b, int j, int m,
+                                       const double fm[8][2], const double Um[2], const double Vm[2], bool init,
+                                       double lagR[2][2]) {
+  const int Mp = a.M + 1;
+  const long long o0 = (((long long)b * 2 + 0) * a.J + j) * Mp * 2 + 2 * m;
+  const long long o1 = o0 + (long long)a.J * Mp * 2;
+  double u[2] = {Um[0], Um[1]}, v[2] = {Vm[0], Vm[1]};
+  if (a.test == 1 && !init) {
+    u[0] = a.g0uv[o0];
+    u[1] = a.g0uv[o0 + 1];
+    v[0] = a.g0uv[o1];
+    v[1] = a.g0uv[o1 + 1];
+  }
+  const double itd = cf[C_INVTD];
+  const double cur[2][2] = {{fm[6][0] + u[0] * itd, fm[6][1] + u[1] * itd},
+                            {fm[7][0] + v[0] * itd, fm[7][1] + v[1] * itd}};
+  if (init) {
+    for (int f = 0; f < 2; ++f)
+      for (int c = 0; c < 2; ++c) lagR[f][c] = (double)(R)cur[f][c];
+  } else {
+    lagR[0][0] = a.lag[o0];
+    lagR[0][1] = a.lag[o0 + 1];
+    lagR[1][0] = a.lag[o1];
+    lagR[1][1] = a.lag[o1 + 1];
+  }
+  a.lag[o0] = (R)cur[0][0];
+  a.lag[o0 + 1] = (R)cur[0][1];
+  a.lag[o1] = (R)cur[1][0];
+  a.lag[o1 + 1] = (R)cur[1][1];
+  if (init || a.export_raw) {
+    a.lagx[o0] = (R)lagR[0][0];
+    a.lagx[o0 + 1] = (R)lagR[0][1];
+    a.lagx[o1] = (R)lagR[1][0];
+    a.lagx[o1 + 1] = (R)lagR[1][1];
+  }
+}
+
 // w-weighted Legendre inputs of one (member, j, m) from the truncated Fourier
 // coefficients f[8][2] (A, B, C, D, E, PhiF, F, G) and, for the explicit
 // scheme, the wind coefficients Um/Vm.  P-basis slots 0..4, H-basis 5..8:
@@ -166,55 +261,171 @@ __device__ inline void assemble_slots(const double* cf, double tc, double wj, in
   }
 }
 
-// FP64 spectral update of one (m, n) (modEuler_tdiff.py:95-103, :185-193,
-// :261; explicit_tdiff.py:71, :138, :212) from the transformed slots
-// fp[5][2] (P basis) and fh[4][2] (H basis) and the stored levels.  The carry
-// terms Pw(X) of the reference are f2 * S (Gauss quadrature).  Writes the new
-// current level (the explicit scheme first moves current -> previous) and
-// returns it in eta/delta/Phi.
-__device__ inline void spectral_update(const double* cf, const CoefLayout& cl, double f2, int n, int expflag,
-                                       int diffflag, const double fp[5][2], const double fh[4][2], double* Scur,
-                                       double* Sprev, long long so, long long fstride, double eta_n[2],
-                                       double delta_n[2], double Phi_n[2]) {
+// Semi-implicit Legendre inputs of one (member, j, m) (semi_implicit_tdiff.py:163-190),
+// w-weighted, in the slot order spectral_update reads: P0 tc i m A*, P1 tc i m B*,
+// P2 tc i m C, P3 E, P4 2 dt PhiF (forced); H5 tc A*, H6 tc B*, H7 tc D; H8 unused.
+// A* = A - G~, B* = B + F~ when forced, with (F~, G~) the lagged momentum remainder
+// (relax) or the current F, G.
+__device__ inline void assemble_slots_si(const double* cf, double tc, double wj, int m, int forcflag, int relax,
+                                         const double f[8][2], const double lagR[2][2], double out[kNT][2]) {
+  const double dt = cf[C_DT], mm = (double)m;
+  const double* Fe = relax ? lagR[0] : f[6];
+  const double* Ge = relax ? lagR[1] : f[7];
+  double Ae[2], Be[2];
+  for (int c = 0; c < 2; ++c) {
+    Ae[c] = forcflag ? f[0][c] - Ge[c] : f[0][c];
+    Be[c] = forcflag ? f[1][c] + Fe[c] : f[1][c];
+  }
+  const double wt = wj * tc;
+  // i m z = (-m z.im, m z.re)
+  out[0][0] = -wt * mm * Ae[1];
+  out[0][1] = wt * mm * Ae[0];
+  out[1][0] = -wt * mm * Be[1];
+  out[1][1] = wt * mm * Be[0];
+  out[2][0] = -wt * mm * f[2][1];
+  out[2][1] = wt * mm * f[2][0];
+  out[3][0] = wj * f[4][0];
+  out[3][1] = wj * f[4][1];
+  out[4][0] = forcflag ? wj * 2.0 * dt * f[5][0] : 0.0;
+  out[4][1] = forcflag ? wj * 2.0 * dt * f[5][1] : 0.0;
+  for (int c = 0; c < 2; ++c) {
+    out[5][c] = wt * Ae[c];
+    out[6][c] = wt * Be[c];
+    out[7][c] = wt * f[3][c];
+    out[8][c] = 0.0;
+  }
+}
+
+// FP64 spectral update of one (member, m, n) from the transformed Legendre inputs
+// fp (P basis) / fh (H basis), followed by the time filter on the stored levels.
+// The carry terms Pw(X) of the reference are f2 * S (Gauss quadrature).
+//   modified Euler (modEuler_tdiff.py:30-268) or explicit (explicit_tdiff.py:20-233)
+//   with the implicit sigma6 diffusion (filters.py:11-27, :95-151);
+//   semi-implicit (fl.si; semi_implicit_tdiff.py:126-264): leapfrog base f2 * S_prev,
+//     slots fp = (Pw of tc i m A*, tc i m B*, tc i m C, E, 2 dt PhiF), fh = (Hw of
+//     tc A*, tc B*, tc D), implicit gravity waves and (fl.relax) relaxation/drag
+//     solved per degree in closed form, exponential hyperdiffusion.
+// Time filter (model.py:991-1110): RAW (fl.raw) moves d = S_prev - 2 S_curr + new
+//   into both levels; semi-implicit alone filters the prev lineage (classic RA);
+//   explicit moves current -> previous; modified Euler keeps only the new level.
+// Returns eff (the stored current level: Fourier rows of eta, delta, Phi) and nw (the
+// new level before the RAW adjustment: the winds are inverted from it).
+__device__ inline void spectral_update(const double* cf, const CoefLayout& cl, double f2, int m, int n,
+                                       const StepFlags& fl, const double fp[5][2], const double fh[4][2],
+                                       double* Scur, double* Sprev, long long so, long long fstride, double eff[3][2],
+                                       double nw[3][2]) {
   const double dt = cf[C_DT], aa = cf[C_A], Phibar = cf[C_PHIBAR];
   const double nn = (double)n * (double)(n + 1);
-  for (int c = 0; c < 2; ++c) {
-    const double eo = f2 * Scur[0 * fstride + so + c];
-    const double d_o = f2 * Scur[1 * fstride + so + c];
-    const double po = f2 * Scur[2 * fstride + so + c];
-    if (!expflag) {
-      const double t2 = (2.0 * dt / (aa * aa)) / 4.0;
-      Phi_n[c] = po - dt * Phibar * d_o - Phibar * 0.5 * ((1.0 / (aa * aa)) * (nn * po)) + fp[0][c] + nn * fp[1][c] +
-                 fh[0][c];
-      delta_n[c] = d_o + nn * (t2 * po) - Phibar * (nn * d_o) / (aa * aa) + fp[2][c] + nn * fp[3][c] + fh[1][c] +
-                   nn * fh[2][c];
-      eta_n[c] = eo + fp[4][c] + fh[3][c];
-    } else {
-      const double ep = f2 * Sprev[0 * fstride + so + c];
-      const double dp = f2 * Sprev[1 * fstride + so + c];
-      const double pp = f2 * Sprev[2 * fstride + so + c];
-      Phi_n[c] = pp - 2.0 * dt * Phibar * d_o + fp[0][c] + fh[0][c];
-      delta_n[c] = dp + fp[2][c] + fh[1][c];
-      eta_n[c] = ep + fp[4][c] + fh[3][c];
-    }
-  }
-  if (diffflag) {
-    const double sg = cf[cl.n_sigma + n], sp = cf[cl.n_sigmaphi + n];
+  // The prev level is read only by the schemes and filters that use it (the modified-Euler
+  // step without RAW reads and writes the current level only).
+  const bool need_prev = fl.si || fl.raw || fl.expflag;
+  double sc[3][2], sp[3][2];
+  for (int f = 0; f < 3; ++f)
     for (int c = 0; c < 2; ++c) {
-      Phi_n[c] *= sp;
-      delta_n[c] *= sg;
-      eta_n[c] *= sg;
+      sc[f][c] = Scur[f * fstride + so + c];
+      sp[f][c] = need_prev ? Sprev[f * fstride + so + c] : 0.0;
+    }
+  if (fl.si) {
+    const double two_dt = 2.0 * dt, sia = cf[C_SIALPHA];
+    const double lam = nn / (aa * aa);
+    const double itd = fl.relax ? cf[C_INVTD] : 0.0, itr = fl.relax ? cf[C_INVTAURAD] : 0.0;
+    const double xi = two_dt * sia, expl = two_dt * (1.0 - sia);
+    const double kd = xi * itd, kr = xi * itr, gd = expl * itd, gr = expl * itr;
+    const double b = xi * lam, cc = xi * Phibar;
+    const double fmn = (m == 0 && n == 1) ? cf[C_FMN01] : 0.0;
+    for (int c = 0; c < 2; ++c) {
+      const double e0 = f2 * sp[0][c], d0 = f2 * sp[1][c], p0 = f2 * sp[2][c];
+      const double fr = c == 0 ? fmn : 0.0;
+      const double eta_star = e0 - fp[0][c] + fh[1][c] - gd * (e0 - fr);
+      const double delta_star =
+          d0 + fp[1][c] + fh[0][c] + two_dt * lam * fp[3][c] + expl * lam * p0 - gd * d0;
+      double Phi_star = p0 - fp[2][c] + fh[2][c] - expl * Phibar * d0 - gr * p0;
+      // fp[4] = Pw(2 dt PhiF) is zero unless forced; relax implies forced.
+      Phi_star += fp[4][c];
+      if (fl.relax) Phi_star += two_dt * itr * (f2 * sc[2][c]);
+      const double dn = (delta_star + b * Phi_star / (1.0 + kr)) / ((1.0 + kd) + b * cc / (1.0 + kr));
+      nw[0][c] = (eta_star + kd * fr) / (1.0 + kd);
+      nw[1][c] = dn;
+      nw[2][c] = (Phi_star - cc * dn) / (1.0 + kr);
+    }
+    if (fl.diffflag) {
+      const double sg = cf[cl.n_sigexp + n], sph = cf[cl.n_sigexpphi + n];
+      for (int c = 0; c < 2; ++c) {
+        nw[0][c] *= sg;
+        nw[1][c] *= sg;
+        nw[2][c] *= sph;
+      }
+    }
+  } else {
+    for (int c = 0; c < 2; ++c) {
+      const double eo = f2 * sc[0][c];
+      const double d_o = f2 * sc[1][c];
+      const double po = f2 * sc[2][c];
+      if (!fl.expflag) {
+        const double t2 = (2.0 * dt / (aa * aa)) / 4.0;
+        nw[2][c] = po - dt * Phibar * d_o - Phibar * 0.5 * ((1.0 / (aa * aa)) * (nn * po)) + fp[0][c] + nn * fp[1][c] +
+                   fh[0][c];
+        nw[1][c] = d_o + nn * (t2 * po) - Phibar * (nn * d_o) / (aa * aa) + fp[2][c] + nn * fp[3][c] + fh[1][c] +
+                   nn * fh[2][c];
+        nw[0][c] = eo + fp[4][c] + fh[3][c];
+      } else {
+        const double ep = f2 * sp[0][c];
+        const double dp = f2 * sp[1][c];
+        const double pp = f2 * sp[2][c];
+        nw[2][c] = pp - 2.0 * dt * Phibar * d_o + fp[0][c] + fh[0][c];
+        nw[1][c] = dp + fp[2][c] + fh[1][c];
+        nw[0][c] = ep + fp[4][c] + fh[3][c];
+      }
+    }
+    if (fl.diffflag) {
+      const double sg = cf[cl.n_sigma + n], sph = cf[cl.n_sigmaphi + n];
+      for (int c = 0; c < 2; ++c) {
+        nw[2][c] *= sph;
+        nw[1][c] *= sg;
+        nw[0][c] *= sg;
+      }
     }
   }
-  for (int c = 0; c < 2; ++c) {
-    if (expflag) {
-      Sprev[0 * fstride + so + c] = Scur[0 * fstride + so + c];
-      Sprev[1 * fstride + so + c] = Scur[1 * fstride + so + c];
-      Sprev[2 * fstride + so + c] = Scur[2 * fstride + so + c];
+  const double alpha = cf[C_ALPHA];
+  for (int f = 0; f < 3; ++f)
+    for (int c = 0; c < 2; ++c) {
+      double cur = nw[f][c];
+      if (fl.raw) {
+        if (fl.do_ra) {
+          const double w = cf[C_WILLIAMS];
+          const double d = sp[f][c] - 2.0 * sc[f][c] + nw[f][c];
+          Sprev[f * fstride + so + c] = sc[f][c] + alpha * w * d;
+          cur = nw[f][c] - alpha * (1.0 - w) * d;
+        } else {
+          Sprev[f * fstride + so + c] = sc[f][c];
+        }
+      } else if (fl.si) {
+        Sprev[f * fstride + so + c] =
+            fl.do_ra ? sc[f][c] + alpha * (sp[f][c] - 2.0 * sc[f][c] + nw[f][c]) : sc[f][c];
+      } else if (fl.expflag) {
+        Sprev[f * fstride + so + c] = sc[f][c];
+      }
+      Scur[f * fstride + so + c] = cur;
+      eff[f][c] = cur;
     }
-    Scur[0 * fstride + so + c] = eta_n[c];
-    Scur[1 * fstride + so + c] = delta_n[c];
-    Scur[2 * fstride + so + c] = Phi_n[c];
+}
+
+// Per-member coefficients of the initial fields' non-band-limited residue in the filtered
+// lineages (prev, curr): the new level is band-limited, so the residue only moves between the
+// two filtered levels (same recurrences as spectral_update's time filter).
+__device__ inline void residue_step(const double* cf, const StepFlags& fl, double* rc) {
+  const double cp = rc[0], cc = rc[1], alpha = cf[C_ALPHA];
+  if (fl.raw && fl.do_ra) {
+    const double w = cf[C_WILLIAMS];
+    const double d = cp - 2.0 * cc;
+    rc[0] = cc + alpha * w * d;
+    rc[1] = -alpha * (1.0 - w) * d;
+  } else if (fl.si && fl.do_ra) {
+    rc[0] = cc + alpha * (cp - 2.0 * cc);
+    rc[1] = 0.0;
+  } else {
+    rc[0] = cc;
+    rc[1] = 0.0;
   }
 }
 

@@ -43,7 +43,8 @@ struct Geometry {
   std::vector<double> mus, w, lam, P, H;
 };
 
-bool build_geometry(int M, bool exact_basis, Geometry& g) {
+// Geometry of truncation M; quadrature from (qmus, qw) when given (checked by the caller).
+bool build_geometry(int M, bool exact_basis, Geometry& g, const double* qmus = nullptr, const double* qw = nullptr) {
   int J, I;
   if (!resolution(M, &J, &I, nullptr)) return false;
   g.M = M;
@@ -53,7 +54,12 @@ bool build_geometry(int M, bool exact_basis, Geometry& g) {
   g.mus.resize(J);
   g.w.resize(J);
   g.lam.resize(I);
-  gauss_legendre(J, g.mus.data(), g.w.data());
+  if (qmus && qw) {
+    g.mus.assign(qmus, qmus + J);
+    g.w.assign(qw, qw + J);
+  } else {
+    gauss_legendre(J, g.mus.data(), g.w.data());
+  }
   lambdas(I, g.lam.data());
   g.P.resize((size_t)J * (M + 1) * (M + 1));
   g.H.resize(g.P.size());
@@ -83,6 +89,7 @@ struct swx_plan {
   bool use_mfma = false;       // MFMA grid kernel available for this plan
   bool use_bf16x3 = false;     // split-BF16x3 grid kernel (precision SWX_PREC_BF16X3)
   bool use_adj_mfma = false;   // MFMA grid adjoint kernel (FP32 transforms, where its LDS fits)
+  bool use_leg_adj_mfma = false;  // MFMA Legendre adjoint kernels (FP32 transforms)
   void* TinvB = nullptr;       // its DFT tables: bf16 hi / lo pairs in fragment order
   void* TfwdB = nullptr;
   bool opt_plain_grid = false;  // SWX_OPT_GRID_KERNEL = 1
@@ -104,6 +111,13 @@ struct swx_plan {
   void* g0uv = nullptr;
   double* phys = nullptr;
   double* uvic = nullptr;
+  // semi-implicit / RAW plans: the initial fields' residue (Fourier rows in the G layout,
+  // physical [B][3][J][I] for RAW), its per-member coefficients [B][2] and the exported
+  // lagged momentum remainder (semi-implicit)
+  void* R0f = nullptr;
+  double* R0p = nullptr;
+  double* resid_c = nullptr;
+  void* lagx = nullptr;
   double* part_uv = nullptr;
   unsigned char* dead = nullptr;
   bool initialized = false;
@@ -195,6 +209,8 @@ void fill_coef(const swx_plan* p, std::vector<double>& out) {
     c[C_AREAPLANET] = 4.0 * pi * a * a;
     c[C_FMN01] = pr[SWX_P_OMEGA] / std::sqrt(0.375);
     c[C_INVTAURAD] = 1.0 / pr[SWX_P_TAURAD];
+    c[C_SIALPHA] = pr[SWX_P_SI_ALPHA];
+    c[C_WILLIAMS] = pr[SWX_P_WILLIAMS_ALPHA];
     const double a3 = a * a * a, a6 = a3 * a3;
     for (int n = 0; n <= N; ++n) {
       const double nv = n, n1 = n + 1.0;
@@ -202,6 +218,9 @@ void fill_coef(const swx_plan* p, std::vector<double>& out) {
       c[cl.n_sigma + n] = 1.0 / (1.0 + (2.0 * dt * K6) * (ncoeff - 8.0 / a6));
       c[cl.n_sigmaphi + n] = 1.0 / (1.0 + (2.0 * dt * K6Phi) * ncoeff);
       c[cl.n_tcmn + n] = n == 0 ? 0.0 : a / (nv * n1);
+      // Exponential hyperdiffusion of the semi-implicit leapfrog (filters.py:154-213).
+      c[cl.n_sigexp + n] = std::exp(-(2.0 * dt * K6) * std::max(ncoeff - 8.0 / a6, 0.0));
+      c[cl.n_sigexpphi + n] = std::exp(-(2.0 * dt * K6Phi) * ncoeff);
     }
     const double dl = lam[2] - lam[1];
     const double dp = std::asin(mus[2]) - std::asin(mus[1]);
@@ -212,6 +231,13 @@ void fill_coef(const swx_plan* p, std::vector<double>& out) {
     }
   }
 }
+
+// Semi-implicit relaxation/drag taken implicitly: forced runs with test None
+// (model.py:905-925).
+int relax_implicit(const swx_plan* p) { return p->d.semi_implicit && p->d.forcflag && p->d.test == -1 ? 1 : 0; }
+
+// Time filter active at step t: modalflag and t > 2 (model.py:991).
+int filter_on(const swx_plan* p, int t) { return (p->d.modalflag && t > 2) ? 1 : 0; }
 
 template <typename R>
 LegArgs<R> leg_args(swx_plan* p, int mode) {
@@ -240,6 +266,10 @@ LegArgs<R> leg_args(swx_plan* p, int mode) {
   a.G = (R*)p->G;
   a.part_uv = p->part_uv;
   a.dead = p->dead;
+  a.si = p->d.semi_implicit;
+  a.raw_filter = p->d.raw_filter;
+  a.relax = relax_implicit(p);
+  a.resid_c = p->resid_c;
   return a;
 }
 
@@ -271,6 +301,12 @@ GridArgs<R> grid_args(swx_plan* p, int mode) {
   a.raw = (R*)p->raw;
   a.lag = (R*)p->lag;
   a.g0uv = (R*)p->g0uv;
+  a.si = p->d.semi_implicit;
+  a.raw_filter = p->d.raw_filter;
+  a.relax = relax_implicit(p);
+  a.lagx = (R*)p->lagx;
+  a.resid_c = p->resid_c;
+  a.R0p = p->R0p;
   return a;
 }
 
@@ -337,6 +373,16 @@ int do_init(swx_plan* p, const double* eta0, const double* delta0, const double*
   SWX_HIP(launch_grid<R>(gi, s));
   const size_t gbytes = (size_t)B * kNG * p->geo.J * (p->geo.M + 1) * 2 * p->rsz;
   SWX_HIP(hipMemcpyAsync(p->Gprev, p->G, gbytes, hipMemcpyDeviceToDevice, s));
+  if (p->d.semi_implicit || p->d.raw_filter) {
+    // 4) Residue of the initial fields outside the band-limited state: the filtered levels
+    //    of these schemes carry it (swx_kernels.hip:k_residue_init).
+    LegArgs<R> lp = leg_args<R>(p, LEG_PRIME);
+    lp.diagnostics = 0;
+    lp.G = (R*)p->R0f;
+    SWX_HIP(launch_leg<R>(lp, s));
+    SWX_HIP(launch_residue_init<R>((const R*)p->G, (R*)p->R0f, p->R0p, eta0, delta0, Phi0, (const R*)p->tw, p->resid_c,
+                                   B, p->geo.J, p->geo.I, p->geo.M + 1, s));
+  }
   p->initialized = true;
   return SWX_OK;
 }
@@ -350,8 +396,18 @@ int phys_window(const swx_plan* p) {
   if (p->opt_full_phys || p->d.diagnostics) return 1 << 30;
   if (p->opt_phys_steps > 0) return p->opt_phys_steps;
   if (p->opt_phys_steps < 0) return 0;
+  // Contraction of the prev level per step: alpha (classic RA) or, for RAW,
+  // alpha w / (1 - alpha (1 - w)) (swx_physics.h:physical_carry).
   double amax = 0.0;
-  for (int b = 0; b < p->d.n_members; ++b) amax = std::max(amax, std::fabs(p->params[(size_t)b * SWX_NPARAM + SWX_P_ALPHA]));
+  for (int b = 0; b < p->d.n_members; ++b) {
+    const double* pr = &p->params[(size_t)b * SWX_NPARAM];
+    double c = std::fabs(pr[SWX_P_ALPHA]);
+    if (p->d.raw_filter) {
+      const double al = pr[SWX_P_ALPHA], w = pr[SWX_P_WILLIAMS_ALPHA];
+      c = std::fabs(al * w / (1.0 - al * (1.0 - w)));
+    }
+    amax = std::max(amax, c);
+  }
   if (amax >= 0.5) return 1 << 30;
   if (amax == 0.0) return 16;
   const int k = (int)std::ceil(64.0 * std::log(2.0) / std::log(1.0 / amax)) + 2;
@@ -372,9 +428,10 @@ int do_steps(swx_plan* p, int t0, int nsteps, double* hist_fields, double* hist_
     l.hist_dead = hist_dead;
     l.hist_T = hist_T;
     l.hist_k = hist_k0 + k;
+    l.do_ra = filter_on(p, t);
     SWX_HIP(launch_leg_any<R>(p, l, s));
     GridArgs<R> g = grid_args<R>(p, GRID_STEP);
-    g.do_ra = (p->d.modalflag && t > 2) ? 1 : 0;
+    g.do_ra = filter_on(p, t);
     g.phys_on = (k >= phys_from || hist_fields || hist_phiext) ? 1 : 0;
     g.export_raw = last ? 1 : 0;
     g.hist_fields = hist_fields;
@@ -425,6 +482,13 @@ int prime(swx_plan* p, bool init, hipStream_t s) {
   if (init) {
     const size_t gbytes = (size_t)p->d.n_members * kNG * p->geo.J * (p->geo.M + 1) * 2 * p->rsz;
     SWX_HIP(hipMemcpyAsync(p->Gprev, p->G, gbytes, hipMemcpyDeviceToDevice, s));
+    if (p->d.semi_implicit || p->d.raw_filter) {
+      // A spectral initial state is band-limited: no residue.
+      const int B = p->d.n_members;
+      SWX_HIP(hipMemsetAsync(p->R0f, 0, gbytes, s));
+      SWX_HIP(hipMemsetAsync(p->resid_c, 0, (size_t)B * 2 * 8, s));
+      if (p->R0p) SWX_HIP(hipMemsetAsync(p->R0p, 0, (size_t)B * 3 * p->geo.J * p->geo.I * 8, s));
+    }
   }
   return SWX_OK;
 }
@@ -460,7 +524,21 @@ LegAdjArgs<R> leg_adj_args(swx_plan* p, int mode) {
   a.Tbar = (R*)p->Tbar;
   a.G = (R*)p->G;
   a.Gbar = (const R*)p->Gbar;
+  a.fwdP = p->fragP[0];
+  a.fwdH = p->fragP[1];
+  a.invP = p->fragP[2];
+  a.invH = p->fragP[3];
   return a;
+}
+
+// Legendre adjoint launch: the matrix-core kernels for FP32 plans (unless the plain
+// Legendre kernel is forced), else the plain kernel.
+template <typename R>
+hipError_t launch_leg_adj_any(swx_plan* p, const LegAdjArgs<R>& a, hipStream_t s) {
+  if constexpr (sizeof(R) == 4) {
+    if (p->use_leg_adj_mfma && !p->opt_plain_leg) return launch_leg_adj_mfma(a, s);
+  }
+  return launch_leg_adj<R>(a, s);
 }
 
 template <typename R>
@@ -496,7 +574,7 @@ int adjoint_terminal(swx_plan* p, const double* ge, const double* gd, const doub
   g.g_U = gu;
   g.g_V = gv;
   SWX_HIP(launch_grid_adj<R>(g, s));
-  SWX_HIP(launch_leg_adj<R>(leg_adj_args<R>(p, LADJ_TERM), s));
+  SWX_HIP(launch_leg_adj_any<R>(p, leg_adj_args<R>(p, LADJ_TERM), s));
   return SWX_OK;
 }
 
@@ -504,7 +582,7 @@ template <typename R>
 int adjoint_step(swx_plan* p, const double* St, hipStream_t s) {
   LegAdjArgs<R> a = leg_adj_args<R>(p, LADJ_A);
   a.St = St;
-  SWX_HIP(launch_leg_adj<R>(a, s));
+  SWX_HIP(launch_leg_adj_any<R>(p, a, s));
   const GridAdjArgs<R> g = grid_adj_args<R>(p, GADJ_STEP);
   if constexpr (sizeof(R) == 4) {
     if (p->use_adj_mfma && !p->opt_plain_grid) {
@@ -515,7 +593,7 @@ int adjoint_step(swx_plan* p, const double* St, hipStream_t s) {
   } else {
     SWX_HIP(launch_grid_adj<R>(g, s));
   }
-  SWX_HIP(launch_leg_adj<R>(leg_adj_args<R>(p, LADJ_B), s));
+  SWX_HIP(launch_leg_adj_any<R>(p, leg_adj_args<R>(p, LADJ_B), s));
   return SWX_OK;
 }
 
@@ -584,8 +662,9 @@ int ens_steps(swx_ensemble* e, int t0, int nsteps, bool save_last, hipStream_t s
         SWX_HIP(hipMemcpyAsync(p->Gprev, p->G, gbytes, hipMemcpyDeviceToDevice, s));
       }
       l[q] = leg_args<float>(p, LEG_STEP);
+      l[q].do_ra = filter_on(p, t);
       g[q] = grid_args<float>(p, GRID_STEP);
-      g[q].do_ra = (p->d.modalflag && t > 2) ? 1 : 0;
+      g[q].do_ra = filter_on(p, t);
       g[q].phys_on = k >= phys_from[q] ? 1 : 0;
       g[q].export_raw = last ? 1 : 0;
     }
@@ -599,6 +678,19 @@ int ens_steps(swx_ensemble* e, int t0, int nsteps, bool save_last, hipStream_t s
     } else {
       for (int q = 0; q < 2; ++q) SWX_HIP(launch_grid_any<float>(e->p[q], g[q], s));
     }
+  }
+  return SWX_OK;
+}
+
+// Parameter checks the kernels rely on: the RAW physical carry recovers the filter
+// increment from the filtered level, which needs alpha (1 - williams_alpha) != 1.
+int check_params(const swx_plan_desc* d, const double* params) {
+  if (!d->raw_filter) return SWX_OK;
+  for (int b = 0; b < d->n_members; ++b) {
+    const double al = params[(size_t)b * SWX_NPARAM + SWX_P_ALPHA];
+    const double w = params[(size_t)b * SWX_NPARAM + SWX_P_WILLIAMS_ALPHA];
+    if (!(std::fabs(1.0 - al * (1.0 - w)) > 1e-12))
+      return fail(SWX_ERR_ARG, "raw_filter: alpha * (1 - williams_alpha) must differ from 1");
   }
   return SWX_OK;
 }
@@ -631,15 +723,41 @@ int swx_geometry(int M, int exact_basis, double* mus, double* w, double* lambdas
 
 int swx_plan_create(const swx_plan_desc* desc, const double* params, swx_plan** out) {
   if (!desc || !params || !out) return fail(SWX_ERR_ARG, "null argument");
+  int rc0;
   *out = nullptr;
   if (desc->n_members < 1) return fail(SWX_ERR_ARG, "n_members must be >= 1");
   if (desc->precision != SWX_PREC_FP64 && desc->precision != SWX_PREC_FP32 && desc->precision != SWX_PREC_BF16X3)
     return fail(SWX_ERR_ARG, "unsupported precision " + std::to_string(desc->precision));
   if (!(desc->test == -1 || desc->test == 1 || desc->test == 2))
     return fail(SWX_ERR_ARG, "test must be -1 (None), 1 or 2");
+  // model.py:1531-1537
+  if (desc->semi_implicit && desc->expflag)
+    return fail(SWX_ERR_ARG, "semi_implicit=True is incompatible with expflag=True (pick one scheme).");
+  if (desc->raw_filter && desc->expflag)
+    return fail(SWX_ERR_ARG,
+                "raw_filter=True is incompatible with expflag=True (the explicit scheme reads the unfiltered "
+                "spectral prev carry, so the RAW lineage change would alter its parity behavior).");
+  if ((rc0 = check_params(desc, params))) return rc0;
+  if ((desc->quad_mus == nullptr) != (desc->quad_w == nullptr))
+    return fail(SWX_ERR_ARG, "quad_mus and quad_w must be given together");
+  if (desc->quad_mus) {
+    int Jq = 0;
+    if (!resolution(desc->M, &Jq, nullptr, nullptr))
+      return fail(SWX_ERR_ARG, "Unsupported value of M=" + std::to_string(desc->M) + ". Only 42, 63, and 106 are supported.");
+    double wsum = 0.0;
+    for (int j = 0; j < Jq; ++j) {
+      const double x = desc->quad_mus[j], wj = desc->quad_w[j];
+      if (!(x > -1.0 && x < 1.0) || !(wj > 0.0) || (j > 0 && !(x > desc->quad_mus[j - 1])))
+        return fail(SWX_ERR_ARG, "quad_mus must ascend inside (-1, 1) and quad_w be positive");
+      wsum += wj;
+    }
+    if (!(std::fabs(wsum - 2.0) < 1e-10)) return fail(SWX_ERR_ARG, "quad_w must sum to 2");
+  }
   swx_plan* p = new swx_plan();
   p->d = *desc;
-  if (!build_geometry(desc->M, desc->exact_basis != 0, p->geo)) {
+  p->d.quad_mus = nullptr;  // the plan keeps its own copy (geo)
+  p->d.quad_w = nullptr;
+  if (!build_geometry(desc->M, desc->exact_basis != 0, p->geo, desc->quad_mus, desc->quad_w)) {
     delete p;
     return fail(SWX_ERR_ARG, "Unsupported value of M=" + std::to_string(desc->M) + ". Only 42, 63, and 106 are supported.");
   }
@@ -715,6 +833,13 @@ int swx_plan_create(const swx_plan_desc* desc, const double* params, swx_plan** 
     swx_plan_destroy(p);
     return rc;
   }
+  if ((desc->semi_implicit || desc->raw_filter) &&
+      ((rc = dalloc(p, &p->R0f, (size_t)B * kNG * cplxJM * rs)) || (rc = dalloc(p, &p->resid_c, (size_t)B * 2 * 8)) ||
+       (desc->semi_implicit && (rc = dalloc(p, &p->lagx, (size_t)B * 2 * cplxJM * rs))) ||
+       (desc->raw_filter && (rc = dalloc(p, &p->R0p, (size_t)B * 3 * J * I * 8))))) {
+    swx_plan_destroy(p);
+    return rc;
+  }
   if (desc->precision == SWX_PREC_FP32 || desc->precision == SWX_PREC_BF16X3) {
     int maxlds = 0;
     (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device);
@@ -723,6 +848,8 @@ int swx_plan_create(const swx_plan_desc* desc, const double* params, swx_plan** 
     p->use_adj_mfma = p->use_mfma && adj_lds > 0 && adj_lds <= (size_t)maxlds;
     p->leg_mb = leg_mfma_default_mb(M);
     p->use_leg_mfma = leg_mfma_lds_bytes(M, p->leg_mb) <= (size_t)maxlds;
+    const size_t ladj_lds = leg_adj_mfma_lds_bytes(M);
+    p->use_leg_adj_mfma = p->use_leg_mfma && ladj_lds > 0 && ladj_lds <= (size_t)maxlds;
     if (p->use_leg_mfma) {
       // Fragment-ordered copies of the basis blocks (layouts in LegArgs).
       const int KNP = leg_knp(N);
@@ -845,6 +972,7 @@ int swx_plan_set_params(swx_plan* p, const double* params, void* stream) {
   int rc = check_plan(p);
   if (rc) return rc;
   if (!params) return fail(SWX_ERR_ARG, "null params");
+  if ((rc = check_params(&p->d, params))) return rc;
   p->params.assign(params, params + (size_t)p->d.n_members * SWX_NPARAM);
   std::vector<double> coef;
   fill_coef(p, coef);
@@ -859,7 +987,7 @@ void swx_plan_destroy(swx_plan* p) {
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   void* bufs[] = {p->Lam, p->Tbar, p->Gbar, p->pgrad, p->Pf, p->Hf, p->Pi, p->Hi, p->fragP[0], p->fragP[1],
                   p->fragP[2], p->fragP[3], p->tw, p->TinvF, p->TfwdF, p->TinvB, p->TfwdB, p->geo_d, p->f2, p->coef, p->S, p->T, p->G, p->Gprev,
-                  p->raw, p->lag, p->g0uv, p->phys, p->uvic, p->part_uv, p->dead};
+                  p->raw, p->lag, p->g0uv, p->phys, p->uvic, p->part_uv, p->dead, p->R0f, p->R0p, p->resid_c, p->lagx};
   for (void* q : bufs)
     if (q) (void)hipFree(q);
   delete p;
@@ -1079,23 +1207,25 @@ int swx_time_kernels(swx_plan* p, int t0, int nsteps, float* ms_leg, float* ms_g
   auto launch_grid_step = [&](int t) -> int {
     if (p->rsz == 8) {
       GridArgs<double> g = grid_args<double>(p, GRID_STEP);
-      g.do_ra = (p->d.modalflag && t > 2) ? 1 : 0;
+      g.do_ra = filter_on(p, t);
       g.phys_on = phys_on;
       SWX_HIP(launch_grid<double>(g, s));
     } else {
       GridArgs<float> g = grid_args<float>(p, GRID_STEP);
-      g.do_ra = (p->d.modalflag && t > 2) ? 1 : 0;
+      g.do_ra = filter_on(p, t);
       g.phys_on = phys_on;
       SWX_HIP(launch_grid_any<float>(p, g, s));
     }
     return SWX_OK;
   };
-  auto launch_leg_step = [&]() -> int {
+  auto launch_leg_step = [&](int t) -> int {
     if (p->rsz == 8) {
       LegArgs<double> l = leg_args<double>(p, LEG_STEP);
+      l.do_ra = filter_on(p, t);
       SWX_HIP(launch_leg<double>(l, s));
     } else {
       LegArgs<float> l = leg_args<float>(p, LEG_STEP);
+      l.do_ra = filter_on(p, t);
       SWX_HIP(launch_leg_any<float>(p, l, s));
     }
     return SWX_OK;
@@ -1106,7 +1236,7 @@ int swx_time_kernels(swx_plan* p, int t0, int nsteps, float* ms_leg, float* ms_g
   if (!phys_on) {
     SWX_HIP(hipEventRecord(ev[0], s));
     for (int k = 0; k < nsteps && rc == SWX_OK; ++k)
-      if ((rc = launch_leg_step()) == SWX_OK) rc = launch_grid_step(t0 + k);
+      if ((rc = launch_leg_step(t0 + k)) == SWX_OK) rc = launch_grid_step(t0 + k);
     SWX_HIP(hipEventRecord(ev[1], s));
     // With the physical carry off the grid launch reads G and rewrites the same T:
     // repeating it leaves the state as it is.
@@ -1122,7 +1252,7 @@ int swx_time_kernels(swx_plan* p, int t0, int nsteps, float* ms_leg, float* ms_g
     // Carry written every step (repeats would advance the RA filter): events around each launch.
     for (int k = 0; k < nsteps && rc == SWX_OK; ++k) {
       SWX_HIP(hipEventRecord(ev[0], s));
-      if ((rc = launch_leg_step()) != SWX_OK) break;
+      if ((rc = launch_leg_step(t0 + k)) != SWX_OK) break;
       SWX_HIP(hipEventRecord(ev[1], s));
       if ((rc = launch_grid_step(t0 + k)) != SWX_OK) break;
       SWX_HIP(hipEventRecord(ev[2], s));
@@ -1179,6 +1309,8 @@ int swx_state_get(swx_plan* p, double* out, void* stream) {
   if (rc) return rc;
   if (!out) return fail(SWX_ERR_ARG, "null output");
   if (!p->initialized) return fail(SWX_ERR_STATE, "swx_state_get before initialization");
+  if (p->d.semi_implicit || p->d.raw_filter)
+    return fail(SWX_ERR_ARG, "spectral checkpoints hold one level: not available with semi_implicit / raw_filter");
   return copy_state(p, out, nullptr, false, false, (hipStream_t)stream);
 }
 
@@ -1187,6 +1319,8 @@ int swx_state_set(swx_plan* p, const double* in, void* stream) {
   if (rc) return rc;
   if (!in) return fail(SWX_ERR_ARG, "null state");
   if (!p->initialized) return fail(SWX_ERR_STATE, "swx_state_set before initialization");
+  if (p->d.semi_implicit || p->d.raw_filter)
+    return fail(SWX_ERR_ARG, "spectral checkpoints hold one level: not available with semi_implicit / raw_filter");
   hipStream_t s = (hipStream_t)stream;
   if ((rc = copy_state(p, nullptr, in, true, true, s))) return rc;
   return p->rsz == 8 ? prime<double>(p, false, s) : prime<float>(p, false, s);
@@ -1195,7 +1329,8 @@ int swx_state_set(swx_plan* p, const double* in, void* stream) {
 int swx_adjoint_begin(swx_plan* p, void* stream) {
   int rc = check_plan(p);
   if (rc) return rc;
-  if (p->d.expflag) return fail(SWX_ERR_ARG, "the adjoint covers the modified-Euler scheme (expflag = 0)");
+  if (p->d.expflag || p->d.semi_implicit || p->d.raw_filter)
+    return fail(SWX_ERR_ARG, "the adjoint covers the modified-Euler scheme (expflag = 0, no semi_implicit / raw_filter)");
   if ((rc = ensure_adjoint(p))) return rc;
   hipStream_t s = (hipStream_t)stream;
   const int B = p->d.n_members, J = p->geo.J, Mp = p->geo.M + 1, Np = p->geo.N + 1;
@@ -1271,16 +1406,56 @@ int swx_export_state(swx_plan* p, const swx_state_out* o, void* stream) {
     return SWX_OK;
   };
   const bool pinned = p->d.test == 1;
-  if ((rc = cf(o->etam_curr, p->G, 0)) || (rc = cf(o->deltam_curr, p->G, 1)) || (rc = cf(o->Phim_curr, p->G, 2)) ||
-      (rc = cf(o->etam_prev, p->Gprev, 0)) || (rc = cf(o->deltam_prev, p->Gprev, 1)) ||
-      (rc = cf(o->Phim_prev, p->Gprev, 2)) ||
-      (rc = pinned ? cp(o->Um_curr, p->g0uv, 2, 0, cjm) : cf(o->Um_curr, p->G, 3)) ||
+  if (p->d.semi_implicit || p->d.raw_filter) {
+    // Both Fourier levels are filtered: each is the inverse Legendre transform of its
+    // spectral level (into Gprev, free outside the modified-Euler export) plus the initial
+    // fields' residue times that level's coefficient.
+    for (int lvl = 1; lvl >= 0; --lvl) {
+      double* dst[3] = {lvl ? o->etam_curr : o->etam_prev, lvl ? o->deltam_curr : o->deltam_prev,
+                        lvl ? o->Phim_curr : o->Phim_prev};
+      if (!dst[0] && !dst[1] && !dst[2]) continue;
+      for (int f = 0; f < 3; ++f) {
+        hipError_t e = hipSuccess;
+        if (p->rsz == 8) {
+          if (f == 0) {
+            LegArgs<double> l = leg_args<double>(p, LEG_PRIME);
+            l.diagnostics = 0;
+            l.G = (double*)p->Gprev;
+            l.prime_prev = lvl == 0;
+            e = launch_leg<double>(l, s);
+          }
+          if (e == hipSuccess && dst[f])
+            e = launch_copy_fourier<double>(dst[f], (const double*)p->Gprev, f, B, J, Mp, s, (const double*)p->R0f,
+                                            p->resid_c, lvl);
+        } else {
+          if (f == 0) {
+            LegArgs<float> l = leg_args<float>(p, LEG_PRIME);
+            l.diagnostics = 0;
+            l.G = (float*)p->Gprev;
+            l.prime_prev = lvl == 0;
+            e = launch_leg<float>(l, s);
+          }
+          if (e == hipSuccess && dst[f])
+            e = launch_copy_fourier<float>(dst[f], (const float*)p->Gprev, f, B, J, Mp, s, (const float*)p->R0f,
+                                           p->resid_c, lvl);
+        }
+        if (e != hipSuccess) return fail(SWX_ERR_HIP, std::string("export: ") + hipGetErrorString(e));
+      }
+    }
+  } else if ((rc = cf(o->etam_curr, p->G, 0)) || (rc = cf(o->deltam_curr, p->G, 1)) ||
+             (rc = cf(o->Phim_curr, p->G, 2)) || (rc = cf(o->etam_prev, p->Gprev, 0)) ||
+             (rc = cf(o->deltam_prev, p->Gprev, 1)) || (rc = cf(o->Phim_prev, p->Gprev, 2))) {
+    return rc;
+  }
+  // Semi-implicit plans move the lagged remainder every step; the exported one is in lagx.
+  void* lagsrc = p->d.semi_implicit ? p->lagx : p->lag;
+  if ((rc = pinned ? cp(o->Um_curr, p->g0uv, 2, 0, cjm) : cf(o->Um_curr, p->G, 3)) ||
       (rc = pinned ? cp(o->Vm_curr, p->g0uv, 2, 1, cjm) : cf(o->Vm_curr, p->G, 4)) ||
       (rc = cp(o->Am_curr, p->raw, kNRaw, 0, cjm)) || (rc = cp(o->Bm_curr, p->raw, kNRaw, 1, cjm)) ||
       (rc = cp(o->Cm_curr, p->raw, kNRaw, 2, cjm)) || (rc = cp(o->Dm_curr, p->raw, kNRaw, 3, cjm)) ||
       (rc = cp(o->Em_curr, p->raw, kNRaw, 4, cjm)) || (rc = cp(o->PhiFm_curr, p->raw, kNRaw, 5, cjm)) ||
       (rc = cp(o->Fm_curr, p->raw, kNRaw, 6, cjm)) || (rc = cp(o->Gm_curr, p->raw, kNRaw, 7, cjm)) ||
-      (rc = cp(o->Rum_lag, p->lag, 2, 0, cjm)) || (rc = cp(o->Rvm_lag, p->lag, 2, 1, cjm)))
+      (rc = cp(o->Rum_lag, lagsrc, 2, 0, cjm)) || (rc = cp(o->Rvm_lag, lagsrc, 2, 1, cjm)))
     return rc;
   if ((rc = cpd(o->eta_prev, p->phys, 8, 0, pl)) || (rc = cpd(o->eta_curr, p->phys, 8, 1, pl)) ||
       (rc = cpd(o->delta_prev, p->phys, 8, 2, pl)) || (rc = cpd(o->delta_curr, p->phys, 8, 3, pl)) ||

@@ -6,9 +6,8 @@ time integration itself runs in the HIP kernels of libswampe_hip.so through
 :class:`my_swampe_amd.plan.Plan`; arrays are returned as torch tensors on the
 GPU (the counterpart of JAX device arrays).
 
-Out of scope here (they raise NotImplementedError): the opt-in semi-implicit
-scheme and RAW filter, continuation from pickles (contflag), and the legacy
-plotting/saving side effects of run_model.
+Out of scope here (they raise NotImplementedError): continuation from pickles
+(contflag) and the legacy plotting/saving side effects of run_model.
 """
 
 from __future__ import annotations
@@ -120,16 +119,21 @@ _GEOM: Dict[int, tuple] = {}
 
 
 def _geometry(M: int):
+    """(mus, w, lambdas, Pmn, Hmn): the reference's Gauss quadrature (what the plans use) and the
+    library's basis."""
     if M not in _GEOM:
-        _GEOM[M] = _capi.geometry(M)
+        mus, w, lam, P, H = _capi.geometry(M)
+        mus, w = _capi.reference_quadrature(mus.size)
+        _GEOM[M] = (mus, w, lam, P, H)
     return _GEOM[M]
 
 
-def param_row(*, dt, a, omega, g, Phibar, taurad, taudrag, DPhieq, K6, K6Phi, alpha, blowup_rms=8000.0) -> np.ndarray:
+def param_row(*, dt, a, omega, g, Phibar, taurad, taudrag, DPhieq, K6, K6Phi, alpha, blowup_rms=8000.0,
+              si_alpha=0.5, williams_alpha=0.53) -> np.ndarray:
     """One member's SWX_NPARAM parameter vector (order of include/swampe_capi.h)."""
     return np.array([float(dt), float(a), float(omega), float(g), float(Phibar), float(taurad), float(taudrag),
                      float(DPhieq), float(K6), -1.0 if K6Phi is None else float(K6Phi), float(alpha),
-                     float(blowup_rms)], dtype=np.float64)
+                     float(blowup_rms), float(si_alpha), float(williams_alpha)], dtype=np.float64)
 
 
 def build_static(*, M: int, dt: Scalar, a: Scalar, omega: Scalar, g: Scalar, Phibar: Scalar, taurad: Scalar,
@@ -265,10 +269,6 @@ def _validate(*, tmax, dt, test, semi_implicit, expflag, raw_filter, contflag, c
     start = 2 if starttime is None else int(starttime)
     if not contflag and start > tmax:
         raise ValueError(f"starttime={start} must be <= tmax={tmax}.")
-    if semi_implicit:
-        raise NotImplementedError("semi_implicit=True is not provided by my_swampe_amd (modified Euler and explicit only).")
-    if raw_filter:
-        raise NotImplementedError("raw_filter=True is not provided by my_swampe_amd.")
     if contflag:
         raise NotImplementedError("contflag=True (continuation from pickles) is not provided by my_swampe_amd.")
     if test not in (None, 1, 2):
@@ -358,15 +358,19 @@ def run_model_scan(*, M: int, dt: Scalar, tmax: int, Phibar: Scalar, omega: Scal
         raise ValueError(f"precision must be one of {tuple(_capi.PRECISIONS)}, got {precision!r}")
     torch = require_gpu()
     flags = RunFlags(forcflag=bool(forcflag), diffflag=bool(diffflag), expflag=bool(expflag),
-                     modalflag=bool(modalflag), diagnostics=bool(diagnostics), alpha=alpha,
-                     williams_alpha=williams_alpha, si_alpha=si_alpha)
+                     modalflag=bool(modalflag), diagnostics=bool(diagnostics), semi_implicit=bool(semi_implicit),
+                     raw_filter=bool(raw_filter), alpha=alpha, williams_alpha=williams_alpha, si_alpha=si_alpha)
     static = build_static(M=int(M), dt=dt, a=a, omega=omega, g=g, Phibar=Phibar, taurad=taurad, taudrag=taudrag,
                           DPhieq=DPhieq, K6=K6, K6Phi=K6Phi, test=test, alpha=alpha)
     eta0, delta0, Phi0, U0, V0 = _initial_fields(static, test, a1, eta0_init, delta0_init, Phi0_init,
                                                  U0_init, V0_init)
-    plan = Plan(static.M, static.params[None, :], precision=precision, expflag=flags.expflag,
+    params = static.params.copy()
+    params[_capi.PARAM_NAMES.index("si_alpha")] = float(si_alpha)
+    params[_capi.PARAM_NAMES.index("williams_alpha")] = float(williams_alpha)
+    plan = Plan(static.M, params[None, :], precision=precision, expflag=flags.expflag,
                 forcflag=flags.forcflag, diffflag=flags.diffflag, modalflag=flags.modalflag,
-                diagnostics=flags.diagnostics, test=test)
+                diagnostics=flags.diagnostics, test=test, semi_implicit=flags.semi_implicit,
+                raw_filter=flags.raw_filter)
     try:
         plan.init(eta0, delta0, Phi0, U0, V0)
         nsteps = int(tmax) - start

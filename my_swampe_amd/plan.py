@@ -38,7 +38,10 @@ class Plan:
 
     def __init__(self, M: int, params: np.ndarray, *, precision: str = "fp64", expflag: bool = False,
                  forcflag: bool = True, diffflag: bool = True, modalflag: bool = True,
-                 diagnostics: bool = False, test: Optional[int] = None, exact_basis: bool = False):
+                 diagnostics: bool = False, test: Optional[int] = None, exact_basis: bool = False,
+                 semi_implicit: bool = False, raw_filter: bool = False, quadrature="reference"):
+        """quadrature: "reference" (the reference's scipy Gauss nodes and weights, the default),
+        None (the library's built-in quadrature) or a (mus, w) pair of length-J arrays."""
         torch = require_gpu()
         lib = _capi.load()
         params = np.ascontiguousarray(np.asarray(params, dtype=np.float64))
@@ -54,11 +57,23 @@ class Plan:
         self.precision = precision
         self.test = test
         self.device = torch.device("cuda", torch.cuda.current_device())
+        quad = None
+        if isinstance(quadrature, str):
+            if quadrature != "reference":
+                raise ValueError('quadrature must be "reference", None or a (mus, w) pair')
+            quad = _capi.reference_quadrature(self.J)
+        elif quadrature is not None:
+            quad = tuple(np.ascontiguousarray(np.asarray(q, dtype=np.float64)) for q in quadrature)
+            if len(quad) != 2 or any(q.shape != (self.J,) for q in quad):
+                raise ValueError(f"quadrature must be two arrays of length J={self.J}")
         desc = _capi.PlanDesc(M=self.M, n_members=self.B, precision=_capi.PRECISIONS[precision],
                               expflag=int(bool(expflag)), forcflag=int(bool(forcflag)),
                               diffflag=int(bool(diffflag)), modalflag=int(bool(modalflag)),
                               diagnostics=int(bool(diagnostics)), test=-1 if test is None else int(test),
-                              exact_basis=int(bool(exact_basis)))
+                              exact_basis=int(bool(exact_basis)), semi_implicit=int(bool(semi_implicit)),
+                              raw_filter=int(bool(raw_filter)),
+                              quad_mus=None if quad is None else quad[0].ctypes.data,
+                              quad_w=None if quad is None else quad[1].ctypes.data)
         handle = ctypes.c_void_p()
         _capi.check(lib.swx_plan_create(ctypes.byref(desc), params.ctypes.data_as(ctypes.c_void_p),
                                         ctypes.byref(handle)), "swx_plan_create")

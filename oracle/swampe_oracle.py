@@ -738,7 +738,9 @@ def run_model_scan(*, M, dt, tmax, Phibar, omega, a, test=None, g=9.8, forcflag=
     if semi_implicit and expflag:
         raise ValueError("semi_implicit=True is incompatible with expflag=True (pick one scheme).")
     if raw_filter and expflag:
-        raise ValueError("raw_filter=True is incompatible with expflag=True.")
+        raise ValueError(
+            "raw_filter=True is incompatible with expflag=True (the explicit scheme reads the "
+            "unfiltered spectral prev carry, so the RAW lineage change would alter its parity behavior).")
     s = build_static(M=M, dt=dt, a=a, omega=omega, g=g, Phibar=Phibar, taurad=taurad, taudrag=taudrag,
                      DPhieq=DPhieq, K6=K6, K6Phi=K6Phi, test=test)
     start = 2 if starttime is None else int(starttime)

@@ -105,10 +105,6 @@ def test_validation_errors_raise_before_device_work():
 
 def test_out_of_scope_options_raise_not_implemented():
     with pytest.raises(NotImplementedError):
-        _run(tmax=4, semi_implicit=True)
-    with pytest.raises(NotImplementedError):
-        _run(tmax=4, raw_filter=True)
-    with pytest.raises(NotImplementedError):
         _run(tmax=4, test=99)
 
 

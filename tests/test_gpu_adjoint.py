@@ -160,18 +160,20 @@ def test_checkpoint_interval_does_not_change_gradients(grads):
     assert grads["stats"]["checkpoints"] == 2 and grads["stats"]["segment_states"] == 3
 
 
-@pytest.mark.parametrize("M", [42, 63])
-def test_mfma_grid_adjoint_matches_plain_fp32(M, torch_gpu):
-    """The MFMA grid adjoint kernel vs the plain FP32 grid adjoint on the same reverse sweep:
-    3 members with different forcing, 4 steps, random terminal Phi cotangent.  Relative L2 of
-    dL/dS_0 and of the parameter gradients <= 1e-5 (FP32 transforms in both, different
-    summation orders)."""
+@pytest.mark.parametrize("kernel,M", [("grid", 42), ("grid", 63), ("leg", 42), ("leg", 63), ("leg", 106)])
+def test_mfma_adjoint_matches_plain_fp32(kernel, M, torch_gpu):
+    """The matrix-core grid (or Legendre) adjoint kernels vs the plain FP32 ones, selected with
+    SWX_OPT_GRID_KERNEL (or SWX_OPT_LEG_KERNEL, which switches the forward and reverse Legendre
+    kernels together) on the same reverse sweep: 3 members with different forcing, 4 steps,
+    random terminal Phi cotangent.  Relative L2 of dL/dS_0 and of the parameter gradients
+    <= 1e-5 (FP32 transforms in both, different summation orders)."""
     from my_swampe_amd import _capi
     from my_swampe_amd.model import param_row
     from my_swampe_amd.plan import Plan
 
     torch = torch_gpu
-    print(f"[gpu-case] mfma-grid-adjoint M={M}", flush=True)
+    print(f"[gpu-case] mfma-adjoint kernel={kernel} M={M}", flush=True)
+    option = _capi.OPT_GRID_KERNEL if kernel == "grid" else _capi.OPT_LEG_KERNEL
     B, nsteps = 3, 4
     rng = np.random.default_rng(M)
     params = np.stack([param_row(dt=120.0, a=8.2e7, omega=3.2e-5, g=9.8, Phibar=3e5, taurad=10 ** rng.uniform(4.5, 5.5),
@@ -183,7 +185,7 @@ def test_mfma_grid_adjoint_matches_plain_fp32(M, torch_gpu):
     out = {}
     for kind, opt in (("plain", 1), ("mfma", 0)):
         plan = Plan(M, params, precision="fp32")
-        plan.set_option(_capi.OPT_GRID_KERNEL, opt)
+        plan.set_option(option, opt)
         plan.init_spectral(torch.tensor(c, device="cuda"))
         states = []
         for k in range(nsteps):
@@ -199,6 +201,6 @@ def test_mfma_grid_adjoint_matches_plain_fp32(M, torch_gpu):
         plan.close()
     rl = float(np.linalg.norm(out["mfma"][0] - out["plain"][0]) / np.linalg.norm(out["plain"][0]))
     rp = float(np.linalg.norm(out["mfma"][1] - out["plain"][1]) / np.linalg.norm(out["plain"][1]))
-    print(M, "dL/dS0 relL2", rl, "param grad relL2", rp)
+    print(kernel, M, "dL/dS0 relL2", rl, "param grad relL2", rp)
     assert np.isfinite(out["mfma"][0]).all() and np.isfinite(out["mfma"][1]).all()
     assert rl <= 1e-5 and rp <= 1e-5, (rl, rp)

@@ -1,0 +1,336 @@
+"""Semi-implicit leapfrog and Robert-Asselin-Williams (RAW) filter on the HIP path vs the oracle (GPU).
+
+Semi-implicit scheme: semi_implicit_tdiff.py:79-264 (implicit gravity waves and, when
+forced with test None, implicit relaxation/drag; exponential hyperdiffusion).  RAW filter:
+model.py:1007-1087.  Tolerances: over the reference fixtures' run length (4 steps) the FP64
+path is held to the reference's parity contract (eta/delta 1e-10, Phi 5e-8, U/V 1e-9 atol;
+unit_tests/test_parity_reference_regression.py:58-64) on the physical fields and to 1e-11 of
+the largest magnitude on the Fourier carries.  Longer FP64 runs, which the contract does not
+cover, are held to 1e-10 of each field's largest magnitude (the modified-Euler path measures
+about 2e-11 on delta after 8 steps, the loosest field).  The FP32 transform path is held to
+the relative L2 bounds written in its tests.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import ATOL
+from oracle import swampe_oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("eta", "delta", "Phi", "U", "V")
+CARRIES = ("etam_curr", "deltam_curr", "Phim_curr", "etam_prev", "deltam_prev", "Phim_prev", "Um_curr", "Vm_curr",
+           "Am_curr", "Bm_curr", "Cm_curr", "Dm_curr", "Em_curr", "PhiFm_curr", "Fm_curr", "Gm_curr", "Rum_lag",
+           "Rvm_lag")
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _print_case(tag, **kw):
+    print(f"[gpu-case] {tag} {kw}", flush=True)
+
+
+def _scale_atol(ref_arr, atol_rel):
+    return atol_rel * max(1.0, float(np.max(np.abs(ref_arr))))
+
+
+def _compare_runs(got, ref, history=True, rel=None):
+    """Every exported carry field, the per-step history and the diagnostics: at the parity-contract
+    atols (rel None), or at `rel` times each field's largest magnitude."""
+    last, rl = got["last_state"], ref["last_state"]
+
+    def tol(f, r):
+        return ATOL[f] if rel is None else _scale_atol(r, rel)
+
+    for f in FIELDS:
+        for lvl in ("_curr", "_prev") if f in ("eta", "delta", "Phi") else ("_curr",):
+            r = getattr(rl, f + lvl)
+            np.testing.assert_allclose(_np(getattr(last, f + lvl)), r, rtol=0, atol=tol(f, r), err_msg=f + lvl)
+    for fld in CARRIES:
+        r = getattr(rl, fld)
+        np.testing.assert_allclose(_np(getattr(last, fld)), r, rtol=0, atol=_scale_atol(r, rel or 1e-11),
+                                   err_msg=fld)
+    if history:
+        for f in FIELDS:
+            r = ref["outs"][f]
+            np.testing.assert_allclose(_np(got["outs"][f]), r, rtol=0, atol=tol(f, r), err_msg=f)
+        for f in ("rms", "spin_min", "phi_min", "phi_max"):
+            np.testing.assert_allclose(_np(got["outs"][f]), ref["outs"][f], rtol=1e-10, atol=1e-10, err_msg=f)
+
+
+BASE = dict(M=42, dt=1200.0, tmax=6, Phibar=3.0e5, omega=3.2e-5, a=8.2e7, test=None, g=9.8, forcflag=True,
+            taurad=36000.0, taudrag=21600.0, DPhieq=1.0e6, diffflag=True, modalflag=True, alpha=0.01,
+            expflag=False, K6=1.24e33)
+TEST_PLANET = dict(Phibar=3e3, omega=7.2921159e-5, a=6.37122e6)
+VARIANTS = {
+    "si_forced": dict(semi_implicit=True),
+    "si_unforced": dict(semi_implicit=True, forcflag=False),
+    "si_offcentred": dict(semi_implicit=True, si_alpha=0.6, dt=600.0),
+    "si_taudrag_sentinel": dict(semi_implicit=True, taudrag=-1.0),
+    "si_raw_production": dict(semi_implicit=True, raw_filter=True, alpha=0.05, dt=600.0),
+    "si_raw_no_diffusion_no_modal": dict(semi_implicit=True, raw_filter=True, diffflag=False, modalflag=False),
+    "modeuler_raw": dict(raw_filter=True),
+    "modeuler_raw_w07": dict(raw_filter=True, williams_alpha=0.7, alpha=0.05),
+    # Williamson test 1: a cosine bell, not band-limited (the residue path of the filtered levels).
+    "si_test1": dict(semi_implicit=True, test=1, forcflag=False, diffflag=False, dt=600.0, **TEST_PLANET),
+    "si_raw_test2": dict(semi_implicit=True, raw_filter=True, test=2, dt=600.0, Phibar=3e3 * 9.8,
+                         omega=7.2921159e-5, a=6.37122e6),
+}
+
+
+@pytest.mark.parametrize("name", list(VARIANTS))
+def test_si_raw_variants_vs_oracle(name, torch_gpu):
+    """Semi-implicit / RAW variants, 4 steps (3 with the filter active), FP64, every carry field."""
+    from my_swampe_amd.model import run_model_scan
+
+    kw = dict(BASE)
+    kw.update(VARIANTS[name])
+    _print_case("si-raw-variant", name=name)
+    ref = orc.run_model_scan(**kw)
+    got = run_model_scan(**kw, diagnostics=True)
+    _compare_runs(got, ref)
+
+
+@pytest.mark.parametrize("name", ["si_forced", "si_raw_production", "modeuler_raw_w07"])
+def test_si_raw_longer_runs_vs_oracle(name, torch_gpu):
+    """12 steps (11 with the filter active), FP64: every field within 1e-10 of its largest magnitude."""
+    from my_swampe_amd.model import run_model_scan
+
+    kw = dict(BASE, tmax=14)
+    kw.update(VARIANTS[name])
+    _print_case("si-raw-long", name=name)
+    ref = orc.run_model_scan(**kw)
+    got = run_model_scan(**kw, diagnostics=True)
+    _compare_runs(got, ref, rel=1e-10)
+
+
+def _rough_ic(M, seed, omega, phi_noise=5.0):
+    """Band-limited rest + noise fields plus grid-point white noise: initial fields far outside the
+    spectral band (the filtered levels then carry their residue)."""
+    eta0, delta0, Phi0, U0, V0 = orc.rest_noise_ic(M, seed, omega)
+    rng = np.random.default_rng(seed + 100)
+    Phi0 = Phi0 + phi_noise * rng.standard_normal(Phi0.shape)
+    eta0 = eta0 + 1e-7 * rng.standard_normal(eta0.shape)
+    delta0 = delta0 + 1e-7 * rng.standard_normal(delta0.shape)
+    return eta0, delta0, Phi0, U0, V0
+
+
+@pytest.mark.parametrize("scheme", ["si", "si_raw", "modeuler_raw"])
+def test_non_band_limited_initial_fields(scheme, torch_gpu):
+    """Grid-noise initial fields, 10 steps: the residue of the initial fields outside the band
+    follows the filtered physical and Fourier levels as in the reference.  Bound: 1e-9 of each
+    field's largest magnitude.  Grid noise puts energy at the truncation, where the one
+    approximation of the kernels shows most: they carry Pw(invLeg(S)) as f2 * S, while the
+    reference's quadrature weights leave its basis orthogonal only to ~7e-14 (T42); the
+    modified-Euler path measures up to 3.3e-10 on the nonlinear products after 10 steps."""
+    from my_swampe_amd.model import run_model_scan
+
+    eta0, delta0, Phi0, U0, V0 = _rough_ic(42, 5, 3.2e-5)
+    kw = dict(BASE, tmax=12, dt=600.0, alpha=0.05, eta0_init=eta0, delta0_init=delta0, Phi0_init=Phi0,
+              U0_init=U0, V0_init=V0, semi_implicit=scheme.startswith("si"), raw_filter=scheme.endswith("raw"))
+    _print_case("rough-ic", scheme=scheme)
+    ref = orc.run_model_scan(**kw)
+    got = run_model_scan(**kw, diagnostics=True)
+    _compare_runs(got, ref, rel=1e-9)
+
+
+def test_export_right_after_initialization(torch_gpu):
+    """tmax = 2 (no step): the exported carry is the two-level initialization itself."""
+    from my_swampe_amd.model import run_model_scan
+
+    eta0, delta0, Phi0, U0, V0 = _rough_ic(42, 6, 3.2e-5)
+    kw = dict(BASE, tmax=2, eta0_init=eta0, delta0_init=delta0, Phi0_init=Phi0, U0_init=U0, V0_init=V0,
+              semi_implicit=True, raw_filter=True)
+    _print_case("export-at-init")
+    ref = orc.run_model_scan(**kw)
+    got = run_model_scan(**kw, diagnostics=False, return_history=False)
+    _compare_runs(got, ref, history=False)
+
+
+def test_si_raw_t106_vs_oracle(torch_gpu):
+    """T106 (the reference's underflowed basis normalization) with the semi-implicit scheme and RAW."""
+    from my_swampe_amd.model import run_model_scan
+
+    kw = dict(BASE, M=106, tmax=6, dt=600.0, semi_implicit=True, raw_filter=True, alpha=0.05)
+    _print_case("si-raw-t106")
+    ref = orc.run_model_scan(**kw, diagnostics=False, return_history=False)
+    got = run_model_scan(**kw, diagnostics=False, return_history=False)
+    _compare_runs(got, ref, history=False)
+
+
+def test_raw_williams_one_equals_classic_filter(torch_gpu):
+    """unit_tests/test_raw_filter.py:37-46 on the HIP path: williams_alpha = 1 gives the classic
+    Robert-Asselin trajectory bit for bit (modified Euler; the physical history)."""
+    from my_swampe_amd.model import run_model_scan
+
+    kw = dict(BASE, dt=600.0, tmax=14, diagnostics=False)
+    _print_case("raw-w1")
+    base = run_model_scan(**kw)
+    raw = run_model_scan(**kw, raw_filter=True, williams_alpha=1.0)
+    for f in FIELDS:
+        np.testing.assert_array_equal(_np(base["outs"][f]), _np(raw["outs"][f]), err_msg=f)
+
+
+def test_si_steady_advection_on_gpu(torch_gpu):
+    """unit_tests/test_semi_implicit.py:108-119: Williamson test 1 keeps its steady vorticity to
+    roundoff over 6 hours at dt = 600 s (FP64 transforms)."""
+    from my_swampe_amd.model import run_model_scan
+
+    _print_case("si-steady")
+    out = run_model_scan(M=42, dt=600.0, tmax=2 + 36, test=1, forcflag=False, semi_implicit=True,
+                         diagnostics=False, **TEST_PLANET)
+    eta = _np(out["outs"]["eta"])
+    drift = float(np.sqrt(np.mean((eta[-1] - eta[0]) ** 2))) / float(np.std(eta[0]))
+    assert drift < 1e-8, drift
+
+
+def _plan_run(precision, params, ics, nsteps, *, plain=False, graph=0, si=True, raw=True):
+    from my_swampe_amd import _capi
+    from my_swampe_amd.plan import Plan
+    import torch
+
+    plan = Plan(42, params, precision=precision, semi_implicit=si, raw_filter=raw)
+    if plain:
+        plan.set_option(_capi.OPT_GRID_KERNEL, 1)
+        plan.set_option(_capi.OPT_LEG_KERNEL, 1)
+    plan.init(*ics)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        if graph:
+            plan.run_graph(2, nsteps, graph_steps=graph, stream=s)
+        else:
+            plan.run(2, nsteps, stream=s)
+        ex = plan.export(fields=("eta_curr", "delta_curr", "Phi_curr", "Phi_prev", "U_curr", "Phim_prev",
+                                 "Rum_lag"), spec=True, stream=s)
+    s.synchronize()
+    out = {k: _np(v) for k, v in ex.items()}
+    plan.close()
+    return out
+
+
+def test_si_raw_graph_replay_matches_eager(torch_gpu):
+    """HIP-graph replay of semi-implicit + RAW steps equals eager stepping bit for bit (the lagged
+    remainder, residue coefficients and filter flags advance inside the graph)."""
+    from my_swampe_amd.model import param_row
+
+    p = param_row(dt=600.0, a=8.2e7, omega=3.2e-5, g=9.8, Phibar=3e5, taurad=36000.0, taudrag=21600.0,
+                  DPhieq=1e6, K6=1.24e33, K6Phi=None, alpha=0.05)
+    ics = _rough_ic(42, 7, 3.2e-5)
+    _print_case("si-raw-graph")
+    for prec in ("fp64", "fp32"):
+        a = _plan_run(prec, np.stack([p, p]), ics, 40)
+        b = _plan_run(prec, np.stack([p, p]), ics, 40, graph=8)
+        for k in a:
+            np.testing.assert_array_equal(a[k], b[k], err_msg=f"{prec} {k}")
+
+
+def test_si_raw_fp32_vs_oracle(torch_gpu):
+    """FP32 transforms (matrix-core kernels, then the plain FP32 kernels) with the FP64 update vs the
+    FP64 oracle: semi-implicit + RAW, production dt = 600 s, alpha = 0.05, 40 steps at T42 from
+    rest + noise with grid-point noise.  Bounds (relative L2): Phi and eta <= 1e-5, delta <= 1e-3
+    (the bounds of the modified-Euler FP32 path); MFMA vs plain FP32 <= 5e-6 on Phi."""
+    from my_swampe_amd.model import param_row
+
+    kw = dict(dt=600.0, a=8.2e7, omega=3.2e-5, g=9.8, Phibar=3e5, taurad=36000.0, taudrag=21600.0, DPhieq=1e6,
+              K6=1.24e33, K6Phi=None, alpha=0.05)
+    ics = _rough_ic(42, 8, 3.2e-5)
+    _print_case("si-raw-fp32")
+    ref = orc.run_model_scan(M=42, tmax=42, test=None, diagnostics=False, return_history=False,
+                             eta0_init=ics[0], delta0_init=ics[1], Phi0_init=ics[2], U0_init=ics[3], V0_init=ics[4],
+                             semi_implicit=True, raw_filter=True, **kw)["last_state"]
+    p = param_row(**kw)
+    mf = _plan_run("fp32", p[None, :], ics, 40)
+    pl = _plan_run("fp32", p[None, :], ics, 40, plain=True)
+    rel = {}
+    for f in ("eta_curr", "delta_curr", "Phi_curr"):
+        r = getattr(ref, f)
+        rel[f] = float(np.linalg.norm(mf[f][0] - r) / np.linalg.norm(r))
+        rel[f + "_plain"] = float(np.linalg.norm(pl[f][0] - r) / np.linalg.norm(r))
+    rel["mfma_vs_plain"] = float(np.linalg.norm(mf["Phi_curr"] - pl["Phi_curr"]) / np.linalg.norm(pl["Phi_curr"]))
+    print("relL2", rel)
+    for sfx in ("", "_plain"):
+        assert rel["Phi_curr" + sfx] <= 1e-5 and rel["eta_curr" + sfx] <= 1e-5, rel
+        assert rel["delta_curr" + sfx] <= 1e-3, rel
+    assert rel["mfma_vs_plain"] <= 5e-6, rel
+
+
+def test_si_raw_plans_reject_single_level_checkpoints(torch_gpu):
+    """Spectral checkpoints and the adjoint cover the modified-Euler scheme only: clear errors."""
+    from my_swampe_amd.model import param_row
+    from my_swampe_amd.plan import Plan
+
+    p = param_row(dt=600.0, a=8.2e7, omega=3.2e-5, g=9.8, Phibar=3e5, taurad=36000.0, taudrag=21600.0,
+                  DPhieq=1e6, K6=1.24e33, K6Phi=None, alpha=0.05)
+    plan = Plan(42, p[None, :], semi_implicit=True, raw_filter=True)
+    plan.init(*_rough_ic(42, 9, 3.2e-5))
+    with pytest.raises(ValueError, match="one level"):
+        plan.state_get()
+    with pytest.raises(ValueError, match="adjoint"):
+        plan.adjoint_begin()
+    plan.close()
+
+
+# Properties of unit_tests/test_semi_implicit.py and unit_tests/test_raw_filter.py on the HIP
+# FP64 path (the reference's own assertions and bounds).
+_SYNTH = dict(M=42, Phibar=3.0e5, omega=3.2e-5, a=8.2e7, taurad=36000.0, taudrag=21600.0, DPhieq=1.0e6, test=None)
+_WASP43B = dict(M=42, Phibar=4.0e6, omega=8.939689388812098e-05, a=71920952.0, taurad=36000.0, taudrag=21600.0,
+                DPhieq=3.5e6, g=49.66, test=None)
+
+
+def test_si_runs_finite_default_regime_and_with_raw(torch_gpu):
+    """unit_tests/test_semi_implicit.py:51-57 and :157-168."""
+    from my_swampe_amd.model import run_model_scan
+
+    _print_case("si-finite")
+    out = run_model_scan(**_SYNTH, dt=240.0, tmax=120, semi_implicit=True)
+    assert int(out["dead_first_idx"]) == -1
+    for key in FIELDS:
+        assert np.all(np.isfinite(_np(out["outs"][key][-1]))), key
+    out = run_model_scan(**_SYNTH, dt=600.0, tmax=60, semi_implicit=True, raw_filter=True, williams_alpha=0.53)
+    assert int(out["dead_first_idx"]) == -1
+    assert np.all(np.isfinite(_np(out["outs"]["Phi"][-1])))
+
+
+def test_si_stable_beyond_explicit_gravity_wave_limit(torch_gpu):
+    """unit_tests/test_semi_implicit.py:60-73: one day of the WASP-43b regime at dt = 1200 s."""
+    from my_swampe_amd.model import assert_finite_state, run_model_scan_final
+
+    _print_case("si-wasp43b")
+    out = run_model_scan_final(**_WASP43B, dt=1200.0, tmax=74, semi_implicit=True, diagnostics=False)
+    assert assert_finite_state(out["last_state"])
+
+
+def test_si_equilibrium_is_dt_insensitive(torch_gpu):
+    """unit_tests/test_semi_implicit.py:76-105: 3 days of the WASP-43b regime at dt = 240 s and
+    1200 s reach the same forced quasi-steady Phi (reference bound 1e-3 relative RMS)."""
+    from my_swampe_amd.model import run_model_scan
+
+    def terminal(dt):
+        steps = int(round(3.0 * 86400.0 / dt))
+        out = run_model_scan(**_WASP43B, dt=dt, tmax=2 + steps, semi_implicit=True, diagnostics=False,
+                             return_history=False)
+        return _np(out["last_state"].Phi_curr)
+
+    _print_case("si-dt-insensitive")
+    ref = terminal(240.0)
+    coarse = terminal(1200.0)
+    rel = float(np.sqrt(np.mean((coarse - ref) ** 2))) / float(np.std(ref))
+    print("rel", rel)
+    assert np.isfinite(rel) and rel < 1e-3
+
+
+def test_raw_williams_optimum_changes_trajectory(torch_gpu):
+    """unit_tests/test_raw_filter.py:49-56."""
+    from my_swampe_amd.model import run_model_scan
+
+    kw = dict(_SYNTH, dt=240.0, tmax=40)
+    _print_case("raw-changes")
+    base = run_model_scan(**kw)
+    raw = run_model_scan(**kw, raw_filter=True, williams_alpha=0.53)
+    d = np.max(np.abs(_np(base["outs"]["Phi"][-1]) - _np(raw["outs"]["Phi"][-1])))
+    assert np.isfinite(d) and d > 0.0
