@@ -251,8 +251,15 @@ int swx_adjoint_result(swx_plan *plan, double *lam_out, double *pgrad_out, void 
  *     2, 4, 8 at T63 and T106), timed on the plan's own buffers; setting it
  *     forces another available size.  Results do not depend on it. */
 #define SWX_OPT_LEG_MB 5
+/*   SWX_OPT_GRID_FOLD: which FP32 matrix-core grid kernel runs: 1 = the full
+ *     longitude DFTs, 2 = the half-period folded DFTs (x(i + I/2) from the
+ *     even/odd wavenumber sums: half the matrix-core work).  Set at plan
+ *     creation to the faster of the two, timed on the plan's own buffers;
+ *     setting it forces one.  Results agree to FP32 rounding. */
+#define SWX_OPT_GRID_FOLD 6
 int swx_plan_set_option(swx_plan *plan, int option, int value);
-/* Current value of an option (for SWX_OPT_LEG_MB: the size in use). */
+/* Current value of an option (for SWX_OPT_LEG_MB and SWX_OPT_GRID_FOLD: the
+ * choice in use). */
 int swx_plan_get_option(const swx_plan *plan, int option, int *value);
 
 /* Last error message of the calling thread. */

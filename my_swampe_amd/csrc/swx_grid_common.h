@@ -38,7 +38,12 @@ __device__ __forceinline__ void grid_point_step(const GridArgs<float>& a, const 
 // 16-row k' tiles go through LDS (sC, 2*8*16*16 floats) per round, then one
 // (m, row) pair per thread assembles (swx_physics.h:assemble_slots) and writes T.
 // U_m, V_m of the step are read from the Fourier rows G.
-template <int Mp, int NT2>
+//
+// Row order of the accumulator tiles: TE == 0, row r of the 16 TE-free tiles is
+// (m, re/im) = (r / 2, r % 2); TE > 0 (half-period folded forward DFT), tiles t < TE hold the
+// even wavenumbers (m = 2 (r / 2)) and tiles t >= TE the odd ones (m = 2 (r' / 2) + 1, r' the
+// row counted from tile TE).
+template <int Mp, int NT2, int TE = 0>
 __device__ __forceinline__ void grid_assemble(const GridArgs<float>& a, const v4f* acc, float* sC, int b, int j0,
                                               const double* cf, const CoefLayout& cl, const double* geo,
                                               const GeoLayout& gl) {
@@ -61,7 +66,8 @@ __device__ __forceinline__ void grid_assemble(const GridArgs<float>& a, const v4
     __syncthreads();
     {
       const int dt2 = tid >> 7, ml = (tid >> 4) & 7, nn = tid & 15;
-      const int m = 8 * (t0 + dt2) + ml;
+      const int tt = t0 + dt2;
+      const int m = TE == 0 ? 8 * tt + ml : (tt < TE ? 2 * (8 * tt + ml) : 2 * (8 * (tt - TE) + ml) + 1);
       if (t0 + dt2 < NT2 && m < Mp) {
         const int jj = j0 + nn;
         double fm[8][2];

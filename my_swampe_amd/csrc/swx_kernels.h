@@ -191,6 +191,18 @@ hipError_t launch_grid(const GridArgs<R>& a, hipStream_t s);
 hipError_t launch_grid_mfma(const GridArgs<float>& a, const float* TinvF, const float* TfwdF, hipStream_t s);
 int grid_mfma_kq1(int M);
 size_t grid_mfma_lds_bytes(int M);
+// Half-period folded MFMA grid kernel (same eligibility; half the matrix-core work).  k rows
+// (m, re/im) in folded order: slot s = k / 2 is m = 2 s for s < KE4 / 2, else
+// m = 2 (s - KE4 / 2) + 1; output tile t < TE holds even m (row r: m = 2 (r / 2)), t >= TE odd
+// m (row r counted from tile TE: m = 2 (r / 2) + 1); invalid where m > M.  Tables:
+//   inv[it][lane][ks] = Tinv(i = 16 it + ln, k(4 ks + lg)),  it < H / 16, ks < KQ1
+//   fwd[t][g][lane][q] = Tfwd(k(16 t + ln), i = 16 g + 4 q + lg),  g < H / 16, q < 4
+struct FoldDims {
+  int H, K1, KE4, KQ1, TE, NT2;
+};
+bool grid_mfma_fold_dims(int M, FoldDims* d);
+size_t grid_mfma_fold_lds_bytes(int M);
+hipError_t launch_grid_mfma_fold(const GridArgs<float>& a, const float* TinvF, const float* TfwdF, hipStream_t s);
 // Split-BF16x3 grid kernel (same eligibility as the MFMA grid kernel); tables
 // of bf16 hi / lo pairs in its fragment order (swx_bf16x3.hip).
 hipError_t launch_grid_bf16x3(const GridArgs<float>& a, const void* TinvB, const void* TfwdB, hipStream_t s);

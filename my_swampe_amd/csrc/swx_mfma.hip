@@ -170,6 +170,191 @@ __device__ __forceinline__ void grid_mfma_body(const GridArgs<float>& a, const f
   grid_assemble<Mp, NT2>(a, acc, sC, b, j0, cf, cl, geo, gl);
 }
 
+// ----------------------------------------------------------------------------
+// Half-period folded variant.  With H = I/2, e^{2 pi i m (i + H) / I} = (-1)^m e^{2 pi i m i / I},
+// so both GEMMs need only the longitudes i < H:
+//   inverse  x(i) = E(i) + O(i),  x(i + H) = E(i) - O(i),  E / O = the sums over even / odd m;
+//   forward  Y[m] = sum_{i<H} F[m][i] (y(i) + (-1)^m y(i + H)).
+// The k rows (m, re/im) are reordered even-m block first (KE4 rows), then the odd-m block, for
+// both GEMMs; each lane runs the physics of the point pair (i, i + H) and stores
+// s = y(i) + y(i + H) and d = y(i) - y(i + H), which the even / odd output tiles consume.
+// Half the matrix-core work of grid_mfma_body.  Tables (fragment order as there):
+//   inv[it][lane][ks] = Tinv(i = 16 it + ln, k'(4 ks + lg)),  ks < KQ1
+//   fwd[t][g][lane][q] = Tfwd(k'(16 t + ln), i = 16 g + 4 q + lg),  g < H / 16
+// with k' the folded order (grid_mfma_fold_row).
+template <int MT>
+struct MfmaFoldShape {
+  static constexpr int Mp = MT + 1;
+  static constexpr int I = MfmaShape<MT>::I;
+  static constexpr int H = I / 2;
+  static constexpr int NE = (Mp + 1) / 2, NO = Mp / 2;    // even / odd wavenumbers
+  static constexpr int KE4 = (2 * NE + 3) / 4 * 4, KO4 = (2 * NO + 3) / 4 * 4;
+  static constexpr int K1 = KE4 + KO4;                    // inverse inner dimension (folded order)
+  static constexpr int KQ1 = (K1 / 4 + 3) / 4 * 4;
+  static constexpr int TE = (2 * NE + 15) / 16, TO = (2 * NO + 15) / 16;
+  static constexpr int NT2 = TE + TO;                     // forward output tiles
+  static constexpr int ICH = 64;                          // half-period longitudes per chunk
+  static constexpr int NCH = (H + ICH - 1) / ICH;
+  static constexpr int ROWS = 16;
+  static constexpr int XT_FLOATS = 5 * K1 * 16;
+  static constexpr int SD_FLOATS = 2 * 8 * ICH * 16;
+  // One chunk (T42): X^T is dead once GEMM 1 is done, so s / d reuse its space.
+  static constexpr int LDS_FLOATS = NCH == 1 ? (XT_FLOATS > SD_FLOATS ? XT_FLOATS : SD_FLOATS)
+                                             : XT_FLOATS + SD_FLOATS;
+};
+
+template <int MT>
+__device__ __forceinline__ void grid_mfma_fold_body(const GridArgs<float>& a, const float* __restrict__ TinvF,
+                                                    const float* __restrict__ TfwdF, const int blk, float* lds) {
+  using S = MfmaFoldShape<MT>;
+  constexpr int Mp = S::Mp, I = S::I, H = S::H, K1 = S::K1, KE4 = S::KE4, NT2 = S::NT2, TE = S::TE, ICH = S::ICH;
+  constexpr int KQ1 = S::KQ1;
+  float* sXT = lds;                                                  // [5][K1][16]
+  float* sS = lds + (S::NCH == 1 ? 0 : S::XT_FLOATS);                // [8][ICH][16] s
+  float* sD = sS + 8 * ICH * 16;                                     // [8][ICH][16] d
+  float* sC = sS;                                                    // [2][8][16][16] after the last chunk
+
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int ln = lane & 15, lg = lane >> 4;
+  const int J = a.J, N = a.N;
+  const int row0 = blk * S::ROWS;
+  const int b = row0 / J, j0 = row0 % J;
+  const double* cf = a.coef + (long long)b * a.ncoef;
+  const CoefLayout cl = CoefLayout::make(N, J);
+  const GeoLayout gl = GeoLayout::make(J, I);
+  const double* geo = a.geo;
+  const long long plane = (long long)J * I;
+  double* phys = a.phys + (long long)b * 8 * plane;
+  const double alpha = cf[C_ALPHA];
+
+  // Phase 0: Fourier rows -> X^T in the folded k order.  Slot s < K1 / 2 is the row pair
+  // (2s, 2s + 1) = (re, im) of m = 2s (s < KE4 / 2) or m = 2 (s - KE4 / 2) + 1.
+  {
+    constexpr int NE = 5 * (K1 / 2) * 8;
+    constexpr int NQ = (NE + 255) / 256;
+    v4f gv[NQ];
+#pragma unroll
+    for (int u = 0; u < NQ; ++u) {
+      const int e = tid + 256 * u;
+      const int jq = e & 7, fs = e >> 3;
+      const int f = fs / (K1 / 2) < 5 ? fs / (K1 / 2) : 4, sl = fs % (K1 / 2);
+      const int m = sl < KE4 / 2 ? 2 * sl : 2 * (sl - KE4 / 2) + 1;
+      const v4f v = *reinterpret_cast<const v4f*>(a.G + fidx(b, f, m < Mp ? m : Mp - 1, j0 + 2 * jq, Mp, J));
+      gv[u] = (e < NE && m < Mp) ? v : v4f{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int u = 0; u < NQ; ++u) {
+      const int e = tid + 256 * u;
+      if (e < NE) {
+        const int jq = e & 7, fs = e >> 3;
+        const int f = fs / (K1 / 2), sl = fs % (K1 / 2);
+        float* xr = sXT + (f * K1 + 2 * sl) * 16 + 2 * jq;
+        *reinterpret_cast<float2*>(xr) = make_float2(gv[u][0], gv[u][2]);
+        *reinterpret_cast<float2*>(xr + 16) = make_float2(gv[u][1], gv[u][3]);
+      }
+    }
+  }
+  __syncthreads();
+
+  v4f acc[2 * NT2];
+#pragma unroll
+  for (int p = 0; p < 2 * NT2; ++p) acc[p] = v4f{0.f, 0.f, 0.f, 0.f};
+
+  const int n = ln;
+  const int j = j0 + n;
+  const double inv_denE = 1.0 / geo[gl.denomE + j];
+  const v4f* Tf = reinterpret_cast<const v4f*>(TfwdF) + lane;
+  for (int ic = 0; ic < H; ic += ICH) {
+    const int ntc = (H - ic) / 16 < ICH / 16 ? (H - ic) / 16 : ICH / 16;  // 16-longitude tiles in this chunk
+    // ---- Phase 1: even and odd inverse sums of this wave's tile, all five fields.
+    v4f cE[5], cO[5];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) cE[f] = cO[f] = v4f{0.f, 0.f, 0.f, 0.f};
+    const bool active = wave < ntc;
+    const int ib = ic + 16 * wave;
+    if (active) {
+      v4f afr[KQ1 / 4];
+      const v4f* Ta = reinterpret_cast<const v4f*>(TinvF + ((long long)(ib / 16) * 64 + lane) * KQ1);
+#pragma unroll
+      for (int q4 = 0; q4 < KQ1 / 4; ++q4) afr[q4] = Ta[q4];
+#pragma unroll
+      for (int ks = 0; ks < K1 / 4; ++ks) {
+        const float* xb = sXT + (4 * ks + lg) * 16 + ln;
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+          if (ks < KE4 / 4)
+            cE[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(afr[ks / 4][ks % 4], xb[f * K1 * 16], cE[f], 0, 0, 0);
+          else
+            cO[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(afr[ks / 4][ks % 4], xb[f * K1 * 16], cO[f], 0, 0, 0);
+        }
+      }
+    }
+    // First k-group of this chunk's forward fragments, in flight during the physics.
+    v4f an[NT2];
+#pragma unroll
+    for (int t = 0; t < NT2; ++t) an[t] = Tf[((long long)t * (H / 16) + ic / 16) * 64];
+    if constexpr (S::NCH == 1) __syncthreads();  // s / d overwrite X^T
+    // ---- Physics on the lane's 4 point pairs (row n, i = ib + 4 lg + q and i + H).
+    if (active) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int il = 16 * wave + 4 * lg + q;
+        const int i = ic + il;
+        double pa[8], pb[8];
+        grid_point_step(a, cf, geo, gl, phys, plane, alpha, b, j, i, (double)(cE[0][q] + cO[0][q]),
+                        (double)(cE[1][q] + cO[1][q]), (double)(cE[2][q] + cO[2][q]),
+                        (double)(cE[3][q] + cO[3][q]), (double)(cE[4][q] + cO[4][q]), inv_denE, pa);
+        grid_point_step(a, cf, geo, gl, phys, plane, alpha, b, j, i + H, (double)(cE[0][q] - cO[0][q]),
+                        (double)(cE[1][q] - cO[1][q]), (double)(cE[2][q] - cO[2][q]),
+                        (double)(cE[3][q] - cO[3][q]), (double)(cE[4][q] - cO[4][q]), inv_denE, pb);
+#pragma unroll
+        for (int f = 0; f < 8; ++f) {
+          sS[(f * ICH + il) * 16 + n] = (float)(pa[f] + pb[f]);
+          sD[(f * ICH + il) * 16 + n] = (float)(pa[f] - pb[f]);
+        }
+      }
+    }
+    __syncthreads();
+    // ---- Phase 2: forward sums over this chunk's tiles; wave w owns fields 2w, 2w + 1; the
+    // even-m output tiles read s, the odd-m ones d.
+#pragma unroll 1
+    for (int gg = 0; gg < ntc; ++gg) {
+      v4f ac[NT2];
+#pragma unroll
+      for (int t = 0; t < NT2; ++t) ac[t] = an[t];
+      if (gg + 1 < ntc) {
+#pragma unroll
+        for (int t = 0; t < NT2; ++t) an[t] = Tf[((long long)t * (H / 16) + ic / 16 + gg + 1) * 64];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int il = 16 * gg + 4 * q + lg;
+        const float s0 = sS[((2 * wave) * ICH + il) * 16 + ln];
+        const float s1 = sS[((2 * wave + 1) * ICH + il) * 16 + ln];
+        const float d0 = sD[((2 * wave) * ICH + il) * 16 + ln];
+        const float d1 = sD[((2 * wave + 1) * ICH + il) * 16 + ln];
+#pragma unroll
+        for (int t = 0; t < NT2; ++t) {
+          acc[2 * t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[t][q], t < TE ? s0 : d0, acc[2 * t], 0, 0, 0);
+          acc[2 * t + 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[t][q], t < TE ? s1 : d1, acc[2 * t + 1], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- Phase 3: assemble the Legendre inputs (swx_grid_common.h), folded row order.
+  grid_assemble<Mp, NT2, TE>(a, acc, sC, b, j0, cf, cl, geo, gl);
+}
+
+template <int MT>
+__global__ void __launch_bounds__(256) k_grid_mfma_fold(GridArgs<float> a, const float* __restrict__ TinvF,
+                                                        const float* __restrict__ TfwdF) {
+  extern __shared__ __align__(16) float lds[];
+  grid_mfma_fold_body<MT>(a, TinvF, TfwdF, blockIdx.x, lds);
+}
+
 template <int MT>
 __global__ void __launch_bounds__(256) k_grid_mfma(GridArgs<float> a, const float* __restrict__ TinvF,
                                                    const float* __restrict__ TfwdF) {
@@ -197,6 +382,54 @@ static hipError_t launch_mfma_t(const GridArgs<float>& a, const float* TinvF, co
   dim3 grid((unsigned)((long long)a.B * a.J / S::ROWS));
   hipLaunchKernelGGL(k_grid_mfma<MT>, grid, dim3(256), smem, s, a, TinvF, TfwdF);
   return hipGetLastError();
+}
+
+template <int MT>
+static hipError_t launch_mfma_fold_t(const GridArgs<float>& a, const float* TinvF, const float* TfwdF,
+                                     hipStream_t s) {
+  using S = MfmaFoldShape<MT>;
+  const size_t smem = sizeof(float) * S::LDS_FLOATS;
+  dim3 grid((unsigned)((long long)a.B * a.J / S::ROWS));
+  hipLaunchKernelGGL(k_grid_mfma_fold<MT>, grid, dim3(256), smem, s, a, TinvF, TfwdF);
+  return hipGetLastError();
+}
+
+size_t grid_mfma_fold_lds_bytes(int M) {
+  switch (M) {
+    case 42: return sizeof(float) * MfmaFoldShape<42>::LDS_FLOATS;
+    case 63: return sizeof(float) * MfmaFoldShape<63>::LDS_FLOATS;
+    case 106: return sizeof(float) * MfmaFoldShape<106>::LDS_FLOATS;
+    default: return 0;
+  }
+}
+
+template <int MT>
+static void fold_dims_t(FoldDims* d) {
+  using S = MfmaFoldShape<MT>;
+  d->H = S::H;
+  d->K1 = S::K1;
+  d->KE4 = S::KE4;
+  d->KQ1 = S::KQ1;
+  d->TE = S::TE;
+  d->NT2 = S::NT2;
+}
+
+bool grid_mfma_fold_dims(int M, FoldDims* d) {
+  switch (M) {
+    case 42: fold_dims_t<42>(d); return true;
+    case 63: fold_dims_t<63>(d); return true;
+    case 106: fold_dims_t<106>(d); return true;
+    default: return false;
+  }
+}
+
+hipError_t launch_grid_mfma_fold(const GridArgs<float>& a, const float* TinvF, const float* TfwdF, hipStream_t s) {
+  switch (a.M) {
+    case 42: return launch_mfma_fold_t<42>(a, TinvF, TfwdF, s);
+    case 63: return launch_mfma_fold_t<63>(a, TinvF, TfwdF, s);
+    case 106: return launch_mfma_fold_t<106>(a, TinvF, TfwdF, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 size_t grid_mfma_lds_bytes(int M) {

@@ -86,6 +86,11 @@ struct swx_plan {
   void* tw = nullptr;
   float* TinvF = nullptr;  // MFMA grid kernel DFT tables, fragment order (FP32 plans)
   float* TfwdF = nullptr;
+  float* TinvH = nullptr;  // tables of the half-period folded MFMA grid kernel
+  float* TfwdH = nullptr;
+  bool has_fold = false;   // folded kernel available (its LDS fits)
+  int grid_fold = 1;       // SWX_OPT_GRID_FOLD in use: 1 = full DFTs, 2 = folded
+  float grid_fold_ms[2] = {0, 0};  // the two variants' measured launch times
   bool use_mfma = false;       // MFMA grid kernel available for this plan
   bool use_bf16x3 = false;     // split-BF16x3 grid kernel (precision SWX_PREC_BF16X3)
   bool use_adj_mfma = false;   // MFMA grid adjoint kernel (FP32 transforms, where its LDS fits)
@@ -328,6 +333,7 @@ hipError_t launch_grid_any(swx_plan* p, const GridArgs<R>& g, hipStream_t s) {
   if constexpr (sizeof(R) == 4) {
     if (grid_uses_mfma(p, g)) {
       if (p->use_bf16x3) return launch_grid_bf16x3(g, p->TinvB, p->TfwdB, s);
+      if (p->has_fold && p->grid_fold == 2) return launch_grid_mfma_fold(g, p->TinvH, p->TfwdH, s);
       return launch_grid_mfma(g, p->TinvF, p->TfwdF, s);
     }
   }
@@ -637,6 +643,44 @@ int autotune_leg(swx_plan* p) {
   return SWX_OK;
 }
 
+// Grid kernel variant tuner: time the full-DFT and the half-period folded MFMA grid kernels
+// (2 warm-up + 5 timed launches each, null stream) on the plan's own buffers (zeroed Fourier
+// rows; no physical carry, filters or lag) and keep the faster.
+int autotune_grid(swx_plan* p) {
+  const size_t gbytes = (size_t)p->d.n_members * kNG * p->geo.J * (p->geo.M + 1) * 2 * p->rsz;
+  SWX_HIP(hipMemset(p->G, 0, gbytes));
+  GridArgs<float> g = grid_args<float>(p, GRID_STEP);
+  g.phys_on = 0;
+  g.do_ra = 0;
+  g.si = 0;
+  g.raw_filter = 0;
+  hipEvent_t e0, e1;
+  SWX_HIP(hipEventCreate(&e0));
+  SWX_HIP(hipEventCreate(&e1));
+  float best = 0.f;
+  for (int v = 1; v <= 2; ++v) {
+    auto launch = [&]() {
+      return v == 2 ? launch_grid_mfma_fold(g, p->TinvH, p->TfwdH, nullptr)
+                    : launch_grid_mfma(g, p->TinvF, p->TfwdF, nullptr);
+    };
+    for (int w = 0; w < 2; ++w) SWX_HIP(launch());
+    SWX_HIP(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < 5; ++r) SWX_HIP(launch());
+    SWX_HIP(hipEventRecord(e1, nullptr));
+    SWX_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    SWX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    p->grid_fold_ms[v - 1] = ms / 5;
+    if (best == 0.f || ms / 5 < best) {
+      best = ms / 5;
+      p->grid_fold = v;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return SWX_OK;
+}
+
 // Two FP32 plans of different truncations stepped together: each step is one Legendre launch
 // and one grid launch covering both plans' workgroups (MFMA pair kernels); a step where either
 // plan needs a plain kernel falls back to one launch per plan.
@@ -915,6 +959,43 @@ int swx_plan_create(const swx_plan_desc* desc, const double* params, swx_plan** 
         swx_plan_destroy(p);
         return rc;
       }
+      FoldDims fd{};
+      if (grid_mfma_fold_dims(M, &fd) && grid_mfma_fold_lds_bytes(M) <= (size_t)maxlds) {
+        // Folded order of the k rows and of the output tile rows (swx_kernels.h FoldDims).
+        auto kfold = [&](int kp) -> int {  // inverse row -> 2 m + c, or -1
+          if (kp >= fd.K1) return -1;
+          const int sl = kp / 2, c = kp & 1;
+          const int m = sl < fd.KE4 / 2 ? 2 * sl : 2 * (sl - fd.KE4 / 2) + 1;
+          return m < Mp ? 2 * m + c : -1;
+        };
+        auto ofold = [&](int t, int r) -> int {  // output tile row -> 2 m + c, or -1
+          const int rr = 16 * (t < fd.TE ? t : t - fd.TE) + r;
+          const int m = t < fd.TE ? 2 * (rr / 2) : 2 * (rr / 2) + 1;
+          return m < Mp ? 2 * m + (rr & 1) : -1;
+        };
+        std::vector<double> hi((size_t)(fd.H / 16) * 64 * fd.KQ1, 0.0), hf((size_t)fd.NT2 * (fd.H / 16) * 64 * 4, 0.0);
+        for (int lane = 0; lane < 64; ++lane) {
+          const int ln = lane & 15, lg = lane >> 4;
+          for (int it = 0; it < fd.H / 16; ++it)
+            for (int ks = 0; ks < fd.KQ1; ++ks) {
+              const int k = kfold(4 * ks + lg);
+              if (k >= 0) hi[((size_t)it * 64 + lane) * fd.KQ1 + ks] = tinv(16 * it + ln, k);
+            }
+          for (int t = 0; t < fd.NT2; ++t) {
+            const int k = ofold(t, ln);
+            if (k < 0) continue;
+            for (int g = 0; g < fd.H / 16; ++g)
+              for (int q = 0; q < 4; ++q)
+                hf[(((size_t)t * (fd.H / 16) + g) * 64 + lane) * 4 + q] = tfwd(k, 16 * g + 4 * q + lg);
+          }
+        }
+        if ((rc = dalloc(p, &p->TinvH, hi.size() * 4)) || (rc = dalloc(p, &p->TfwdH, hf.size() * 4)) ||
+            (rc = upload_real(p->TinvH, hi, 4)) || (rc = upload_real(p->TfwdH, hf, 4))) {
+          swx_plan_destroy(p);
+          return rc;
+        }
+        p->has_fold = true;
+      }
       if (desc->precision == SWX_PREC_BF16X3 && grid_bf16x3_lds_bytes(M) <= (size_t)maxlds) {
         // Fragment order of the split-BF16x3 kernel: per (tile, k-step, lane) 8 hi then 8 lo
         // values, k (or i) = 32 ks + 8 (lane / 16) + e.
@@ -960,6 +1041,10 @@ int swx_plan_create(const swx_plan_desc* desc, const double* params, swx_plan** 
     swx_plan_destroy(p);
     return fail(SWX_ERR_HIP, "hipMemset failed");
   }
+  if (p->use_mfma && p->has_fold && (rc = autotune_grid(p)) != SWX_OK) {
+    swx_plan_destroy(p);
+    return rc;
+  }
   if (p->use_leg_mfma && (rc = autotune_leg(p)) != SWX_OK) {
     swx_plan_destroy(p);
     return rc;
@@ -986,7 +1071,7 @@ void swx_plan_destroy(swx_plan* p) {
   if (!p) return;
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   void* bufs[] = {p->Lam, p->Tbar, p->Gbar, p->pgrad, p->Pf, p->Hf, p->Pi, p->Hi, p->fragP[0], p->fragP[1],
-                  p->fragP[2], p->fragP[3], p->tw, p->TinvF, p->TfwdF, p->TinvB, p->TfwdB, p->geo_d, p->f2, p->coef, p->S, p->T, p->G, p->Gprev,
+                  p->fragP[2], p->fragP[3], p->tw, p->TinvF, p->TfwdF, p->TinvH, p->TfwdH, p->TinvB, p->TfwdB, p->geo_d, p->f2, p->coef, p->S, p->T, p->G, p->Gprev,
                   p->raw, p->lag, p->g0uv, p->phys, p->uvic, p->part_uv, p->dead, p->R0f, p->R0p, p->resid_c, p->lagx};
   for (void* q : bufs)
     if (q) (void)hipFree(q);
@@ -1186,6 +1271,12 @@ int swx_plan_set_option(swx_plan* p, int option, int value) {
       if (value < -1) return fail(SWX_ERR_ARG, "SWX_OPT_PHYS_STEPS must be >= -1");
       p->opt_phys_steps = value;
       break;
+    case SWX_OPT_GRID_FOLD:
+      if (value != 1 && value != 2) return fail(SWX_ERR_ARG, "SWX_OPT_GRID_FOLD must be 1 (full DFTs) or 2 (folded)");
+      if (value == 2 && !p->has_fold)
+        return fail(SWX_ERR_ARG, "SWX_OPT_GRID_FOLD: the folded grid kernel is not available for this plan");
+      p->grid_fold = value;
+      break;
     default:
       return fail(SWX_ERR_ARG, "unknown option " + std::to_string(option));
   }
@@ -1279,6 +1370,7 @@ int swx_plan_get_option(const swx_plan* p, int option, int* value) {
     case SWX_OPT_LEG_KERNEL: *value = p->opt_plain_leg ? 1 : 0; break;
     case SWX_OPT_PHYS_STEPS: *value = p->opt_phys_steps; break;
     case SWX_OPT_LEG_MB: *value = p->leg_mb; break;
+    case SWX_OPT_GRID_FOLD: *value = p->grid_fold; break;
     default: return fail(SWX_ERR_ARG, "unknown option " + std::to_string(option));
   }
   return SWX_OK;

@@ -266,9 +266,9 @@ def test_fp32_transforms_close_to_fp64(torch_gpu):
 
 @pytest.mark.parametrize("M", [42, 63, 106])
 def test_mfma_grid_kernel_matches_plain_fp32(M, torch_gpu):
-    """The MFMA grid and Legendre kernels (FP32) vs the plain FP32 kernels and the FP64 oracle, 12 forced steps
-    from a balanced zonal flow (Williamson test 2 fields, U ~ 40 m/s).  Relative L2 <= 5e-6 for
-    Phi, eta and U."""
+    """The MFMA grid (full and half-period folded DFTs) and Legendre kernels (FP32) vs the plain FP32
+    kernels and the FP64 oracle, 12 forced steps from a balanced zonal flow (Williamson test 2
+    fields, U ~ 40 m/s).  Relative L2 <= 5e-6 for Phi, eta and U."""
     from my_swampe_amd import _capi
     from my_swampe_amd.model import param_row
     from my_swampe_amd.plan import Plan
@@ -282,10 +282,11 @@ def test_mfma_grid_kernel_matches_plain_fp32(M, torch_gpu):
     ics = orc.analytic_ic(s, 2, 0.3)
     _print_case("mfma-vs-plain", M=M)
     res = {}
-    for kind in ("plain", "mfma"):
+    for kind in ("plain", "mfma", "mfma_fold"):
         plan = Plan(M, np.stack([p] * 3), precision="fp32")  # 3 members: a partial member group
         plan.set_option(_capi.OPT_GRID_KERNEL, 1 if kind == "plain" else 0)
         plan.set_option(_capi.OPT_LEG_KERNEL, 1 if kind == "plain" else 0)
+        plan.set_option(_capi.OPT_GRID_FOLD, 2 if kind == "mfma_fold" else 1)
         plan.init(*ics)
         plan.run(2, 12)
         res[kind] = {k: _np(v[0]) for k, v in plan.export(fields=("Phi_curr", "U_curr", "eta_curr")).items()
@@ -296,7 +297,7 @@ def test_mfma_grid_kernel_matches_plain_fp32(M, torch_gpu):
                              U0_init=ics[3], V0_init=ics[4], diagnostics=False, return_history=False)["last_state"]
     for f in ("Phi_curr", "U_curr", "eta_curr"):
         r = getattr(ref, f)
-        for kind in ("plain", "mfma"):
+        for kind in ("plain", "mfma", "mfma_fold"):
             rel = np.linalg.norm(res[kind][f] - r) / np.linalg.norm(r)
             print(M, kind, f, "relL2 vs oracle", rel)
             assert rel < 5e-6, (M, kind, f, rel)
