@@ -84,8 +84,10 @@ def run_batch(M, B, precision, steps, warmup, graph_steps=32, full_phys=False, s
         ex = plan.export(fields=("Phi_curr",), stream=stream)
         stream.synchronize()
     finite = bool(torch.isfinite(ex["Phi_curr"]).all())
+    fold = plan.get_option(_capi.OPT_GRID_FOLD) if precision == "fp32" else 1
+    leg_mb = plan.get_option(_capi.OPT_LEG_MB) if precision != "fp64" else 0
     plan.close()
-    return dict(wall=wall, ms_leg=ms_leg, ms_grid=ms_grid, finite=finite)
+    return dict(wall=wall, ms_leg=ms_leg, ms_grid=ms_grid, finite=finite, fold=fold, leg_mb=leg_mb)
 
 
 def spectral_ics(M: int, B: int, omega: float = 3.2e-5):
@@ -287,6 +289,8 @@ def main():
     leg_tflops = leg_f * B / (main_run["ms_leg"] * 1e-3) / 1e12
     peak = PEAK[args.precision]
     grid_kernel = {"fp32": f"k_grid_mfma<{M}>", "bf16x3": f"k_grid_bf16x3<{M}>", "fp64": "k_grid<double>"}[args.precision]
+    if args.precision == "fp32" and main_run["fold"] == 2:
+        grid_kernel = f"k_grid_mfma_fold<{M}>"
     out = {
         "metric": METRIC,
         "value": value,
@@ -306,6 +310,10 @@ def main():
                      "frac": grid_tflops / peak,
                      "traffic": measured_traffic(grid_kernel, B, args.precision),
                      "kernel": f"{grid_kernel} (inverse DFT + grid physics + forward DFT)",
+                     "note": ("achieved = the dense-DFT algorithmic flops (4 J 13 I (M+1) per member, no "
+                              "symmetry folding) / kernel time; the folded kernel executes half of them"
+                              if grid_kernel.startswith("k_grid_mfma_fold") else "achieved = algorithmic DFT flops / kernel time"),
+                     "leg_member_group": main_run["leg_mb"],
                      "algorithmic_flops_per_launch": dft_f * B,
                      "kernel_ms": main_run["ms_grid"], "leg_kernel_ms": main_run["ms_leg"],
                      "leg_tflops": leg_tflops},

@@ -210,8 +210,10 @@ size_t grid_bf16x3_lds_bytes(int M);
 int grid_bf16x3_ks1(int M);
 // Two FP32 plans of different truncations (M0 < M1: 42/63, 42/106, 63/106) in one launch.
 bool mfma_pair_supported(int M0, int M1);
-hipError_t launch_grid_mfma_pair(const GridArgs<float>& a0, const float* ti0, const float* tf0,
-                                 const GridArgs<float>& a1, const float* ti1, const float* tf1, hipStream_t s);
+// fold0 / fold1: that plan's half-period folded variant (its tables passed as ti / tf).
+hipError_t launch_grid_mfma_pair(const GridArgs<float>& a0, const float* ti0, const float* tf0, bool fold0,
+                                 const GridArgs<float>& a1, const float* ti1, const float* tf1, bool fold1,
+                                 hipStream_t s);
 hipError_t launch_leg_mfma_pair(const LegArgs<float>& a0, const LegArgs<float>& a1, hipStream_t s);
 // MFMA Legendre kernel (FP32, LEG_STEP without diagnostics).
 // mb = members per workgroup, one of leg_mfma_candidates(M) (the plan picks the fastest).

@@ -362,17 +362,25 @@ __global__ void __launch_bounds__(256) k_grid_mfma(GridArgs<float> a, const floa
   grid_mfma_body<MT>(a, TinvF, TfwdF, blockIdx.x, lds);
 }
 
-// Two plans of different truncations in one launch: blocks [0, nblk0) are plan 0's.
-template <int MA, int MB>
+// Two plans of different truncations in one launch: blocks [0, nblk0) are plan 0's; each plan
+// runs its own variant (FA / FB: half-period folded), so it reaches the state it reaches alone.
+template <int MA, int MB, bool FA, bool FB>
 __global__ void __launch_bounds__(256) k_grid_mfma_pair(GridArgs<float> a0, const float* __restrict__ ti0,
                                                         const float* __restrict__ tf0, GridArgs<float> a1,
                                                         const float* __restrict__ ti1, const float* __restrict__ tf1,
                                                         int nblk0) {
   extern __shared__ __align__(16) float lds[];
-  if ((int)blockIdx.x < nblk0)
-    grid_mfma_body<MA>(a0, ti0, tf0, blockIdx.x, lds);
-  else
-    grid_mfma_body<MB>(a1, ti1, tf1, blockIdx.x - nblk0, lds);
+  if ((int)blockIdx.x < nblk0) {
+    if constexpr (FA)
+      grid_mfma_fold_body<MA>(a0, ti0, tf0, blockIdx.x, lds);
+    else
+      grid_mfma_body<MA>(a0, ti0, tf0, blockIdx.x, lds);
+  } else {
+    if constexpr (FB)
+      grid_mfma_fold_body<MB>(a1, ti1, tf1, blockIdx.x - nblk0, lds);
+    else
+      grid_mfma_body<MB>(a1, ti1, tf1, blockIdx.x - nblk0, lds);
+  }
 }
 
 template <int MT>
@@ -769,14 +777,27 @@ hipError_t launch_leg_mfma(const LegArgs<float>& a, int mb, hipStream_t s) {
   }
 }
 
-template <int MA, int MB>
-static hipError_t launch_grid_pair_t(const GridArgs<float>& a0, const float* ti0, const float* tf0,
-                                     const GridArgs<float>& a1, const float* ti1, const float* tf1, hipStream_t s) {
-  const size_t smem = sizeof(float) * (size_t)cmax(MfmaShape<MA>::LDS_FLOATS, MfmaShape<MB>::LDS_FLOATS);
+template <int MA, int MB, bool FA, bool FB>
+static hipError_t launch_grid_pair_ft(const GridArgs<float>& a0, const float* ti0, const float* tf0,
+                                      const GridArgs<float>& a1, const float* ti1, const float* tf1, hipStream_t s) {
+  constexpr int LA = FA ? MfmaFoldShape<MA>::LDS_FLOATS : MfmaShape<MA>::LDS_FLOATS;
+  constexpr int LB = FB ? MfmaFoldShape<MB>::LDS_FLOATS : MfmaShape<MB>::LDS_FLOATS;
+  const size_t smem = sizeof(float) * (size_t)cmax(LA, LB);
   const int n0 = (int)((long long)a0.B * a0.J / MfmaShape<MA>::ROWS);
   const int n1 = (int)((long long)a1.B * a1.J / MfmaShape<MB>::ROWS);
-  hipLaunchKernelGGL((k_grid_mfma_pair<MA, MB>), dim3(n0 + n1), dim3(256), smem, s, a0, ti0, tf0, a1, ti1, tf1, n0);
+  hipLaunchKernelGGL((k_grid_mfma_pair<MA, MB, FA, FB>), dim3(n0 + n1), dim3(256), smem, s, a0, ti0, tf0, a1, ti1,
+                     tf1, n0);
   return hipGetLastError();
+}
+
+template <int MA, int MB>
+static hipError_t launch_grid_pair_t(const GridArgs<float>& a0, const float* ti0, const float* tf0, bool f0,
+                                     const GridArgs<float>& a1, const float* ti1, const float* tf1, bool f1,
+                                     hipStream_t s) {
+  if (f0 && f1) return launch_grid_pair_ft<MA, MB, true, true>(a0, ti0, tf0, a1, ti1, tf1, s);
+  if (f0) return launch_grid_pair_ft<MA, MB, true, false>(a0, ti0, tf0, a1, ti1, tf1, s);
+  if (f1) return launch_grid_pair_ft<MA, MB, false, true>(a0, ti0, tf0, a1, ti1, tf1, s);
+  return launch_grid_pair_ft<MA, MB, false, false>(a0, ti0, tf0, a1, ti1, tf1, s);
 }
 
 template <int MA, int MBA, int MB, int MBB>
@@ -792,11 +813,12 @@ bool mfma_pair_supported(int M0, int M1) {
   return (M0 == 42 && M1 == 63) || (M0 == 42 && M1 == 106) || (M0 == 63 && M1 == 106);
 }
 
-hipError_t launch_grid_mfma_pair(const GridArgs<float>& a0, const float* ti0, const float* tf0,
-                                 const GridArgs<float>& a1, const float* ti1, const float* tf1, hipStream_t s) {
-  if (a0.M == 42 && a1.M == 63) return launch_grid_pair_t<42, 63>(a0, ti0, tf0, a1, ti1, tf1, s);
-  if (a0.M == 42 && a1.M == 106) return launch_grid_pair_t<42, 106>(a0, ti0, tf0, a1, ti1, tf1, s);
-  if (a0.M == 63 && a1.M == 106) return launch_grid_pair_t<63, 106>(a0, ti0, tf0, a1, ti1, tf1, s);
+hipError_t launch_grid_mfma_pair(const GridArgs<float>& a0, const float* ti0, const float* tf0, bool fold0,
+                                 const GridArgs<float>& a1, const float* ti1, const float* tf1, bool fold1,
+                                 hipStream_t s) {
+  if (a0.M == 42 && a1.M == 63) return launch_grid_pair_t<42, 63>(a0, ti0, tf0, fold0, a1, ti1, tf1, fold1, s);
+  if (a0.M == 42 && a1.M == 106) return launch_grid_pair_t<42, 106>(a0, ti0, tf0, fold0, a1, ti1, tf1, fold1, s);
+  if (a0.M == 63 && a1.M == 106) return launch_grid_pair_t<63, 106>(a0, ti0, tf0, fold0, a1, ti1, tf1, fold1, s);
   return hipErrorInvalidValue;
 }
 

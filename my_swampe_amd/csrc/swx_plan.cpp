@@ -686,6 +686,8 @@ int autotune_grid(swx_plan* p) {
 // plan needs a plain kernel falls back to one launch per plan.
 struct swx_ensemble {
   swx_plan* p[2] = {nullptr, nullptr};  // p[0]->geo.M < p[1]->geo.M
+  bool fuse_grid = true;                // one grid launch for both plans (timed at creation)
+  float grid_ms[2] = {0, 0};            // fused, separate
   hipGraphExec_t gexec = nullptr;
   int gexec_steps = 0;
   hipStream_t gexec_stream = nullptr;
@@ -717,8 +719,12 @@ int ens_steps(swx_ensemble* e, int t0, int nsteps, bool save_last, hipStream_t s
     } else {
       for (int q = 0; q < 2; ++q) SWX_HIP(launch_leg_any<float>(e->p[q], l[q], s));
     }
-    if (grid_uses_mfma(e->p[0], g[0]) && grid_uses_mfma(e->p[1], g[1])) {
-      SWX_HIP(launch_grid_mfma_pair(g[0], e->p[0]->TinvF, e->p[0]->TfwdF, g[1], e->p[1]->TinvF, e->p[1]->TfwdF, s));
+    if (e->fuse_grid && grid_uses_mfma(e->p[0], g[0]) && grid_uses_mfma(e->p[1], g[1])) {
+      const swx_plan* q0 = e->p[0];
+      const swx_plan* q1 = e->p[1];
+      const bool f0 = q0->has_fold && q0->grid_fold == 2, f1 = q1->has_fold && q1->grid_fold == 2;
+      SWX_HIP(launch_grid_mfma_pair(g[0], f0 ? q0->TinvH : q0->TinvF, f0 ? q0->TfwdH : q0->TfwdF, f0, g[1],
+                                    f1 ? q1->TinvH : q1->TinvF, f1 ? q1->TfwdH : q1->TfwdF, f1, s));
     } else {
       for (int q = 0; q < 2; ++q) SWX_HIP(launch_grid_any<float>(e->p[q], g[q], s));
     }
@@ -737,6 +743,68 @@ int check_params(const swx_plan_desc* d, const double* params) {
       return fail(SWX_ERR_ARG, "raw_filter: alpha * (1 - williams_alpha) must differ from 1");
   }
   return SWX_OK;
+}
+
+// Grid step of both plans as one fused launch or as each plan's own launch: the same kernel
+// bodies (identical results); the faster is kept.  The pair launch sizes its LDS for the
+// larger of the two bodies, which can halve the occupancy of the smaller one.  Timed at
+// ensemble creation (2 warm-up + 5 timed rounds, null stream) with the physical carry,
+// filters and lag off, so the launches write only the Legendre inputs T, which are saved
+// before and restored after.
+int tune_ensemble(swx_ensemble* e) {
+  GridArgs<float> g[2];
+  void* saved[2] = {nullptr, nullptr};
+  size_t tb[2];
+  int rc = SWX_OK;
+  for (int q = 0; q < 2 && rc == SWX_OK; ++q) {
+    swx_plan* p = e->p[q];
+    g[q] = grid_args<float>(p, GRID_STEP);
+    g[q].phys_on = 0;
+    g[q].do_ra = 0;
+    g[q].si = 0;
+    g[q].raw_filter = 0;
+    tb[q] = (size_t)p->d.n_members * (p->geo.M + 1) * kNT * p->geo.J * 2 * p->rsz;
+    if (hipMalloc(&saved[q], tb[q]) != hipSuccess ||
+        hipMemcpy(saved[q], p->T, tb[q], hipMemcpyDeviceToDevice) != hipSuccess)
+      rc = fail(SWX_ERR_HIP, "ensemble tuning: saving the Legendre inputs failed");
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (rc == SWX_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(SWX_ERR_HIP, "hipEventCreate failed");
+  for (int v = 0; v < 2 && rc == SWX_OK; ++v) {
+    auto round = [&]() -> hipError_t {
+      if (v == 0) {
+        const swx_plan* q0 = e->p[0];
+        const swx_plan* q1 = e->p[1];
+        const bool f0 = q0->has_fold && q0->grid_fold == 2, f1 = q1->has_fold && q1->grid_fold == 2;
+        return launch_grid_mfma_pair(g[0], f0 ? q0->TinvH : q0->TinvF, f0 ? q0->TfwdH : q0->TfwdF, f0, g[1],
+                                     f1 ? q1->TinvH : q1->TinvF, f1 ? q1->TfwdH : q1->TfwdF, f1, nullptr);
+      }
+      hipError_t er = launch_grid_any<float>(e->p[0], g[0], nullptr);
+      return er == hipSuccess ? launch_grid_any<float>(e->p[1], g[1], nullptr) : er;
+    };
+    hipError_t er = hipSuccess;
+    for (int w = 0; w < 2 && er == hipSuccess; ++w) er = round();
+    if (er == hipSuccess) er = hipEventRecord(e0, nullptr);
+    for (int r = 0; r < 5 && er == hipSuccess; ++r) er = round();
+    if (er == hipSuccess) er = hipEventRecord(e1, nullptr);
+    if (er == hipSuccess) er = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (er == hipSuccess) er = hipEventElapsedTime(&ms, e0, e1);
+    if (er != hipSuccess) rc = fail(SWX_ERR_HIP, std::string("ensemble tuning: ") + hipGetErrorString(er));
+    e->grid_ms[v] = ms / 5;
+  }
+  if (rc == SWX_OK) e->fuse_grid = e->grid_ms[0] <= e->grid_ms[1];
+  for (int q = 0; q < 2; ++q) {
+    if (saved[q]) {
+      if (rc == SWX_OK && hipMemcpy(e->p[q]->T, saved[q], tb[q], hipMemcpyDeviceToDevice) != hipSuccess)
+        rc = fail(SWX_ERR_HIP, "ensemble tuning: restoring the Legendre inputs failed");
+      (void)hipFree(saved[q]);
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  return rc;
 }
 
 }  // namespace
@@ -1180,6 +1248,15 @@ int swx_ensemble_create(swx_plan* a, swx_plan* b, swx_ensemble** out) {
   swx_ensemble* e = new swx_ensemble();
   e->p[0] = p0;
   e->p[1] = p1;
+  // The plans' pending work (their own streams) must not overlap the tuning launches.
+  if (hipDeviceSynchronize() != hipSuccess) {
+    delete e;
+    return fail(SWX_ERR_HIP, "hipDeviceSynchronize failed");
+  }
+  if ((rc = tune_ensemble(e)) != SWX_OK) {
+    delete e;
+    return rc;
+  }
   *out = e;
   return SWX_OK;
 }
