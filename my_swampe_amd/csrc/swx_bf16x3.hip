@@ -67,7 +67,8 @@ __global__ void __launch_bounds__(256) k_grid_bf16x3(GridArgs<float> a, const v8
   __bf16* sXl = sXh + 5 * 16 * XS;
   __bf16* sYh = reinterpret_cast<__bf16*>(lds_raw + S::X_BYTES);  // [8][16][YS]
   __bf16* sYl = sYh + 8 * 16 * YS;
-  float* sC = reinterpret_cast<float*>(lds_raw + S::X_BYTES);    // [2][8][16][16] after the last chunk
+  float* sC = reinterpret_cast<float*>(lds_raw + S::X_BYTES);    // assembly tiles after the last chunk
+  static_assert(S::Y_BYTES >= (int)sizeof(float) * kAsmFloats, "assembly tiles fit the product region");
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -119,6 +120,7 @@ __global__ void __launch_bounds__(256) k_grid_bf16x3(GridArgs<float> a, const v8
   __syncthreads();
 
   v4f acc[2 * NT2];
+  float2 uv[(NT2 + 1) / 2][2];  // not prefetched here: the registers keep two waves per SIMD
 #pragma unroll
   for (int p = 0; p < 2 * NT2; ++p) acc[p] = v4f{0.f, 0.f, 0.f, 0.f};
 
@@ -212,7 +214,7 @@ __global__ void __launch_bounds__(256) k_grid_bf16x3(GridArgs<float> a, const v8
   }
 
   // ---- Phase 3: assemble the Legendre inputs (swx_grid_common.h).
-  grid_assemble<Mp, NT2>(a, acc, sC, b, j0, cf, cl, geo, gl);
+  grid_assemble<Mp, NT2, 0, false>(a, acc, sC, b, j0, cf, cl, geo, gl, uv);
 }
 
 template <int MT>

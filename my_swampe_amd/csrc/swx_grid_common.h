@@ -35,72 +35,112 @@ __device__ __forceinline__ void grid_point_step(const GridArgs<float>& a, const 
 // Assembly of the 9 Legendre inputs of 16 rows (member b, latitudes j0..j0+15)
 // from the forward-DFT accumulators: wave w holds acc[2t + h] = fields 2w + h,
 // wavenumber rows 16t + 4 (lane/16) + q, latitude column lane % 16.  Two
-// 16-row k' tiles go through LDS (sC, 2*8*16*16 floats) per round, then one
-// (m, row) pair per thread assembles (swx_physics.h:assemble_slots) and writes T.
-// U_m, V_m of the step are read from the Fourier rows G.
+// 16-row k' tiles go through LDS (sC, [2][8][16][17] floats: an odd row stride, so the
+// tile writes and the per-thread row reads hit distinct banks) per round, then one
+// (m, row) pair per thread assembles (swx_physics.h:assemble_slots, with the row's
+// constants computed once) and writes T.  U_m, V_m of the step are read from the
+// Fourier rows G.
 //
 // Row order of the accumulator tiles: TE == 0, row r of the 16 TE-free tiles is
 // (m, re/im) = (r / 2, r % 2); TE > 0 (half-period folded forward DFT), tiles t < TE hold the
 // even wavenumbers (m = 2 (r / 2)) and tiles t >= TE the odd ones (m = 2 (r' / 2) + 1, r' the
 // row counted from tile TE).
-template <int Mp, int NT2, int TE = 0>
+constexpr int kAsmStride = 17;
+constexpr int kAsmFloats = 2 * 8 * 16 * kAsmStride;
+
+// Wavenumber handled by this thread in assembly round t0 / 2 (-1: none).
+template <int Mp, int NT2, int TE>
+__device__ __forceinline__ int assemble_m(int t0) {
+  const int tid = threadIdx.x, dt2 = tid >> 7, ml = (tid >> 4) & 7;
+  const int tt = t0 + dt2;
+  const int m = TE == 0 ? 8 * tt + ml : (tt < TE ? 2 * (8 * tt + ml) : 2 * (8 * (tt - TE) + ml) + 1);
+  return (tt < NT2 && m < Mp) ? m : -1;
+}
+
+// U_m, V_m of every assembly round of this thread, loaded ahead (the caller issues this
+// before its forward-DFT GEMM so that the loads complete under the matrix-core work).  Up
+// to 4 rounds (T42, T63); with more the registers are needed elsewhere and each round
+// loads its own.
+template <int NT2>
+constexpr bool kAsmPrefetch = (NT2 + 1) / 2 <= 4;
+
+template <int Mp, int NT2, int TE = 0, bool PRE = true>
+__device__ __forceinline__ void assemble_prefetch_uv(const GridArgs<float>& a, int b, int j0,
+                                                     float2 (&uv)[(NT2 + 1) / 2][2]) {
+  if constexpr (!(PRE && kAsmPrefetch<NT2>)) return;
+  const int jj = j0 + (threadIdx.x & 15);
+#pragma unroll
+  for (int r = 0; r < (NT2 + 1) / 2; ++r) {
+    const int m = assemble_m<Mp, NT2, TE>(2 * r);
+    const int mc = m < 0 ? 0 : m;  // in-bounds address; unused when m < 0
+    uv[r][0] = *reinterpret_cast<const float2*>(a.G + fidx(b, 3, mc, jj, Mp, a.J));
+    uv[r][1] = *reinterpret_cast<const float2*>(a.G + fidx(b, 4, mc, jj, Mp, a.J));
+  }
+}
+
+template <int Mp, int NT2, int TE = 0, bool PRE = true>
 __device__ __forceinline__ void grid_assemble(const GridArgs<float>& a, const v4f* acc, float* sC, int b, int j0,
                                               const double* cf, const CoefLayout& cl, const double* geo,
-                                              const GeoLayout& gl) {
+                                              const GeoLayout& gl, const float2 (&uv)[(NT2 + 1) / 2][2]) {
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, ln = lane & 15, lg = lane >> 4;
   const int J = a.J;
+  // This thread's latitude is the same in every round.
+  const int dt2 = tid >> 7, ml = (tid >> 4) & 7, nn = tid & 15;
+  const int jj = j0 + nn;
+  const double tcj = cf[cl.n_tc + jj], wj = geo[gl.w + jj];
+  const SlotConsts sk = slot_consts(cf, tcj, wj, a.expflag, a.forcflag);
 #pragma unroll
   for (int t0 = 0; t0 < NT2; t0 += 2) {
 #pragma unroll
-    for (int dt2 = 0; dt2 < 2; ++dt2) {
-      const int t = t0 + dt2;
+    for (int d2 = 0; d2 < 2; ++d2) {
+      const int t = t0 + d2;
       if (t < NT2) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           const int f = 2 * wave + h;
 #pragma unroll
-          for (int q = 0; q < 4; ++q) sC[((dt2 * 8 + f) * 16 + 4 * lg + q) * 16 + ln] = acc[2 * t + h][q];
+          for (int q = 0; q < 4; ++q) sC[((d2 * 8 + f) * 16 + 4 * lg + q) * kAsmStride + ln] = acc[2 * t + h][q];
         }
       }
     }
     __syncthreads();
     {
-      const int dt2 = tid >> 7, ml = (tid >> 4) & 7, nn = tid & 15;
-      const int tt = t0 + dt2;
-      const int m = TE == 0 ? 8 * tt + ml : (tt < TE ? 2 * (8 * tt + ml) : 2 * (8 * (tt - TE) + ml) + 1);
-      if (t0 + dt2 < NT2 && m < Mp) {
-        const int jj = j0 + nn;
+      const int m = assemble_m<Mp, NT2, TE>(t0);
+      if (m >= 0) {
         double fm[8][2];
 #pragma unroll
         for (int f = 0; f < 8; ++f) {
-          fm[f][0] = sC[((dt2 * 8 + f) * 16 + 2 * ml) * 16 + nn];
-          fm[f][1] = sC[((dt2 * 8 + f) * 16 + 2 * ml + 1) * 16 + nn];
+          fm[f][0] = sC[((dt2 * 8 + f) * 16 + 2 * ml) * kAsmStride + nn];
+          fm[f][1] = sC[((dt2 * 8 + f) * 16 + 2 * ml + 1) * kAsmStride + nn];
         }
-        const float* gu = a.G + fidx(b, 3, m, jj, Mp, J);
-        const float* gv = a.G + fidx(b, 4, m, jj, Mp, J);
-        double Um[2] = {gu[0], gu[1]}, Vm[2] = {gv[0], gv[1]};
+        float2 gu, gv;
+        if constexpr (PRE && kAsmPrefetch<NT2>) {
+          gu = uv[t0 / 2][0];
+          gv = uv[t0 / 2][1];
+        } else {
+          gu = *reinterpret_cast<const float2*>(a.G + fidx(b, 3, m, jj, Mp, J));
+          gv = *reinterpret_cast<const float2*>(a.G + fidx(b, 4, m, jj, Mp, J));
+        }
+        double Um[2] = {gu.x, gu.y}, Vm[2] = {gv.x, gv.y};
         if (a.export_raw) {
           float* rb = a.raw + (long long)b * kNRaw * J * Mp * 2;
 #pragma unroll
-          for (int f = 0; f < kNRaw; ++f) {
-            rb[((long long)f * J + jj) * Mp * 2 + 2 * m] = (float)fm[f][0];
-            rb[((long long)f * J + jj) * Mp * 2 + 2 * m + 1] = (float)fm[f][1];
-          }
+          for (int f = 0; f < kNRaw; ++f)
+            *reinterpret_cast<float2*>(rb + ((long long)f * J + jj) * Mp * 2 + 2 * m) =
+                make_float2((float)fm[f][0], (float)fm[f][1]);
         }
         double out[kNT][2];
         if (a.si) {
           double lagR[2][2];
           si_lag_exchange(a, cf, b, jj, m, fm, Um, Vm, false, lagR);
-          assemble_slots_si(cf, cf[cl.n_tc + jj], geo[gl.w + jj], m, a.forcflag, a.relax, fm, lagR, out);
+          assemble_slots_si(cf, tcj, wj, m, a.forcflag, a.relax, fm, lagR, out);
         } else {
-          assemble_slots(cf, cf[cl.n_tc + jj], geo[gl.w + jj], m, a.expflag, a.forcflag, fm, Um, Vm, out);
+          assemble_slots(sk, m, fm, Um, Vm, out);
         }
         float* Tm = a.T + (((long long)b * Mp + m) * kNT) * J * 2;
 #pragma unroll
-        for (int k = 0; k < kNT; ++k) {
-          Tm[(k * J + jj) * 2] = (float)out[k][0];
-          Tm[(k * J + jj) * 2 + 1] = (float)out[k][1];
-        }
+        for (int k = 0; k < kNT; ++k)
+          *reinterpret_cast<float2*>(Tm + (k * J + jj) * 2) = make_float2((float)out[k][0], (float)out[k][1]);
       }
     }
     __syncthreads();

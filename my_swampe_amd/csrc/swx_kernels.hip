@@ -499,6 +499,7 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs<R> a) {
   // Assemble the w-weighted Legendre inputs (FP64 math).
   const double wj = geo[gl.w + j];
   const double tc = cf[cl.n_tc + j];
+  const SlotConsts sk = slot_consts(cf, tc, wj, a.expflag, a.forcflag);
   for (int m = tid; m < Mp; m += nt) {
     double fm[8][2];
     for (int f = 0; f < 8; ++f) {
@@ -525,7 +526,7 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs<R> a) {
       si_lag_exchange(a, cf, b, j, m, fm, u2, v2, init, lagR);
       assemble_slots_si(cf, tc, wj, m, a.forcflag, a.relax, fm, lagR, out);
     } else {
-      assemble_slots(cf, tc, wj, m, a.expflag, a.forcflag, fm, Um, Vm, out);
+      assemble_slots(sk, m, fm, Um, Vm, out);
     }
     R* Tm = a.T + (((long long)b * Mp + m) * kNT) * J * 2;
     for (int k = 0; k < kNT; ++k) {

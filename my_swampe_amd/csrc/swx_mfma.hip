@@ -52,7 +52,8 @@ __device__ __forceinline__ void grid_mfma_body(const GridArgs<float>& a, const f
   static_assert(ICH % 16 == 0 && I % ICH == 0, "chunking");
   float* sXT = lds;                    // [5][K1p][16]
   float* sYT = lds + 5 * K1p * 16;     // [8][ICH][16]
-  float* sC = sYT;                     // [2][8][16][16] after the last chunk
+  float* sC = sYT;                     // assembly tiles (kAsmFloats) after the last chunk
+  static_assert(8 * S::ICH * 16 >= kAsmFloats, "assembly tiles fit the product region");
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -97,6 +98,8 @@ __device__ __forceinline__ void grid_mfma_body(const GridArgs<float>& a, const f
   __syncthreads();
 
   v4f acc[2 * NT2];
+  float2 uv[(NT2 + 1) / 2][2];
+  assemble_prefetch_uv<Mp, NT2, 0>(a, b, j0, uv);
 #pragma unroll
   for (int p = 0; p < 2 * NT2; ++p) acc[p] = v4f{0.f, 0.f, 0.f, 0.f};
 
@@ -167,7 +170,7 @@ __device__ __forceinline__ void grid_mfma_body(const GridArgs<float>& a, const f
   }
 
   // ---- Phase 3: assemble the Legendre inputs (swx_grid_common.h).
-  grid_assemble<Mp, NT2>(a, acc, sC, b, j0, cf, cl, geo, gl);
+  grid_assemble<Mp, NT2>(a, acc, sC, b, j0, cf, cl, geo, gl, uv);
 }
 
 // ----------------------------------------------------------------------------
@@ -212,7 +215,8 @@ __device__ __forceinline__ void grid_mfma_fold_body(const GridArgs<float>& a, co
   float* sXT = lds;                                                  // [5][K1][16]
   float* sS = lds + (S::NCH == 1 ? 0 : S::XT_FLOATS);                // [8][ICH][16] s
   float* sD = sS + 8 * ICH * 16;                                     // [8][ICH][16] d
-  float* sC = sS;                                                    // [2][8][16][16] after the last chunk
+  float* sC = sS;                                                    // assembly tiles after the last chunk
+  static_assert(S::SD_FLOATS >= kAsmFloats, "assembly tiles fit the s / d region");
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -258,6 +262,8 @@ __device__ __forceinline__ void grid_mfma_fold_body(const GridArgs<float>& a, co
   __syncthreads();
 
   v4f acc[2 * NT2];
+  float2 uv[(NT2 + 1) / 2][2];
+  assemble_prefetch_uv<Mp, NT2, TE>(a, b, j0, uv);
 #pragma unroll
   for (int p = 0; p < 2 * NT2; ++p) acc[p] = v4f{0.f, 0.f, 0.f, 0.f};
 
@@ -345,7 +351,7 @@ __device__ __forceinline__ void grid_mfma_fold_body(const GridArgs<float>& a, co
   }
 
   // ---- Phase 3: assemble the Legendre inputs (swx_grid_common.h), folded row order.
-  grid_assemble<Mp, NT2, TE>(a, acc, sC, b, j0, cf, cl, geo, gl);
+  grid_assemble<Mp, NT2, TE>(a, acc, sC, b, j0, cf, cl, geo, gl, uv);
 }
 
 template <int MT>
@@ -518,6 +524,21 @@ struct LegShape {
   static constexpr int LDS_FLOATS = SA + SX;
 };
 
+// Basis fragments of forward tile t (row tile t / CT, P or H by column tile t % CT) for this
+// lane; a tile past the last (t >= ntile) loads tile 0's, which is never used.
+template <int MT, int MB>
+__device__ __forceinline__ void leg_fwd_frags(const LegArgs<float>& a, int m, int t, int ntile, int lane,
+                                              v4f (&av)[LegShape<MT, MB>::J / 16]) {
+  using S = LegShape<MT, MB>;
+  constexpr int CT = S::RT / 16, J = S::J, KNP = S::KNP;
+  const int tc = t < ntile ? t : 0;
+  const int nt16 = tc / CT, ct = tc % CT;
+  const v4f* ar = reinterpret_cast<const v4f*>((ct * 16 < S::RP ? a.fwdP : a.fwdH) +
+                                               ((long long)(m * (KNP / 16) + nt16) * 64 + lane) * (J / 4));
+#pragma unroll
+  for (int q4 = 0; q4 < J / 16; ++q4) av[q4] = ar[q4];
+}
+
 // One workgroup's work: zonal wavenumber m, member group `grp` (members grp*MB ...).
 template <int MT, int MB>
 __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int m, const int grp, float* lds) {
@@ -535,6 +556,35 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
   const int Kt = (K + 15) / 16;
   const int K4 = ceil4(K);
   const CoefLayout cl = CoefLayout::make(N, J);
+
+  // The current spectral level of this thread's phase-U pairs (p = tid + 256 u), loaded now so
+  // that the loads complete under the staging and the forward GEMM (up to 2 pairs per thread;
+  // larger groups load in phase U).
+  constexpr int NPF = (MB * Np + 255) / 256;
+  constexpr bool kPre = NPF <= 2;
+  double scp[kPre ? NPF : 1][3][2];
+  if constexpr (kPre) {
+    const long long fstride = (long long)Mp * Np * 2;
+#pragma unroll
+    for (int u = 0; u < NPF; ++u) {
+      const int p = tid + 256 * u;
+      const int mb = p / K, np = p % K;
+      const bool ok = p < MB * K && b0 + mb < B;
+      const double* Sc = a.S + ((long long)(ok ? b0 + mb : 0) * 2 + 1) * 3 * Mp * Np * 2 +
+                         ((long long)m * Np + (ok ? m + np : m)) * 2;
+#pragma unroll
+      for (int f = 0; f < 3; ++f)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) scp[u][f][c] = Sc[f * fstride + c];
+    }
+  }
+
+  // The first forward tile's basis fragments, in flight during the staging.
+  constexpr int CT = RT / 16;
+  constexpr int MAXT = (ceil16(Np) / 16 * CT + 3) / 4;
+  const int ntile = Kt * CT;
+  v4f av0[J / 16];
+  leg_fwd_frags<MT, MB>(a, m, wave, ntile, lane, av0);
 
   // Phase 0: stage the Legendre inputs [j][member, slot, re/im]; zero sX.  Each member's
   // [slot][j][re/im] block is contiguous in T: float4 loads (two latitudes), all issued first.
@@ -573,25 +623,24 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
     for (int e = tid; e < J * padP; e += 256) sA[(e / padP) * RS + MB * 10 + e % padP] = 0.0f;
   __syncthreads();
 
-  // Phase F: forward contraction tiles (Kt row tiles x RT/16 column tiles).
-  constexpr int CT = RT / 16;
-  constexpr int MAXT = (ceil16(Np) / 16 * CT + 3) / 4;
+  // Phase F: forward contraction tiles (Kt row tiles x RT/16 column tiles).  The tiles' A
+  // fragments (fragment-ordered table, contiguous per lane) are double-buffered: the next
+  // tile's loads are in flight during the current tile's MFMAs.
   v4f acc[MAXT];
 #pragma unroll
   for (int q = 0; q < MAXT; ++q) acc[q] = v4f{0.f, 0.f, 0.f, 0.f};
-  const int ntile = Kt * CT;
+  v4f avn[J / 16];
+#pragma unroll
+  for (int q4 = 0; q4 < J / 16; ++q4) avn[q4] = av0[q4];
 #pragma unroll
   for (int q = 0; q < MAXT; ++q) {
     const int t = wave + 4 * q;
-    if (t < ntile) {
-      const int nt16 = t / CT, ct = t % CT;
-      const bool isP = ct * 16 < RP;
-      // The tile's A fragments (fragment-ordered table, contiguous per lane) first, then the MFMAs.
-      const v4f* ar = reinterpret_cast<const v4f*>((isP ? a.fwdP : a.fwdH) +
-                                                   ((long long)(m * (KNP / 16) + nt16) * 64 + lane) * (J / 4));
-      v4f av[J / 16];
+    v4f av[J / 16];
 #pragma unroll
-      for (int q4 = 0; q4 < J / 16; ++q4) av[q4] = ar[q4];
+    for (int q4 = 0; q4 < J / 16; ++q4) av[q4] = avn[q4];
+    if (q + 1 < MAXT) leg_fwd_frags<MT, MB>(a, m, wave + 4 * (q + 1), ntile, lane, avn);
+    if (t < ntile) {
+      const int ct = t % CT;
 #pragma unroll
       for (int ks = 0; ks < J / 4; ++ks)
         acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[ks / 4][ks % 4], sA[(4 * ks + lg) * RS + ct * 16 + ln],
@@ -613,7 +662,10 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
   // Phase U: FP64 update per (member, n) and the inverse-transform inputs.
   if ((a.si || a.raw_filter) && m == 0 && tid < MB && b0 + tid < B)
     residue_step(a.coef + (long long)(b0 + tid) * a.ncoef, a.flags(), a.resid_c + 2 * (b0 + tid));
-  for (int p = tid; p < MB * K; p += 256) {
+#pragma unroll
+  for (int u = 0; u < (MB * Np + 255) / 256; ++u) {
+    const int p = tid + 256 * u;
+    if (p >= MB * K) break;
     const int mb = p / K, np = p % K;
     const int b = b0 + mb;
     if (b >= B) continue;
@@ -629,7 +681,7 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
     const long long so = ((long long)m * Np + n) * 2;
     double eff[3][2], nw[3][2], dh[2], eh[2];
     spectral_update(cf, cl, a.f2[(long long)m * Np + n], m, n, a.flags(), fp, fh, Sb + 3 * fstride, Sb, so,
-                    fstride, eff, nw);
+                    fstride, eff, nw, kPre ? scp[kPre ? u : 0] : nullptr);
     wind_potentials(cf, cl, m, n, nw[0], nw[1], dh, eh);
     float* xr = sX + np * XS;
     for (int c = 0; c < 2; ++c) {

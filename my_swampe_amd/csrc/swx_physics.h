@@ -45,10 +45,16 @@ __device__ inline void grid_products(const double* cf, const double* geo, const 
   out[7] = G;
 }
 
+// 1/x from the hardware reciprocal estimate and one Newton step (a few ulps).
+__device__ __forceinline__ double rcp_nr(double x) {
+  const double r = __builtin_amdgcn_rcp(x);
+  return r * fma(-x, r, 2.0);
+}
+
 // grid_products with reciprocals in place of the divisions (1/(Phi + Phibar) once per
-// point, 1/taudrag from the coefficient table, 1/(2(1-mu^2)) passed in by the caller, who
-// holds one latitude per lane): for the FP32-transform kernels, whose products are rounded
-// to FP32, where the last-bit differences of the FP64 intermediates do not show.
+// point by rcp_nr, 1/taudrag from the coefficient table, 1/(2(1-mu^2)) passed in by the
+// caller, who holds one latitude per lane): for the FP32-transform kernels, whose products
+// are rounded to FP32, where the last-bit differences of the FP64 intermediates do not show.
 __device__ inline void grid_products_recip(const double* cf, const double* geo, const GeoLayout& gl, int j, int i,
                                            int test, double eta, double Phi, double U, double V, double inv_denE,
                                            double out[8]) {
@@ -60,7 +66,7 @@ __device__ inline void grid_products_recip(const double* cf, const double* geo, 
     const double Q = cf[C_INVTAURAD] * (Phieq - (Phi + Phibar));
     const double Qc = Q < 0.0 ? 0.0 : Q;
     const double tot = Phi + Phibar;
-    const double r = -Qc / (fabs(tot) > 0.0 ? tot : DBL_MIN);
+    const double r = -Qc * rcp_nr(fabs(tot) > 0.0 ? tot : DBL_MIN);
     const double itd = cf[C_INVTD];  // 0 for the taudrag == -1 sentinel
     PhiF = Q;
     F = U * r - U * itd;
@@ -171,6 +177,39 @@ __device__ inline void si_lag_exchange(const GridArgs<R>& a, const double* cf, i
   }
 }
 
+// Per-(member, row) constants of the slot assembly, computed once per row so that the
+// per-wavenumber work has no divisions (reciprocals of a^2 and taudrag; the products with
+// them differ from the divided forms by at most an ulp).
+struct SlotConsts {
+  double dt, Phibar, tc, wj;
+  double t1, t2, te, ia2, i2a2, itau;
+  int expflag, forcflag;
+};
+
+__device__ inline SlotConsts slot_consts(const double* cf, double tc, double wj, int expflag, int forcflag) {
+  SlotConsts k;
+  const double dt = cf[C_DT], aa = cf[C_A];
+  k.dt = dt;
+  k.Phibar = cf[C_PHIBAR];
+  k.tc = tc;
+  k.wj = wj;
+  k.expflag = expflag;
+  k.forcflag = forcflag;
+  k.ia2 = 1.0 / (aa * aa);
+  k.i2a2 = 1.0 / (2.0 * aa * aa);
+  if (!expflag) {
+    k.t1 = tc / 4.0;
+    k.t2 = (2.0 * dt / (aa * aa)) / 4.0;
+    k.te = forcflag ? tc : tc / 2.0;
+  } else {
+    k.t1 = tc;
+    k.t2 = 0.0;
+    k.te = tc;
+  }
+  k.itau = 1.0 / cf[C_TAUDRAG];
+  return k;
+}
+
 // w-weighted Legendre inputs of one (member, j, m) from the truncated Fourier
 // coefficients f[8][2] (A, B, C, D, E, PhiF, F, G) and, for the explicit
 // scheme, the wind coefficients Um/Vm.  P-basis slots 0..4, H-basis 5..8:
@@ -190,12 +229,11 @@ __device__ inline void si_lag_exchange(const GridArgs<R>& a, const double* cf, i
 //   explicit (explicit_tdiff.py:63-75, :134-153, :207-224), t1 = tstepcoeff:
 //     P0 -t1 i m C + 2 dt PhiF;  H5 t1 D;  P2 forcing (-t1 i m U + t1 i m taudrag F)/taudrag
 //     H6 (t1 V - taudrag t1 G)/taudrag;  P4 -t1 i m A (+ forcing);  H8 t1 B (+ forcing).
-__device__ inline void assemble_slots(const double* cf, double tc, double wj, int m, int expflag, int forcflag,
-                                      const double f[8][2], const double Um[2], const double Vm[2],
-                                      double out[kNT][2]) {
-  const double dt = cf[C_DT], aa = cf[C_A], Phibar = cf[C_PHIBAR];
+__device__ inline void assemble_slots(const SlotConsts& k, int m, const double f[8][2], const double Um[2],
+                                      const double Vm[2], double out[kNT][2]) {
+  const double dt = k.dt, Phibar = k.Phibar;
   const double mm = (double)m;
-  for (int k = 0; k < kNT; ++k) out[k][0] = out[k][1] = 0.0;
+  for (int s = 0; s < kNT; ++s) out[s][0] = out[s][1] = 0.0;
   const double* Am = f[0];
   const double* Bm = f[1];
   const double* Cm = f[2];
@@ -205,19 +243,16 @@ __device__ inline void assemble_slots(const double* cf, double tc, double wj, in
   const double* Fm = f[6];
   const double* Gm = f[7];
   // i m z = (-m z.im, m z.re)
-  if (!expflag) {
-    const double t1 = tc / 4.0;
-    const double t2 = (2.0 * dt / (aa * aa)) / 4.0;
-    const double te = forcflag ? tc : tc / 2.0;
-    const double ia2 = 1.0 / (aa * aa);
+  if (!k.expflag) {
+    const double t1 = k.t1, t2 = k.t2, te = k.te;
     double Bs[2], As[2], Bf[2], Af[2];
     for (int c = 0; c < 2; ++c) {
       Bf[c] = Bm[c] + Fm[c];
       Af[c] = Am[c] - Gm[c];
-      Bs[c] = forcflag ? Bf[c] : Bm[c];
-      As[c] = forcflag ? Af[c] : Am[c];
+      Bs[c] = k.forcflag ? Bf[c] : Bm[c];
+      As[c] = k.forcflag ? Af[c] : Am[c];
     }
-    const double fPF = forcflag ? 1.0 : 0.0;
+    const double fPF = k.forcflag ? 1.0 : 0.0;
     const double imC[2] = {-mm * Cm[1], mm * Cm[0]};
     const double imBs[2] = {-mm * Bs[1], mm * Bs[0]};
     const double imBf[2] = {-mm * Bf[1], mm * Bf[0]};
@@ -226,16 +261,15 @@ __device__ inline void assemble_slots(const double* cf, double tc, double wj, in
       out[0][c] = -(t1 * imC[c]) - 0.5 * Phibar * (t1 * imBs[c]) + fPF * (dt * PFm[c]);
       out[1][c] = -0.5 * Phibar * (t2 * Em[c]);
       out[2][c] = t1 * imBf[c];
-      out[3][c] = t2 * Em[c] + (t1 * imC[c]) / (2.0 * aa * aa) + fPF * ((dt / 2.0) * PFm[c]) * ia2;
+      out[3][c] = t2 * Em[c] + (t1 * imC[c]) * k.i2a2 + fPF * ((dt / 2.0) * PFm[c]) * k.ia2;
       out[4][c] = -(te * imAs[c]);
       out[5][c] = t1 * Dm[c] - 0.5 * Phibar * (t1 * As[c]);
       out[6][c] = t1 * Af[c];
-      out[7][c] = (t1 * Dm[c]) / (2.0 * aa * aa);
+      out[7][c] = (t1 * Dm[c]) * k.i2a2;
       out[8][c] = te * Bs[c];
     }
   } else {
-    const double t1 = tc;
-    const double tau = cf[C_TAUDRAG];
+    const double t1 = k.t1;
     const double imC[2] = {-mm * Cm[1], mm * Cm[0]};
     const double imA[2] = {-mm * Am[1], mm * Am[0]};
     const double imU[2] = {-mm * Um[1], mm * Um[0]};
@@ -243,21 +277,21 @@ __device__ inline void assemble_slots(const double* cf, double tc, double wj, in
     const double imF[2] = {-mm * Fm[1], mm * Fm[0]};
     const double imG[2] = {-mm * Gm[1], mm * Gm[0]};
     for (int c = 0; c < 2; ++c) {
-      out[0][c] = -(t1 * imC[c]) + (forcflag ? (2.0 * dt) * PFm[c] : 0.0);
+      out[0][c] = -(t1 * imC[c]) + (k.forcflag ? (2.0 * dt) * PFm[c] : 0.0);
       out[5][c] = t1 * Dm[c];
       out[4][c] = -(t1 * imA[c]);
       out[8][c] = t1 * Bm[c];
-      if (forcflag) {
-        out[2][c] = -((t1 * imU[c]) / tau) + t1 * imF[c];
-        out[6][c] = (t1 * Vm[c]) / tau - t1 * Gm[c];
-        out[4][c] += -((t1 * imV[c]) / tau) + t1 * imG[c];
-        out[8][c] += (t1 * Um[c]) / tau + t1 * Fm[c];
+      if (k.forcflag) {
+        out[2][c] = -((t1 * imU[c]) * k.itau) + t1 * imF[c];
+        out[6][c] = (t1 * Vm[c]) * k.itau - t1 * Gm[c];
+        out[4][c] += -((t1 * imV[c]) * k.itau) + t1 * imG[c];
+        out[8][c] += (t1 * Um[c]) * k.itau + t1 * Fm[c];
       }
     }
   }
-  for (int k = 0; k < kNT; ++k) {
-    out[k][0] *= wj;
-    out[k][1] *= wj;
+  for (int s = 0; s < kNT; ++s) {
+    out[s][0] *= k.wj;
+    out[s][1] *= k.wj;
   }
 }
 
@@ -310,10 +344,11 @@ __device__ inline void assemble_slots_si(const double* cf, double tc, double wj,
 //   explicit moves current -> previous; modified Euler keeps only the new level.
 // Returns eff (the stored current level: Fourier rows of eta, delta, Phi) and nw (the
 // new level before the RAW adjustment: the winds are inverted from it).
+// sc_pre: the current level of this (member, m, n) already loaded by the caller, or null.
 __device__ inline void spectral_update(const double* cf, const CoefLayout& cl, double f2, int m, int n,
                                        const StepFlags& fl, const double fp[5][2], const double fh[4][2],
                                        double* Scur, double* Sprev, long long so, long long fstride, double eff[3][2],
-                                       double nw[3][2]) {
+                                       double nw[3][2], const double (*sc_pre)[2] = nullptr) {
   const double dt = cf[C_DT], aa = cf[C_A], Phibar = cf[C_PHIBAR];
   const double nn = (double)n * (double)(n + 1);
   // The prev level is read only by the schemes and filters that use it (the modified-Euler
@@ -322,7 +357,7 @@ __device__ inline void spectral_update(const double* cf, const CoefLayout& cl, d
   double sc[3][2], sp[3][2];
   for (int f = 0; f < 3; ++f)
     for (int c = 0; c < 2; ++c) {
-      sc[f][c] = Scur[f * fstride + so + c];
+      sc[f][c] = sc_pre ? sc_pre[f][c] : Scur[f * fstride + so + c];
       sp[f][c] = need_prev ? Sprev[f * fstride + so + c] : 0.0;
     }
   if (fl.si) {
