@@ -60,6 +60,10 @@ hipError_t launch_pgrad_reduce(const double* pg, double* out, int B, int J, hipS
 // grid_adj_mfma_lds_bytes is 0 where it is not provided (T106).
 hipError_t launch_grid_adj_mfma(const GridAdjArgs<float>& a, const float* TinvF, const float* TfwdF, hipStream_t s);
 size_t grid_adj_mfma_lds_bytes(int M);
+// Its half-period folded variant (the tables of the folded forward grid kernel).
+hipError_t launch_grid_adj_mfma_fold(const GridAdjArgs<float>& a, const float* TinvH, const float* TfwdH,
+                                     hipStream_t s);
+size_t grid_adj_mfma_fold_lds_bytes(int M);
 // Matrix-core Legendre adjoint (all modes, FP32; swx_adjoint_leg_mfma.hip) and its largest
 // per-workgroup LDS.
 hipError_t launch_leg_adj_mfma(const LegAdjArgs<float>& a, hipStream_t s);

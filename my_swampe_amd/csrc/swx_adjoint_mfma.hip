@@ -229,6 +229,272 @@ __global__ void __launch_bounds__(256) k_grid_adj_mfma(GridAdjArgs<float> a, con
   }
 }
 
+// ----------------------------------------------------------------------------
+// Half-period folded variant (the forward grid kernel's fold, swx_mfma.hip): GEMM 1 gives
+// the even / odd wavenumber sums E, O of the 12 fields on i < H = I/2 (x(i) = E + O,
+// x(i + H) = E - O), the pointwise adjoint runs on both points of each pair, and GEMM 2 takes
+// s = y(i) + y(i + H) for the even-m output tiles and d = y(i) - y(i + H) for the odd ones.
+// Tables and row orders: those of k_grid_mfma_fold (swx_kernels.h FoldDims).  At T42 (one
+// chunk) s / d reuse the X^T space, which keeps two workgroups per CU.
+template <int MT>
+struct AdjFoldShape {
+  static constexpr int Mp = MT + 1;
+  static constexpr int I = MT == 42 ? 128 : (MT == 63 ? 192 : 320);
+  static constexpr int H = I / 2;
+  static constexpr int NE = (Mp + 1) / 2, NO = Mp / 2;
+  static constexpr int KE4 = (2 * NE + 3) / 4 * 4, KO4 = (2 * NO + 3) / 4 * 4;
+  static constexpr int K1 = KE4 + KO4;
+  static constexpr int KQ1 = (K1 / 4 + 3) / 4 * 4;
+  static constexpr int TE = (2 * NE + 15) / 16, TO = (2 * NO + 15) / 16;
+  static constexpr int NT2 = TE + TO;
+  static constexpr int ICH = 64;
+  static constexpr int NCH = (H + ICH - 1) / ICH;
+  static constexpr int NX = 12;
+  static constexpr int XT_FLOATS = NX * K1 * 16;
+  static constexpr int SD_FLOATS = 2 * 4 * ICH * 16;
+  static constexpr int MAIN_FLOATS = NCH == 1 ? (XT_FLOATS > SD_FLOATS ? XT_FLOATS : SD_FLOATS) : XT_FLOATS + SD_FLOATS;
+  static constexpr int LDS_FLOATS = MAIN_FLOATS + 2 * 4 * 16 * 3;  // + row partials [4][16][3] doubles
+};
+
+template <int MT>
+__global__ void __launch_bounds__(256) k_grid_adj_mfma_fold(GridAdjArgs<float> a, const float* __restrict__ TinvF,
+                                                            const float* __restrict__ TfwdF) {
+  using S = AdjFoldShape<MT>;
+  constexpr int Mp = S::Mp, I = S::I, H = S::H, K1 = S::K1, KE4 = S::KE4, KQ1 = S::KQ1, NT2 = S::NT2, TE = S::TE;
+  constexpr int ICH = S::ICH, NX = S::NX;
+  extern __shared__ __align__(16) float lds[];
+  float* sXT = lds;                                                       // [NX][K1][16]
+  float* sS = lds + (S::NCH == 1 ? 0 : S::XT_FLOATS);                      // [4][ICH][16] s
+  float* sD = sS + 4 * ICH * 16;                                          // [4][ICH][16] d
+  double* sRed = reinterpret_cast<double*>(lds + S::MAIN_FLOATS);         // [4][16][3]
+
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, ln = lane & 15, lg = lane >> 4;
+  const int J = a.J, N = a.N;
+  const int row0 = blockIdx.x * 16;
+  const int b = row0 / J, j0 = row0 % J;
+  const double* cf = a.coef + (long long)b * a.ncoef;
+  const CoefLayout cl = CoefLayout::make(N, J);
+  const GeoLayout gl = GeoLayout::make(J, I);
+  const double* geo = a.geo;
+  const long long plane = (long long)J * I;
+  auto slot_m = [](int sl) { return sl < KE4 / 2 ? 2 * sl : 2 * (sl - KE4 / 2) + 1; };
+
+  // Phase 0a: Fourier rows of S_t (G fields 0, 2, 3, 4) -> X^T fields 0..3, folded k order.
+  {
+    constexpr int NE4 = 4 * (K1 / 2) * 8;
+    constexpr int NQ = (NE4 + 255) / 256;
+    v4f gv[NQ];
+#pragma unroll
+    for (int u = 0; u < NQ; ++u) {
+      const int e = tid + 256 * u;
+      const int jq = e & 7, fs = e >> 3;
+      const int fx = fs / (K1 / 2) < 4 ? fs / (K1 / 2) : 3, m = slot_m(fs % (K1 / 2));
+      const int fg = fx == 0 ? 0 : fx + 1;
+      const v4f v = *reinterpret_cast<const v4f*>(a.G + fidx(b, fg, m < Mp ? m : Mp - 1, j0 + 2 * jq, Mp, J));
+      gv[u] = (e < NE4 && m < Mp) ? v : v4f{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int u = 0; u < NQ; ++u) {
+      const int e = tid + 256 * u;
+      if (e < NE4) {
+        const int jq = e & 7, fs = e >> 3;
+        const int fx = fs / (K1 / 2), sl = fs % (K1 / 2);
+        float* xr = sXT + (fx * K1 + 2 * sl) * 16 + 2 * jq;
+        *reinterpret_cast<float2*>(xr) = make_float2(gv[u][0], gv[u][2]);
+        *reinterpret_cast<float2*>(xr + 16) = make_float2(gv[u][1], gv[u][3]);
+      }
+    }
+  }
+  // Phase 0b: Fourier cotangents per (m, row) -> X^T fields 4..11, scaled by 1 / (I w_m).
+  for (int p = tid; p < (K1 / 2) * 16; p += 256) {
+    const int sl = p >> 4, r = p & 15;
+    const int m = slot_m(sl);
+    float* xr = sXT + (4 * K1 + 2 * sl) * 16 + r;
+    if (m < Mp) {
+      const int jj = j0 + r;
+      const float* Tb = a.Tbar + (((long long)b * Mp + m) * kNT) * J * 2;
+      double tau[kNT][2];
+#pragma unroll
+      for (int k = 0; k < kNT; ++k) {
+        tau[k][0] = Tb[(k * J + jj) * 2];
+        tau[k][1] = Tb[(k * J + jj) * 2 + 1];
+      }
+      double fb[8][2];
+      assemble_slots_adjoint(cf, cf[cl.n_tc + jj], geo[gl.w + jj], m, a.forcflag, tau, fb);
+      const double sc = 1.0 / ((double)I * (m == 0 ? 1.0 : 2.0));
+#pragma unroll
+      for (int f = 0; f < 8; ++f) {
+        xr[(f * K1) * 16] = (float)(fb[f][0] * sc);
+        xr[(f * K1 + 1) * 16] = (float)(fb[f][1] * sc);
+      }
+    } else {
+#pragma unroll
+      for (int f = 0; f < 8; ++f) {
+        xr[(f * K1) * 16] = 0.0f;
+        xr[(f * K1 + 1) * 16] = 0.0f;
+      }
+    }
+  }
+  __syncthreads();
+
+  v4f acc[NT2];
+#pragma unroll
+  for (int t = 0; t < NT2; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+  double dpar[3] = {0.0, 0.0, 0.0};
+  const int j = j0 + ln;
+  const v4f* Tf = reinterpret_cast<const v4f*>(TfwdF) + lane;
+
+  for (int ic = 0; ic < H; ic += ICH) {
+    const int ntc = (H - ic) / 16 < ICH / 16 ? (H - ic) / 16 : ICH / 16;
+    const bool active = wave < ntc;
+    const int ib = ic + 16 * wave;
+    // ---- Phase 1: even / odd sums of the 12 fields on this wave's tile.
+    v4f cE[NX], cO[NX];
+#pragma unroll
+    for (int f = 0; f < NX; ++f) cE[f] = cO[f] = v4f{0.f, 0.f, 0.f, 0.f};
+    if (active) {
+      v4f afr[KQ1 / 4];
+      const v4f* Ta = reinterpret_cast<const v4f*>(TinvF + ((long long)(ib / 16) * 64 + lane) * KQ1);
+#pragma unroll
+      for (int q4 = 0; q4 < KQ1 / 4; ++q4) afr[q4] = Ta[q4];
+#pragma unroll
+      for (int ks = 0; ks < K1 / 4; ++ks) {
+        const float* xb = sXT + (4 * ks + lg) * 16 + ln;
+#pragma unroll
+        for (int f = 0; f < NX; ++f) {
+          if (ks < KE4 / 4)
+            cE[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(afr[ks / 4][ks % 4], xb[f * K1 * 16], cE[f], 0, 0, 0);
+          else
+            cO[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(afr[ks / 4][ks % 4], xb[f * K1 * 16], cO[f], 0, 0, 0);
+        }
+      }
+    }
+    v4f an[NT2];
+#pragma unroll
+    for (int t = 0; t < NT2; ++t) an[t] = Tf[((long long)t * (H / 16) + ic / 16) * 64];
+    if constexpr (S::NCH == 1) __syncthreads();  // s / d overwrite X^T
+    // ---- Pointwise adjoint on the lane's 4 point pairs (row ln, i = ib + 4 lg + q and i + H).
+    if (active) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int il = 16 * wave + 4 * lg + q;
+        const int i0 = ic + il;
+        double o2[2][4];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int i = i0 + h * H;
+          const float sg = h == 0 ? 1.0f : -1.0f;
+          double U = cE[2][q] + sg * cO[2][q], V = cE[3][q] + sg * cO[3][q];
+          if (a.test == 1) {
+            U = a.uvic[((long long)b * 2 + 0) * plane + (long long)j * I + i];
+            V = a.uvic[((long long)b * 2 + 1) * plane + (long long)j * I + i];
+          }
+          double pbar[8];
+#pragma unroll
+          for (int f = 0; f < 8; ++f) pbar[f] = cE[4 + f][q] + sg * cO[4 + f][q];
+          grid_products_adjoint(cf, geo, gl, j, i, a.test, cE[0][q] + sg * cO[0][q], cE[1][q] + sg * cO[1][q], U,
+                                V, pbar, o2[h], dpar);
+        }
+        // The pinned winds of test 1 do not depend on the state.
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+          const bool zero = a.test == 1 && f >= 2;
+          sS[(f * ICH + il) * 16 + ln] = zero ? 0.0f : (float)(o2[0][f] + o2[1][f]);
+          sD[(f * ICH + il) * 16 + ln] = zero ? 0.0f : (float)(o2[0][f] - o2[1][f]);
+        }
+      }
+    }
+    __syncthreads();
+    // ---- Phase 2: GEMM 2 partial sums, wave w owning cotangent field w; even-m tiles read s,
+    // odd-m tiles d; the next group's A fragments are in flight while the current one runs.
+#pragma unroll 1
+    for (int gg = 0; gg < ntc; ++gg) {
+      v4f ac[NT2];
+#pragma unroll
+      for (int t = 0; t < NT2; ++t) ac[t] = an[t];
+      if (gg + 1 < ntc) {
+#pragma unroll
+        for (int t = 0; t < NT2; ++t) an[t] = Tf[((long long)t * (H / 16) + ic / 16 + gg + 1) * 64];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int il = 16 * gg + 4 * q + lg;
+        const float bs = sS[(wave * ICH + il) * 16 + ln];
+        const float bd = sD[(wave * ICH + il) * 16 + ln];
+#pragma unroll
+        for (int t = 0; t < NT2; ++t)
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[t][q], t < TE ? bs : bd, acc[t], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+
+  // Per-row parameter partials: row ln is held by lanes ln + 16 g of every wave.
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double v = dpar[c];
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    dpar[c] = v;
+  }
+  if (lg == 0)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) sRed[(wave * 16 + ln) * 3 + c] = dpar[c];
+  __syncthreads();
+  if (tid < 16 * 3) {
+    const int r = tid / 3, c = tid % 3;
+    const double sum = sRed[(0 * 16 + r) * 3 + c] + sRed[(1 * 16 + r) * 3 + c] + sRed[(2 * 16 + r) * 3 + c] +
+                       sRed[(3 * 16 + r) * 3 + c];
+    a.pgrad[((long long)b * J + j0 + r) * 3 + c] += sum;
+  }
+
+  // Gbar = I w_m (Tfwd xbar) in the folded output rows: wave w -> field eta, Phi, U, V.
+  const int fo = wave == 0 ? 0 : wave + 1;
+#pragma unroll
+  for (int t = 0; t < NT2; ++t) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int rr = 16 * (t < TE ? t : t - TE) + 4 * lg + q;
+      const int m = t < TE ? 2 * (rr >> 1) : 2 * (rr >> 1) + 1, c = rr & 1;
+      if (m < Mp) {
+        const float v = (m == 0 && c == 1) ? 0.0f : (float)I * (m == 0 ? 1.0f : 2.0f) * acc[t][q];
+        a.Gbar[fidx(b, fo, m, j, Mp, J) + c] = v;
+      }
+    }
+  }
+  // delta's cotangent is zero (the products do not depend on delta).
+  for (int p = tid; p < Mp * 16 * 2; p += 256) {
+    const int m = p >> 5, r = (p >> 1) & 15, c = p & 1;
+    a.Gbar[fidx(b, 1, m, j0 + r, Mp, J) + c] = 0.0f;
+  }
+}
+
+template <int MT>
+static hipError_t launch_adj_mfma_fold_t(const GridAdjArgs<float>& a, const float* TinvH, const float* TfwdH,
+                                         hipStream_t s) {
+  const size_t smem = sizeof(float) * AdjFoldShape<MT>::LDS_FLOATS;
+  hipLaunchKernelGGL(k_grid_adj_mfma_fold<MT>, dim3((unsigned)((long long)a.B * a.J / 16)), dim3(256), smem, s, a,
+                     TinvH, TfwdH);
+  return hipGetLastError();
+}
+
+size_t grid_adj_mfma_fold_lds_bytes(int M) {
+  switch (M) {
+    case 42: return sizeof(float) * AdjFoldShape<42>::LDS_FLOATS;
+    case 63: return sizeof(float) * AdjFoldShape<63>::LDS_FLOATS;
+    default: return 0;  // T106: the 12-field staging exceeds the workgroup's LDS
+  }
+}
+
+hipError_t launch_grid_adj_mfma_fold(const GridAdjArgs<float>& a, const float* TinvH, const float* TfwdH,
+                                     hipStream_t s) {
+  switch (a.M) {
+    case 42: return launch_adj_mfma_fold_t<42>(a, TinvH, TfwdH, s);
+    case 63: return launch_adj_mfma_fold_t<63>(a, TinvH, TfwdH, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 template <int MT>
 static hipError_t launch_adj_mfma_t(const GridAdjArgs<float>& a, const float* TinvF, const float* TfwdF,
                                     hipStream_t s) {

@@ -95,6 +95,7 @@ struct swx_plan {
   bool use_bf16x3 = false;     // split-BF16x3 grid kernel (precision SWX_PREC_BF16X3)
   bool use_adj_mfma = false;   // MFMA grid adjoint kernel (FP32 transforms, where its LDS fits)
   bool use_leg_adj_mfma = false;  // MFMA Legendre adjoint kernels (FP32 transforms)
+  bool use_adj_fold = false;      // folded MFMA grid adjoint available (its LDS fits)
   void* TinvB = nullptr;       // its DFT tables: bf16 hi / lo pairs in fragment order
   void* TfwdB = nullptr;
   bool opt_plain_grid = false;  // SWX_OPT_GRID_KERNEL = 1
@@ -592,7 +593,11 @@ int adjoint_step(swx_plan* p, const double* St, hipStream_t s) {
   const GridAdjArgs<R> g = grid_adj_args<R>(p, GADJ_STEP);
   if constexpr (sizeof(R) == 4) {
     if (p->use_adj_mfma && !p->opt_plain_grid) {
-      SWX_HIP(launch_grid_adj_mfma(g, p->TinvF, p->TfwdF, s));
+      // The grid variant of the forward (SWX_OPT_GRID_FOLD) applies to the reverse step too.
+      if (p->use_adj_fold && p->grid_fold == 2)
+        SWX_HIP(launch_grid_adj_mfma_fold(g, p->TinvH, p->TfwdH, s));
+      else
+        SWX_HIP(launch_grid_adj_mfma(g, p->TinvF, p->TfwdF, s));
     } else {
       SWX_HIP(launch_grid_adj<R>(g, s));
     }
@@ -1063,6 +1068,8 @@ int swx_plan_create(const swx_plan_desc* desc, const double* params, swx_plan** 
           return rc;
         }
         p->has_fold = true;
+        const size_t adjf = grid_adj_mfma_fold_lds_bytes(M);
+        p->use_adj_fold = adjf > 0 && adjf <= (size_t)maxlds;
       }
       if (desc->precision == SWX_PREC_BF16X3 && grid_bf16x3_lds_bytes(M) <= (size_t)maxlds) {
         // Fragment order of the split-BF16x3 kernel: per (tile, k-step, lane) 8 hi then 8 lo

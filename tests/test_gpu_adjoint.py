@@ -160,11 +160,13 @@ def test_checkpoint_interval_does_not_change_gradients(grads):
     assert grads["stats"]["checkpoints"] == 2 and grads["stats"]["segment_states"] == 3
 
 
-@pytest.mark.parametrize("kernel,M", [("grid", 42), ("grid", 63), ("leg", 42), ("leg", 63), ("leg", 106)])
+@pytest.mark.parametrize("kernel,M", [("grid", 42), ("grid", 63), ("grid_full", 42), ("grid_full", 63), ("leg", 42),
+                                      ("leg", 63), ("leg", 106)])
 def test_mfma_adjoint_matches_plain_fp32(kernel, M, torch_gpu):
     """The matrix-core grid (or Legendre) adjoint kernels vs the plain FP32 ones, selected with
     SWX_OPT_GRID_KERNEL (or SWX_OPT_LEG_KERNEL, which switches the forward and reverse Legendre
-    kernels together) on the same reverse sweep: 3 members with different forcing, 4 steps,
+    kernels together) on the same reverse sweep; "grid" runs the half-period folded grid kernels,
+    "grid_full" the full-DFT ones (SWX_OPT_GRID_FOLD): 3 members with different forcing, 4 steps,
     random terminal Phi cotangent.  Relative L2 of dL/dS_0 and of the parameter gradients
     <= 1e-5 (FP32 transforms in both, different summation orders)."""
     from my_swampe_amd import _capi
@@ -173,7 +175,7 @@ def test_mfma_adjoint_matches_plain_fp32(kernel, M, torch_gpu):
 
     torch = torch_gpu
     print(f"[gpu-case] mfma-adjoint kernel={kernel} M={M}", flush=True)
-    option = _capi.OPT_GRID_KERNEL if kernel == "grid" else _capi.OPT_LEG_KERNEL
+    option = _capi.OPT_LEG_KERNEL if kernel == "leg" else _capi.OPT_GRID_KERNEL
     B, nsteps = 3, 4
     rng = np.random.default_rng(M)
     params = np.stack([param_row(dt=120.0, a=8.2e7, omega=3.2e-5, g=9.8, Phibar=3e5, taurad=10 ** rng.uniform(4.5, 5.5),
@@ -186,6 +188,7 @@ def test_mfma_adjoint_matches_plain_fp32(kernel, M, torch_gpu):
     for kind, opt in (("plain", 1), ("mfma", 0)):
         plan = Plan(M, params, precision="fp32")
         plan.set_option(option, opt)
+        plan.set_option(_capi.OPT_GRID_FOLD, 1 if kernel == "grid_full" else 2)
         plan.init_spectral(torch.tensor(c, device="cuda"))
         states = []
         for k in range(nsteps):
