@@ -66,13 +66,38 @@ def _params(cfg: RolloutConfig, taurad, taudrag, DPhieq) -> np.ndarray:
                      for b in range(B)])
 
 
+# Idle plans by configuration: a rollout takes one (or creates it), sets the members'
+# parameters, and its backward hands it back, so repeated rollouts of one configuration (an
+# optimizer loop) skip the plan's table construction and kernel tuning.  A rollout whose
+# backward never runs keeps its plan until the autograd graph is freed.
+_POOL: dict = {}
+_POOL_MAX = 4  # idle plans kept per configuration
+
+
+def _plan_key(cfg: RolloutConfig, B: int):
+    return (cfg.M, B, cfg.precision, cfg.forcflag, cfg.diffflag, cfg.modalflag)
+
+
 def _make_plan(cfg: RolloutConfig, params) -> Plan:
-    plan = Plan(cfg.M, params, precision=cfg.precision, forcflag=cfg.forcflag, diffflag=cfg.diffflag,
-                modalflag=cfg.modalflag, diagnostics=False)
+    idle = _POOL.get(_plan_key(cfg, params.shape[0]))
+    if idle:
+        plan = idle.pop()
+        plan.set_params(params)
+    else:
+        plan = Plan(cfg.M, params, precision=cfg.precision, forcflag=cfg.forcflag, diffflag=cfg.diffflag,
+                    modalflag=cfg.modalflag, diagnostics=False)
     # The rollout reads the spectral state and the final Phi_curr only: the physical grid carry
     # is written in the last step of each run call.
     plan.set_option(_capi.OPT_PHYS_STEPS, 1)
     return plan
+
+
+def _release_plan(cfg: RolloutConfig, plan: Plan) -> None:
+    idle = _POOL.setdefault(_plan_key(cfg, plan.B), [])
+    if len(idle) < _POOL_MAX:
+        idle.append(plan)
+    else:
+        plan.close()
 
 
 def forward_with_checkpoints(plan: Plan, cfg: RolloutConfig, c):
@@ -154,7 +179,7 @@ class _Rollout:
                 lam, pg = adjoint_through_checkpoints(plan, cfg, ctx.ckpts, gPhi.contiguous())
                 torch.cuda.current_stream(plan.device).synchronize()
                 cfg.stats["plan_device_bytes"] = plan.device_bytes()
-                plan.close()
+                _release_plan(cfg, plan)
                 ctx.plan = None
                 ctx.ckpts = None
                 sr, sd, sp = ctx.shapes
