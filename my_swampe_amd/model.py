@@ -6,8 +6,9 @@ time integration itself runs in the HIP kernels of libswampe_hip.so through
 :class:`my_swampe_amd.plan.Plan`; arrays are returned as torch tensors on the
 GPU (the counterpart of JAX device arrays).
 
-Out of scope here (they raise NotImplementedError): continuation from pickles
-(contflag) and the legacy plotting/saving side effects of run_model.
+Continuation (contflag / contTime) and run_model's saveflag use the reference's restart-file
+layout (continuation.py).  Out of scope here (NotImplementedError): the legacy plotting side
+effects of run_model (plotflag).
 """
 
 from __future__ import annotations
@@ -18,7 +19,7 @@ from typing import Any, Dict, NamedTuple, Optional
 
 import numpy as np
 
-from . import _capi
+from . import _capi, continuation
 from .plan import Plan, require_gpu
 
 Scalar = Any
@@ -245,8 +246,10 @@ def _rms_winds(static: Static, U, V):
     return torch.sqrt(torch.sum(area * integrand / (4.0 * math.pi * a ** 2)))
 
 
-def _validate(*, tmax, dt, test, semi_implicit, expflag, raw_filter, contflag, contTime, starttime):
-    """Host-side validation before any device work (model.py:1520-1613)."""
+def _validate(*, tmax, dt, test, semi_implicit, expflag, raw_filter, contflag, contTime, starttime, timeunits):
+    """Host-side validation before any device work (model.py:1520-1613).
+
+    Returns (test, start step, continuation file token, its fallback spelling)."""
     if tmax < 2:
         raise ValueError("tmax must be >= 2 (SWAMPE uses a 2-level initialization).")
     if _is_python_scalar(dt) and float(dt) <= 0.0:
@@ -259,21 +262,46 @@ def _validate(*, tmax, dt, test, semi_implicit, expflag, raw_filter, contflag, c
         raise ValueError(
             "raw_filter=True is incompatible with expflag=True (the explicit scheme reads the "
             "unfiltered spectral prev carry, so the RAW lineage change would alter its parity behavior).")
+    token = fallback = None
+    stamp = None
     if contflag:
         if contTime is None:
             raise ValueError("contflag=True requires contTime.")
         try:
-            float(contTime)
+            stamp = float(contTime)
         except (TypeError, ValueError) as e:
             raise ValueError(f"contTime must be numeric (int/float or numeric string), got {contTime!r}.") from e
-    start = 2 if starttime is None else int(starttime)
-    if not contflag and start > tmax:
+        # File token: the integer spelling when the timestamp is integer-like (model.py:1588-1595).
+        fallback = str(contTime)
+        token = str(int(round(stamp))) if abs(stamp - round(stamp)) < 1e-12 else fallback
+    if starttime is not None:
+        start = int(starttime)
+    elif not contflag:
+        start = 2
+    else:
+        try:
+            dt_float = float(dt)
+        except TypeError as e:
+            raise TypeError("dt must be a Python float when contflag=True (continuation uses Python I/O).") from e
+        start = continuation.compute_t_from_timestamp(timeunits, stamp, dt_float)
+    if start > tmax:
         raise ValueError(f"starttime={start} must be <= tmax={tmax}.")
-    if contflag:
-        raise NotImplementedError("contflag=True (continuation from pickles) is not provided by my_swampe_amd.")
     if test not in (None, 1, 2):
         raise NotImplementedError(f"test={test} is not implemented (None, 1 or 2).")
-    return test, start
+    return test, start, token, fallback
+
+
+def _continuation_fields(token: str, fallback: str, custompath):
+    """eta, delta, Phi of a restart (model.py:1645-1679): the canonical token's files, else the
+    token as given; the winds are then diagnosed from eta and delta."""
+    def read(prefix):
+        try:
+            return continuation.read_pickle(f"{prefix}-{token}", custompath=custompath)
+        except FileNotFoundError:
+            if fallback != token:
+                return continuation.read_pickle(f"{prefix}-{fallback}", custompath=custompath)
+            raise
+    return read("eta"), read("delta"), read("Phi")
 
 
 def _check_ic_args(shape, eta0_init, delta0_init, Phi0_init, U0_init, V0_init):
@@ -350,8 +378,9 @@ def run_model_scan(*, M: int, dt: Scalar, tmax: int, Phibar: Scalar, omega: Scal
     ``remat_step``, ``jit_scan`` and ``donate_state`` are accepted for
     signature compatibility and have no effect.
     """
-    test, start = _validate(tmax=tmax, dt=dt, test=test, semi_implicit=semi_implicit, expflag=expflag,
-                            raw_filter=raw_filter, contflag=contflag, contTime=contTime, starttime=starttime)
+    test, start, token, fallback = _validate(tmax=tmax, dt=dt, test=test, semi_implicit=semi_implicit,
+                                             expflag=expflag, raw_filter=raw_filter, contflag=contflag,
+                                             contTime=contTime, starttime=starttime, timeunits=timeunits)
     J, I = _capi.grid_size(int(M))  # ValueError for an unsupported truncation (initial_conditions.py:80-81)
     _check_ic_args((J, I), eta0_init, delta0_init, Phi0_init, U0_init, V0_init)
     if precision not in _capi.PRECISIONS:
@@ -362,6 +391,11 @@ def run_model_scan(*, M: int, dt: Scalar, tmax: int, Phibar: Scalar, omega: Scal
                      raw_filter=bool(raw_filter), alpha=alpha, williams_alpha=williams_alpha, si_alpha=si_alpha)
     static = build_static(M=int(M), dt=dt, a=a, omega=omega, g=g, Phibar=Phibar, taurad=taurad, taudrag=taudrag,
                           DPhieq=DPhieq, K6=K6, K6Phi=K6Phi, test=test, alpha=alpha)
+    have_ic = (eta0_init is not None) or (delta0_init is not None) or (Phi0_init is not None)
+    if contflag and not have_ic:
+        eta0_init, delta0_init, Phi0_init = _continuation_fields(token, fallback, custompath)
+        U0_init = V0_init = None
+        _check_ic_args((J, I), eta0_init, delta0_init, Phi0_init, None, None)
     eta0, delta0, Phi0, U0, V0 = _initial_fields(static, test, a1, eta0_init, delta0_init, Phi0_init,
                                                  U0_init, V0_init)
     params = static.params.copy()
@@ -467,10 +501,13 @@ def run_model(M: int, dt: float, tmax: int, Phibar: float, omega: float, a: floa
               verbose: bool = True, *, K6Phi: Optional[float] = None, semi_implicit: bool = False,
               si_alpha: float = 0.5, raw_filter: bool = False, williams_alpha: float = 0.53,
               jit_scan: bool = True, as_numpy: bool = True, precision: str = "fp64") -> Dict[str, Any]:
-    """Legacy SWAMPE signature with (tmax, 2) spinup/geopot diagnostics (model.py:1954-2219)."""
-    if plotflag or saveflag:
-        raise NotImplementedError("plotflag/saveflag side effects are not provided by my_swampe_amd; "
-                                  "pass plotflag=False, saveflag=False (or use run_model_gpu).")
+    """Legacy SWAMPE signature with (tmax, 2) spinup/geopot diagnostics (model.py:1954-2219).
+
+    saveflag writes the restart files of every step t with t % savefreq == 0
+    (model.py:2165-2182, continuation.save_data)."""
+    if plotflag:
+        raise NotImplementedError("plotflag side effects (matplotlib figures) are not provided by my_swampe_amd; "
+                                  "pass plotflag=False (or use run_model_gpu).")
     torch = require_gpu()
     res = run_model_scan(M=M, dt=dt, tmax=tmax, Phibar=Phibar, omega=omega, a=a, test=test, g=g,
                          forcflag=forcflag, taurad=taurad, taudrag=taudrag, DPhieq=DPhieq, a1=a1,
@@ -498,6 +535,13 @@ def run_model(M: int, dt: float, tmax: int, Phibar: float, omega: float, a: floa
         spin[0, 1] = _rms_winds(static, U0, V0)
         geo[0, 0] = torch.min(Phi0)
         geo[0, 1] = torch.max(Phi0)
+    if saveflag:
+        t_host = t_seq.cpu().numpy()
+        for k, t in enumerate(t_host):
+            if int(t) % int(savefreq) == 0:
+                ts = continuation.compute_timestamp(timeunits, int(t), dt)
+                continuation.save_data(ts, outs["eta"][k], outs["delta"][k], outs["Phi"][k], outs["U"][k],
+                                       outs["V"][k], spin, geo, custompath=custompath)
     if verbose:
         print("GCM run completed!")
     n = outs["eta"].shape[0]
