@@ -221,6 +221,14 @@ int swx_state_set(swx_plan *plan, const double *in, void *stream);
  * (model.py:1281-1342).  The forward carry of the plan
  * (Fourier rows) is used as scratch. */
 int swx_adjoint_begin(swx_plan *plan, void *stream);
+/* Add the cotangents of the grid fields observed at the sweep's current
+ * position (the state whose dL/dS the plan holds) to it: the time-series
+ * counterpart of swx_adjoint_terminal (which assigns), for losses that read
+ * the trajectory at several steps (the phase-curve likelihood accumulated
+ * over the scan, DEVELOPMENT.md:587-616).  No counterpart in the reference's
+ * code (jax.grad differentiates such a scan by itself). */
+int swx_adjoint_observe(swx_plan *plan, const double *g_eta, const double *g_delta, const double *g_Phi,
+                        const double *g_U, const double *g_V, void *stream);
 int swx_adjoint_terminal(swx_plan *plan, const double *g_eta, const double *g_delta, const double *g_Phi,
                          const double *g_U, const double *g_V, void *stream);
 int swx_adjoint_step(swx_plan *plan, const double *state_t, void *stream);

@@ -140,6 +140,57 @@ def adjoint_through_checkpoints(plan: Plan, cfg: RolloutConfig, ckpts, gPhi):
     return plan.adjoint_result()
 
 
+def forward_series(plan: Plan, cfg: RolloutConfig, c, obs_steps):
+    """Run the rollout from c, returning the checkpoints and Phi after each step count in obs_steps."""
+    torch = require_gpu()
+    K = cfg.every()
+    plan.init_spectral(c)
+    ckpts: List = []
+    obs = []
+    pending = list(obs_steps)
+    if pending and pending[0] == 0:
+        obs.append(plan.export(fields=("Phi_curr",))["Phi_curr"])
+        pending.pop(0)
+    done = 0
+    while done < cfg.nsteps:
+        seg_end = min(done + K, cfg.nsteps)
+        ckpts.append(plan.state_get())
+        while done < seg_end:
+            stop = min(seg_end, pending[0]) if pending else seg_end
+            plan.run(cfg.t0 + done, stop - done)
+            done = stop
+            if pending and pending[0] == done:
+                obs.append(plan.export(fields=("Phi_curr",))["Phi_curr"])
+                pending.pop(0)
+    return ckpts, torch.stack(obs, dim=1)
+
+
+def adjoint_series(plan: Plan, cfg: RolloutConfig, ckpts, obs_steps, gPhi):
+    """Reverse sweep with the cotangents gPhi (B, n_obs, J, I) of Phi after each observed step count
+    added when the sweep reaches that state."""
+    K = cfg.every()
+    where = {int(n): k for k, n in enumerate(obs_steps)}
+    plan.adjoint_begin()
+    if cfg.nsteps in where:
+        plan.adjoint_terminal(g_Phi=gPhi[:, where[cfg.nsteps]].contiguous())
+    plan.set_option(_capi.OPT_PHYS_STEPS, -1)
+    for seg in reversed(range(len(ckpts))):
+        start = seg * K
+        n = min(K, cfg.nsteps - start)
+        plan.state_set(ckpts[seg])
+        tape = [ckpts[seg]]
+        for s in range(n - 1):
+            plan.run(cfg.t0 + start + s, 1)
+            tape.append(plan.state_get())
+        for s in reversed(range(n)):
+            plan.adjoint_step(tape[s])
+            if start + s in where:
+                plan.adjoint_observe(g_Phi=gPhi[:, where[start + s]].contiguous())
+    plan.set_option(_capi.OPT_PHYS_STEPS, 1)
+    cfg.stats.update(checkpoints=len(ckpts))
+    return plan.adjoint_result()
+
+
 def _fold(g, shape):
     """Per-member gradient g (B,) folded onto the parameter's shape (a shared scalar sums members)."""
     if len(shape) == 0:
@@ -189,6 +240,44 @@ class _Rollout:
         return Fn
 
 
+class _RolloutSeries:
+    """torch.autograd.Function factory of the time-series rollout (torch imported lazily)."""
+
+    _cls = None
+
+    @classmethod
+    def get(cls):
+        if cls._cls is not None:
+            return cls._cls
+        torch = require_gpu()
+
+        class Fn(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, c, taurad, taudrag, DPhieq, cfg: RolloutConfig, obs_steps):
+                params = _params(cfg, taurad.detach().cpu().numpy(), taudrag.detach().cpu().numpy(),
+                                 DPhieq.detach().cpu().numpy())
+                plan = _make_plan(cfg, params)
+                ckpts, phis = forward_series(plan, cfg, c.detach(), obs_steps)
+                torch.cuda.current_stream(plan.device).synchronize()
+                ctx.plan, ctx.cfg, ctx.ckpts, ctx.obs = plan, cfg, ckpts, obs_steps
+                ctx.shapes = (taurad.shape, taudrag.shape, DPhieq.shape)
+                return phis
+
+            @staticmethod
+            def backward(ctx, gPhi):
+                plan, cfg = ctx.plan, ctx.cfg
+                lam, pg = adjoint_series(plan, cfg, ctx.ckpts, ctx.obs, gPhi.contiguous())
+                torch.cuda.current_stream(plan.device).synchronize()
+                _release_plan(cfg, plan)
+                ctx.plan = None
+                ctx.ckpts = None
+                sr, sd, sp = ctx.shapes
+                return lam, _fold(pg[:, 0], sr), _fold(pg[:, 1], sd), _fold(pg[:, 2], sp), None, None
+
+        cls._cls = Fn
+        return Fn
+
+
 def rollout_phi(c, taurad, taudrag, DPhieq, *, M: int, nsteps: int, dt: float, a: float, omega: float,
                 Phibar: float, K6: float = 1.24e33, K6Phi: Optional[float] = None, alpha: float = 0.01,
                 precision: str = "fp64", checkpoint_every: Optional[int] = None, diffflag: bool = True,
@@ -205,3 +294,24 @@ def rollout_phi(c, taurad, taudrag, DPhieq, *, M: int, nsteps: int, dt: float, a
                         modalflag=modalflag, forcflag=forcflag, stats=stats if stats is not None else {})
     conv = lambda x: x if torch.is_tensor(x) else torch.tensor(x, dtype=torch.float64, device=dev)  # noqa: E731
     return _Rollout.get().apply(c, conv(taurad), conv(taudrag), conv(DPhieq), cfg)
+
+
+def rollout_phi_series(c, taurad, taudrag, DPhieq, *, obs_steps, M: int, dt: float, a: float, omega: float,
+                       Phibar: float, K6: float = 1.24e33, K6Phi: Optional[float] = None, alpha: float = 0.01,
+                       precision: str = "fp64", checkpoint_every: Optional[int] = None, diffflag: bool = True,
+                       modalflag: bool = True, forcflag: bool = True, stats: Optional[dict] = None):
+    """Phi grids (B, len(obs_steps), J, I) after each step count in obs_steps (ascending, >= 0) of one
+    differentiable rollout of max(obs_steps) steps: a loss over the whole time series (a phase-curve
+    likelihood) gets its gradient from one reverse sweep whose observation cotangents are added
+    as the sweep passes them (swx_adjoint_observe), with the checkpoints of rollout_phi.
+    """
+    torch = require_gpu()
+    steps = [int(n) for n in obs_steps]
+    if not steps or any(n < 0 for n in steps) or any(b <= a_ for a_, b in zip(steps, steps[1:])):
+        raise ValueError("obs_steps must be a non-empty strictly increasing sequence of step counts >= 0")
+    dev = torch.device("cuda", torch.cuda.current_device())
+    cfg = RolloutConfig(M=M, nsteps=steps[-1], dt=dt, a=a, omega=omega, Phibar=Phibar, K6=K6, K6Phi=K6Phi,
+                        alpha=alpha, precision=precision, checkpoint_every=checkpoint_every, diffflag=diffflag,
+                        modalflag=modalflag, forcflag=forcflag, stats=stats if stats is not None else {})
+    conv = lambda x: x if torch.is_tensor(x) else torch.tensor(x, dtype=torch.float64, device=dev)  # noqa: E731
+    return _RolloutSeries.get().apply(c, conv(taurad), conv(taudrag), conv(DPhieq), cfg, tuple(steps))

@@ -571,9 +571,10 @@ GridAdjArgs<R> grid_adj_args(swx_plan* p, int mode) {
   return a;
 }
 
+// accumulate = 0: Lam = invLeg^T(DFT^T(g)) (terminal); 1: Lam += (observation mid-sweep).
 template <typename R>
 int adjoint_terminal(swx_plan* p, const double* ge, const double* gd, const double* gp, const double* gu,
-                     const double* gv, hipStream_t s) {
+                     const double* gv, hipStream_t s, int accumulate = 0) {
   GridAdjArgs<R> g = grid_adj_args<R>(p, GADJ_TERM);
   g.g_eta = ge;
   g.g_delta = gd;
@@ -581,7 +582,7 @@ int adjoint_terminal(swx_plan* p, const double* ge, const double* gd, const doub
   g.g_U = gu;
   g.g_V = gv;
   SWX_HIP(launch_grid_adj<R>(g, s));
-  SWX_HIP(launch_leg_adj_any<R>(p, leg_adj_args<R>(p, LADJ_TERM), s));
+  SWX_HIP(launch_leg_adj_any<R>(p, leg_adj_args<R>(p, accumulate ? LADJ_B : LADJ_TERM), s));
   return SWX_OK;
 }
 
@@ -1523,6 +1524,16 @@ int swx_adjoint_terminal(swx_plan* p, const double* g_eta, const double* g_delta
   hipStream_t s = (hipStream_t)stream;
   return p->rsz == 8 ? adjoint_terminal<double>(p, g_eta, g_delta, g_Phi, g_U, g_V, s)
                      : adjoint_terminal<float>(p, g_eta, g_delta, g_Phi, g_U, g_V, s);
+}
+
+int swx_adjoint_observe(swx_plan* p, const double* g_eta, const double* g_delta, const double* g_Phi,
+                        const double* g_U, const double* g_V, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!p->Lam) return fail(SWX_ERR_STATE, "swx_adjoint_observe before swx_adjoint_begin");
+  hipStream_t s = (hipStream_t)stream;
+  return p->rsz == 8 ? adjoint_terminal<double>(p, g_eta, g_delta, g_Phi, g_U, g_V, s, 1)
+                     : adjoint_terminal<float>(p, g_eta, g_delta, g_Phi, g_U, g_V, s, 1);
 }
 
 int swx_adjoint_step(swx_plan* p, const double* state_t, void* stream) {

@@ -186,6 +186,14 @@ class Plan:
         _capi.check(self._lib.swx_adjoint_terminal(self._h, *[_ptr(f) for f in fields], self._stream(stream)),
                     "swx_adjoint_terminal")
 
+    def adjoint_observe(self, g_eta=None, g_delta=None, g_Phi=None, g_U=None, g_V=None, stream=None) -> None:
+        """Add the cotangents of the grid fields observed at the sweep's current state."""
+        fields = [self._field(g, n) for g, n in ((g_eta, "g_eta"), (g_delta, "g_delta"), (g_Phi, "g_Phi"),
+                                                  (g_U, "g_U"), (g_V, "g_V"))]
+        self._keep_obs = fields
+        _capi.check(self._lib.swx_adjoint_observe(self._h, *[_ptr(f) for f in fields], self._stream(stream)),
+                    "swx_adjoint_observe")
+
     def adjoint_step(self, state_t, stream=None) -> None:
         t = self._spec(state_t)
         self._keep_adj_state = t

@@ -207,3 +207,45 @@ def test_mfma_adjoint_matches_plain_fp32(kernel, M, torch_gpu):
     print(kernel, M, "dL/dS0 relL2", rl, "param grad relL2", rp)
     assert np.isfinite(out["mfma"][0]).all() and np.isfinite(out["mfma"][1]).all()
     assert rl <= 1e-5 and rp <= 1e-5, (rl, rp)
+
+
+def test_time_series_rollout_gradient_is_the_sum_of_terminal_gradients(torch_gpu):
+    """rollout_phi_series (observation cotangents added mid-sweep by swx_adjoint_observe) vs one
+    rollout per observed step: the observed fields are identical, and the gradient of a loss
+    summed over the series equals the sum of the single-observation gradients (1e-10 relative,
+    FP64; the terminal gradients are the ones pinned by finite differences above)."""
+    from my_swampe_amd.autograd import rollout_phi, rollout_phi_series
+
+    torch = torch_gpu
+    print("[gpu-case] adjoint time series", flush=True)
+    dev = torch.device("cuda")
+    obs = (0, 2, 5, 8)
+    rng = np.random.default_rng(21)
+    tgts = torch.tensor(2e3 * rng.standard_normal((len(obs), 64, 128)), device=dev)
+    c0 = torch.tensor(np.stack([_ic(seed=5), _ic(seed=6)]), device=dev)
+    cfg = {k: v for k, v in CFG.items() if k != "nsteps"}
+
+    def leaves():
+        return (c0.clone().requires_grad_(True),) + tuple(
+            torch.tensor(x, device=dev, requires_grad=True) for x in (TAURAD, TAUDRAG, DPHIEQ))
+
+    x = leaves()
+    phis = rollout_phi_series(*x, obs_steps=obs, checkpoint_every=3, **cfg)
+    sum(_loss(phis[:, k], tgts[k]) for k in range(len(obs))).backward()
+    series = [t.grad.detach().clone() for t in x]
+
+    total = [torch.zeros_like(t) for t in series]
+    for k, n in enumerate(obs):
+        y = leaves()
+        if n > 0:
+            phi = rollout_phi(*y, nsteps=n, checkpoint_every=3, **cfg)
+        else:  # the initial field: a zero-step series
+            phi = rollout_phi_series(*y, obs_steps=(0,), **cfg)[:, 0]
+        assert torch.equal(phi.detach(), phis[:, k].detach()), n
+        _loss(phi, tgts[k]).backward()
+        for acc, t in zip(total, y):
+            if t.grad is not None:
+                acc += t.grad
+    for i, (a, b) in enumerate(zip(series, total)):
+        a, b = a.cpu().numpy(), b.cpu().numpy()
+        np.testing.assert_allclose(a, b, rtol=0, atol=1e-10 * np.abs(b).max(), err_msg=str(i))
