@@ -234,6 +234,28 @@ int swx_adjoint_terminal(swx_plan *plan, const double *g_eta, const double *g_de
 int swx_adjoint_step(swx_plan *plan, const double *state_t, void *stream);
 int swx_adjoint_result(swx_plan *plan, double *lam_out, double *pgrad_out, void *stream);
 
+/* Phase-curve observable (replaces the projection of
+ * unit_tests/test_parity_reference_regression.py:67-82, the retrieval's
+ * visibility-weighted disc integral of Phi):
+ *   flux[b][k] = sum_j sqrt(max(1 - mu_j^2, 0)) sum_i Phi_b[j][i] max(cos(lambda_i - phases[k]), 0)
+ * with lambda_i = -pi + 2 pi i / I, then divided by mean_k |flux[b][k]| when
+ * normalize != 0 and that mean is > 0.  Phi_b = Phi + b * member_stride, b < n,
+ * each a row-major J x I grid; mus: the J Gauss nodes; phases: K angles;
+ * flux: (n, K); flux_raw: (n, K) before normalization, or null.  All device
+ * pointers.  1 <= J, I <= 1024, 1 <= K <= 2048; n = 0 does nothing. */
+int swx_phase_curve(const double *Phi, long long member_stride, int n, int J, int I, const double *mus,
+                    const double *phases, int K, int normalize, double *flux, double *flux_raw, void *stream);
+/* Vector-Jacobian product of swx_phase_curve: g_flux (n, K), the cotangent of
+ * flux, and flux_raw from the forward call -> g_Phi (n, J, I), dense,
+ * overwritten.  No counterpart in the reference's code (jax.grad
+ * differentiates the projection by itself). */
+int swx_phase_curve_vjp(const double *g_flux, const double *flux_raw, int n, int J, int I, const double *mus,
+                        const double *phases, int K, int normalize, double *g_Phi, void *stream);
+/* swx_phase_curve of every member's current Phi grid (the Phi_curr that
+ * swx_export_state returns) with the plan's Gauss nodes: flux (B, K). */
+int swx_plan_phase_curve(swx_plan *plan, const double *phases, int K, int normalize, double *flux,
+                         double *flux_raw, void *stream);
+
 /* Plan options (swx_plan_set_option).
  *   SWX_OPT_GRID_KERNEL: 0 = automatic (MFMA grid kernel for FP32 forward
  *     stepping when it fits), 1 = always the plain grid kernel.

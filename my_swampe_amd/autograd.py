@@ -140,39 +140,36 @@ def adjoint_through_checkpoints(plan: Plan, cfg: RolloutConfig, ckpts, gPhi):
     return plan.adjoint_result()
 
 
-def forward_series(plan: Plan, cfg: RolloutConfig, c, obs_steps):
-    """Run the rollout from c, returning the checkpoints and Phi after each step count in obs_steps."""
-    torch = require_gpu()
+def _forward_observed(plan: Plan, cfg: RolloutConfig, c, obs_steps, observe):
+    """Run the rollout from c, calling observe(k) when the state has advanced obs_steps[k] steps;
+    returns the checkpoints (spectral states before each segment)."""
     K = cfg.every()
     plan.init_spectral(c)
     ckpts: List = []
-    obs = []
-    pending = list(obs_steps)
-    if pending and pending[0] == 0:
-        obs.append(plan.export(fields=("Phi_curr",))["Phi_curr"])
-        pending.pop(0)
+    pending = list(enumerate(obs_steps))
+    if pending and pending[0][1] == 0:
+        observe(pending.pop(0)[0])
     done = 0
     while done < cfg.nsteps:
         seg_end = min(done + K, cfg.nsteps)
         ckpts.append(plan.state_get())
         while done < seg_end:
-            stop = min(seg_end, pending[0]) if pending else seg_end
+            stop = min(seg_end, pending[0][1]) if pending else seg_end
             plan.run(cfg.t0 + done, stop - done)
             done = stop
-            if pending and pending[0] == done:
-                obs.append(plan.export(fields=("Phi_curr",))["Phi_curr"])
-                pending.pop(0)
-    return ckpts, torch.stack(obs, dim=1)
+            if pending and pending[0][1] == done:
+                observe(pending.pop(0)[0])
+    return ckpts
 
 
-def adjoint_series(plan: Plan, cfg: RolloutConfig, ckpts, obs_steps, gPhi):
-    """Reverse sweep with the cotangents gPhi (B, n_obs, J, I) of Phi after each observed step count
-    added when the sweep reaches that state."""
+def _adjoint_observed(plan: Plan, cfg: RolloutConfig, ckpts, obs_steps, cotangent):
+    """Reverse sweep whose observation k adds cotangent(k) (the (B, J, I) cotangent of Phi after
+    obs_steps[k] steps) when the sweep reaches that state."""
     K = cfg.every()
     where = {int(n): k for k, n in enumerate(obs_steps)}
     plan.adjoint_begin()
     if cfg.nsteps in where:
-        plan.adjoint_terminal(g_Phi=gPhi[:, where[cfg.nsteps]].contiguous())
+        plan.adjoint_terminal(g_Phi=cotangent(where[cfg.nsteps]))
     plan.set_option(_capi.OPT_PHYS_STEPS, -1)
     for seg in reversed(range(len(ckpts))):
         start = seg * K
@@ -185,10 +182,61 @@ def adjoint_series(plan: Plan, cfg: RolloutConfig, ckpts, obs_steps, gPhi):
         for s in reversed(range(n)):
             plan.adjoint_step(tape[s])
             if start + s in where:
-                plan.adjoint_observe(g_Phi=gPhi[:, where[start + s]].contiguous())
+                plan.adjoint_observe(g_Phi=cotangent(where[start + s]))
     plan.set_option(_capi.OPT_PHYS_STEPS, 1)
     cfg.stats.update(checkpoints=len(ckpts))
     return plan.adjoint_result()
+
+
+def forward_series(plan: Plan, cfg: RolloutConfig, c, obs_steps):
+    """Run the rollout from c, returning the checkpoints and Phi after each step count in obs_steps."""
+    torch = require_gpu()
+    obs = []
+    ckpts = _forward_observed(plan, cfg, c, obs_steps,
+                              lambda k: obs.append(plan.export(fields=("Phi_curr",))["Phi_curr"]))
+    return ckpts, torch.stack(obs, dim=1)
+
+
+def adjoint_series(plan: Plan, cfg: RolloutConfig, ckpts, obs_steps, gPhi):
+    """Reverse sweep with the cotangents gPhi (B, n_obs, J, I) of Phi after each observed step count
+    added when the sweep reaches that state."""
+    return _adjoint_observed(plan, cfg, ckpts, obs_steps, lambda k: gPhi[:, k].contiguous())
+
+
+def forward_phase_curve(plan: Plan, cfg: RolloutConfig, c, obs_steps, phases, normalize: bool):
+    """Run the rollout from c, keeping only the phase curves of Phi at the observed step counts:
+    returns the checkpoints, the curves (B, n_obs, K) and the un-normalized curves (n_obs, B, K).
+    phases: float64 device tensor (n_obs, K)."""
+    torch = require_gpu()
+    n_obs, K = phases.shape
+    flux = torch.empty((n_obs, plan.B, K), dtype=torch.float64, device=plan.device)
+    raw = torch.empty_like(flux)
+
+    def observe(k):
+        f, r = plan.phase_curve(phases[k], normalize)
+        flux[k].copy_(f)
+        raw[k].copy_(r)
+
+    ckpts = _forward_observed(plan, cfg, c, obs_steps, observe)
+    return ckpts, flux.permute(1, 0, 2).contiguous(), raw
+
+
+def adjoint_phase_curve(plan: Plan, cfg: RolloutConfig, ckpts, obs_steps, phases, normalize: bool, raw, g_flux):
+    """Reverse sweep of a loss of the phase curves: g_flux (B, n_obs, K); each observation's grid
+    cotangent is formed on the GPU (swx_phase_curve_vjp) when the sweep reaches it."""
+    from .observables import phase_curve_vjp
+
+    mus = plan.mus_device()
+    return _adjoint_observed(
+        plan, cfg, ckpts, obs_steps,
+        lambda k: phase_curve_vjp(g_flux[:, k], raw[k], phases[k], plan.J, plan.I, normalize, mus))
+
+
+def _check_steps(obs_steps):
+    steps = [int(n) for n in obs_steps]
+    if not steps or any(n < 0 for n in steps) or any(b <= a_ for a_, b in zip(steps, steps[1:])):
+        raise ValueError("obs_steps must be a non-empty strictly increasing sequence of step counts >= 0")
+    return steps
 
 
 def _fold(g, shape):
@@ -278,6 +326,47 @@ class _RolloutSeries:
         return Fn
 
 
+class _RolloutPhaseCurve:
+    """torch.autograd.Function factory of the phase-curve rollout (torch imported lazily)."""
+
+    _cls = None
+
+    @classmethod
+    def get(cls):
+        if cls._cls is not None:
+            return cls._cls
+        torch = require_gpu()
+
+        class Fn(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, c, taurad, taudrag, DPhieq, cfg: RolloutConfig, obs_steps, phases, normalize):
+                params = _params(cfg, taurad.detach().cpu().numpy(), taudrag.detach().cpu().numpy(),
+                                 DPhieq.detach().cpu().numpy())
+                plan = _make_plan(cfg, params)
+                ckpts, flux, raw = forward_phase_curve(plan, cfg, c.detach(), obs_steps, phases, normalize)
+                torch.cuda.current_stream(plan.device).synchronize()
+                ctx.plan, ctx.cfg, ctx.ckpts, ctx.obs = plan, cfg, ckpts, obs_steps
+                ctx.phases, ctx.normalize, ctx.raw = phases, normalize, raw
+                ctx.shapes = (taurad.shape, taudrag.shape, DPhieq.shape)
+                return flux
+
+            @staticmethod
+            def backward(ctx, g_flux):
+                plan, cfg = ctx.plan, ctx.cfg
+                lam, pg = adjoint_phase_curve(plan, cfg, ctx.ckpts, ctx.obs, ctx.phases, ctx.normalize, ctx.raw,
+                                              g_flux.contiguous())
+                torch.cuda.current_stream(plan.device).synchronize()
+                _release_plan(cfg, plan)
+                ctx.plan = None
+                ctx.ckpts = None
+                sr, sd, sp = ctx.shapes
+                return (lam, _fold(pg[:, 0], sr), _fold(pg[:, 1], sd), _fold(pg[:, 2], sp), None, None, None,
+                        None)
+
+        cls._cls = Fn
+        return Fn
+
+
 def rollout_phi(c, taurad, taudrag, DPhieq, *, M: int, nsteps: int, dt: float, a: float, omega: float,
                 Phibar: float, K6: float = 1.24e33, K6Phi: Optional[float] = None, alpha: float = 0.01,
                 precision: str = "fp64", checkpoint_every: Optional[int] = None, diffflag: bool = True,
@@ -306,12 +395,39 @@ def rollout_phi_series(c, taurad, taudrag, DPhieq, *, obs_steps, M: int, dt: flo
     as the sweep passes them (swx_adjoint_observe), with the checkpoints of rollout_phi.
     """
     torch = require_gpu()
-    steps = [int(n) for n in obs_steps]
-    if not steps or any(n < 0 for n in steps) or any(b <= a_ for a_, b in zip(steps, steps[1:])):
-        raise ValueError("obs_steps must be a non-empty strictly increasing sequence of step counts >= 0")
+    steps = _check_steps(obs_steps)
     dev = torch.device("cuda", torch.cuda.current_device())
     cfg = RolloutConfig(M=M, nsteps=steps[-1], dt=dt, a=a, omega=omega, Phibar=Phibar, K6=K6, K6Phi=K6Phi,
                         alpha=alpha, precision=precision, checkpoint_every=checkpoint_every, diffflag=diffflag,
                         modalflag=modalflag, forcflag=forcflag, stats=stats if stats is not None else {})
     conv = lambda x: x if torch.is_tensor(x) else torch.tensor(x, dtype=torch.float64, device=dev)  # noqa: E731
     return _RolloutSeries.get().apply(c, conv(taurad), conv(taudrag), conv(DPhieq), cfg, tuple(steps))
+
+
+def rollout_phase_curve(c, taurad, taudrag, DPhieq, *, obs_steps, phase_angles, normalize: bool = True, M: int,
+                        dt: float, a: float, omega: float, Phibar: float, K6: float = 1.24e33,
+                        K6Phi: Optional[float] = None, alpha: float = 0.01, precision: str = "fp64",
+                        checkpoint_every: Optional[int] = None, diffflag: bool = True, modalflag: bool = True,
+                        forcflag: bool = True, stats: Optional[dict] = None):
+    """Phase curves (B, len(obs_steps), K) of Phi after each step count in obs_steps of one
+    differentiable rollout: the observable of the phase-curve likelihood evaluated inside the
+    rollout, so that neither pass keeps a grid per observation (the forward keeps the curves, the
+    reverse sweep forms each observation's grid cotangent on the GPU when it reaches it).
+
+    phase_angles: (K,) angles used at every observation, or (len(obs_steps), K) per observation.
+    The projection is observables.phase_curve's with the plan's Gauss nodes.
+    """
+    torch = require_gpu()
+    steps = _check_steps(obs_steps)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    ph = torch.as_tensor(phase_angles, dtype=torch.float64, device=dev)
+    if ph.dim() == 1:
+        ph = ph.expand(len(steps), -1)
+    if ph.dim() != 2 or ph.shape[0] != len(steps) or ph.shape[1] == 0:
+        raise ValueError("phase_angles must have shape (K,) or (len(obs_steps), K) with K >= 1")
+    cfg = RolloutConfig(M=M, nsteps=steps[-1], dt=dt, a=a, omega=omega, Phibar=Phibar, K6=K6, K6Phi=K6Phi,
+                        alpha=alpha, precision=precision, checkpoint_every=checkpoint_every, diffflag=diffflag,
+                        modalflag=modalflag, forcflag=forcflag, stats=stats if stats is not None else {})
+    conv = lambda x: x if torch.is_tensor(x) else torch.tensor(x, dtype=torch.float64, device=dev)  # noqa: E731
+    return _RolloutPhaseCurve.get().apply(c, conv(taurad), conv(taudrag), conv(DPhieq), cfg, tuple(steps),
+                                          ph.contiguous(), bool(normalize))

@@ -19,6 +19,7 @@
 #include "geometry.h"
 #include "swx_adjoint.h"
 #include "swx_kernels.h"
+#include "swx_observe.h"
 
 using namespace swx;
 
@@ -1656,6 +1657,50 @@ int swx_export_state(swx_plan* p, const swx_state_out* o, void* stream) {
   }
   if (o->dead) SWX_HIP(hipMemcpyAsync(o->dead, p->dead, (size_t)B, hipMemcpyDeviceToDevice, s));
   return SWX_OK;
+}
+
+static int phase_curve_dims(int n, int J, int I, int K) {
+  if (n < 0) return fail(SWX_ERR_ARG, "phase curve: negative grid count");
+  if (J < 1 || I < 1 || J > kPhaseCurveMaxI || I > kPhaseCurveMaxI)
+    return fail(SWX_ERR_ARG, "phase curve: grid J x I = " + std::to_string(J) + " x " + std::to_string(I) +
+                                 " outside 1.." + std::to_string(kPhaseCurveMaxI));
+  if (K < 1 || K > kPhaseCurveMaxK)
+    return fail(SWX_ERR_ARG, "phase curve: " + std::to_string(K) + " phases outside 1.." +
+                                 std::to_string(kPhaseCurveMaxK));
+  return SWX_OK;
+}
+
+int swx_phase_curve(const double* Phi, long long member_stride, int n, int J, int I, const double* mus,
+                    const double* phases, int K, int normalize, double* flux, double* flux_raw, void* stream) {
+  int rc = phase_curve_dims(n, J, I, K);
+  if (rc) return rc;
+  if (n == 0) return SWX_OK;
+  if (!Phi || !mus || !phases || !flux) return fail(SWX_ERR_ARG, "phase curve: null pointer");
+  if (member_stride < (long long)J * I) return fail(SWX_ERR_ARG, "phase curve: member stride below J * I");
+  SWX_HIP(launch_phase_curve(Phi, member_stride, n, J, I, mus, phases, K, normalize, flux, flux_raw,
+                             (hipStream_t)stream));
+  return SWX_OK;
+}
+
+int swx_phase_curve_vjp(const double* g_flux, const double* flux_raw, int n, int J, int I, const double* mus,
+                        const double* phases, int K, int normalize, double* g_Phi, void* stream) {
+  int rc = phase_curve_dims(n, J, I, K);
+  if (rc) return rc;
+  if (n == 0) return SWX_OK;
+  if (!g_flux || !flux_raw || !mus || !phases || !g_Phi) return fail(SWX_ERR_ARG, "phase curve vjp: null pointer");
+  SWX_HIP(launch_phase_curve_vjp(g_flux, flux_raw, n, J, I, mus, phases, K, normalize, g_Phi, (hipStream_t)stream));
+  return SWX_OK;
+}
+
+int swx_plan_phase_curve(swx_plan* p, const double* phases, int K, int normalize, double* flux, double* flux_raw,
+                         void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!p->initialized) return fail(SWX_ERR_STATE, "phase curve before swx_init");
+  const int J = p->geo.J, I = p->geo.I;
+  // Physical plane 5 of [B][8][J][I] is Phi_curr (the grid swx_export_state returns).
+  return swx_phase_curve(p->phys + 5LL * J * I, 8LL * J * I, p->d.n_members, J, I, p->geo_d + p->gl.mus, phases,
+                         K, normalize, flux, flux_raw, stream);
 }
 
 }  // extern "C"

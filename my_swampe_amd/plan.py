@@ -79,6 +79,8 @@ class Plan:
                                         ctypes.byref(handle)), "swx_plan_create")
         self._h = handle
         self._lib = lib
+        self._quad_mus = None if quad is None else quad[0]
+        self._mus_d = None
 
     # -- lifetime ---------------------------------------------------------
     def close(self) -> None:
@@ -207,6 +209,28 @@ class Plan:
         _capi.check(self._lib.swx_adjoint_result(self._h, _ptr(lam), _ptr(pg), self._stream(stream)),
                     "swx_adjoint_result")
         return lam, pg
+
+    def mus_device(self):
+        """The plan's Gauss nodes (J,) as a float64 device tensor."""
+        if self._mus_d is None:
+            torch = _torch()
+            mus = self._quad_mus if self._quad_mus is not None else _capi.geometry(self.M)[0]
+            self._mus_d = torch.as_tensor(np.ascontiguousarray(mus, dtype=np.float64), device=self.device)
+        return self._mus_d
+
+    def phase_curve(self, phases, normalize: bool = True, stream=None):
+        """(flux, flux_raw), each float64 (B, K): the phase curve of every member's current Phi
+        grid (the exported Phi_curr) at the K phase angles, normalized by mean|flux| per member
+        when normalize is set (flux_raw: before normalization)."""
+        torch = _torch()
+        ph = torch.as_tensor(phases, dtype=torch.float64, device=self.device).reshape(-1).contiguous()
+        K = ph.numel()
+        flux = torch.empty((self.B, K), dtype=torch.float64, device=self.device)
+        raw = torch.empty((self.B, K), dtype=torch.float64, device=self.device)
+        _capi.check(self._lib.swx_plan_phase_curve(self._h, _ptr(ph), K, int(bool(normalize)), _ptr(flux), _ptr(raw),
+                                                   self._stream(stream)), "swx_plan_phase_curve")
+        self._keep_pc = ph
+        return flux, raw
 
     def get_option(self, option: int) -> int:
         v = ctypes.c_int()

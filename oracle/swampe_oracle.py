@@ -29,6 +29,9 @@ of its own formula):
   fixtures (terminal fields, snapshots, spinup/geopot diagnostics and the
   phase-curve projection), the quirk formulas of
   unit_tests/test_parity_quirks.py, and member-independence.
+* phase_curve / phase_curve_vjp: the fixtures' phase curves, the closed form of a
+  uniform grid at grid-aligned phases (cot(pi/I) sum_j sqrt(1 - mu_j^2)), the
+  longitude-roll shift property, and central differences of phase_curve for the VJP.
 * explicit scheme (expflag=True): only the delta carry-only quirk
   (test_parity_quirks.py:85-135) pins it; the rest is "parity unpinned"
   beyond this restatement (no reference fixture exists for it).
@@ -47,7 +50,7 @@ __all__ = [
     "build_lambdas", "gauss_legendre", "scaling_table", "pmn_hmn", "spectral_params",
     "fwd_fft_trunc", "invrs_fft", "fwd_leg", "invrs_leg", "invrs_uv", "diagnostic_eta_delta",
     "Static", "OracleState", "build_static", "init_state_from_fields", "analytic_ic",
-    "diagnose_winds", "step_once", "run_model_scan", "run_model", "rms_winds",
+    "diagnose_winds", "step_once", "run_model_scan", "run_model", "rms_winds", "phase_curve", "phase_curve_vjp",
 ]
 
 # ----------------------------------------------------------------------------
@@ -824,3 +827,55 @@ def rest_noise_ic(M: int, seed: int, omega: float, phi_std: float = 100.0, nmax:
     eta0 = np.broadcast_to(2.0 * omega * mus[:, None], (J, I)).copy()
     z = np.zeros((J, I))
     return eta0, z.copy(), Phi0, z.copy(), z.copy()
+
+
+# ----------------------------------------------------------------------------
+# Phase-curve observable (unit_tests/test_parity_reference_regression.py:67-82)
+# ----------------------------------------------------------------------------
+
+
+def phase_curve(phi, phase_angles, normalize: bool = True, raw: bool = False):
+    """Visibility-weighted disc integral of a (J, I) geopotential grid per phase angle
+    (unit_tests/test_parity_reference_regression.py:67-82): longitudes linspace(-pi, pi, I,
+    endpoint=False), nodes numpy leggauss(J), weight sqrt(clip(1 - mu^2, 0)), visibility
+    max(cos(lambda - phase), 0), summed over the whole grid per phase; then divided by
+    mean|flux| when that is > 0.  raw=True also returns the flux before normalization."""
+    phi = np.asarray(phi, dtype=np.float64)
+    phase_angles = np.asarray(phase_angles, dtype=np.float64)
+    J, I = phi.shape
+    lambdas = np.linspace(-np.pi, np.pi, I, endpoint=False)[None, :]
+    mus = np.polynomial.legendre.leggauss(J)[0][:, None]
+    area_weight = np.sqrt(np.clip(1.0 - mus * mus, 0.0, None))
+    flux = np.empty_like(phase_angles, dtype=np.float64)
+    for k, phase in enumerate(phase_angles):
+        visible = np.maximum(np.cos(lambdas - phase), 0.0)
+        flux[k] = float(np.sum(phi * visible * area_weight))
+    out = flux
+    if normalize:
+        norm = np.mean(np.abs(flux))
+        if norm > 0:
+            out = flux / norm
+    return (out, flux) if raw else out
+
+
+def phase_curve_vjp(g_flux, flux_raw, J: int, I: int, phase_angles, normalize: bool = True):
+    """Cotangent of the (J, I) grid from the cotangent g_flux of phase_curve's output.
+
+    With n = mean|f| and output f/n: df_k = g_k/n - sign(f_k) (sum_l g_l f_l)/(K n^2); the grid
+    cotangent is sqrt(1 - mu_j^2) sum_k df_k max(cos(lambda_i - phase_k), 0).  (The derivative
+    of the reference's projection under reverse-mode differentiation; no reference counterpart.)
+    """
+    g = np.asarray(g_flux, dtype=np.float64)
+    f = np.asarray(flux_raw, dtype=np.float64)
+    phases = np.asarray(phase_angles, dtype=np.float64)
+    K = f.size
+    gf = g.copy()
+    if normalize:
+        n = np.mean(np.abs(f))
+        if n > 0:
+            gf = g / n - np.sign(f) * float(np.dot(g, f)) / (K * n * n)
+    lambdas = np.linspace(-np.pi, np.pi, I, endpoint=False)
+    mus = np.polynomial.legendre.leggauss(J)[0]
+    area_weight = np.sqrt(np.clip(1.0 - mus * mus, 0.0, None))
+    vis = np.maximum(np.cos(lambdas[None, :] - phases[:, None]), 0.0)  # (K, I)
+    return area_weight[:, None] * (gf @ vis)[None, :]

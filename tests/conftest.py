@@ -35,18 +35,9 @@ def pytest_configure(config):
 
 def phase_curve_from_phi(phi, phase_angles):
     """Retrieval phase-curve projection (unit_tests/test_parity_reference_regression.py:67-82)."""
-    import numpy as np
+    from oracle.swampe_oracle import phase_curve
 
-    J, I = phi.shape
-    lambdas = np.linspace(-np.pi, np.pi, I, endpoint=False)[None, :]
-    mus = np.polynomial.legendre.leggauss(J)[0][:, None]
-    area_weight = np.sqrt(np.clip(1.0 - mus * mus, 0.0, None))
-    flux = np.empty_like(phase_angles, dtype=np.float64)
-    for k, phase in enumerate(phase_angles):
-        visible = np.maximum(np.cos(lambdas - phase), 0.0)
-        flux[k] = float(np.sum(phi * visible * area_weight))
-    norm = np.mean(np.abs(flux))
-    return flux / norm if norm > 0 else flux
+    return phase_curve(phi, phase_angles)
 
 
 @pytest.fixture(scope="session")
