@@ -139,6 +139,16 @@ void swx_plan_destroy(swx_plan *plan);
 int swx_init(swx_plan *plan, const double *eta0, const double *delta0, const double *Phi0,
              const double *U0, const double *V0, void *stream);
 
+/* After swx_init from a carry's current-level grids (eta, delta, Phi, U, V
+ * _curr): set the physical prev level of eta, delta, Phi (device [B][J][I]
+ * each, all three or none) and the per-member dead flags (device [B] bytes,
+ * or NULL), so that the plan holds a mid-run carry of the modified-Euler
+ * scheme, whose step reads no other prev-level field.  Replaces passing a
+ * State pytree into _step_once / simulate_scan(_last) (model.py:795-803,
+ * :1237-1342); other schemes return SWX_ERR_ARG. */
+int swx_set_prev_level(swx_plan *plan, const double *eta_prev, const double *delta_prev, const double *Phi_prev,
+                       const unsigned char *dead, void *stream);
+
 /* Advance every member by nsteps steps with step indices t0, t0+1, ...
  * Replaces the lax.scan of simulate_scan / simulate_scan_last
  * (model.py:1237-1342) over _step_once (model.py:795-1194).

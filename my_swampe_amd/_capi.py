@@ -37,7 +37,7 @@ EXPORTED_SYMBOLS = (
     "swx_geometry", "swx_grid_size", "swx_plan_create", "swx_plan_set_params", "swx_plan_destroy",
     "swx_init", "swx_run", "swx_export_state", "swx_run_graph", "swx_time_kernels", "swx_plan_set_option",
     "swx_init_spectral", "swx_state_get", "swx_state_set", "swx_adjoint_begin", "swx_adjoint_terminal", "swx_adjoint_observe",
-    "swx_phase_curve", "swx_phase_curve_vjp", "swx_plan_phase_curve",
+    "swx_phase_curve", "swx_phase_curve_vjp", "swx_plan_phase_curve", "swx_set_prev_level",
     "swx_adjoint_step", "swx_adjoint_result", "swx_last_error", "swx_version", "swx_plan_device_bytes",
     "swx_ensemble_create", "swx_ensemble_run_graph", "swx_ensemble_destroy", "swx_plan_get_option",
 )
@@ -95,6 +95,7 @@ def load() -> ctypes.CDLL:
     lib.swx_plan_destroy.argtypes = [c_void_p]
     lib.swx_plan_destroy.restype = None
     lib.swx_init.argtypes = [c_void_p] + [c_void_p] * 5 + [c_void_p]
+    lib.swx_set_prev_level.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
     lib.swx_run.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]
     lib.swx_export_state.argtypes = [c_void_p, POINTER(StateOut), c_void_p]
     lib.swx_run_graph.argtypes = [c_void_p, c_int, c_int, c_int, c_void_p]
@@ -126,7 +127,7 @@ def load() -> ctypes.CDLL:
                  "swx_init_spectral", "swx_state_get", "swx_state_set", "swx_adjoint_begin",
                  "swx_adjoint_terminal", "swx_adjoint_step", "swx_adjoint_result", "swx_plan_device_bytes",
                  "swx_ensemble_create", "swx_ensemble_run_graph", "swx_plan_get_option", "swx_adjoint_observe",
-                 "swx_phase_curve", "swx_phase_curve_vjp", "swx_plan_phase_curve"):
+                 "swx_phase_curve", "swx_phase_curve_vjp", "swx_plan_phase_curve", "swx_set_prev_level"):
         getattr(lib, name).restype = c_int
     _lib = lib
     return lib

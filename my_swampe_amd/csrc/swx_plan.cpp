@@ -1166,6 +1166,27 @@ int swx_init(swx_plan* p, const double* eta0, const double* delta0, const double
   return do_init<float>(p, eta0, delta0, Phi0, U0, V0, s);
 }
 
+int swx_set_prev_level(swx_plan* p, const double* eta_prev, const double* delta_prev, const double* Phi_prev,
+                       const unsigned char* dead, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!p->initialized) return fail(SWX_ERR_STATE, "swx_set_prev_level before swx_init");
+  if (p->d.expflag || p->d.semi_implicit || p->d.raw_filter)
+    return fail(SWX_ERR_ARG, "swx_set_prev_level: only the modified-Euler scheme reads no spectral prev level");
+  if ((eta_prev == nullptr) != (delta_prev == nullptr) || (eta_prev == nullptr) != (Phi_prev == nullptr))
+    return fail(SWX_ERR_ARG, "swx_set_prev_level: give all of eta_prev, delta_prev, Phi_prev or none");
+  hipStream_t s = (hipStream_t)stream;
+  const int B = p->d.n_members;
+  const long long pl = (long long)p->geo.J * p->geo.I;
+  if (eta_prev) {
+    const double* src[3] = {eta_prev, delta_prev, Phi_prev};
+    // Physical planes 0, 2, 4 of [B][8][J][I] are the prev levels of eta, delta, Phi.
+    for (int f = 0; f < 3; ++f) SWX_HIP(launch_copy_real<double>(p->phys + 2 * f * pl, 8 * pl, src[f], pl, pl, B, s));
+  }
+  if (dead) SWX_HIP(hipMemcpyAsync(p->dead, dead, (size_t)B, hipMemcpyDeviceToDevice, s));
+  return SWX_OK;
+}
+
 int swx_run(swx_plan* p, int t0, int nsteps, double* hist_fields, double* hist_scalars, double* hist_phiext,
             unsigned char* hist_dead, int hist_T, int hist_k0, void* stream) {
   int rc = check_plan(p);

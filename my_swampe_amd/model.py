@@ -560,3 +560,207 @@ def run_model_gpu(*args, **kwargs) -> Dict[str, Any]:
     kwargs.setdefault("as_numpy", False)
     kwargs.setdefault("jit_scan", True)
     return run_model(*args, **kwargs)
+
+
+# ---------------------------------------------------------------------------------------------
+# Operator-level seams (model.py:644-772, :795-1342): one step or one scan on a State carry.
+#
+# A State is a view of a native plan's carry.  Each (Static, RunFlags, test, members) keeps a
+# plan; a State this module returned for that plan continues from the plan's carry without any
+# copy, and any other State is loaded into the plan from its physical grids (the current level's
+# eta, delta, Phi, U, V give the spectral state and the Fourier products exactly as the
+# reference's carry holds them, since those are transforms of the same grids; the physical prev
+# level and the dead flags are copied).  Loading a mid-run carry needs the modified-Euler scheme,
+# whose step reads no spectral prev level; a two-level carry (prev == curr) loads in every
+# scheme.  Fields may carry a leading member axis (B, J, I): B members step in one plan.
+# ---------------------------------------------------------------------------------------------
+
+class _Session:
+    __slots__ = ("plan", "last", "batched")
+
+    def __init__(self, plan: Plan, batched: bool):
+        self.plan, self.last, self.batched = plan, None, batched
+
+
+_SESSIONS: "Dict[tuple, _Session]" = {}
+_SESSIONS_MAX = 4
+
+
+def _plan_params(static: Static, flags: RunFlags) -> np.ndarray:
+    """Plan parameters of a Static: its scalar fields (so that dataclasses.replace(static,
+    taurad=...) takes effect) over the values it was built with, and the flags' filter scalars."""
+    p = np.array(static.params, dtype=np.float64).copy()
+    for name in ("dt", "a", "omega", "g", "Phibar", "taurad", "taudrag"):
+        p[_capi.PARAM_NAMES.index(name)] = float(getattr(static, name))
+    for name in ("alpha", "blowup_rms", "si_alpha", "williams_alpha"):
+        p[_capi.PARAM_NAMES.index(name)] = float(getattr(flags, name))
+    return p
+
+
+def _session(static: Static, flags: RunFlags, test: Optional[int], B: int, batched: bool,
+             precision: str = "fp64") -> _Session:
+    params = _plan_params(static, flags)
+    key = (int(static.M), params.tobytes(), bool(flags.forcflag), bool(flags.diffflag), bool(flags.expflag),
+           bool(flags.modalflag), bool(flags.diagnostics), bool(flags.semi_implicit), bool(flags.raw_filter),
+           test, int(B), bool(batched), precision)
+    sess = _SESSIONS.pop(key, None)
+    if sess is None:
+        plan = Plan(static.M, np.broadcast_to(params, (B, params.size)).copy(), precision=precision,
+                    expflag=flags.expflag, forcflag=flags.forcflag, diffflag=flags.diffflag,
+                    modalflag=flags.modalflag, diagnostics=flags.diagnostics, test=test,
+                    semi_implicit=flags.semi_implicit, raw_filter=flags.raw_filter)
+        sess = _Session(plan, batched)
+        while len(_SESSIONS) >= _SESSIONS_MAX:
+            old = _SESSIONS.pop(next(iter(_SESSIONS)))
+            old.plan.close()
+    _SESSIONS[key] = sess  # most recently used last
+    return sess
+
+
+def _members(state_or_field, static: Static):
+    shape = tuple(state_or_field.shape)
+    if shape == (static.J, static.I):
+        return 1, False
+    if len(shape) == 3 and shape[1:] == (static.J, static.I):
+        return shape[0], True
+    raise ValueError(f"fields must have shape {(static.J, static.I)} or (B, {static.J}, {static.I}), got {shape}.")
+
+
+def _as_field(x, static: Static):
+    torch = require_gpu()
+    return torch.as_tensor(np.asarray(x) if not torch.is_tensor(x) else x, dtype=torch.float64).to(static.mus.device)
+
+
+def _load_state(sess: _Session, state: State, static: Static, flags: RunFlags, test, Uic, Vic) -> None:
+    if state is sess.last:
+        return
+    torch = require_gpu()
+    f = {n: _as_field(getattr(state, n), static)
+         for n in ("eta_prev", "eta_curr", "delta_prev", "delta_curr", "Phi_prev", "Phi_curr", "U_curr", "V_curr")}
+    two_level = all(torch.equal(f[n + "_prev"], f[n + "_curr"]) for n in ("eta", "delta", "Phi"))
+    if not two_level and (flags.expflag or flags.semi_implicit or flags.raw_filter):
+        raise NotImplementedError("loading a mid-run State (prev level != current level) is supported for the "
+                                  "modified-Euler scheme only; step it from the State this module returned")
+    U, V = f["U_curr"], f["V_curr"]
+    if test == 1 and Uic is not None:
+        U, V = _as_field(Uic, static), _as_field(Vic, static)
+    plan = sess.plan
+    plan.init(f["eta_curr"], f["delta_curr"], f["Phi_curr"], U, V)
+    dead = None if state.dead is None else torch.as_tensor(state.dead, device=plan.device)
+    if not two_level:
+        plan.set_prev_level(f["eta_prev"], f["delta_prev"], f["Phi_prev"], dead)
+    elif dead is not None and bool(dead.any()):
+        if flags.expflag or flags.semi_implicit or flags.raw_filter:
+            raise NotImplementedError("dead members can be loaded in the modified-Euler scheme only")
+        plan.set_prev_level(dead=dead)
+    sess.last = None
+
+
+def _export_state(sess: _Session) -> State:
+    st = _state_from_export(sess.plan.export(), squeeze=not sess.batched)
+    sess.last = st
+    return st
+
+
+def _init_state_from_fields(*, static: Static, flags: RunFlags, test: Optional[int], eta0, delta0, Phi0,
+                            U0=None, V0=None) -> State:
+    """Two-level initial carry (prev == curr) from physical fields (model.py:694-772); U0/V0 None
+    diagnoses the winds spectrally from eta0/delta0."""
+    e = _as_field(eta0, static)
+    B, batched = _members(e, static)
+    sess = _session(static, flags, test, B, batched)
+    sess.plan.init(e, _as_field(delta0, static), _as_field(Phi0, static),
+                   None if U0 is None else _as_field(U0, static), None if V0 is None else _as_field(V0, static))
+    return _export_state(sess)
+
+
+def _diagnose_winds(eta, delta, static: Static):
+    """(U, V) diagnosed from eta and delta through the spectral transforms (model.py:644-666)."""
+    e = _as_field(eta, static)
+    B, batched = _members(e, static)
+    sess = _session(static, RunFlags(diagnostics=False), None, B, batched)
+    torch = require_gpu()
+    sess.plan.init(e, _as_field(delta, static), torch.zeros_like(e))
+    sess.last = None
+    ex = sess.plan.export(fields=("U_curr", "V_curr"))
+    U, V = ex["U_curr"], ex["V_curr"]
+    return (U, V) if batched else (U[0], V[0])
+
+
+def _t_list(t_seq):
+    torch = require_gpu()
+    if torch.is_tensor(t_seq):
+        return [int(x) for x in t_seq.detach().cpu().reshape(-1).tolist()]
+    return [int(x) for x in np.asarray(t_seq).reshape(-1)]
+
+
+def _runs(ts):
+    """Split step indices into runs of consecutive values: [(t0, n), ...]."""
+    out = []
+    for t in ts:
+        if out and out[-1][0] + out[-1][1] == t:
+            out[-1][1] += 1
+        else:
+            out.append([t, 1])
+    return [(a, n) for a, n in out]
+
+
+def _step_once(state: State, t, static: Static, flags: RunFlags, test: Optional[int], Uic=None, Vic=None):
+    """One step: (new State, out) with out = t, dead, eta, delta, Phi, U, V (the new fields) and rms,
+    spin_min, phi_min, phi_max (model.py:795-1194)."""
+    B, batched = _members(state.Phi_curr, static)
+    sess = _session(static, flags, test, B, batched)
+    _load_state(sess, state, static, flags, test, Uic, Vic)
+    torch = require_gpu()
+    tt = int(t)
+    hist = sess.plan.run(tt, 1, history=True)
+    t_seq = torch.tensor([tt], dtype=torch.int32, device=sess.plan.device)
+    outs = _outs_from_history(hist, t_seq, flags.diagnostics, 1, sess.plan, squeeze=False)
+    out = {k: (torch.as_tensor(t, dtype=torch.int32, device=sess.plan.device) if k == "t" else
+               (v[:, 0] if batched else v[0, 0])) for k, v in outs.items()}
+    return _export_state(sess), out
+
+
+def _step_once_state_only(state: State, t, static: Static, flags: RunFlags, test: Optional[int], Uic=None,
+                          Vic=None) -> State:
+    """One step returning only the new State (model.py:1197-1235)."""
+    B, batched = _members(state.Phi_curr, static)
+    sess = _session(static, flags, test, B, batched)
+    _load_state(sess, state, static, flags, test, Uic, Vic)
+    sess.plan.run(int(t), 1)
+    return _export_state(sess)
+
+
+def simulate_scan(*, static: Static, flags: RunFlags, state0: State, t_seq, test: Optional[int], Uic=None,
+                  Vic=None):
+    """(last State, outs) over the step indices t_seq, outs stacked along the leading time axis as
+    the reference's scan returns them (model.py:1237-1278)."""
+    torch = require_gpu()
+    B, batched = _members(state0.Phi_curr, static)
+    sess = _session(static, flags, test, B, batched)
+    _load_state(sess, state0, static, flags, test, Uic, Vic)
+    ts = _t_list(t_seq)
+    parts = []
+    for t0, n in _runs(ts):
+        parts.append(sess.plan.run(t0, n, history=True))
+    t_dev = torch.tensor(ts, dtype=torch.int32, device=sess.plan.device)
+    if parts:
+        hist = {k: torch.cat([p[k] for p in parts], dim=1 if k != "fields" else 2) for k in parts[0]}
+    else:
+        hist = {}
+    outs = _outs_from_history(hist, t_dev, flags.diagnostics, len(ts), sess.plan, squeeze=False)
+    # Time-major like lax.scan's stacked outputs: (T, ...) or (T, B, ...).
+    outs = {k: (v if k == "t" else (v.transpose(0, 1) if batched else v[0])) for k, v in outs.items()}
+    return _export_state(sess), outs
+
+
+def simulate_scan_last(*, static: Static, flags: RunFlags, state0: State, t_seq, test: Optional[int], Uic=None,
+                       Vic=None, remat_step: bool = False) -> State:
+    """Final State after the step indices t_seq, no history (model.py:1281-1342).  remat_step is
+    accepted for signature compatibility (the hand-written adjoint has its own checkpointing)."""
+    B, batched = _members(state0.Phi_curr, static)
+    sess = _session(static, flags, test, B, batched)
+    _load_state(sess, state0, static, flags, test, Uic, Vic)
+    for t0, n in _runs(_t_list(t_seq)):
+        sess.plan.run(t0, n)
+    return _export_state(sess)

@@ -130,6 +130,21 @@ class Plan:
         _capi.check(self._lib.swx_init(self._h, _ptr(e), _ptr(d), _ptr(p), _ptr(u), _ptr(v),
                                        self._stream(stream)), "swx_init")
 
+    def set_prev_level(self, eta_prev=None, delta_prev=None, Phi_prev=None, dead=None, stream=None) -> None:
+        """After init() from a carry's current-level grids: set the physical prev level (all three
+        fields or none) and the per-member dead flags (modified-Euler plans)."""
+        torch = _torch()
+        if (eta_prev is None) != (delta_prev is None) or (eta_prev is None) != (Phi_prev is None):
+            raise ValueError("give all of eta_prev, delta_prev, Phi_prev or none")
+        e, d, p = (self._field(x, n) for x, n in ((eta_prev, "eta_prev"), (delta_prev, "delta_prev"),
+                                                  (Phi_prev, "Phi_prev")))
+        dd = None
+        if dead is not None:
+            dd = torch.as_tensor(dead, device=self.device).to(torch.uint8).reshape(-1).expand(self.B).contiguous()
+        self._keep_prev = (e, d, p, dd)
+        _capi.check(self._lib.swx_set_prev_level(self._h, _ptr(e), _ptr(d), _ptr(p), _ptr(dd), self._stream(stream)),
+                    "swx_set_prev_level")
+
     def run(self, t0: int, nsteps: int, *, history: bool = False, stream=None) -> Dict[str, object]:
         """Advance nsteps steps; with history, return per-step fields and diagnostics (device tensors)."""
         torch = _torch()
