@@ -58,35 +58,48 @@ def flops_per_member_step(M: int, J: int, I: int):
     return leg, dft
 
 
-def run_batch(M, B, precision, steps, warmup, graph_steps=32, full_phys=False, si_raw=False):
+def run_batch(M, B, precision, steps, warmup, graph_steps=32, full_phys=False, si_raw=False, parts=1):
     """si_raw: the semi-implicit leapfrog with the RAW filter in its production setting
-    (dt = 600 s, alpha = 0.05, williams_alpha = 0.53), else forced modified Euler at dt = 120 s."""
+    (dt = 600 s, alpha = 0.05, williams_alpha = 0.53), else forced modified Euler at dt = 120 s.
+    parts > 1: the members split into that many plans stepped concurrently (plan.Pipeline); the
+    kernel timings are then not taken."""
     import torch
     from my_swampe_amd import _capi
-    from my_swampe_amd.plan import Plan
+    from my_swampe_amd.plan import Pipeline, split_plans
 
     params = member_params(B, dt=600.0, alpha=0.05) if si_raw else member_params(B)
     c0 = torch.tensor(spectral_ics(M, B), device="cuda")
-    plan = Plan(M, params, precision=precision, diagnostics=False, semi_implicit=si_raw, raw_filter=si_raw)
-    if full_phys:
-        plan.set_option(_capi.OPT_FULL_PHYS_CARRY, 1)
+    plans = split_plans(M, params, parts, precision=precision, diagnostics=False, semi_implicit=si_raw,
+                        raw_filter=si_raw)
+    for plan in plans:
+        if full_phys:
+            plan.set_option(_capi.OPT_FULL_PHYS_CARRY, 1)
+    runner = Pipeline(plans) if parts > 1 else plans[0]
     stream = torch.cuda.Stream()
     torch.cuda.synchronize()
+    ms_leg = ms_grid = None
     with torch.cuda.stream(stream):
-        plan.init_spectral(c0, stream=stream)
-        plan.run_graph(2, warmup, graph_steps=graph_steps, stream=stream)
+        b0 = 0
+        for plan in plans:
+            plan.init_spectral(c0[b0:b0 + plan.B].contiguous(), stream=stream)
+            b0 += plan.B
+        runner.run_graph(2, warmup, graph_steps=graph_steps, stream=stream)
         stream.synchronize()
         t0 = time.perf_counter()
-        plan.run_graph(2 + warmup, steps, graph_steps=graph_steps, stream=stream)
+        runner.run_graph(2 + warmup, steps, graph_steps=graph_steps, stream=stream)
         stream.synchronize()
         wall = time.perf_counter() - t0
-        ms_leg, ms_grid = plan.time_kernels(2 + warmup + steps, 20, stream=stream)
-        ex = plan.export(fields=("Phi_curr",), stream=stream)
+        if parts == 1:
+            ms_leg, ms_grid = plans[0].time_kernels(2 + warmup + steps, 20, stream=stream)
+        exs = [plan.export(fields=("Phi_curr",), stream=stream) for plan in plans]
         stream.synchronize()
-    finite = bool(torch.isfinite(ex["Phi_curr"]).all())
-    fold = plan.get_option(_capi.OPT_GRID_FOLD) if precision == "fp32" else 1
-    leg_mb = plan.get_option(_capi.OPT_LEG_MB) if precision != "fp64" else 0
-    plan.close()
+    finite = all(bool(torch.isfinite(ex["Phi_curr"]).all()) for ex in exs)
+    fold = plans[0].get_option(_capi.OPT_GRID_FOLD) if precision == "fp32" else 1
+    leg_mb = plans[0].get_option(_capi.OPT_LEG_MB) if precision != "fp64" else 0
+    if parts > 1:
+        runner.close()
+    for plan in plans:
+        plan.close()
     return dict(wall=wall, ms_leg=ms_leg, ms_grid=ms_grid, finite=finite, fold=fold, leg_mb=leg_mb)
 
 
@@ -135,9 +148,10 @@ def run_fwd_adjoint(M, B, precision, steps, reps=2):
 def run_mixed(steps, warmup, B42=256, B63=64, graph_steps=32):
     """256 x T42 + 64 x T63 members, FP32 transforms, dt = 120 s, per-member forcing parameters
     (seeds 7 and 8): both truncations in one Legendre and one grid launch per step (Ensemble),
-    and the same two plans stepped one after the other for comparison."""
+    the same two plans stepped one after the other, and the two plans stepped concurrently on two
+    streams (Pipeline)."""
     import torch
-    from my_swampe_amd.plan import Ensemble, Plan
+    from my_swampe_amd.plan import Ensemble, Pipeline, Plan
 
     stream = torch.cuda.Stream()
     torch.cuda.synchronize()
@@ -163,6 +177,15 @@ def run_mixed(steps, warmup, B42=256, B63=64, graph_steps=32):
             pl.run_graph(t1 + warmup, steps, graph_steps=graph_steps, stream=stream)
         stream.synchronize()
         walls["separate"] = time.perf_counter() - t0
+        pipe = Pipeline(plans)
+        t2 = t1 + warmup + steps
+        pipe.run_graph(t2, warmup, graph_steps=graph_steps, stream=stream)
+        stream.synchronize()
+        t0 = time.perf_counter()
+        pipe.run_graph(t2 + warmup, steps, graph_steps=graph_steps, stream=stream)
+        stream.synchronize()
+        walls["concurrent"] = time.perf_counter() - t0
+        pipe.close()
         ex = [pl.export(fields=("Phi_curr",), stream=stream)["Phi_curr"] for pl in plans]
         stream.synchronize()
         finite = all(bool(torch.isfinite(x).all()) for x in ex)
@@ -173,6 +196,8 @@ def run_mixed(steps, warmup, B42=256, B63=64, graph_steps=32):
     return dict(members={"T42": B42, "T63": B63}, steps=steps, ms_per_step=walls["one_launch"] / steps * 1e3,
                 member_steps_per_s=steps * n / walls["one_launch"],
                 separate_ms_per_step=walls["separate"] / steps * 1e3,
+                concurrent_ms_per_step=walls["concurrent"] / steps * 1e3,
+                concurrent_member_steps_per_s=steps * n / walls["concurrent"],
                 separate_member_steps_per_s=steps * n / walls["separate"], finite=finite)
 
 
@@ -282,8 +307,14 @@ def main():
         print(json.dumps({"error": "no ROCm GPU visible"}))
         sys.exit(1)
     M, J, I, B = 42, 64, 128, args.members
+    # Kernel timings (roofline) on one plan of all B members; the throughput `value` from the same
+    # members split into two plans stepped concurrently (plan.Pipeline), the way the library runs
+    # a large ensemble (one plan's Legendre launch overlaps the other's grid launch).
     main_run = run_batch(M, B, args.precision, args.steps, args.warmup)
-    value = args.steps * B / main_run["wall"]
+    parts = 2 if B >= 2 else 1
+    pipe_run = run_batch(M, B, args.precision, args.steps, args.warmup, parts=parts) if parts > 1 else main_run
+    one_plan_value = args.steps * B / main_run["wall"]
+    value = args.steps * B / pipe_run["wall"]
     leg_f, dft_f = flops_per_member_step(M, J, I)
     grid_tflops = dft_f * B / (main_run["ms_grid"] * 1e-3) / 1e12
     leg_tflops = leg_f * B / (main_run["ms_leg"] * 1e-3) / 1e12
@@ -298,14 +329,15 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": main_run["wall"] / args.steps * 1e3,
+        "ms_per_step": pipe_run["wall"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": f"{args.precision} transforms, fp64 state",
         "data": "synthetic (seeded rest+noise ICs, per-member forcing params)",
         "config": {"workload": f"T42x{B} forced modified-Euler forward, dt=120 s, graph-replayed steps",
-                   "members": B, "M": M, "J": J, "I": I, "precision": args.precision},
+                   "members": B, "M": M, "J": J, "I": I, "precision": args.precision,
+                   "plans": parts, "one_plan_member_steps_per_s": one_plan_value},
         "roofline": {"bound": "mfma", "achieved": grid_tflops, "peak": peak, "unit": "TFLOP/s",
                      "frac": grid_tflops / peak,
                      "traffic": measured_traffic(grid_kernel, B, args.precision),
@@ -317,7 +349,7 @@ def main():
                      "algorithmic_flops_per_launch": dft_f * B,
                      "kernel_ms": main_run["ms_grid"], "leg_kernel_ms": main_run["ms_leg"],
                      "leg_tflops": leg_tflops},
-        "finite": main_run["finite"],
+        "finite": main_run["finite"] and pipe_run["finite"],
     }
     if not args.no_extra:
         full = run_batch(M, B, args.precision, max(args.steps // 2, 100), 10, full_phys=True)

@@ -188,6 +188,20 @@ int swx_ensemble_create(swx_plan *a, swx_plan *b, swx_ensemble **out);
 int swx_ensemble_run_graph(swx_ensemble *ens, int t0, int nsteps, int graph_steps, void *stream);
 void swx_ensemble_destroy(swx_ensemble *ens);
 
+/* Concurrent stepping of 2 to 4 initialized plans (for example one ensemble's
+ * members split in halves): plan 0 runs on the caller's stream and plan k on
+ * an internal stream k (default priority), forked after the caller's prior work
+ * and joined before its later work, so that the step kernels of different
+ * plans overlap on the device (one plan's Legendre launch beside another's
+ * grid launch).  swx_pipeline_run_graph advances every plan as swx_run_graph
+ * does; results are those of each plan run alone.  The plans must outlive the
+ * pipeline and must not be stepped through another stream meanwhile.
+ * Replaces the single vmapped scan over all members (model.py:1281-1342). */
+typedef struct swx_pipeline swx_pipeline;
+int swx_pipeline_create(swx_plan *const *plans, int n, swx_pipeline **out);
+int swx_pipeline_run_graph(swx_pipeline *pipe, int t0, int nsteps, int graph_steps, void *stream);
+void swx_pipeline_destroy(swx_pipeline *pipe);
+
 /* Device memory (bytes) the plan holds: basis and DFT tables, carry, work
  * buffers, and the adjoint workspace once allocated.  Caller-owned tensors
  * (checkpoints, outputs) are not included.  No reference counterpart. */

@@ -40,6 +40,7 @@ EXPORTED_SYMBOLS = (
     "swx_phase_curve", "swx_phase_curve_vjp", "swx_plan_phase_curve", "swx_set_prev_level",
     "swx_adjoint_step", "swx_adjoint_result", "swx_last_error", "swx_version", "swx_plan_device_bytes",
     "swx_ensemble_create", "swx_ensemble_run_graph", "swx_ensemble_destroy", "swx_plan_get_option",
+    "swx_pipeline_create", "swx_pipeline_run_graph", "swx_pipeline_destroy",
 )
 
 STATE_FIELDS = (
@@ -120,6 +121,10 @@ def load() -> ctypes.CDLL:
     lib.swx_ensemble_run_graph.argtypes = [c_void_p, c_int, c_int, c_int, c_void_p]
     lib.swx_ensemble_destroy.argtypes = [c_void_p]
     lib.swx_ensemble_destroy.restype = None
+    lib.swx_pipeline_create.argtypes = [POINTER(c_void_p), c_int, POINTER(c_void_p)]
+    lib.swx_pipeline_run_graph.argtypes = [c_void_p, c_int, c_int, c_int, c_void_p]
+    lib.swx_pipeline_destroy.argtypes = [c_void_p]
+    lib.swx_pipeline_destroy.restype = None
     for name in EXPORTED_SYMBOLS:
         getattr(lib, name)  # AttributeError when a declared symbol is missing
     for name in ("swx_geometry", "swx_grid_size", "swx_plan_create", "swx_plan_set_params", "swx_init",
@@ -127,7 +132,8 @@ def load() -> ctypes.CDLL:
                  "swx_init_spectral", "swx_state_get", "swx_state_set", "swx_adjoint_begin",
                  "swx_adjoint_terminal", "swx_adjoint_step", "swx_adjoint_result", "swx_plan_device_bytes",
                  "swx_ensemble_create", "swx_ensemble_run_graph", "swx_plan_get_option", "swx_adjoint_observe",
-                 "swx_phase_curve", "swx_phase_curve_vjp", "swx_plan_phase_curve", "swx_set_prev_level"):
+                 "swx_phase_curve", "swx_phase_curve_vjp", "swx_plan_phase_curve", "swx_set_prev_level",
+                 "swx_pipeline_create", "swx_pipeline_run_graph"):
         getattr(lib, name).restype = c_int
     _lib = lib
     return lib

@@ -691,6 +691,16 @@ int autotune_grid(swx_plan* p) {
 // Two FP32 plans of different truncations stepped together: each step is one Legendre launch
 // and one grid launch covering both plans' workgroups (MFMA pair kernels); a step where either
 // plan needs a plain kernel falls back to one launch per plan.
+// Plans stepped concurrently: plan 0 on the caller's stream, plan k on internal stream k.
+constexpr int kMaxPipeline = 4;
+struct swx_pipeline {
+  int n = 0;
+  swx_plan* p[kMaxPipeline] = {};
+  hipStream_t s[kMaxPipeline] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[kMaxPipeline] = {};
+};
+
 struct swx_ensemble {
   swx_plan* p[2] = {nullptr, nullptr};  // p[0]->geo.M < p[1]->geo.M
   bool fuse_grid = true;                // one grid launch for both plans (timed at creation)
@@ -1345,6 +1355,73 @@ void swx_ensemble_destroy(swx_ensemble* e) {
   if (!e) return;
   if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
   delete e;
+}
+
+int swx_pipeline_create(swx_plan* const* plans, int n, swx_pipeline** out) {
+  if (!out) return fail(SWX_ERR_ARG, "null output");
+  *out = nullptr;
+  if (!plans || n < 2 || n > kMaxPipeline)
+    return fail(SWX_ERR_ARG, "a pipeline takes 2 to " + std::to_string(kMaxPipeline) + " plans");
+  for (int k = 0; k < n; ++k) {
+    int rc = check_plan(plans[k]);
+    if (rc) return rc;
+    for (int q = 0; q < k; ++q)
+      if (plans[q] == plans[k]) return fail(SWX_ERR_ARG, "a pipeline needs distinct plans");
+  }
+  swx_pipeline* pl = new swx_pipeline();
+  pl->n = n;
+  bool ok = hipEventCreateWithFlags(&pl->fork, hipEventDisableTiming) == hipSuccess;
+  for (int k = 0; k < n && ok; ++k) {
+    pl->p[k] = plans[k];
+    if (k == 0) continue;
+    // Default-priority streams that do not synchronize with the null stream.
+    ok = hipStreamCreateWithFlags(&pl->s[k], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&pl->join[k], hipEventDisableTiming) == hipSuccess;
+  }
+  if (!ok) {
+    swx_pipeline_destroy(pl);
+    return fail(SWX_ERR_HIP, "pipeline stream or event creation failed");
+  }
+  *out = pl;
+  return SWX_OK;
+}
+
+int swx_pipeline_run_graph(swx_pipeline* pl, int t0, int nsteps, int graph_steps, void* stream) {
+  if (!pl) return fail(SWX_ERR_ARG, "null pipeline");
+  hipStream_t s0 = (hipStream_t)stream;
+  if (s0 == nullptr) return fail(SWX_ERR_ARG, "a pipeline runs on a non-default stream");
+  for (int k = 0; k < pl->n; ++k) {
+    int rc = check_plan(pl->p[k]);
+    if (rc) return rc;
+    if (!pl->p[k]->initialized) return fail(SWX_ERR_STATE, "swx_pipeline_run_graph before init");
+  }
+  if (nsteps <= 0) return SWX_OK;
+  // Fork: the internal streams start after the caller's prior work.
+  SWX_HIP(hipEventRecord(pl->fork, s0));
+  for (int k = 1; k < pl->n; ++k) SWX_HIP(hipStreamWaitEvent(pl->s[k], pl->fork, 0));
+  for (int k = 0; k < pl->n; ++k) {
+    int rc = swx_run_graph(pl->p[k], t0, nsteps, graph_steps, k == 0 ? stream : (void*)pl->s[k]);
+    if (rc) return rc;
+  }
+  // Join: the caller's later work waits for every plan.
+  for (int k = 1; k < pl->n; ++k) {
+    SWX_HIP(hipEventRecord(pl->join[k], pl->s[k]));
+    SWX_HIP(hipStreamWaitEvent(s0, pl->join[k], 0));
+  }
+  return SWX_OK;
+}
+
+void swx_pipeline_destroy(swx_pipeline* pl) {
+  if (!pl) return;
+  for (int k = 1; k < kMaxPipeline; ++k) {
+    if (pl->s[k]) {
+      (void)hipStreamSynchronize(pl->s[k]);
+      (void)hipStreamDestroy(pl->s[k]);
+    }
+    if (pl->join[k]) (void)hipEventDestroy(pl->join[k]);
+  }
+  if (pl->fork) (void)hipEventDestroy(pl->fork);
+  delete pl;
 }
 
 int swx_plan_set_option(swx_plan* p, int option, int value) {

@@ -322,3 +322,49 @@ class Ensemble:
             self.close()
         except Exception:
             pass
+
+
+class Pipeline:
+    """2 to 4 initialized plans stepped concurrently (swx_pipeline): plan 0 on the caller's
+    stream, the others on internal streams forked from and joined back to it, so that one plan's
+    Legendre launch overlaps another's grid launch.  Each plan's results are those it gives alone."""
+
+    def __init__(self, plans):
+        self._lib = _capi.load()
+        self.plans = tuple(plans)  # kept alive while the pipeline exists
+        if len({p.device for p in self.plans}) != 1:
+            raise ValueError("pipeline plans must be on the same device")
+        arr = (ctypes.c_void_p * len(self.plans))(*[p._h.value for p in self.plans])
+        h = ctypes.c_void_p()
+        _capi.check(self._lib.swx_pipeline_create(arr, len(self.plans), ctypes.byref(h)), "swx_pipeline_create")
+        self._h = h
+        self.device = self.plans[0].device
+
+    def run_graph(self, t0: int, nsteps: int, graph_steps: int = 32, stream=None) -> None:
+        """Advance every plan by nsteps steps from t0 (as Plan.run_graph does for each)."""
+        _capi.check(self._lib.swx_pipeline_run_graph(self._h, int(t0), int(nsteps), int(graph_steps),
+                                                     self.plans[0]._stream(stream)), "swx_pipeline_run_graph")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.swx_pipeline_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover - best effort
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def split_plans(M: int, params: np.ndarray, parts: int = 2, **kw):
+    """The members of params ([B, NPARAM]) split into `parts` plans of consecutive members (sizes
+    differ by at most one), each created with Plan(M, ..., **kw): the plans of a Pipeline."""
+    params = np.asarray(params, dtype=np.float64)
+    if params.ndim == 1:
+        params = params[None, :]
+    B = params.shape[0]
+    if not 1 <= parts <= min(B, 4):
+        raise ValueError("parts must be between 1 and min(members, 4)")
+    bounds = np.linspace(0, B, parts + 1).round().astype(int)
+    return [Plan(M, params[bounds[k]:bounds[k + 1]], **kw) for k in range(parts)]

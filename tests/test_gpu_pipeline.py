@@ -1,0 +1,94 @@
+"""Concurrent stepping of several plans (swx_pipeline): each plan's results equal those it gives
+when stepped alone, and an ensemble split into plans equals the one-plan ensemble bit for bit
+(members are independent; the kernel variants are pinned to the same choice on every plan)."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(B, seed=7):
+    from my_swampe_amd.model import param_row
+
+    rng = np.random.default_rng(seed)
+    return np.stack([param_row(dt=120.0, a=8.2e7, omega=3.2e-5, g=9.8, Phibar=3e5, taurad=10 ** rng.uniform(4, 6),
+                               taudrag=10 ** rng.uniform(4, 6), DPhieq=rng.uniform(1e5, 5e5), K6=1.24e33, K6Phi=None,
+                               alpha=0.01) for _ in range(B)])
+
+
+def _ics(M, B):
+    rng = np.random.default_rng(1000)
+    c = np.zeros((B, 3, M + 1, M + 1), complex)
+    c[:, 0, 0, 1] = 3.2e-5 / np.sqrt(0.375)
+    for m in range(11):
+        for n in range(m, 11):
+            c[:, 2, m, n] = 30.0 * (rng.standard_normal(B) + (1j * rng.standard_normal(B) if m else 0))
+    return c
+
+
+def _pin(plan, precision):
+    from my_swampe_amd import _capi
+
+    if precision == "fp32":
+        plan.set_option(_capi.OPT_GRID_FOLD, 2)
+        plan.set_option(_capi.OPT_LEG_MB, 4)
+
+
+@pytest.mark.parametrize("precision,parts", [("fp32", 2), ("fp32", 3), ("fp64", 2)])
+def test_split_ensemble_equals_one_plan(torch_gpu, precision, parts):
+    from my_swampe_amd.plan import Pipeline, Plan, split_plans
+
+    torch = torch_gpu
+    print(f"[gpu-case] pipeline {precision} parts={parts}", flush=True)
+    M, B, steps = 42, 40, 100
+    params, c0 = _params(B), torch.tensor(_ics(M, B), device="cuda")
+    s = torch.cuda.Stream()
+    one = Plan(M, params, precision=precision)
+    _pin(one, precision)
+    plans = split_plans(M, params, parts, precision=precision)
+    sizes = [p.B for p in plans]
+    assert sum(sizes) == B and max(sizes) - min(sizes) <= 1, sizes
+    with torch.cuda.stream(s):
+        one.init_spectral(c0, stream=s)
+        one.run_graph(2, steps, graph_steps=16, stream=s)
+        want = one.export(fields=("Phi_curr", "eta_curr", "U_curr"), spec=True, stream=s)
+        b0 = 0
+        for p in plans:
+            _pin(p, precision)
+            p.init_spectral(c0[b0:b0 + p.B].contiguous(), stream=s)
+            b0 += p.B
+        pipe = Pipeline(plans)
+        pipe.run_graph(2, 40, graph_steps=16, stream=s)
+        pipe.run_graph(42, steps - 40, graph_steps=16, stream=s)
+        got = [p.export(fields=("Phi_curr", "eta_curr", "U_curr"), spec=True, stream=s) for p in plans]
+    s.synchronize()
+    for name in ("Phi_curr", "eta_curr", "U_curr", "spec"):
+        cat = torch.cat([g[name] for g in got])
+        assert torch.equal(cat, want[name]), name
+    pipe.close()
+    for p in plans + [one]:
+        p.close()
+
+
+def test_pipeline_arguments(torch_gpu):
+    from my_swampe_amd.plan import Pipeline, Plan, split_plans
+
+    torch = torch_gpu
+    print("[gpu-case] pipeline arguments", flush=True)
+    a = Plan(42, _params(2), precision="fp32")
+    with pytest.raises(ValueError):
+        Pipeline([a])
+    with pytest.raises(ValueError):
+        Pipeline([a, a])
+    with pytest.raises(ValueError):
+        split_plans(42, _params(3), 5)
+    b = Plan(42, _params(2, seed=3), precision="fp32")
+    pipe = Pipeline([a, b])
+    with pytest.raises(Exception):  # run before the plans are initialized
+        pipe.run_graph(2, 10, stream=torch.cuda.Stream())
+    pipe.close()
+    a.close()
+    b.close()
