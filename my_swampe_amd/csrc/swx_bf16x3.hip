@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(256) k_grid_bf16x3(GridArgs<float> a, const v8
 
   const int n = ln;
   const int j = j0 + n;
-  const double inv_denE = 1.0 / geo[gl.denomE + j];
+  const PointConsts pc = point_consts(cf, geo, gl, j);
   for (int ic = 0; ic < I; ic += ICH) {
     // ---- Phase 1: inverse DFT of this wave's 16-wide longitude tile, all five fields.
     const int it = ic / 16 + wave;
@@ -163,11 +163,11 @@ __global__ void __launch_bounds__(256) k_grid_bf16x3(GridArgs<float> a, const v8
       float pr[8][4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        double prod[8];
-        grid_point_step(a, cf, geo, gl, phys, plane, alpha, b, j, ic + 16 * wave + 4 * lg + q, c1[0][q], c1[1][q],
-                        c1[2][q], c1[3][q], c1[4][q], inv_denE, prod);
+        float prod[8];
+        grid_point_step(a, cf, geo, gl, pc, phys, plane, alpha, b, j, ic + 16 * wave + 4 * lg + q, c1[0][q],
+                        c1[1][q], c1[2][q], c1[3][q], c1[4][q], prod);
 #pragma unroll
-        for (int f = 0; f < 8; ++f) pr[f][q] = (float)prod[f];
+        for (int f = 0; f < 8; ++f) pr[f][q] = prod[f];
       }
 #pragma unroll
       for (int f = 0; f < 8; ++f) {

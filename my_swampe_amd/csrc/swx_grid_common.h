@@ -3,6 +3,8 @@
 // the forward-DFT accumulators.
 #pragma once
 
+#include <cfloat>
+
 #include "swx_kernels.h"
 #include "swx_physics.h"
 
@@ -10,26 +12,77 @@ namespace swx {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-// One grid point (row j, longitude i) of member b: the initial fields' residue
-// (RAW), pinned winds (test 1), the time filter of the physical prev level and
-// the physical carry (when a.phys_on; swx_physics.h:physical_carry), then the 8
-// products of the forward transform (forcing.py, initial_conditions.py:283-292)
-// in reciprocal form (inv_denE = 1 / (2 (1 - mu_j^2))).
+// Per-member and per-row constants of the FP32 grid physics.
+struct PointConsts {
+  float dph, itr, itd, Phibar, s1m, inv_denE;
+};
+
+__device__ __forceinline__ PointConsts point_consts(const double* cf, const double* geo, const GeoLayout& gl, int j) {
+  PointConsts c;
+  c.dph = (float)cf[C_DPHIEQ];
+  c.itr = (float)cf[C_INVTAURAD];
+  c.itd = (float)cf[C_INVTD];  // 0 for the taudrag == -1 sentinel
+  c.Phibar = (float)cf[C_PHIBAR];
+  c.s1m = (float)geo[gl.s1m + j];
+  c.inv_denE = (float)(1.0 / geo[gl.denomE + j]);
+  return c;
+}
+
+// The 8 products of the forward transform at one point (forcing.py:14-155,
+// initial_conditions.py:283-292) in FP32, the precision of the transforms around them:
+//   Q = (Phieq - (Phi + Phibar)) / taurad, formed as (DPhieq cos(lambda) sqrt(1 - mu^2) - Phi) / taurad
+//       (Phieq - Phibar is that term on the strict dayside and 0 elsewhere, dcos = day * cos(lambda)),
+//   R = -max(Q, 0) / (Phi + Phibar) (a zero denominator replaced by the smallest normal),
+//   F = U R - U / taudrag, G = V R - V / taudrag; A..E = U eta, V eta, U Phi, V Phi,
+//   (U^2 + V^2) / (2 (1 - mu^2)); PhiF = Q.  Unforced tests: PhiF = F = G = 0.
+__device__ __forceinline__ void grid_products_f32(const PointConsts& pc, int test, float dcos, float eta, float Phi,
+                                                  float U, float V, float out[8]) {
+  float PhiF = 0.f, F = 0.f, G = 0.f;
+  if (test < 0) {
+    const float Q = pc.itr * ((pc.dph * dcos) * pc.s1m - Phi);
+    const float Qc = Q < 0.f ? 0.f : Q;
+    const float tot = Phi + pc.Phibar;
+    const float r = -Qc * __builtin_amdgcn_rcpf(tot != 0.f ? tot : FLT_MIN);
+    PhiF = Q;
+    F = U * r - U * pc.itd;
+    G = V * r - V * pc.itd;
+  }
+  out[0] = U * eta;
+  out[1] = V * eta;
+  out[2] = U * Phi;
+  out[3] = V * Phi;
+  out[4] = (U * U + V * V) * pc.inv_denE;
+  out[5] = PhiF;
+  out[6] = F;
+  out[7] = G;
+}
+
+// One grid point (row j, longitude i) of member b: the initial fields' residue (RAW), pinned
+// winds (test 1), the time filter of the physical prev level and the physical carry (when
+// a.phys_on; swx_physics.h:physical_carry), all in FP64 on the FP64 grid planes, then the 8
+// products in FP32 (grid_products_f32).
 __device__ __forceinline__ void grid_point_step(const GridArgs<float>& a, const double* cf, const double* geo,
-                                                const GeoLayout& gl, double* phys, long long plane, double alpha,
-                                                int b, int j, int i, double eta, double delta, double Phi, double U,
-                                                double V, double inv_denE, double prod[8]) {
-  const long long pix = (long long)j * a.I + i;
-  if (a.raw_filter) add_residue(a.resid_c, a.R0p, b, plane, pix, eta, delta, Phi);
-  if (a.test == 1) {
-    U = a.uvic[((long long)b * 2 + 0) * plane + pix];
-    V = a.uvic[((long long)b * 2 + 1) * plane + pix];
+                                                const GeoLayout& gl, const PointConsts& pc, double* phys,
+                                                long long plane, double alpha, int b, int j, int i, float eta,
+                                                float delta, float Phi, float U, float V, float prod[8]) {
+  if (a.raw_filter || a.test == 1 || a.phys_on) {
+    const long long pix = (long long)j * a.I + i;
+    double e = eta, d = delta, p = Phi, u = U, v = V;
+    if (a.raw_filter) add_residue(a.resid_c, a.R0p, b, plane, pix, e, d, p);
+    if (a.test == 1) {
+      u = a.uvic[((long long)b * 2 + 0) * plane + pix];
+      v = a.uvic[((long long)b * 2 + 1) * plane + pix];
+    }
+    if (a.phys_on) {
+      double Phi_mid;
+      physical_carry(cf, a.do_ra, a.raw_filter, alpha, phys, plane, pix, e, d, p, u, v, Phi_mid);
+    }
+    eta = (float)e;
+    Phi = (float)p;
+    U = (float)u;
+    V = (float)v;
   }
-  if (a.phys_on) {
-    double Phi_mid;
-    physical_carry(cf, a.do_ra, a.raw_filter, alpha, phys, plane, pix, eta, delta, Phi, U, V, Phi_mid);
-  }
-  grid_products_recip(cf, geo, gl, j, i, a.test, eta, Phi, U, V, inv_denE, prod);
+  grid_products_f32(pc, a.test, (float)geo[gl.dcos + i], eta, Phi, U, V, prod);
 }
 
 // Assembly of the 9 Legendre inputs of 16 rows (member b, latitudes j0..j0+15)

@@ -54,9 +54,9 @@ struct StepFlags {
 };
 
 // Shared geometry table (FP64): mus[J], w[J], s1m[J], cosphi[J], denomE[J],
-// cosl[I], day[I] (1.0 on the strict dayside, else 0.0).
+// cosl[I], day[I] (1.0 on the strict dayside, else 0.0), dcos[I] = day * cosl.
 struct GeoLayout {
-  int mus, w, s1m, cosphi, denomE, cosl, day, n;
+  int mus, w, s1m, cosphi, denomE, cosl, day, dcos, n;
   __host__ __device__ static GeoLayout make(int J, int I) {
     GeoLayout g;
     g.mus = 0;
@@ -66,7 +66,8 @@ struct GeoLayout {
     g.denomE = 4 * J;
     g.cosl = 5 * J;
     g.day = 5 * J + I;
-    g.n = 5 * J + 2 * I;
+    g.dcos = 5 * J + 2 * I;
+    g.n = 5 * J + 3 * I;
     return g;
   }
 };

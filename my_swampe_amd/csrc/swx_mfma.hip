@@ -105,7 +105,7 @@ __device__ __forceinline__ void grid_mfma_body(const GridArgs<float>& a, const f
 
   const int n = ln;
   const int j = j0 + n;
-  const double inv_denE = 1.0 / geo[gl.denomE + j];
+  const PointConsts pc = point_consts(cf, geo, gl, j);
   for (int ic = 0; ic < I; ic += ICH) {
     // ---- Phase 1: inverse DFT of one 16-wide i-tile, all five fields.  The
     // wave's A fragments (its i-tile of Tinv) are loaded up front.
@@ -135,11 +135,11 @@ __device__ __forceinline__ void grid_mfma_body(const GridArgs<float>& a, const f
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int il = 16 * wave + 4 * lg + q;
-      double prod[8];
-      grid_point_step(a, cf, geo, gl, phys, plane, alpha, b, j, ic + il, c1[0][q], c1[1][q], c1[2][q], c1[3][q],
-                      c1[4][q], inv_denE, prod);
+      float prod[8];
+      grid_point_step(a, cf, geo, gl, pc, phys, plane, alpha, b, j, ic + il, c1[0][q], c1[1][q], c1[2][q], c1[3][q],
+                      c1[4][q], prod);
 #pragma unroll
-      for (int f = 0; f < 8; ++f) sYT[(f * ICH + il) * 16 + n] = (float)prod[f];
+      for (int f = 0; f < 8; ++f) sYT[(f * ICH + il) * 16 + n] = prod[f];
     }
     __syncthreads();
     // ---- Phase 2: forward DFT partial sums over this chunk, one k-group (16
@@ -269,7 +269,7 @@ __device__ __forceinline__ void grid_mfma_fold_body(const GridArgs<float>& a, co
 
   const int n = ln;
   const int j = j0 + n;
-  const double inv_denE = 1.0 / geo[gl.denomE + j];
+  const PointConsts pc = point_consts(cf, geo, gl, j);
   const v4f* Tf = reinterpret_cast<const v4f*>(TfwdF) + lane;
   for (int ic = 0; ic < H; ic += ICH) {
     const int ntc = (H - ic) / 16 < ICH / 16 ? (H - ic) / 16 : ICH / 16;  // 16-longitude tiles in this chunk
@@ -307,17 +307,15 @@ __device__ __forceinline__ void grid_mfma_fold_body(const GridArgs<float>& a, co
       for (int q = 0; q < 4; ++q) {
         const int il = 16 * wave + 4 * lg + q;
         const int i = ic + il;
-        double pa[8], pb[8];
-        grid_point_step(a, cf, geo, gl, phys, plane, alpha, b, j, i, (double)(cE[0][q] + cO[0][q]),
-                        (double)(cE[1][q] + cO[1][q]), (double)(cE[2][q] + cO[2][q]),
-                        (double)(cE[3][q] + cO[3][q]), (double)(cE[4][q] + cO[4][q]), inv_denE, pa);
-        grid_point_step(a, cf, geo, gl, phys, plane, alpha, b, j, i + H, (double)(cE[0][q] - cO[0][q]),
-                        (double)(cE[1][q] - cO[1][q]), (double)(cE[2][q] - cO[2][q]),
-                        (double)(cE[3][q] - cO[3][q]), (double)(cE[4][q] - cO[4][q]), inv_denE, pb);
+        float pa[8], pb[8];
+        grid_point_step(a, cf, geo, gl, pc, phys, plane, alpha, b, j, i, cE[0][q] + cO[0][q], cE[1][q] + cO[1][q],
+                        cE[2][q] + cO[2][q], cE[3][q] + cO[3][q], cE[4][q] + cO[4][q], pa);
+        grid_point_step(a, cf, geo, gl, pc, phys, plane, alpha, b, j, i + H, cE[0][q] - cO[0][q],
+                        cE[1][q] - cO[1][q], cE[2][q] - cO[2][q], cE[3][q] - cO[3][q], cE[4][q] - cO[4][q], pb);
 #pragma unroll
         for (int f = 0; f < 8; ++f) {
-          sS[(f * ICH + il) * 16 + n] = (float)(pa[f] + pb[f]);
-          sD[(f * ICH + il) * 16 + n] = (float)(pa[f] - pb[f]);
+          sS[(f * ICH + il) * 16 + n] = pa[f] + pb[f];
+          sD[(f * ICH + il) * 16 + n] = pa[f] - pb[f];
         }
       }
     }
@@ -520,7 +518,8 @@ struct LegShape {
   // the per-row accesses of the staging and update phases do not conflict.
   static constexpr int RS = RT + 1, XS = XT + 1;
   static constexpr int SA = cmax(cmax(J * RS, KNP * RS), XT * JO);
-  static constexpr int SX = KNP * XS;
+  // Inverse inputs: rows n < K, zero to the k-step multiple (phase I reads rows < ceil4(K)).
+  static constexpr int SX = ceil4(Np) * XS;
   static constexpr int LDS_FLOATS = SA + SX;
 };
 

@@ -935,6 +935,7 @@ int swx_plan_create(const swx_plan_desc* desc, const double* params, swx_plan** 
     const double l = p->geo.lam[i];
     geo[p->gl.cosl + i] = std::cos(l);
     geo[p->gl.day + i] = (l > -pi / 2 && l < pi / 2) ? 1.0 : 0.0;
+    geo[p->gl.dcos + i] = geo[p->gl.day + i] * geo[p->gl.cosl + i];
   }
   std::vector<double> coef;
   fill_coef(p, coef);
