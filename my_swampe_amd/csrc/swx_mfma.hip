@@ -539,7 +539,11 @@ __device__ __forceinline__ void leg_fwd_frags(const LegArgs<float>& a, int m, in
 }
 
 // One workgroup's work: zonal wavenumber m, member group `grp` (members grp*MB ...).
-template <int MT, int MB>
+// GEN = false: the modified-Euler scheme without the RAW filter only, whose spectral update
+// the compiler then specializes (the semi-implicit, RAW and explicit paths of spectral_update
+// hold many more live doubles; dropping them lowers the register count from 152 to 112 at
+// T42, 4 members: 4 waves per SIMD instead of 3).
+template <int MT, int MB, bool GEN = true>
 __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int m, const int grp, float* lds) {
   using S = LegShape<MT, MB>;
   constexpr int Np = S::Np, Mp = MT + 1, J = S::J, RP = S::RP, RT = S::RT, XP = S::XP, XT = S::XT, JO = S::JO;
@@ -659,8 +663,14 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
   __syncthreads();
 
   // Phase U: FP64 update per (member, n) and the inverse-transform inputs.
-  if ((a.si || a.raw_filter) && m == 0 && tid < MB && b0 + tid < B)
-    residue_step(a.coef + (long long)(b0 + tid) * a.ncoef, a.flags(), a.resid_c + 2 * (b0 + tid));
+  StepFlags fl = a.flags();
+  if constexpr (!GEN) {
+    fl.si = 0;
+    fl.raw = 0;
+    fl.expflag = 0;
+  }
+  if (GEN && (a.si || a.raw_filter) && m == 0 && tid < MB && b0 + tid < B)
+    residue_step(a.coef + (long long)(b0 + tid) * a.ncoef, fl, a.resid_c + 2 * (b0 + tid));
 #pragma unroll
   for (int u = 0; u < (MB * Np + 255) / 256; ++u) {
     const int p = tid + 256 * u;
@@ -679,7 +689,7 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
     const long long fstride = (long long)Mp * Np * 2;
     const long long so = ((long long)m * Np + n) * 2;
     double eff[3][2], nw[3][2], dh[2], eh[2];
-    spectral_update(cf, cl, a.f2[(long long)m * Np + n], m, n, a.flags(), fp, fh, Sb + 3 * fstride, Sb, so,
+    spectral_update(cf, cl, a.f2[(long long)m * Np + n], m, n, fl, fp, fh, Sb + 3 * fstride, Sb, so,
                     fstride, eff, nw, kPre ? scp[kPre ? u : 0] : nullptr);
     wind_potentials(cf, cl, m, n, nw[0], nw[1], dh, eh);
     float* xr = sX + np * XS;
@@ -761,29 +771,35 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
   }
 }
 
-template <int MT, int MB>
+template <int MT, int MB, bool GEN>
 __global__ void __launch_bounds__(256) k_leg_mfma(LegArgs<float> a) {
   extern __shared__ __align__(16) float lds[];
-  leg_mfma_body<MT, MB>(a, blockIdx.x, blockIdx.y, lds);
+  leg_mfma_body<MT, MB, GEN>(a, blockIdx.x, blockIdx.y, lds);
 }
 
 // Two plans of different truncations in one launch: block = grp * (M+1) + m, plan 0's first.
-template <int MA, int MBA, int MB, int MBB>
+template <int MA, int MBA, int MB, int MBB, bool GEN>
 __global__ void __launch_bounds__(256) k_leg_mfma_pair(LegArgs<float> a0, LegArgs<float> a1, int nblk0) {
   extern __shared__ __align__(16) float lds[];
   const int blk = blockIdx.x;
   if (blk < nblk0)
-    leg_mfma_body<MA, MBA>(a0, blk % (MA + 1), blk / (MA + 1), lds);
+    leg_mfma_body<MA, MBA, GEN>(a0, blk % (MA + 1), blk / (MA + 1), lds);
   else
-    leg_mfma_body<MB, MBB>(a1, (blk - nblk0) % (MB + 1), (blk - nblk0) / (MB + 1), lds);
+    leg_mfma_body<MB, MBB, GEN>(a1, (blk - nblk0) % (MB + 1), (blk - nblk0) / (MB + 1), lds);
 }
+
+// Whether a launch needs the general spectral update (any scheme other than plain modified Euler).
+static bool leg_generic(const LegArgs<float>& a) { return a.si || a.raw_filter || a.expflag; }
 
 template <int MT, int MB>
 static hipError_t launch_leg_mfma_t(const LegArgs<float>& a, hipStream_t s) {
   using S = LegShape<MT, MB>;
   const size_t smem = sizeof(float) * S::LDS_FLOATS;
   dim3 grid(MT + 1, (a.B + MB - 1) / MB);
-  hipLaunchKernelGGL((k_leg_mfma<MT, MB>), grid, dim3(256), smem, s, a);
+  if (leg_generic(a))
+    hipLaunchKernelGGL((k_leg_mfma<MT, MB, true>), grid, dim3(256), smem, s, a);
+  else
+    hipLaunchKernelGGL((k_leg_mfma<MT, MB, false>), grid, dim3(256), smem, s, a);
   return hipGetLastError();
 }
 
@@ -856,7 +872,10 @@ static hipError_t launch_leg_pair_t(const LegArgs<float>& a0, const LegArgs<floa
   const size_t smem = sizeof(float) * (size_t)cmax(LegShape<MA, MBA>::LDS_FLOATS, LegShape<MB, MBB>::LDS_FLOATS);
   const int n0 = (MA + 1) * ((a0.B + MBA - 1) / MBA);
   const int n1 = (MB + 1) * ((a1.B + MBB - 1) / MBB);
-  hipLaunchKernelGGL((k_leg_mfma_pair<MA, MBA, MB, MBB>), dim3(n0 + n1), dim3(256), smem, s, a0, a1, n0);
+  if (leg_generic(a0) || leg_generic(a1))
+    hipLaunchKernelGGL((k_leg_mfma_pair<MA, MBA, MB, MBB, true>), dim3(n0 + n1), dim3(256), smem, s, a0, a1, n0);
+  else
+    hipLaunchKernelGGL((k_leg_mfma_pair<MA, MBA, MB, MBB, false>), dim3(n0 + n1), dim3(256), smem, s, a0, a1, n0);
   return hipGetLastError();
 }
 
