@@ -121,6 +121,7 @@ template <int Mp, int NT2, int TE = 0, bool PRE = true>
 __device__ __forceinline__ void assemble_prefetch_uv(const GridArgs<float>& a, int b, int j0,
                                                      float2 (&uv)[(NT2 + 1) / 2][2]) {
   if constexpr (!(PRE && kAsmPrefetch<NT2>)) return;
+  if (!a.si && !a.expflag) return;  // the modified-Euler slots do not read U_m, V_m
   const int jj = j0 + (threadIdx.x & 15);
 #pragma unroll
   for (int r = 0; r < (NT2 + 1) / 2; ++r) {
@@ -142,6 +143,7 @@ __device__ __forceinline__ void grid_assemble(const GridArgs<float>& a, const v4
   const int jj = j0 + nn;
   const double tcj = cf[cl.n_tc + jj], wj = geo[gl.w + jj];
   const SlotConsts sk = slot_consts(cf, tcj, wj, a.expflag, a.forcflag);
+  const SlotConstsF skf = slot_consts_f32(sk);
 #pragma unroll
   for (int t0 = 0; t0 < NT2; t0 += 2) {
 #pragma unroll
@@ -160,40 +162,53 @@ __device__ __forceinline__ void grid_assemble(const GridArgs<float>& a, const v4
     {
       const int m = assemble_m<Mp, NT2, TE>(t0);
       if (m >= 0) {
-        double fm[8][2];
+        float fs[8][2];
 #pragma unroll
         for (int f = 0; f < 8; ++f) {
-          fm[f][0] = sC[((dt2 * 8 + f) * 16 + 2 * ml) * kAsmStride + nn];
-          fm[f][1] = sC[((dt2 * 8 + f) * 16 + 2 * ml + 1) * kAsmStride + nn];
+          fs[f][0] = sC[((dt2 * 8 + f) * 16 + 2 * ml) * kAsmStride + nn];
+          fs[f][1] = sC[((dt2 * 8 + f) * 16 + 2 * ml + 1) * kAsmStride + nn];
         }
-        float2 gu, gv;
-        if constexpr (PRE && kAsmPrefetch<NT2>) {
-          gu = uv[t0 / 2][0];
-          gv = uv[t0 / 2][1];
-        } else {
-          gu = *reinterpret_cast<const float2*>(a.G + fidx(b, 3, m, jj, Mp, J));
-          gv = *reinterpret_cast<const float2*>(a.G + fidx(b, 4, m, jj, Mp, J));
-        }
-        double Um[2] = {gu.x, gu.y}, Vm[2] = {gv.x, gv.y};
         if (a.export_raw) {
           float* rb = a.raw + (long long)b * kNRaw * J * Mp * 2;
 #pragma unroll
           for (int f = 0; f < kNRaw; ++f)
-            *reinterpret_cast<float2*>(rb + ((long long)f * J + jj) * Mp * 2 + 2 * m) =
-                make_float2((float)fm[f][0], (float)fm[f][1]);
-        }
-        double out[kNT][2];
-        if (a.si) {
-          double lagR[2][2];
-          si_lag_exchange(a, cf, b, jj, m, fm, Um, Vm, false, lagR);
-          assemble_slots_si(cf, tcj, wj, m, a.forcflag, a.relax, fm, lagR, out);
-        } else {
-          assemble_slots(sk, m, fm, Um, Vm, out);
+            *reinterpret_cast<float2*>(rb + ((long long)f * J + jj) * Mp * 2 + 2 * m) = make_float2(fs[f][0], fs[f][1]);
         }
         float* Tm = a.T + (((long long)b * Mp + m) * kNT) * J * 2;
+        if (!a.si && !a.expflag) {
+          float out[kNT][2];
+          assemble_slots_f32(skf, m, fs, out);
 #pragma unroll
-        for (int k = 0; k < kNT; ++k)
-          *reinterpret_cast<float2*>(Tm + (k * J + jj) * 2) = make_float2((float)out[k][0], (float)out[k][1]);
+          for (int k = 0; k < kNT; ++k)
+            *reinterpret_cast<float2*>(Tm + (k * J + jj) * 2) = make_float2(out[k][0], out[k][1]);
+        } else {
+          double fm[8][2];
+#pragma unroll
+          for (int f = 0; f < 8; ++f) {
+            fm[f][0] = fs[f][0];
+            fm[f][1] = fs[f][1];
+          }
+          float2 gu, gv;
+          if constexpr (PRE && kAsmPrefetch<NT2>) {
+            gu = uv[t0 / 2][0];
+            gv = uv[t0 / 2][1];
+          } else {
+            gu = *reinterpret_cast<const float2*>(a.G + fidx(b, 3, m, jj, Mp, J));
+            gv = *reinterpret_cast<const float2*>(a.G + fidx(b, 4, m, jj, Mp, J));
+          }
+          double Um[2] = {gu.x, gu.y}, Vm[2] = {gv.x, gv.y};
+          double out[kNT][2];
+          if (a.si) {
+            double lagR[2][2];
+            si_lag_exchange(a, cf, b, jj, m, fm, Um, Vm, false, lagR);
+            assemble_slots_si(cf, tcj, wj, m, a.forcflag, a.relax, fm, lagR, out);
+          } else {
+            assemble_slots(sk, m, fm, Um, Vm, out);
+          }
+#pragma unroll
+          for (int k = 0; k < kNT; ++k)
+            *reinterpret_cast<float2*>(Tm + (k * J + jj) * 2) = make_float2((float)out[k][0], (float)out[k][1]);
+        }
       }
     }
     __syncthreads();

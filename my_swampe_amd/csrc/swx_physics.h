@@ -295,6 +295,55 @@ __device__ inline void assemble_slots(const SlotConsts& k, int m, const double f
   }
 }
 
+// The modified-Euler slots of assemble_slots in FP32, for the matrix-core grid kernels whose
+// inputs (forward-DFT outputs) and outputs (the Legendre inputs) are FP32: the same
+// combinations with the quadrature weight folded into the coefficients.
+struct SlotConstsF {
+  float t1w, t2w, tew, hPb, i2a2, dtw, dt2ia2w;
+  int forcflag;
+};
+
+__device__ inline SlotConstsF slot_consts_f32(const SlotConsts& k) {
+  SlotConstsF c;
+  c.t1w = (float)(k.t1 * k.wj);
+  c.t2w = (float)(k.t2 * k.wj);
+  c.tew = (float)(k.te * k.wj);
+  c.hPb = (float)(0.5 * k.Phibar);
+  c.i2a2 = (float)k.i2a2;
+  c.dtw = k.forcflag ? (float)(k.dt * k.wj) : 0.f;
+  c.dt2ia2w = k.forcflag ? (float)((k.dt / 2.0) * k.ia2 * k.wj) : 0.f;
+  c.forcflag = k.forcflag;
+  return c;
+}
+
+__device__ inline void assemble_slots_f32(const SlotConstsF& k, int m, const float f[8][2], float out[kNT][2]) {
+  const float mm = (float)m;
+  const float *Am = f[0], *Bm = f[1], *Cm = f[2], *Dm = f[3], *Em = f[4], *PFm = f[5], *Fm = f[6], *Gm = f[7];
+  float Bf[2], Af[2], Bs[2], As[2];
+  for (int c = 0; c < 2; ++c) {
+    Bf[c] = Bm[c] + Fm[c];
+    Af[c] = Am[c] - Gm[c];
+    Bs[c] = k.forcflag ? Bf[c] : Bm[c];
+    As[c] = k.forcflag ? Af[c] : Am[c];
+  }
+  // i m z = (-m z.im, m z.re)
+  const float imC[2] = {-mm * Cm[1], mm * Cm[0]};
+  const float imBs[2] = {-mm * Bs[1], mm * Bs[0]};
+  const float imBf[2] = {-mm * Bf[1], mm * Bf[0]};
+  const float imAs[2] = {-mm * As[1], mm * As[0]};
+  for (int c = 0; c < 2; ++c) {
+    out[0][c] = -(k.t1w * imC[c]) - k.hPb * (k.t1w * imBs[c]) + k.dtw * PFm[c];
+    out[1][c] = -k.hPb * (k.t2w * Em[c]);
+    out[2][c] = k.t1w * imBf[c];
+    out[3][c] = k.t2w * Em[c] + (k.t1w * imC[c]) * k.i2a2 + k.dt2ia2w * PFm[c];
+    out[4][c] = -(k.tew * imAs[c]);
+    out[5][c] = k.t1w * Dm[c] - k.hPb * (k.t1w * As[c]);
+    out[6][c] = k.t1w * Af[c];
+    out[7][c] = (k.t1w * Dm[c]) * k.i2a2;
+    out[8][c] = k.tew * Bs[c];
+  }
+}
+
 // Semi-implicit Legendre inputs of one (member, j, m) (semi_implicit_tdiff.py:163-190),
 // w-weighted, in the slot order spectral_update reads: P0 tc i m A*, P1 tc i m B*,
 // P2 tc i m C, P3 E, P4 2 dt PhiF (forced); H5 tc A*, H6 tc B*, H7 tc D; H8 unused.
