@@ -37,6 +37,8 @@ __device__ __forceinline__ PointConsts point_consts(const double* cf, const doub
 //   (U^2 + V^2) / (2 (1 - mu^2)); PhiF = Q.  Unforced tests: PhiF = F = G = 0.
 __device__ __forceinline__ void grid_products_f32(const PointConsts& pc, int test, float dcos, float eta, float Phi,
                                                   float U, float V, float out[8]) {
+  // Unfused arithmetic: every kernel variant that inlines this rounds identically.
+#pragma clang fp contract(off)
   float PhiF = 0.f, F = 0.f, G = 0.f;
   if (test < 0) {
     const float Q = pc.itr * ((pc.dph * dcos) * pc.s1m - Phi);
@@ -61,11 +63,13 @@ __device__ __forceinline__ void grid_products_f32(const PointConsts& pc, int tes
 // winds (test 1), the time filter of the physical prev level and the physical carry (when
 // a.phys_on; swx_physics.h:physical_carry), all in FP64 on the FP64 grid planes, then the 8
 // products in FP32 (grid_products_f32).
+// FAST: the caller guarantees no residue, pinned winds or physical carry at this launch.
+template <bool FAST = false>
 __device__ __forceinline__ void grid_point_step(const GridArgs<float>& a, const double* cf, const double* geo,
                                                 const GeoLayout& gl, const PointConsts& pc, double* phys,
                                                 long long plane, double alpha, int b, int j, int i, float eta,
                                                 float delta, float Phi, float U, float V, float prod[8]) {
-  if (a.raw_filter || a.test == 1 || a.phys_on) {
+  if (!FAST && (a.raw_filter || a.test == 1 || a.phys_on)) {
     const long long pix = (long long)j * a.I + i;
     double e = eta, d = delta, p = Phi, u = U, v = V;
     if (a.raw_filter) add_residue(a.resid_c, a.R0p, b, plane, pix, e, d, p);
@@ -132,7 +136,8 @@ __device__ __forceinline__ void assemble_prefetch_uv(const GridArgs<float>& a, i
   }
 }
 
-template <int Mp, int NT2, int TE = 0, bool PRE = true>
+// FAST: the caller guarantees the modified-Euler slots (no semi-implicit or explicit scheme).
+template <int Mp, int NT2, int TE = 0, bool PRE = true, bool FAST = false>
 __device__ __forceinline__ void grid_assemble(const GridArgs<float>& a, const v4f* acc, float* sC, int b, int j0,
                                               const double* cf, const CoefLayout& cl, const double* geo,
                                               const GeoLayout& gl, const float2 (&uv)[(NT2 + 1) / 2][2]) {
@@ -175,7 +180,7 @@ __device__ __forceinline__ void grid_assemble(const GridArgs<float>& a, const v4
             *reinterpret_cast<float2*>(rb + ((long long)f * J + jj) * Mp * 2 + 2 * m) = make_float2(fs[f][0], fs[f][1]);
         }
         float* Tm = a.T + (((long long)b * Mp + m) * kNT) * J * 2;
-        if (!a.si && !a.expflag) {
+        if (FAST || (!a.si && !a.expflag)) {
           float out[kNT][2];
           assemble_slots_f32(skf, m, fs, out);
 #pragma unroll

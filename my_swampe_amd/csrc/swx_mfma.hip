@@ -204,9 +204,16 @@ struct MfmaFoldShape {
   // One chunk (T42): X^T is dead once GEMM 1 is done, so s / d reuse its space.
   static constexpr int LDS_FLOATS = NCH == 1 ? (XT_FLOATS > SD_FLOATS ? XT_FLOATS : SD_FLOATS)
                                              : XT_FLOATS + SD_FLOATS;
+  // Fast variant: s goes through LDS first (even-m tiles), then d in the same buffer (odd-m
+  // tiles) while the lanes hold d in registers: half the product region.
+  static constexpr int S1_FLOATS = 8 * ICH * 16 > kAsmFloats ? 8 * ICH * 16 : kAsmFloats;
+  static constexpr int LDS_FAST = NCH == 1 ? (XT_FLOATS > S1_FLOATS ? XT_FLOATS : S1_FLOATS) : XT_FLOATS + S1_FLOATS;
 };
 
-template <int MT>
+// FAST: a step without residue, pinned winds, physical carry, semi-implicit or explicit slots
+// (the graph-replayed steps of the modified-Euler scheme): FP32-only point and slot code, and
+// the two-round s / d exchange in half the LDS.
+template <int MT, bool FAST = false>
 __device__ __forceinline__ void grid_mfma_fold_body(const GridArgs<float>& a, const float* __restrict__ TinvF,
                                                     const float* __restrict__ TfwdF, const int blk, float* lds) {
   using S = MfmaFoldShape<MT>;
@@ -300,63 +307,129 @@ __device__ __forceinline__ void grid_mfma_fold_body(const GridArgs<float>& a, co
     v4f an[NT2];
 #pragma unroll
     for (int t = 0; t < NT2; ++t) an[t] = Tf[((long long)t * (H / 16) + ic / 16) * 64];
-    if constexpr (S::NCH == 1) __syncthreads();  // s / d overwrite X^T
-    // ---- Physics on the lane's 4 point pairs (row n, i = ib + 4 lg + q and i + H).
-    if (active) {
+    if constexpr (FAST) {
+      if constexpr (S::NCH == 1) __syncthreads();  // s overwrites X^T
+      // ---- Physics (FP32 only), s to LDS, d kept in registers.
+      float dv[8][4];
+      if (active) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int il = 16 * wave + 4 * lg + q;
-        const int i = ic + il;
-        float pa[8], pb[8];
-        grid_point_step(a, cf, geo, gl, pc, phys, plane, alpha, b, j, i, cE[0][q] + cO[0][q], cE[1][q] + cO[1][q],
-                        cE[2][q] + cO[2][q], cE[3][q] + cO[3][q], cE[4][q] + cO[4][q], pa);
-        grid_point_step(a, cf, geo, gl, pc, phys, plane, alpha, b, j, i + H, cE[0][q] - cO[0][q],
-                        cE[1][q] - cO[1][q], cE[2][q] - cO[2][q], cE[3][q] - cO[3][q], cE[4][q] - cO[4][q], pb);
+        for (int q = 0; q < 4; ++q) {
+          const int il = 16 * wave + 4 * lg + q;
+          const int i = ic + il;
+          float pa[8], pb[8];
+          grid_point_step<true>(a, cf, geo, gl, pc, phys, plane, alpha, b, j, i, cE[0][q] + cO[0][q],
+                                cE[1][q] + cO[1][q], cE[2][q] + cO[2][q], cE[3][q] + cO[3][q], cE[4][q] + cO[4][q],
+                                pa);
+          grid_point_step<true>(a, cf, geo, gl, pc, phys, plane, alpha, b, j, i + H, cE[0][q] - cO[0][q],
+                                cE[1][q] - cO[1][q], cE[2][q] - cO[2][q], cE[3][q] - cO[3][q], cE[4][q] - cO[4][q],
+                                pb);
 #pragma unroll
-        for (int f = 0; f < 8; ++f) {
-          sS[(f * ICH + il) * 16 + n] = pa[f] + pb[f];
-          sD[(f * ICH + il) * 16 + n] = pa[f] - pb[f];
+          for (int f = 0; f < 8; ++f) {
+            sS[(f * ICH + il) * 16 + n] = pa[f] + pb[f];
+            dv[f][q] = pa[f] - pb[f];
+          }
         }
       }
-    }
-    __syncthreads();
-    // ---- Phase 2: forward sums over this chunk's tiles; wave w owns fields 2w, 2w + 1; the
-    // even-m output tiles read s, the odd-m ones d.
+      __syncthreads();
+      // ---- Phase 2 in two rounds: the even-m tiles over s, then the odd-m tiles over d.
+#pragma unroll
+      for (int round = 0; round < 2; ++round) {
+        const int t0 = round == 0 ? 0 : TE, t1 = round == 0 ? TE : NT2;
+        if (round == 1) {
+          if (active) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+              for (int f = 0; f < 8; ++f) sS[(f * ICH + 16 * wave + 4 * lg + q) * 16 + n] = dv[f][q];
+          }
+          __syncthreads();  // (an[t] for the odd-m tiles still holds this chunk's first k-group)
+        }
 #pragma unroll 1
-    for (int gg = 0; gg < ntc; ++gg) {
-      v4f ac[NT2];
+        for (int gg = 0; gg < ntc; ++gg) {
+          v4f ac[NT2];
 #pragma unroll
-      for (int t = 0; t < NT2; ++t) ac[t] = an[t];
-      if (gg + 1 < ntc) {
+          for (int t = 0; t < NT2; ++t)
+            if (t >= t0 && t < t1) ac[t] = an[t];
+          if (gg + 1 < ntc) {
 #pragma unroll
-        for (int t = 0; t < NT2; ++t) an[t] = Tf[((long long)t * (H / 16) + ic / 16 + gg + 1) * 64];
+            for (int t = 0; t < NT2; ++t)
+              if (t >= t0 && t < t1) an[t] = Tf[((long long)t * (H / 16) + ic / 16 + gg + 1) * 64];
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int il = 16 * gg + 4 * q + lg;
+            const float v0 = sS[((2 * wave) * ICH + il) * 16 + ln];
+            const float v1 = sS[((2 * wave + 1) * ICH + il) * 16 + ln];
+#pragma unroll
+            for (int t = 0; t < NT2; ++t) {
+              if (t >= t0 && t < t1) {
+                acc[2 * t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[t][q], v0, acc[2 * t], 0, 0, 0);
+                acc[2 * t + 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[t][q], v1, acc[2 * t + 1], 0, 0, 0);
+              }
+            }
+          }
+        }
+        __syncthreads();
       }
+    } else {
+      if constexpr (S::NCH == 1) __syncthreads();  // s / d overwrite X^T
+      // ---- Physics on the lane's 4 point pairs (row n, i = ib + 4 lg + q and i + H).
+      if (active) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int il = 16 * gg + 4 * q + lg;
-        const float s0 = sS[((2 * wave) * ICH + il) * 16 + ln];
-        const float s1 = sS[((2 * wave + 1) * ICH + il) * 16 + ln];
-        const float d0 = sD[((2 * wave) * ICH + il) * 16 + ln];
-        const float d1 = sD[((2 * wave + 1) * ICH + il) * 16 + ln];
+        for (int q = 0; q < 4; ++q) {
+          const int il = 16 * wave + 4 * lg + q;
+          const int i = ic + il;
+          float pa[8], pb[8];
+          grid_point_step(a, cf, geo, gl, pc, phys, plane, alpha, b, j, i, cE[0][q] + cO[0][q], cE[1][q] + cO[1][q],
+                          cE[2][q] + cO[2][q], cE[3][q] + cO[3][q], cE[4][q] + cO[4][q], pa);
+          grid_point_step(a, cf, geo, gl, pc, phys, plane, alpha, b, j, i + H, cE[0][q] - cO[0][q],
+                          cE[1][q] - cO[1][q], cE[2][q] - cO[2][q], cE[3][q] - cO[3][q], cE[4][q] - cO[4][q], pb);
 #pragma unroll
-        for (int t = 0; t < NT2; ++t) {
-          acc[2 * t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[t][q], t < TE ? s0 : d0, acc[2 * t], 0, 0, 0);
-          acc[2 * t + 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[t][q], t < TE ? s1 : d1, acc[2 * t + 1], 0, 0, 0);
+          for (int f = 0; f < 8; ++f) {
+            sS[(f * ICH + il) * 16 + n] = pa[f] + pb[f];
+            sD[(f * ICH + il) * 16 + n] = pa[f] - pb[f];
+          }
         }
       }
+      __syncthreads();
+      // ---- Phase 2: forward sums over this chunk's tiles; wave w owns fields 2w, 2w + 1; the
+      // even-m output tiles read s, the odd-m ones d.
+#pragma unroll 1
+      for (int gg = 0; gg < ntc; ++gg) {
+        v4f ac[NT2];
+#pragma unroll
+        for (int t = 0; t < NT2; ++t) ac[t] = an[t];
+        if (gg + 1 < ntc) {
+#pragma unroll
+          for (int t = 0; t < NT2; ++t) an[t] = Tf[((long long)t * (H / 16) + ic / 16 + gg + 1) * 64];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int il = 16 * gg + 4 * q + lg;
+          const float s0 = sS[((2 * wave) * ICH + il) * 16 + ln];
+          const float s1 = sS[((2 * wave + 1) * ICH + il) * 16 + ln];
+          const float d0 = sD[((2 * wave) * ICH + il) * 16 + ln];
+          const float d1 = sD[((2 * wave + 1) * ICH + il) * 16 + ln];
+#pragma unroll
+          for (int t = 0; t < NT2; ++t) {
+            acc[2 * t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[t][q], t < TE ? s0 : d0, acc[2 * t], 0, 0, 0);
+            acc[2 * t + 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ac[t][q], t < TE ? s1 : d1, acc[2 * t + 1], 0, 0, 0);
+          }
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 
   // ---- Phase 3: assemble the Legendre inputs (swx_grid_common.h), folded row order.
-  grid_assemble<Mp, NT2, TE>(a, acc, sC, b, j0, cf, cl, geo, gl, uv);
+  grid_assemble<Mp, NT2, TE, true, FAST>(a, acc, sC, b, j0, cf, cl, geo, gl, uv);
 }
 
-template <int MT>
+template <int MT, bool FAST>
 __global__ void __launch_bounds__(256) k_grid_mfma_fold(GridArgs<float> a, const float* __restrict__ TinvF,
                                                         const float* __restrict__ TfwdF) {
   extern __shared__ __align__(16) float lds[];
-  grid_mfma_fold_body<MT>(a, TinvF, TfwdF, blockIdx.x, lds);
+  grid_mfma_fold_body<MT, FAST>(a, TinvF, TfwdF, blockIdx.x, lds);
 }
 
 template <int MT>
@@ -396,13 +469,22 @@ static hipError_t launch_mfma_t(const GridArgs<float>& a, const float* TinvF, co
   return hipGetLastError();
 }
 
+// Whether a folded-grid launch can take the fast variant (no FP64 bookkeeping at this step).
+static bool grid_fold_fast(const GridArgs<float>& a) {
+  return !a.phys_on && !a.raw_filter && a.test != 1 && !a.si && !a.expflag;
+}
+
 template <int MT>
 static hipError_t launch_mfma_fold_t(const GridArgs<float>& a, const float* TinvF, const float* TfwdF,
                                      hipStream_t s) {
   using S = MfmaFoldShape<MT>;
-  const size_t smem = sizeof(float) * S::LDS_FLOATS;
   dim3 grid((unsigned)((long long)a.B * a.J / S::ROWS));
-  hipLaunchKernelGGL(k_grid_mfma_fold<MT>, grid, dim3(256), smem, s, a, TinvF, TfwdF);
+  if (grid_fold_fast(a))
+    hipLaunchKernelGGL((k_grid_mfma_fold<MT, true>), grid, dim3(256), sizeof(float) * S::LDS_FAST, s, a, TinvF,
+                       TfwdF);
+  else
+    hipLaunchKernelGGL((k_grid_mfma_fold<MT, false>), grid, dim3(256), sizeof(float) * S::LDS_FLOATS, s, a, TinvF,
+                       TfwdF);
   return hipGetLastError();
 }
 

@@ -317,6 +317,8 @@ __device__ inline SlotConstsF slot_consts_f32(const SlotConsts& k) {
 }
 
 __device__ inline void assemble_slots_f32(const SlotConstsF& k, int m, const float f[8][2], float out[kNT][2]) {
+  // Unfused arithmetic: every kernel variant that inlines this rounds identically.
+#pragma clang fp contract(off)
   const float mm = (float)m;
   const float *Am = f[0], *Bm = f[1], *Cm = f[2], *Dm = f[3], *Em = f[4], *PFm = f[5], *Fm = f[6], *Gm = f[7];
   float Bf[2], Af[2], Bs[2], As[2];
