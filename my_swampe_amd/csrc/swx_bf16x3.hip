@@ -161,11 +161,20 @@ __global__ void __launch_bounds__(256) k_grid_bf16x3(GridArgs<float> a, const v8
     // products go to LDS split into hi / lo, [f][row][i].
     {
       float pr[8][4];
+      float fe[4], fd[4], fp[4], fu[4], fv[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        fe[q] = c1[0][q];
+        fd[q] = c1[1][q];
+        fp[q] = c1[2][q];
+        fu[q] = c1[3][q];
+        fv[q] = c1[4][q];
+      }
+      grid_bookkeeping4(a, cf, phys, plane, alpha, b, j, ic + 16 * wave + 4 * lg, fe, fd, fp, fu, fv);
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         float prod[8];
-        grid_point_step(a, cf, geo, gl, pc, phys, plane, alpha, b, j, ic + 16 * wave + 4 * lg + q, c1[0][q],
-                        c1[1][q], c1[2][q], c1[3][q], c1[4][q], prod);
+        grid_point_step(a, geo, gl, pc, ic + 16 * wave + 4 * lg + q, fe[q], fp[q], fu[q], fv[q], prod);
 #pragma unroll
         for (int f = 0; f < 8; ++f) pr[f][q] = prod[f];
       }

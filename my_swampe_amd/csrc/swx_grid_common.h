@@ -59,34 +59,84 @@ __device__ __forceinline__ void grid_products_f32(const PointConsts& pc, int tes
   out[7] = G;
 }
 
-// One grid point (row j, longitude i) of member b: the initial fields' residue (RAW), pinned
-// winds (test 1), the time filter of the physical prev level and the physical carry (when
-// a.phys_on; swx_physics.h:physical_carry), all in FP64 on the FP64 grid planes, then the 8
-// products in FP32 (grid_products_f32).
-// FAST: the caller guarantees no residue, pinned winds or physical carry at this launch.
-template <bool FAST = false>
-__device__ __forceinline__ void grid_point_step(const GridArgs<float>& a, const double* cf, const double* geo,
-                                                const GeoLayout& gl, const PointConsts& pc, double* phys,
-                                                long long plane, double alpha, int b, int j, int i, float eta,
-                                                float delta, float Phi, float U, float V, float prod[8]) {
-  if (!FAST && (a.raw_filter || a.test == 1 || a.phys_on)) {
-    const long long pix = (long long)j * a.I + i;
-    double e = eta, d = delta, p = Phi, u = U, v = V;
-    if (a.raw_filter) add_residue(a.resid_c, a.R0p, b, plane, pix, e, d, p);
-    if (a.test == 1) {
-      u = a.uvic[((long long)b * 2 + 0) * plane + pix];
-      v = a.uvic[((long long)b * 2 + 1) * plane + pix];
-    }
-    if (a.phys_on) {
-      double Phi_mid;
-      physical_carry(cf, a.do_ra, a.raw_filter, alpha, phys, plane, pix, e, d, p, u, v, Phi_mid);
-    }
-    eta = (float)e;
-    Phi = (float)p;
-    U = (float)u;
-    V = (float)v;
-  }
+// The 8 FP32 products of one grid point (row j, longitude i); the FP64 bookkeeping of its
+// fields (residue, pinned winds, physical carry) is grid_bookkeeping4's, done before.
+__device__ __forceinline__ void grid_point_step(const GridArgs<float>& a, const double* geo, const GeoLayout& gl,
+                                                const PointConsts& pc, int i, float eta, float Phi, float U, float V,
+                                                float prod[8]) {
   grid_products_f32(pc, a.test, (float)geo[gl.dcos + i], eta, Phi, U, V, prod);
+}
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// The FP64 bookkeeping of the 4 consecutive longitudes i0 .. i0 + 3 (i0 % 4 == 0) of row j of
+// member b, before their products: the initial fields' residue (RAW), pinned winds (test 1),
+// the time filter of the physical prev level and the physical carry (when a.phys_on;
+// swx_physics.h:physical_carry, the same arithmetic on 4 points).  The FP64 planes move as
+// 32-byte vectors, so the 4 lanes of a row cover whole 128-byte lines (a per-point access
+// pattern writes each line in 8-byte pieces).  The fields come in and go out as FP32
+// transform values.  Does nothing when none of the three applies.
+__device__ __forceinline__ void grid_bookkeeping4(const GridArgs<float>& a, const double* cf, double* phys,
+                                                  long long plane, double alpha, int b, int j, int i0,
+                                                  float (&eta)[4], float (&delta)[4], float (&Phi)[4],
+                                                  float (&U)[4], float (&V)[4]) {
+  if (!(a.raw_filter || a.test == 1 || a.phys_on)) return;
+  const long long pix = (long long)j * a.I + i0;
+  v4d e, d, p, u, v;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    e[q] = eta[q];
+    d[q] = delta[q];
+    p[q] = Phi[q];
+    u[q] = U[q];
+    v[q] = V[q];
+  }
+  if (a.raw_filter) {
+    const double c = a.resid_c[2 * b + 1];
+    if (fabs(c) > 1e-20) {
+      const double* r = a.R0p + (long long)b * 3 * plane + pix;
+      e += c * *reinterpret_cast<const v4d*>(r);
+      d += c * *reinterpret_cast<const v4d*>(r + plane);
+      p += c * *reinterpret_cast<const v4d*>(r + 2 * plane);
+    }
+  }
+  if (a.test == 1) {
+    u = *reinterpret_cast<const v4d*>(a.uvic + ((long long)b * 2 + 0) * plane + pix);
+    v = *reinterpret_cast<const v4d*>(a.uvic + ((long long)b * 2 + 1) * plane + pix);
+  }
+  if (a.phys_on) {
+    double* ph = phys + pix;
+    const v4d ep = *reinterpret_cast<const v4d*>(ph), ec = *reinterpret_cast<const v4d*>(ph + plane);
+    const v4d dp = *reinterpret_cast<const v4d*>(ph + 2 * plane), dc = *reinterpret_cast<const v4d*>(ph + 3 * plane);
+    const v4d pp = *reinterpret_cast<const v4d*>(ph + 4 * plane), pc = *reinterpret_cast<const v4d*>(ph + 5 * plane);
+    v4d em = ec, dm = dc, pm = pc;
+    if (a.do_ra) {
+      double g = alpha;
+      if (a.raw_filter) {
+        const double w = cf[C_WILLIAMS];
+        g = alpha * w / (1.0 - alpha * (1.0 - w));
+      }
+      em = ec + g * (ep - 2.0 * ec + e);
+      dm = dc + g * (dp - 2.0 * dc + d);
+      pm = pc + g * (pp - 2.0 * pc + p);
+    }
+    *reinterpret_cast<v4d*>(ph) = em;
+    *reinterpret_cast<v4d*>(ph + plane) = e;
+    *reinterpret_cast<v4d*>(ph + 2 * plane) = dm;
+    *reinterpret_cast<v4d*>(ph + 3 * plane) = d;
+    *reinterpret_cast<v4d*>(ph + 4 * plane) = pm;
+    *reinterpret_cast<v4d*>(ph + 5 * plane) = p;
+    *reinterpret_cast<v4d*>(ph + 6 * plane) = u;
+    *reinterpret_cast<v4d*>(ph + 7 * plane) = v;
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    eta[q] = (float)e[q];
+    delta[q] = (float)d[q];
+    Phi[q] = (float)p[q];
+    U[q] = (float)u[q];
+    V[q] = (float)v[q];
+  }
 }
 
 // Assembly of the 9 Legendre inputs of 16 rows (member b, latitudes j0..j0+15)

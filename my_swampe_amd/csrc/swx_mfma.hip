@@ -132,14 +132,25 @@ __device__ __forceinline__ void grid_mfma_body(const GridArgs<float>& a, const f
         c1[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(afr[ks / 4][ks % 4], xb[f * K1p * 16], c1[f], 0, 0, 0);
     }
     // ---- Grid physics on this lane's 4 points (row n, i = ib + 4 lg + q).
+    {
+      float fe[4], fd[4], fp[4], fu[4], fv[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int il = 16 * wave + 4 * lg + q;
-      float prod[8];
-      grid_point_step(a, cf, geo, gl, pc, phys, plane, alpha, b, j, ic + il, c1[0][q], c1[1][q], c1[2][q], c1[3][q],
-                      c1[4][q], prod);
+      for (int q = 0; q < 4; ++q) {
+        fe[q] = c1[0][q];
+        fd[q] = c1[1][q];
+        fp[q] = c1[2][q];
+        fu[q] = c1[3][q];
+        fv[q] = c1[4][q];
+      }
+      grid_bookkeeping4(a, cf, phys, plane, alpha, b, j, ib + 4 * lg, fe, fd, fp, fu, fv);
 #pragma unroll
-      for (int f = 0; f < 8; ++f) sYT[(f * ICH + il) * 16 + n] = prod[f];
+      for (int q = 0; q < 4; ++q) {
+        const int il = 16 * wave + 4 * lg + q;
+        float prod[8];
+        grid_point_step(a, geo, gl, pc, ic + il, fe[q], fp[q], fu[q], fv[q], prod);
+#pragma unroll
+        for (int f = 0; f < 8; ++f) sYT[(f * ICH + il) * 16 + n] = prod[f];
+      }
     }
     __syncthreads();
     // ---- Phase 2: forward DFT partial sums over this chunk, one k-group (16
@@ -317,12 +328,10 @@ __device__ __forceinline__ void grid_mfma_fold_body(const GridArgs<float>& a, co
           const int il = 16 * wave + 4 * lg + q;
           const int i = ic + il;
           float pa[8], pb[8];
-          grid_point_step<true>(a, cf, geo, gl, pc, phys, plane, alpha, b, j, i, cE[0][q] + cO[0][q],
-                                cE[1][q] + cO[1][q], cE[2][q] + cO[2][q], cE[3][q] + cO[3][q], cE[4][q] + cO[4][q],
-                                pa);
-          grid_point_step<true>(a, cf, geo, gl, pc, phys, plane, alpha, b, j, i + H, cE[0][q] - cO[0][q],
-                                cE[1][q] - cO[1][q], cE[2][q] - cO[2][q], cE[3][q] - cO[3][q], cE[4][q] - cO[4][q],
-                                pb);
+          grid_point_step(a, geo, gl, pc, i, cE[0][q] + cO[0][q], cE[2][q] + cO[2][q], cE[3][q] + cO[3][q],
+                          cE[4][q] + cO[4][q], pa);
+          grid_point_step(a, geo, gl, pc, i + H, cE[0][q] - cO[0][q], cE[2][q] - cO[2][q], cE[3][q] - cO[3][q],
+                          cE[4][q] - cO[4][q], pb);
 #pragma unroll
           for (int f = 0; f < 8; ++f) {
             sS[(f * ICH + il) * 16 + n] = pa[f] + pb[f];
@@ -375,15 +384,31 @@ __device__ __forceinline__ void grid_mfma_fold_body(const GridArgs<float>& a, co
       if constexpr (S::NCH == 1) __syncthreads();  // s / d overwrite X^T
       // ---- Physics on the lane's 4 point pairs (row n, i = ib + 4 lg + q and i + H).
       if (active) {
+        // Fields of the lane's 4 longitudes in each half period: h = 0 at i, h = 1 at i + H.
+        float fe[2][4], fd[2][4], fp[2][4], fu[2][4], fv[2][4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          fe[0][q] = cE[0][q] + cO[0][q];
+          fd[0][q] = cE[1][q] + cO[1][q];
+          fp[0][q] = cE[2][q] + cO[2][q];
+          fu[0][q] = cE[3][q] + cO[3][q];
+          fv[0][q] = cE[4][q] + cO[4][q];
+          fe[1][q] = cE[0][q] - cO[0][q];
+          fd[1][q] = cE[1][q] - cO[1][q];
+          fp[1][q] = cE[2][q] - cO[2][q];
+          fu[1][q] = cE[3][q] - cO[3][q];
+          fv[1][q] = cE[4][q] - cO[4][q];
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          grid_bookkeeping4(a, cf, phys, plane, alpha, b, j, ib + 4 * lg + h * H, fe[h], fd[h], fp[h], fu[h], fv[h]);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int il = 16 * wave + 4 * lg + q;
           const int i = ic + il;
           float pa[8], pb[8];
-          grid_point_step(a, cf, geo, gl, pc, phys, plane, alpha, b, j, i, cE[0][q] + cO[0][q], cE[1][q] + cO[1][q],
-                          cE[2][q] + cO[2][q], cE[3][q] + cO[3][q], cE[4][q] + cO[4][q], pa);
-          grid_point_step(a, cf, geo, gl, pc, phys, plane, alpha, b, j, i + H, cE[0][q] - cO[0][q],
-                          cE[1][q] - cO[1][q], cE[2][q] - cO[2][q], cE[3][q] - cO[3][q], cE[4][q] - cO[4][q], pb);
+          grid_point_step(a, geo, gl, pc, i, fe[0][q], fp[0][q], fu[0][q], fv[0][q], pa);
+          grid_point_step(a, geo, gl, pc, i + H, fe[1][q], fp[1][q], fu[1][q], fv[1][q], pb);
 #pragma unroll
           for (int f = 0; f < 8; ++f) {
             sS[(f * ICH + il) * 16 + n] = pa[f] + pb[f];
