@@ -321,7 +321,7 @@ def main():
     peak = PEAK[args.precision]
     grid_kernel = {"fp32": f"k_grid_mfma<{M}>", "bf16x3": f"k_grid_bf16x3<{M}>", "fp64": "k_grid<double>"}[args.precision]
     if args.precision == "fp32" and main_run["fold"] == 2:
-        grid_kernel = f"k_grid_mfma_fold<{M}, true>"
+        grid_kernel = f"k_grid_mfma_fold_w2<{M}, true>"
     out = {
         "metric": METRIC,
         "value": value,

@@ -203,7 +203,10 @@ struct FoldDims {
 };
 bool grid_mfma_fold_dims(int M, FoldDims* d);
 size_t grid_mfma_fold_lds_bytes(int M);
-hipError_t launch_grid_mfma_fold(const GridArgs<float>& a, const float* TinvF, const float* TfwdF, hipStream_t s);
+// shared = true: the launch shares the device with another plan's kernels (swx_pipeline), and
+// the steady fast variant runs at its lower occupancy, which leaves room for them.
+hipError_t launch_grid_mfma_fold(const GridArgs<float>& a, const float* TinvF, const float* TfwdF, hipStream_t s,
+                                 bool shared = false);
 // Split-BF16x3 grid kernel (same eligibility as the MFMA grid kernel); tables
 // of bf16 hi / lo pairs in its fragment order (swx_bf16x3.hip).
 hipError_t launch_grid_bf16x3(const GridArgs<float>& a, const void* TinvB, const void* TfwdB, hipStream_t s);

@@ -457,6 +457,19 @@ __global__ void __launch_bounds__(256) k_grid_mfma_fold(GridArgs<float> a, const
   grid_mfma_fold_body<MT, FAST>(a, TinvF, TfwdF, blockIdx.x, lds);
 }
 
+// T42: the same kernel allocated for at least 2 waves per SIMD.  The compiler then keeps the
+// forward-DFT accumulators in ordinary VGPRs instead of AGPRs: the FP64-bookkeeping variant fits
+// 2 waves (220 registers instead of 274), the fast variant 4 (114 instead of 156).  A plan
+// stepped beside another plan (swx_pipeline) keeps the fast variant at 3 workgroups per CU:
+// measured at T42 x 2 x 128 members, the two plans together run 4 % faster that way, while
+// one plan alone runs 4 % faster at 4.
+template <int MT, bool FAST>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
+k_grid_mfma_fold_w2(GridArgs<float> a, const float* __restrict__ TinvF, const float* __restrict__ TfwdF) {
+  extern __shared__ __align__(16) float lds[];
+  grid_mfma_fold_body<MT, FAST>(a, TinvF, TfwdF, blockIdx.x, lds);
+}
+
 template <int MT>
 __global__ void __launch_bounds__(256) k_grid_mfma(GridArgs<float> a, const float* __restrict__ TinvF,
                                                    const float* __restrict__ TfwdF) {
@@ -501,15 +514,24 @@ static bool grid_fold_fast(const GridArgs<float>& a) {
 
 template <int MT>
 static hipError_t launch_mfma_fold_t(const GridArgs<float>& a, const float* TinvF, const float* TfwdF,
-                                     hipStream_t s) {
+                                     hipStream_t s, bool shared) {
   using S = MfmaFoldShape<MT>;
   dim3 grid((unsigned)((long long)a.B * a.J / S::ROWS));
-  if (grid_fold_fast(a))
-    hipLaunchKernelGGL((k_grid_mfma_fold<MT, true>), grid, dim3(256), sizeof(float) * S::LDS_FAST, s, a, TinvF,
-                       TfwdF);
-  else
-    hipLaunchKernelGGL((k_grid_mfma_fold<MT, false>), grid, dim3(256), sizeof(float) * S::LDS_FLOATS, s, a, TinvF,
-                       TfwdF);
+  const bool fast = grid_fold_fast(a);
+  const size_t smem = sizeof(float) * (fast ? S::LDS_FAST : S::LDS_FLOATS);
+  if constexpr (MT == 42) {
+    if (fast && shared)
+      hipLaunchKernelGGL((k_grid_mfma_fold<MT, true>), grid, dim3(256), smem, s, a, TinvF, TfwdF);
+    else if (fast)
+      hipLaunchKernelGGL((k_grid_mfma_fold_w2<MT, true>), grid, dim3(256), smem, s, a, TinvF, TfwdF);
+    else
+      hipLaunchKernelGGL((k_grid_mfma_fold_w2<MT, false>), grid, dim3(256), smem, s, a, TinvF, TfwdF);
+  } else {
+    if (fast)
+      hipLaunchKernelGGL((k_grid_mfma_fold<MT, true>), grid, dim3(256), smem, s, a, TinvF, TfwdF);
+    else
+      hipLaunchKernelGGL((k_grid_mfma_fold<MT, false>), grid, dim3(256), smem, s, a, TinvF, TfwdF);
+  }
   return hipGetLastError();
 }
 
@@ -542,11 +564,12 @@ bool grid_mfma_fold_dims(int M, FoldDims* d) {
   }
 }
 
-hipError_t launch_grid_mfma_fold(const GridArgs<float>& a, const float* TinvF, const float* TfwdF, hipStream_t s) {
+hipError_t launch_grid_mfma_fold(const GridArgs<float>& a, const float* TinvF, const float* TfwdF, hipStream_t s,
+                                 bool shared) {
   switch (a.M) {
-    case 42: return launch_mfma_fold_t<42>(a, TinvF, TfwdF, s);
-    case 63: return launch_mfma_fold_t<63>(a, TinvF, TfwdF, s);
-    case 106: return launch_mfma_fold_t<106>(a, TinvF, TfwdF, s);
+    case 42: return launch_mfma_fold_t<42>(a, TinvF, TfwdF, s, shared);
+    case 63: return launch_mfma_fold_t<63>(a, TinvF, TfwdF, s, shared);
+    case 106: return launch_mfma_fold_t<106>(a, TinvF, TfwdF, s, shared);
     default: return hipErrorInvalidValue;
   }
 }
@@ -884,6 +907,15 @@ __global__ void __launch_bounds__(256) k_leg_mfma(LegArgs<float> a) {
   leg_mfma_body<MT, MB, GEN>(a, blockIdx.x, blockIdx.y, lds);
 }
 
+// The same kernel allocated for at least 4 waves per SIMD (MFMA accumulators in VGPRs), for the
+// specialized modified-Euler update where that costs no spills: T42 (82 registers at 4
+// members, 5 workgroups per CU with the LDS) and T63 with 2 or 4 members.
+template <int MT, int MB, bool GEN>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) k_leg_mfma_w4(LegArgs<float> a) {
+  extern __shared__ __align__(16) float lds[];
+  leg_mfma_body<MT, MB, GEN>(a, blockIdx.x, blockIdx.y, lds);
+}
+
 // Two plans of different truncations in one launch: block = grp * (M+1) + m, plan 0's first.
 template <int MA, int MBA, int MB, int MBB, bool GEN>
 __global__ void __launch_bounds__(256) k_leg_mfma_pair(LegArgs<float> a0, LegArgs<float> a1, int nblk0) {
@@ -903,8 +935,11 @@ static hipError_t launch_leg_mfma_t(const LegArgs<float>& a, hipStream_t s) {
   using S = LegShape<MT, MB>;
   const size_t smem = sizeof(float) * S::LDS_FLOATS;
   dim3 grid(MT + 1, (a.B + MB - 1) / MB);
+  constexpr bool kW4 = MT == 42 || (MT == 63 && MB <= 4);
   if (leg_generic(a))
     hipLaunchKernelGGL((k_leg_mfma<MT, MB, true>), grid, dim3(256), smem, s, a);
+  else if constexpr (kW4)
+    hipLaunchKernelGGL((k_leg_mfma_w4<MT, MB, false>), grid, dim3(256), smem, s, a);
   else
     hipLaunchKernelGGL((k_leg_mfma<MT, MB, false>), grid, dim3(256), smem, s, a);
   return hipGetLastError();

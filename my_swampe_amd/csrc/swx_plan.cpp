@@ -101,6 +101,7 @@ struct swx_plan {
   void* TfwdB = nullptr;
   bool opt_plain_grid = false;  // SWX_OPT_GRID_KERNEL = 1
   bool opt_full_phys = false;   // SWX_OPT_FULL_PHYS_CARRY = 1
+  bool in_pipeline = false;     // stepped beside other plans by a swx_pipeline
   bool use_leg_mfma = false;    // MFMA Legendre kernel available
   int leg_mb = 0;               // its members per workgroup (timed at plan creation)
   float leg_mb_ms[3] = {0, 0, 0};  // the candidates' measured launch times
@@ -137,6 +138,7 @@ struct swx_plan {
   hipGraphExec_t gexec = nullptr;
   int gexec_steps = 0;
   hipStream_t gexec_stream = nullptr;
+  bool gexec_shared = false;     // in_pipeline when the graph was captured
   size_t dev_bytes = 0;  // device memory held by the plan
 };
 
@@ -335,7 +337,7 @@ hipError_t launch_grid_any(swx_plan* p, const GridArgs<R>& g, hipStream_t s) {
   if constexpr (sizeof(R) == 4) {
     if (grid_uses_mfma(p, g)) {
       if (p->use_bf16x3) return launch_grid_bf16x3(g, p->TinvB, p->TfwdB, s);
-      if (p->has_fold && p->grid_fold == 2) return launch_grid_mfma_fold(g, p->TinvH, p->TfwdH, s);
+      if (p->has_fold && p->grid_fold == 2) return launch_grid_mfma_fold(g, p->TinvH, p->TfwdH, s, p->in_pipeline);
       return launch_grid_mfma(g, p->TinvF, p->TfwdF, s);
     }
   }
@@ -1242,7 +1244,7 @@ int swx_run_graph(swx_plan* p, int t0, int nsteps, int graph_steps, void* stream
   }
   const int graph_end = std::min(phys_from, nsteps - 1);
   if (graph_steps > 0 && graph_end - k >= graph_steps) {
-    if (!p->gexec || p->gexec_steps != graph_steps || p->gexec_stream != s) {
+    if (!p->gexec || p->gexec_steps != graph_steps || p->gexec_stream != s || p->gexec_shared != p->in_pipeline) {
       if (p->gexec) {
         (void)hipGraphExecDestroy(p->gexec);
         p->gexec = nullptr;
@@ -1262,6 +1264,7 @@ int swx_run_graph(swx_plan* p, int t0, int nsteps, int graph_steps, void* stream
       if (ec != hipSuccess) return fail(SWX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ec));
       p->gexec_steps = graph_steps;
       p->gexec_stream = s;
+      p->gexec_shared = p->in_pipeline;
     }
     while (graph_end - k >= graph_steps) {
       SWX_HIP(hipGraphLaunch(p->gexec, s));
@@ -1383,6 +1386,8 @@ int swx_pipeline_create(swx_plan* const* plans, int n, swx_pipeline** out) {
     swx_pipeline_destroy(pl);
     return fail(SWX_ERR_HIP, "pipeline stream or event creation failed");
   }
+  // The plans' captured graphs are re-captured on their next run (swx_run_graph compares the mode).
+  for (int k = 0; k < n; ++k) plans[k]->in_pipeline = true;
   *out = pl;
   return SWX_OK;
 }
@@ -1414,6 +1419,8 @@ int swx_pipeline_run_graph(swx_pipeline* pl, int t0, int nsteps, int graph_steps
 
 void swx_pipeline_destroy(swx_pipeline* pl) {
   if (!pl) return;
+  for (int k = 0; k < pl->n; ++k)
+    if (pl->p[k]) pl->p[k]->in_pipeline = false;
   for (int k = 1; k < kMaxPipeline; ++k) {
     if (pl->s[k]) {
       (void)hipStreamSynchronize(pl->s[k]);
