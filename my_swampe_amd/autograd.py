@@ -110,9 +110,12 @@ def forward_with_checkpoints(plan: Plan, cfg: RolloutConfig, c):
     while done < cfg.nsteps:
         n = min(K, cfg.nsteps - done)
         ckpts.append(plan.state_get())
+        # Only the last segment's final step needs the grid fields (the exported Phi_curr).
+        plan.set_option(_capi.OPT_PHYS_STEPS, 1 if done + n == cfg.nsteps else -1)
         plan.run(t, n)
         t += n
         done += n
+    plan.set_option(_capi.OPT_PHYS_STEPS, 1)
     return ckpts
 
 
@@ -155,10 +158,14 @@ def _forward_observed(plan: Plan, cfg: RolloutConfig, c, obs_steps, observe):
         ckpts.append(plan.state_get())
         while done < seg_end:
             stop = min(seg_end, pending[0][1]) if pending else seg_end
+            # The grid fields are read only where an observation falls.
+            observed = bool(pending) and pending[0][1] == stop
+            plan.set_option(_capi.OPT_PHYS_STEPS, 1 if observed else -1)
             plan.run(cfg.t0 + done, stop - done)
             done = stop
-            if pending and pending[0][1] == done:
+            if observed:
                 observe(pending.pop(0)[0])
+    plan.set_option(_capi.OPT_PHYS_STEPS, 1)
     return ckpts
 
 

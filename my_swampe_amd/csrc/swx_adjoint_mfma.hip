@@ -257,8 +257,8 @@ struct AdjFoldShape {
 };
 
 template <int MT>
-__global__ void __launch_bounds__(256) k_grid_adj_mfma_fold(GridAdjArgs<float> a, const float* __restrict__ TinvF,
-                                                            const float* __restrict__ TfwdF) {
+__device__ __forceinline__ void grid_adj_mfma_fold_body(const GridAdjArgs<float>& a, const float* __restrict__ TinvF,
+                                                        const float* __restrict__ TfwdF) {
   using S = AdjFoldShape<MT>;
   constexpr int Mp = S::Mp, I = S::I, H = S::H, K1 = S::K1, KE4 = S::KE4, KQ1 = S::KQ1, NT2 = S::NT2, TE = S::TE;
   constexpr int ICH = S::ICH, NX = S::NX;
@@ -470,11 +470,28 @@ __global__ void __launch_bounds__(256) k_grid_adj_mfma_fold(GridAdjArgs<float> a
 }
 
 template <int MT>
+__global__ void __launch_bounds__(256) k_grid_adj_mfma_fold(GridAdjArgs<float> a, const float* __restrict__ TinvF,
+                                                            const float* __restrict__ TfwdF) {
+  grid_adj_mfma_fold_body<MT>(a, TinvF, TfwdF);
+}
+
+// T42: allocated for 2+ waves per SIMD (accumulators in VGPRs): 214 registers instead of 318,
+// 2 workgroups per CU instead of 1, no spills (T63 would spill and keeps the kernel above).
+template <int MT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
+k_grid_adj_mfma_fold_w2(GridAdjArgs<float> a, const float* __restrict__ TinvF, const float* __restrict__ TfwdF) {
+  grid_adj_mfma_fold_body<MT>(a, TinvF, TfwdF);
+}
+
+template <int MT>
 static hipError_t launch_adj_mfma_fold_t(const GridAdjArgs<float>& a, const float* TinvH, const float* TfwdH,
                                          hipStream_t s) {
   const size_t smem = sizeof(float) * AdjFoldShape<MT>::LDS_FLOATS;
-  hipLaunchKernelGGL(k_grid_adj_mfma_fold<MT>, dim3((unsigned)((long long)a.B * a.J / 16)), dim3(256), smem, s, a,
-                     TinvH, TfwdH);
+  const dim3 grid((unsigned)((long long)a.B * a.J / 16));
+  if constexpr (MT == 42)
+    hipLaunchKernelGGL(k_grid_adj_mfma_fold_w2<MT>, grid, dim3(256), smem, s, a, TinvH, TfwdH);
+  else
+    hipLaunchKernelGGL(k_grid_adj_mfma_fold<MT>, grid, dim3(256), smem, s, a, TinvH, TfwdH);
   return hipGetLastError();
 }
 
