@@ -129,20 +129,22 @@ def run_fwd_adjoint(M, B, precision, steps, reps=2):
     J, I = (64, 128) if M == 42 else ((96, 192) if M == 63 else (160, 320))
     tgt = torch.zeros((J, I), dtype=torch.float64, device=dev)
     times = []
+    stats = {}
     for r in range(reps + 1):
         c = c0.clone().requires_grad_(True)
         tr, td, dp = (x.clone().requires_grad_(True) for x in (tr0, td0, dp0))
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         phi = rollout_phi(c, tr, td, dp, M=M, nsteps=steps, dt=120.0, a=8.2e7, omega=3.2e-5, Phibar=3e5,
-                          precision=precision)
+                          precision=precision, stats=stats)
         loss = ((phi - tgt) ** 2).mean(dim=(1, 2)).sum()
         loss.backward()
         torch.cuda.synchronize()
         if r > 0:  # first repetition warms up plan creation and caches
             times.append(time.perf_counter() - t0)
     finite = bool(torch.isfinite(c.grad).all()) and bool(torch.isfinite(tr.grad).all())
-    return dict(wall=float(np.median(times)), finite=finite)
+    return dict(wall=float(np.median(times)), finite=finite, checkpoints=stats.get("checkpoints"),
+                segment_states=stats.get("segment_states"))
 
 
 def run_mixed(steps, warmup, B42=256, B63=64, graph_steps=32):
@@ -382,7 +384,8 @@ def main():
         fa_v = na * B / fa["wall"]
         pa_v = na / pa["wall"]
         out["fwd_adjoint"] = {
-            "steps": na, "members": B, "checkpoint_every": int(np.ceil(np.sqrt(na))),
+            "steps": na, "members": B, "checkpoints": fa["checkpoints"], "segment_states": fa["segment_states"],
+            "checkpoint_policy": "every state kept when the trajectory fits 32 GiB, else every ceil(sqrt(steps))",
             "wall_s": fa["wall"], "member_steps_per_s": fa_v, "finite": fa["finite"],
             "plain_fp64_1member_wall_s": pa["wall"], "plain_fp64_1member_member_steps_per_s": pa_v,
             "ratio_vs_plain_fp64_1member": fa_v / pa_v,

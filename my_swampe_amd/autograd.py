@@ -4,11 +4,13 @@
 coefficients ``c`` (B, 3, M+1, N+1) of eta, delta, Phi by ``nsteps`` forced
 modified-Euler steps on the GPU and returns the final geopotential grid
 (B, J, I).  Its backward runs the native reverse-mode kernels
-(swx_adjoint_*): the forward pass keeps only checkpoints of the spectral state
-every K steps (K = ceil(sqrt(nsteps)) by default), and the backward pass
-recomputes each segment from its checkpoint, keeping that segment's states,
-then steps the adjoint back through it.  Memory: (nsteps/K + K) spectral
-states per member instead of the full trajectory.
+(swx_adjoint_*): the forward pass keeps checkpoints of the spectral state every
+K steps, and the backward pass recomputes each segment from its checkpoint,
+keeping that segment's states, then steps the adjoint back through it.  Memory:
+(nsteps/K + K) spectral states per member.  By default K = 1 (every state kept,
+no recompute) when the whole trajectory fits RolloutConfig.checkpoint_budget
+(32 GiB of device memory: 1400 steps of T42 x 256 members), else
+K = ceil(sqrt(nsteps)); checkpoint_every sets K explicitly.
 
 Gradients are returned for c (complex, d/dRe + i d/dIm) and for the
 per-member taurad, taudrag and DPhieq; the other physical constants are fixed.
@@ -49,9 +51,20 @@ class RolloutConfig:
     t0: int = 2
     stats: dict = field(default_factory=dict)
 
+    members: int = 1
+    # Device memory the default checkpoint policy may fill with saved states (bytes).
+    checkpoint_budget: int = 32 << 30
+
+    def state_bytes(self) -> int:
+        return self.members * 3 * (self.M + 1) * (self.M + 1) * 16
+
     def every(self) -> int:
+        """Steps per checkpoint.  Default: every step when the whole trajectory of spectral
+        states fits the budget (no recompute in the reverse sweep), else ceil(sqrt(nsteps))."""
         if self.checkpoint_every:
             return max(1, int(self.checkpoint_every))
+        if self.nsteps * self.state_bytes() <= self.checkpoint_budget:
+            return 1
         return max(1, int(math.ceil(math.sqrt(max(self.nsteps, 1)))))
 
 
@@ -129,7 +142,8 @@ def adjoint_through_checkpoints(plan: Plan, cfg: RolloutConfig, ckpts, gPhi):
     for seg in reversed(range(len(ckpts))):
         start = seg * K
         n = min(K, cfg.nsteps - start)
-        plan.state_set(ckpts[seg])
+        if n > 1:  # recompute the segment's states from its checkpoint
+            plan.state_set(ckpts[seg])
         tape = [ckpts[seg]]
         t = cfg.t0 + start
         for s in range(n - 1):
@@ -181,7 +195,8 @@ def _adjoint_observed(plan: Plan, cfg: RolloutConfig, ckpts, obs_steps, cotangen
     for seg in reversed(range(len(ckpts))):
         start = seg * K
         n = min(K, cfg.nsteps - start)
-        plan.state_set(ckpts[seg])
+        if n > 1:  # recompute the segment's states from its checkpoint
+            plan.state_set(ckpts[seg])
         tape = [ckpts[seg]]
         for s in range(n - 1):
             plan.run(cfg.t0 + start + s, 1)
@@ -385,7 +400,7 @@ def rollout_phi(c, taurad, taudrag, DPhieq, *, M: int, nsteps: int, dt: float, a
     """
     torch = require_gpu()
     dev = torch.device("cuda", torch.cuda.current_device())
-    cfg = RolloutConfig(M=M, nsteps=nsteps, dt=dt, a=a, omega=omega, Phibar=Phibar, K6=K6, K6Phi=K6Phi,
+    cfg = RolloutConfig(members=int(c.shape[0]), M=M, nsteps=nsteps, dt=dt, a=a, omega=omega, Phibar=Phibar, K6=K6, K6Phi=K6Phi,
                         alpha=alpha, precision=precision, checkpoint_every=checkpoint_every, diffflag=diffflag,
                         modalflag=modalflag, forcflag=forcflag, stats=stats if stats is not None else {})
     conv = lambda x: x if torch.is_tensor(x) else torch.tensor(x, dtype=torch.float64, device=dev)  # noqa: E731
@@ -404,7 +419,7 @@ def rollout_phi_series(c, taurad, taudrag, DPhieq, *, obs_steps, M: int, dt: flo
     torch = require_gpu()
     steps = _check_steps(obs_steps)
     dev = torch.device("cuda", torch.cuda.current_device())
-    cfg = RolloutConfig(M=M, nsteps=steps[-1], dt=dt, a=a, omega=omega, Phibar=Phibar, K6=K6, K6Phi=K6Phi,
+    cfg = RolloutConfig(members=int(c.shape[0]), M=M, nsteps=steps[-1], dt=dt, a=a, omega=omega, Phibar=Phibar, K6=K6, K6Phi=K6Phi,
                         alpha=alpha, precision=precision, checkpoint_every=checkpoint_every, diffflag=diffflag,
                         modalflag=modalflag, forcflag=forcflag, stats=stats if stats is not None else {})
     conv = lambda x: x if torch.is_tensor(x) else torch.tensor(x, dtype=torch.float64, device=dev)  # noqa: E731
@@ -432,7 +447,7 @@ def rollout_phase_curve(c, taurad, taudrag, DPhieq, *, obs_steps, phase_angles, 
         ph = ph.expand(len(steps), -1)
     if ph.dim() != 2 or ph.shape[0] != len(steps) or ph.shape[1] == 0:
         raise ValueError("phase_angles must have shape (K,) or (len(obs_steps), K) with K >= 1")
-    cfg = RolloutConfig(M=M, nsteps=steps[-1], dt=dt, a=a, omega=omega, Phibar=Phibar, K6=K6, K6Phi=K6Phi,
+    cfg = RolloutConfig(members=int(c.shape[0]), M=M, nsteps=steps[-1], dt=dt, a=a, omega=omega, Phibar=Phibar, K6=K6, K6Phi=K6Phi,
                         alpha=alpha, precision=precision, checkpoint_every=checkpoint_every, diffflag=diffflag,
                         modalflag=modalflag, forcflag=forcflag, stats=stats if stats is not None else {})
     conv = lambda x: x if torch.is_tensor(x) else torch.tensor(x, dtype=torch.float64, device=dev)  # noqa: E731

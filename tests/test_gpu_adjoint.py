@@ -148,16 +148,19 @@ def test_checkpoint_interval_does_not_change_gradients(grads):
     from my_swampe_amd.autograd import rollout_phi
 
     dev = torch.device("cuda")
-    for every in (1, 4, 6):
+    for every, nck, seg in ((1, 6, 1), (3, 2, 3), (4, 2, 4), (6, 1, 6)):
         c = grads["c"].clone().requires_grad_(True)
         tr = torch.tensor(TAURAD, device=dev, requires_grad=True)
         td = torch.tensor(TAUDRAG, device=dev)
         dp = torch.tensor(DPHIEQ, device=dev)
-        L = _loss(rollout_phi(c, tr, td, dp, **CFG, checkpoint_every=every), grads["tgt"])
+        st = {}
+        L = _loss(rollout_phi(c, tr, td, dp, **CFG, checkpoint_every=every, stats=st), grads["tgt"])
         L.backward()
         np.testing.assert_array_equal(c.grad.cpu().numpy(), grads["gc"].cpu().numpy())
         np.testing.assert_array_equal(tr.grad.cpu().numpy(), grads["gtr"])
-    assert grads["stats"]["checkpoints"] == 2 and grads["stats"]["segment_states"] == 3
+        assert st["checkpoints"] == nck and st["segment_states"] == seg, (every, st)
+    # Default policy: the 6-step trajectory fits the checkpoint budget, so every state is kept.
+    assert grads["stats"]["checkpoints"] == 6 and grads["stats"]["segment_states"] == 1
 
 
 @pytest.mark.parametrize("kernel,M", [("grid", 42), ("grid", 63), ("grid_full", 42), ("grid_full", 63), ("leg", 42),
