@@ -363,8 +363,12 @@ def split_plans(M: int, params: np.ndarray, parts: int = 2, **kw):
     params = np.asarray(params, dtype=np.float64)
     if params.ndim == 1:
         params = params[None, :]
-    B = params.shape[0]
+    bounds = split_bounds(params.shape[0], parts)
+    return [Plan(M, params[bounds[k]:bounds[k + 1]], **kw) for k in range(parts)]
+
+
+def split_bounds(B: int, parts: int):
+    """Member boundaries [0, ..., B] of `parts` consecutive groups whose sizes differ by at most one."""
     if not 1 <= parts <= min(B, 4):
         raise ValueError("parts must be between 1 and min(members, 4)")
-    bounds = np.linspace(0, B, parts + 1).round().astype(int)
-    return [Plan(M, params[bounds[k]:bounds[k + 1]], **kw) for k in range(parts)]
+    return [int(x) for x in np.linspace(0, B, parts + 1).round().astype(int)]
