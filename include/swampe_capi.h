@@ -256,6 +256,15 @@ int swx_adjoint_observe(swx_plan *plan, const double *g_eta, const double *g_del
 int swx_adjoint_terminal(swx_plan *plan, const double *g_eta, const double *g_delta, const double *g_Phi,
                          const double *g_U, const double *g_V, void *stream);
 int swx_adjoint_step(swx_plan *plan, const double *state_t, void *stream);
+/* Forward sweep that records its trajectory for the reverse sweep: for k = 0 .. nsteps-1,
+ * copy the current spectral state into tape[k] (device, [B][3][M+1][N+1] complex128 per k, as
+ * swx_state_get), then advance one step (step index t0 + k); only the last step writes the
+ * physical grid carry.  swx_adjoint_sweep then runs swx_adjoint_step(tape[k]) for
+ * k = nsteps-1 .. 0 (after swx_adjoint_begin and the terminal cotangents).  One call per sweep
+ * instead of one per step: the reverse-mode scan of model.py:1281-1342 without a
+ * rematerialized segment.  Not available with semi_implicit / raw_filter (as swx_state_get). */
+int swx_run_tape(swx_plan *plan, int t0, int nsteps, double *tape, void *stream);
+int swx_adjoint_sweep(swx_plan *plan, int nsteps, const double *tape, void *stream);
 int swx_adjoint_result(swx_plan *plan, double *lam_out, double *pgrad_out, void *stream);
 
 /* Phase-curve observable (replaces the projection of
@@ -311,6 +320,11 @@ int swx_plan_phase_curve(swx_plan *plan, const double *phases, int K, int normal
  *     creation to the faster of the two, timed on the plan's own buffers;
  *     setting it forces one.  Results agree to FP32 rounding. */
 #define SWX_OPT_GRID_FOLD 6
+/*   SWX_OPT_CONCURRENT: 1 = the plan is stepped beside other plans' kernels (swx_pipeline_create
+ *     sets it for its plans; set it when driving plans concurrently on several streams
+ *     yourself): the steady T42 grid launch then keeps a lower occupancy that leaves room for
+ *     the other kernels.  Results do not depend on it. */
+#define SWX_OPT_CONCURRENT 7
 int swx_plan_set_option(swx_plan *plan, int option, int value);
 /* Current value of an option (for SWX_OPT_LEG_MB and SWX_OPT_GRID_FOLD: the
  * choice in use). */

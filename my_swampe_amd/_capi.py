@@ -26,6 +26,7 @@ OPT_LEG_KERNEL = 3
 OPT_PHYS_STEPS = 4
 OPT_LEG_MB = 5
 OPT_GRID_FOLD = 6
+OPT_CONCURRENT = 7
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
 
 PARAM_NAMES = ("dt", "a", "omega", "g", "Phibar", "taurad", "taudrag", "DPhieq", "K6", "K6Phi", "alpha", "blowup_rms",
@@ -40,7 +41,7 @@ EXPORTED_SYMBOLS = (
     "swx_phase_curve", "swx_phase_curve_vjp", "swx_plan_phase_curve", "swx_set_prev_level",
     "swx_adjoint_step", "swx_adjoint_result", "swx_last_error", "swx_version", "swx_plan_device_bytes",
     "swx_ensemble_create", "swx_ensemble_run_graph", "swx_ensemble_destroy", "swx_plan_get_option",
-    "swx_pipeline_create", "swx_pipeline_run_graph", "swx_pipeline_destroy",
+    "swx_pipeline_create", "swx_pipeline_run_graph", "swx_pipeline_destroy", "swx_run_tape", "swx_adjoint_sweep",
 )
 
 STATE_FIELDS = (
@@ -108,6 +109,8 @@ def load() -> ctypes.CDLL:
     lib.swx_adjoint_terminal.argtypes = [c_void_p] + [c_void_p] * 5 + [c_void_p]
     lib.swx_adjoint_observe.argtypes = [c_void_p] + [c_void_p] * 5 + [c_void_p]
     lib.swx_adjoint_step.argtypes = [c_void_p, c_void_p, c_void_p]
+    lib.swx_run_tape.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p]
+    lib.swx_adjoint_sweep.argtypes = [c_void_p, c_int, c_void_p, c_void_p]
     lib.swx_adjoint_result.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p]
     lib.swx_phase_curve.argtypes = [c_void_p, ctypes.c_longlong, c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_int,
                                     c_void_p, c_void_p, c_void_p]
@@ -133,7 +136,7 @@ def load() -> ctypes.CDLL:
                  "swx_adjoint_terminal", "swx_adjoint_step", "swx_adjoint_result", "swx_plan_device_bytes",
                  "swx_ensemble_create", "swx_ensemble_run_graph", "swx_plan_get_option", "swx_adjoint_observe",
                  "swx_phase_curve", "swx_phase_curve_vjp", "swx_plan_phase_curve", "swx_set_prev_level",
-                 "swx_pipeline_create", "swx_pipeline_run_graph"):
+                 "swx_pipeline_create", "swx_pipeline_run_graph", "swx_run_tape", "swx_adjoint_sweep"):
         getattr(lib, name).restype = c_int
     _lib = lib
     return lib

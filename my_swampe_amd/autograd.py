@@ -114,9 +114,12 @@ def _release_plan(cfg: RolloutConfig, plan: Plan) -> None:
 
 
 def forward_with_checkpoints(plan: Plan, cfg: RolloutConfig, c):
-    """Run the rollout from c, returning the checkpoints (spectral states before each segment)."""
+    """Run the rollout from c, returning the checkpoints (spectral states before each segment):
+    with one checkpoint per step, the whole trajectory as one tape tensor (swx_run_tape)."""
     K = cfg.every()
     plan.init_spectral(c)
+    if K == 1:
+        return plan.run_tape(cfg.t0, cfg.nsteps)
     ckpts: List = []
     t = cfg.t0
     done = 0
@@ -137,6 +140,10 @@ def adjoint_through_checkpoints(plan: Plan, cfg: RolloutConfig, ckpts, gPhi):
     K = cfg.every()
     plan.adjoint_begin()
     plan.adjoint_terminal(g_Phi=gPhi)
+    if not isinstance(ckpts, list):  # a run_tape trajectory: one native reverse sweep
+        plan.adjoint_sweep(ckpts)
+        cfg.stats.update(checkpoints=int(ckpts.shape[0]), segment_states=1)
+        return plan.adjoint_result()
     plan.set_option(_capi.OPT_PHYS_STEPS, -1)  # the recompute needs the spectral states only
     peak = 0
     for seg in reversed(range(len(ckpts))):

@@ -1469,6 +1469,9 @@ int swx_plan_set_option(swx_plan* p, int option, int value) {
         return fail(SWX_ERR_ARG, "SWX_OPT_GRID_FOLD: the folded grid kernel is not available for this plan");
       p->grid_fold = value;
       break;
+    case SWX_OPT_CONCURRENT:
+      p->in_pipeline = value != 0;
+      break;
     default:
       return fail(SWX_ERR_ARG, "unknown option " + std::to_string(option));
   }
@@ -1563,6 +1566,7 @@ int swx_plan_get_option(const swx_plan* p, int option, int* value) {
     case SWX_OPT_PHYS_STEPS: *value = p->opt_phys_steps; break;
     case SWX_OPT_LEG_MB: *value = p->leg_mb; break;
     case SWX_OPT_GRID_FOLD: *value = p->grid_fold; break;
+    case SWX_OPT_CONCURRENT: *value = p->in_pipeline ? 1 : 0; break;
     default: return fail(SWX_ERR_ARG, "unknown option " + std::to_string(option));
   }
   return SWX_OK;
@@ -1641,6 +1645,42 @@ int swx_adjoint_observe(swx_plan* p, const double* g_eta, const double* g_delta,
   hipStream_t s = (hipStream_t)stream;
   return p->rsz == 8 ? adjoint_terminal<double>(p, g_eta, g_delta, g_Phi, g_U, g_V, s, 1)
                      : adjoint_terminal<float>(p, g_eta, g_delta, g_Phi, g_U, g_V, s, 1);
+}
+
+int swx_run_tape(swx_plan* p, int t0, int nsteps, double* tape, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!p->initialized) return fail(SWX_ERR_STATE, "swx_run_tape before initialization");
+  if (nsteps < 0) return fail(SWX_ERR_ARG, "nsteps must be >= 0");
+  if (nsteps > 0 && !tape) return fail(SWX_ERR_ARG, "null tape");
+  if (p->d.semi_implicit || p->d.raw_filter)
+    return fail(SWX_ERR_ARG, "spectral tapes hold one level: not available with semi_implicit / raw_filter");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t per = (size_t)p->d.n_members * 3 * (p->geo.M + 1) * (p->geo.N + 1) * 2;  // doubles per state
+  for (int k = 0; k < nsteps; ++k) {
+    if ((rc = copy_state(p, tape + k * per, nullptr, false, false, s))) return rc;
+    // Only the last step writes the physical grid carry (its current level is exact).
+    const bool last = k == nsteps - 1;
+    rc = p->rsz == 8 ? do_steps<double>(p, t0 + k, 1, nullptr, nullptr, nullptr, nullptr, 0, 0, last, s, last ? 0 : 1 << 30)
+                     : do_steps<float>(p, t0 + k, 1, nullptr, nullptr, nullptr, nullptr, 0, 0, last, s, last ? 0 : 1 << 30);
+    if (rc) return rc;
+  }
+  return SWX_OK;
+}
+
+int swx_adjoint_sweep(swx_plan* p, int nsteps, const double* tape, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!p->Lam) return fail(SWX_ERR_STATE, "swx_adjoint_sweep before swx_adjoint_begin");
+  if (nsteps < 0) return fail(SWX_ERR_ARG, "nsteps must be >= 0");
+  if (nsteps > 0 && !tape) return fail(SWX_ERR_ARG, "null tape");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t per = (size_t)p->d.n_members * 3 * (p->geo.M + 1) * (p->geo.N + 1) * 2;
+  for (int k = nsteps - 1; k >= 0; --k) {
+    rc = p->rsz == 8 ? adjoint_step<double>(p, tape + k * per, s) : adjoint_step<float>(p, tape + k * per, s);
+    if (rc) return rc;
+  }
+  return SWX_OK;
 }
 
 int swx_adjoint_step(swx_plan* p, const double* state_t, void* stream) {

@@ -216,6 +216,26 @@ class Plan:
         self._keep_adj_state = t
         _capi.check(self._lib.swx_adjoint_step(self._h, _ptr(t), self._stream(stream)), "swx_adjoint_step")
 
+    def run_tape(self, t0: int, nsteps: int, stream=None):
+        """Advance nsteps steps from step index t0, returning the spectral state before each step as
+        a complex128 tape (nsteps, B, 3, M+1, N+1); only the last step writes the grid carry."""
+        torch = _torch()
+        Mp = self.M + 1
+        tape = torch.empty((int(nsteps), self.B, 3, Mp, Mp), dtype=torch.complex128, device=self.device)
+        _capi.check(self._lib.swx_run_tape(self._h, int(t0), int(nsteps), _ptr(tape), self._stream(stream)),
+                    "swx_run_tape")
+        return tape
+
+    def adjoint_sweep(self, tape, stream=None) -> None:
+        """Reverse steps over a run_tape trajectory (after adjoint_begin and the terminal cotangents)."""
+        torch = _torch()
+        if tape.dtype != torch.complex128 or tuple(tape.shape[1:]) != (self.B, 3, self.M + 1, self.M + 1):
+            raise ValueError("tape must be complex128 (nsteps, B, 3, M+1, N+1)")
+        t = tape.contiguous()
+        self._keep_tape = t
+        _capi.check(self._lib.swx_adjoint_sweep(self._h, int(t.shape[0]), _ptr(t), self._stream(stream)),
+                    "swx_adjoint_sweep")
+
     def adjoint_result(self, stream=None):
         """(dL/dS_0 as complex128 (B, 3, M+1, N+1), dL/d(taurad, taudrag, DPhieq) as float64 (B, 3))."""
         torch = _torch()

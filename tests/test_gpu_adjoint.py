@@ -252,3 +252,4 @@ def test_time_series_rollout_gradient_is_the_sum_of_terminal_gradients(torch_gpu
     for i, (a, b) in enumerate(zip(series, total)):
         a, b = a.cpu().numpy(), b.cpu().numpy()
         np.testing.assert_allclose(a, b, rtol=0, atol=1e-10 * np.abs(b).max(), err_msg=str(i))
+

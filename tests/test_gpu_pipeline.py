@@ -74,6 +74,7 @@ def test_split_ensemble_equals_one_plan(torch_gpu, precision, parts):
 
 
 def test_pipeline_arguments(torch_gpu):
+    from my_swampe_amd import _capi
     from my_swampe_amd.plan import Pipeline, Plan, split_plans
 
     torch = torch_gpu
@@ -87,8 +88,13 @@ def test_pipeline_arguments(torch_gpu):
         split_plans(42, _params(3), 5)
     b = Plan(42, _params(2, seed=3), precision="fp32")
     pipe = Pipeline([a, b])
+    # The pipeline marks its plans as stepped concurrently, and releases them when closed.
+    assert a.get_option(_capi.OPT_CONCURRENT) == 1 and b.get_option(_capi.OPT_CONCURRENT) == 1
     with pytest.raises(Exception):  # run before the plans are initialized
         pipe.run_graph(2, 10, stream=torch.cuda.Stream())
     pipe.close()
+    assert a.get_option(_capi.OPT_CONCURRENT) == 0
+    a.set_option(_capi.OPT_CONCURRENT, 1)
+    assert a.get_option(_capi.OPT_CONCURRENT) == 1
     a.close()
     b.close()
