@@ -38,7 +38,8 @@ struct LegAdjShape {
   static constexpr int XP = c16(20 * MB), XH = c16(12 * MB), XT = XP + XH;
   static constexpr int XS = XT + 1;  // odd row stride: the per-degree rows start in different banks
   static constexpr int JO = J + 4;   // column stride of the GEMM results
-  static constexpr int LDS_A = KNP * XS + XT * JO;
+  // Mode A: the GEMM results overwrite its B operand once every wave has read it.
+  static constexpr int LDS_A = cmaxi(KNP * XS, XT * JO);
   // modes B / TERM
   static constexpr int RP = c16(10 * MB), RH = c16(4 * MB), RT = RP + RH;
   static constexpr int RS = RT + 1;
@@ -52,7 +53,7 @@ __global__ void __launch_bounds__(256) k_leg_adj_a_mfma(LegAdjArgs<float> a) {
   static_assert(KNP == leg_knp(MT), "fragment tables use leg_knp rows");
   extern __shared__ __align__(16) float lds[];
   float* sX = lds;                 // [KNP][XS] GEMM B operand
-  float* sO = lds + KNP * XS;      // [XT][JO] results, column-major
+  float* sO = lds;                 // [XT][JO] results, column-major (after the GEMM, over sX)
   const int m = blockIdx.x, b0 = blockIdx.y * MB;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, ln = lane & 15, lg = lane >> 4;
   const int B = a.B;
@@ -83,11 +84,21 @@ __global__ void __launch_bounds__(256) k_leg_adj_a_mfma(LegAdjArgs<float> a) {
   }
   __syncthreads();
 
-  // Inverse-shaped GEMM: wave w owns latitude tiles w, w+4, ... and every column tile.
+  // Inverse-shaped GEMM: wave w owns latitude tiles w, w+4, ... and every column tile; the
+  // tiles' results stay in registers until every wave has finished reading sX.
   constexpr int CIP = XP / 16, CIH = (XT - XP) / 16;
   constexpr int KSM = KNP / 4;
+  constexpr int NJT = (J / 16 + 3) / 4;  // latitude tiles per wave
   const int ksn = (K + 3) / 4;
-  for (int jt = wave; jt < J / 16; jt += 4) {
+  v4f dP[NJT][CIP], dH[NJT][CIH];
+#pragma unroll
+  for (int u = 0; u < NJT; ++u) {
+    const int jt = wave + 4 * u;
+#pragma unroll
+    for (int c = 0; c < CIP; ++c) dP[u][c] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < CIH; ++c) dH[u][c] = v4f{0.f, 0.f, 0.f, 0.f};
+    if (jt >= J / 16) continue;
     const long long fo = ((long long)(m * (J / 16) + jt) * 64 + lane) * KSM;
     const v4f* pr = reinterpret_cast<const v4f*>(a.invP + fo);
     const v4f* hr = reinterpret_cast<const v4f*>(a.invH + fo);
@@ -98,28 +109,29 @@ __global__ void __launch_bounds__(256) k_leg_adj_a_mfma(LegAdjArgs<float> a) {
       pa[q4] = need ? pr[q4] : v4f{0.f, 0.f, 0.f, 0.f};
       ha[q4] = need ? hr[q4] : v4f{0.f, 0.f, 0.f, 0.f};
     }
-    v4f dP[CIP], dH[CIH];
-#pragma unroll
-    for (int c = 0; c < CIP; ++c) dP[c] = v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int c = 0; c < CIH; ++c) dH[c] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int ks = 0; ks < KSM; ++ks) {
       if (ks < ksn) {
         const float* xr = sX + (4 * ks + lg) * XS + ln;
 #pragma unroll
         for (int c = 0; c < CIP; ++c)
-          dP[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(pa[ks / 4][ks % 4], xr[c * 16], dP[c], 0, 0, 0);
+          dP[u][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(pa[ks / 4][ks % 4], xr[c * 16], dP[u][c], 0, 0, 0);
 #pragma unroll
         for (int c = 0; c < CIH; ++c)
-          dH[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(ha[ks / 4][ks % 4], xr[XP + c * 16], dH[c], 0, 0, 0);
+          dH[u][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(ha[ks / 4][ks % 4], xr[XP + c * 16], dH[u][c], 0, 0, 0);
       }
     }
+  }
+  __syncthreads();  // sX is no longer read: the results overwrite it
+#pragma unroll
+  for (int u = 0; u < NJT; ++u) {
+    const int jt = wave + 4 * u;
+    if (jt >= J / 16) continue;
     float* out = sO + ln * JO + jt * 16 + 4 * lg;
 #pragma unroll
-    for (int c = 0; c < CIP; ++c) *reinterpret_cast<v4f*>(out + c * 16 * JO) = dP[c];
+    for (int c = 0; c < CIP; ++c) *reinterpret_cast<v4f*>(out + c * 16 * JO) = dP[u][c];
 #pragma unroll
-    for (int c = 0; c < CIH; ++c) *reinterpret_cast<v4f*>(out + (XP + c * 16) * JO) = dH[c];
+    for (int c = 0; c < CIH; ++c) *reinterpret_cast<v4f*>(out + (XP + c * 16) * JO) = dH[u][c];
   }
   __syncthreads();
 
