@@ -6,7 +6,10 @@
 //     x[f][i][n] = sum_k Tinv[i][k] X[f][k][n],  k = 2m + (re, im),
 //     Tinv[i][2m] = w_m cos(2 pi m i / I), Tinv[i][2m+1] = -w_m sin(...),
 //     w_0 = 1, w_m = 2 (Hermitian layout, index-based phase).
-//   grid physics (forcing.py, initial_conditions.py:283-292, model.py:988-1006)
+//   grid physics (forcing.py, initial_conditions.py:283-292, model.py:988-1006):
+//     the FP64 bookkeeping of the physical carry, residue and pinned winds where a
+//     step needs it (swx_grid_common.h:grid_bookkeeping4), then the 8 products in
+//     FP32 (grid_products_f32), the precision of the transforms around them
 //   GEMM 2 (forward truncated DFT, spectral_transform.py:290-328):
 //     Y[f][k'][n] = sum_i Tfwd[k'][i] y[f][i][n],  Tfwd[2m][i] = cos / I,
 //     Tfwd[2m+1][i] = -sin / I.
