@@ -37,21 +37,29 @@ def hipcc() -> str:
     return str(cand) if cand.exists() else "hipcc"
 
 
-def build(verbose: bool = True) -> Path:
-    """Compile every source and link the shared library; returns its path."""
+def build(verbose: bool = True, jobs: int = 0) -> Path:
+    """Compile every source (in parallel, `jobs` at a time; 0 = one per CPU up to 8) and link the
+    shared library; returns its path."""
+    from concurrent.futures import ThreadPoolExecutor
+
     LIB_DIR.mkdir(exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src in HIP_SOURCES + HOST_SOURCES:
         path = CSRC / src
         if not path.exists():
             continue
         obj = LIB_DIR / (path.stem + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") or src == "swx_plan.cpp" else []
-        cmd = [hipcc()] + HIPCC_FLAGS + ["-I", str(ROOT / "include"), "-c"] + lang + [str(path), "-o", str(obj)]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append([hipcc()] + HIPCC_FLAGS + ["-I", str(ROOT / "include"), "-c"] + lang + [str(path), "-o", str(obj)])
         objs.append(str(obj))
+    if verbose:
+        for cmd in cmds:
+            print(" ".join(cmd), flush=True)
+    workers = jobs if jobs > 0 else max(1, min(8, os.cpu_count() or 1))
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        for r in pool.map(lambda c: subprocess.run(c, capture_output=not verbose), cmds):
+            if r.returncode != 0:
+                raise subprocess.CalledProcessError(r.returncode, r.args, r.stdout, r.stderr)
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp)] + objs
     if verbose:
