@@ -203,16 +203,12 @@ def run_mixed(steps, warmup, B42=256, B63=64, graph_steps=32):
                 separate_member_steps_per_s=steps * n / walls["separate"], finite=finite)
 
 
-def run_t106_fwd_adjoint(B=64, steps=5000, every=70, precision="fp32"):
-    """T106 x 64 members, dt = 60 s, 5000 steps forward + adjoint with a spectral checkpoint every
-    70 steps: wall time and peak device memory (torch-held checkpoints and segment states at their
-    peak plus the plan's own buffers)."""
+def _t106_rollout(c0, params, steps, every, precision):
     import torch
     from my_swampe_amd.autograd import rollout_phi
 
-    dev = torch.device("cuda")
-    params = member_params(B, dt=60.0)
-    c = torch.tensor(spectral_ics(106, B), device=dev).requires_grad_(True)
+    dev = c0.device
+    c = c0.clone().requires_grad_(True)
     tr, td, dp = (torch.tensor(params[:, k], device=dev).requires_grad_(True) for k in (5, 6, 7))
     tgt = torch.zeros((160, 320), dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
@@ -227,12 +223,49 @@ def run_t106_fwd_adjoint(B=64, steps=5000, every=70, precision="fp32"):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     torch_peak = torch.cuda.max_memory_allocated() - base
-    finite = bool(torch.isfinite(c.grad).all()) and bool(torch.isfinite(tr.grad).all())
-    return dict(members=B, steps=steps, dt=60.0, checkpoint_every=every, precision=precision, wall_s=wall,
-                member_steps_per_s=steps * B / wall, checkpoints=stats.get("checkpoints"),
-                segment_states=stats.get("segment_states"), torch_peak_bytes=int(torch_peak),
-                plan_device_bytes=int(stats.get("plan_device_bytes", 0)),
-                peak_hbm_bytes=int(torch_peak + stats.get("plan_device_bytes", 0)), finite=finite)
+    return dict(wall=wall, stats=stats, torch_peak=int(torch_peak), phi=phi.detach(),
+                grads=[c.grad.detach(), tr.grad.detach(), td.grad.detach(), dp.grad.detach()])
+
+
+def run_t106_fwd_adjoint(B=64, steps=5000, every=70, precision="fp32", compare=("bf16x3", "fp64")):
+    """T106 x 64 members, dt = 60 s, 5000 steps forward + adjoint with a spectral checkpoint every
+    70 steps: wall time and peak device memory (torch-held checkpoints and segment states at their
+    peak plus the plan's own buffers); the same run with the other transform precisions timed
+    beside it, and the final Phi and gradients of the reduced-precision runs against FP64
+    (relative L2: initial spectral state, taurad, taudrag, DPhieq)."""
+    import torch
+
+    dev = torch.device("cuda")
+    params = member_params(B, dt=60.0)
+    c0 = torch.tensor(spectral_ics(106, B), device=dev)
+    runs = {precision: _t106_rollout(c0, params, steps, every, precision)}
+    for prec in compare:
+        if prec != precision:
+            runs[prec] = _t106_rollout(c0, params, steps, every, prec)
+    main = runs[precision]
+    stats = main["stats"]
+    finite = all(bool(torch.isfinite(g).all()) for g in main["grads"])
+    out = dict(members=B, steps=steps, dt=60.0, checkpoint_every=every, precision=precision, wall_s=main["wall"],
+               member_steps_per_s=steps * B / main["wall"], checkpoints=stats.get("checkpoints"),
+               segment_states=stats.get("segment_states"), torch_peak_bytes=main["torch_peak"],
+               plan_device_bytes=int(stats.get("plan_device_bytes", 0)),
+               peak_hbm_bytes=int(main["torch_peak"] + stats.get("plan_device_bytes", 0)), finite=finite)
+    rel = lambda a, b: float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))  # noqa: E731
+    ref = runs.get("fp64")
+    for prec, r in runs.items():
+        if prec == precision:
+            continue
+        out[f"{prec}_wall_s"] = r["wall"]
+        out[f"{prec}_member_steps_per_s"] = steps * B / r["wall"]
+    if ref is not None:
+        for prec, r in runs.items():
+            if prec == "fp64":
+                continue
+            names = ("init_state", "taurad", "taudrag", "DPhieq")
+            out[f"{prec}_vs_fp64_rel_l2"] = dict(
+                final_Phi=rel(r["phi"], ref["phi"]),
+                **{f"grad_{n}": rel(g, gr) for n, g, gr in zip(names, r["grads"], ref["grads"])})
+    return out
 
 
 def cpu_baseline(seconds_target: float = 12.0):
