@@ -287,6 +287,35 @@ def cpu_baseline(seconds_target: float = 12.0):
                 sample=f"FP64 numpy oracle, T42 forced modEuler, 1 member, {steps} steps ({steps // 240} runs of 240)")
 
 
+def accuracy_vs_oracle(steps: int = 2000):
+    """T42, 1 member, forced modified Euler, dt = 120 s, rest + seeded Phi noise (std 100, n <= 10):
+    the final eta, delta, Phi of the HIP path in FP64, FP32 and split-BF16x3 transforms against the
+    FP64 oracle over `steps` steps (max-abs and relative L2 per field), and the oracle's time."""
+    import torch
+    from my_swampe_amd.model import run_model_scan
+    from oracle import swampe_oracle as orc
+
+    eta0, delta0, Phi0, U0, V0 = orc.rest_noise_ic(42, 0, 3.2e-5)
+    kw = dict(M=42, dt=120.0, tmax=steps + 2, Phibar=3e5, omega=3.2e-5, a=8.2e7, taurad=86400.0, taudrag=86400.0,
+              DPhieq=3e5, eta0_init=eta0, delta0_init=delta0, Phi0_init=Phi0, U0_init=U0, V0_init=V0,
+              diagnostics=False, return_history=False)
+    t0 = time.perf_counter()
+    ref = orc.run_model_scan(**kw)["last_state"]
+    oracle_s = time.perf_counter() - t0
+    out = dict(steps=steps, members=1, oracle_seconds=oracle_s)
+    for prec in ("fp64", "fp32", "bf16x3"):
+        st = run_model_scan(**kw, precision=prec)["last_state"]
+        torch.cuda.synchronize()
+        res = {}
+        for f in ("eta", "delta", "Phi"):
+            g = getattr(st, f + "_curr").cpu().numpy()
+            r = getattr(ref, f + "_curr")
+            res[f] = dict(max_abs=float(np.max(np.abs(g - r))),
+                          rel_l2=float(np.linalg.norm(g - r) / max(np.linalg.norm(r), 1e-300)))
+        out[prec] = res
+    return out
+
+
 def measured_traffic(kernel: str, members: int, precision: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter passes (FETCH_SIZE,
     WRITE_SIZE, each scaled by the copy-kernel calibration), or None for another workload."""
@@ -429,6 +458,7 @@ def main():
         out["t106_fwd_adjoint"] = run_t106_fwd_adjoint(precision=args.precision)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
+        out["accuracy_T42_1member"] = accuracy_vs_oracle()
     print(json.dumps(out), flush=True)
 
 
