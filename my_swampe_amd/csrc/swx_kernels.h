@@ -221,7 +221,9 @@ hipError_t launch_grid_mfma_pair(const GridArgs<float>& a0, const float* ti0, co
 hipError_t launch_leg_mfma_pair(const LegArgs<float>& a0, const LegArgs<float>& a1, hipStream_t s);
 // MFMA Legendre kernel (FP32, LEG_STEP without diagnostics).
 // mb = members per workgroup, one of leg_mfma_candidates(M) (the plan picks the fastest).
-hipError_t launch_leg_mfma(const LegArgs<float>& a, int mb, hipStream_t s);
+// shared = true: the launch shares the GPU with concurrently stepped plans (work items in
+// member-group order instead of heaviest first).
+hipError_t launch_leg_mfma(const LegArgs<float>& a, int mb, hipStream_t s, bool shared = false);
 size_t leg_mfma_lds_bytes(int M, int mb);
 int leg_mfma_candidates(int M, int* mbs);  // up to 3 values
 int leg_mfma_default_mb(int M);

@@ -904,19 +904,41 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
   }
 }
 
+// Work-item order of a launch with grid (M+1, ngrp).  heavy_first = 1: workgroup
+// L = x + y (M+1) takes zonal wavenumber m = L / ngrp and member group L % ngrp; dispatch runs in
+// L order, so the heavy items (small m: K = N + 1 - m degrees) start first and the light ones
+// fill the tail, and the workgroups resident at any time share a few m's basis tables (L2
+// locality).  heavy_first = 0: m = x, group = y (every member group's m's in turn), which
+// measures faster when the launch shares the GPU with concurrently stepped plans.
+__device__ __forceinline__ void leg_item(int heavy_first, int& m, int& grp) {
+  if (heavy_first) {
+    const int L = blockIdx.x + blockIdx.y * gridDim.x;
+    m = L / gridDim.y;
+    grp = L % gridDim.y;
+  } else {
+    m = blockIdx.x;
+    grp = blockIdx.y;
+  }
+}
+
 template <int MT, int MB, bool GEN>
-__global__ void __launch_bounds__(256) k_leg_mfma(LegArgs<float> a) {
+__global__ void __launch_bounds__(256) k_leg_mfma(LegArgs<float> a, int heavy_first) {
   extern __shared__ __align__(16) float lds[];
-  leg_mfma_body<MT, MB, GEN>(a, blockIdx.x, blockIdx.y, lds);
+  int m, grp;
+  leg_item(heavy_first, m, grp);
+  leg_mfma_body<MT, MB, GEN>(a, m, grp, lds);
 }
 
 // The same kernel allocated for at least 4 waves per SIMD (MFMA accumulators in VGPRs), for the
 // specialized modified-Euler update where that costs no spills: T42 (82 registers at 4
 // members, 5 workgroups per CU with the LDS) and T63 with 2 or 4 members.
 template <int MT, int MB, bool GEN>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) k_leg_mfma_w4(LegArgs<float> a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_leg_mfma_w4(LegArgs<float> a, int heavy_first) {
   extern __shared__ __align__(16) float lds[];
-  leg_mfma_body<MT, MB, GEN>(a, blockIdx.x, blockIdx.y, lds);
+  int m, grp;
+  leg_item(heavy_first, m, grp);
+  leg_mfma_body<MT, MB, GEN>(a, m, grp, lds);
 }
 
 // Two plans of different truncations in one launch: block = grp * (M+1) + m, plan 0's first.
@@ -934,17 +956,18 @@ __global__ void __launch_bounds__(256) k_leg_mfma_pair(LegArgs<float> a0, LegArg
 static bool leg_generic(const LegArgs<float>& a) { return a.si || a.raw_filter || a.expflag; }
 
 template <int MT, int MB>
-static hipError_t launch_leg_mfma_t(const LegArgs<float>& a, hipStream_t s) {
+static hipError_t launch_leg_mfma_t(const LegArgs<float>& a, hipStream_t s, bool shared) {
   using S = LegShape<MT, MB>;
   const size_t smem = sizeof(float) * S::LDS_FLOATS;
   dim3 grid(MT + 1, (a.B + MB - 1) / MB);
   constexpr bool kW4 = MT == 42 || (MT == 63 && MB <= 4);
+  const int heavy_first = shared ? 0 : 1;
   if (leg_generic(a))
-    hipLaunchKernelGGL((k_leg_mfma<MT, MB, true>), grid, dim3(256), smem, s, a);
+    hipLaunchKernelGGL((k_leg_mfma<MT, MB, true>), grid, dim3(256), smem, s, a, heavy_first);
   else if constexpr (kW4)
-    hipLaunchKernelGGL((k_leg_mfma_w4<MT, MB, false>), grid, dim3(256), smem, s, a);
+    hipLaunchKernelGGL((k_leg_mfma_w4<MT, MB, false>), grid, dim3(256), smem, s, a, heavy_first);
   else
-    hipLaunchKernelGGL((k_leg_mfma<MT, MB, false>), grid, dim3(256), smem, s, a);
+    hipLaunchKernelGGL((k_leg_mfma<MT, MB, false>), grid, dim3(256), smem, s, a, heavy_first);
   return hipGetLastError();
 }
 
@@ -975,16 +998,16 @@ size_t leg_mfma_lds_bytes(int M, int mb) {
   }
 }
 
-hipError_t launch_leg_mfma(const LegArgs<float>& a, int mb, hipStream_t s) {
+hipError_t launch_leg_mfma(const LegArgs<float>& a, int mb, hipStream_t s, bool shared) {
   switch (a.M * 100 + mb) {
-    case 4204: return launch_leg_mfma_t<42, 4>(a, s);
-    case 4208: return launch_leg_mfma_t<42, 8>(a, s);
-    case 6302: return launch_leg_mfma_t<63, 2>(a, s);
-    case 6304: return launch_leg_mfma_t<63, 4>(a, s);
-    case 6308: return launch_leg_mfma_t<63, 8>(a, s);
-    case 10602: return launch_leg_mfma_t<106, 2>(a, s);
-    case 10604: return launch_leg_mfma_t<106, 4>(a, s);
-    case 10608: return launch_leg_mfma_t<106, 8>(a, s);
+    case 4204: return launch_leg_mfma_t<42, 4>(a, s, shared);
+    case 4208: return launch_leg_mfma_t<42, 8>(a, s, shared);
+    case 6302: return launch_leg_mfma_t<63, 2>(a, s, shared);
+    case 6304: return launch_leg_mfma_t<63, 4>(a, s, shared);
+    case 6308: return launch_leg_mfma_t<63, 8>(a, s, shared);
+    case 10602: return launch_leg_mfma_t<106, 2>(a, s, shared);
+    case 10604: return launch_leg_mfma_t<106, 4>(a, s, shared);
+    case 10608: return launch_leg_mfma_t<106, 8>(a, s, shared);
     default: return hipErrorInvalidValue;
   }
 }

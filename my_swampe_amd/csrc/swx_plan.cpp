@@ -347,7 +347,7 @@ hipError_t launch_grid_any(swx_plan* p, const GridArgs<R>& g, hipStream_t s) {
 template <typename R>
 hipError_t launch_leg_any(swx_plan* p, const LegArgs<R>& l, hipStream_t s) {
   if constexpr (sizeof(R) == 4) {
-    if (leg_uses_mfma(p, l)) return launch_leg_mfma(l, p->leg_mb, s);
+    if (leg_uses_mfma(p, l)) return launch_leg_mfma(l, p->leg_mb, s, p->in_pipeline);
   }
   return launch_leg<R>(l, s);
 }
