@@ -54,7 +54,9 @@ __global__ void __launch_bounds__(256) k_leg_adj_a_mfma(LegAdjArgs<float> a) {
   extern __shared__ __align__(16) float lds[];
   float* sX = lds;                 // [KNP][XS] GEMM B operand
   float* sO = lds;                 // [XT][JO] results, column-major (after the GEMM, over sX)
-  const int m = blockIdx.x, b0 = blockIdx.y * MB;
+  int m, grp;
+  leg_item(1, m, grp);
+  const int b0 = grp * MB;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, ln = lane & 15, lg = lane >> 4;
   const int B = a.B;
   const int K = Np - m;
@@ -177,7 +179,9 @@ __global__ void __launch_bounds__(256) k_leg_adj_b_mfma(LegAdjArgs<float> a) {
   static_assert(KNP == leg_knp(MT), "fragment tables use leg_knp rows");
   extern __shared__ __align__(16) float lds[];
   float* sA = lds;  // [J][RS] Fourier cotangents, then [KNP][RS] contractions
-  const int m = blockIdx.x, b0 = blockIdx.y * MB;
+  int m, grp;
+  leg_item(1, m, grp);
+  const int b0 = grp * MB;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, ln = lane & 15, lg = lane >> 4;
   const int B = a.B;
   const int K = Np - m;

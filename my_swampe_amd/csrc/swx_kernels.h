@@ -86,6 +86,23 @@ constexpr int kNRaw = 8;
 // Degree rows padded to whole 16-row MFMA tiles.
 __host__ __device__ constexpr int leg_knp(int N) { return (N + 1 + 15) / 16 * 16; }
 
+// Work-item order of a Legendre launch with grid (M+1, groups).  heavy_first = 1: workgroup
+// L = x + y (M+1) takes zonal wavenumber m = L / groups and group L % groups; dispatch runs in
+// L order, so the heavy items (small m: K = N + 1 - m degrees) start first and the light ones
+// fill the tail, and the workgroups resident at any time share a few m's basis tables (L2
+// locality).  heavy_first = 0: m = x, group = y (every group's m's in turn), which measures
+// faster when the launch shares the GPU with concurrently stepped plans.
+__device__ __forceinline__ void leg_item(int heavy_first, int& m, int& grp) {
+  if (heavy_first) {
+    const int L = blockIdx.x + blockIdx.y * gridDim.x;
+    m = L / gridDim.y;
+    grp = L % gridDim.y;
+  } else {
+    m = blockIdx.x;
+    grp = blockIdx.y;
+  }
+}
+
 // Fourier rows G / Gprev / Gbar are stored m-major: [member][field][m][j]
 // complex, so a Legendre block (one m) reads and writes contiguous j.  Offset
 // in reals of (b, f, m, j):

@@ -62,8 +62,8 @@ __device__ void block_reduce(double (&v)[NV], const int (&op)[NV], double* scrat
 template <typename R>
 __global__ void __launch_bounds__(128) k_leg(LegArgs<R> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int m = blockIdx.x;
-  const int b = blockIdx.y;
+  int m, b;
+  leg_item(1, m, b);
   const int tid = threadIdx.x;
   const int nt = blockDim.x;
   const int M = a.M, N = a.N, J = a.J;

@@ -904,23 +904,6 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
   }
 }
 
-// Work-item order of a launch with grid (M+1, ngrp).  heavy_first = 1: workgroup
-// L = x + y (M+1) takes zonal wavenumber m = L / ngrp and member group L % ngrp; dispatch runs in
-// L order, so the heavy items (small m: K = N + 1 - m degrees) start first and the light ones
-// fill the tail, and the workgroups resident at any time share a few m's basis tables (L2
-// locality).  heavy_first = 0: m = x, group = y (every member group's m's in turn), which
-// measures faster when the launch shares the GPU with concurrently stepped plans.
-__device__ __forceinline__ void leg_item(int heavy_first, int& m, int& grp) {
-  if (heavy_first) {
-    const int L = blockIdx.x + blockIdx.y * gridDim.x;
-    m = L / gridDim.y;
-    grp = L % gridDim.y;
-  } else {
-    m = blockIdx.x;
-    grp = blockIdx.y;
-  }
-}
-
 template <int MT, int MB, bool GEN>
 __global__ void __launch_bounds__(256) k_leg_mfma(LegArgs<float> a, int heavy_first) {
   extern __shared__ __align__(16) float lds[];
