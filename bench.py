@@ -147,6 +147,42 @@ def run_fwd_adjoint(M, B, precision, steps, reps=2):
                 segment_states=stats.get("segment_states"))
 
 
+def run_t63_fp64_adjoint(steps=2000):
+    """T63, 1 member, FP64, `steps` steps of forced modified Euler (dt = 120 s): the scalar loss
+    mean((Phi_final - Phi_target)^2) with the target from a seeded run at taurad = 5e4 s,
+    taudrag = 2e5 s; forward + adjoint to the gradients w.r.t. taurad, taudrag, DPhieq and the
+    initial spectral state, with a checkpoint every ceil(sqrt(steps)) steps (recompute in the
+    reverse sweep).  Wall time of the forward + adjoint (median of 2 after a warm-up)."""
+    import math
+
+    import torch
+    from my_swampe_amd.autograd import rollout_phi
+
+    dev = torch.device("cuda")
+    every = math.isqrt(steps - 1) + 1
+    kw = dict(M=63, nsteps=steps, dt=120.0, a=8.2e7, omega=3.2e-5, Phibar=3e5, precision="fp64",
+              checkpoint_every=every)
+    c0 = torch.tensor(spectral_ics(63, 1), device=dev)
+    with torch.no_grad():
+        tgt = rollout_phi(c0, 5e4, 2e5, 3e5, **kw)
+    times, stats = [], {}
+    for r in range(3):
+        c = c0.clone().requires_grad_(True)
+        tr, td, dp = (torch.tensor([v], dtype=torch.float64, device=dev, requires_grad=True)
+                      for v in (86400.0, 86400.0, 3e5))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        loss = ((rollout_phi(c, tr, td, dp, stats=stats, **kw) - tgt) ** 2).mean()
+        loss.backward()
+        torch.cuda.synchronize()
+        if r > 0:
+            times.append(time.perf_counter() - t0)
+    wall = float(np.median(times))
+    finite = all(bool(torch.isfinite(x.grad).all()) for x in (c, tr, td, dp))
+    return dict(members=1, steps=steps, precision="fp64", checkpoint_every=every,
+                checkpoints=stats.get("checkpoints"), wall_s=wall, member_steps_per_s=steps / wall, finite=finite)
+
+
 def run_mixed(steps, warmup, B42=256, B63=64, graph_steps=32):
     """256 x T42 + 64 x T63 members, FP32 transforms, dt = 120 s, per-member forcing parameters
     (seeds 7 and 8): both truncations in one Legendre and one grid launch per step (Ensemble),
@@ -452,6 +488,7 @@ def main():
             "plain_fp64_1member_wall_s": pa["wall"], "plain_fp64_1member_member_steps_per_s": pa_v,
             "ratio_vs_plain_fp64_1member": fa_v / pa_v,
         }
+        out["t63_fp64_adjoint"] = run_t63_fp64_adjoint()
     if not args.no_mixed and args.precision == "fp32":
         out["mixed_T42x256_T63x64"] = run_mixed(args.steps, args.warmup)
     if not args.no_t106:
