@@ -246,6 +246,8 @@ int leg_mfma_candidates(int M, int* mbs);  // up to 3 values
 int leg_mfma_default_mb(int M);
 int grid_mfma_k2p(int M);
 
+// One single-wave launch that spins for `ticks` of the device wall clock (bounded by it).
+hipError_t launch_wait_ticks(unsigned long long ticks, hipStream_t s);
 template <typename R>
 hipError_t launch_copy_real(double* dst, long long dstride, const R* src, long long sstride, long long n, int B,
                             hipStream_t s);

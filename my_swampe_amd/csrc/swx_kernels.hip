@@ -596,6 +596,18 @@ __global__ void __launch_bounds__(256) k_residue_init(const R* G, R* R0f, double
   }
 }
 
+// One wave that waits `ticks` of the constant-rate wall clock (s_memrealtime) and exits: the
+// probe by which a pipeline checks that two of its streams run at the same time.
+__global__ void __launch_bounds__(64) k_wait_ticks(unsigned long long ticks) {
+  const unsigned long long t0 = wall_clock64();
+  while (wall_clock64() - t0 < ticks) __builtin_amdgcn_s_sleep(8);
+}
+
+hipError_t launch_wait_ticks(unsigned long long ticks, hipStream_t s) {
+  hipLaunchKernelGGL(k_wait_ticks, dim3(1), dim3(64), 0, s, ticks);
+  return hipGetLastError();
+}
+
 // ----------------------------------------------------------------------------
 // Launchers
 // ----------------------------------------------------------------------------

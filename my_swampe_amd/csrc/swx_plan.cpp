@@ -1361,6 +1361,33 @@ void swx_ensemble_destroy(swx_ensemble* e) {
   delete e;
 }
 
+// Whether kernels on streams a and b run at the same time: one single-wave wait of ~200 us of
+// device wall clock on each, both released by one event; on distinct hardware queues the pair
+// takes about one wait, on a shared queue about two.  1 = overlap, 0 = serialized, -1 = error.
+static int streams_overlap(hipStream_t a, hipStream_t b) {
+  int dev = 0, khz = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return -1;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) khz = 100000;
+  const unsigned long long ticks = (unsigned long long)khz / 5;  // 200 us
+  hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+  int res = -1;
+  float ta = 0.f, tb = 0.f;
+  bool ok = true;
+  for (auto& ev : e) ok = ok && hipEventCreate(&ev) == hipSuccess;
+  // Warm both streams up first (a stream's first launch can carry set-up time).
+  ok = ok && launch_wait_ticks(1, a) == hipSuccess && launch_wait_ticks(1, b) == hipSuccess &&
+       hipStreamSynchronize(a) == hipSuccess && hipStreamSynchronize(b) == hipSuccess;
+  ok = ok && hipEventRecord(e[0], a) == hipSuccess && hipStreamWaitEvent(b, e[0], 0) == hipSuccess &&
+       launch_wait_ticks(ticks, a) == hipSuccess && launch_wait_ticks(ticks, b) == hipSuccess &&
+       hipEventRecord(e[1], a) == hipSuccess && hipEventRecord(e[2], b) == hipSuccess &&
+       hipEventSynchronize(e[1]) == hipSuccess && hipEventSynchronize(e[2]) == hipSuccess &&
+       hipEventElapsedTime(&ta, e[0], e[1]) == hipSuccess && hipEventElapsedTime(&tb, e[0], e[2]) == hipSuccess;
+  if (ok) res = (ta > tb ? ta : tb) < 0.3f ? 1 : 0;
+  for (auto& ev : e)
+    if (ev) (void)hipEventDestroy(ev);
+  return res;
+}
+
 int swx_pipeline_create(swx_plan* const* plans, int n, swx_pipeline** out) {
   if (!out) return fail(SWX_ERR_ARG, "null output");
   *out = nullptr;
@@ -1377,11 +1404,32 @@ int swx_pipeline_create(swx_plan* const* plans, int n, swx_pipeline** out) {
   bool ok = hipEventCreateWithFlags(&pl->fork, hipEventDisableTiming) == hipSuccess;
   for (int k = 0; k < n && ok; ++k) {
     pl->p[k] = plans[k];
-    if (k == 0) continue;
-    // Default-priority streams that do not synchronize with the null stream.
+    // One default-priority stream per plan that does not synchronize with the null stream.
+    // Every plan runs on a stream of its own, created here back to back: the runtime gives a
+    // new stream its least-used hardware queue, so these land on distinct queues and the
+    // plans overlap.  (The caller's stream may share a queue with one of them: a plan run on
+    // it could then serialize behind the other.)
     ok = hipStreamCreateWithFlags(&pl->s[k], hipStreamNonBlocking) == hipSuccess &&
          hipEventCreateWithFlags(&pl->join[k], hipEventDisableTiming) == hipSuccess;
   }
+  // The runtime can still hand a new stream a queue that another of these streams uses.  Check
+  // each pair; on a clash keep the clashing stream aside while a new one is created (the
+  // runtime then prefers another queue) and try again, a few times at most.
+  constexpr int kQueueTries = 4;
+  hipStream_t spare[kMaxPipeline * kQueueTries] = {};
+  int nspare = 0;
+  for (int k = 1; k < n && ok; ++k) {
+    for (int tries = 0; tries < kQueueTries && ok; ++tries) {
+      int all = 1;
+      for (int j = 0; j < k && all == 1; ++j) all = streams_overlap(pl->s[j], pl->s[k]);
+      if (all < 0) ok = false;
+      if (all != 0 || tries + 1 == kQueueTries) break;
+      spare[nspare++] = pl->s[k];
+      pl->s[k] = nullptr;
+      ok = hipStreamCreateWithFlags(&pl->s[k], hipStreamNonBlocking) == hipSuccess;
+    }
+  }
+  for (int q = 0; q < nspare; ++q) (void)hipStreamDestroy(spare[q]);
   if (!ok) {
     swx_pipeline_destroy(pl);
     return fail(SWX_ERR_HIP, "pipeline stream or event creation failed");
@@ -1404,13 +1452,13 @@ int swx_pipeline_run_graph(swx_pipeline* pl, int t0, int nsteps, int graph_steps
   if (nsteps <= 0) return SWX_OK;
   // Fork: the internal streams start after the caller's prior work.
   SWX_HIP(hipEventRecord(pl->fork, s0));
-  for (int k = 1; k < pl->n; ++k) SWX_HIP(hipStreamWaitEvent(pl->s[k], pl->fork, 0));
+  for (int k = 0; k < pl->n; ++k) SWX_HIP(hipStreamWaitEvent(pl->s[k], pl->fork, 0));
   for (int k = 0; k < pl->n; ++k) {
-    int rc = swx_run_graph(pl->p[k], t0, nsteps, graph_steps, k == 0 ? stream : (void*)pl->s[k]);
+    int rc = swx_run_graph(pl->p[k], t0, nsteps, graph_steps, (void*)pl->s[k]);
     if (rc) return rc;
   }
   // Join: the caller's later work waits for every plan.
-  for (int k = 1; k < pl->n; ++k) {
+  for (int k = 0; k < pl->n; ++k) {
     SWX_HIP(hipEventRecord(pl->join[k], pl->s[k]));
     SWX_HIP(hipStreamWaitEvent(s0, pl->join[k], 0));
   }
@@ -1421,7 +1469,7 @@ void swx_pipeline_destroy(swx_pipeline* pl) {
   if (!pl) return;
   for (int k = 0; k < pl->n; ++k)
     if (pl->p[k]) pl->p[k]->in_pipeline = false;
-  for (int k = 1; k < kMaxPipeline; ++k) {
+  for (int k = 0; k < kMaxPipeline; ++k) {
     if (pl->s[k]) {
       (void)hipStreamSynchronize(pl->s[k]);
       (void)hipStreamDestroy(pl->s[k]);

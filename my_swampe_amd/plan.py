@@ -345,9 +345,10 @@ class Ensemble:
 
 
 class Pipeline:
-    """2 to 4 initialized plans stepped concurrently (swx_pipeline): plan 0 on the caller's
-    stream, the others on internal streams forked from and joined back to it, so that one plan's
-    Legendre launch overlaps another's grid launch.  Each plan's results are those it gives alone."""
+    """2 to 4 initialized plans stepped concurrently (swx_pipeline): each plan on an internal
+    stream of its own (distinct hardware queues), forked from and joined back to the caller's
+    stream, so that one plan's Legendre launch overlaps another's grid launch.  Each plan's
+    results are those it gives alone."""
 
     def __init__(self, plans):
         self._lib = _capi.load()
