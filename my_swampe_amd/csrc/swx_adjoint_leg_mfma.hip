@@ -63,7 +63,8 @@ __global__ void __launch_bounds__(256) k_leg_adj_a_mfma(LegAdjArgs<float> a) {
   const long long fst = (long long)Mp * Np * 2;
   const CoefLayout cl = CoefLayout::make(MT, J);
 
-  for (int e = tid; e < KNP * XS; e += 256) sX[e] = 0.0f;
+  static_assert(KNP % 4 == 0, "sX is zeroed as float4s");
+  for (int e = tid; e < KNP * XS / 4; e += 256) reinterpret_cast<v4f*>(sX)[e] = v4f{0.f, 0.f, 0.f, 0.f};
   __syncthreads();
   // Spectral part (FP64) per (member, n).
   for (int p = tid; p < MB * K; p += 256) {

@@ -751,7 +751,8 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
       }
     }
   }
-  for (int e = tid; e < S::SX; e += 256) sX[e] = 0.0f;
+  static_assert(S::SA % 4 == 0 && S::SX % 4 == 0, "sX is zeroed as float4s");
+  for (int e = tid; e < S::SX / 4; e += 256) reinterpret_cast<v4f*>(sX)[e] = v4f{0.f, 0.f, 0.f, 0.f};
   constexpr int padH = RT - RP - MB * 8, padP = RP - MB * 10;
   if constexpr (padH > 0)
     for (int e = tid; e < J * padH; e += 256) sA[(e / padH) * RS + RP + MB * 8 + e % padH] = 0.0f;
