@@ -191,8 +191,8 @@ void swx_ensemble_destroy(swx_ensemble *ens);
 /* Concurrent stepping of 2 to 4 initialized plans (for example one ensemble's
  * members split in halves): plan k runs on internal stream k (default priority,
  * created with the pipeline so that the streams sit on distinct hardware queues),
- * forked after the caller's prior work on `stream` and joined before its later
- * work, so that the step kernels of different
+ * forked after the caller's prior work on `stream` (the null stream too) and
+ * joined before its later work, so that the step kernels of different
  * plans overlap on the device (one plan's Legendre launch beside another's
  * grid launch).  swx_pipeline_run_graph advances every plan as swx_run_graph
  * does; results are those of each plan run alone.  The plans must outlive the

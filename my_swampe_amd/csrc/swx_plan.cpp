@@ -1442,8 +1442,9 @@ int swx_pipeline_create(swx_plan* const* plans, int n, swx_pipeline** out) {
 
 int swx_pipeline_run_graph(swx_pipeline* pl, int t0, int nsteps, int graph_steps, void* stream) {
   if (!pl) return fail(SWX_ERR_ARG, "null pipeline");
+  // The caller's stream (the null stream too) only forks and joins: the plans replay their
+  // graphs on the pipeline's own streams.
   hipStream_t s0 = (hipStream_t)stream;
-  if (s0 == nullptr) return fail(SWX_ERR_ARG, "a pipeline runs on a non-default stream");
   for (int k = 0; k < pl->n; ++k) {
     int rc = check_plan(pl->p[k]);
     if (rc) return rc;

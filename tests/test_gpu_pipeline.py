@@ -98,3 +98,41 @@ def test_pipeline_arguments(torch_gpu):
     assert a.get_option(_capi.OPT_CONCURRENT) == 1
     a.close()
     b.close()
+
+
+def test_pipeline_on_the_default_stream(torch_gpu):
+    """The caller's stream only forks and joins (the plans replay their graphs on the pipeline's
+    own streams), so the default stream serves as well; results equal each plan run alone."""
+    from my_swampe_amd.plan import Pipeline, Plan
+
+    torch = torch_gpu
+    print("[gpu-case] pipeline on the default stream", flush=True)
+    M, B = 42, 3
+    c0 = torch.tensor(_ics(M, 2 * B), device="cuda")
+    alone = []
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        for k in range(2):
+            p = Plan(M, _params(B, seed=11 + k), precision="fp32")
+            _pin(p, "fp32")
+            p.init_spectral(c0[k * B:(k + 1) * B].contiguous(), stream=s)
+            p.run_graph(2, 30, graph_steps=8, stream=s)
+            alone.append(p.export(fields=("Phi_curr",), spec=True, stream=s))
+            p.close()
+    s.synchronize()
+    plans = []
+    for k in range(2):
+        p = Plan(M, _params(B, seed=11 + k), precision="fp32")
+        _pin(p, "fp32")
+        p.init_spectral(c0[k * B:(k + 1) * B].contiguous())
+        plans.append(p)
+    pipe = Pipeline(plans)
+    pipe.run_graph(2, 30, graph_steps=8)
+    got = [p.export(fields=("Phi_curr",), spec=True) for p in plans]
+    torch.cuda.synchronize()
+    for want, have in zip(alone, got):
+        for name in ("Phi_curr", "spec"):
+            assert torch.equal(want[name], have[name]), name
+    pipe.close()
+    for p in plans:
+        p.close()
