@@ -3,8 +3,8 @@
 // A plan holds one truncation group (M, J, I) with B ensemble members: the
 // FP64 geometry (host), the basis/twiddle tables on the device in the
 // transform precision, per-member coefficient tables, and the device carry.
-// Stepping is two kernel launches per step (k_leg, k_grid); the forward
-// benchmark driver replays a captured HIP graph of many steps.
+// Stepping is two kernel launches per step (k_leg, k_grid); forward-only
+// stepping (swx_run_graph) replays a captured HIP graph of many steps.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
