@@ -136,3 +136,40 @@ def test_pipeline_on_the_default_stream(torch_gpu):
     pipe.close()
     for p in plans:
         p.close()
+
+
+def test_pipeline_of_two_truncations(torch_gpu):
+    """A T42 plan and a T63 plan stepped concurrently give what each gives alone (bitwise, with
+    the kernel variants pinned to the same choice)."""
+    from my_swampe_amd.plan import Pipeline, Plan
+
+    torch = torch_gpu
+    print("[gpu-case] pipeline T42 + T63", flush=True)
+    s = torch.cuda.Stream()
+    cases = ((42, 4, 21), (63, 2, 22))
+    ics = {M: torch.tensor(_ics(M, B), device="cuda") for M, B, _ in cases}
+    alone = {}
+    with torch.cuda.stream(s):
+        for M, B, seed in cases:
+            p = Plan(M, _params(B, seed=seed), precision="fp32")
+            _pin(p, "fp32")
+            p.init_spectral(ics[M], stream=s)
+            p.run_graph(2, 24, graph_steps=8, stream=s)
+            alone[M] = p.export(fields=("Phi_curr", "U_curr"), spec=True, stream=s)
+            p.close()
+        plans = []
+        for M, B, seed in cases:
+            p = Plan(M, _params(B, seed=seed), precision="fp32")
+            _pin(p, "fp32")
+            p.init_spectral(ics[M], stream=s)
+            plans.append(p)
+        pipe = Pipeline(plans)
+        pipe.run_graph(2, 24, graph_steps=8, stream=s)
+        got = {M: p.export(fields=("Phi_curr", "U_curr"), spec=True, stream=s) for (M, _, _), p in zip(cases, plans)}
+    s.synchronize()
+    for M in alone:
+        for name in ("Phi_curr", "U_curr", "spec"):
+            assert torch.equal(alone[M][name], got[M][name]), (M, name)
+    pipe.close()
+    for p in plans:
+        p.close()
