@@ -226,11 +226,12 @@ struct MfmaFoldShape {
 
 // Wave priority P over the folded kernel's MFMA loops (1 in, 0 after): the sequencer then
 // issues for a wave in a GEMM ahead of waves in the physics or the staging, so the matrix cores
-// stay fed while those catch up.  Measured per launch: T42 x 256 37.7 -> 35.9 us, T106 x 64
-// 148 -> 140 us; T63 x 128 (2 workgroups per CU) 57 -> 68 us, so T63 keeps the default.
-template <int MT, int P>
+// stay fed while those catch up.  Measured per launch of the steady (FAST) variant: T42 x 256
+// 37.7 -> 35.9 us, T106 x 64 148 -> 140 us; T63 x 128 (2 workgroups per CU) 57 -> 68 us, so T63
+// keeps the default, and so does the bookkeeping variant (T42: 71.7 -> 73.8 us).
+template <int MT, bool FAST, int P>
 __device__ __forceinline__ void fold_gemm_prio() {
-  if constexpr (MT != 63) __builtin_amdgcn_s_setprio(P);
+  if constexpr (MT != 63 && FAST) __builtin_amdgcn_s_setprio(P);
 }
 
 // FAST: a step without residue, pinned winds, physical carry, semi-implicit or explicit slots
@@ -314,7 +315,7 @@ __device__ __forceinline__ void grid_mfma_fold_body(const GridArgs<float>& a, co
       const v4f* Ta = reinterpret_cast<const v4f*>(TinvF + ((long long)(ib / 16) * 64 + lane) * KQ1);
 #pragma unroll
       for (int q4 = 0; q4 < KQ1 / 4; ++q4) afr[q4] = Ta[q4];
-      fold_gemm_prio<MT, 1>();
+      fold_gemm_prio<MT, FAST, 1>();
 #pragma unroll
       for (int ks = 0; ks < K1 / 4; ++ks) {
         const float* xb = sXT + (4 * ks + lg) * 16 + ln;
@@ -326,7 +327,7 @@ __device__ __forceinline__ void grid_mfma_fold_body(const GridArgs<float>& a, co
             cO[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(afr[ks / 4][ks % 4], xb[f * K1 * 16], cO[f], 0, 0, 0);
         }
       }
-      fold_gemm_prio<MT, 0>();
+      fold_gemm_prio<MT, FAST, 0>();
     }
     // First k-group of this chunk's forward fragments, in flight during the physics.
     v4f an[NT2];
@@ -367,7 +368,7 @@ __device__ __forceinline__ void grid_mfma_fold_body(const GridArgs<float>& a, co
           }
           __syncthreads();  // (an[t] for the odd-m tiles still holds this chunk's first k-group)
         }
-        fold_gemm_prio<MT, 1>();
+        fold_gemm_prio<MT, FAST, 1>();
 #pragma unroll 1
         for (int gg = 0; gg < ntc; ++gg) {
           v4f ac[NT2];
@@ -393,7 +394,7 @@ __device__ __forceinline__ void grid_mfma_fold_body(const GridArgs<float>& a, co
             }
           }
         }
-        fold_gemm_prio<MT, 0>();
+        fold_gemm_prio<MT, FAST, 0>();
         __syncthreads();
       }
     } else {
