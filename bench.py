@@ -58,11 +58,14 @@ def flops_per_member_step(M: int, J: int, I: int):
     return leg, dft
 
 
-def run_batch(M, B, precision, steps, warmup, graph_steps=32, full_phys=False, si_raw=False, parts=1):
+def run_batch(M, B, precision, steps, warmup, graph_steps=32, full_phys=False, si_raw=False, parts=1, reps=1,
+              clock_warmup_s=0.0):
     """si_raw: the semi-implicit leapfrog with the RAW filter in its production setting
     (dt = 600 s, alpha = 0.05, williams_alpha = 0.53), else forced modified Euler at dt = 120 s.
     parts > 1: the members split into that many plans stepped concurrently (plan.Pipeline); the
-    kernel timings are then not taken."""
+    kernel timings are then not taken.  The timed call of `steps` steps runs `reps` times in a row
+    (the state advancing); `wall` is the median, `walls` all of them.  clock_warmup_s: untimed
+    stepping before the warmup steps, so the timed calls find the GPU clocks ramped."""
     import torch
     from my_swampe_amd import _capi
     from my_swampe_amd.plan import Pipeline, split_plans
@@ -83,14 +86,31 @@ def run_batch(M, B, precision, steps, warmup, graph_steps=32, full_phys=False, s
         for plan in plans:
             plan.init_spectral(c0[b0:b0 + plan.B].contiguous(), stream=stream)
             b0 += plan.B
-        runner.run_graph(2, warmup, graph_steps=graph_steps, stream=stream)
+        t = 2
+        if clock_warmup_s > 0:
+            t1 = time.perf_counter()
+            while time.perf_counter() - t1 < clock_warmup_s:
+                runner.run_graph(t, 50, graph_steps=graph_steps, stream=stream)
+                stream.synchronize()
+                t += 50
+        runner.run_graph(t, warmup, graph_steps=graph_steps, stream=stream)
+        t += warmup
         stream.synchronize()
-        t0 = time.perf_counter()
-        runner.run_graph(2 + warmup, steps, graph_steps=graph_steps, stream=stream)
-        stream.synchronize()
-        wall = time.perf_counter() - t0
+        clock_before = gpu_sclk_mhz()
+        c0 = [plan.step_counts() for plan in plans]
+        walls = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            runner.run_graph(t, steps, graph_steps=graph_steps, stream=stream)
+            stream.synchronize()
+            walls.append(time.perf_counter() - t0)
+            t += steps
+        clock_after = gpu_sclk_mhz()
+        c1 = [plan.step_counts() for plan in plans]
+        counts = {k: sum(b[k] - a[k] for a, b in zip(c0, c1)) for k in c0[0]}
+        wall = float(np.median(walls))
         if parts == 1:
-            ms_leg, ms_grid = plans[0].time_kernels(2 + warmup + steps, 20, stream=stream)
+            ms_leg, ms_grid = plans[0].time_kernels(t, 20, stream=stream)
         exs = [plan.export(fields=("Phi_curr",), stream=stream) for plan in plans]
         stream.synchronize()
     finite = all(bool(torch.isfinite(ex["Phi_curr"]).all()) for ex in exs)
@@ -100,7 +120,25 @@ def run_batch(M, B, precision, steps, warmup, graph_steps=32, full_phys=False, s
         runner.close()
     for plan in plans:
         plan.close()
-    return dict(wall=wall, ms_leg=ms_leg, ms_grid=ms_grid, finite=finite, fold=fold, leg_mb=leg_mb)
+    return dict(wall=wall, walls=walls, ms_leg=ms_leg, ms_grid=ms_grid, finite=finite, fold=fold, leg_mb=leg_mb,
+                counts=counts, clock_mhz=[clock_before, clock_after])
+
+
+def gpu_sclk_mhz():
+    """Current shader clock (MHz) of the visible GPU from the kernel driver's DPM table (the
+    entry marked '*'), or None where it cannot be read."""
+    import glob
+
+    vals = []
+    for path in sorted(glob.glob("/sys/class/drm/card*/device/pp_dpm_sclk")):
+        try:
+            with open(path) as f:
+                for line in f:
+                    if line.rstrip().endswith("*"):
+                        vals.append(int(line.split(":")[1].strip().lower().split("mhz")[0]))
+        except (OSError, ValueError, IndexError):
+            continue
+    return vals[0] if len(vals) == 1 else (vals or None)
 
 
 def spectral_ics(M: int, B: int, omega: float = 3.2e-5):
@@ -385,6 +423,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--precision", default="fp32", choices=("fp32", "fp64", "bf16x3"))
+    ap.add_argument("--reps", type=int, default=3, help="timed calls of --steps steps each (median reported)")
+    ap.add_argument("--clock-warmup", type=float, default=0.3, metavar="S",
+                    help="seconds of untimed stepping before the warmup steps of the first run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -407,14 +448,16 @@ def main():
         print(json.dumps({"error": "no ROCm GPU visible"}))
         sys.exit(1)
     M, J, I, B = 42, 64, 128, args.members
-    # Kernel timings (roofline) on one plan of all B members; the throughput `value` from the same
-    # members split into two plans stepped concurrently (plan.Pipeline), the way the library runs
-    # a large ensemble (one plan's Legendre launch overlaps the other's grid launch).
-    main_run = run_batch(M, B, args.precision, args.steps, args.warmup)
+    # One plan of all B members (one Legendre and one grid launch per step): its throughput is
+    # `value` and its kernel timings give the roofline.  The same members split into two plans
+    # stepped concurrently on two streams (plan.Pipeline) are timed beside it.
+    main_run = run_batch(M, B, args.precision, args.steps, args.warmup, reps=args.reps,
+                         clock_warmup_s=args.clock_warmup)
     parts = 2 if B >= 2 else 1
-    pipe_run = run_batch(M, B, args.precision, args.steps, args.warmup, parts=parts) if parts > 1 else main_run
-    one_plan_value = args.steps * B / main_run["wall"]
-    value = args.steps * B / pipe_run["wall"]
+    pipe_run = (run_batch(M, B, args.precision, args.steps, args.warmup, parts=parts, reps=args.reps)
+                if parts > 1 else main_run)
+    value = args.steps * B / main_run["wall"]
+    pipe_value = args.steps * B / pipe_run["wall"]
     leg_f, dft_f = flops_per_member_step(M, J, I)
     grid_tflops = dft_f * B / (main_run["ms_grid"] * 1e-3) / 1e12
     leg_tflops = leg_f * B / (main_run["ms_leg"] * 1e-3) / 1e12
@@ -429,15 +472,20 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": pipe_run["wall"] / args.steps * 1e3,
+        "ms_per_step": main_run["wall"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": f"{args.precision} transforms, fp64 state",
         "data": "synthetic (seeded rest+noise ICs, per-member forcing params)",
-        "config": {"workload": f"T42x{B} forced modified-Euler forward, dt=120 s, graph-replayed steps",
+        "config": {"workload": f"T42x{B} forced modified-Euler forward, dt=120 s, FP32 transforms",
                    "members": B, "M": M, "J": J, "I": I, "precision": args.precision,
-                   "plans": parts, "one_plan_member_steps_per_s": one_plan_value},
+                   "plans": 1, "timed_calls": args.reps, "value_from": "median of the timed calls",
+                   "call_ms": [w * 1e3 for w in main_run["walls"]],
+                   "timed_step_counts": main_run["counts"],
+                   "gpu_sclk_mhz_before_after": main_run["clock_mhz"],
+                   "two_plan_pipeline": {"plans": parts, "member_steps_per_s": pipe_value,
+                                         "call_ms": [w * 1e3 for w in pipe_run["walls"]]}},
         "roofline": {"bound": "mfma", "achieved": grid_tflops, "peak": peak, "unit": "TFLOP/s",
                      "frac": grid_tflops / peak,
                      "traffic": measured_traffic(grid_kernel, B, args.precision),

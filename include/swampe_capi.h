@@ -207,6 +207,11 @@ void swx_pipeline_destroy(swx_pipeline *pipe);
  * buffers, and the adjoint workspace once allocated.  Caller-owned tensors
  * (checkpoints, outputs) are not included.  No reference counterpart. */
 int swx_plan_device_bytes(const swx_plan *plan, size_t *bytes);
+/* Steps the plan has run since its creation, by how: counts[0] replayed from a captured graph,
+ * counts[1] launched eagerly, counts[2] (of either) advancing the spectral prev lineage,
+ * counts[3] writing the physical grid carry.  Bookkeeping of this build (the reference's jitted
+ * scan has no counterpart): it lets a caller or a benchmark state which path its steps took. */
+int swx_plan_step_counts(const swx_plan *plan, long long *counts);
 
 /* Measurement hook (HIP events on `stream`): run nsteps steps eagerly
  * (t0 .. t0+nsteps-1, the steady physical-carry mode of swx_run_graph) timed
@@ -326,6 +331,16 @@ int swx_plan_phase_curve(swx_plan *plan, const double *phases, int K, int normal
  *     yourself): the steady T42 grid launch then keeps a lower occupancy that leaves room for
  *     the other kernels.  Results do not depend on it. */
 #define SWX_OPT_CONCURRENT 7
+/*   SWX_OPT_PHYS_LINEAGE: how a modified-Euler plan without diagnostics (test != 1) carries the
+ *     physical prev level through a swx_run / swx_run_graph call without history that is longer
+ *     than the Robert-Asselin window: 0 (default) = as a spectral lineage advanced by the
+ *     Legendre kernel in the window steps (no FP64 grid planes streamed), the grid fields rebuilt
+ *     from it and from the current Fourier rows when they are read (export, phase curve, a call
+ *     that writes the grid carry); -1 = the grid carry of SWX_OPT_FULL_PHYS_CARRY /
+ *     SWX_OPT_PHYS_STEPS; k > 0 = the lineage over the last k steps of such a call (the default
+ *     window is exact; shorter ones are for tests).  The two carries give the same fields to the
+ *     rounding of the transforms (FP64 plans ~1e-15 relative, FP32 ~1e-7). */
+#define SWX_OPT_PHYS_LINEAGE 8
 int swx_plan_set_option(swx_plan *plan, int option, int value);
 /* Current value of an option (for SWX_OPT_LEG_MB and SWX_OPT_GRID_FOLD: the
  * choice in use). */

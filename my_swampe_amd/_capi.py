@@ -27,6 +27,7 @@ OPT_PHYS_STEPS = 4
 OPT_LEG_MB = 5
 OPT_GRID_FOLD = 6
 OPT_CONCURRENT = 7
+OPT_PHYS_LINEAGE = 8
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
 
 PARAM_NAMES = ("dt", "a", "omega", "g", "Phibar", "taurad", "taudrag", "DPhieq", "K6", "K6Phi", "alpha", "blowup_rms",
@@ -40,7 +41,7 @@ EXPORTED_SYMBOLS = (
     "swx_init_spectral", "swx_state_get", "swx_state_set", "swx_adjoint_begin", "swx_adjoint_terminal", "swx_adjoint_observe",
     "swx_phase_curve", "swx_phase_curve_vjp", "swx_plan_phase_curve", "swx_set_prev_level",
     "swx_adjoint_step", "swx_adjoint_result", "swx_last_error", "swx_version", "swx_plan_device_bytes",
-    "swx_ensemble_create", "swx_ensemble_run_graph", "swx_ensemble_destroy", "swx_plan_get_option",
+    "swx_plan_step_counts", "swx_ensemble_create", "swx_ensemble_run_graph", "swx_ensemble_destroy", "swx_plan_get_option",
     "swx_pipeline_create", "swx_pipeline_run_graph", "swx_pipeline_destroy", "swx_run_tape", "swx_adjoint_sweep",
 )
 
@@ -119,6 +120,7 @@ def load() -> ctypes.CDLL:
     lib.swx_plan_phase_curve.argtypes = [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]
     lib.swx_time_kernels.argtypes = [c_void_p, c_int, c_int, POINTER(ctypes.c_float), POINTER(ctypes.c_float), c_void_p]
     lib.swx_plan_device_bytes.argtypes = [c_void_p, POINTER(ctypes.c_size_t)]
+    lib.swx_plan_step_counts.argtypes = [c_void_p, POINTER(ctypes.c_longlong)]
     lib.swx_plan_get_option.argtypes = [c_void_p, c_int, POINTER(c_int)]
     lib.swx_ensemble_create.argtypes = [c_void_p, c_void_p, POINTER(c_void_p)]
     lib.swx_ensemble_run_graph.argtypes = [c_void_p, c_int, c_int, c_int, c_void_p]
@@ -134,6 +136,7 @@ def load() -> ctypes.CDLL:
                  "swx_run", "swx_export_state", "swx_run_graph", "swx_time_kernels", "swx_plan_set_option",
                  "swx_init_spectral", "swx_state_get", "swx_state_set", "swx_adjoint_begin",
                  "swx_adjoint_terminal", "swx_adjoint_step", "swx_adjoint_result", "swx_plan_device_bytes",
+                 "swx_plan_step_counts",
                  "swx_ensemble_create", "swx_ensemble_run_graph", "swx_plan_get_option", "swx_adjoint_observe",
                  "swx_phase_curve", "swx_phase_curve_vjp", "swx_plan_phase_curve", "swx_set_prev_level",
                  "swx_pipeline_create", "swx_pipeline_run_graph", "swx_run_tape", "swx_adjoint_sweep"):

@@ -51,6 +51,10 @@ struct StepFlags {
   int raw;     // Robert-Asselin-Williams filter
   int relax;   // semi-implicit: implicit relaxation/drag (forcflag and test None)
   int do_ra;   // time filter active this step (modalflag and t > 2)
+  // Modified Euler: carry the time-filtered prev lineage of the physical fields in the
+  // spectral prev level, S_prev' = S_curr + alpha (S_prev - 2 S_curr + S_new) when filtered,
+  // else S_curr (the same recurrence the physical prev level follows, in spectral form).
+  int lineage;
 };
 
 // Shared geometry table (FP64): mus[J], w[J], s1m[J], cosphi[J], denomE[J],
@@ -155,7 +159,11 @@ struct LegArgs {
   double* resid_c;
   // LEG_PRIME: 1 = transform the prev level instead of the current one.
   int prime_prev;
-  __host__ __device__ StepFlags flags() const { return StepFlags{expflag, diffflag, si, raw_filter, relax, do_ra}; }
+  // LEG_STEP, modified Euler: 1 = advance the spectral prev lineage (StepFlags::lineage).
+  int lineage;
+  __host__ __device__ StepFlags flags() const {
+    return StepFlags{expflag, diffflag, si, raw_filter, relax, do_ra, lineage};
+  }
 };
 
 template <typename R>
@@ -256,6 +264,12 @@ hipError_t launch_copy_real(double* dst, long long dstride, const R* src, long l
 template <typename R>
 hipError_t launch_copy_fourier(double* dst, const R* src, int f, int B, int J, int Mp, hipStream_t s,
                                const R* res = nullptr, const double* rc = nullptr, int lvl = 0);
+// Physical carry of a modified-Euler plan from its Fourier rows: prev levels (planes 0, 2, 4 of
+// phys [B][8][J][I]) = inverse DFT of fields 0..2 of Q (the rows of the spectral prev lineage),
+// current levels and winds (planes 1, 3, 5, 6, 7) = inverse DFT of fields 0..4 of G.
+template <typename R>
+hipError_t launch_phys_synth(const R* G, const R* Q, double* phys, const R* tw, int B, int J, int I, int Mp,
+                             hipStream_t s);
 // Initial fields' residue of semi-implicit / RAW plans (swx_kernels.hip:k_residue_init).
 template <typename R>
 hipError_t launch_residue_init(const R* G, R* R0f, double* R0p, const double* in_eta, const double* in_delta,

@@ -254,6 +254,32 @@ __global__ void __launch_bounds__(128) k_leg(LegArgs<R> a) {
 // Grid kernel
 // ----------------------------------------------------------------------------
 
+// Inverse DFT of Hermitian-symmetric rows at longitude i: x_f = Re c_f0 + 2 sum_{m >= 1}
+// Re(c_fm e^{+2 pi i m i / I}) (spectral_transform.py:331-348 with the real part taken,
+// model.py:973-977) for the fields f0 <= f < NF of sG ([f][Mp][2]); x_f of f < f0 is left
+// unset.  k_grid and k_phys_synth share it, so a row inverted by either rounds the same.
+template <typename R, int NF>
+__device__ __forceinline__ void inv_dft_point(const R* sG, const R* sTc, const R* sTs, int Mp, int I, int i, int f0,
+                                              double x[NF]) {
+  R acc[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) acc[f] = 0;
+  int idx = 0;
+  for (int m = 1; m < Mp; ++m) {
+    idx += i;
+    if (idx >= I) idx -= I;
+    const R c = sTc[idx], s = sTs[idx];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      if (f < f0) continue;
+      acc[f] += sG[(f * Mp + m) * 2] * c - sG[(f * Mp + m) * 2 + 1] * s;
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < NF; ++f)
+    if (f >= f0) x[f] = (double)sG[(f * Mp) * 2] + 2.0 * (double)acc[f];
+}
+
 template <typename R>
 __global__ void __launch_bounds__(256) k_grid(GridArgs<R> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -342,24 +368,8 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs<R> a) {
       if (all_from_g || (a.mode == GRID_INIT && a.diag_winds)) {
         // Inverse DFT of the Hermitian-symmetric row: x = Re c0 + 2 sum_m Re(c_m e^{+i theta})
         // (spectral_transform.py:331-348 with the real part taken, model.py:973-977).
-        const int f0 = all_from_g ? 0 : 3;
-        R acc[kNG];
-#pragma unroll
-        for (int f = 0; f < kNG; ++f) acc[f] = 0;
-        int idx = 0;
-        for (int m = 1; m < Mp; ++m) {
-          idx += i;
-          if (idx >= I) idx -= I;
-          const R c = sTc[idx], s = sTs[idx];
-#pragma unroll
-          for (int f = 0; f < kNG; ++f) {
-            if (f < f0) continue;
-            acc[f] += sG[(f * Mp + m) * 2] * c - sG[(f * Mp + m) * 2 + 1] * s;
-          }
-        }
         double x[kNG];
-#pragma unroll
-        for (int f = 0; f < kNG; ++f) x[f] = (double)sG[(f * Mp) * 2] + 2.0 * (double)acc[f];
+        inv_dft_point<R, kNG>(sG, sTc, sTs, Mp, I, i, all_from_g ? 0 : 3, x);
         if (all_from_g) {
           eta = x[0];
           delta = x[1];
@@ -556,6 +566,46 @@ __global__ void k_copy_fourier(double* dst, const R* src, int f, int J, int Mp, 
   }
 }
 
+// Physical carry of a modified-Euler plan rebuilt from Fourier rows, one block per (row j,
+// member b): the current levels and winds from the current rows G (what the step's grid kernel
+// writes for them), the prev levels from Q, the rows of the spectral prev lineage that the
+// Legendre kernel advances (swx_physics.h:spectral_update, StepFlags::lineage).
+template <typename R>
+__global__ void __launch_bounds__(256) k_phys_synth(const R* G, const R* Q, double* phys, const R* tw, int J, int I,
+                                                    int Mp) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  R* sTc = reinterpret_cast<R*>(smem_raw);  // [I]
+  R* sTs = sTc + I;                          // [I]
+  R* sG = sTs + I;                           // [kNG][Mp][2]
+  R* sQ = sG + kNG * Mp * 2;                 // [3][Mp][2]
+  const int j = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+  for (int i = tid; i < I; i += blockDim.x) {
+    sTc[i] = tw[i];
+    sTs[i] = tw[I + i];
+  }
+  for (int e = tid; e < kNG * Mp * 2; e += blockDim.x) {
+    const int f = e / (Mp * 2), r = e % (Mp * 2);
+    sG[e] = G[fidx(b, f, r >> 1, j, Mp, J) + (r & 1)];
+    if (f < 3) sQ[e] = Q[fidx(b, f, r >> 1, j, Mp, J) + (r & 1)];
+  }
+  __syncthreads();
+  const long long plane = (long long)J * I;
+  double* ph = phys + (long long)b * 8 * plane + (long long)j * I;
+  for (int i = tid; i < I; i += blockDim.x) {
+    double x[kNG], q[3];
+    inv_dft_point<R, kNG>(sG, sTc, sTs, Mp, I, i, 0, x);
+    inv_dft_point<R, 3>(sQ, sTc, sTs, Mp, I, i, 0, q);
+    ph[0 * plane + i] = q[0];
+    ph[1 * plane + i] = x[0];
+    ph[2 * plane + i] = q[1];
+    ph[3 * plane + i] = x[1];
+    ph[4 * plane + i] = q[2];
+    ph[5 * plane + i] = x[2];
+    ph[6 * plane + i] = x[3];
+    ph[7 * plane + i] = x[4];
+  }
+}
+
 // Initial fields' non-band-limited residue of a semi-implicit / RAW plan, one block per
 // (row j, member b).  On entry G holds the raw Fourier rows of the initial fields and R0f
 // the rows of their band-limited part (the inverse Legendre transform of the initial
@@ -649,6 +699,14 @@ hipError_t launch_copy_fourier(double* dst, const R* src, int f, int B, int J, i
 }
 
 template <typename R>
+hipError_t launch_phys_synth(const R* G, const R* Q, double* phys, const R* tw, int B, int J, int I, int Mp,
+                             hipStream_t s) {
+  const size_t lds = (size_t)(2 * I + (kNG + 3) * Mp * 2) * sizeof(R);
+  hipLaunchKernelGGL(k_phys_synth<R>, dim3(J, B), dim3(256), lds, s, G, Q, phys, tw, J, I, Mp);
+  return hipGetLastError();
+}
+
+template <typename R>
 hipError_t launch_residue_init(const R* G, R* R0f, double* R0p, const double* in_eta, const double* in_delta,
                                const double* in_Phi, const R* tw, double* resid_c, int B, int J, int I, int Mp,
                                hipStream_t s) {
@@ -661,6 +719,10 @@ template hipError_t launch_copy_fourier<double>(double*, const double*, int, int
                                                 const double*, const double*, int);
 template hipError_t launch_copy_fourier<float>(double*, const float*, int, int, int, int, hipStream_t, const float*,
                                                const double*, int);
+template hipError_t launch_phys_synth<double>(const double*, const double*, double*, const double*, int, int, int,
+                                              int, hipStream_t);
+template hipError_t launch_phys_synth<float>(const float*, const float*, double*, const float*, int, int, int, int,
+                                             hipStream_t);
 template hipError_t launch_residue_init<double>(const double*, double*, double*, const double*, const double*,
                                                 const double*, const double*, double*, int, int, int, int,
                                                 hipStream_t);

@@ -690,7 +690,8 @@ __device__ __forceinline__ void leg_fwd_frags(const LegArgs<float>& a, int m, in
 // the compiler then specializes (the semi-implicit, RAW and explicit paths of spectral_update
 // hold many more live doubles; dropping them lowers the register count from 152 to 112 at
 // T42, 4 members: 4 waves per SIMD instead of 3).
-template <int MT, int MB, bool GEN = true>
+// LIN (with GEN = false): the step also advances the spectral prev lineage (StepFlags::lineage).
+template <int MT, int MB, bool GEN = true, bool LIN = false>
 __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int m, const int grp, float* lds) {
   using S = LegShape<MT, MB>;
   constexpr int Np = S::Np, Mp = MT + 1, J = S::J, RP = S::RP, RT = S::RT, XP = S::XP, XT = S::XT, JO = S::JO;
@@ -816,6 +817,7 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
     fl.si = 0;
     fl.raw = 0;
     fl.expflag = 0;
+    fl.lineage = LIN ? 1 : 0;
   }
   if (GEN && (a.si || a.raw_filter) && m == 0 && tid < MB && b0 + tid < B)
     residue_step(a.coef + (long long)(b0 + tid) * a.ncoef, fl, a.resid_c + 2 * (b0 + tid));
@@ -919,24 +921,24 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
   }
 }
 
-template <int MT, int MB, bool GEN>
+template <int MT, int MB, bool GEN, bool LIN = false>
 __global__ void __launch_bounds__(256) k_leg_mfma(LegArgs<float> a, int heavy_first) {
   extern __shared__ __align__(16) float lds[];
   int m, grp;
   leg_item(heavy_first, m, grp);
-  leg_mfma_body<MT, MB, GEN>(a, m, grp, lds);
+  leg_mfma_body<MT, MB, GEN, LIN>(a, m, grp, lds);
 }
 
 // The same kernel allocated for at least 4 waves per SIMD (MFMA accumulators in VGPRs), for the
 // specialized modified-Euler update where that costs no spills: T42 (82 registers at 4
 // members, 5 workgroups per CU with the LDS) and T63 with 2 or 4 members.
-template <int MT, int MB, bool GEN>
+template <int MT, int MB, bool GEN, bool LIN = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_leg_mfma_w4(LegArgs<float> a, int heavy_first) {
   extern __shared__ __align__(16) float lds[];
   int m, grp;
   leg_item(heavy_first, m, grp);
-  leg_mfma_body<MT, MB, GEN>(a, m, grp, lds);
+  leg_mfma_body<MT, MB, GEN, LIN>(a, m, grp, lds);
 }
 
 // Two plans of different truncations in one launch: block = grp * (M+1) + m, plan 0's first.
@@ -962,10 +964,16 @@ static hipError_t launch_leg_mfma_t(const LegArgs<float>& a, hipStream_t s, bool
   const int heavy_first = shared ? 0 : 1;
   if (leg_generic(a))
     hipLaunchKernelGGL((k_leg_mfma<MT, MB, true>), grid, dim3(256), smem, s, a, heavy_first);
-  else if constexpr (kW4)
-    hipLaunchKernelGGL((k_leg_mfma_w4<MT, MB, false>), grid, dim3(256), smem, s, a, heavy_first);
-  else
+  else if constexpr (kW4) {
+    if (a.lineage)
+      hipLaunchKernelGGL((k_leg_mfma_w4<MT, MB, false, true>), grid, dim3(256), smem, s, a, heavy_first);
+    else
+      hipLaunchKernelGGL((k_leg_mfma_w4<MT, MB, false>), grid, dim3(256), smem, s, a, heavy_first);
+  } else if (a.lineage) {
+    hipLaunchKernelGGL((k_leg_mfma<MT, MB, false, true>), grid, dim3(256), smem, s, a, heavy_first);
+  } else {
     hipLaunchKernelGGL((k_leg_mfma<MT, MB, false>), grid, dim3(256), smem, s, a, heavy_first);
+  }
   return hipGetLastError();
 }
 
@@ -1038,7 +1046,8 @@ static hipError_t launch_leg_pair_t(const LegArgs<float>& a0, const LegArgs<floa
   const size_t smem = sizeof(float) * (size_t)cmax(LegShape<MA, MBA>::LDS_FLOATS, LegShape<MB, MBB>::LDS_FLOATS);
   const int n0 = (MA + 1) * ((a0.B + MBA - 1) / MBA);
   const int n1 = (MB + 1) * ((a1.B + MBB - 1) / MBB);
-  if (leg_generic(a0) || leg_generic(a1))
+  // The general update also covers the prev lineage of a pair's final window steps.
+  if (leg_generic(a0) || leg_generic(a1) || a0.lineage || a1.lineage)
     hipLaunchKernelGGL((k_leg_mfma_pair<MA, MBA, MB, MBB, true>), dim3(n0 + n1), dim3(256), smem, s, a0, a1, n0);
   else
     hipLaunchKernelGGL((k_leg_mfma_pair<MA, MBA, MB, MBB, false>), dim3(n0 + n1), dim3(256), smem, s, a0, a1, n0);

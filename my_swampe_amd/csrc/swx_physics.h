@@ -392,7 +392,8 @@ __device__ inline void assemble_slots_si(const double* cf, double tc, double wj,
 //     solved per degree in closed form, exponential hyperdiffusion.
 // Time filter (model.py:991-1110): RAW (fl.raw) moves d = S_prev - 2 S_curr + new
 //   into both levels; semi-implicit alone filters the prev lineage (classic RA);
-//   explicit moves current -> previous; modified Euler keeps only the new level.
+//   explicit moves current -> previous; modified Euler keeps only the new level, and with
+//   fl.lineage also advances the classic RA filter of the prev level in spectral form.
 // Returns eff (the stored current level: Fourier rows of eta, delta, Phi) and nw (the
 // new level before the RAW adjustment: the winds are inverted from it).
 // sc_pre: the current level of this (member, m, n) already loaded by the caller, or null.
@@ -404,7 +405,7 @@ __device__ inline void spectral_update(const double* cf, const CoefLayout& cl, d
   const double nn = (double)n * (double)(n + 1);
   // The prev level is read only by the schemes and filters that use it (the modified-Euler
   // step without RAW reads and writes the current level only).
-  const bool need_prev = fl.si || fl.raw || fl.expflag;
+  const bool need_prev = fl.si || fl.raw || fl.expflag || fl.lineage;
   double sc[3][2], sp[3][2];
   for (int f = 0; f < 3; ++f)
     for (int c = 0; c < 2; ++c) {
@@ -490,6 +491,11 @@ __device__ inline void spectral_update(const double* cf, const CoefLayout& cl, d
             fl.do_ra ? sc[f][c] + alpha * (sp[f][c] - 2.0 * sc[f][c] + nw[f][c]) : sc[f][c];
       } else if (fl.expflag) {
         Sprev[f * fstride + so + c] = sc[f][c];
+      } else if (fl.lineage) {
+        // Modified Euler: the prev lineage of the physical fields (model.py:988-1005) in
+        // spectral form; the step itself never reads it.
+        Sprev[f * fstride + so + c] =
+            fl.do_ra ? sc[f][c] + alpha * (sp[f][c] - 2.0 * sc[f][c] + nw[f][c]) : sc[f][c];
       }
       Scur[f * fstride + so + c] = cur;
       eff[f][c] = cur;

@@ -107,6 +107,13 @@ struct swx_plan {
   float leg_mb_ms[3] = {0, 0, 0};  // the candidates' measured launch times
   bool opt_plain_leg = false;   // SWX_OPT_LEG_KERNEL = 1
   int opt_phys_steps = 0;       // SWX_OPT_PHYS_STEPS (0 = automatic, -1 = never)
+  int opt_lineage = 0;          // SWX_OPT_PHYS_LINEAGE (0 = automatic, -1 = off, k > 0 = window)
+  // The grid carry `phys` is behind the state: the last call advanced the physical prev level as
+  // the spectral lineage (S prev level), and `phys` is rebuilt from the rows on demand.
+  bool phys_stale = false;
+  void* Gsyn = nullptr;         // rows of the lineage for that rebuild (allocated on first use)
+  // Steps run since creation: graph-replayed, eager, with the lineage, with the grid carry written.
+  long long nstep[4] = {0, 0, 0, 0};
   double* geo_d = nullptr;
   double* f2 = nullptr;
   double* coef = nullptr;
@@ -394,6 +401,7 @@ int do_init(swx_plan* p, const double* eta0, const double* delta0, const double*
                                    B, p->geo.J, p->geo.I, p->geo.M + 1, s));
   }
   p->initialized = true;
+  p->phys_stale = false;
   return SWX_OK;
 }
 
@@ -424,9 +432,60 @@ int phys_window(const swx_plan* p) {
   return std::max(16, k);
 }
 
+// Modified-Euler plans whose physical prev level follows the classic Robert-Asselin filter
+// alone can carry it as a spectral lineage in the (otherwise unused) spectral prev level: the
+// Legendre kernel advances it in the final steps of a call (swx_physics.h:spectral_update,
+// StepFlags::lineage) and the grid carry is rebuilt from the rows when it is read
+// (rebuild_phys), so no step streams the FP64 grid planes.  Test 1 (pinned winds), diagnostics
+// (the filtered Phi extrema are per-step grid values) and the other schemes keep the grid carry.
+bool lineage_eligible(const swx_plan* p) {
+  return !p->d.semi_implicit && !p->d.raw_filter && !p->d.expflag && p->d.test != 1 && !p->d.diagnostics &&
+         !p->opt_full_phys && p->opt_phys_steps == 0 && p->opt_lineage >= 0;
+}
+
+// Final steps of a call in which the lineage is advanced.  The lineage a call starts from is
+// not read: like the grid carry, its memory contracts by |alpha| per filtered step (an
+// unfiltered step resets it to the current level), so the window of phys_window steps reaches
+// the exported state exactly.  It starts after the call's first step (nsteps > window), whose
+// current level is band-limited, so the initial fields' residue outside the band (which the
+// grid carry holds after an initialization from grids) has left the lineage's reach too.
+int lineage_steps(const swx_plan* p, int nsteps) {
+  if (!lineage_eligible(p)) return 0;
+  const int w = p->opt_lineage > 0 ? p->opt_lineage : phys_window(p);
+  return nsteps > w ? w : 0;
+}
+
+// The grid carry of a plan whose last call left it behind (phys_stale): the prev levels from the
+// inverse transforms of the spectral lineage, the current levels and winds from the current rows.
+template <typename R>
+int rebuild_phys_t(swx_plan* p, void* rows, hipStream_t s) {
+  const int B = p->d.n_members, J = p->geo.J, I = p->geo.I, Mp = p->geo.M + 1;
+  LegArgs<R> l = leg_args<R>(p, LEG_PRIME);
+  l.diagnostics = 0;
+  l.G = (R*)rows;
+  l.prime_prev = 1;
+  SWX_HIP(launch_leg<R>(l, s));
+  SWX_HIP(launch_phys_synth<R>((const R*)p->G, (const R*)rows, p->phys, (const R*)p->tw, B, J, I, Mp, s));
+  p->phys_stale = false;
+  return SWX_OK;
+}
+
+// Bytes of the lineage rows a rebuild writes: [B][kNG][M+1][J] complex.
+size_t lineage_rows_bytes(const swx_plan* p) {
+  return (size_t)p->d.n_members * kNG * p->geo.J * (p->geo.M + 1) * 2 * p->rsz;
+}
+
+int rebuild_phys(swx_plan* p, hipStream_t s) {
+  if (!p->phys_stale) return SWX_OK;
+  int rc;
+  if (!p->Gsyn && (rc = dalloc(p, &p->Gsyn, lineage_rows_bytes(p)))) return rc;
+  return p->rsz == 8 ? rebuild_phys_t<double>(p, p->Gsyn, s) : rebuild_phys_t<float>(p, p->Gsyn, s);
+}
+
 template <typename R>
 int do_steps(swx_plan* p, int t0, int nsteps, double* hist_fields, double* hist_scal, double* hist_phiext,
-             unsigned char* hist_dead, int hist_T, int hist_k0, bool save_last, hipStream_t s, int phys_from = 0) {
+             unsigned char* hist_dead, int hist_T, int hist_k0, bool save_last, hipStream_t s, int phys_from = 0,
+             int lineage_from = 1 << 30, bool capturing = false) {
   const int B = p->d.n_members;
   const size_t gbytes = (size_t)B * kNG * p->geo.J * (p->geo.M + 1) * 2 * p->rsz;
   for (int k = 0; k < nsteps; ++k) {
@@ -439,11 +498,17 @@ int do_steps(swx_plan* p, int t0, int nsteps, double* hist_fields, double* hist_
     l.hist_T = hist_T;
     l.hist_k = hist_k0 + k;
     l.do_ra = filter_on(p, t);
+    l.lineage = k >= lineage_from ? 1 : 0;
     SWX_HIP(launch_leg_any<R>(p, l, s));
     GridArgs<R> g = grid_args<R>(p, GRID_STEP);
     g.do_ra = filter_on(p, t);
     g.phys_on = (k >= phys_from || hist_fields || hist_phiext) ? 1 : 0;
     g.export_raw = last ? 1 : 0;
+    if (!capturing) {
+      p->nstep[1] += 1;
+      p->nstep[2] += l.lineage;
+      p->nstep[3] += g.phys_on;
+    }
     g.hist_fields = hist_fields;
     g.hist_phiext = hist_phiext;
     g.hist_T = hist_T;
@@ -714,7 +779,8 @@ struct swx_ensemble {
 
 namespace {
 
-int ens_steps(swx_ensemble* e, int t0, int nsteps, bool save_last, hipStream_t s, const int* phys_from) {
+int ens_steps(swx_ensemble* e, int t0, int nsteps, bool save_last, hipStream_t s, const int* phys_from,
+              const int* lineage_from, bool capturing = false) {
   for (int k = 0; k < nsteps; ++k) {
     const int t = t0 + k;
     const bool last = save_last && (k == nsteps - 1);
@@ -728,9 +794,15 @@ int ens_steps(swx_ensemble* e, int t0, int nsteps, bool save_last, hipStream_t s
       }
       l[q] = leg_args<float>(p, LEG_STEP);
       l[q].do_ra = filter_on(p, t);
+      l[q].lineage = k >= lineage_from[q] ? 1 : 0;
       g[q] = grid_args<float>(p, GRID_STEP);
       g[q].do_ra = filter_on(p, t);
       g[q].phys_on = k >= phys_from[q] ? 1 : 0;
+      if (!capturing) {
+        p->nstep[1] += 1;
+        p->nstep[2] += l[q].lineage;
+        p->nstep[3] += g[q].phys_on;
+      }
       g[q].export_raw = last ? 1 : 0;
     }
     if (leg_uses_mfma(e->p[0], l[0]) && leg_uses_mfma(e->p[1], l[1])) {
@@ -1162,7 +1234,8 @@ void swx_plan_destroy(swx_plan* p) {
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   void* bufs[] = {p->Lam, p->Tbar, p->Gbar, p->pgrad, p->Pf, p->Hf, p->Pi, p->Hi, p->fragP[0], p->fragP[1],
                   p->fragP[2], p->fragP[3], p->tw, p->TinvF, p->TfwdF, p->TinvH, p->TfwdH, p->TinvB, p->TfwdB, p->geo_d, p->f2, p->coef, p->S, p->T, p->G, p->Gprev,
-                  p->raw, p->lag, p->g0uv, p->phys, p->uvic, p->part_uv, p->dead, p->R0f, p->R0p, p->resid_c, p->lagx};
+                  p->raw, p->lag, p->g0uv, p->phys, p->uvic, p->part_uv, p->dead, p->R0f, p->R0p, p->resid_c, p->lagx,
+                  p->Gsyn};
   for (void* q : bufs)
     if (q) (void)hipFree(q);
   delete p;
@@ -1192,6 +1265,8 @@ int swx_set_prev_level(swx_plan* p, const double* eta_prev, const double* delta_
   const int B = p->d.n_members;
   const long long pl = (long long)p->geo.J * p->geo.I;
   if (eta_prev) {
+    // The current levels and winds stay as the plan holds them.
+    if ((rc = rebuild_phys(p, s))) return rc;
     const double* src[3] = {eta_prev, delta_prev, Phi_prev};
     // Physical planes 0, 2, 4 of [B][8][J][I] are the prev levels of eta, delta, Phi.
     for (int f = 0; f < 3; ++f) SWX_HIP(launch_copy_real<double>(p->phys + 2 * f * pl, 8 * pl, src[f], pl, pl, B, s));
@@ -1210,12 +1285,19 @@ int swx_run(swx_plan* p, int t0, int nsteps, double* hist_fields, double* hist_s
   if (any_hist && (hist_T <= 0 || hist_k0 < 0 || hist_k0 + nsteps > hist_T))
     return fail(SWX_ERR_ARG, "history window out of range");
   hipStream_t s = (hipStream_t)stream;
-  const int phys_from = any_hist ? 0 : std::max(0, nsteps - phys_window(p));
+  const int lin = any_hist ? 0 : lineage_steps(p, nsteps);
+  const int phys_from = lin ? nsteps : (any_hist ? 0 : std::max(0, nsteps - phys_window(p)));
+  const int lineage_from = lin ? nsteps - lin : 1 << 30;
+  // A call that writes the grid carry continues it from the rebuilt prev level.
+  if (!lin && phys_from < nsteps && (rc = rebuild_phys(p, s))) return rc;
   if (p->rsz == 8)
-    return do_steps<double>(p, t0, nsteps, hist_fields, hist_scalars, hist_phiext, hist_dead, hist_T, hist_k0, true, s,
-                            phys_from);
-  return do_steps<float>(p, t0, nsteps, hist_fields, hist_scalars, hist_phiext, hist_dead, hist_T, hist_k0, true, s,
-                         phys_from);
+    rc = do_steps<double>(p, t0, nsteps, hist_fields, hist_scalars, hist_phiext, hist_dead, hist_T, hist_k0, true, s,
+                          phys_from, lineage_from);
+  else
+    rc = do_steps<float>(p, t0, nsteps, hist_fields, hist_scalars, hist_phiext, hist_dead, hist_T, hist_k0, true, s,
+                         phys_from, lineage_from);
+  if (rc == SWX_OK && lin && nsteps > 0) p->phys_stale = true;
+  return rc;
 }
 
 int swx_run_graph(swx_plan* p, int t0, int nsteps, int graph_steps, void* stream) {
@@ -1225,15 +1307,20 @@ int swx_run_graph(swx_plan* p, int t0, int nsteps, int graph_steps, void* stream
   if (nsteps <= 0) return SWX_OK;
   hipStream_t s = (hipStream_t)stream;
   if (graph_steps > 0 && s == nullptr) return fail(SWX_ERR_ARG, "graph replay needs a non-default stream");
-  // phys_on: from step phys_from of this call (the window that reaches the exported state).
-  const int window = phys_window(p);
-  const int phys_from = std::max(0, nsteps - window);
+  // phys_on: from step phys_from of this call (the window that reaches the exported state), or
+  // the spectral lineage from step lineage_from instead (lineage_steps).
+  const int lin = lineage_steps(p, nsteps);
+  const int phys_from = lin ? nsteps : std::max(0, nsteps - phys_window(p));
+  const int lineage_from = lin ? nsteps - lin : 1 << 30;
+  if (!lin && phys_from < nsteps && (rc = rebuild_phys(p, s))) return rc;
   auto steps = [&](int k0, int n, bool save_last) {
     const int pf = std::max(0, phys_from - k0);
+    const int lf = std::max(0, lineage_from - k0);
     if (p->rsz == 8)
-      return do_steps<double>(p, t0 + k0, n, nullptr, nullptr, nullptr, nullptr, 0, 0, save_last, s, pf);
-    return do_steps<float>(p, t0 + k0, n, nullptr, nullptr, nullptr, nullptr, 0, 0, save_last, s, pf);
+      return do_steps<double>(p, t0 + k0, n, nullptr, nullptr, nullptr, nullptr, 0, 0, save_last, s, pf, lf);
+    return do_steps<float>(p, t0 + k0, n, nullptr, nullptr, nullptr, nullptr, 0, 0, save_last, s, pf, lf);
   };
+  if (lin) p->phys_stale = true;
   // Steps with t <= 2 differ (no Robert-Asselin filter) and run eagerly, as do
   // the steps of the physical-carry window and the last step (which saves the
   // previous Fourier level).
@@ -1242,7 +1329,7 @@ int swx_run_graph(swx_plan* p, int t0, int nsteps, int graph_steps, void* stream
     if ((rc = steps(k, 1, k == nsteps - 1))) return rc;
     ++k;
   }
-  const int graph_end = std::min(phys_from, nsteps - 1);
+  const int graph_end = std::min(std::min(phys_from, lineage_from), nsteps - 1);
   if (graph_steps > 0 && graph_end - k >= graph_steps) {
     if (!p->gexec || p->gexec_steps != graph_steps || p->gexec_stream != s || p->gexec_shared != p->in_pipeline) {
       if (p->gexec) {
@@ -1253,9 +1340,11 @@ int swx_run_graph(swx_plan* p, int t0, int nsteps, int graph_steps, void* stream
       SWX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       // Captured steps have t > 2 and lie before the window (phys_on = 0).
       if (p->rsz == 8)
-        rc = do_steps<double>(p, 3, graph_steps, nullptr, nullptr, nullptr, nullptr, 0, 0, false, s, 1 << 30);
+        rc = do_steps<double>(p, 3, graph_steps, nullptr, nullptr, nullptr, nullptr, 0, 0, false, s, 1 << 30,
+                              1 << 30, true);
       else
-        rc = do_steps<float>(p, 3, graph_steps, nullptr, nullptr, nullptr, nullptr, 0, 0, false, s, 1 << 30);
+        rc = do_steps<float>(p, 3, graph_steps, nullptr, nullptr, nullptr, nullptr, 0, 0, false, s, 1 << 30,
+                             1 << 30, true);
       hipError_t ec = hipStreamEndCapture(s, &graph);
       if (rc) return rc;
       if (ec != hipSuccess) return fail(SWX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
@@ -1269,6 +1358,7 @@ int swx_run_graph(swx_plan* p, int t0, int nsteps, int graph_steps, void* stream
     while (graph_end - k >= graph_steps) {
       SWX_HIP(hipGraphLaunch(p->gexec, s));
       k += graph_steps;
+      p->nstep[0] += graph_steps;
     }
   }
   if (k < nsteps) return steps(k, nsteps - k, true);
@@ -1314,11 +1404,19 @@ int swx_ensemble_run_graph(swx_ensemble* e, int t0, int nsteps, int graph_steps,
   if (nsteps <= 0) return SWX_OK;
   hipStream_t s = (hipStream_t)stream;
   if (graph_steps > 0 && s == nullptr) return fail(SWX_ERR_ARG, "graph replay needs a non-default stream");
-  int pf[2];
-  for (int q = 0; q < 2; ++q) pf[q] = std::max(0, nsteps - phys_window(e->p[q]));
+  int pf[2], lf[2];
+  for (int q = 0; q < 2; ++q) {
+    swx_plan* p = e->p[q];
+    const int lin = lineage_steps(p, nsteps);
+    pf[q] = lin ? nsteps : std::max(0, nsteps - phys_window(p));
+    lf[q] = lin ? nsteps - lin : 1 << 30;
+    if (!lin && pf[q] < nsteps && (rc = rebuild_phys(p, s))) return rc;
+    if (lin) p->phys_stale = true;
+  }
   auto steps = [&](int k0, int n, bool save_last) {
     const int pk[2] = {std::max(0, pf[0] - k0), std::max(0, pf[1] - k0)};
-    return ens_steps(e, t0 + k0, n, save_last, s, pk);
+    const int lk[2] = {std::max(0, lf[0] - k0), std::max(0, lf[1] - k0)};
+    return ens_steps(e, t0 + k0, n, save_last, s, pk, lk);
   };
   // As swx_run_graph: t <= 2, the physical-carry windows and the last step run eagerly.
   int k = 0;
@@ -1326,7 +1424,7 @@ int swx_ensemble_run_graph(swx_ensemble* e, int t0, int nsteps, int graph_steps,
     if ((rc = steps(k, 1, k == nsteps - 1))) return rc;
     ++k;
   }
-  const int graph_end = std::min(std::min(pf[0], pf[1]), nsteps - 1);
+  const int graph_end = std::min(std::min(std::min(pf[0], pf[1]), std::min(lf[0], lf[1])), nsteps - 1);
   if (graph_steps > 0 && graph_end - k >= graph_steps) {
     if (!e->gexec || e->gexec_steps != graph_steps || e->gexec_stream != s) {
       if (e->gexec) {
@@ -1336,7 +1434,7 @@ int swx_ensemble_run_graph(swx_ensemble* e, int t0, int nsteps, int graph_steps,
       hipGraph_t graph = nullptr;
       SWX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       const int off[2] = {1 << 30, 1 << 30};
-      rc = ens_steps(e, 3, graph_steps, false, s, off);
+      rc = ens_steps(e, 3, graph_steps, false, s, off, off, true);
       hipError_t ec = hipStreamEndCapture(s, &graph);
       if (rc) return rc;
       if (ec != hipSuccess) return fail(SWX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
@@ -1349,6 +1447,8 @@ int swx_ensemble_run_graph(swx_ensemble* e, int t0, int nsteps, int graph_steps,
     while (graph_end - k >= graph_steps) {
       SWX_HIP(hipGraphLaunch(e->gexec, s));
       k += graph_steps;
+      e->p[0]->nstep[0] += graph_steps;
+      e->p[1]->nstep[0] += graph_steps;
     }
   }
   if (k < nsteps) return steps(k, nsteps - k, true);
@@ -1508,6 +1608,10 @@ int swx_plan_set_option(swx_plan* p, int option, int value) {
       p->leg_mb = value;
       break;
     }
+    case SWX_OPT_PHYS_LINEAGE:
+      if (value < -1) return fail(SWX_ERR_ARG, "SWX_OPT_PHYS_LINEAGE must be >= -1");
+      p->opt_lineage = value;
+      break;
     case SWX_OPT_PHYS_STEPS:
       if (value < -1) return fail(SWX_ERR_ARG, "SWX_OPT_PHYS_STEPS must be >= -1");
       p->opt_phys_steps = value;
@@ -1613,11 +1717,18 @@ int swx_plan_get_option(const swx_plan* p, int option, int* value) {
     case SWX_OPT_FULL_PHYS_CARRY: *value = p->opt_full_phys ? 1 : 0; break;
     case SWX_OPT_LEG_KERNEL: *value = p->opt_plain_leg ? 1 : 0; break;
     case SWX_OPT_PHYS_STEPS: *value = p->opt_phys_steps; break;
+    case SWX_OPT_PHYS_LINEAGE: *value = p->opt_lineage; break;
     case SWX_OPT_LEG_MB: *value = p->leg_mb; break;
     case SWX_OPT_GRID_FOLD: *value = p->grid_fold; break;
     case SWX_OPT_CONCURRENT: *value = p->in_pipeline ? 1 : 0; break;
     default: return fail(SWX_ERR_ARG, "unknown option " + std::to_string(option));
   }
+  return SWX_OK;
+}
+
+int swx_plan_step_counts(const swx_plan* p, long long* counts) {
+  if (!p || !counts) return fail(SWX_ERR_ARG, "null plan or output");
+  for (int k = 0; k < 4; ++k) counts[k] = p->nstep[k];
   return SWX_OK;
 }
 
@@ -1638,6 +1749,7 @@ int swx_init_spectral(swx_plan* p, const double* c, void* stream) {
   rc = p->rsz == 8 ? prime<double>(p, true, s) : prime<float>(p, true, s);
   if (rc) return rc;
   p->initialized = true;
+  p->phys_stale = false;
   return SWX_OK;
 }
 
@@ -1840,6 +1952,10 @@ int swx_export_state(swx_plan* p, const swx_state_out* o, void* stream) {
       (rc = cp(o->Fm_curr, p->raw, kNRaw, 6, cjm)) || (rc = cp(o->Gm_curr, p->raw, kNRaw, 7, cjm)) ||
       (rc = cp(o->Rum_lag, lagsrc, 2, 0, cjm)) || (rc = cp(o->Rvm_lag, lagsrc, 2, 1, cjm)))
     return rc;
+  if ((o->eta_prev || o->eta_curr || o->delta_prev || o->delta_curr || o->Phi_prev || o->Phi_curr || o->U_curr ||
+       o->V_curr) &&
+      (rc = rebuild_phys(p, s)))
+    return rc;
   if ((rc = cpd(o->eta_prev, p->phys, 8, 0, pl)) || (rc = cpd(o->eta_curr, p->phys, 8, 1, pl)) ||
       (rc = cpd(o->delta_prev, p->phys, 8, 2, pl)) || (rc = cpd(o->delta_curr, p->phys, 8, 3, pl)) ||
       (rc = cpd(o->Phi_prev, p->phys, 8, 4, pl)) || (rc = cpd(o->Phi_curr, p->phys, 8, 5, pl)) ||
@@ -1893,6 +2009,7 @@ int swx_plan_phase_curve(swx_plan* p, const double* phases, int K, int normalize
   if (rc) return rc;
   if (!p->initialized) return fail(SWX_ERR_STATE, "phase curve before swx_init");
   const int J = p->geo.J, I = p->geo.I;
+  if ((rc = rebuild_phys(p, (hipStream_t)stream))) return rc;
   // Physical plane 5 of [B][8][J][I] is Phi_curr (the grid swx_export_state returns).
   return swx_phase_curve(p->phys + 5LL * J * I, 8LL * J * I, p->d.n_members, J, I, p->geo_d + p->gl.mus, phases,
                          K, normalize, flux, flux_raw, stream);

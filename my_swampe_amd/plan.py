@@ -278,6 +278,13 @@ class Plan:
         _capi.check(self._lib.swx_plan_device_bytes(self._h, ctypes.byref(n)), "swx_plan_device_bytes")
         return int(n.value)
 
+    def step_counts(self) -> Dict[str, int]:
+        """Steps run since creation: replayed from a graph, launched eagerly, advancing the spectral
+        prev lineage, writing the physical grid carry."""
+        c = (ctypes.c_longlong * 4)()
+        _capi.check(self._lib.swx_plan_step_counts(self._h, c), "swx_plan_step_counts")
+        return dict(graph=c[0], eager=c[1], lineage=c[2], grid_carry=c[3])
+
     def time_kernels(self, t0: int, nsteps: int, stream=None):
         """Average (ms) of the Legendre and grid kernels: nsteps eager steps timed as one block
         (HIP events), then nsteps back-to-back grid launches on the final state (state unchanged);

@@ -303,33 +303,103 @@ def test_mfma_grid_kernel_matches_plain_fp32(M, torch_gpu):
             assert rel < 5e-6, (M, kind, f, rel)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_physical_carry_window_matches_full_carry(precision, torch_gpu):
-    """Writing the physical carry only in the final Robert-Asselin window exports the same state
-    as writing it every step (graph driver, 60 steps)."""
-    from my_swampe_amd import _capi
+_CARRY_FIELDS = ("eta_prev", "delta_prev", "Phi_prev", "eta_curr", "Phi_curr", "U_curr", "V_curr")
+
+
+def _carry_run(precision, options, nsteps=60, scheme="modeuler", members=3):
+    """Export of a plan initialized from the rest + noise grids and stepped `nsteps` times by the graph
+    driver, with plan options `options` ({option: value})."""
     from my_swampe_amd.model import param_row
     from my_swampe_amd.plan import Plan
 
-    torch = torch_gpu
-    p = param_row(dt=120.0, a=8.2e7, omega=3.2e-5, g=9.8, Phibar=3e5, taurad=86400.0, taudrag=86400.0,
-                  DPhieq=3e5, K6=1.24e33, K6Phi=None, alpha=0.01)
+    import torch
+
+    si_raw = scheme == "si_raw"
+    p = param_row(dt=600.0 if si_raw else 120.0, a=8.2e7, omega=3.2e-5, g=9.8, Phibar=3e5, taurad=86400.0,
+                  taudrag=86400.0, DPhieq=3e5, K6=1.24e33, K6Phi=None, alpha=0.05 if si_raw else 0.01,
+                  williams_alpha=0.53)
     ics = orc.rest_noise_ic(42, 0, 3.2e-5)
-    _print_case("phys-window", precision=precision)
-    outs = []
-    for full in (1, 0):
-        plan = Plan(42, np.stack([p, p, p]), precision=precision)
-        plan.set_option(_capi.OPT_FULL_PHYS_CARRY, full)
-        s = torch.cuda.Stream()
-        with torch.cuda.stream(s):
-            plan.init(*ics, stream=s)
-            plan.run_graph(2, 60, graph_steps=8, stream=s)
-            ex = plan.export(fields=("eta_prev", "delta_prev", "Phi_prev", "Phi_curr", "U_curr"), stream=s)
-        s.synchronize()
-        outs.append({k: _np(v) for k, v in ex.items()})
-        plan.close()
-    for k in outs[0]:
-        np.testing.assert_allclose(outs[1][k], outs[0][k], rtol=1e-15, atol=0, err_msg=k)
+    plan = Plan(42, np.stack([p] * members), precision=precision, semi_implicit=si_raw, raw_filter=si_raw)
+    for opt, val in options.items():
+        plan.set_option(opt, val)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        plan.init(*ics, stream=s)
+        plan.run_graph(2, nsteps, graph_steps=8, stream=s)
+        ex = plan.export(fields=_CARRY_FIELDS, stream=s)
+    s.synchronize()
+    out = {k: _np(ex[k]) for k in _CARRY_FIELDS}
+    plan.close()
+    return out
+
+
+@pytest.mark.parametrize("precision,scheme", [("fp64", "modeuler"), ("fp32", "modeuler"), ("fp64", "si_raw"),
+                                              ("fp32", "si_raw")])
+def test_physical_carry_window_matches_full_carry(precision, scheme, torch_gpu):
+    """The grid carry written only in the final Robert-Asselin window exports the same state as the
+    grid carry written every step (graph driver, 60 steps): classic RA on modified Euler, and the
+    semi-implicit leapfrog with the RAW filter (alpha = 0.05, williams_alpha = 0.53), whose window
+    uses the contraction alpha w / (1 - alpha (1 - w))."""
+    from my_swampe_amd import _capi
+
+    _print_case("phys-window", precision=precision, scheme=scheme)
+    full = _carry_run(precision, {_capi.OPT_FULL_PHYS_CARRY: 1}, scheme=scheme)
+    win = _carry_run(precision, {_capi.OPT_PHYS_LINEAGE: -1}, scheme=scheme)
+    for k in full:
+        np.testing.assert_allclose(win[k], full[k], rtol=1e-15, atol=0, err_msg=k)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_lineage_window_matches_longer_lineage(precision, torch_gpu):
+    """The spectral prev lineage advanced in the default final window (16 steps at alpha = 0.01)
+    exports the same state as one advanced over the last 40 of the 60 steps."""
+    from my_swampe_amd import _capi
+
+    _print_case("lineage-window", precision=precision)
+    dflt = _carry_run(precision, {})
+    longer = _carry_run(precision, {_capi.OPT_PHYS_LINEAGE: 40})
+    for k in dflt:
+        np.testing.assert_allclose(dflt[k], longer[k], rtol=1e-15, atol=0, err_msg=k)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_lineage_carry_matches_grid_carry(precision, torch_gpu):
+    """The fields rebuilt from the spectral lineage and the current Fourier rows equal the grid carry
+    written every step to the transforms' rounding: FP64 within 1e-13 of each field's largest
+    magnitude, FP32 within 1e-5 (the grid carry of an FP32 plan sums FP32-rounded grids)."""
+    from my_swampe_amd import _capi
+
+    _print_case("lineage-vs-grid", precision=precision)
+    full = _carry_run(precision, {_capi.OPT_FULL_PHYS_CARRY: 1})
+    lin = _carry_run(precision, {})
+    tol = 1e-13 if precision == "fp64" else 1e-5
+    for k in full:
+        scale = float(np.max(np.abs(full[k])))
+        err = float(np.max(np.abs(lin[k] - full[k])))
+        print(k, "max|diff| / max|field|", err / scale)
+        assert err <= tol * scale, (k, err / scale)
+
+
+def test_lineage_export_vs_oracle(torch_gpu):
+    """FP64, forced modified Euler, 20 steps without diagnostics or history (16 of them advance the
+    spectral lineage; the grid fields are rebuilt at export): every exported grid field, prev levels
+    included, against the oracle within the parity contract atol or 1e-12 of the field's largest
+    magnitude, whichever is larger."""
+    from my_swampe_amd.model import run_model_scan
+
+    _print_case("lineage-oracle")
+    eta0, delta0, Phi0, U0, V0 = orc.rest_noise_ic(42, 0, 3.2e-5)
+    kw = dict(M=42, dt=120.0, tmax=22, Phibar=3e5, omega=3.2e-5, a=8.2e7, taurad=86400.0, taudrag=86400.0,
+              DPhieq=3e5, eta0_init=eta0, delta0_init=delta0, Phi0_init=Phi0, U0_init=U0, V0_init=V0,
+              diagnostics=False, return_history=False)
+    ref = orc.run_model_scan(**kw)["last_state"]
+    got = run_model_scan(**kw)["last_state"]
+    for name in ("eta_prev", "eta_curr", "delta_prev", "delta_curr", "Phi_prev", "Phi_curr", "U_curr", "V_curr"):
+        r = getattr(ref, name)
+        g = _np(getattr(got, name))
+        tol = max(ATOL[name.split("_")[0]], _scale_atol(r, 1e-12))
+        print(name, "max|diff|", float(np.max(np.abs(g - r))), "atol", tol)
+        np.testing.assert_allclose(g, r, rtol=0, atol=tol, err_msg=name)
 
 
 def _noise_spectral(M, B, seed, omega):
