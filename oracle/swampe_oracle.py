@@ -32,9 +32,13 @@ of its own formula):
 * phase_curve / phase_curve_vjp: the fixtures' phase curves, the closed form of a
   uniform grid at grid-aligned phases (cot(pi/I) sum_j sqrt(1 - mu_j^2)), the
   longitude-roll shift property, and central differences of phase_curve for the VJP.
-* explicit scheme (expflag=True): only the delta carry-only quirk
-  (test_parity_quirks.py:85-135) pins it; the rest is "parity unpinned"
-  beyond this restatement (no reference fixture exists for it).
+* explicit scheme (expflag=True; no reference fixture exists for it): the delta
+  carry-only quirk (test_parity_quirks.py:85-135) and identities of the
+  mathematics with delta_mn, zeta_mn from diagnostic_eta_delta: the flux terms of
+  a uniform scalar (Phi from C, D; eta from A, B) equal -2 dt q0 delta_mn(U, V),
+  the continuity term is -2 dt Phibar delta_mn, and drag-only forcing (quirk 5)
+  moves delta by -(4 dt / taudrag) delta_mn(U, V) and eta by 0 (zonal winds) or
+  -(4 dt / taudrag) zeta_mn(0, V) (meridional winds).
 """
 
 from __future__ import annotations

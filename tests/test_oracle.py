@@ -357,3 +357,103 @@ def test_semi_implicit_and_raw_reject_the_explicit_scheme():
         orc.run_model_scan(**_RAW_COMMON, semi_implicit=True, expflag=True)
     with pytest.raises(ValueError):
         orc.run_model_scan(**_RAW_COMMON, raw_filter=True, expflag=True)
+
+
+# ----------------------------------------------------------------------------
+# Explicit scheme (explicit_tdiff.py:20-233) pinned by identities the mathematics fixes.
+# delta_mn(U, V) and zeta_mn(U, V) come from diagnostic_eta_delta, itself pinned by the round trips
+# of unit_tests/test_transform_stack.py:100-180 above; each identity fails on a dropped term, a wrong
+# sign or a wrong coefficient of the flux, continuity or drag terms.
+# ----------------------------------------------------------------------------
+
+
+def _explicit_case(seed=3, M=42, nmax=12):
+    """T42 static (dt = 120 s, taudrag = 5e4 s) and band-limited Fourier winds U_m, V_m from random
+    spectral divergence and vorticity (n <= nmax, real at m = 0)."""
+    s = orc.build_static(M=M, dt=120.0, a=8.2e7, omega=3.2e-5, g=9.8, Phibar=3e5, taurad=86400.0,
+                         taudrag=5.0e4, DPhieq=3e5, K6=1.24e33, K6Phi=None, test=None)
+    rng = np.random.default_rng(seed)
+
+    def spec():
+        c = np.zeros((M + 1, M + 1), complex)
+        for m in range(nmax + 1):
+            for n in range(max(m, 1), nmax + 1):
+                c[m, n] = rng.standard_normal() + (1j * rng.standard_normal() if m else 0.0)
+        return 1e-6 * c
+
+    d_mn, z_mn = spec(), spec()
+    _, _, Um, Vm = orc.invrs_uv(d_mn, z_mn + s.fmn, s.fmn, s.I, s.J, M, M, s.Pmn, s.Hmn, s.tstepcoeffmn, s.marray)
+    return s, Um, Vm
+
+
+def _div_vort(s, Um, Vm):
+    _, _, eta_mn, delta_mn = orc.diagnostic_eta_delta(Um, Vm, s.fmn, s.I, s.J, s.M, s.N, s.Pmn, s.Hmn, s.w,
+                                                      s.tstepcoeff, s.mJarray, s.dt)
+    return delta_mn, eta_mn - s.fmn
+
+
+def _explicit_state(s, **fields):
+    z = np.zeros((s.J, s.M + 1), complex)
+    g = np.zeros((s.J, s.I))
+    base = {k: z for k in ("etam_prev", "etam_curr", "deltam_prev", "deltam_curr", "Phim_prev", "Phim_curr",
+                           "Um_curr", "Vm_curr", "Am_curr", "Bm_curr", "Cm_curr", "Dm_curr", "Em_curr",
+                           "PhiFm_curr", "Fm_curr", "Gm_curr", "Rum_lag", "Rvm_lag")}
+    base.update({k: g for k in ("eta_prev", "eta_curr", "delta_prev", "delta_curr", "Phi_prev", "Phi_curr",
+                                "U_curr", "V_curr")})
+    base.update(fields)
+    return orc.OracleState(**base)
+
+
+def test_explicit_flux_of_uniform_scalar_is_divergence():
+    """Uniform q0 carried by band-limited winds: the flux terms of Phi (C = U q0, D = V q0) and of eta
+    (A = U q0, B = V q0) each equal -2 dt q0 delta_mn(U, V) (explicit_tdiff.py:63-70, :206-212)."""
+    s, Um, Vm = _explicit_case()
+    delta_mn, _ = _div_vort(s, Um, Vm)
+    q0 = 3.7e4
+    st = _explicit_state(s, Cm_curr=q0 * Um, Dm_curr=q0 * Vm)
+    _, _, phi = orc._explicit(st, s, forcflag=False, diffflag=False)
+    scale = np.max(np.abs(2 * s.dt * q0 * delta_mn))
+    np.testing.assert_allclose(phi, -2.0 * s.dt * q0 * delta_mn, rtol=0, atol=1e-12 * scale)
+    st = _explicit_state(s, Am_curr=q0 * Um, Bm_curr=q0 * Vm)
+    eta, _, _ = orc._explicit(st, s, forcflag=False, diffflag=False)
+    np.testing.assert_allclose(eta, -2.0 * s.dt * q0 * delta_mn, rtol=0, atol=1e-12 * scale)
+
+
+def test_explicit_continuity_term():
+    """Only the divergence carried: Phi_new - Pw(Phi_prev) = -2 dt Phibar delta_mn (explicit_tdiff.py:71)."""
+    s, Um, Vm = _explicit_case(seed=5)
+    delta_mn, _ = _div_vort(s, Um, Vm)
+    deltam = orc.invrs_leg(delta_mn, s.I, s.J, s.M, s.N, s.Pmn)[:, : s.M + 1]
+    rng = np.random.default_rng(9)
+    phim_prev = orc.invrs_leg(1e3 * (rng.standard_normal((s.M + 1, s.N + 1)) * np.triu(np.ones((s.M + 1, s.N + 1)))),
+                              s.I, s.J, s.M, s.N, s.Pmn)[:, : s.M + 1].real.astype(complex)
+    st = _explicit_state(s, deltam_curr=deltam, Phim_prev=phim_prev)
+    _, _, phi = orc._explicit(st, s, forcflag=False, diffflag=False)
+    lhs = phi - orc.fwd_leg(phim_prev, s.Pmnw)
+    rhs = -2.0 * s.dt * s.Phibar * delta_mn
+    np.testing.assert_allclose(lhs, rhs, rtol=0, atol=1e-12 * np.max(np.abs(rhs)))
+
+
+def test_explicit_drag_only_forcing():
+    """Quirk 5 (DEVELOPMENT.md:124-128; explicit_tdiff.py:145-152, :216-224): with F = -U/taudrag,
+    G = -V/taudrag and no other forcing, the forced delta increment is -(4 dt / taudrag) delta_mn(U, V);
+    the forced eta increment is 0 for zonal winds only and -(4 dt / taudrag) zeta_mn(0, V) for
+    meridional winds only (the extra U/taudrag, V/taudrag terms double the drag in delta and cancel
+    the U part of it in eta)."""
+    s, Um, Vm = _explicit_case(seed=11)
+    tau = s.taudrag
+    z = np.zeros_like(Um)
+    for U, V in ((Um, Vm), (Um, z), (z, Vm)):
+        st = _explicit_state(s, Um_curr=U, Vm_curr=V, Fm_curr=-U / tau, Gm_curr=-V / tau)
+        eta_f, delta_f, _ = orc._explicit(st, s, forcflag=True, diffflag=False)
+        eta_u, delta_u, _ = orc._explicit(st, s, forcflag=False, diffflag=False)
+        delta_mn, zeta_mn = _div_vort(s, U, V)
+        d_inc = delta_f - delta_u
+        e_inc = eta_f - eta_u
+        k = 4.0 * s.dt / tau
+        scale = k * max(np.max(np.abs(delta_mn)), np.max(np.abs(zeta_mn)))
+        np.testing.assert_allclose(d_inc, -k * delta_mn, rtol=0, atol=1e-12 * scale)
+        if V is z:
+            np.testing.assert_allclose(e_inc, 0.0, rtol=0, atol=1e-12 * scale)
+        if U is z:
+            np.testing.assert_allclose(e_inc, -k * zeta_mn, rtol=0, atol=1e-12 * scale)
