@@ -164,8 +164,20 @@ int swx_set_prev_level(swx_plan *plan, const double *eta_prev, const double *del
 int swx_run(swx_plan *plan, int t0, int nsteps, double *hist_fields, double *hist_scalars,
             double *hist_phiext, unsigned char *hist_dead, int hist_T, int hist_k0, void *stream);
 
-/* Copy the scan carry out to caller-provided device buffers. */
+/* Copy the scan carry out to caller-provided device buffers (the State the reference's
+ * run_model_scan returns as last_state, model.py:1735-1742).  When the last call carried the
+ * physical prev level as the spectral lineage (SWX_OPT_PHYS_LINEAGE), the grid fields are rebuilt
+ * first, into rows the plan allocates on its first rebuild. */
 int swx_export_state(swx_plan *plan, const swx_state_out *out, void *stream);
+
+/* swx_export_state that allocates nothing: the rebuild's rows live in a file-scope device array
+ * of the library (64 MiB, one per GPU, held until the process ends), so the call makes no hipMalloc
+ * or other allocation.  No counterpart in the reference; for callers that cannot allocate (code
+ * under HIP graph capture, serving loops).  Plans whose rows exceed the array (B * 5 * J * (M+1) *
+ * 2 * transform-precision bytes > 64 MiB: more than 609 FP32 / 304 FP64 members at T42, 273 / 136
+ * at T63, 97 / 48 at T106) get SWX_ERR_ARG.  Calls that overlap on one GPU (concurrent streams or
+ * threads) race on the array. */
+int swx_export_state_noalloc(swx_plan *plan, const swx_state_out *out, void *stream);
 
 /* Forward-only benchmark driver: run nsteps steps with no history and no
  * export, replaying a captured HIP graph of `graph_steps` steps where

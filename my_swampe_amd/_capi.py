@@ -41,7 +41,7 @@ EXPORTED_SYMBOLS = (
     "swx_init_spectral", "swx_state_get", "swx_state_set", "swx_adjoint_begin", "swx_adjoint_terminal", "swx_adjoint_observe",
     "swx_phase_curve", "swx_phase_curve_vjp", "swx_plan_phase_curve", "swx_set_prev_level",
     "swx_adjoint_step", "swx_adjoint_result", "swx_last_error", "swx_version", "swx_plan_device_bytes",
-    "swx_plan_step_counts", "swx_ensemble_create", "swx_ensemble_run_graph", "swx_ensemble_destroy", "swx_plan_get_option",
+    "swx_plan_step_counts", "swx_export_state_noalloc", "swx_ensemble_create", "swx_ensemble_run_graph", "swx_ensemble_destroy", "swx_plan_get_option",
     "swx_pipeline_create", "swx_pipeline_run_graph", "swx_pipeline_destroy", "swx_run_tape", "swx_adjoint_sweep",
 )
 
@@ -121,6 +121,7 @@ def load() -> ctypes.CDLL:
     lib.swx_time_kernels.argtypes = [c_void_p, c_int, c_int, POINTER(ctypes.c_float), POINTER(ctypes.c_float), c_void_p]
     lib.swx_plan_device_bytes.argtypes = [c_void_p, POINTER(ctypes.c_size_t)]
     lib.swx_plan_step_counts.argtypes = [c_void_p, POINTER(ctypes.c_longlong)]
+    lib.swx_export_state_noalloc.argtypes = lib.swx_export_state.argtypes
     lib.swx_plan_get_option.argtypes = [c_void_p, c_int, POINTER(c_int)]
     lib.swx_ensemble_create.argtypes = [c_void_p, c_void_p, POINTER(c_void_p)]
     lib.swx_ensemble_run_graph.argtypes = [c_void_p, c_int, c_int, c_int, c_void_p]
@@ -136,7 +137,7 @@ def load() -> ctypes.CDLL:
                  "swx_run", "swx_export_state", "swx_run_graph", "swx_time_kernels", "swx_plan_set_option",
                  "swx_init_spectral", "swx_state_get", "swx_state_set", "swx_adjoint_begin",
                  "swx_adjoint_terminal", "swx_adjoint_step", "swx_adjoint_result", "swx_plan_device_bytes",
-                 "swx_plan_step_counts",
+                 "swx_plan_step_counts", "swx_export_state_noalloc",
                  "swx_ensemble_create", "swx_ensemble_run_graph", "swx_plan_get_option", "swx_adjoint_observe",
                  "swx_phase_curve", "swx_phase_curve_vjp", "swx_plan_phase_curve", "swx_set_prev_level",
                  "swx_pipeline_create", "swx_pipeline_run_graph", "swx_run_tape", "swx_adjoint_sweep"):

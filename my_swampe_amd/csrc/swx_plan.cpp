@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -475,11 +476,16 @@ size_t lineage_rows_bytes(const swx_plan* p) {
   return (size_t)p->d.n_members * kNG * p->geo.J * (p->geo.M + 1) * 2 * p->rsz;
 }
 
-int rebuild_phys(swx_plan* p, hipStream_t s) {
+// rows: device buffer of lineage_rows_bytes(p) for the lineage rows, or null for the plan's own
+// (allocated on the first rebuild).
+int rebuild_phys(swx_plan* p, hipStream_t s, void* rows = nullptr) {
   if (!p->phys_stale) return SWX_OK;
   int rc;
-  if (!p->Gsyn && (rc = dalloc(p, &p->Gsyn, lineage_rows_bytes(p)))) return rc;
-  return p->rsz == 8 ? rebuild_phys_t<double>(p, p->Gsyn, s) : rebuild_phys_t<float>(p, p->Gsyn, s);
+  if (!rows) {
+    if (!p->Gsyn && (rc = dalloc(p, &p->Gsyn, lineage_rows_bytes(p)))) return rc;
+    rows = p->Gsyn;
+  }
+  return p->rsz == 8 ? rebuild_phys_t<double>(p, rows, s) : rebuild_phys_t<float>(p, rows, s);
 }
 
 template <typename R>
@@ -1865,12 +1871,13 @@ int swx_adjoint_result(swx_plan* p, double* lam_out, double* pgrad_out, void* st
   return SWX_OK;
 }
 
-int swx_export_state(swx_plan* p, const swx_state_out* o, void* stream) {
-  int rc = check_plan(p);
-  if (rc) return rc;
-  if (!o) return fail(SWX_ERR_ARG, "null output struct");
-  if (!p->initialized) return fail(SWX_ERR_STATE, "export before swx_init");
-  hipStream_t s = (hipStream_t)stream;
+}  // extern "C"
+
+namespace {
+
+// swx_export_state with the rows of a physical-carry rebuild in `rows` (null: the plan's own).
+int export_state(swx_plan* p, const swx_state_out* o, hipStream_t s, void* rows) {
+  int rc;
   const int B = p->d.n_members, J = p->geo.J, I = p->geo.I, Mp = p->geo.M + 1, Np = p->geo.N + 1;
   const long long cjm = (long long)J * Mp * 2;  // doubles per complex (J, M+1) field
   const long long pl = (long long)J * I;
@@ -1954,7 +1961,7 @@ int swx_export_state(swx_plan* p, const swx_state_out* o, void* stream) {
     return rc;
   if ((o->eta_prev || o->eta_curr || o->delta_prev || o->delta_curr || o->Phi_prev || o->Phi_curr || o->U_curr ||
        o->V_curr) &&
-      (rc = rebuild_phys(p, s)))
+      (rc = rebuild_phys(p, s, rows)))
     return rc;
   if ((rc = cpd(o->eta_prev, p->phys, 8, 0, pl)) || (rc = cpd(o->eta_curr, p->phys, 8, 1, pl)) ||
       (rc = cpd(o->delta_prev, p->phys, 8, 2, pl)) || (rc = cpd(o->delta_curr, p->phys, 8, 3, pl)) ||
@@ -1968,6 +1975,60 @@ int swx_export_state(swx_plan* p, const swx_state_out* o, void* stream) {
   }
   if (o->dead) SWX_HIP(hipMemcpyAsync(o->dead, p->dead, (size_t)B, hipMemcpyDeviceToDevice, s));
   return SWX_OK;
+}
+
+// Scratch of swx_export_state_noalloc: the Fourier rows of the spectral prev lineage that a rebuild
+// of the physical carry writes ([B][5][M+1][J] complex in the transform precision), at most
+// kScratchBytes.  One copy per GPU, resident from the library's load until the process ends; every
+// call on a GPU shares it and it keeps the last call's contents (a rebuild overwrites every row it
+// reads, so nothing is cleared).  Host code never names it: it uses the per-device address from
+// hipGetSymbolAddress (scratch_address).
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr int kMaxScratchDevices = 64;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+std::mutex g_scratch_mu;
+
+// Device address of `scratch` on device `dev` (the current device), looked up once per device.
+int scratch_address(int dev, void** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (dev < 0 || dev >= kMaxScratchDevices) return fail(SWX_ERR_ARG, "device index out of range for the scratch");
+  if (!g_scratch_ptr[dev]) {
+    void* q = nullptr;
+    hipError_t e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !q)
+      return fail(SWX_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = q;
+  }
+  *out = g_scratch_ptr[dev];
+  return SWX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int swx_export_state(swx_plan* p, const swx_state_out* o, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!o) return fail(SWX_ERR_ARG, "null output struct");
+  if (!p->initialized) return fail(SWX_ERR_STATE, "export before swx_init");
+  return export_state(p, o, (hipStream_t)stream, nullptr);
+}
+
+int swx_export_state_noalloc(swx_plan* p, const swx_state_out* o, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!o) return fail(SWX_ERR_ARG, "null output struct");
+  if (!p->initialized) return fail(SWX_ERR_STATE, "export before swx_init");
+  const size_t need = lineage_rows_bytes(p);
+  if (need > kScratchBytes)
+    return fail(SWX_ERR_ARG, "swx_export_state_noalloc: this plan's rebuild rows need " + std::to_string(need) +
+                                 " bytes, above the " + std::to_string(kScratchBytes) +
+                                 "-byte scratch (use swx_export_state, or fewer members per plan)");
+  void* rows = nullptr;
+  if ((rc = scratch_address(p->device, &rows))) return rc;
+  return export_state(p, o, (hipStream_t)stream, rows);
 }
 
 static int phase_curve_dims(int n, int J, int I, int K) {

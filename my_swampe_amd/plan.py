@@ -294,8 +294,10 @@ class Plan:
                                                self._stream(stream)), "swx_time_kernels")
         return a.value, b.value
 
-    def export(self, fields=None, spec: bool = False, stream=None) -> Dict[str, object]:
-        """Copy the carry out as complex128/float64 device tensors with leading member axis."""
+    def export(self, fields=None, spec: bool = False, stream=None, noalloc: bool = False) -> Dict[str, object]:
+        """Copy the carry out as complex128/float64 device tensors with leading member axis.
+        noalloc: through swx_export_state_noalloc (the library allocates nothing; the output
+        tensors are still allocated here)."""
         torch = _torch()
         names = _capi.STATE_FIELDS if fields is None else tuple(fields)
         out = {}
@@ -315,7 +317,12 @@ class Plan:
             sp = torch.empty((self.B, 3, Mp, Mp), dtype=torch.complex128, device=self.device)
             st.spec = sp.data_ptr()
             out["spec"] = sp
-        _capi.check(self._lib.swx_export_state(self._h, ctypes.byref(st), self._stream(stream)), "swx_export_state")
+        if noalloc:
+            _capi.check(self._lib.swx_export_state_noalloc(self._h, ctypes.byref(st), self._stream(stream)),
+                        "swx_export_state_noalloc")
+        else:
+            _capi.check(self._lib.swx_export_state(self._h, ctypes.byref(st), self._stream(stream)),
+                        "swx_export_state")
         return out
 
 
