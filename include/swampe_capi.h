@@ -353,6 +353,11 @@ int swx_plan_phase_curve(swx_plan *plan, const double *phases, int K, int normal
  *     window is exact; shorter ones are for tests).  The two carries give the same fields to the
  *     rounding of the transforms (FP64 plans ~1e-15 relative, FP32 ~1e-7). */
 #define SWX_OPT_PHYS_LINEAGE 8
+/*   SWX_OPT_LEG_FOLD: which FP32 matrix-core Legendre contractions run: 2 (default) = folded by
+ *     the equatorial symmetry of the Gauss nodes and the basis (sums over the southern half of
+ *     the latitudes on T(j) +- T(J-1-j); half the matrix-core work), 1 = over all latitudes.
+ *     Results agree to FP32 rounding. */
+#define SWX_OPT_LEG_FOLD 9
 int swx_plan_set_option(swx_plan *plan, int option, int value);
 /* Current value of an option (for SWX_OPT_LEG_MB and SWX_OPT_GRID_FOLD: the
  * choice in use). */

@@ -28,6 +28,7 @@ OPT_LEG_MB = 5
 OPT_GRID_FOLD = 6
 OPT_CONCURRENT = 7
 OPT_PHYS_LINEAGE = 8
+OPT_LEG_FOLD = 9
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
 
 PARAM_NAMES = ("dt", "a", "omega", "g", "Phibar", "taurad", "taudrag", "DPhieq", "K6", "K6Phi", "alpha", "blowup_rms",

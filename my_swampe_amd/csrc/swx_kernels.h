@@ -90,6 +90,39 @@ constexpr int kNRaw = 8;
 // Degree rows padded to whole 16-row MFMA tiles.
 __host__ __device__ constexpr int leg_knp(int N) { return (N + 1 + 15) / 16 * 16; }
 
+// Equatorially folded Legendre contractions (FP32 matrix-core kernel).  The Gauss nodes pair
+// as mu_{J-1-j} = -mu_j and the basis has parity P(-mu) = (-1)^{n+m} P(mu),
+// H(-mu) = -(-1)^{n+m} H(mu), so the forward sums run over j < J/2 on s = T(j) + T(J-1-j) or
+// a = T(j) - T(J-1-j), and the inverse forms the even-(n-m) and odd-(n-m) partial sums E, O on
+// j < J/2 with X(j) = E + O, X(J-1-j) = E - O (P; H: O - E).  Per m, with K = N + 1 - m degrees:
+// Ke = (K + 1) / 2 even and Ko = K / 2 odd ones.
+//   forward: row tiles of 16 degrees, the ceil(Ke/16) even tiles first, then the odd ones;
+//   inverse: the rows of the inputs are the Ke even degrees then the Ko odd ones; E takes
+//   k-steps [0, leg_fold_ne(K)), O the k-steps [leg_fold_kb(K), leg_fold_kb(K) + leg_fold_no(K))
+//   (each table zero on the other parity's rows).
+__host__ __device__ constexpr int leg_fold_nte(int K) { return ((K + 1) / 2 + 15) / 16; }
+__host__ __device__ constexpr int leg_fold_nto(int K) { return (K / 2 + 15) / 16; }
+__host__ __device__ constexpr int leg_fold_ne(int K) { return ((K + 1) / 2 + 3) / 4; }
+__host__ __device__ constexpr int leg_fold_kb(int K) { return ((K + 1) / 2) / 4; }
+__host__ __device__ constexpr int leg_fold_no(int K) { return (K + 3) / 4 - leg_fold_kb(K); }
+// Table extents over all m (K = 1 .. N + 1): forward row tiles per m, inverse E and O k-step
+// slots per lane (each padded to a multiple of 4: float4 loads).
+__host__ __device__ constexpr int leg_fold_ntf(int N) {
+  int r = 0;
+  for (int K = 1; K <= N + 1; ++K) r = leg_fold_nte(K) + leg_fold_nto(K) > r ? leg_fold_nte(K) + leg_fold_nto(K) : r;
+  return r;
+}
+__host__ __device__ constexpr int leg_fold_kse(int N) {
+  int r = 0;
+  for (int K = 1; K <= N + 1; ++K) r = leg_fold_ne(K) > r ? leg_fold_ne(K) : r;
+  return (r + 3) / 4 * 4;
+}
+__host__ __device__ constexpr int leg_fold_kso(int N) {
+  int r = 0;
+  for (int K = 1; K <= N + 1; ++K) r = leg_fold_no(K) > r ? leg_fold_no(K) : r;
+  return (r + 3) / 4 * 4;
+}
+
 // Work-item order of a Legendre launch with grid (M+1, groups).  heavy_first = 1: workgroup
 // L = x + y (M+1) takes zonal wavenumber m = L / groups and group L % groups; dispatch runs in
 // L order, so the heavy items (small m: K = N + 1 - m degrees) start first and the light ones
@@ -139,6 +172,17 @@ struct LegArgs {
   const float* fwdH;
   const float* invP;
   const float* invH;
+  // Equatorially folded tables (leg_fold_* extents; J2 = J / 2):
+  //   ffwd[m][r][lane][ks] = basis(j = 4 ks + lane/16, n = m + np), ks < J2 / 4, with row
+  //     tile r < nte: np = 2 (16 r + lane%16), else np = 2 (16 (r - nte) + lane%16) + 1;
+  //   finv[m][jt][lane][slot] = basis(j = 16 jt + lane%16, n = m + np(row)), jt < J2 / 16, E
+  //     slots e < kse: row 4 e + lane/16 (np = 2 row while row < Ke), O slots kse + o: row
+  //     4 (kb + o) + lane/16 (np = 2 (row - Ke) + 1 while Ke <= row < K); zero elsewhere.
+  const float* ffwdP;
+  const float* ffwdH;
+  const float* finvP;
+  const float* finvH;
+  int leg_fold;  // 1: the matrix-core kernel uses the folded tables
   const double* coef;
   int ncoef;
   // [M+1][N+1]: Pw(invrs_leg(X)) = f2 * X for the basis in use (1 for an

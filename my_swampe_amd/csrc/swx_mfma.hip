@@ -685,13 +685,30 @@ __device__ __forceinline__ void leg_fwd_frags(const LegArgs<float>& a, int m, in
   for (int q4 = 0; q4 < J / 16; ++q4) av[q4] = ar[q4];
 }
 
+// Folded tables: the fragments of forward tile t (row tile t / CT of the per-m list, P or H by
+// column tile t % CT), J / 8 k-steps.
+template <int MT, int MB>
+__device__ __forceinline__ void leg_fwd_frags_fold(const LegArgs<float>& a, int m, int t, int ntile, int lane,
+                                                   v4f (&av)[LegShape<MT, MB>::J / 32]) {
+  using S = LegShape<MT, MB>;
+  constexpr int CT = S::RT / 16, J = S::J, NTF = leg_fold_ntf(MT);
+  const int tc = t < ntile ? t : 0;
+  const int r = tc / CT, ct = tc % CT;
+  const v4f* ar = reinterpret_cast<const v4f*>((ct * 16 < S::RP ? a.ffwdP : a.ffwdH) +
+                                               ((long long)(m * NTF + r) * 64 + lane) * (J / 8));
+#pragma unroll
+  for (int q4 = 0; q4 < J / 32; ++q4) av[q4] = ar[q4];
+}
+
 // One workgroup's work: zonal wavenumber m, member group `grp` (members grp*MB ...).
 // GEN = false: the modified-Euler scheme without the RAW filter only, whose spectral update
 // the compiler then specializes (the semi-implicit, RAW and explicit paths of spectral_update
 // hold many more live doubles; dropping them lowers the register count from 152 to 112 at
 // T42, 4 members: 4 waves per SIMD instead of 3).
 // LIN (with GEN = false): the step also advances the spectral prev lineage (StepFlags::lineage).
-template <int MT, int MB, bool GEN = true, bool LIN = false>
+// FOLD: the equatorially folded contractions (swx_kernels.h, leg_fold_*): half the matrix-core
+// work and half the basis fragments; the same FP64 update.
+template <int MT, int MB, bool GEN = true, bool LIN = false, bool FOLD = true>
 __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int m, const int grp, float* lds) {
   using S = LegShape<MT, MB>;
   constexpr int Np = S::Np, Mp = MT + 1, J = S::J, RP = S::RP, RT = S::RT, XP = S::XP, XT = S::XT, JO = S::JO;
@@ -707,6 +724,9 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
   const int Kt = (K + 15) / 16;
   const int K4 = ceil4(K);
   const CoefLayout cl = CoefLayout::make(N, J);
+  // Folded layout of this m: even degrees np = 2 i (i < Ke), odd np = 2 i + 1 (i < Ko).
+  const int Ke = (K + 1) / 2;
+  const int nte = leg_fold_nte(K), nto = leg_fold_nto(K);
 
   // The current spectral level of this thread's phase-U pairs (p = tid + 256 u), loaded now so
   // that the loads complete under the staging and the forward GEMM (up to 2 pairs per thread;
@@ -732,14 +752,58 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
 
   // The first forward tile's basis fragments, in flight during the staging.
   constexpr int CT = RT / 16;
-  constexpr int MAXT = (ceil16(Np) / 16 * CT + 3) / 4;
-  const int ntile = Kt * CT;
-  v4f av0[J / 16];
-  leg_fwd_frags<MT, MB>(a, m, wave, ntile, lane, av0);
+  constexpr int NRT = FOLD ? leg_fold_ntf(MT) : ceil16(Np) / 16;  // row tiles per m, at most
+  constexpr int MAXT = (NRT * CT + 3) / 4;
+  constexpr int KSF = FOLD ? J / 8 : J / 4;                          // forward k-steps
+  const int ntile = (FOLD ? nte + nto : Kt) * CT;
+  v4f av0[KSF / 4];
+  if constexpr (FOLD)
+    leg_fwd_frags_fold<MT, MB>(a, m, wave, ntile, lane, av0);
+  else
+    leg_fwd_frags<MT, MB>(a, m, wave, ntile, lane, av0);
 
   // Phase 0: stage the Legendre inputs [j][member, slot, re/im]; zero sX.  Each member's
   // [slot][j][re/im] block is contiguous in T: float4 loads (two latitudes), all issued first.
-  {
+  // Folded: rows j < J/2 hold s = T(j) + T(J-1-j), rows J/2 + j hold a = T(j) - T(J-1-j), from
+  // the float4 of latitudes (j, j+1) and the one of (J-2-j, J-1-j).
+  if constexpr (FOLD) {
+    constexpr int QPM = kNT * J / 4;  // latitude-pair float4s of the southern half per member
+    constexpr int NQ = (MB * QPM + 255) / 256;
+    v4f tv[NQ], tw[NQ];
+#pragma unroll
+    for (int u = 0; u < NQ; ++u) {
+      const int q = tid + 256 * u;
+      const int mb = q / QPM < MB ? q / QPM : MB - 1;
+      const int b = b0 + mb < B ? b0 + mb : B - 1;  // clamped address, zeroed below
+      const int r = q % QPM, k = r / (J / 4), j = 2 * (r % (J / 4));
+      const float* tb = a.T + (((long long)b * Mp + m) * kNT + k) * J * 2;
+      const v4f v = *reinterpret_cast<const v4f*>(tb + 2 * j);
+      const v4f w = *reinterpret_cast<const v4f*>(tb + 2 * (J - 2 - j));
+      const bool ok = q < MB * QPM && b0 + q / QPM < B;
+      tv[u] = ok ? v : v4f{0.f, 0.f, 0.f, 0.f};
+      tw[u] = ok ? w : v4f{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int u = 0; u < NQ; ++u) {
+      const int q = tid + 256 * u;
+      if (q < MB * QPM) {
+        const int mb = q / QPM, r = q % QPM;
+        const int k = r / (J / 4), j = 2 * (r % (J / 4));
+        const int col = k < kNP ? mb * 10 + 2 * k : RP + mb * 8 + 2 * (k - kNP);
+        float* d0 = sA + j * RS + col;
+        float* d1 = sA + (J / 2 + j) * RS + col;
+        // T(j) = tv.xy, T(j+1) = tv.zw, T(J-2-j) = tw.xy, T(J-1-j) = tw.zw
+        d0[0] = tv[u][0] + tw[u][2];
+        d0[1] = tv[u][1] + tw[u][3];
+        d0[RS] = tv[u][2] + tw[u][0];
+        d0[RS + 1] = tv[u][3] + tw[u][1];
+        d1[0] = tv[u][0] - tw[u][2];
+        d1[1] = tv[u][1] - tw[u][3];
+        d1[RS] = tv[u][2] - tw[u][0];
+        d1[RS + 1] = tv[u][3] - tw[u][1];
+      }
+    }
+  } else {
     constexpr int QPM = kNT * J / 2;  // float4 per member
     constexpr int NQ = (MB * QPM + 255) / 256;
     v4f tv[NQ];
@@ -781,22 +845,34 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
   v4f acc[MAXT];
 #pragma unroll
   for (int q = 0; q < MAXT; ++q) acc[q] = v4f{0.f, 0.f, 0.f, 0.f};
-  v4f avn[J / 16];
+  v4f avn[KSF / 4];
 #pragma unroll
-  for (int q4 = 0; q4 < J / 16; ++q4) avn[q4] = av0[q4];
+  for (int q4 = 0; q4 < KSF / 4; ++q4) avn[q4] = av0[q4];
 #pragma unroll
   for (int q = 0; q < MAXT; ++q) {
     const int t = wave + 4 * q;
-    v4f av[J / 16];
+    v4f av[KSF / 4];
 #pragma unroll
-    for (int q4 = 0; q4 < J / 16; ++q4) av[q4] = avn[q4];
-    if (q + 1 < MAXT) leg_fwd_frags<MT, MB>(a, m, wave + 4 * (q + 1), ntile, lane, avn);
+    for (int q4 = 0; q4 < KSF / 4; ++q4) av[q4] = avn[q4];
+    if (q + 1 < MAXT) {
+      if constexpr (FOLD)
+        leg_fwd_frags_fold<MT, MB>(a, m, wave + 4 * (q + 1), ntile, lane, avn);
+      else
+        leg_fwd_frags<MT, MB>(a, m, wave + 4 * (q + 1), ntile, lane, avn);
+    }
     if (t < ntile) {
       const int ct = t % CT;
+      // Folded: the B operand is s (rows 0 ..) for P tiles of even degrees and H tiles of odd
+      // ones, else a (rows J/2 ..).
+      int base = 0;
+      if constexpr (FOLD) {
+        const bool even = t / CT < nte, isP = ct * 16 < RP;
+        base = (even == isP) ? 0 : J / 2;
+      }
+      const float* br = sA + (base + lg) * RS + ct * 16 + ln;
 #pragma unroll
-      for (int ks = 0; ks < J / 4; ++ks)
-        acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[ks / 4][ks % 4], sA[(4 * ks + lg) * RS + ct * 16 + ln],
-                                                      acc[q], 0, 0, 0);
+      for (int ks = 0; ks < KSF; ++ks)
+        acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[ks / 4][ks % 4], br[4 * ks * RS], acc[q], 0, 0, 0);
     }
   }
   __syncthreads();
@@ -830,11 +906,14 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
     if (b >= B) continue;
     const int n = m + np;
     const double* cf = a.coef + (long long)b * a.ncoef;
+    // Row of degree np in the forward results and in the inverse inputs.
+    const int frow = FOLD ? ((np & 1) ? 16 * nte + (np >> 1) : (np >> 1)) : np;
+    const int xrow = FOLD ? ((np & 1) ? Ke + (np >> 1) : (np >> 1)) : np;
     double fp[5][2], fh[4][2];
     for (int k = 0; k < 5; ++k)
-      for (int c = 0; c < 2; ++c) fp[k][c] = sA[np * RS + mb * 10 + 2 * k + c];
+      for (int c = 0; c < 2; ++c) fp[k][c] = sA[frow * RS + mb * 10 + 2 * k + c];
     for (int k = 0; k < 4; ++k)
-      for (int c = 0; c < 2; ++c) fh[k][c] = sA[np * RS + RP + mb * 8 + 2 * k + c];
+      for (int c = 0; c < 2; ++c) fh[k][c] = sA[frow * RS + RP + mb * 8 + 2 * k + c];
     double* Sb = a.S + (long long)b * 2 * 3 * Mp * Np * 2;
     const long long fstride = (long long)Mp * Np * 2;
     const long long so = ((long long)m * Np + n) * 2;
@@ -842,7 +921,7 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
     spectral_update(cf, cl, a.f2[(long long)m * Np + n], m, n, fl, fp, fh, Sb + 3 * fstride, Sb, so,
                     fstride, eff, nw, kPre ? scp[kPre ? u : 0] : nullptr);
     wind_potentials(cf, cl, m, n, nw[0], nw[1], dh, eh);
-    float* xr = sX + np * XS;
+    float* xr = sX + xrow * XS;
     for (int c = 0; c < 2; ++c) {
       xr[mb * 10 + 0 + c] = (float)eff[0][c];
       xr[mb * 10 + 2 + c] = (float)eff[1][c];
@@ -858,43 +937,89 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
   // Phase I: inverse contraction; wave w owns latitude tiles jt = w, w+4, ... and every
   // column tile, so its P and H fragments are loaded once and reused across them.
   constexpr int CIP = XP / 16, CIH = (XT - XP) / 16;
-  constexpr int KSM = KNP / 4;  // k-steps at m = 0
-  const int ksn = K4 / 4;
-  for (int jt = wave; jt < J / 16; jt += 4) {
-    const int j = jt * 16 + ln;
-    const long long fo = ((long long)(m * (J / 16) + jt) * 64 + lane) * KSM;
-    const v4f* pr = reinterpret_cast<const v4f*>(a.invP + fo);
-    const v4f* hr = reinterpret_cast<const v4f*>(a.invH + fo);
-    v4f pa[KSM / 4], ha[KSM / 4];
+  if constexpr (FOLD) {
+    // Units (jt, parity) over the southern latitude tiles jt < J/32: the even-degree partial
+    // sums E into rows jt*16 .. of the column-major output, the odd ones O into rows
+    // J/2 + jt*16 ..; phase W combines them for both hemispheres.
+    constexpr int KSE = leg_fold_kse(MT), KSO = leg_fold_kso(MT), KSL = KSE + KSO;
+    const int ne = leg_fold_ne(K), kb = leg_fold_kb(K), no = leg_fold_no(K);
+    for (int u = wave; u < J / 16; u += 4) {
+      const int jt = u >> 1, odd = u & 1;
+      const long long fo = ((long long)(m * (J / 32) + jt) * 64 + lane) * KSL + (odd ? KSE : 0);
+      const v4f* pr = reinterpret_cast<const v4f*>(a.finvP + fo);
+      const v4f* hr = reinterpret_cast<const v4f*>(a.finvH + fo);
+      constexpr int KSX = KSE > KSO ? KSE : KSO;
+      const int nks = odd ? no : ne;
+      const int row0 = odd ? 4 * kb : 0;
+      v4f pa[KSX / 4], ha[KSX / 4];
 #pragma unroll
-    for (int q4 = 0; q4 < KSM / 4; ++q4) {
-      const bool need = 4 * q4 < ksn;
-      pa[q4] = need ? pr[q4] : v4f{0.f, 0.f, 0.f, 0.f};
-      ha[q4] = need ? hr[q4] : v4f{0.f, 0.f, 0.f, 0.f};
-    }
-    v4f dP[CIP], dH[CIH];
-#pragma unroll
-    for (int c = 0; c < CIP; ++c) dP[c] = v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int c = 0; c < CIH; ++c) dH[c] = v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int ks = 0; ks < KSM; ++ks) {
-      if (ks < ksn) {
-        const float* xr = sX + (4 * ks + lg) * XS + ln;
-#pragma unroll
-        for (int c = 0; c < CIP; ++c)
-          dP[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(pa[ks / 4][ks % 4], xr[c * 16], dP[c], 0, 0, 0);
-#pragma unroll
-        for (int c = 0; c < CIH; ++c)
-          dH[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(ha[ks / 4][ks % 4], xr[XP + c * 16], dH[c], 0, 0, 0);
+      for (int q4 = 0; q4 < KSX / 4; ++q4) {
+        const bool need = 4 * q4 < nks;
+        pa[q4] = need ? pr[q4] : v4f{0.f, 0.f, 0.f, 0.f};
+        ha[q4] = need ? hr[q4] : v4f{0.f, 0.f, 0.f, 0.f};
       }
+      v4f dP[CIP], dH[CIH];
+#pragma unroll
+      for (int c = 0; c < CIP; ++c) dP[c] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int c = 0; c < CIH; ++c) dH[c] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < KSX; ++ks) {
+        if (ks < nks) {
+          const float* xr = sX + (row0 + 4 * ks + lg) * XS + ln;
+#pragma unroll
+          for (int c = 0; c < CIP; ++c)
+            dP[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(pa[ks / 4][ks % 4], xr[c * 16], dP[c], 0, 0, 0);
+#pragma unroll
+          for (int c = 0; c < CIH; ++c)
+            dH[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(ha[ks / 4][ks % 4], xr[XP + c * 16], dH[c], 0, 0, 0);
+        }
+      }
+      float* out = sA + ln * JO + (odd ? J / 2 : 0) + jt * 16 + 4 * lg;
+#pragma unroll
+      for (int c = 0; c < CIP; ++c) *reinterpret_cast<v4f*>(out + c * 16 * JO) = dP[c];
+#pragma unroll
+      for (int c = 0; c < CIH; ++c) *reinterpret_cast<v4f*>(out + (XP + c * 16) * JO) = dH[c];
     }
-    // Column-major [col][JO] output: the 4 rows a lane holds are contiguous (one b128 store).
-    float* out = sA + ln * JO + jt * 16 + 4 * lg;
+  } else {
+    constexpr int KSM = KNP / 4;  // k-steps at m = 0
+    const int ksn = K4 / 4;
+    for (int jt = wave; jt < J / 16; jt += 4) {
+      const int j = jt * 16 + ln;
+      const long long fo = ((long long)(m * (J / 16) + jt) * 64 + lane) * KSM;
+      const v4f* pr = reinterpret_cast<const v4f*>(a.invP + fo);
+      const v4f* hr = reinterpret_cast<const v4f*>(a.invH + fo);
+      v4f pa[KSM / 4], ha[KSM / 4];
 #pragma unroll
-    for (int c = 0; c < CIP; ++c) *reinterpret_cast<v4f*>(out + c * 16 * JO) = dP[c];
+      for (int q4 = 0; q4 < KSM / 4; ++q4) {
+        const bool need = 4 * q4 < ksn;
+        pa[q4] = need ? pr[q4] : v4f{0.f, 0.f, 0.f, 0.f};
+        ha[q4] = need ? hr[q4] : v4f{0.f, 0.f, 0.f, 0.f};
+      }
+      v4f dP[CIP], dH[CIH];
 #pragma unroll
-    for (int c = 0; c < CIH; ++c) *reinterpret_cast<v4f*>(out + (XP + c * 16) * JO) = dH[c];
+      for (int c = 0; c < CIP; ++c) dP[c] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int c = 0; c < CIH; ++c) dH[c] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < KSM; ++ks) {
+        if (ks < ksn) {
+          const float* xr = sX + (4 * ks + lg) * XS + ln;
+#pragma unroll
+          for (int c = 0; c < CIP; ++c)
+            dP[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(pa[ks / 4][ks % 4], xr[c * 16], dP[c], 0, 0, 0);
+#pragma unroll
+          for (int c = 0; c < CIH; ++c)
+            dH[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(ha[ks / 4][ks % 4], xr[XP + c * 16], dH[c], 0, 0, 0);
+        }
+      }
+      // Column-major [col][JO] output: the 4 rows a lane holds are contiguous (one b128 store).
+      float* out = sA + ln * JO + jt * 16 + 4 * lg;
+#pragma unroll
+      for (int c = 0; c < CIP; ++c) *reinterpret_cast<v4f*>(out + c * 16 * JO) = dP[c];
+#pragma unroll
+      for (int c = 0; c < CIH; ++c) *reinterpret_cast<v4f*>(out + (XP + c * 16) * JO) = dH[c];
+    }
   }
   __syncthreads();
 
@@ -904,52 +1029,69 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
     const int mb = p / J, j = p % J;
     const int b = b0 + mb;
     if (b >= B) continue;
-    const float* pr = sA + mb * 10 * JO + j;
-    const float* hr = sA + (XP + mb * 4) * JO + j;
     float* G = a.G + fidx(b, 0, m, j, Mp, J);
     const long long fs = (long long)Mp * J * 2;
-    G[0] = pr[0];
-    G[1] = pr[JO];
-    G[fs] = pr[2 * JO];
-    G[fs + 1] = pr[3 * JO];
-    G[2 * fs] = pr[4 * JO];
-    G[2 * fs + 1] = pr[5 * JO];
-    G[3 * fs] = mr * pr[7 * JO] + hr[0];
-    G[3 * fs + 1] = -mr * pr[6 * JO] + hr[JO];
-    G[4 * fs] = mr * pr[9 * JO] - hr[2 * JO];
-    G[4 * fs + 1] = -mr * pr[8 * JO] - hr[3 * JO];
+    float P[10], H[4];
+    if constexpr (FOLD) {
+      // X(j) = E + O on the southern half, X(J-1-j) = E - O on the northern (H: O - E).
+      const int js = j < J / 2 ? j : J - 1 - j;
+      const float sg = j < J / 2 ? 1.0f : -1.0f;
+      const float* pe = sA + mb * 10 * JO + js;
+      const float* he = sA + (XP + mb * 4) * JO + js;
+#pragma unroll
+      for (int c = 0; c < 10; ++c) P[c] = pe[c * JO] + sg * pe[c * JO + J / 2];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) H[c] = sg * he[c * JO] + he[c * JO + J / 2];
+    } else {
+      const float* pr = sA + mb * 10 * JO + j;
+      const float* hr = sA + (XP + mb * 4) * JO + j;
+#pragma unroll
+      for (int c = 0; c < 10; ++c) P[c] = pr[c * JO];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) H[c] = hr[c * JO];
+    }
+    G[0] = P[0];
+    G[1] = P[1];
+    G[fs] = P[2];
+    G[fs + 1] = P[3];
+    G[2 * fs] = P[4];
+    G[2 * fs + 1] = P[5];
+    G[3 * fs] = mr * P[7] + H[0];
+    G[3 * fs + 1] = -mr * P[6] + H[1];
+    G[4 * fs] = mr * P[9] - H[2];
+    G[4 * fs + 1] = -mr * P[8] - H[3];
   }
 }
 
-template <int MT, int MB, bool GEN, bool LIN = false>
+template <int MT, int MB, bool GEN, bool LIN, bool FOLD>
 __global__ void __launch_bounds__(256) k_leg_mfma(LegArgs<float> a, int heavy_first) {
   extern __shared__ __align__(16) float lds[];
   int m, grp;
   leg_item(heavy_first, m, grp);
-  leg_mfma_body<MT, MB, GEN, LIN>(a, m, grp, lds);
+  leg_mfma_body<MT, MB, GEN, LIN, FOLD>(a, m, grp, lds);
 }
 
 // The same kernel allocated for at least 4 waves per SIMD (MFMA accumulators in VGPRs), for the
 // specialized modified-Euler update where that costs no spills: T42 (82 registers at 4
 // members, 5 workgroups per CU with the LDS) and T63 with 2 or 4 members.
-template <int MT, int MB, bool GEN, bool LIN = false>
+template <int MT, int MB, bool GEN, bool LIN, bool FOLD>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_leg_mfma_w4(LegArgs<float> a, int heavy_first) {
   extern __shared__ __align__(16) float lds[];
   int m, grp;
   leg_item(heavy_first, m, grp);
-  leg_mfma_body<MT, MB, GEN, LIN>(a, m, grp, lds);
+  leg_mfma_body<MT, MB, GEN, LIN, FOLD>(a, m, grp, lds);
 }
 
 // Two plans of different truncations in one launch: block = grp * (M+1) + m, plan 0's first.
-template <int MA, int MBA, int MB, int MBB, bool GEN>
+template <int MA, int MBA, int MB, int MBB, bool GEN, bool FOLD>
 __global__ void __launch_bounds__(256) k_leg_mfma_pair(LegArgs<float> a0, LegArgs<float> a1, int nblk0) {
   extern __shared__ __align__(16) float lds[];
   const int blk = blockIdx.x;
   if (blk < nblk0)
-    leg_mfma_body<MA, MBA, GEN>(a0, blk % (MA + 1), blk / (MA + 1), lds);
+    leg_mfma_body<MA, MBA, GEN, false, FOLD>(a0, blk % (MA + 1), blk / (MA + 1), lds);
   else
-    leg_mfma_body<MB, MBB, GEN>(a1, (blk - nblk0) % (MB + 1), (blk - nblk0) / (MB + 1), lds);
+    leg_mfma_body<MB, MBB, GEN, false, FOLD>(a1, (blk - nblk0) % (MB + 1), (blk - nblk0) / (MB + 1), lds);
 }
 
 // Whether a launch needs the general spectral update (any scheme other than plain modified Euler).
@@ -962,17 +1104,33 @@ static hipError_t launch_leg_mfma_t(const LegArgs<float>& a, hipStream_t s, bool
   dim3 grid(MT + 1, (a.B + MB - 1) / MB);
   constexpr bool kW4 = MT == 42 || (MT == 63 && MB <= 4);
   const int heavy_first = shared ? 0 : 1;
-  if (leg_generic(a))
-    hipLaunchKernelGGL((k_leg_mfma<MT, MB, true>), grid, dim3(256), smem, s, a, heavy_first);
-  else if constexpr (kW4) {
-    if (a.lineage)
-      hipLaunchKernelGGL((k_leg_mfma_w4<MT, MB, false, true>), grid, dim3(256), smem, s, a, heavy_first);
-    else
-      hipLaunchKernelGGL((k_leg_mfma_w4<MT, MB, false>), grid, dim3(256), smem, s, a, heavy_first);
-  } else if (a.lineage) {
-    hipLaunchKernelGGL((k_leg_mfma<MT, MB, false, true>), grid, dim3(256), smem, s, a, heavy_first);
+  if (a.leg_fold) {
+    if (leg_generic(a))
+      hipLaunchKernelGGL((k_leg_mfma<MT, MB, true, false, true>), grid, dim3(256), smem, s, a, heavy_first);
+    else if constexpr (kW4) {
+      if (a.lineage)
+        hipLaunchKernelGGL((k_leg_mfma_w4<MT, MB, false, true, true>), grid, dim3(256), smem, s, a, heavy_first);
+      else
+        hipLaunchKernelGGL((k_leg_mfma_w4<MT, MB, false, false, true>), grid, dim3(256), smem, s, a, heavy_first);
+    } else if (a.lineage) {
+      hipLaunchKernelGGL((k_leg_mfma<MT, MB, false, true, true>), grid, dim3(256), smem, s, a, heavy_first);
+    } else {
+      hipLaunchKernelGGL((k_leg_mfma<MT, MB, false, false, true>), grid, dim3(256), smem, s, a, heavy_first);
+    }
   } else {
-    hipLaunchKernelGGL((k_leg_mfma<MT, MB, false>), grid, dim3(256), smem, s, a, heavy_first);
+    if (leg_generic(a))
+      hipLaunchKernelGGL((k_leg_mfma<MT, MB, true, false, false>), grid, dim3(256), smem, s, a, heavy_first);
+    else if constexpr (kW4) {
+      if (a.lineage)
+        hipLaunchKernelGGL((k_leg_mfma_w4<MT, MB, false, true, false>), grid, dim3(256), smem, s, a, heavy_first);
+      else
+        hipLaunchKernelGGL((k_leg_mfma_w4<MT, MB, false, false, false>), grid, dim3(256), smem, s, a,
+                           heavy_first);
+    } else if (a.lineage) {
+      hipLaunchKernelGGL((k_leg_mfma<MT, MB, false, true, false>), grid, dim3(256), smem, s, a, heavy_first);
+    } else {
+      hipLaunchKernelGGL((k_leg_mfma<MT, MB, false, false, false>), grid, dim3(256), smem, s, a, heavy_first);
+    }
   }
   return hipGetLastError();
 }
@@ -1046,11 +1204,23 @@ static hipError_t launch_leg_pair_t(const LegArgs<float>& a0, const LegArgs<floa
   const size_t smem = sizeof(float) * (size_t)cmax(LegShape<MA, MBA>::LDS_FLOATS, LegShape<MB, MBB>::LDS_FLOATS);
   const int n0 = (MA + 1) * ((a0.B + MBA - 1) / MBA);
   const int n1 = (MB + 1) * ((a1.B + MBB - 1) / MBB);
-  // The general update also covers the prev lineage of a pair's final window steps.
-  if (leg_generic(a0) || leg_generic(a1) || a0.lineage || a1.lineage)
-    hipLaunchKernelGGL((k_leg_mfma_pair<MA, MBA, MB, MBB, true>), dim3(n0 + n1), dim3(256), smem, s, a0, a1, n0);
-  else
-    hipLaunchKernelGGL((k_leg_mfma_pair<MA, MBA, MB, MBB, false>), dim3(n0 + n1), dim3(256), smem, s, a0, a1, n0);
+  // The general update also covers the prev lineage of a pair's final window steps.  Both plans
+  // take the folded contractions when both have them on.
+  const bool gen = leg_generic(a0) || leg_generic(a1) || a0.lineage || a1.lineage;
+  if (a0.leg_fold && a1.leg_fold) {
+    if (gen)
+      hipLaunchKernelGGL((k_leg_mfma_pair<MA, MBA, MB, MBB, true, true>), dim3(n0 + n1), dim3(256), smem, s, a0, a1, n0);
+    else
+      hipLaunchKernelGGL((k_leg_mfma_pair<MA, MBA, MB, MBB, false, true>), dim3(n0 + n1), dim3(256), smem, s, a0, a1,
+                         n0);
+  } else {
+    if (gen)
+      hipLaunchKernelGGL((k_leg_mfma_pair<MA, MBA, MB, MBB, true, false>), dim3(n0 + n1), dim3(256), smem, s, a0, a1,
+                         n0);
+    else
+      hipLaunchKernelGGL((k_leg_mfma_pair<MA, MBA, MB, MBB, false, false>), dim3(n0 + n1), dim3(256), smem, s, a0, a1,
+                         n0);
+  }
   return hipGetLastError();
 }
 

@@ -85,6 +85,8 @@ struct swx_plan {
   void* Pi = nullptr;
   void* Hi = nullptr;
   float* fragP[4] = {nullptr, nullptr, nullptr, nullptr};  // MFMA Legendre tables: fwd P, fwd H, inv P, inv H
+  float* fragF[4] = {nullptr, nullptr, nullptr, nullptr};  // the same, equatorially folded (LegArgs ffwd*, finv*)
+  int leg_fold = 2;  // SWX_OPT_LEG_FOLD: 1 = full contractions, 2 = folded
   void* tw = nullptr;
   float* TinvF = nullptr;  // MFMA grid kernel DFT tables, fragment order (FP32 plans)
   float* TfwdF = nullptr;
@@ -275,6 +277,11 @@ LegArgs<R> leg_args(swx_plan* p, int mode) {
   a.fwdH = p->fragP[1];
   a.invP = p->fragP[2];
   a.invH = p->fragP[3];
+  a.ffwdP = p->fragF[0];
+  a.ffwdH = p->fragF[1];
+  a.finvP = p->fragF[2];
+  a.finvH = p->fragF[3];
+  a.leg_fold = p->leg_fold == 2 && p->fragF[0] ? 1 : 0;
   a.coef = p->coef;
   a.ncoef = p->cl.ncoef;
   a.f2 = p->f2;
@@ -1091,6 +1098,75 @@ int swx_plan_create(const swx_plan_desc* desc, const double* params, swx_plan** 
       }
       for (int q = 0; q < 4 && rc == SWX_OK; ++q)
         if ((rc = dalloc(p, &p->fragP[q], per * 4)) == SWX_OK) rc = upload_real(p->fragP[q], fr[q], 4);
+      // Equatorially folded tables (LegArgs ffwd*, finv*): the basis on the southern nodes j < J/2,
+      // rows by degree parity.  They rely on mu_{J-1-j} = -mu_j and the basis parity
+      // (checked here to 1e-12 of each block's largest value; otherwise the plan keeps the full
+      // contractions).
+      {
+        const int J2 = J / 2, NTF = leg_fold_ntf(N), KSE = leg_fold_kse(N), KSO = leg_fold_kso(N);
+        const int KSL = KSE + KSO;
+        bool sym = J % 32 == 0;
+        for (int j = 0; j < J2 && sym; ++j)
+          sym = std::fabs(p->geo.mus[j] + p->geo.mus[J - 1 - j]) <= 1e-14;
+        for (int m = 0; m <= M && sym; ++m)
+          for (int n = m; n <= N && sym; ++n) {
+            double amax = 0.0, err = 0.0;
+            const double sp = ((n - m) & 1) ? -1.0 : 1.0;
+            for (int j = 0; j < J; ++j) {
+              const size_t a0 = ((size_t)j * Mp + m) * (N + 1) + n, a1 = ((size_t)(J - 1 - j) * Mp + m) * (N + 1) + n;
+              amax = std::max(amax, std::max(std::fabs(p->geo.P[a0]), std::fabs(p->geo.H[a0])));
+              err = std::max(err, std::fabs(p->geo.P[a1] - sp * p->geo.P[a0]));
+              err = std::max(err, std::fabs(p->geo.H[a1] + sp * p->geo.H[a0]));
+            }
+            sym = err <= 1e-12 * amax;
+          }
+        if (sym) {
+          const size_t perf = (size_t)Mp * NTF * 64 * (J / 8), peri = (size_t)Mp * (J / 32) * 64 * KSL;
+          std::vector<std::vector<double>> ff(4);
+          ff[0].assign(perf, 0.0);
+          ff[1].assign(perf, 0.0);
+          ff[2].assign(peri, 0.0);
+          ff[3].assign(peri, 0.0);
+          for (int m = 0; m <= M; ++m) {
+            const int K = N + 1 - m, Ke = (K + 1) / 2, Ko = K / 2;
+            const int nte = leg_fold_nte(K), kb = leg_fold_kb(K);
+            for (int lane = 0; lane < 64; ++lane) {
+              const int ln = lane & 15, lg = lane >> 4;
+              for (int r = 0; r < NTF; ++r)
+                for (int ks = 0; ks < J / 8; ++ks) {
+                  const int i = 16 * (r < nte ? r : r - nte) + ln;
+                  const int np = r < nte ? 2 * i : 2 * i + 1;
+                  if (r < nte ? i >= Ke : i >= Ko) continue;
+                  const int j = 4 * ks + lg;
+                  const size_t src = ((size_t)j * Mp + m) * (N + 1) + (m + np);
+                  const size_t dst = (((size_t)m * NTF + r) * 64 + lane) * (J / 8) + ks;
+                  ff[0][dst] = p->geo.P[src];
+                  ff[1][dst] = p->geo.H[src];
+                }
+              for (int jt = 0; jt < J2 / 16; ++jt)
+                for (int slot = 0; slot < KSL; ++slot) {
+                  int np = -1;
+                  if (slot < KSE) {
+                    const int row = 4 * slot + lg;
+                    if (row < Ke) np = 2 * row;
+                  } else {
+                    const int row = 4 * (kb + slot - KSE) + lg;
+                    if (row >= Ke && row < K) np = 2 * (row - Ke) + 1;
+                  }
+                  if (np < 0) continue;
+                  const int j = 16 * jt + ln;
+                  const size_t src = ((size_t)j * Mp + m) * (N + 1) + (m + np);
+                  const size_t dst = (((size_t)m * (J2 / 16) + jt) * 64 + lane) * KSL + slot;
+                  ff[2][dst] = p->geo.P[src];
+                  ff[3][dst] = p->geo.H[src];
+                }
+            }
+            (void)Ko;
+          }
+          for (int q = 0; q < 4 && rc == SWX_OK; ++q)
+            if ((rc = dalloc(p, &p->fragF[q], ff[q].size() * 4)) == SWX_OK) rc = upload_real(p->fragF[q], ff[q], 4);
+        }
+      }
       if (rc) {
         swx_plan_destroy(p);
         return rc;
@@ -1239,7 +1315,7 @@ void swx_plan_destroy(swx_plan* p) {
   if (!p) return;
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   void* bufs[] = {p->Lam, p->Tbar, p->Gbar, p->pgrad, p->Pf, p->Hf, p->Pi, p->Hi, p->fragP[0], p->fragP[1],
-                  p->fragP[2], p->fragP[3], p->tw, p->TinvF, p->TfwdF, p->TinvH, p->TfwdH, p->TinvB, p->TfwdB, p->geo_d, p->f2, p->coef, p->S, p->T, p->G, p->Gprev,
+                  p->fragP[2], p->fragP[3], p->fragF[0], p->fragF[1], p->fragF[2], p->fragF[3], p->tw, p->TinvF, p->TfwdF, p->TinvH, p->TfwdH, p->TinvB, p->TfwdB, p->geo_d, p->f2, p->coef, p->S, p->T, p->G, p->Gprev,
                   p->raw, p->lag, p->g0uv, p->phys, p->uvic, p->part_uv, p->dead, p->R0f, p->R0p, p->resid_c, p->lagx,
                   p->Gsyn};
   for (void* q : bufs)
@@ -1618,6 +1694,11 @@ int swx_plan_set_option(swx_plan* p, int option, int value) {
       if (value < -1) return fail(SWX_ERR_ARG, "SWX_OPT_PHYS_LINEAGE must be >= -1");
       p->opt_lineage = value;
       break;
+    case SWX_OPT_LEG_FOLD:
+      if (value != 1 && value != 2) return fail(SWX_ERR_ARG, "SWX_OPT_LEG_FOLD must be 1 (full) or 2 (folded)");
+      if (value == 2 && !p->fragF[0]) return fail(SWX_ERR_ARG, "SWX_OPT_LEG_FOLD: no folded tables for this plan");
+      p->leg_fold = value;
+      break;
     case SWX_OPT_PHYS_STEPS:
       if (value < -1) return fail(SWX_ERR_ARG, "SWX_OPT_PHYS_STEPS must be >= -1");
       p->opt_phys_steps = value;
@@ -1724,6 +1805,7 @@ int swx_plan_get_option(const swx_plan* p, int option, int* value) {
     case SWX_OPT_LEG_KERNEL: *value = p->opt_plain_leg ? 1 : 0; break;
     case SWX_OPT_PHYS_STEPS: *value = p->opt_phys_steps; break;
     case SWX_OPT_PHYS_LINEAGE: *value = p->opt_lineage; break;
+    case SWX_OPT_LEG_FOLD: *value = p->leg_fold == 2 && p->fragF[0] ? 2 : 1; break;
     case SWX_OPT_LEG_MB: *value = p->leg_mb; break;
     case SWX_OPT_GRID_FOLD: *value = p->grid_fold; break;
     case SWX_OPT_CONCURRENT: *value = p->in_pipeline ? 1 : 0; break;

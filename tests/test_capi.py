@@ -52,6 +52,24 @@ def test_host_geometry_matches_oracle(M):
     np.testing.assert_allclose(H, Ho, rtol=0, atol=1e-11)
 
 
+@pytest.mark.parametrize("M", [42, 63, 106])
+def test_host_geometry_equatorial_parity(M):
+    """What the folded Legendre kernels rely on: mirrored nodes mu_{J-1-j} = -mu_j, equal weights,
+    P(-mu) = (-1)^{n+m} P(mu) and H(-mu) = -(-1)^{n+m} H(mu) for every (m, n) of the library's basis
+    (reference normalization included), to 1e-12 of each block's largest value."""
+    from my_swampe_amd import _capi
+
+    mus, w, lam, P, H = _capi.geometry(M)
+    J = mus.size
+    np.testing.assert_allclose(mus[::-1], -mus, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(w[::-1], w, rtol=0, atol=1e-15)
+    m, n = np.meshgrid(np.arange(M + 1), np.arange(M + 1), indexing="ij")
+    sign = np.where((n + m) % 2 == 0, 1.0, -1.0)
+    scale = np.maximum(np.abs(P).max(axis=0), np.abs(H).max(axis=0))
+    assert np.all(np.abs(P[::-1] - sign * P) <= 1e-12 * scale)
+    assert np.all(np.abs(H[::-1] + sign * H) <= 1e-12 * scale)
+
+
 def test_exact_basis_is_orthonormal_at_T106():
     from my_swampe_amd import _capi
 

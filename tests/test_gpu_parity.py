@@ -266,9 +266,10 @@ def test_fp32_transforms_close_to_fp64(torch_gpu):
 
 @pytest.mark.parametrize("M", [42, 63, 106])
 def test_mfma_grid_kernel_matches_plain_fp32(M, torch_gpu):
-    """The MFMA grid (full and half-period folded DFTs) and Legendre kernels (FP32) vs the plain FP32
-    kernels and the FP64 oracle, 12 forced steps from a balanced zonal flow (Williamson test 2
-    fields, U ~ 40 m/s).  Relative L2 <= 5e-6 for Phi, eta and U."""
+    """The MFMA grid (full and half-period folded DFTs) and Legendre kernels (FP32; equatorially folded
+    contractions, and over all latitudes in "mfma_legfull") vs the plain FP32 kernels and the FP64
+    oracle, 12 forced steps from a balanced zonal flow (Williamson test 2 fields, U ~ 40 m/s).
+    Relative L2 <= 5e-6 for Phi, eta and U."""
     from my_swampe_amd import _capi
     from my_swampe_amd.model import param_row
     from my_swampe_amd.plan import Plan
@@ -282,11 +283,14 @@ def test_mfma_grid_kernel_matches_plain_fp32(M, torch_gpu):
     ics = orc.analytic_ic(s, 2, 0.3)
     _print_case("mfma-vs-plain", M=M)
     res = {}
-    for kind in ("plain", "mfma", "mfma_fold"):
+    kinds = ("plain", "mfma", "mfma_fold", "mfma_legfull")
+    for kind in kinds:
         plan = Plan(M, np.stack([p] * 3), precision="fp32")  # 3 members: a partial member group
         plan.set_option(_capi.OPT_GRID_KERNEL, 1 if kind == "plain" else 0)
         plan.set_option(_capi.OPT_LEG_KERNEL, 1 if kind == "plain" else 0)
-        plan.set_option(_capi.OPT_GRID_FOLD, 2 if kind == "mfma_fold" else 1)
+        plan.set_option(_capi.OPT_GRID_FOLD, 1 if kind == "mfma" else 2)
+        assert plan.get_option(_capi.OPT_LEG_FOLD) == 2  # the folded Legendre tables are built
+        plan.set_option(_capi.OPT_LEG_FOLD, 1 if kind == "mfma_legfull" else 2)
         plan.init(*ics)
         plan.run(2, 12)
         res[kind] = {k: _np(v[0]) for k, v in plan.export(fields=("Phi_curr", "U_curr", "eta_curr")).items()
@@ -297,7 +301,7 @@ def test_mfma_grid_kernel_matches_plain_fp32(M, torch_gpu):
                              U0_init=ics[3], V0_init=ics[4], diagnostics=False, return_history=False)["last_state"]
     for f in ("Phi_curr", "U_curr", "eta_curr"):
         r = getattr(ref, f)
-        for kind in ("plain", "mfma", "mfma_fold"):
+        for kind in kinds:
             rel = np.linalg.norm(res[kind][f] - r) / np.linalg.norm(r)
             print(M, kind, f, "relL2 vs oracle", rel)
             assert rel < 5e-6, (M, kind, f, rel)
@@ -498,7 +502,8 @@ def test_bf16x3_grid_kernel_vs_oracle(M, torch_gpu):
 
 
 @pytest.mark.parametrize("M", [42, 63, 106])
-def test_legendre_member_group_sizes_agree_bitwise(M, torch_gpu):
+@pytest.mark.parametrize("fold", [1, 2])
+def test_legendre_member_group_sizes_agree_bitwise(M, fold, torch_gpu):
     """Every member-group size of the MFMA Legendre kernel (the autotuner's candidates) gives the
     same state bitwise: a member's arithmetic does not depend on which members share its tile.
     5 members (partial groups for every size), 6 forced steps, FP32 transforms."""
@@ -507,7 +512,7 @@ def test_legendre_member_group_sizes_agree_bitwise(M, torch_gpu):
     from my_swampe_amd.plan import Plan
 
     torch = torch_gpu
-    _print_case("leg-mb", M=M)
+    _print_case("leg-mb", M=M, fold=fold)
     dt = 60.0 if M == 106 else 120.0
     rng = np.random.default_rng(5)
     params = np.stack([param_row(dt=dt, a=8.2e7, omega=3.2e-5, g=9.8, Phibar=3e5, taurad=10 ** rng.uniform(4, 6),
@@ -520,6 +525,7 @@ def test_legendre_member_group_sizes_agree_bitwise(M, torch_gpu):
         plan = Plan(M, params, precision="fp32")
         tuned = plan.get_option(_capi.OPT_LEG_MB)
         plan.set_option(_capi.OPT_LEG_MB, mb)
+        plan.set_option(_capi.OPT_LEG_FOLD, fold)
         plan.init_spectral(c)
         plan.run(2, 6)
         states[mb] = plan.state_get()
