@@ -668,6 +668,14 @@ struct LegShape {
   // Inverse inputs: rows n < K, zero to the k-step multiple (phase I reads rows < ceil4(K)).
   static constexpr int SX = ceil4(Np) * XS;
   static constexpr int LDS_FLOATS = SA + SX;
+  // Folded kernel: the P columns unpadded (10 MB), the H columns right after them.  A P tile that
+  // reaches past them reads H inputs; its outputs there are not stored (forward) or are
+  // overwritten by the H tile's (inverse: the same wave stores it later).  4 KB less LDS at T42.
+  static constexpr int RPF = 10 * MB, CTPF = (RPF + 15) / 16, RSF = RPF + RH + 1;
+  static constexpr int XPF = 10 * MB, CIPF = (XPF + 15) / 16, XTF = XPF + XH, XSF = XTF + 1;
+  static constexpr int SAF = cmax(cmax(J * RSF, 16 * leg_fold_ntf(MT) * RSF), XTF * JO);
+  static constexpr int SXF = ceil4(Np) * XSF;
+  static constexpr int LDS_FLOATS_F = SAF + SXF;
 };
 
 // Basis fragments of forward tile t (row tile t / CT, P or H by column tile t % CT) for this
@@ -691,10 +699,10 @@ template <int MT, int MB>
 __device__ __forceinline__ void leg_fwd_frags_fold(const LegArgs<float>& a, int m, int t, int ntile, int lane,
                                                    v4f (&av)[LegShape<MT, MB>::J / 32]) {
   using S = LegShape<MT, MB>;
-  constexpr int CT = S::RT / 16, J = S::J, NTF = leg_fold_ntf(MT);
+  constexpr int CT = S::CTPF + S::RH / 16, J = S::J, NTF = leg_fold_ntf(MT);
   const int tc = t < ntile ? t : 0;
   const int r = tc / CT, ct = tc % CT;
-  const v4f* ar = reinterpret_cast<const v4f*>((ct * 16 < S::RP ? a.ffwdP : a.ffwdH) +
+  const v4f* ar = reinterpret_cast<const v4f*>((ct < S::CTPF ? a.ffwdP : a.ffwdH) +
                                                ((long long)(m * NTF + r) * 64 + lane) * (J / 8));
 #pragma unroll
   for (int q4 = 0; q4 < J / 32; ++q4) av[q4] = ar[q4];
@@ -711,12 +719,17 @@ __device__ __forceinline__ void leg_fwd_frags_fold(const LegArgs<float>& a, int 
 template <int MT, int MB, bool GEN = true, bool LIN = false, bool FOLD = true>
 __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int m, const int grp, float* lds) {
   using S = LegShape<MT, MB>;
-  constexpr int Np = S::Np, Mp = MT + 1, J = S::J, RP = S::RP, RT = S::RT, XP = S::XP, XT = S::XT, JO = S::JO;
-  constexpr int RS = S::RS, XS = S::XS;
+  constexpr int Np = S::Np, Mp = MT + 1, J = S::J, JO = S::JO;
+  // Column layout (folded: the P groups unpadded, LegShape).  P column tiles 0 .. CTP-1 start at
+  // 16 ct, H tiles at RP + 16 (ct - CTP).
+  constexpr int RP = FOLD ? S::RPF : S::RP, RT = RP + S::RH, RS = FOLD ? S::RSF : S::RS;
+  constexpr int CTP = FOLD ? S::CTPF : S::RP / 16;
+  constexpr int XP = FOLD ? S::XPF : S::XP, XT = XP + S::XH, XS = XT + 1;
+  constexpr int SA = FOLD ? S::SAF : S::SA;
   constexpr int KNP = S::KNP;
   static_assert(KNP == leg_knp(MT), "fragment tables use leg_knp rows");
   float* sA = lds;           // [J][RS] inputs, then [K][RS] forward results, then [XT][JO] inverse results
-  float* sX = lds + S::SA;   // [KNP][XS] inverse-transform inputs
+  float* sX = lds + SA;      // [ceil4(Np)][XS] inverse-transform inputs
   const int b0 = grp * MB;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, ln = lane & 15, lg = lane >> 4;
   const int B = a.B, N = a.N;
@@ -734,6 +747,10 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
   constexpr int NPF = (MB * Np + 255) / 256;
   constexpr bool kPre = NPF <= 2;
   double scp[kPre ? NPF : 1][3][2];
+  // With one pair per thread, its coefficients and f2 too (the update's other global reads).
+  constexpr bool kPreC = NPF == 1;
+  UpdCoef ucp[1];
+  double f2p[1];
   if constexpr (kPre) {
     const long long fstride = (long long)Mp * Np * 2;
 #pragma unroll
@@ -747,11 +764,16 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
       for (int f = 0; f < 3; ++f)
 #pragma unroll
         for (int c = 0; c < 2; ++c) scp[u][f][c] = Sc[f * fstride + c];
+      if constexpr (kPreC) {
+        const int n = ok ? m + np : m;
+        ucp[0] = load_upd_coef(a.coef + (long long)(ok ? b0 + mb : 0) * a.ncoef, cl, m, n);
+        f2p[0] = a.f2[(long long)m * Np + n];
+      }
     }
   }
 
   // The first forward tile's basis fragments, in flight during the staging.
-  constexpr int CT = RT / 16;
+  constexpr int CT = CTP + S::RH / 16;
   constexpr int NRT = FOLD ? leg_fold_ntf(MT) : ceil16(Np) / 16;  // row tiles per m, at most
   constexpr int MAXT = (NRT * CT + 3) / 4;
   constexpr int KSF = FOLD ? J / 8 : J / 4;                          // forward k-steps
@@ -830,8 +852,9 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
       }
     }
   }
-  static_assert(S::SA % 4 == 0 && S::SX % 4 == 0, "sX is zeroed as float4s");
-  for (int e = tid; e < S::SX / 4; e += 256) reinterpret_cast<v4f*>(sX)[e] = v4f{0.f, 0.f, 0.f, 0.f};
+  constexpr int SX = FOLD ? S::SXF : S::SX;
+  static_assert(SA % 4 == 0 && SX % 4 == 0, "sX is zeroed as float4s");
+  for (int e = tid; e < SX / 4; e += 256) reinterpret_cast<v4f*>(sX)[e] = v4f{0.f, 0.f, 0.f, 0.f};
   constexpr int padH = RT - RP - MB * 8, padP = RP - MB * 10;
   if constexpr (padH > 0)
     for (int e = tid; e < J * padH; e += 256) sA[(e / padH) * RS + RP + MB * 8 + e % padH] = 0.0f;
@@ -862,27 +885,32 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
     }
     if (t < ntile) {
       const int ct = t % CT;
+      const int colb = ct < CTP ? 16 * ct : RP + 16 * (ct - CTP);
       // Folded: the B operand is s (rows 0 ..) for P tiles of even degrees and H tiles of odd
       // ones, else a (rows J/2 ..).
       int base = 0;
       if constexpr (FOLD) {
-        const bool even = t / CT < nte, isP = ct * 16 < RP;
+        const bool even = t / CT < nte, isP = ct < CTP;
         base = (even == isP) ? 0 : J / 2;
       }
-      const float* br = sA + (base + lg) * RS + ct * 16 + ln;
+      const float* br = sA + (base + lg) * RS + colb + ln;
 #pragma unroll
       for (int ks = 0; ks < KSF; ++ks)
         acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[ks / 4][ks % 4], br[4 * ks * RS], acc[q], 0, 0, 0);
     }
   }
   __syncthreads();
+  // Forward results, row tile by row tile (folded: the even-degree tiles, then the odd ones).
 #pragma unroll
   for (int q = 0; q < MAXT; ++q) {
     const int t = wave + 4 * q;
     if (t < ntile) {
       const int nt16 = t / CT, ct = t % CT;
+      const int col = (ct < CTP ? 16 * ct : RP + 16 * (ct - CTP)) + ln;
+      if (ct >= CTP || col < RP) {  // (a folded P tile's columns past RP are the H tile's)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) sA[(nt16 * 16 + 4 * lg + r) * RS + ct * 16 + ln] = acc[q][r];
+        for (int r = 0; r < 4; ++r) sA[(nt16 * 16 + 4 * lg + r) * RS + col] = acc[q][r];
+      }
     }
   }
   __syncthreads();
@@ -918,9 +946,9 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
     const long long fstride = (long long)Mp * Np * 2;
     const long long so = ((long long)m * Np + n) * 2;
     double eff[3][2], nw[3][2], dh[2], eh[2];
-    spectral_update(cf, cl, a.f2[(long long)m * Np + n], m, n, fl, fp, fh, Sb + 3 * fstride, Sb, so,
-                    fstride, eff, nw, kPre ? scp[kPre ? u : 0] : nullptr);
-    wind_potentials(cf, cl, m, n, nw[0], nw[1], dh, eh);
+    spectral_update(cf, cl, kPreC ? f2p[0] : a.f2[(long long)m * Np + n], m, n, fl, fp, fh, Sb + 3 * fstride, Sb,
+                    so, fstride, eff, nw, kPre ? scp[kPre ? u : 0] : nullptr, kPreC ? &ucp[0] : nullptr);
+    wind_potentials(cf, cl, m, n, nw[0], nw[1], dh, eh, kPreC ? &ucp[0] : nullptr);
     float* xr = sX + xrow * XS;
     for (int c = 0; c < 2; ++c) {
       xr[mb * 10 + 0 + c] = (float)eff[0][c];
@@ -936,7 +964,7 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
 
   // Phase I: inverse contraction; wave w owns latitude tiles jt = w, w+4, ... and every
   // column tile, so its P and H fragments are loaded once and reused across them.
-  constexpr int CIP = XP / 16, CIH = (XT - XP) / 16;
+  constexpr int CIP = FOLD ? S::CIPF : XP / 16, CIH = (XT - XP) / 16;
   if constexpr (FOLD) {
     // Units (jt, parity) over the southern latitude tiles jt < J/32: the even-degree partial
     // sums E into rows jt*16 .. of the column-major output, the odd ones O into rows
@@ -1083,6 +1111,17 @@ k_leg_mfma_w4(LegArgs<float> a, int heavy_first) {
   leg_mfma_body<MT, MB, GEN, LIN, FOLD>(a, m, grp, lds);
 }
 
+// The folded T42 kernel allocated for 5 waves per SIMD (at most 96 registers): with its 32 KB of
+// LDS, 5 workgroups per CU, also for the steps that advance the prev lineage.
+template <int MT, int MB, bool GEN, bool LIN, bool FOLD>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8)))
+k_leg_mfma_w5(LegArgs<float> a, int heavy_first) {
+  extern __shared__ __align__(16) float lds[];
+  int m, grp;
+  leg_item(heavy_first, m, grp);
+  leg_mfma_body<MT, MB, GEN, LIN, FOLD>(a, m, grp, lds);
+}
+
 // Two plans of different truncations in one launch: block = grp * (M+1) + m, plan 0's first.
 template <int MA, int MBA, int MB, int MBB, bool GEN, bool FOLD>
 __global__ void __launch_bounds__(256) k_leg_mfma_pair(LegArgs<float> a0, LegArgs<float> a1, int nblk0) {
@@ -1100,14 +1139,20 @@ static bool leg_generic(const LegArgs<float>& a) { return a.si || a.raw_filter |
 template <int MT, int MB>
 static hipError_t launch_leg_mfma_t(const LegArgs<float>& a, hipStream_t s, bool shared) {
   using S = LegShape<MT, MB>;
-  const size_t smem = sizeof(float) * S::LDS_FLOATS;
+  const size_t smem = sizeof(float) * (a.leg_fold ? S::LDS_FLOATS_F : S::LDS_FLOATS);
   dim3 grid(MT + 1, (a.B + MB - 1) / MB);
   constexpr bool kW4 = MT == 42 || (MT == 63 && MB <= 4);
+  constexpr bool kW5 = MT == 42 && MB == 4;
   const int heavy_first = shared ? 0 : 1;
   if (a.leg_fold) {
     if (leg_generic(a))
       hipLaunchKernelGGL((k_leg_mfma<MT, MB, true, false, true>), grid, dim3(256), smem, s, a, heavy_first);
-    else if constexpr (kW4) {
+    else if constexpr (kW5) {
+      if (a.lineage)
+        hipLaunchKernelGGL((k_leg_mfma_w5<MT, MB, false, true, true>), grid, dim3(256), smem, s, a, heavy_first);
+      else
+        hipLaunchKernelGGL((k_leg_mfma_w5<MT, MB, false, false, true>), grid, dim3(256), smem, s, a, heavy_first);
+    } else if constexpr (kW4) {
       if (a.lineage)
         hipLaunchKernelGGL((k_leg_mfma_w4<MT, MB, false, true, true>), grid, dim3(256), smem, s, a, heavy_first);
       else
@@ -1201,13 +1246,15 @@ static hipError_t launch_grid_pair_t(const GridArgs<float>& a0, const float* ti0
 
 template <int MA, int MBA, int MB, int MBB>
 static hipError_t launch_leg_pair_t(const LegArgs<float>& a0, const LegArgs<float>& a1, hipStream_t s) {
-  const size_t smem = sizeof(float) * (size_t)cmax(LegShape<MA, MBA>::LDS_FLOATS, LegShape<MB, MBB>::LDS_FLOATS);
+  const bool fold = a0.leg_fold && a1.leg_fold;
+  const size_t smem = sizeof(float) * (size_t)(fold ? cmax(LegShape<MA, MBA>::LDS_FLOATS_F, LegShape<MB, MBB>::LDS_FLOATS_F)
+                                                    : cmax(LegShape<MA, MBA>::LDS_FLOATS, LegShape<MB, MBB>::LDS_FLOATS));
   const int n0 = (MA + 1) * ((a0.B + MBA - 1) / MBA);
   const int n1 = (MB + 1) * ((a1.B + MBB - 1) / MBB);
   // The general update also covers the prev lineage of a pair's final window steps.  Both plans
   // take the folded contractions when both have them on.
   const bool gen = leg_generic(a0) || leg_generic(a1) || a0.lineage || a1.lineage;
-  if (a0.leg_fold && a1.leg_fold) {
+  if (fold) {
     if (gen)
       hipLaunchKernelGGL((k_leg_mfma_pair<MA, MBA, MB, MBB, true, true>), dim3(n0 + n1), dim3(256), smem, s, a0, a1, n0);
     else

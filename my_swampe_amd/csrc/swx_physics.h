@@ -396,12 +396,33 @@ __device__ inline void assemble_slots_si(const double* cf, double tc, double wj,
 //   fl.lineage also advances the classic RA filter of the prev level in spectral form.
 // Returns eff (the stored current level: Fourier rows of eta, delta, Phi) and nw (the
 // new level before the RAW adjustment: the winds are inverted from it).
+// The coefficients of one (member, m, n) that the modified-Euler update and the wind potentials
+// read, for callers that load them ahead of the update (the values of the coefficient table).
+struct UpdCoef {
+  double dt, aa, Phibar, alpha, sg, sph, tcmn, fmn;
+};
+
+__device__ inline UpdCoef load_upd_coef(const double* cf, const CoefLayout& cl, int m, int n) {
+  UpdCoef c;
+  c.dt = cf[C_DT];
+  c.aa = cf[C_A];
+  c.Phibar = cf[C_PHIBAR];
+  c.alpha = cf[C_ALPHA];
+  c.sg = cf[cl.n_sigma + n];
+  c.sph = cf[cl.n_sigmaphi + n];
+  c.tcmn = cf[cl.n_tcmn + n];
+  c.fmn = (m == 0 && n == 1) ? cf[C_FMN01] : 0.0;
+  return c;
+}
+
 // sc_pre: the current level of this (member, m, n) already loaded by the caller, or null.
+// pc: its coefficients already loaded by the caller (load_upd_coef), or null.
 __device__ inline void spectral_update(const double* cf, const CoefLayout& cl, double f2, int m, int n,
                                        const StepFlags& fl, const double fp[5][2], const double fh[4][2],
                                        double* Scur, double* Sprev, long long so, long long fstride, double eff[3][2],
-                                       double nw[3][2], const double (*sc_pre)[2] = nullptr) {
-  const double dt = cf[C_DT], aa = cf[C_A], Phibar = cf[C_PHIBAR];
+                                       double nw[3][2], const double (*sc_pre)[2] = nullptr,
+                                       const UpdCoef* pc = nullptr) {
+  const double dt = pc ? pc->dt : cf[C_DT], aa = pc ? pc->aa : cf[C_A], Phibar = pc ? pc->Phibar : cf[C_PHIBAR];
   const double nn = (double)n * (double)(n + 1);
   // The prev level is read only by the schemes and filters that use it (the modified-Euler
   // step without RAW reads and writes the current level only).
@@ -465,7 +486,7 @@ __device__ inline void spectral_update(const double* cf, const CoefLayout& cl, d
       }
     }
     if (fl.diffflag) {
-      const double sg = cf[cl.n_sigma + n], sph = cf[cl.n_sigmaphi + n];
+      const double sg = pc ? pc->sg : cf[cl.n_sigma + n], sph = pc ? pc->sph : cf[cl.n_sigmaphi + n];
       for (int c = 0; c < 2; ++c) {
         nw[2][c] *= sph;
         nw[1][c] *= sg;
@@ -473,7 +494,7 @@ __device__ inline void spectral_update(const double* cf, const CoefLayout& cl, d
       }
     }
   }
-  const double alpha = cf[C_ALPHA];
+  const double alpha = pc ? pc->alpha : cf[C_ALPHA];
   for (int f = 0; f < 3; ++f)
     for (int c = 0; c < 2; ++c) {
       double cur = nw[f][c];
@@ -524,9 +545,10 @@ __device__ inline void residue_step(const double* cf, const StepFlags& fl, doubl
 // Wind potentials of the inverse transform (spectral_transform.py:577-582):
 // delta_hat = delta a/(n(n+1)), eta_hat = (eta - f) a/(n(n+1)), zero at n = 0.
 __device__ inline void wind_potentials(const double* cf, const CoefLayout& cl, int m, int n, const double eta[2],
-                                       const double delta[2], double dh[2], double eh[2]) {
-  const double tcmn = cf[cl.n_tcmn + n];
-  const double f = (m == 0 && n == 1) ? cf[C_FMN01] : 0.0;
+                                       const double delta[2], double dh[2], double eh[2],
+                                       const UpdCoef* pc = nullptr) {
+  const double tcmn = pc ? pc->tcmn : cf[cl.n_tcmn + n];
+  const double f = pc ? pc->fmn : ((m == 0 && n == 1) ? cf[C_FMN01] : 0.0);
   for (int c = 0; c < 2; ++c) {
     dh[c] = (n == 0) ? 0.0 : delta[c] * tcmn;
     eh[c] = (n == 0) ? 0.0 : (eta[c] - (c == 0 ? f : 0.0)) * tcmn;
