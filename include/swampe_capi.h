@@ -335,8 +335,9 @@ int swx_plan_phase_curve(swx_plan *plan, const double *phases, int K, int normal
 /*   SWX_OPT_GRID_FOLD: which FP32 matrix-core grid kernel runs: 1 = the full
  *     longitude DFTs, 2 = the half-period folded DFTs (x(i + I/2) from the
  *     even/odd wavenumber sums: half the matrix-core work).  Set at plan
- *     creation to the faster of the two, timed on the plan's own buffers;
- *     setting it forces one.  Results agree to FP32 rounding. */
+ *     creation to 2 wherever the folded kernel fits (a fixed rule, so plans of
+ *     one shape round alike); setting it forces one.  Results agree to FP32
+ *     rounding. */
 #define SWX_OPT_GRID_FOLD 6
 /*   SWX_OPT_CONCURRENT: 1 = the plan is stepped beside other plans' kernels (swx_pipeline_create
  *     sets it for its plans; set it when driving plans concurrently on several streams

@@ -401,15 +401,15 @@ def run_model_scan(*, M: int, dt: Scalar, tmax: int, Phibar: Scalar, omega: Scal
     params = static.params.copy()
     params[_capi.PARAM_NAMES.index("si_alpha")] = float(si_alpha)
     params[_capi.PARAM_NAMES.index("williams_alpha")] = float(williams_alpha)
-    plan = Plan(static.M, params[None, :], precision=precision, expflag=flags.expflag,
-                forcflag=flags.forcflag, diffflag=flags.diffflag, modalflag=flags.modalflag,
-                diagnostics=flags.diagnostics, test=test, semi_implicit=flags.semi_implicit,
-                raw_filter=flags.raw_filter)
+    plan = _run_plan(int(static.M), precision, flags, test, params)
     try:
         plan.init(eta0, delta0, Phi0, U0, V0)
         nsteps = int(tmax) - start
         t_seq = torch.arange(start, tmax, dtype=torch.int32, device=plan.device)
-        hist = plan.run(start, nsteps, history=return_history)
+        if return_history:
+            hist = plan.run(start, nsteps, history=True)
+        else:
+            _advance(plan, start, nsteps)
         ex = plan.export()
         last_state = _state_from_export(ex, squeeze=True)
         if not return_history:
@@ -424,7 +424,48 @@ def run_model_scan(*, M: int, dt: Scalar, tmax: int, Phibar: Scalar, omega: Scal
                     dead_first_idx=first)
     finally:
         torch.cuda.current_stream(plan.device).synchronize()
-        plan.close()
+
+
+# Plans of run_model_scan, kept across calls per structure (truncation, precision, scheme flags,
+# test) as the reference keeps its compiled scan per static configuration (model.py:88-153):
+# a call loads its parameters (swx_plan_set_params) and initial state into the cached plan, so
+# the basis tables, the tuned tile sizes and the captured step graphs are built once.
+_RUN_PLANS: "Dict[tuple, Plan]" = {}
+_RUN_PLANS_MAX = 4
+_GRAPH_STEPS = 32
+
+
+def _run_plan(M: int, precision: str, flags: RunFlags, test: Optional[int], params: np.ndarray) -> Plan:
+    key = (M, precision, bool(flags.expflag), bool(flags.forcflag), bool(flags.diffflag), bool(flags.modalflag),
+           bool(flags.diagnostics), bool(flags.semi_implicit), bool(flags.raw_filter), test)
+    plan = _RUN_PLANS.pop(key, None)
+    if plan is None:
+        plan = Plan(M, params[None, :], precision=precision, expflag=flags.expflag, forcflag=flags.forcflag,
+                    diffflag=flags.diffflag, modalflag=flags.modalflag, diagnostics=flags.diagnostics, test=test,
+                    semi_implicit=flags.semi_implicit, raw_filter=flags.raw_filter)
+        while len(_RUN_PLANS) >= _RUN_PLANS_MAX:
+            _RUN_PLANS.pop(next(iter(_RUN_PLANS))).close()
+    else:
+        plan.set_params(params[None, :])
+    _RUN_PLANS[key] = plan  # most recently used last
+    return plan
+
+
+def _advance(plan: Plan, t0: int, nsteps: int) -> None:
+    """nsteps steps without history from step index t0: the graph driver (captured step graphs
+    replayed, swx_run_graph; bitwise the eager steps) on a stream of the plan's own, ordered after
+    and before the caller's current stream."""
+    if nsteps <= 0:
+        return
+    torch = require_gpu()
+    cur = torch.cuda.current_stream(plan.device)
+    s = getattr(plan, "_graph_stream", None)
+    if s is None:
+        s = torch.cuda.Stream(device=plan.device)
+        plan._graph_stream = s
+    s.wait_stream(cur)
+    plan.run_graph(t0, nsteps, graph_steps=_GRAPH_STEPS, stream=s)
+    cur.wait_stream(s)
 
 
 def _outs_from_history(hist, t_seq, diagnostics, nsteps, plan: Plan, squeeze: bool):
@@ -727,7 +768,7 @@ def _step_once_state_only(state: State, t, static: Static, flags: RunFlags, test
     B, batched = _members(state.Phi_curr, static)
     sess = _session(static, flags, test, B, batched)
     _load_state(sess, state, static, flags, test, Uic, Vic)
-    sess.plan.run(int(t), 1)
+    _advance(sess.plan, int(t), 1)
     return _export_state(sess)
 
 
@@ -762,5 +803,5 @@ def simulate_scan_last(*, static: Static, flags: RunFlags, state0: State, t_seq,
     sess = _session(static, flags, test, B, batched)
     _load_state(sess, state0, static, flags, test, Uic, Vic)
     for t0, n in _runs(_t_list(t_seq)):
-        sess.plan.run(t0, n)
+        _advance(sess.plan, t0, n)
     return _export_state(sess)

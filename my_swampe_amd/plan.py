@@ -84,9 +84,20 @@ class Plan:
 
     # -- lifetime ---------------------------------------------------------
     def close(self) -> None:
+        """Destroy the native plan, after closing the ensembles and pipelines that step it (their
+        native objects hold the plan)."""
+        for user in list(getattr(self, "_users", ())):
+            user.close()
         if getattr(self, "_h", None) is not None and self._h.value:
             self._lib.swx_plan_destroy(self._h)
             self._h = None
+
+    def _add_user(self, user) -> None:
+        import weakref
+
+        if getattr(self, "_users", None) is None:
+            self._users = weakref.WeakSet()
+        self._users.add(user)
 
     def __del__(self):  # pragma: no cover - best effort
         try:
@@ -340,6 +351,8 @@ class Ensemble:
         _capi.check(self._lib.swx_ensemble_create(a._h, b._h, ctypes.byref(h)), "swx_ensemble_create")
         self._h = h
         self.device = a.device
+        for p in self.plans:
+            p._add_user(self)
 
     def run_graph(self, t0: int, nsteps: int, graph_steps: int = 32, stream=None) -> None:
         """Advance both plans by nsteps steps from t0 (as Plan.run_graph does for each)."""
@@ -374,6 +387,8 @@ class Pipeline:
         _capi.check(self._lib.swx_pipeline_create(arr, len(self.plans), ctypes.byref(h)), "swx_pipeline_create")
         self._h = h
         self.device = self.plans[0].device
+        for p in self.plans:
+            p._add_user(self)
 
     def run_graph(self, t0: int, nsteps: int, graph_steps: int = 32, stream=None) -> None:
         """Advance every plan by nsteps steps from t0 (as Plan.run_graph does for each)."""

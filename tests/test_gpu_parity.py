@@ -224,9 +224,9 @@ def test_graph_replay_matches_eager(torch_gpu):
     outs = []
     for use_graph in (False, True):
         plan = Plan(42, np.stack([p, p]))
-        plan.init(eta0, delta0, Phi0, U0, V0)
         s = torch.cuda.Stream()
         with torch.cuda.stream(s):
+            plan.init(eta0, delta0, Phi0, U0, V0, stream=s)  # initialization ordered before the steps
             if use_graph:
                 plan.run_graph(2, 40, graph_steps=8, stream=s)
             else:

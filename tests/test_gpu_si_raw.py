@@ -198,9 +198,9 @@ def _plan_run(precision, params, ics, nsteps, *, plain=False, graph=0, si=True, 
     if plain:
         plan.set_option(_capi.OPT_GRID_KERNEL, 1)
         plan.set_option(_capi.OPT_LEG_KERNEL, 1)
-    plan.init(*ics)
     s = torch.cuda.Stream()
     with torch.cuda.stream(s):
+        plan.init(*ics, stream=s)  # initialization ordered before the steps
         if graph:
             plan.run_graph(2, nsteps, graph_steps=graph, stream=s)
         else:
