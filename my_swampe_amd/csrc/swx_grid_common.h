@@ -69,6 +69,77 @@ __device__ __forceinline__ void grid_point_step(const GridArgs<float>& a, const 
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
+// A frozen member's rows j0 .. j0 + nrows - 1 (diagnostics on, member dead; nrows <= 16, 256
+// threads): the history receives its current level (eta, delta, Phi, U, V of the carry) and the
+// Phi extrema of its current Phi, the plain grid kernel's frozen path (model.py:847-864).
+__device__ inline void grid_frozen_rows(const GridArgs<float>& a, const double* phys, long long plane, int b, int j0,
+                                        int nrows) {
+  const int tid = threadIdx.x, I = a.I;
+  const int r = tid >> 4, part = tid & 15;
+  double mn = DBL_MAX, mx = -DBL_MAX;
+  if (r < nrows) {
+    const long long row = (long long)(j0 + r) * I;
+    const int src[5] = {1, 3, 5, 6, 7};
+    for (int i = part; i < I; i += 16) {
+      const double phi = phys[5 * plane + row + i];
+      mn = mn < phi ? mn : phi;
+      mx = mx > phi ? mx : phi;
+      if (a.hist_fields)
+        for (int f = 0; f < 5; ++f)
+          a.hist_fields[(((long long)b * 5 + f) * a.hist_T + a.hist_k) * plane + row + i] = phys[src[f] * plane + row + i];
+    }
+  }
+  for (int o = 1; o < 16; o <<= 1) {  // the 16 threads of a row are 16 consecutive lanes
+    const double omn = __shfl_xor(mn, o), omx = __shfl_xor(mx, o);
+    mn = mn < omn ? mn : omn;
+    mx = mx > omx ? mx : omx;
+  }
+  if (part == 0 && r < nrows && a.hist_phiext) {
+    double* h = a.hist_phiext + (((long long)b * a.hist_T + a.hist_k) * a.J + j0 + r) * 2;
+    h[0] = mn;
+    h[1] = mx;
+  }
+}
+
+// Row totals of the diagnostics partials of grid_bookkeeping4 (lane l holds row l % 16 of rows
+// j0 ..; 4 waves, 256 threads; `red`: 256 doubles of LDS, free): the RMS integrand and min |wind|
+// to part_uv (read by the next step's blow-up gate), the Phi extrema to the history.
+__device__ inline void grid_diag_rows(const GridArgs<float>& a, double (&diag)[4], int b, int j0, double* red) {
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, ln = lane & 15;
+  for (int o = 16; o < 64; o <<= 1) {
+    const double s0 = __shfl_xor(diag[0], o), s1 = __shfl_xor(diag[1], o);
+    const double s2 = __shfl_xor(diag[2], o), s3 = __shfl_xor(diag[3], o);
+    diag[0] += s0;
+    diag[1] = diag[1] < s1 ? diag[1] : s1;
+    diag[2] = diag[2] < s2 ? diag[2] : s2;
+    diag[3] = diag[3] > s3 ? diag[3] : s3;
+  }
+  if (lane < 16)
+    for (int k = 0; k < 4; ++k) red[(wave * 16 + ln) * 4 + k] = diag[k];
+  __syncthreads();
+  if (tid < 16) {
+    double v[4];
+    for (int k = 0; k < 4; ++k) v[k] = red[tid * 4 + k];
+    for (int w = 1; w < 4; ++w) {
+      const double* q = red + (w * 16 + tid) * 4;
+      v[0] += q[0];
+      v[1] = v[1] < q[1] ? v[1] : q[1];
+      v[2] = v[2] < q[2] ? v[2] : q[2];
+      v[3] = v[3] > q[3] ? v[3] : q[3];
+    }
+    const int j = j0 + tid;
+    if (a.diagnostics) {
+      a.part_uv[((long long)b * a.J + j) * 2 + 0] = v[0];
+      a.part_uv[((long long)b * a.J + j) * 2 + 1] = v[1];
+    }
+    if (a.hist_phiext) {
+      double* h = a.hist_phiext + (((long long)b * a.hist_T + a.hist_k) * a.J + j) * 2;
+      h[0] = v[2];
+      h[1] = v[3];
+    }
+  }
+}
+
 // The FP64 bookkeeping of the 4 consecutive longitudes i0 .. i0 + 3 (i0 % 4 == 0) of row j of
 // member b, before their products: the initial fields' residue (RAW), pinned winds (test 1),
 // the time filter of the physical prev level and the physical carry (when a.phys_on;
@@ -76,11 +147,18 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // 32-byte vectors, so the 4 lanes of a row cover whole 128-byte lines (a per-point access
 // pattern writes each line in 8-byte pieces).  The fields come in and go out as FP32
 // transform values.  Does nothing when none of the three applies.
+// DIAG: steps with history (a.hist_fields) also write the 5 fields of the step to it, and `diag`
+// (non-null with diagnostics or per-step Phi extrema) accumulates the row's partials as the plain grid kernel
+// forms them (time_stepping.py:537-576; model.py:838-845, :1112-1117): diag[0] += wrms ((U/cos)^2 +
+// (V/cos)^2) / aplanet, diag[1] = min |wind|, diag[2] / diag[3] = min / max of the filtered prev
+// Phi (the plain kernel's expressions and comparisons).
+template <bool DIAG = false>
 __device__ __forceinline__ void grid_bookkeeping4(const GridArgs<float>& a, const double* cf, double* phys,
                                                   long long plane, double alpha, int b, int j, int i0,
                                                   float (&eta)[4], float (&delta)[4], float (&Phi)[4],
-                                                  float (&U)[4], float (&V)[4]) {
-  if (!(a.raw_filter || a.test == 1 || a.phys_on)) return;
+                                                  float (&U)[4], float (&V)[4], double* diag = nullptr,
+                                                  double wrms = 0.0, double aplanet = 1.0, double cosphi = 1.0) {
+  if (!(a.raw_filter || a.test == 1 || a.phys_on || (DIAG && (a.hist_fields || diag)))) return;
   const long long pix = (long long)j * a.I + i0;
   v4d e, d, p, u, v;
 #pragma unroll
@@ -104,6 +182,7 @@ __device__ __forceinline__ void grid_bookkeeping4(const GridArgs<float>& a, cons
     u = *reinterpret_cast<const v4d*>(a.uvic + ((long long)b * 2 + 0) * plane + pix);
     v = *reinterpret_cast<const v4d*>(a.uvic + ((long long)b * 2 + 1) * plane + pix);
   }
+  v4d pmid = p;  // the filtered prev Phi of the diagnostics (the current level without the carry)
   if (a.phys_on) {
     double* ph = phys + pix;
     const v4d ep = *reinterpret_cast<const v4d*>(ph), ec = *reinterpret_cast<const v4d*>(ph + plane);
@@ -128,6 +207,27 @@ __device__ __forceinline__ void grid_bookkeeping4(const GridArgs<float>& a, cons
     *reinterpret_cast<v4d*>(ph + 5 * plane) = p;
     *reinterpret_cast<v4d*>(ph + 6 * plane) = u;
     *reinterpret_cast<v4d*>(ph + 7 * plane) = v;
+    pmid = pm;
+  }
+  if (DIAG && a.hist_fields) {
+    double* h = a.hist_fields + (long long)b * 5 * a.hist_T * plane + (long long)a.hist_k * plane + pix;
+    const long long fs = (long long)a.hist_T * plane;
+    *reinterpret_cast<v4d*>(h) = e;
+    *reinterpret_cast<v4d*>(h + fs) = d;
+    *reinterpret_cast<v4d*>(h + 2 * fs) = p;
+    *reinterpret_cast<v4d*>(h + 3 * fs) = u;
+    *reinterpret_cast<v4d*>(h + 4 * fs) = v;
+  }
+  if (DIAG && diag) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double uc = u[q] / cosphi, vc = v[q] / cosphi;
+      const double sp = sqrt(u[q] * u[q] + v[q] * v[q]);
+      diag[0] += wrms * (uc * uc + vc * vc) / aplanet;
+      diag[1] = diag[1] < sp ? diag[1] : sp;
+      diag[2] = diag[2] < pmid[q] ? diag[2] : pmid[q];
+      diag[3] = diag[3] > pmid[q] ? diag[3] : pmid[q];
+    }
   }
 #pragma unroll
   for (int q = 0; q < 4; ++q) {

@@ -249,6 +249,10 @@ struct GridArgs {
 
 template <typename R>
 hipError_t launch_leg(const LegArgs<R>& a, hipStream_t s);
+// The blow-up gate of a diagnostics step on its own (swx_kernels.hip:k_blowup_gate), for the
+// matrix-core Legendre kernel, which then skips the members it marks dead.
+template <typename R>
+hipError_t launch_blowup_gate(const LegArgs<R>& a, hipStream_t s);
 template <typename R>
 hipError_t launch_grid(const GridArgs<R>& a, hipStream_t s);
 // MFMA grid kernel (FP32, GRID_STEP without history/diagnostics).  DFT tables

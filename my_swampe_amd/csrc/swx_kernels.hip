@@ -250,6 +250,42 @@ __global__ void __launch_bounds__(128) k_leg(LegArgs<R> a) {
   }
 }
 
+// Blow-up gate of a diagnostics step ahead of the matrix-core Legendre kernel, one block per
+// member: the same reduction of the grid kernel's row partials as k_leg's (model.py:838-845,
+// :1192-1194), dead flag, per-step rms / spin_min and dead history.  The Legendre kernel then
+// leaves dead members' carry as it is.
+__global__ void __launch_bounds__(128) k_blowup_gate(const double* part_uv, unsigned char* dead, const double* coef,
+                                                     int ncoef, int J, double* hist_scal, unsigned char* hist_dead,
+                                                     int hist_T, int hist_k) {
+  __shared__ double sRed[2 * 128];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  double v[2] = {0.0, DBL_MAX};
+  const int op[2] = {0, 1};
+  for (int j = tid; j < J; j += blockDim.x) {
+    v[0] += part_uv[((long long)b * J + j) * 2 + 0];
+    v[1] = dmin(v[1], part_uv[((long long)b * J + j) * 2 + 1]);
+  }
+  block_reduce<2>(v, op, sRed);
+  if (tid == 0) {
+    const double rms = sqrt(v[0]);
+    const bool dead_next = (dead[b] != 0) || (rms > coef[(long long)b * ncoef + C_BLOWUP]);
+    dead[b] = dead_next ? 1 : 0;
+    if (hist_scal) {
+      double* h = hist_scal + ((long long)b * hist_T + hist_k) * 4;
+      h[0] = rms;
+      h[1] = v[1];
+    }
+    if (hist_dead) hist_dead[(long long)b * hist_T + hist_k] = dead_next ? 1 : 0;
+  }
+}
+
+template <typename R>
+hipError_t launch_blowup_gate(const LegArgs<R>& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_blowup_gate, dim3(a.B), dim3(128), 0, s, a.part_uv, a.dead, a.coef, a.ncoef, a.J, a.hist_scal,
+                     a.hist_dead, a.hist_T, a.hist_k);
+  return hipGetLastError();
+}
+
 // ----------------------------------------------------------------------------
 // Grid kernel
 // ----------------------------------------------------------------------------
@@ -729,6 +765,7 @@ template hipError_t launch_residue_init<double>(const double*, double*, double*,
 template hipError_t launch_residue_init<float>(const float*, float*, double*, const double*, const double*,
                                                const double*, const float*, double*, int, int, int, int,
                                                hipStream_t);
+template hipError_t launch_blowup_gate<float>(const LegArgs<float>&, hipStream_t);
 template hipError_t launch_leg<double>(const LegArgs<double>&, hipStream_t);
 template hipError_t launch_leg<float>(const LegArgs<float>&, hipStream_t);
 template hipError_t launch_grid<double>(const GridArgs<double>&, hipStream_t);

@@ -237,7 +237,8 @@ __device__ __forceinline__ void fold_gemm_prio() {
 // FAST: a step without residue, pinned winds, physical carry, semi-implicit or explicit slots
 // (the graph-replayed steps of the modified-Euler scheme): FP32-only point and slot code, and
 // the two-round s / d exchange in half the LDS.
-template <int MT, bool FAST = false>
+// DIAG (with FAST = false): per-step history and diagnostics (grid_bookkeeping4<true>).
+template <int MT, bool FAST = false, bool DIAG = false>
 __device__ __forceinline__ void grid_mfma_fold_body(const GridArgs<float>& a, const float* __restrict__ TinvF,
                                                     const float* __restrict__ TfwdF, const int blk, float* lds) {
   using S = MfmaFoldShape<MT>;
@@ -262,6 +263,15 @@ __device__ __forceinline__ void grid_mfma_fold_body(const GridArgs<float>& a, co
   const long long plane = (long long)J * I;
   double* phys = a.phys + (long long)b * 8 * plane;
   const double alpha = cf[C_ALPHA];
+  // Per-step diagnostics and history (the bookkeeping variant only; model.py:838-864, :1175-1189).
+  const bool want_diag = DIAG && (a.diagnostics || a.hist_phiext);
+  if (DIAG && a.diagnostics && a.dead[b]) {
+    // A frozen member (the whole workgroup: one member's 16 rows) emits its current level and
+    // keeps its Legendre inputs (model.py:847-864), as the plain kernel does.
+    grid_frozen_rows(a, phys, plane, b, j0, S::ROWS);
+    return;
+  }
+  double diag[4] = {0.0, DBL_MAX, DBL_MAX, -DBL_MAX};
 
   // Phase 0: Fourier rows -> X^T in the folded k order.  Slot s < K1 / 2 is the row pair
   // (2s, 2s + 1) = (re, im) of m = 2s (s < KE4 / 2) or m = 2 (s - KE4 / 2) + 1.
@@ -418,7 +428,9 @@ __device__ __forceinline__ void grid_mfma_fold_body(const GridArgs<float>& a, co
         }
 #pragma unroll
         for (int h = 0; h < 2; ++h)
-          grid_bookkeeping4(a, cf, phys, plane, alpha, b, j, ib + 4 * lg + h * H, fe[h], fd[h], fp[h], fu[h], fv[h]);
+          grid_bookkeeping4<DIAG>(a, cf, phys, plane, alpha, b, j, ib + 4 * lg + h * H, fe[h], fd[h], fp[h], fu[h],
+                                  fv[h], want_diag ? diag : nullptr, cf[cl.n_areac + j], cf[C_AREAPLANET],
+                                  geo[gl.cosphi + j]);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int il = 16 * wave + 4 * lg + q;
@@ -465,13 +477,17 @@ __device__ __forceinline__ void grid_mfma_fold_body(const GridArgs<float>& a, co
 
   // ---- Phase 3: assemble the Legendre inputs (swx_grid_common.h), folded row order.
   grid_assemble<Mp, NT2, TE, true, FAST>(a, acc, sC, b, j0, cf, cl, geo, gl, uv);
+  if (DIAG && want_diag) {
+    __syncthreads();
+    grid_diag_rows(a, diag, b, j0, reinterpret_cast<double*>(lds));
+  }
 }
 
-template <int MT, bool FAST>
+template <int MT, bool FAST, bool DIAG = false>
 __global__ void __launch_bounds__(256) k_grid_mfma_fold(GridArgs<float> a, const float* __restrict__ TinvF,
                                                         const float* __restrict__ TfwdF) {
   extern __shared__ __align__(16) float lds[];
-  grid_mfma_fold_body<MT, FAST>(a, TinvF, TfwdF, blockIdx.x, lds);
+  grid_mfma_fold_body<MT, FAST, DIAG>(a, TinvF, TfwdF, blockIdx.x, lds);
 }
 
 // T42: the same kernel allocated for at least 2 waves per SIMD.  The compiler then keeps the
@@ -480,11 +496,11 @@ __global__ void __launch_bounds__(256) k_grid_mfma_fold(GridArgs<float> a, const
 // stepped beside another plan (swx_pipeline) keeps the fast variant at 3 workgroups per CU:
 // measured at T42 x 2 x 128 members, the two plans together run 4 % faster that way, while
 // one plan alone runs 4 % faster at 4.
-template <int MT, bool FAST>
+template <int MT, bool FAST, bool DIAG = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
 k_grid_mfma_fold_w2(GridArgs<float> a, const float* __restrict__ TinvF, const float* __restrict__ TfwdF) {
   extern __shared__ __align__(16) float lds[];
-  grid_mfma_fold_body<MT, FAST>(a, TinvF, TfwdF, blockIdx.x, lds);
+  grid_mfma_fold_body<MT, FAST, DIAG>(a, TinvF, TfwdF, blockIdx.x, lds);
 }
 
 template <int MT>
@@ -535,7 +551,16 @@ static hipError_t launch_mfma_fold_t(const GridArgs<float>& a, const float* Tinv
   using S = MfmaFoldShape<MT>;
   dim3 grid((unsigned)((long long)a.B * a.J / S::ROWS));
   const bool fast = grid_fold_fast(a);
+  const bool diag = a.diagnostics || a.hist_fields || a.hist_phiext;
   const size_t smem = sizeof(float) * (fast ? S::LDS_FAST : S::LDS_FLOATS);
+  if (diag) {
+    // Steps with per-step outputs have the physical carry on (never the fast variant).
+    if constexpr (MT == 42)
+      hipLaunchKernelGGL((k_grid_mfma_fold_w2<MT, false, true>), grid, dim3(256), smem, s, a, TinvF, TfwdF);
+    else
+      hipLaunchKernelGGL((k_grid_mfma_fold<MT, false, true>), grid, dim3(256), smem, s, a, TinvF, TfwdF);
+    return hipGetLastError();
+  }
   if constexpr (MT == 42) {
     if (fast && shared)
       hipLaunchKernelGGL((k_grid_mfma_fold<MT, true>), grid, dim3(256), smem, s, a, TinvF, TfwdF);
@@ -932,6 +957,7 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
     const int mb = p / K, np = p % K;
     const int b = b0 + mb;
     if (b >= B) continue;
+    if (a.diagnostics && a.dead[b]) continue;  // frozen by the blow-up gate: carry kept as it is
     const int n = m + np;
     const double* cf = a.coef + (long long)b * a.ncoef;
     // Row of degree np in the forward results and in the inverse inputs.
@@ -1057,6 +1083,7 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
     const int mb = p / J, j = p % J;
     const int b = b0 + mb;
     if (b >= B) continue;
+    if (a.diagnostics && a.dead[b]) continue;
     float* G = a.G + fidx(b, 0, m, j, Mp, J);
     const long long fs = (long long)Mp * J * 2;
     float P[10], H[4];

@@ -333,17 +333,20 @@ GridArgs<R> grid_args(swx_plan* p, int mode) {
   return a;
 }
 
-// The MFMA grid kernel covers the forward stepping of FP32 plans without
-// per-step history or diagnostics; everything else uses the plain kernel.
+// The MFMA grid kernels cover the forward stepping of FP32 plans; steps with per-step history or
+// diagnostics need the folded FP32 kernel (its FP64 bookkeeping variant writes them); everything
+// else uses the plain kernel.
 template <typename R>
 bool grid_uses_mfma(const swx_plan* p, const GridArgs<R>& g) {
-  return sizeof(R) == 4 && p->use_mfma && !p->opt_plain_grid && g.mode == GRID_STEP && !g.diagnostics &&
-         !g.hist_fields && !g.hist_phiext && !(g.test == 1 && g.expflag);
+  const bool per_step = g.diagnostics || g.hist_fields || g.hist_phiext;
+  const bool fold = p->has_fold && p->grid_fold == 2 && !p->use_bf16x3;
+  return sizeof(R) == 4 && p->use_mfma && !p->opt_plain_grid && g.mode == GRID_STEP && !(g.test == 1 && g.expflag) &&
+         (!per_step || fold);
 }
 
 template <typename R>
 bool leg_uses_mfma(const swx_plan* p, const LegArgs<R>& l) {
-  return sizeof(R) == 4 && p->use_leg_mfma && !p->opt_plain_leg && l.mode == LEG_STEP && !l.diagnostics;
+  return sizeof(R) == 4 && p->use_leg_mfma && !p->opt_plain_leg && l.mode == LEG_STEP;
 }
 
 template <typename R>
@@ -361,7 +364,13 @@ hipError_t launch_grid_any(swx_plan* p, const GridArgs<R>& g, hipStream_t s) {
 template <typename R>
 hipError_t launch_leg_any(swx_plan* p, const LegArgs<R>& l, hipStream_t s) {
   if constexpr (sizeof(R) == 4) {
-    if (leg_uses_mfma(p, l)) return launch_leg_mfma(l, p->leg_mb, s, p->in_pipeline);
+    if (leg_uses_mfma(p, l)) {
+      if (l.diagnostics) {
+        hipError_t e = launch_blowup_gate<R>(l, s);
+        if (e != hipSuccess) return e;
+      }
+      return launch_leg_mfma(l, p->leg_mb, s, p->in_pipeline);
+    }
   }
   return launch_leg<R>(l, s);
 }
