@@ -236,7 +236,11 @@ __global__ void __launch_bounds__(256) k_grid_adj_mfma(GridAdjArgs<float> a, con
 // s = y(i) + y(i + H) for the even-m output tiles and d = y(i) - y(i + H) for the odd ones.
 // Tables and row orders: those of k_grid_mfma_fold (swx_kernels.h FoldDims).  At T42 (one
 // chunk) s / d reuse the X^T space, which keeps two workgroups per CU.
-template <int MT>
+// NP = 2 (T106, whose 12-field staging exceeds the LDS): the pointwise adjoint and both transposed
+// transforms are linear in the 8 product cotangents, so they run in two passes of 4 cotangent
+// fields each over the same 4 fields of S_t, the GEMM-2 accumulators and the parameter partials
+// summing the passes (8 staged fields instead of 12).
+template <int MT, int NP = 1>
 struct AdjFoldShape {
   static constexpr int Mp = MT + 1;
   static constexpr int I = MT == 42 ? 128 : (MT == 63 ? 192 : 320);
@@ -249,19 +253,20 @@ struct AdjFoldShape {
   static constexpr int NT2 = TE + TO;
   static constexpr int ICH = 64;
   static constexpr int NCH = (H + ICH - 1) / ICH;
-  static constexpr int NX = 12;
+  static constexpr int NPB = 8 / NP;     // product cotangents per pass
+  static constexpr int NX = 4 + NPB;     // staged fields: eta, Phi, U, V of S_t, then this pass's cotangents
   static constexpr int XT_FLOATS = NX * K1 * 16;
   static constexpr int SD_FLOATS = 2 * 4 * ICH * 16;
   static constexpr int MAIN_FLOATS = NCH == 1 ? (XT_FLOATS > SD_FLOATS ? XT_FLOATS : SD_FLOATS) : XT_FLOATS + SD_FLOATS;
   static constexpr int LDS_FLOATS = MAIN_FLOATS + 2 * 4 * 16 * 3;  // + row partials [4][16][3] doubles
 };
 
-template <int MT>
+template <int MT, int NP = 1>
 __device__ __forceinline__ void grid_adj_mfma_fold_body(const GridAdjArgs<float>& a, const float* __restrict__ TinvF,
                                                         const float* __restrict__ TfwdF) {
-  using S = AdjFoldShape<MT>;
+  using S = AdjFoldShape<MT, NP>;
   constexpr int Mp = S::Mp, I = S::I, H = S::H, K1 = S::K1, KE4 = S::KE4, KQ1 = S::KQ1, NT2 = S::NT2, TE = S::TE;
-  constexpr int ICH = S::ICH, NX = S::NX;
+  constexpr int ICH = S::ICH, NX = S::NX, NPB = S::NPB;
   extern __shared__ __align__(16) float lds[];
   float* sXT = lds;                                                       // [NX][K1][16]
   float* sS = lds + (S::NCH == 1 ? 0 : S::XT_FLOATS);                      // [4][ICH][16] s
@@ -305,7 +310,19 @@ __device__ __forceinline__ void grid_adj_mfma_fold_body(const GridAdjArgs<float>
       }
     }
   }
-  // Phase 0b: Fourier cotangents per (m, row) -> X^T fields 4..11, scaled by 1 / (I w_m).
+  v4f acc[NT2];
+#pragma unroll
+  for (int t = 0; t < NT2; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+  double dpar[3] = {0.0, 0.0, 0.0};
+  const int j = j0 + ln;
+  const v4f* Tf = reinterpret_cast<const v4f*>(TfwdF) + lane;
+
+  static_assert(NP == 1 || S::NCH > 1, "s / d may overlay X^T only with one pass");
+#pragma unroll
+  for (int pass = 0; pass < NP; ++pass) {
+  const int pb0 = pass * NPB;  // this pass's product cotangents: pb0 .. pb0 + NPB - 1
+  if (pass > 0) __syncthreads();  // the previous pass's last reads of X^T are done
+  // Phase 0b: Fourier cotangents per (m, row) -> X^T fields 4.., scaled by 1 / (I w_m).
   for (int p = tid; p < (K1 / 2) * 16; p += 256) {
     const int sl = p >> 4, r = p & 15;
     const int m = slot_m(sl);
@@ -323,13 +340,13 @@ __device__ __forceinline__ void grid_adj_mfma_fold_body(const GridAdjArgs<float>
       assemble_slots_adjoint(cf, cf[cl.n_tc + jj], geo[gl.w + jj], m, a.forcflag, tau, fb);
       const double sc = 1.0 / ((double)I * (m == 0 ? 1.0 : 2.0));
 #pragma unroll
-      for (int f = 0; f < 8; ++f) {
-        xr[(f * K1) * 16] = (float)(fb[f][0] * sc);
-        xr[(f * K1 + 1) * 16] = (float)(fb[f][1] * sc);
+      for (int f = 0; f < NPB; ++f) {
+        xr[(f * K1) * 16] = (float)(fb[pb0 + f][0] * sc);
+        xr[(f * K1 + 1) * 16] = (float)(fb[pb0 + f][1] * sc);
       }
     } else {
 #pragma unroll
-      for (int f = 0; f < 8; ++f) {
+      for (int f = 0; f < NPB; ++f) {
         xr[(f * K1) * 16] = 0.0f;
         xr[(f * K1 + 1) * 16] = 0.0f;
       }
@@ -337,18 +354,11 @@ __device__ __forceinline__ void grid_adj_mfma_fold_body(const GridAdjArgs<float>
   }
   __syncthreads();
 
-  v4f acc[NT2];
-#pragma unroll
-  for (int t = 0; t < NT2; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
-  double dpar[3] = {0.0, 0.0, 0.0};
-  const int j = j0 + ln;
-  const v4f* Tf = reinterpret_cast<const v4f*>(TfwdF) + lane;
-
   for (int ic = 0; ic < H; ic += ICH) {
     const int ntc = (H - ic) / 16 < ICH / 16 ? (H - ic) / 16 : ICH / 16;
     const bool active = wave < ntc;
     const int ib = ic + 16 * wave;
-    // ---- Phase 1: even / odd sums of the 12 fields on this wave's tile.
+    // ---- Phase 1: even / odd sums of the NX staged fields on this wave's tile.
     v4f cE[NX], cO[NX];
 #pragma unroll
     for (int f = 0; f < NX; ++f) cE[f] = cO[f] = v4f{0.f, 0.f, 0.f, 0.f};
@@ -391,7 +401,8 @@ __device__ __forceinline__ void grid_adj_mfma_fold_body(const GridAdjArgs<float>
           }
           double pbar[8];
 #pragma unroll
-          for (int f = 0; f < 8; ++f) pbar[f] = cE[4 + f][q] + sg * cO[4 + f][q];
+          for (int f = 0; f < 8; ++f)
+            pbar[f] = (f >= pb0 && f < pb0 + NPB) ? cE[4 + f - pb0][q] + sg * cO[4 + f - pb0][q] : 0.0;
           grid_products_adjoint(cf, geo, gl, j, i, a.test, cE[0][q] + sg * cO[0][q], cE[1][q] + sg * cO[1][q], U,
                                 V, pbar, o2[h], dpar);
         }
@@ -428,6 +439,7 @@ __device__ __forceinline__ void grid_adj_mfma_fold_body(const GridAdjArgs<float>
     }
     __syncthreads();
   }
+  }  // passes
 
   // Per-row parameter partials: row ln is held by lanes ln + 16 g of every wave.
 #pragma unroll
@@ -469,10 +481,10 @@ __device__ __forceinline__ void grid_adj_mfma_fold_body(const GridAdjArgs<float>
   }
 }
 
-template <int MT>
+template <int MT, int NP = 1>
 __global__ void __launch_bounds__(256) k_grid_adj_mfma_fold(GridAdjArgs<float> a, const float* __restrict__ TinvF,
                                                             const float* __restrict__ TfwdF) {
-  grid_adj_mfma_fold_body<MT>(a, TinvF, TfwdF);
+  grid_adj_mfma_fold_body<MT, NP>(a, TinvF, TfwdF);
 }
 
 // T42: allocated for 2+ waves per SIMD (accumulators in VGPRs): 214 registers instead of 318,
@@ -483,15 +495,21 @@ k_grid_adj_mfma_fold_w2(GridAdjArgs<float> a, const float* __restrict__ TinvF, c
   grid_adj_mfma_fold_body<MT>(a, TinvF, TfwdF);
 }
 
+// Passes of the folded grid adjoint per truncation (two at T106, whose 12-field staging would
+// not fit the LDS).
+template <int MT>
+constexpr int adj_fold_passes() { return MT == 106 ? 2 : 1; }
+
 template <int MT>
 static hipError_t launch_adj_mfma_fold_t(const GridAdjArgs<float>& a, const float* TinvH, const float* TfwdH,
                                          hipStream_t s) {
-  const size_t smem = sizeof(float) * AdjFoldShape<MT>::LDS_FLOATS;
+  constexpr int NP = adj_fold_passes<MT>();
+  const size_t smem = sizeof(float) * AdjFoldShape<MT, NP>::LDS_FLOATS;
   const dim3 grid((unsigned)((long long)a.B * a.J / 16));
   if constexpr (MT == 42)
     hipLaunchKernelGGL(k_grid_adj_mfma_fold_w2<MT>, grid, dim3(256), smem, s, a, TinvH, TfwdH);
   else
-    hipLaunchKernelGGL(k_grid_adj_mfma_fold<MT>, grid, dim3(256), smem, s, a, TinvH, TfwdH);
+    hipLaunchKernelGGL((k_grid_adj_mfma_fold<MT, NP>), grid, dim3(256), smem, s, a, TinvH, TfwdH);
   return hipGetLastError();
 }
 
@@ -499,7 +517,8 @@ size_t grid_adj_mfma_fold_lds_bytes(int M) {
   switch (M) {
     case 42: return sizeof(float) * AdjFoldShape<42>::LDS_FLOATS;
     case 63: return sizeof(float) * AdjFoldShape<63>::LDS_FLOATS;
-    default: return 0;  // T106: the 12-field staging exceeds the workgroup's LDS
+    case 106: return sizeof(float) * AdjFoldShape<106, adj_fold_passes<106>()>::LDS_FLOATS;
+    default: return 0;
   }
 }
 
@@ -508,6 +527,7 @@ hipError_t launch_grid_adj_mfma_fold(const GridAdjArgs<float>& a, const float* T
   switch (a.M) {
     case 42: return launch_adj_mfma_fold_t<42>(a, TinvH, TfwdH, s);
     case 63: return launch_adj_mfma_fold_t<63>(a, TinvH, TfwdH, s);
+    case 106: return launch_adj_mfma_fold_t<106>(a, TinvH, TfwdH, s);
     default: return hipErrorInvalidValue;
   }
 }

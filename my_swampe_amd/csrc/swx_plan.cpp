@@ -98,7 +98,7 @@ struct swx_plan {
   bool use_bf16x3 = false;     // split-BF16x3 grid kernel (precision SWX_PREC_BF16X3)
   bool use_adj_mfma = false;   // MFMA grid adjoint kernel (FP32 transforms, where its LDS fits)
   bool use_leg_adj_mfma = false;  // MFMA Legendre adjoint kernels (FP32 transforms)
-  bool use_adj_fold = false;      // folded MFMA grid adjoint available (its LDS fits)
+  bool use_adj_fold = false;      // folded MFMA grid adjoint available (its LDS fits; two passes at T106)
   void* TinvB = nullptr;       // its DFT tables: bf16 hi / lo pairs in fragment order
   void* TfwdB = nullptr;
   bool opt_plain_grid = false;  // SWX_OPT_GRID_KERNEL = 1
@@ -682,12 +682,12 @@ int adjoint_step(swx_plan* p, const double* St, hipStream_t s) {
   SWX_HIP(launch_leg_adj_any<R>(p, a, s));
   const GridAdjArgs<R> g = grid_adj_args<R>(p, GADJ_STEP);
   if constexpr (sizeof(R) == 4) {
-    if (p->use_adj_mfma && !p->opt_plain_grid) {
-      // The grid variant of the forward (SWX_OPT_GRID_FOLD) applies to the reverse step too.
-      if (p->use_adj_fold && p->grid_fold == 2)
-        SWX_HIP(launch_grid_adj_mfma_fold(g, p->TinvH, p->TfwdH, s));
-      else
-        SWX_HIP(launch_grid_adj_mfma(g, p->TinvF, p->TfwdF, s));
+    // The grid variant of the forward (SWX_OPT_GRID_FOLD) applies to the reverse step too; at T106
+    // only the folded (two-pass) kernel fits the LDS.
+    if (!p->opt_plain_grid && p->use_adj_fold && p->grid_fold == 2) {
+      SWX_HIP(launch_grid_adj_mfma_fold(g, p->TinvH, p->TfwdH, s));
+    } else if (!p->opt_plain_grid && p->use_adj_mfma) {
+      SWX_HIP(launch_grid_adj_mfma(g, p->TinvF, p->TfwdF, s));
     } else {
       SWX_HIP(launch_grid_adj<R>(g, s));
     }

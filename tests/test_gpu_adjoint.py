@@ -163,13 +163,14 @@ def test_checkpoint_interval_does_not_change_gradients(grads):
     assert grads["stats"]["checkpoints"] == 6 and grads["stats"]["segment_states"] == 1
 
 
-@pytest.mark.parametrize("kernel,M", [("grid", 42), ("grid", 63), ("grid_full", 42), ("grid_full", 63), ("leg", 42),
-                                      ("leg", 63), ("leg", 106)])
+@pytest.mark.parametrize("kernel,M", [("grid", 42), ("grid", 63), ("grid", 106), ("grid_full", 42), ("grid_full", 63),
+                                      ("leg", 42), ("leg", 63), ("leg", 106)])
 def test_mfma_adjoint_matches_plain_fp32(kernel, M, torch_gpu):
     """The matrix-core grid (or Legendre) adjoint kernels vs the plain FP32 ones, selected with
     SWX_OPT_GRID_KERNEL (or SWX_OPT_LEG_KERNEL, which switches the forward and reverse Legendre
     kernels together) on the same reverse sweep; "grid" runs the half-period folded grid kernels,
-    "grid_full" the full-DFT ones (SWX_OPT_GRID_FOLD): 3 members with different forcing, 4 steps,
+    "grid_full" the full-DFT ones (SWX_OPT_GRID_FOLD; at T106 only the folded two-pass grid adjoint
+    fits the LDS): 3 members with different forcing, 4 steps,
     random terminal Phi cotangent.  Relative L2 of dL/dS_0 and of the parameter gradients
     <= 1e-5 (FP32 transforms in both, different summation orders)."""
     from my_swampe_amd import _capi
