@@ -49,6 +49,7 @@ struct GridAdjArgs {
   const double* g_Phi;
   const double* g_U;
   const double* g_V;
+  void* stc;            // folded MFMA adjoint at T106: cache of S_t's even / odd sums (FP32)
 };
 
 template <typename R>
@@ -60,9 +61,11 @@ hipError_t launch_pgrad_reduce(const double* pg, double* out, int B, int J, hipS
 // grid_adj_mfma_lds_bytes is 0 where it is not provided (T106).
 hipError_t launch_grid_adj_mfma(const GridAdjArgs<float>& a, const float* TinvF, const float* TfwdF, hipStream_t s);
 size_t grid_adj_mfma_lds_bytes(int M);
-// Its half-period folded variant (the tables of the folded forward grid kernel).
+// Its half-period folded variant (the tables of the folded forward grid kernel).  At T106 it needs
+// GridAdjArgs::stc of grid_adj_mfma_fold_cache_bytes (0 elsewhere).
 hipError_t launch_grid_adj_mfma_fold(const GridAdjArgs<float>& a, const float* TinvH, const float* TfwdH,
                                      hipStream_t s);
+size_t grid_adj_mfma_fold_cache_bytes(int M, int B, int J);
 size_t grid_adj_mfma_fold_lds_bytes(int M);
 // Matrix-core Legendre adjoint (all modes, FP32; swx_adjoint_leg_mfma.hip) and its largest
 // per-workgroup LDS.

@@ -236,11 +236,11 @@ __global__ void __launch_bounds__(256) k_grid_adj_mfma(GridAdjArgs<float> a, con
 // s = y(i) + y(i + H) for the even-m output tiles and d = y(i) - y(i + H) for the odd ones.
 // Tables and row orders: those of k_grid_mfma_fold (swx_kernels.h FoldDims).  At T42 (one
 // chunk) s / d reuse the X^T space, which keeps two workgroups per CU.
-// NP = 2 (T106, whose 12-field staging exceeds the LDS): the pointwise adjoint and both transposed
-// transforms are linear in the 8 product cotangents, so they run in two passes of 4 cotangent
-// fields each over the same 4 fields of S_t, the GEMM-2 accumulators and the parameter partials
-// summing the passes (8 staged fields instead of 12).
-template <int MT, int NP = 1>
+// SC (T106, whose 12-field staging exceeds the LDS): GEMM 1 of the 4 fields of S_t runs first, on
+// their own staging, and the lane's even / odd sums go to a per-chunk cache in device memory
+// (GridAdjArgs::stc); the 8 cotangent fields then take the whole X^T space (8 staged fields
+// instead of 12) and the main loop reads S_t back from the cache, each lane its own values.
+template <int MT, bool SC = false>
 struct AdjFoldShape {
   static constexpr int Mp = MT + 1;
   static constexpr int I = MT == 42 ? 128 : (MT == 63 ? 192 : 320);
@@ -253,23 +253,51 @@ struct AdjFoldShape {
   static constexpr int NT2 = TE + TO;
   static constexpr int ICH = 64;
   static constexpr int NCH = (H + ICH - 1) / ICH;
-  static constexpr int NPB = 8 / NP;     // product cotangents per pass
-  static constexpr int NX = 4 + NPB;     // staged fields: eta, Phi, U, V of S_t, then this pass's cotangents
+  static constexpr int PB = SC ? 0 : 4;   // X^T field of the first product cotangent
+  static constexpr int NX = PB + 8;       // staged fields in the main loop
   static constexpr int XT_FLOATS = NX * K1 * 16;
   static constexpr int SD_FLOATS = 2 * 4 * ICH * 16;
   static constexpr int MAIN_FLOATS = NCH == 1 ? (XT_FLOATS > SD_FLOATS ? XT_FLOATS : SD_FLOATS) : XT_FLOATS + SD_FLOATS;
   static constexpr int LDS_FLOATS = MAIN_FLOATS + 2 * 4 * 16 * 3;  // + row partials [4][16][3] doubles
+  // S_t cache: per (workgroup, chunk, wave) 8 float4 (E, O of 4 fields) x 64 lanes.
+  static constexpr long long cache_floats(long long nblk) { return SC ? nblk * NCH * 4 * 8 * 64 * 4 : 0; }
 };
 
-template <int MT, int NP = 1>
+// GEMM 1 on one wave's 16-point tile: even / odd wavenumber sums of NF staged X^T fields
+// (field stride K1 x 16 floats from xt).
+template <int K1, int KE4, int KQ1, int NF>
+__device__ __forceinline__ void adj_fold_gemm1(const float* __restrict__ TinvF, const float* xt, int ib, int lane,
+                                               v4f (&cE)[NF], v4f (&cO)[NF]) {
+  const int ln = lane & 15, lg = lane >> 4;
+#pragma unroll
+  for (int f = 0; f < NF; ++f) cE[f] = cO[f] = v4f{0.f, 0.f, 0.f, 0.f};
+  v4f afr[KQ1 / 4];
+  const v4f* Ta = reinterpret_cast<const v4f*>(TinvF + ((long long)(ib / 16) * 64 + lane) * KQ1);
+#pragma unroll
+  for (int q4 = 0; q4 < KQ1 / 4; ++q4) afr[q4] = Ta[q4];
+#pragma unroll
+  for (int ks = 0; ks < K1 / 4; ++ks) {
+    const float* xb = xt + (4 * ks + lg) * 16 + ln;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      if (ks < KE4 / 4)
+        cE[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(afr[ks / 4][ks % 4], xb[f * K1 * 16], cE[f], 0, 0, 0);
+      else
+        cO[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(afr[ks / 4][ks % 4], xb[f * K1 * 16], cO[f], 0, 0, 0);
+    }
+  }
+}
+
+template <int MT, bool SC = false>
 __device__ __forceinline__ void grid_adj_mfma_fold_body(const GridAdjArgs<float>& a, const float* __restrict__ TinvF,
                                                         const float* __restrict__ TfwdF) {
-  using S = AdjFoldShape<MT, NP>;
+  using S = AdjFoldShape<MT, SC>;
   constexpr int Mp = S::Mp, I = S::I, H = S::H, K1 = S::K1, KE4 = S::KE4, KQ1 = S::KQ1, NT2 = S::NT2, TE = S::TE;
-  constexpr int ICH = S::ICH, NX = S::NX, NPB = S::NPB;
+  constexpr int ICH = S::ICH, NX = S::NX, PB = S::PB, NCH = S::NCH;
+  static_assert(!SC || NCH > 1, "the cached variant keeps s / d apart from X^T");
   extern __shared__ __align__(16) float lds[];
   float* sXT = lds;                                                       // [NX][K1][16]
-  float* sS = lds + (S::NCH == 1 ? 0 : S::XT_FLOATS);                      // [4][ICH][16] s
+  float* sS = lds + (NCH == 1 ? 0 : S::XT_FLOATS);                        // [4][ICH][16] s
   float* sD = sS + 4 * ICH * 16;                                          // [4][ICH][16] d
   double* sRed = reinterpret_cast<double*>(lds + S::MAIN_FLOATS);         // [4][16][3]
 
@@ -283,6 +311,10 @@ __device__ __forceinline__ void grid_adj_mfma_fold_body(const GridAdjArgs<float>
   const double* geo = a.geo;
   const long long plane = (long long)J * I;
   auto slot_m = [](int sl) { return sl < KE4 / 2 ? 2 * sl : 2 * (sl - KE4 / 2) + 1; };
+  // This wave's S_t cache entries of chunk c (SC): 8 float4 at stride 64.
+  auto cache_at = [&](int c) {
+    return reinterpret_cast<v4f*>(a.stc) + (((long long)blockIdx.x * NCH + c) * 4 + wave) * 8 * 64 + lane;
+  };
 
   // Phase 0a: Fourier rows of S_t (G fields 0, 2, 3, 4) -> X^T fields 0..3, folded k order.
   {
@@ -310,6 +342,79 @@ __device__ __forceinline__ void grid_adj_mfma_fold_body(const GridAdjArgs<float>
       }
     }
   }
+  // The slot cotangents of the thread's phase-0b items, in flight during phase S.
+  constexpr int NP0 = (K1 / 2) * 16;
+  constexpr int NIT = (NP0 + 255) / 256;
+  float tv[NIT][kNT][2];
+#pragma unroll
+  for (int u = 0; u < NIT; ++u) {
+    const int p = tid + 256 * u;
+    const int sl = (p < NP0 ? p : 0) >> 4, r = p & 15;
+    const int m = slot_m(sl);
+    const float* Tb = a.Tbar + (((long long)b * Mp + (m < Mp ? m : Mp - 1)) * kNT) * J * 2;
+#pragma unroll
+    for (int k = 0; k < kNT; ++k) {
+      tv[u][k][0] = Tb[(k * J + j0 + r) * 2];
+      tv[u][k][1] = Tb[(k * J + j0 + r) * 2 + 1];
+    }
+  }
+  if constexpr (SC) {
+    // Phase S: GEMM 1 of the 4 fields of S_t per chunk into the cache.
+    __syncthreads();
+#pragma unroll 1
+    for (int ic = 0; ic < H; ic += ICH) {
+      const int ntc = (H - ic) / 16 < ICH / 16 ? (H - ic) / 16 : ICH / 16;
+      if (wave < ntc) {
+        v4f sE[4], sO[4];
+        adj_fold_gemm1<K1, KE4, KQ1, 4>(TinvF, sXT, ic + 16 * wave, lane, sE, sO);
+        v4f* dst = cache_at(ic / ICH);
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+          dst[(2 * f) * 64] = sE[f];
+          dst[(2 * f + 1) * 64] = sO[f];
+        }
+      }
+    }
+    __syncthreads();  // the cotangents overwrite X^T fields 0..3 next
+  }
+
+  // Phase 0b: Fourier cotangents per (m, row) -> X^T fields PB.., scaled by 1 / (I w_m); the slot
+  // cotangents of all the thread's items are loaded first.
+  {
+#pragma unroll
+    for (int u = 0; u < NIT; ++u) {
+      const int p = tid + 256 * u;
+      if (p >= NP0) continue;
+      const int sl = p >> 4, r = p & 15;
+      const int m = slot_m(sl);
+      float* xr = sXT + (PB * K1 + 2 * sl) * 16 + r;
+      if (m < Mp) {
+        const int jj = j0 + r;
+        double tau[kNT][2];
+#pragma unroll
+        for (int k = 0; k < kNT; ++k) {
+          tau[k][0] = tv[u][k][0];
+          tau[k][1] = tv[u][k][1];
+        }
+        double fb[8][2];
+        assemble_slots_adjoint(cf, cf[cl.n_tc + jj], geo[gl.w + jj], m, a.forcflag, tau, fb);
+        const double sc = 1.0 / ((double)I * (m == 0 ? 1.0 : 2.0));
+#pragma unroll
+        for (int f = 0; f < 8; ++f) {
+          xr[(f * K1) * 16] = (float)(fb[f][0] * sc);
+          xr[(f * K1 + 1) * 16] = (float)(fb[f][1] * sc);
+        }
+      } else {
+#pragma unroll
+        for (int f = 0; f < 8; ++f) {
+          xr[(f * K1) * 16] = 0.0f;
+          xr[(f * K1 + 1) * 16] = 0.0f;
+        }
+      }
+    }
+  }
+  __syncthreads();
+
   v4f acc[NT2];
 #pragma unroll
   for (int t = 0; t < NT2; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
@@ -317,72 +422,31 @@ __device__ __forceinline__ void grid_adj_mfma_fold_body(const GridAdjArgs<float>
   const int j = j0 + ln;
   const v4f* Tf = reinterpret_cast<const v4f*>(TfwdF) + lane;
 
-  static_assert(NP == 1 || S::NCH > 1, "s / d may overlay X^T only with one pass");
-#pragma unroll
-  for (int pass = 0; pass < NP; ++pass) {
-  const int pb0 = pass * NPB;  // this pass's product cotangents: pb0 .. pb0 + NPB - 1
-  if (pass > 0) __syncthreads();  // the previous pass's last reads of X^T are done
-  // Phase 0b: Fourier cotangents per (m, row) -> X^T fields 4.., scaled by 1 / (I w_m).
-  for (int p = tid; p < (K1 / 2) * 16; p += 256) {
-    const int sl = p >> 4, r = p & 15;
-    const int m = slot_m(sl);
-    float* xr = sXT + (4 * K1 + 2 * sl) * 16 + r;
-    if (m < Mp) {
-      const int jj = j0 + r;
-      const float* Tb = a.Tbar + (((long long)b * Mp + m) * kNT) * J * 2;
-      double tau[kNT][2];
-#pragma unroll
-      for (int k = 0; k < kNT; ++k) {
-        tau[k][0] = Tb[(k * J + jj) * 2];
-        tau[k][1] = Tb[(k * J + jj) * 2 + 1];
-      }
-      double fb[8][2];
-      assemble_slots_adjoint(cf, cf[cl.n_tc + jj], geo[gl.w + jj], m, a.forcflag, tau, fb);
-      const double sc = 1.0 / ((double)I * (m == 0 ? 1.0 : 2.0));
-#pragma unroll
-      for (int f = 0; f < NPB; ++f) {
-        xr[(f * K1) * 16] = (float)(fb[pb0 + f][0] * sc);
-        xr[(f * K1 + 1) * 16] = (float)(fb[pb0 + f][1] * sc);
-      }
-    } else {
-#pragma unroll
-      for (int f = 0; f < NPB; ++f) {
-        xr[(f * K1) * 16] = 0.0f;
-        xr[(f * K1 + 1) * 16] = 0.0f;
-      }
-    }
-  }
-  __syncthreads();
-
   for (int ic = 0; ic < H; ic += ICH) {
     const int ntc = (H - ic) / 16 < ICH / 16 ? (H - ic) / 16 : ICH / 16;
     const bool active = wave < ntc;
     const int ib = ic + 16 * wave;
-    // ---- Phase 1: even / odd sums of the NX staged fields on this wave's tile.
-    v4f cE[NX], cO[NX];
+    // ---- Phase 1: even / odd sums of the NX staged fields on this wave's tile (SC: S_t from
+    // the cache, loaded ahead of the MFMA chain).
+    v4f st[SC ? 8 : 1];
+    if constexpr (SC) {
+      if (active) {
+        const v4f* src = cache_at(ic / ICH);
 #pragma unroll
-    for (int f = 0; f < NX; ++f) cE[f] = cO[f] = v4f{0.f, 0.f, 0.f, 0.f};
-    if (active) {
-      v4f afr[KQ1 / 4];
-      const v4f* Ta = reinterpret_cast<const v4f*>(TinvF + ((long long)(ib / 16) * 64 + lane) * KQ1);
-#pragma unroll
-      for (int q4 = 0; q4 < KQ1 / 4; ++q4) afr[q4] = Ta[q4];
-#pragma unroll
-      for (int ks = 0; ks < K1 / 4; ++ks) {
-        const float* xb = sXT + (4 * ks + lg) * 16 + ln;
-#pragma unroll
-        for (int f = 0; f < NX; ++f) {
-          if (ks < KE4 / 4)
-            cE[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(afr[ks / 4][ks % 4], xb[f * K1 * 16], cE[f], 0, 0, 0);
-          else
-            cO[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(afr[ks / 4][ks % 4], xb[f * K1 * 16], cO[f], 0, 0, 0);
-        }
+        for (int v = 0; v < 8; ++v) st[v] = src[v * 64];
       }
+    }
+    v4f cE[NX], cO[NX];
+    if (active) {
+      adj_fold_gemm1<K1, KE4, KQ1, NX>(TinvF, sXT, ib, lane, cE, cO);
+    } else {
+#pragma unroll
+      for (int f = 0; f < NX; ++f) cE[f] = cO[f] = v4f{0.f, 0.f, 0.f, 0.f};
     }
     v4f an[NT2];
 #pragma unroll
     for (int t = 0; t < NT2; ++t) an[t] = Tf[((long long)t * (H / 16) + ic / 16) * 64];
-    if constexpr (S::NCH == 1) __syncthreads();  // s / d overwrite X^T
+    if constexpr (NCH == 1) __syncthreads();  // s / d overwrite X^T
     // ---- Pointwise adjoint on the lane's 4 point pairs (row ln, i = ib + 4 lg + q and i + H).
     if (active) {
 #pragma unroll
@@ -394,17 +458,22 @@ __device__ __forceinline__ void grid_adj_mfma_fold_body(const GridAdjArgs<float>
         for (int h = 0; h < 2; ++h) {
           const int i = i0 + h * H;
           const float sg = h == 0 ? 1.0f : -1.0f;
-          double U = cE[2][q] + sg * cO[2][q], V = cE[3][q] + sg * cO[3][q];
+          double x4[4];
+#pragma unroll
+          for (int f = 0; f < 4; ++f) {
+            if constexpr (SC)
+              x4[f] = st[2 * f][q] + sg * st[2 * f + 1][q];
+            else
+              x4[f] = cE[f][q] + sg * cO[f][q];
+          }
           if (a.test == 1) {
-            U = a.uvic[((long long)b * 2 + 0) * plane + (long long)j * I + i];
-            V = a.uvic[((long long)b * 2 + 1) * plane + (long long)j * I + i];
+            x4[2] = a.uvic[((long long)b * 2 + 0) * plane + (long long)j * I + i];
+            x4[3] = a.uvic[((long long)b * 2 + 1) * plane + (long long)j * I + i];
           }
           double pbar[8];
 #pragma unroll
-          for (int f = 0; f < 8; ++f)
-            pbar[f] = (f >= pb0 && f < pb0 + NPB) ? cE[4 + f - pb0][q] + sg * cO[4 + f - pb0][q] : 0.0;
-          grid_products_adjoint(cf, geo, gl, j, i, a.test, cE[0][q] + sg * cO[0][q], cE[1][q] + sg * cO[1][q], U,
-                                V, pbar, o2[h], dpar);
+          for (int f = 0; f < 8; ++f) pbar[f] = cE[PB + f][q] + sg * cO[PB + f][q];
+          grid_products_adjoint(cf, geo, gl, j, i, a.test, x4[0], x4[1], x4[2], x4[3], pbar, o2[h], dpar);
         }
         // The pinned winds of test 1 do not depend on the state.
 #pragma unroll
@@ -439,7 +508,6 @@ __device__ __forceinline__ void grid_adj_mfma_fold_body(const GridAdjArgs<float>
     }
     __syncthreads();
   }
-  }  // passes
 
   // Per-row parameter partials: row ln is held by lanes ln + 16 g of every wave.
 #pragma unroll
@@ -481,10 +549,10 @@ __device__ __forceinline__ void grid_adj_mfma_fold_body(const GridAdjArgs<float>
   }
 }
 
-template <int MT, int NP = 1>
+template <int MT, bool SC = false>
 __global__ void __launch_bounds__(256) k_grid_adj_mfma_fold(GridAdjArgs<float> a, const float* __restrict__ TinvF,
                                                             const float* __restrict__ TfwdF) {
-  grid_adj_mfma_fold_body<MT, NP>(a, TinvF, TfwdF);
+  grid_adj_mfma_fold_body<MT, SC>(a, TinvF, TfwdF);
 }
 
 // T42: allocated for 2+ waves per SIMD (accumulators in VGPRs): 214 registers instead of 318,
@@ -495,21 +563,21 @@ k_grid_adj_mfma_fold_w2(GridAdjArgs<float> a, const float* __restrict__ TinvF, c
   grid_adj_mfma_fold_body<MT>(a, TinvF, TfwdF);
 }
 
-// Passes of the folded grid adjoint per truncation (two at T106, whose 12-field staging would
-// not fit the LDS).
+// The S_t-cached form per truncation: T106, whose 12-field staging would not fit the LDS.
 template <int MT>
-constexpr int adj_fold_passes() { return MT == 106 ? 2 : 1; }
+constexpr bool adj_fold_cached() { return MT == 106; }
 
 template <int MT>
 static hipError_t launch_adj_mfma_fold_t(const GridAdjArgs<float>& a, const float* TinvH, const float* TfwdH,
                                          hipStream_t s) {
-  constexpr int NP = adj_fold_passes<MT>();
-  const size_t smem = sizeof(float) * AdjFoldShape<MT, NP>::LDS_FLOATS;
+  constexpr bool SC = adj_fold_cached<MT>();
+  if (SC && a.stc == nullptr) return hipErrorInvalidValue;
+  const size_t smem = sizeof(float) * AdjFoldShape<MT, SC>::LDS_FLOATS;
   const dim3 grid((unsigned)((long long)a.B * a.J / 16));
   if constexpr (MT == 42)
     hipLaunchKernelGGL(k_grid_adj_mfma_fold_w2<MT>, grid, dim3(256), smem, s, a, TinvH, TfwdH);
   else
-    hipLaunchKernelGGL((k_grid_adj_mfma_fold<MT, NP>), grid, dim3(256), smem, s, a, TinvH, TfwdH);
+    hipLaunchKernelGGL((k_grid_adj_mfma_fold<MT, SC>), grid, dim3(256), smem, s, a, TinvH, TfwdH);
   return hipGetLastError();
 }
 
@@ -517,7 +585,15 @@ size_t grid_adj_mfma_fold_lds_bytes(int M) {
   switch (M) {
     case 42: return sizeof(float) * AdjFoldShape<42>::LDS_FLOATS;
     case 63: return sizeof(float) * AdjFoldShape<63>::LDS_FLOATS;
-    case 106: return sizeof(float) * AdjFoldShape<106, adj_fold_passes<106>()>::LDS_FLOATS;
+    case 106: return sizeof(float) * AdjFoldShape<106, adj_fold_cached<106>()>::LDS_FLOATS;
+    default: return 0;
+  }
+}
+
+size_t grid_adj_mfma_fold_cache_bytes(int M, int B, int J) {
+  const long long nblk = (long long)B * J / 16;
+  switch (M) {
+    case 106: return sizeof(float) * (size_t)AdjFoldShape<106, adj_fold_cached<106>()>::cache_floats(nblk);
     default: return 0;
   }
 }

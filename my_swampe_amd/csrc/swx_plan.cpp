@@ -143,6 +143,7 @@ struct swx_plan {
   void* Tbar = nullptr;
   void* Gbar = nullptr;
   double* pgrad = nullptr;
+  void* adj_stc = nullptr;       // S_t cache of the folded MFMA grid adjoint (T106)
   // captured forward graph
   hipGraphExec_t gexec = nullptr;
   int gexec_steps = 0;
@@ -597,6 +598,8 @@ int ensure_adjoint(swx_plan* p) {
       (rc = dalloc(p, &p->Tbar, (size_t)B * Mp * kNT * J * 2 * p->rsz)) ||
       (rc = dalloc(p, &p->Gbar, (size_t)B * kNG * J * Mp * 2 * p->rsz)) || (rc = dalloc(p, &p->pgrad, (size_t)B * J * 3 * 8)))
     return rc;
+  const size_t stc = p->use_adj_fold ? grid_adj_mfma_fold_cache_bytes(p->geo.M, B, J) : 0;
+  if (stc > 0 && (rc = dalloc(p, &p->adj_stc, stc))) return rc;
   return SWX_OK;
 }
 
@@ -657,6 +660,7 @@ GridAdjArgs<R> grid_adj_args(swx_plan* p, int mode) {
   a.Gbar = (R*)p->Gbar;
   a.uvic = p->uvic;
   a.pgrad = p->pgrad;
+  a.stc = p->adj_stc;
   return a;
 }
 
@@ -1293,7 +1297,7 @@ int swx_plan_set_params(swx_plan* p, const double* params, void* stream) {
 void swx_plan_destroy(swx_plan* p) {
   if (!p) return;
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
-  void* bufs[] = {p->Lam, p->Tbar, p->Gbar, p->pgrad, p->Pf, p->Hf, p->Pi, p->Hi, p->fragP[0], p->fragP[1],
+  void* bufs[] = {p->Lam, p->Tbar, p->Gbar, p->pgrad, p->adj_stc, p->Pf, p->Hf, p->Pi, p->Hi, p->fragP[0], p->fragP[1],
                   p->fragP[2], p->fragP[3], p->fragF[0], p->fragF[1], p->fragF[2], p->fragF[3], p->tw, p->TinvF, p->TfwdF, p->TinvH, p->TfwdH, p->TinvB, p->TfwdB, p->geo_d, p->f2, p->coef, p->S, p->T, p->G, p->Gprev,
                   p->raw, p->lag, p->g0uv, p->phys, p->uvic, p->part_uv, p->dead, p->R0f, p->R0p, p->resid_c, p->lagx,
                   p->Gsyn};
