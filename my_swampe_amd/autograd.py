@@ -146,19 +146,23 @@ def adjoint_through_checkpoints(plan: Plan, cfg: RolloutConfig, ckpts, gPhi):
         return plan.adjoint_result()
     plan.set_option(_capi.OPT_PHYS_STEPS, -1)  # the recompute needs the spectral states only
     peak = 0
+    seg_tape = None
     for seg in reversed(range(len(ckpts))):
         start = seg * K
         n = min(K, cfg.nsteps - start)
-        if n > 1:  # recompute the segment's states from its checkpoint
-            plan.state_set(ckpts[seg])
-        tape = [ckpts[seg]]
-        t = cfg.t0 + start
-        for s in range(n - 1):
-            plan.run(t + s, 1)
-            tape.append(plan.state_get())
-        peak = max(peak, len(tape))
-        for s in reversed(range(n)):
-            plan.adjoint_step(tape[s])
+        if n == 1:
+            plan.adjoint_step(ckpts[seg])
+            peak = max(peak, 1)
+            continue
+        # Recompute the segment's states from its checkpoint in one native call (the states
+        # before its first n - 1 steps), take the last one, and sweep the segment natively.
+        if seg_tape is None:
+            seg_tape = ckpts[seg].new_empty((K,) + tuple(ckpts[seg].shape))
+        plan.state_set(ckpts[seg])
+        plan.run_tape(cfg.t0 + start, n - 1, out=seg_tape)
+        seg_tape[n - 1].copy_(plan.state_get())
+        peak = max(peak, n)
+        plan.adjoint_sweep(seg_tape[:n])
     plan.set_option(_capi.OPT_PHYS_STEPS, 1)
     cfg.stats.update(checkpoints=len(ckpts), segment_states=peak)
     return plan.adjoint_result()

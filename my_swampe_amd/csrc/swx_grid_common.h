@@ -12,6 +12,11 @@ namespace swx {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
+// Workgroup barrier for an LDS hand-off: waits for the wave's own LDS operations only, so global
+// loads issued before it stay in flight across it (__syncthreads() also drains those).  For
+// kernels whose threads exchange data through LDS alone.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // Per-member and per-row constants of the FP32 grid physics.
 struct PointConsts {
   float dph, itr, itd, Phibar, s1m, inv_denE;

@@ -227,12 +227,20 @@ class Plan:
         self._keep_adj_state = t
         _capi.check(self._lib.swx_adjoint_step(self._h, _ptr(t), self._stream(stream)), "swx_adjoint_step")
 
-    def run_tape(self, t0: int, nsteps: int, stream=None):
+    def run_tape(self, t0: int, nsteps: int, stream=None, out=None):
         """Advance nsteps steps from step index t0, returning the spectral state before each step as
-        a complex128 tape (nsteps, B, 3, M+1, N+1); only the last step writes the grid carry."""
+        a complex128 tape (nsteps, B, 3, M+1, N+1); only the last step writes the grid carry.  With
+        `out` (a contiguous complex128 tensor whose first nsteps entries take the states) the tape is
+        written there and `out` is returned."""
         torch = _torch()
         Mp = self.M + 1
-        tape = torch.empty((int(nsteps), self.B, 3, Mp, Mp), dtype=torch.complex128, device=self.device)
+        if out is None:
+            tape = torch.empty((int(nsteps), self.B, 3, Mp, Mp), dtype=torch.complex128, device=self.device)
+        else:
+            if (out.dtype != torch.complex128 or not out.is_contiguous() or out.shape[0] < int(nsteps)
+                    or tuple(out.shape[1:]) != (self.B, 3, Mp, Mp)):
+                raise ValueError("out must be a contiguous complex128 (>= nsteps, B, 3, M+1, N+1) tensor")
+            tape = out
         _capi.check(self._lib.swx_run_tape(self._h, int(t0), int(nsteps), _ptr(tape), self._stream(stream)),
                     "swx_run_tape")
         return tape
