@@ -342,23 +342,73 @@ def run_t106_fwd_adjoint(B=64, steps=5000, every=70, precision="fp32", compare=(
     return out
 
 
-def cpu_baseline(seconds_target: float = 12.0):
-    """FP64 oracle (numpy, one core): T42, 1 member, forced modified Euler, dt=120 s, runs of 240 steps."""
+CPU_STEPS = 240  # oracle steps per timed run (one member)
+
+
+def _cpu_worker(steps: int, start_at: float) -> None:
+    """One CPU worker: the FP64 numpy oracle, T42, 1 member, forced modified Euler, dt = 120 s.
+    Warms its geometry caches, waits for the common start time, runs `steps` steps and prints
+    {"t0", "t1"} (wall-clock seconds) as JSON."""
     from oracle import swampe_oracle as orc
 
     eta0, delta0, Phi0, U0, V0 = orc.rest_noise_ic(42, 0, 3.2e-5)
     kw = dict(M=42, dt=120.0, Phibar=3e5, omega=3.2e-5, a=8.2e7, taurad=86400.0, taudrag=86400.0, DPhieq=3e5,
               eta0_init=eta0, delta0_init=delta0, Phi0_init=Phi0, U0_init=U0, V0_init=V0, diagnostics=False,
               return_history=False)
-    orc.run_model_scan(tmax=4, **kw)  # warm caches (geometry)
-    steps, t = 0, 0.0
-    while t < seconds_target:
-        t0 = time.perf_counter()
-        orc.run_model_scan(tmax=242, **kw)
-        t += time.perf_counter() - t0
-        steps += 240
-    return dict(value=steps / t, unit="member-steps/s", cores=1, kind="port",
-                sample=f"FP64 numpy oracle, T42 forced modEuler, 1 member, {steps} steps ({steps // 240} runs of 240)")
+    orc.run_model_scan(tmax=4, **kw)  # geometry and basis caches
+    while time.time() < start_at:
+        time.sleep(0.005)
+    t0 = time.time()
+    orc.run_model_scan(tmax=steps + 2, **kw)
+    print(json.dumps({"t0": t0, "t1": time.time()}), flush=True)
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_round(workers: int, steps: int) -> float:
+    """`workers` single-threaded oracle processes started together: member-steps/s of the round
+    (workers x steps over the span from the common start to the last finish)."""
+    import subprocess
+
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
+    start_at = time.time() + 4.0  # after every worker has imported and warmed up
+    procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker", str(steps), repr(start_at)],
+                              env=env, stdout=subprocess.PIPE, text=True) for _ in range(workers)]
+    ends = []
+    for pr in procs:
+        outp, _ = pr.communicate()
+        if pr.returncode != 0:
+            raise RuntimeError(f"CPU worker failed ({pr.returncode})")
+        ends.append(json.loads(outp.strip().splitlines()[-1]))
+    t0 = min(r["t0"] for r in ends)
+    return workers * steps / (max(r["t1"] for r in ends) - t0)
+
+
+def cpu_baseline(runs: int = 3, steps: int = CPU_STEPS):
+    """FP64 numpy oracle on the host (scripts/benchmark_scan.py:116-141 protocol: median of
+    repeated runs): T42, forced modified Euler, dt = 120 s, one member per single-threaded process;
+    one process (single core) and one process per available core, at most 16 (all cores, members
+    in parallel), `runs` rounds each.  Run before the GPU is initialised (the workers are separate
+    interpreters)."""
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    workers = max(1, min(16, nproc))
+    single = [_cpu_round(1, steps) for _ in range(runs)]
+    allc = [_cpu_round(workers, steps) for _ in range(runs)]
+    med = lambda v: float(np.median(v))  # noqa: E731
+    return dict(value=med(allc), unit="member-steps/s", cores=workers, kind="port",
+                sample=(f"FP64 numpy oracle, T42 forced modEuler, dt=120 s, {steps} steps per member; {workers} "
+                        f"single-threaded processes (one member each) started together, median of {runs} rounds"),
+                single_core=dict(value=med(single), runs=single, cores=1),
+                all_core_runs=allc, nproc=nproc, cpu_model=_cpu_model())
 
 
 def accuracy_vs_oracle(steps: int = 2000):
@@ -417,6 +467,9 @@ def copy_calibration(mib: int):
 
 
 def main():
+    if len(sys.argv) == 4 and sys.argv[1] == "--cpu-worker":
+        _cpu_worker(int(sys.argv[2]), float(sys.argv[3]))
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
@@ -442,6 +495,8 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; members shard across GPUs with no per-step "
                                    "exchange, multi-GPU runs are not provided"}))
         sys.exit(2)
+    # The CPU baseline's worker processes start before anything touches the GPU.
+    cpu = None if args.no_cpu_baseline else cpu_baseline()
     import torch
 
     if not torch.cuda.is_available():
@@ -541,8 +596,8 @@ def main():
         out["mixed_T42x256_T63x64"] = run_mixed(args.steps, args.warmup)
     if not args.no_t106:
         out["t106_fwd_adjoint"] = run_t106_fwd_adjoint(precision=args.precision)
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline()
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
         out["accuracy_T42_1member"] = accuracy_vs_oracle()
     print(json.dumps(out), flush=True)
 

@@ -834,9 +834,11 @@ __global__ void __launch_bounds__(512) k_grid_adj_mfma_fold_sc8(GridAdjArgs<floa
   grid_adj_mfma_fold_sc8_body<MT>(a, TinvF, TfwdF);
 }
 
-// The S_t-cached form per truncation: T106, whose 12-field staging would not fit the LDS.
+// The S_t-cached eight-wave form per truncation: T106 (whose 12-field staging would not fit the
+// LDS) and T63 (one workgroup per CU either way; two waves per SIMD instead of one); T42 keeps
+// two four-wave workgroups per CU.
 template <int MT>
-constexpr bool adj_fold_cached() { return MT == 106; }
+constexpr bool adj_fold_cached() { return MT != 42; }
 
 template <int MT>
 static hipError_t launch_adj_mfma_fold_t(const GridAdjArgs<float>& a, const float* TinvH, const float* TfwdH,
@@ -857,11 +859,19 @@ static hipError_t launch_adj_mfma_fold_t(const GridAdjArgs<float>& a, const floa
   return hipGetLastError();
 }
 
+template <int MT>
+constexpr size_t adj_fold_lds_bytes() {
+  if constexpr (adj_fold_cached<MT>())
+    return sizeof(float) * AdjFoldSC8Shape<MT>::LDS_FLOATS;
+  else
+    return sizeof(float) * AdjFoldShape<MT>::LDS_FLOATS;
+}
+
 size_t grid_adj_mfma_fold_lds_bytes(int M) {
   switch (M) {
-    case 42: return sizeof(float) * AdjFoldShape<42>::LDS_FLOATS;
-    case 63: return sizeof(float) * AdjFoldShape<63>::LDS_FLOATS;
-    case 106: return sizeof(float) * AdjFoldSC8Shape<106>::LDS_FLOATS;
+    case 42: return adj_fold_lds_bytes<42>();
+    case 63: return adj_fold_lds_bytes<63>();
+    case 106: return adj_fold_lds_bytes<106>();
     default: return 0;
   }
 }
@@ -869,6 +879,8 @@ size_t grid_adj_mfma_fold_lds_bytes(int M) {
 size_t grid_adj_mfma_fold_cache_bytes(int M, int B, int J) {
   const long long nblk = (long long)B * J / 16;
   switch (M) {
+    case 42: return sizeof(float) * (size_t)AdjFoldShape<42, adj_fold_cached<42>()>::cache_floats(nblk);
+    case 63: return sizeof(float) * (size_t)AdjFoldShape<63, adj_fold_cached<63>()>::cache_floats(nblk);
     case 106: return sizeof(float) * (size_t)AdjFoldShape<106, adj_fold_cached<106>()>::cache_floats(nblk);
     default: return 0;
   }

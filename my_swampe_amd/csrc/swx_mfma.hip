@@ -531,6 +531,21 @@ __global__ void __launch_bounds__(256) k_grid_mfma_pair(GridArgs<float> a0, cons
   }
 }
 
+// The steady form of the pair launch (both plans folded, a step without bookkeeping): the
+// FP32-only bodies allocated for W waves per SIMD (two, or one with a T106 body, whose registers
+// need a whole SIMD), plan 1's (the larger truncation's) workgroups first so that plan 0's
+// shorter ones fill the tail.
+template <int MA, int MB, int W>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8)))
+k_grid_mfma_pair_fast(GridArgs<float> a0, const float* __restrict__ ti0, const float* __restrict__ tf0,
+                      GridArgs<float> a1, const float* __restrict__ ti1, const float* __restrict__ tf1, int nblk1) {
+  extern __shared__ __align__(16) float lds[];
+  if ((int)blockIdx.x < nblk1)
+    grid_mfma_fold_body<MB, true>(a1, ti1, tf1, blockIdx.x, lds);
+  else
+    grid_mfma_fold_body<MA, true>(a0, ti0, tf0, blockIdx.x - nblk1, lds);
+}
+
 template <int MT>
 static hipError_t launch_mfma_t(const GridArgs<float>& a, const float* TinvF, const float* TfwdF, hipStream_t s) {
   using S = MfmaShape<MT>;
@@ -1160,6 +1175,25 @@ __global__ void __launch_bounds__(256) k_leg_mfma_pair(LegArgs<float> a0, LegArg
     leg_mfma_body<MB, MBB, GEN, false, FOLD>(a1, (blk - nblk0) % (MB + 1), (blk - nblk0) / (MB + 1), lds);
 }
 
+// The folded specialised-update pair (plain modified Euler in both plans; LIN0 / LIN1: the plan's
+// step also advances its prev lineage) with member groups sized for one LDS budget (the larger
+// of the two bodies' folded LDS) and allocated for W waves per SIMD; heavy-first order: plan 1's
+// workgroups (m = L / g1, group L % g1) first, then plan 0's, so the light small-K items of the
+// smaller truncation fill the tail.
+template <int MA, int MBA, int MB, int MBB, int W, bool LIN0, bool LIN1>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8)))
+k_leg_mfma_pair_hf(LegArgs<float> a0, LegArgs<float> a1, int g0, int g1) {
+  extern __shared__ __align__(16) float lds[];
+  const int n1 = (MB + 1) * g1;
+  const int L = blockIdx.x;
+  if (L < n1) {
+    leg_mfma_body<MB, MBB, false, LIN1, true>(a1, L / g1, L % g1, lds);
+  } else {
+    const int L0 = L - n1;
+    leg_mfma_body<MA, MBA, false, LIN0, true>(a0, L0 / g0, L0 % g0, lds);
+  }
+}
+
 // Whether a launch needs the general spectral update (any scheme other than plain modified Euler).
 static bool leg_generic(const LegArgs<float>& a) { return a.si || a.raw_filter || a.expflag; }
 
@@ -1265,6 +1299,15 @@ template <int MA, int MB>
 static hipError_t launch_grid_pair_t(const GridArgs<float>& a0, const float* ti0, const float* tf0, bool f0,
                                      const GridArgs<float>& a1, const float* ti1, const float* tf1, bool f1,
                                      hipStream_t s) {
+  if (f0 && f1 && grid_fold_fast(a0) && grid_fold_fast(a1)) {
+    const size_t smem = sizeof(float) * (size_t)cmax(MfmaFoldShape<MA>::LDS_FAST, MfmaFoldShape<MB>::LDS_FAST);
+    const int n0 = (int)((long long)a0.B * a0.J / MfmaFoldShape<MA>::ROWS);
+    const int n1 = (int)((long long)a1.B * a1.J / MfmaFoldShape<MB>::ROWS);
+    constexpr int W = MB == 106 ? 1 : 2;
+    hipLaunchKernelGGL((k_grid_mfma_pair_fast<MA, MB, W>), dim3(n0 + n1), dim3(256), smem, s, a0, ti0, tf0, a1, ti1,
+                       tf1, n1);
+    return hipGetLastError();
+  }
   if (f0 && f1) return launch_grid_pair_ft<MA, MB, true, true>(a0, ti0, tf0, a1, ti1, tf1, s);
   if (f0) return launch_grid_pair_ft<MA, MB, true, false>(a0, ti0, tf0, a1, ti1, tf1, s);
   if (f1) return launch_grid_pair_ft<MA, MB, false, true>(a0, ti0, tf0, a1, ti1, tf1, s);
@@ -1312,6 +1355,26 @@ hipError_t launch_grid_mfma_pair(const GridArgs<float>& a0, const float* ti0, co
 }
 
 hipError_t launch_leg_mfma_pair(const LegArgs<float>& a0, const LegArgs<float>& a1, hipStream_t s) {
+  if (a0.M == 42 && a1.M == 63 && a0.leg_fold && a1.leg_fold && !leg_generic(a0) && !leg_generic(a1)) {
+    // T42 in groups of 4 members, T63 of 2: both fit the T42 body's 29 KB of LDS (5 workgroups
+    // per CU) and its 96-register allocation.
+    constexpr int MBA = 4, MBB = 2;
+    const size_t smem = sizeof(float) * (size_t)cmax(LegShape<42, MBA>::LDS_FLOATS_F, LegShape<63, MBB>::LDS_FLOATS_F);
+    const int g0 = (a0.B + MBA - 1) / MBA, g1 = (a1.B + MBB - 1) / MBB;
+    const dim3 grid(43 * g0 + 64 * g1);
+    if (a0.lineage && a1.lineage)
+      hipLaunchKernelGGL((k_leg_mfma_pair_hf<42, MBA, 63, MBB, 5, true, true>), grid, dim3(256), smem, s, a0, a1, g0, g1);
+    else if (a0.lineage)
+      hipLaunchKernelGGL((k_leg_mfma_pair_hf<42, MBA, 63, MBB, 5, true, false>), grid, dim3(256), smem, s, a0, a1, g0,
+                         g1);
+    else if (a1.lineage)
+      hipLaunchKernelGGL((k_leg_mfma_pair_hf<42, MBA, 63, MBB, 5, false, true>), grid, dim3(256), smem, s, a0, a1, g0,
+                         g1);
+    else
+      hipLaunchKernelGGL((k_leg_mfma_pair_hf<42, MBA, 63, MBB, 5, false, false>), grid, dim3(256), smem, s, a0, a1, g0,
+                         g1);
+    return hipGetLastError();
+  }
   if (a0.M == 42 && a1.M == 63) return launch_leg_pair_t<42, 8, 63, 4>(a0, a1, s);
   if (a0.M == 42 && a1.M == 106) return launch_leg_pair_t<42, 8, 106, 4>(a0, a1, s);
   if (a0.M == 63 && a1.M == 106) return launch_leg_pair_t<63, 4, 106, 4>(a0, a1, s);

@@ -564,11 +564,11 @@ __device__ inline void grid_products_adjoint(const double* cf, const double* geo
                                              int test, double eta, double Phi, double U, double V,
                                              const double pbar[8], double out[4], double dpar[3]) {
   const double Ab = pbar[0], Bb = pbar[1], Cb = pbar[2], Db = pbar[3], Eb = pbar[4];
-  const double den = geo[gl.denomE + j];
+  const double two_den = 2.0 / geo[gl.denomE + j];  // per row
   double etab = Ab * U + Bb * V;
   double Phib = Cb * U + Db * V;
-  double Ub = Ab * eta + Cb * Phi + Eb * (2.0 * U / den);
-  double Vb = Bb * eta + Db * Phi + Eb * (2.0 * V / den);
+  double Ub = Ab * eta + Cb * Phi + Eb * (two_den * U);
+  double Vb = Bb * eta + Db * Phi + Eb * (two_den * V);
   if (test < 0) {
     const double Phibar = cf[C_PHIBAR];
     const double day = geo[gl.day + i];
@@ -583,15 +583,17 @@ __device__ inline void grid_products_adjoint(const double* cf, const double* geo
     const double Fb = pbar[6], Gb = pbar[7];
     const double td = cf[C_TAUDRAG];
     const double inv_td = td == -1.0 ? 0.0 : 1.0 / td;
-    // F = -U Qc / tot - U / td, G = -V Qc / tot - V / td
-    Ub += Fb * (-Qc / safe - inv_td);
-    Vb += Gb * (-Qc / safe - inv_td);
-    const double Qcb = Fb * (-U / safe) + Gb * (-V / safe);
-    Phib += (Fb * U + Gb * V) * Qc / (safe * safe);
+    // F = -U Qc / tot - U / td, G = -V Qc / tot - V / td (one reciprocal of tot per point)
+    const double inv = 1.0 / safe;
+    const double FUGV = Fb * U + Gb * V;
+    Ub += Fb * (-Qc * inv - inv_td);
+    Vb += Gb * (-Qc * inv - inv_td);
+    const double Qcb = -FUGV * inv;
+    Phib += FUGV * Qc * (inv * inv);
     const double Qb = pbar[5] + Qcb * pass;
     Phib += Qb * (-itr);
     dpar[0] += Qb * (-Q * itr);                                   // d/dtaurad
-    if (td != -1.0) dpar[1] += (Fb * U + Gb * V) * (inv_td * inv_td);  // d/dtaudrag
+    if (td != -1.0) dpar[1] += FUGV * (inv_td * inv_td);  // d/dtaudrag
     if (day != 0.0) dpar[2] += Qb * itr * dphi_term;              // d/dDPhieq
   }
   out[0] = etab;
