@@ -183,6 +183,7 @@ struct LegArgs {
   const float* finvP;
   const float* finvH;
   int leg_fold;  // 1: the matrix-core kernel uses the folded tables
+  int bf16x3;    // 1: its contractions as split-BF16x3 products (folded kernel only)
   const double* coef;
   int ncoef;
   // [M+1][N+1]: Pw(invrs_leg(X)) = f2 * X for the basis in use (1 for an

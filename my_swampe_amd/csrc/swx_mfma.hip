@@ -748,6 +748,36 @@ __device__ __forceinline__ void leg_fwd_frags_fold(const LegArgs<float>& a, int 
   for (int q4 = 0; q4 < J / 32; ++q4) av[q4] = ar[q4];
 }
 
+// Split-BF16x3 products on the matrix cores (v_mfma_f32_16x16x16_bf16): four consecutive k-steps
+// of the FP32 16x16x4 form in one instruction.  A lane's 4 A values and 4 B values are those of
+// the same k's (k-step 4 g + e, its lane group), so pairing element e of A with element e of B
+// sums exactly the 16 products of the four k-steps.  x = hi + lo with hi = bf16(x) and
+// lo = bf16(x - hi); a b is accumulated in FP32 as lo hi' + hi lo' + hi hi' (lo lo', about
+// 2^-16 relative, dropped).
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
+
+struct SplitBF16 {
+  v4s hi, lo;
+};
+
+__device__ __forceinline__ SplitBF16 split_bf16(v4f x) {
+  v4bf h, l;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    h[e] = (__bf16)x[e];
+    l[e] = (__bf16)(x[e] - (float)h[e]);
+  }
+  return SplitBF16{__builtin_bit_cast(v4s, h), __builtin_bit_cast(v4s, l)};
+}
+
+__device__ __forceinline__ v4f mfma_bf16x3(const SplitBF16& a, v4f b, v4f c) {
+  const SplitBF16 bs = split_bf16(b);
+  c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a.lo, bs.hi, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a.hi, bs.lo, c, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a.hi, bs.hi, c, 0, 0, 0);
+}
+
 // One workgroup's work: zonal wavenumber m, member group `grp` (members grp*MB ...).
 // GEN = false: the modified-Euler scheme without the RAW filter only, whose spectral update
 // the compiler then specializes (the semi-implicit, RAW and explicit paths of spectral_update
@@ -756,8 +786,10 @@ __device__ __forceinline__ void leg_fwd_frags_fold(const LegArgs<float>& a, int 
 // LIN (with GEN = false): the step also advances the spectral prev lineage (StepFlags::lineage).
 // FOLD: the equatorially folded contractions (swx_kernels.h, leg_fold_*): half the matrix-core
 // work and half the basis fragments; the same FP64 update.
-template <int MT, int MB, bool GEN = true, bool LIN = false, bool FOLD = true>
+// BF3 (with FOLD): the forward and inverse contractions as split-BF16x3 products.
+template <int MT, int MB, bool GEN = true, bool LIN = false, bool FOLD = true, bool BF3 = false>
 __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int m, const int grp, float* lds) {
+  static_assert(!BF3 || FOLD, "split-BF16x3 contractions are provided for the folded kernel");
   using S = LegShape<MT, MB>;
   constexpr int Np = S::Np, Mp = MT + 1, J = S::J, JO = S::JO;
   // Column layout (folded: the P groups unpadded, LegShape).  P column tiles 0 .. CTP-1 start at
@@ -934,9 +966,17 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
         base = (even == isP) ? 0 : J / 2;
       }
       const float* br = sA + (base + lg) * RS + colb + ln;
+      if constexpr (BF3) {
 #pragma unroll
-      for (int ks = 0; ks < KSF; ++ks)
-        acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[ks / 4][ks % 4], br[4 * ks * RS], acc[q], 0, 0, 0);
+        for (int g = 0; g < KSF / 4; ++g) {
+          const v4f bv = {br[16 * g * RS], br[(16 * g + 4) * RS], br[(16 * g + 8) * RS], br[(16 * g + 12) * RS]};
+          acc[q] = mfma_bf16x3(split_bf16(av[g]), bv, acc[q]);
+        }
+      } else {
+#pragma unroll
+        for (int ks = 0; ks < KSF; ++ks)
+          acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[ks / 4][ks % 4], br[4 * ks * RS], acc[q], 0, 0, 0);
+      }
     }
   }
   __syncthreads();
@@ -1032,16 +1072,49 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
       for (int c = 0; c < CIP; ++c) dP[c] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int c = 0; c < CIH; ++c) dH[c] = v4f{0.f, 0.f, 0.f, 0.f};
+      if constexpr (BF3) {
+        // Groups of 4 k-steps; the steps past nks (and their staging rows) enter as zeros.
 #pragma unroll
-      for (int ks = 0; ks < KSX; ++ks) {
-        if (ks < nks) {
-          const float* xr = sX + (row0 + 4 * ks + lg) * XS + ln;
+        for (int g = 0; g < KSX / 4; ++g) {
+          if (4 * g < nks) {
+            v4f pz, hz;
+            int rw[4];
 #pragma unroll
-          for (int c = 0; c < CIP; ++c)
-            dP[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(pa[ks / 4][ks % 4], xr[c * 16], dP[c], 0, 0, 0);
+            for (int e = 0; e < 4; ++e) {
+              const bool in = 4 * g + e < nks;
+              pz[e] = in ? pa[g][e] : 0.0f;
+              hz[e] = in ? ha[g][e] : 0.0f;
+              rw[e] = in ? row0 + 4 * (4 * g + e) + lg : 0;
+            }
+            const SplitBF16 ps = split_bf16(pz), hs = split_bf16(hz);
 #pragma unroll
-          for (int c = 0; c < CIH; ++c)
-            dH[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(ha[ks / 4][ks % 4], xr[XP + c * 16], dH[c], 0, 0, 0);
+            for (int c = 0; c < CIP; ++c) {
+              v4f bv;
+#pragma unroll
+              for (int e = 0; e < 4; ++e) bv[e] = 4 * g + e < nks ? sX[rw[e] * XS + ln + c * 16] : 0.0f;
+              dP[c] = mfma_bf16x3(ps, bv, dP[c]);
+            }
+#pragma unroll
+            for (int c = 0; c < CIH; ++c) {
+              v4f bv;
+#pragma unroll
+              for (int e = 0; e < 4; ++e) bv[e] = 4 * g + e < nks ? sX[rw[e] * XS + ln + XP + c * 16] : 0.0f;
+              dH[c] = mfma_bf16x3(hs, bv, dH[c]);
+            }
+          }
+        }
+      } else {
+#pragma unroll
+        for (int ks = 0; ks < KSX; ++ks) {
+          if (ks < nks) {
+            const float* xr = sX + (row0 + 4 * ks + lg) * XS + ln;
+#pragma unroll
+            for (int c = 0; c < CIP; ++c)
+              dP[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(pa[ks / 4][ks % 4], xr[c * 16], dP[c], 0, 0, 0);
+#pragma unroll
+            for (int c = 0; c < CIH; ++c)
+              dH[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(ha[ks / 4][ks % 4], xr[XP + c * 16], dH[c], 0, 0, 0);
+          }
         }
       }
       float* out = sA + ln * JO + (odd ? J / 2 : 0) + jt * 16 + 4 * lg;
@@ -1070,16 +1143,48 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
       for (int c = 0; c < CIP; ++c) dP[c] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int c = 0; c < CIH; ++c) dH[c] = v4f{0.f, 0.f, 0.f, 0.f};
+      if constexpr (BF3) {
 #pragma unroll
-      for (int ks = 0; ks < KSM; ++ks) {
-        if (ks < ksn) {
-          const float* xr = sX + (4 * ks + lg) * XS + ln;
+        for (int g = 0; g < KSM / 4; ++g) {
+          if (4 * g < ksn) {
+            v4f pz, hz;
+            int rw[4];
 #pragma unroll
-          for (int c = 0; c < CIP; ++c)
-            dP[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(pa[ks / 4][ks % 4], xr[c * 16], dP[c], 0, 0, 0);
+            for (int e = 0; e < 4; ++e) {
+              const bool in = 4 * g + e < ksn;
+              pz[e] = in ? pa[g][e] : 0.0f;
+              hz[e] = in ? ha[g][e] : 0.0f;
+              rw[e] = in ? 4 * (4 * g + e) + lg : 0;
+            }
+            const SplitBF16 ps = split_bf16(pz), hs = split_bf16(hz);
 #pragma unroll
-          for (int c = 0; c < CIH; ++c)
-            dH[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(ha[ks / 4][ks % 4], xr[XP + c * 16], dH[c], 0, 0, 0);
+            for (int c = 0; c < CIP; ++c) {
+              v4f bv;
+#pragma unroll
+              for (int e = 0; e < 4; ++e) bv[e] = 4 * g + e < ksn ? sX[rw[e] * XS + ln + c * 16] : 0.0f;
+              dP[c] = mfma_bf16x3(ps, bv, dP[c]);
+            }
+#pragma unroll
+            for (int c = 0; c < CIH; ++c) {
+              v4f bv;
+#pragma unroll
+              for (int e = 0; e < 4; ++e) bv[e] = 4 * g + e < ksn ? sX[rw[e] * XS + ln + XP + c * 16] : 0.0f;
+              dH[c] = mfma_bf16x3(hs, bv, dH[c]);
+            }
+          }
+        }
+      } else {
+#pragma unroll
+        for (int ks = 0; ks < KSM; ++ks) {
+          if (ks < ksn) {
+            const float* xr = sX + (4 * ks + lg) * XS + ln;
+#pragma unroll
+            for (int c = 0; c < CIP; ++c)
+              dP[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(pa[ks / 4][ks % 4], xr[c * 16], dP[c], 0, 0, 0);
+#pragma unroll
+            for (int c = 0; c < CIH; ++c)
+              dH[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(ha[ks / 4][ks % 4], xr[XP + c * 16], dH[c], 0, 0, 0);
+          }
         }
       }
       // Column-major [col][JO] output: the 4 rows a lane holds are contiguous (one b128 store).
@@ -1133,12 +1238,12 @@ __device__ __forceinline__ void leg_mfma_body(const LegArgs<float>& a, const int
   }
 }
 
-template <int MT, int MB, bool GEN, bool LIN, bool FOLD>
+template <int MT, int MB, bool GEN, bool LIN, bool FOLD, bool BF3 = false>
 __global__ void __launch_bounds__(256) k_leg_mfma(LegArgs<float> a, int heavy_first) {
   extern __shared__ __align__(16) float lds[];
   int m, grp;
   leg_item(heavy_first, m, grp);
-  leg_mfma_body<MT, MB, GEN, LIN, FOLD>(a, m, grp, lds);
+  leg_mfma_body<MT, MB, GEN, LIN, FOLD, BF3>(a, m, grp, lds);
 }
 
 // The same kernel allocated for at least 4 waves per SIMD (MFMA accumulators in VGPRs), for the
@@ -1205,7 +1310,14 @@ static hipError_t launch_leg_mfma_t(const LegArgs<float>& a, hipStream_t s, bool
   constexpr bool kW4 = MT == 42 || (MT == 63 && MB <= 4);
   constexpr bool kW5 = MT == 42 && MB == 4;
   const int heavy_first = shared ? 0 : 1;
-  if (a.leg_fold) {
+  if (a.leg_fold && a.bf16x3) {
+    if (leg_generic(a))
+      hipLaunchKernelGGL((k_leg_mfma<MT, MB, true, false, true, true>), grid, dim3(256), smem, s, a, heavy_first);
+    else if (a.lineage)
+      hipLaunchKernelGGL((k_leg_mfma<MT, MB, false, true, true, true>), grid, dim3(256), smem, s, a, heavy_first);
+    else
+      hipLaunchKernelGGL((k_leg_mfma<MT, MB, false, false, true, true>), grid, dim3(256), smem, s, a, heavy_first);
+  } else if (a.leg_fold) {
     if (leg_generic(a))
       hipLaunchKernelGGL((k_leg_mfma<MT, MB, true, false, true>), grid, dim3(256), smem, s, a, heavy_first);
     else if constexpr (kW5) {

@@ -282,6 +282,7 @@ LegArgs<R> leg_args(swx_plan* p, int mode) {
   a.finvP = p->fragF[2];
   a.finvH = p->fragF[3];
   a.leg_fold = p->leg_fold == 2 && p->fragF[0] ? 1 : 0;
+  a.bf16x3 = p->d.precision == SWX_PREC_BF16X3 && a.leg_fold ? 1 : 0;
   a.coef = p->coef;
   a.ncoef = p->cl.ncoef;
   a.f2 = p->f2;

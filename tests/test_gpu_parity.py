@@ -468,10 +468,11 @@ def test_mixed_truncation_ensemble_matches_separate_plans(pair, torch_gpu):
 
 @pytest.mark.parametrize("M", [42, 63, 106])
 def test_bf16x3_grid_kernel_vs_oracle(M, torch_gpu):
-    """Split-BF16x3 grid kernel (with the FP32 MFMA Legendre kernel and FP64 update) vs the FP64
-    oracle and vs the FP32 MFMA path, 12 forced steps from the balanced zonal flow of
-    test_mfma_grid_kernel_matches_plain_fp32 (3 members).  Relative L2 <= 2e-5 for Phi, U, eta
-    (the dropped lo*lo term and the 16-bit split leave ~2^-17 per product; FP32 reaches 2e-7)."""
+    """Split-BF16x3 transforms (the BF16x3 grid kernel and the MFMA Legendre kernel's BF16x3
+    contractions, FP64 update) vs the FP64 oracle and vs the FP32 MFMA path, 12 forced steps from
+    the balanced zonal flow of test_mfma_grid_kernel_matches_plain_fp32 (3 members).  Relative
+    L2 <= 2e-5 for Phi, U, eta (the dropped lo*lo term and the 16-bit split leave ~2^-17 per
+    product; FP32 reaches 2e-7)."""
     from my_swampe_amd.model import param_row
     from my_swampe_amd.plan import Plan
 
@@ -499,6 +500,45 @@ def test_bf16x3_grid_kernel_vs_oracle(M, torch_gpu):
         rel = {k: float(np.linalg.norm(res[k][f] - r) / np.linalg.norm(r)) for k in res}
         print(M, f, "relL2 vs oracle", rel)
         assert rel["bf16x3"] < 2e-5, (M, f, rel)
+
+
+@pytest.mark.parametrize("M", [42, 63, 106])
+def test_bf16x3_legendre_contractions(M, torch_gpu):
+    """The Legendre kernel's split-BF16x3 contractions alone: a BF16x3 plan and an FP32 plan both
+    on the plain FP32 grid kernel (SWX_OPT_GRID_KERNEL = 1), so they differ only in the Legendre
+    products.  40 forced steps (the call's last steps advance the prev lineage, the others not:
+    both specialised-update variants run), 3 members.  The two differ (the BF16x3 path runs) and
+    stay within relative L2 2e-5 of each other and of the FP64 oracle on Phi, U, eta."""
+    from my_swampe_amd import _capi
+    from my_swampe_amd.model import param_row
+    from my_swampe_amd.plan import Plan
+
+    dt = 120.0 if M == 42 else 60.0
+    a, omega, Phibar = 6.37122e6, 7.2921159e-5, 3e4
+    p = param_row(dt=dt, a=a, omega=omega, g=9.8, Phibar=Phibar, taurad=86400.0, taudrag=86400.0,
+                  DPhieq=3e4, K6=1.24e33, K6Phi=None, alpha=0.01)
+    s = orc.build_static(M=M, dt=dt, a=a, omega=omega, g=9.8, Phibar=Phibar, taurad=86400.0, taudrag=86400.0,
+                         DPhieq=3e4, K6=1.24e33, K6Phi=None, test=2)
+    ics = orc.analytic_ic(s, 2, 0.3)
+    _print_case("bf16x3-legendre", M=M)
+    res = {}
+    for prec in ("fp32", "bf16x3"):
+        plan = Plan(M, np.stack([p] * 3), precision=prec)
+        plan.set_option(_capi.OPT_GRID_KERNEL, 1)
+        plan.init(*ics)
+        plan.run(2, 40)
+        res[prec] = {k: _np(v[0]) for k, v in plan.export(fields=("Phi_curr", "U_curr", "eta_curr")).items()
+                     if k != "dead"}
+        plan.close()
+    ref = orc.run_model_scan(M=M, dt=dt, tmax=42, Phibar=Phibar, omega=omega, a=a, taurad=86400.0,
+                             taudrag=86400.0, DPhieq=3e4, eta0_init=ics[0], delta0_init=ics[1], Phi0_init=ics[2],
+                             U0_init=ics[3], V0_init=ics[4], diagnostics=False, return_history=False)["last_state"]
+    for f in ("Phi_curr", "U_curr", "eta_curr"):
+        r = getattr(ref, f)
+        d = float(np.linalg.norm(res["bf16x3"][f] - res["fp32"][f]) / np.linalg.norm(res["fp32"][f]))
+        rel = float(np.linalg.norm(res["bf16x3"][f] - r) / np.linalg.norm(r))
+        print(M, f, "bf16x3 vs fp32", d, "vs oracle", rel)
+        assert 0.0 < d < 2e-5 and rel < 2e-5, (M, f, d, rel)
 
 
 @pytest.mark.parametrize("M", [42, 63, 106])

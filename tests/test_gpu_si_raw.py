@@ -259,6 +259,28 @@ def test_si_raw_fp32_vs_oracle(torch_gpu):
     assert rel["mfma_vs_plain"] <= 5e-6, rel
 
 
+def test_si_raw_bf16x3_vs_oracle(torch_gpu):
+    """Split-BF16x3 transforms (the BF16x3 grid kernel and the Legendre kernel's general-update
+    variant with BF16x3 contractions) with the FP64 update vs the FP64 oracle: the case of
+    test_si_raw_fp32_vs_oracle.  Relative L2: Phi and eta <= 3e-5 (the 16-bit operand split leaves
+    about 2^-17 per product; FP32 reaches 1e-6 here), delta <= 1e-3."""
+    from my_swampe_amd.model import param_row
+
+    kw = dict(dt=600.0, a=8.2e7, omega=3.2e-5, g=9.8, Phibar=3e5, taurad=36000.0, taudrag=21600.0, DPhieq=1e6,
+              K6=1.24e33, K6Phi=None, alpha=0.05)
+    ics = _rough_ic(42, 8, 3.2e-5)
+    _print_case("si-raw-bf16x3")
+    ref = orc.run_model_scan(M=42, tmax=42, test=None, diagnostics=False, return_history=False,
+                             eta0_init=ics[0], delta0_init=ics[1], Phi0_init=ics[2], U0_init=ics[3], V0_init=ics[4],
+                             semi_implicit=True, raw_filter=True, **kw)["last_state"]
+    got = _plan_run("bf16x3", param_row(**kw)[None, :], ics, 40)
+    rel = {f: float(np.linalg.norm(got[f][0] - getattr(ref, f)) / np.linalg.norm(getattr(ref, f)))
+           for f in ("eta_curr", "delta_curr", "Phi_curr")}
+    print("relL2", rel)
+    assert rel["Phi_curr"] <= 3e-5 and rel["eta_curr"] <= 3e-5, rel
+    assert rel["delta_curr"] <= 1e-3, rel
+
+
 def test_si_raw_plans_reject_single_level_checkpoints(torch_gpu):
     """Spectral checkpoints and the adjoint cover the modified-Euler scheme only: clear errors."""
     from my_swampe_amd.model import param_row
