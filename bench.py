@@ -484,6 +484,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=100)
     ap.add_argument("--no-t106", action="store_true", help="skip the T106 x 64, 5000-step fwd+adjoint run")
+    ap.add_argument("--t106-steps", type=int, default=5000, help="steps of the T106 x 64 fwd+adjoint run")
+    ap.add_argument("--t106-compare", default="bf16x3,fp64",
+                    help="transform precisions timed beside the T106 run (comma list, or 'none')")
     ap.add_argument("--no-mixed", action="store_true", help="skip the 256 x T42 + 64 x T63 one-launch run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only run a plain device-to-device copy of this size (HBM counter calibration)")
@@ -595,7 +598,8 @@ def main():
     if not args.no_mixed and args.precision == "fp32":
         out["mixed_T42x256_T63x64"] = run_mixed(args.steps, args.warmup)
     if not args.no_t106:
-        out["t106_fwd_adjoint"] = run_t106_fwd_adjoint(precision=args.precision)
+        cmp = () if args.t106_compare == "none" else tuple(x for x in args.t106_compare.split(",") if x)
+        out["t106_fwd_adjoint"] = run_t106_fwd_adjoint(steps=args.t106_steps, precision=args.precision, compare=cmp)
     if cpu is not None:
         out["cpu_baseline"] = cpu
         out["accuracy_T42_1member"] = accuracy_vs_oracle()
