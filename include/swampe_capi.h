@@ -307,6 +307,18 @@ int swx_phase_curve_vjp(const double *g_flux, const double *flux_raw, int n, int
 int swx_plan_phase_curve(swx_plan *plan, const double *phases, int K, int normalize, double *flux,
                          double *flux_raw, void *stream);
 
+/* Vorticity and divergence from Fourier winds, spectral and on the grid: replaces
+ * diagnostic_eta_delta (src/my_swampe/spectral_transform.py:599-650) with the same arguments,
+ * as device FP64 arrays: Um, Vm (n, J, M+1) complex (n members, contiguous); fmn (M+1, N+1)
+ * complex; Pmn, Hmn (J, M+1, N+1); w (J); tstepcoeff, mJarray (J, M+1); dt.  Outputs (the
+ * reference's return tuple): eta, delta (n, J, I) complex, the grids of invrs_fft(invrs_leg(.))
+ * (either may be null), etamn (= zeta_mn + fmn), deltamn (n, M+1, N+1) complex.  Errors:
+ * SWX_ERR_ARG for negative n, N < M, I < 2 M + 1 with a grid requested, dt = 0, null inputs. */
+int swx_diagnostic_eta_delta(const double *Um, const double *Vm, const double *fmn, int n, int I, int J, int M, int N,
+                             const double *Pmn, const double *Hmn, const double *w, const double *tstepcoeff,
+                             const double *mJarray, double dt, double *eta, double *delta, double *etamn,
+                             double *deltamn, void *stream);
+
 /* Plan options (swx_plan_set_option).
  *   SWX_OPT_GRID_KERNEL: 0 = automatic (MFMA grid kernel for FP32 forward
  *     stepping when it fits), 1 = always the plain grid kernel.

@@ -21,6 +21,7 @@
 #include "swx_adjoint.h"
 #include "swx_kernels.h"
 #include "swx_observe.h"
+#include "swx_transforms.h"
 
 using namespace swx;
 
@@ -2127,6 +2128,29 @@ int swx_phase_curve_vjp(const double* g_flux, const double* flux_raw, int n, int
   if (n == 0) return SWX_OK;
   if (!g_flux || !flux_raw || !mus || !phases || !g_Phi) return fail(SWX_ERR_ARG, "phase curve vjp: null pointer");
   SWX_HIP(launch_phase_curve_vjp(g_flux, flux_raw, n, J, I, mus, phases, K, normalize, g_Phi, (hipStream_t)stream));
+  return SWX_OK;
+}
+
+int swx_diagnostic_eta_delta(const double* Um, const double* Vm, const double* fmn, int n, int I, int J, int M, int N,
+                             const double* Pmn, const double* Hmn, const double* w, const double* tstepcoeff,
+                             const double* mJarray, double dt, double* eta, double* delta, double* etamn,
+                             double* deltamn, void* stream) {
+  if (n < 0) return fail(SWX_ERR_ARG, "diagnostic_eta_delta: negative member count");
+  if (M < 0 || N < M || J < 1) return fail(SWX_ERR_ARG, "diagnostic_eta_delta: need M >= 0, N >= M, J >= 1");
+  if ((eta || delta) && I < 2 * M + 1)
+    return fail(SWX_ERR_ARG, "diagnostic_eta_delta: I = " + std::to_string(I) + " below 2 M + 1");
+  if (!(dt != 0.0)) return fail(SWX_ERR_ARG, "diagnostic_eta_delta: dt must be nonzero");
+  if (n == 0) return SWX_OK;
+  if (!Um || !Vm || !fmn || !Pmn || !Hmn || !w || !tstepcoeff || !mJarray || !etamn || !deltamn)
+    return fail(SWX_ERR_ARG, "diagnostic_eta_delta: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  SWX_HIP(launch_diag_eta_delta_spec(Um, Vm, fmn, n, J, M, N, Pmn, Hmn, w, tstepcoeff, mJarray, dt, etamn, deltamn,
+                                     s));
+  if (eta && delta) {
+    SWX_HIP(launch_spec_to_grid(etamn, deltamn, n, J, I, M, N, Pmn, eta, delta, s));
+  } else if (eta || delta) {
+    SWX_HIP(launch_spec_to_grid(eta ? etamn : deltamn, nullptr, n, J, I, M, N, Pmn, eta ? eta : delta, nullptr, s));
+  }
   return SWX_OK;
 }
 
