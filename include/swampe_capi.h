@@ -179,11 +179,15 @@ int swx_export_state(swx_plan *plan, const swx_state_out *out, void *stream);
  * threads) race on the array. */
 int swx_export_state_noalloc(swx_plan *plan, const swx_state_out *out, void *stream);
 
-/* Forward-only benchmark driver: run nsteps steps with no history and no
- * export, replaying a captured HIP graph of `graph_steps` steps where
- * possible (graph_steps <= 0 disables graphs).  Same semantics as swx_run
- * with all history pointers NULL.  No reference counterpart beyond the
- * jitted lax.scan it replaces (model.py:1281-1342). */
+/* Forward-only driver: run nsteps steps with no history and no export as
+ * captured HIP graphs (graph_steps <= 0 disables graphs): the steps with
+ * t <= 2 eagerly, then blocks of `graph_steps` steps while more than that
+ * many remain, then the rest of the call (at most graph_steps steps) as one
+ * graph.  The plan keeps up to 8 instantiated graphs keyed by the shape of
+ * their step sequence (least recently replayed replaced), so repeated calls
+ * of one length capture once.  Same semantics as swx_run with all history
+ * pointers NULL.  No reference counterpart beyond the jitted lax.scan it
+ * replaces (model.py:1281-1342). */
 int swx_run_graph(swx_plan *plan, int t0, int nsteps, int graph_steps, void *stream);
 
 /* Mixed truncations in one launch: an ensemble of two initialized FP32 plans of
@@ -191,7 +195,7 @@ int swx_run_graph(swx_plan *plan, int t0, int nsteps, int graph_steps, void *str
  * parameters and tables).  swx_ensemble_run_graph advances both plans by
  * nsteps steps from t0 exactly as swx_run_graph does for each, with one
  * Legendre launch and one grid launch per step covering both plans' members,
- * replayed from a captured HIP graph of `graph_steps` steps.  The plans must
+ * replayed from captured HIP graphs as swx_run_graph replays them.  The plans must
  * outlive the ensemble and are not modified otherwise.  Replaces batching
  * members of different truncations, which the reference cannot do in one
  * vmapped scan (one Static per jit, model.py:1281-1342). */

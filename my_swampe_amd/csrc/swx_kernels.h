@@ -305,6 +305,8 @@ int grid_mfma_k2p(int M);
 
 // One single-wave launch that spins for `ticks` of the device wall clock (bounded by it).
 hipError_t launch_wait_ticks(unsigned long long ticks, hipStream_t s);
+// dst <- src, `bytes` a multiple of 16 and both 16-byte aligned (hipErrorInvalidValue otherwise).
+hipError_t launch_copy_words(void* dst, const void* src, size_t bytes, hipStream_t s);
 template <typename R>
 hipError_t launch_copy_real(double* dst, long long dstride, const R* src, long long sstride, long long n, int B,
                             hipStream_t s);

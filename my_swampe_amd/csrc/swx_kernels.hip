@@ -589,6 +589,13 @@ __global__ void k_copy_real(double* dst, long long dstride, const R* src, long l
     dst[b * dstride + e] = (double)src[b * sstride + e];
 }
 
+// Plain device copy of n16 16-byte words (the previous Fourier rows a call's last step saves; a
+// kernel rather than a copy command, so that captured step sequences hold kernel nodes only).
+__global__ void k_copy_words(uint4* __restrict__ dst, const uint4* __restrict__ src, long long n16) {
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n16; e += (long long)gridDim.x * blockDim.x)
+    dst[e] = src[e];
+}
+
 template <typename R>
 __global__ void k_copy_fourier(double* dst, const R* src, int f, int J, int Mp, const R* res, const double* rc,
                                int lvl) {
@@ -722,6 +729,18 @@ hipError_t launch_copy_real(double* dst, long long dstride, const R* src, long l
   int gx = (int)((n + 255) / 256);
   if (gx > 1024) gx = 1024;
   hipLaunchKernelGGL(k_copy_real<R>, dim3(gx, B), dim3(256), 0, s, dst, dstride, src, sstride, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_words(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return hipSuccess;
+  if (bytes % 16 || (reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) % 16)
+    return hipErrorInvalidValue;
+  const long long n16 = (long long)(bytes / 16);
+  long long gx = (n16 + 255) / 256;
+  if (gx > 2048) gx = 2048;
+  hipLaunchKernelGGL(k_copy_words, dim3((unsigned)gx), dim3(256), 0, s, reinterpret_cast<uint4*>(dst),
+                     reinterpret_cast<const uint4*>(src), n16);
   return hipGetLastError();
 }
 

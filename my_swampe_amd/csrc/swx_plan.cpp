@@ -4,7 +4,7 @@
 // FP64 geometry (host), the basis/twiddle tables on the device in the
 // transform precision, per-member coefficient tables, and the device carry.
 // Stepping is two kernel launches per step (k_leg, k_grid); forward-only
-// stepping (swx_run_graph) replays a captured HIP graph of many steps.
+// stepping (swx_run_graph) replays captured HIP graphs of the step sequence.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -71,6 +71,34 @@ bool build_geometry(int M, bool exact_basis, Geometry& g, const double* qmus = n
 }
 
 }  // namespace
+
+// Captured step sequences of a plan or an ensemble, keyed by their shape: steps, the first
+// grid-carry and the first lineage step of each plan (equal to steps where none), whether the last
+// step saves the previous Fourier rows, and the stepping mode (standalone / in a pipeline).  Every
+// captured step has t > 2, so the same key always records the same launches.  Up to kSlots are
+// kept; the least recently replayed one is replaced.
+struct GraphCache {
+  static constexpr int kSlots = 8, kKey = 7;
+  struct Slot {
+    hipGraphExec_t exec = nullptr;
+    int key[kKey] = {};
+    hipStream_t stream = nullptr;
+    unsigned long long used = 0;
+  };
+  Slot slot[kSlots];
+  unsigned long long clock = 0;
+  void clear() {
+    for (Slot& sl : slot) {
+      if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
+      sl = Slot{};
+    }
+  }
+  int live() const {
+    int n = 0;
+    for (const Slot& sl : slot) n += sl.exec != nullptr;
+    return n;
+  }
+};
 
 struct swx_plan {
   swx_plan_desc d{};
@@ -145,11 +173,8 @@ struct swx_plan {
   void* Gbar = nullptr;
   double* pgrad = nullptr;
   void* adj_stc = nullptr;       // S_t cache of the folded MFMA grid adjoint (T106)
-  // captured forward graph
-  hipGraphExec_t gexec = nullptr;
-  int gexec_steps = 0;
-  hipStream_t gexec_stream = nullptr;
-  bool gexec_shared = false;     // in_pipeline when the graph was captured
+  // captured forward step sequences (swx_run_graph)
+  GraphCache graphs;
   size_t dev_bytes = 0;  // device memory held by the plan
 };
 
@@ -515,7 +540,7 @@ int do_steps(swx_plan* p, int t0, int nsteps, double* hist_fields, double* hist_
   for (int k = 0; k < nsteps; ++k) {
     const int t = t0 + k;
     const bool last = save_last && (k == nsteps - 1);
-    if (last) SWX_HIP(hipMemcpyAsync(p->Gprev, p->G, gbytes, hipMemcpyDeviceToDevice, s));
+    if (last) SWX_HIP(launch_copy_words(p->Gprev, p->G, gbytes, s));
     LegArgs<R> l = leg_args<R>(p, LEG_STEP);
     l.hist_scal = hist_scal;
     l.hist_dead = hist_dead;
@@ -539,6 +564,47 @@ int do_steps(swx_plan* p, int t0, int nsteps, double* hist_fields, double* hist_
     g.hist_k = hist_k0 + k;
     SWX_HIP(launch_grid_any<R>(p, g, s));
   }
+  return SWX_OK;
+}
+
+// Replays the step sequence that `capture` records on stream s, cached under `key`; a sequence
+// the cache does not hold is captured and instantiated first.
+template <class F>
+int graph_replay(GraphCache& gc, const int (&key)[GraphCache::kKey], hipStream_t s, F&& capture) {
+  GraphCache::Slot* hit = nullptr;
+  GraphCache::Slot* victim = nullptr;
+  for (GraphCache::Slot& sl : gc.slot) {
+    if (sl.exec && sl.stream == s && std::equal(key, key + GraphCache::kKey, sl.key)) {
+      hit = &sl;
+      break;
+    }
+    if (!victim || (!sl.exec && victim->exec) || (sl.exec && victim->exec && sl.used < victim->used)) victim = &sl;
+  }
+  if (!hit) {
+    if (victim->exec) {
+      (void)hipGraphExecDestroy(victim->exec);
+      *victim = GraphCache::Slot{};
+    }
+    hipGraph_t graph = nullptr;
+    SWX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rc = capture();
+    hipError_t ec = hipStreamEndCapture(s, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (ec != hipSuccess) return fail(SWX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+    hipGraphExec_t exec = nullptr;
+    ec = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ec != hipSuccess) return fail(SWX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ec));
+    victim->exec = exec;
+    std::copy(key, key + GraphCache::kKey, victim->key);
+    victim->stream = s;
+    hit = victim;
+  }
+  hit->used = ++gc.clock;
+  SWX_HIP(hipGraphLaunch(hit->exec, s));
   return SWX_OK;
 }
 
@@ -770,9 +836,7 @@ struct swx_ensemble {
   swx_plan* p[2] = {nullptr, nullptr};  // p[0]->geo.M < p[1]->geo.M
   bool fuse_grid = true;                // one grid launch for both plans (timed at creation)
   float grid_ms[2] = {0, 0};            // fused, separate
-  hipGraphExec_t gexec = nullptr;
-  int gexec_steps = 0;
-  hipStream_t gexec_stream = nullptr;
+  GraphCache graphs;                    // captured step sequences (swx_ensemble_run_graph)
 };
 
 namespace {
@@ -788,7 +852,7 @@ int ens_steps(swx_ensemble* e, int t0, int nsteps, bool save_last, hipStream_t s
       swx_plan* p = e->p[q];
       if (last) {
         const size_t gbytes = (size_t)p->d.n_members * kNG * p->geo.J * (p->geo.M + 1) * 2 * p->rsz;
-        SWX_HIP(hipMemcpyAsync(p->Gprev, p->G, gbytes, hipMemcpyDeviceToDevice, s));
+        SWX_HIP(launch_copy_words(p->Gprev, p->G, gbytes, s));
       }
       l[q] = leg_args<float>(p, LEG_STEP);
       l[q].do_ra = filter_on(p, t);
@@ -1298,7 +1362,7 @@ int swx_plan_set_params(swx_plan* p, const double* params, void* stream) {
 
 void swx_plan_destroy(swx_plan* p) {
   if (!p) return;
-  if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  p->graphs.clear();
   void* bufs[] = {p->Lam, p->Tbar, p->Gbar, p->pgrad, p->adj_stc, p->Pf, p->Hf, p->Pi, p->Hi, p->fragP[0], p->fragP[1],
                   p->fragP[2], p->fragP[3], p->fragF[0], p->fragF[1], p->fragF[2], p->fragF[3], p->tw, p->TinvF, p->TfwdF, p->TinvH, p->TfwdH, p->TinvB, p->TfwdB, p->geo_d, p->f2, p->coef, p->S, p->T, p->G, p->Gprev,
                   p->raw, p->lag, p->g0uv, p->phys, p->uvic, p->part_uv, p->dead, p->R0f, p->R0p, p->resid_c, p->lagx,
@@ -1388,47 +1452,36 @@ int swx_run_graph(swx_plan* p, int t0, int nsteps, int graph_steps, void* stream
     return do_steps<float>(p, t0 + k0, n, nullptr, nullptr, nullptr, nullptr, 0, 0, save_last, s, pf, lf);
   };
   if (lin) p->phys_stale = true;
-  // Steps with t <= 2 differ (no Robert-Asselin filter) and run eagerly, as do
-  // the steps of the physical-carry window and the last step (which saves the
-  // previous Fourier level).
+  // Steps with t <= 2 differ (no Robert-Asselin filter) and run eagerly.  The rest of the call
+  // replays captured sequences: blocks of graph_steps steps while more than that many remain,
+  // then the call's tail (at most graph_steps steps: the end of the window and the last step,
+  // which saves the previous Fourier rows) as one sequence.  Each sequence is keyed by its shape
+  // relative to its first step (GraphCache), so calls of the same length replay the same graphs.
   int k = 0;
   while (k < nsteps && t0 + k <= 2) {
     if ((rc = steps(k, 1, k == nsteps - 1))) return rc;
     ++k;
   }
-  const int graph_end = std::min(std::min(phys_from, lineage_from), nsteps - 1);
-  if (graph_steps > 0 && graph_end - k >= graph_steps) {
-    if (!p->gexec || p->gexec_steps != graph_steps || p->gexec_stream != s || p->gexec_shared != p->in_pipeline) {
-      if (p->gexec) {
-        (void)hipGraphExecDestroy(p->gexec);
-        p->gexec = nullptr;
-      }
-      hipGraph_t graph = nullptr;
-      SWX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      // Captured steps have t > 2 and lie before the window (phys_on = 0).
+  if (graph_steps <= 0) return k < nsteps ? steps(k, nsteps - k, true) : SWX_OK;
+  const int shared = p->in_pipeline ? 1 : 0;
+  auto replay = [&](int n, bool save_last) {
+    const int pf = std::min(n, std::max(0, phys_from - k)), lf = std::min(n, std::max(0, lineage_from - k));
+    const int key[GraphCache::kKey] = {n, pf, lf, n, n, save_last ? 1 : 0, shared};
+    const int r = graph_replay(p->graphs, key, s, [&] {
       if (p->rsz == 8)
-        rc = do_steps<double>(p, 3, graph_steps, nullptr, nullptr, nullptr, nullptr, 0, 0, false, s, 1 << 30,
-                              1 << 30, true);
-      else
-        rc = do_steps<float>(p, 3, graph_steps, nullptr, nullptr, nullptr, nullptr, 0, 0, false, s, 1 << 30,
-                             1 << 30, true);
-      hipError_t ec = hipStreamEndCapture(s, &graph);
-      if (rc) return rc;
-      if (ec != hipSuccess) return fail(SWX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
-      ec = hipGraphInstantiate(&p->gexec, graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      if (ec != hipSuccess) return fail(SWX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ec));
-      p->gexec_steps = graph_steps;
-      p->gexec_stream = s;
-      p->gexec_shared = p->in_pipeline;
-    }
-    while (graph_end - k >= graph_steps) {
-      SWX_HIP(hipGraphLaunch(p->gexec, s));
-      k += graph_steps;
-      p->nstep[0] += graph_steps;
-    }
-  }
-  if (k < nsteps) return steps(k, nsteps - k, true);
+        return do_steps<double>(p, 3, n, nullptr, nullptr, nullptr, nullptr, 0, 0, save_last, s, pf, lf, true);
+      return do_steps<float>(p, 3, n, nullptr, nullptr, nullptr, nullptr, 0, 0, save_last, s, pf, lf, true);
+    });
+    if (r) return r;
+    p->nstep[0] += n;
+    p->nstep[2] += n - lf;
+    p->nstep[3] += n - pf;
+    k += n;
+    return SWX_OK;
+  };
+  while (nsteps - k > graph_steps)
+    if ((rc = replay(graph_steps, false))) return rc;
+  if (k < nsteps) return replay(nsteps - k, true);
   return SWX_OK;
 }
 
@@ -1485,46 +1538,40 @@ int swx_ensemble_run_graph(swx_ensemble* e, int t0, int nsteps, int graph_steps,
     const int lk[2] = {std::max(0, lf[0] - k0), std::max(0, lf[1] - k0)};
     return ens_steps(e, t0 + k0, n, save_last, s, pk, lk);
   };
-  // As swx_run_graph: t <= 2, the physical-carry windows and the last step run eagerly.
+  // As swx_run_graph: steps with t <= 2 eagerly, then blocks of graph_steps steps and the tail as
+  // captured sequences.
   int k = 0;
   while (k < nsteps && t0 + k <= 2) {
     if ((rc = steps(k, 1, k == nsteps - 1))) return rc;
     ++k;
   }
-  const int graph_end = std::min(std::min(std::min(pf[0], pf[1]), std::min(lf[0], lf[1])), nsteps - 1);
-  if (graph_steps > 0 && graph_end - k >= graph_steps) {
-    if (!e->gexec || e->gexec_steps != graph_steps || e->gexec_stream != s) {
-      if (e->gexec) {
-        (void)hipGraphExecDestroy(e->gexec);
-        e->gexec = nullptr;
-      }
-      hipGraph_t graph = nullptr;
-      SWX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      const int off[2] = {1 << 30, 1 << 30};
-      rc = ens_steps(e, 3, graph_steps, false, s, off, off, true);
-      hipError_t ec = hipStreamEndCapture(s, &graph);
-      if (rc) return rc;
-      if (ec != hipSuccess) return fail(SWX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
-      ec = hipGraphInstantiate(&e->gexec, graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      if (ec != hipSuccess) return fail(SWX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ec));
-      e->gexec_steps = graph_steps;
-      e->gexec_stream = s;
+  if (graph_steps <= 0) return k < nsteps ? steps(k, nsteps - k, true) : SWX_OK;
+  auto replay = [&](int n, bool save_last) {
+    int pk[2], lk[2];
+    for (int q = 0; q < 2; ++q) {
+      pk[q] = std::min(n, std::max(0, pf[q] - k));
+      lk[q] = std::min(n, std::max(0, lf[q] - k));
     }
-    while (graph_end - k >= graph_steps) {
-      SWX_HIP(hipGraphLaunch(e->gexec, s));
-      k += graph_steps;
-      e->p[0]->nstep[0] += graph_steps;
-      e->p[1]->nstep[0] += graph_steps;
+    const int key[GraphCache::kKey] = {n, pk[0], lk[0], pk[1], lk[1], save_last ? 1 : 0, 0};
+    const int r = graph_replay(e->graphs, key, s, [&] { return ens_steps(e, 3, n, save_last, s, pk, lk, true); });
+    if (r) return r;
+    for (int q = 0; q < 2; ++q) {
+      e->p[q]->nstep[0] += n;
+      e->p[q]->nstep[2] += n - lk[q];
+      e->p[q]->nstep[3] += n - pk[q];
     }
-  }
-  if (k < nsteps) return steps(k, nsteps - k, true);
+    k += n;
+    return SWX_OK;
+  };
+  while (nsteps - k > graph_steps)
+    if ((rc = replay(graph_steps, false))) return rc;
+  if (k < nsteps) return replay(nsteps - k, true);
   return SWX_OK;
 }
 
 void swx_ensemble_destroy(swx_ensemble* e) {
   if (!e) return;
-  if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+  e->graphs.clear();
   delete e;
 }
 
@@ -1601,7 +1648,7 @@ int swx_pipeline_create(swx_plan* const* plans, int n, swx_pipeline** out) {
     swx_pipeline_destroy(pl);
     return fail(SWX_ERR_HIP, "pipeline stream or event creation failed");
   }
-  // The plans' captured graphs are re-captured on their next run (swx_run_graph compares the mode).
+  // The plans' graphs of the standalone mode stay cached; swx_run_graph keys its graphs by the mode.
   for (int k = 0; k < n; ++k) plans[k]->in_pipeline = true;
   *out = pl;
   return SWX_OK;
@@ -1700,10 +1747,7 @@ int swx_plan_set_option(swx_plan* p, int option, int value) {
     default:
       return fail(SWX_ERR_ARG, "unknown option " + std::to_string(option));
   }
-  if (p->gexec) {
-    (void)hipGraphExecDestroy(p->gexec);
-    p->gexec = nullptr;
-  }
+  p->graphs.clear();
   return SWX_OK;
 }
 

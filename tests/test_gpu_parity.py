@@ -239,6 +239,52 @@ def test_graph_replay_matches_eager(torch_gpu):
         np.testing.assert_array_equal(outs[0][k], outs[1][k], err_msg=k)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_graph_calls_of_many_lengths_match_eager(precision, torch_gpu):
+    """Calls of the graph driver shorter than graph_steps (one captured sequence each, window and
+    last step included), repeated lengths (cached graphs replayed) and more distinct lengths than
+    the plan caches (least recently replayed graphs replaced): every call's state equals that of
+    eager calls of the same lengths (bitwise), with exports and synchronizations between the calls;
+    the calls after the first step replay graphs only."""
+    from my_swampe_amd.plan import Plan
+    from my_swampe_amd.model import param_row
+
+    torch = torch_gpu
+    p = param_row(dt=120.0, a=8.2e7, omega=3.2e-5, g=9.8, Phibar=3e5, taurad=86400.0, taudrag=86400.0,
+                  DPhieq=3e5, K6=1.24e33, K6Phi=None, alpha=0.01)
+    ics = orc.rest_noise_ic(42, 0, 3.2e-5)
+    lengths = [20, 20, 20, 5, 7, 9, 11, 13, 3, 4, 6, 17, 20, 40, 20]
+    _print_case("graph-lengths", precision=precision)
+    outs = []
+    for use_graph in (False, True):
+        plan = Plan(42, np.stack([p, p]), precision=precision)
+        s = torch.cuda.Stream()
+        got = []
+        with torch.cuda.stream(s):
+            plan.init(*ics, stream=s)
+            plan.run(2, 1, stream=s)
+            t = 3
+            c0 = plan.step_counts()
+            for n in lengths:
+                if use_graph:
+                    plan.run_graph(t, n, graph_steps=16, stream=s)
+                else:
+                    plan.run(t, n, stream=s)
+                t += n
+                ex = plan.export(fields=("Phi_curr", "eta_prev", "U_curr"), spec=True, stream=s)
+                s.synchronize()
+                got.append({k: _np(v) for k, v in ex.items()})
+            c1 = plan.step_counts()
+        if use_graph:
+            assert c1["graph"] - c0["graph"] == sum(lengths), (c0, c1)
+            assert c1["eager"] == c0["eager"], (c0, c1)
+        outs.append(got)
+        plan.close()
+    for i, (a, b) in enumerate(zip(*outs)):
+        for k in a:
+            np.testing.assert_array_equal(a[k], b[k], err_msg=f"call {i} ({lengths[i]} steps): {k}")
+
+
 def test_fp32_transforms_close_to_fp64(torch_gpu):
     """FP32 transforms with the FP64 state update vs the FP64 oracle, 60 forced steps at T42
     (dt = 120 s, a = 8.2e7, omega = 3.2e-5, Phibar = 3e5, DPhieq = 3e5, rest + noise).
