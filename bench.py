@@ -58,6 +58,21 @@ def flops_per_member_step(M: int, J: int, I: int):
     return leg, dft
 
 
+def fold_grid_mfma_flops(M: int, J: int, I: int, B: int) -> int:
+    """Matrix-core flops one launch of the half-period folded grid kernel executes (padded tiles
+    included): per 16-row block, the inverse GEMM's K1 / 4 k-steps x 5 fields x H / 16 longitude
+    tiles and the forward GEMM's 4 waves x H / 16 k-groups x 4 k-steps x NT2 output tiles x 2
+    fields, each a 16x16x4 MFMA of 2048 flops (K1 = even-m and odd-m rows each padded to 4,
+    NT2 = even-m and odd-m output tiles of 16 rows, H = I / 2)."""
+    Mp = M + 1
+    ne, no = (Mp + 1) // 2, Mp // 2
+    k1 = (2 * ne + 3) // 4 * 4 + (2 * no + 3) // 4 * 4
+    nt2 = (2 * ne + 15) // 16 + (2 * no + 15) // 16
+    h16 = I // 2 // 16
+    per_block = (k1 // 4) * 5 * h16 + 4 * h16 * 4 * nt2 * 2
+    return B * J // 16 * per_block * 2048
+
+
 def run_batch(M, B, precision, steps, warmup, graph_steps=32, full_phys=False, si_raw=False, parts=1, reps=1,
               clock_warmup_s=0.0):
     """si_raw: the semi-implicit leapfrog with the RAW filter in its production setting
@@ -511,6 +526,9 @@ def main():
         print(json.dumps({"error": "no ROCm GPU visible"}))
         sys.exit(1)
     M, J, I, B = 42, 64, 128, args.members
+    from my_swampe_amd.plan import measure_ceilings
+
+    ceil = measure_ceilings()
     # One plan of all B members (one Legendre and one grid launch per step): its throughput is
     # `value` and its kernel timings give the roofline.  The same members split into two plans
     # stepped concurrently on two streams (plan.Pipeline) are timed beside it.
@@ -525,6 +543,8 @@ def main():
     grid_tflops = dft_f * B / (main_run["ms_grid"] * 1e-3) / 1e12
     leg_tflops = leg_f * B / (main_run["ms_leg"] * 1e-3) / 1e12
     peak = PEAK[args.precision]
+    peak_measured = {"fp32": ceil["mfma_fp32_tflops"], "fp64": ceil["mfma_fp64_tflops"],
+                     "bf16x3": ceil["mfma_bf16_tflops"] / 3}[args.precision]
     grid_kernel = {"fp32": f"k_grid_mfma<{M}>", "bf16x3": f"k_grid_bf16x3<{M}>", "fp64": "k_grid<double>"}[args.precision]
     if args.precision == "fp32" and main_run["fold"] == 2:
         grid_kernel = f"k_grid_mfma_fold_w2<{M}, true>"
@@ -559,9 +579,26 @@ def main():
                      "leg_member_group": main_run["leg_mb"],
                      "algorithmic_flops_per_launch": dft_f * B,
                      "kernel_ms": main_run["ms_grid"], "leg_kernel_ms": main_run["ms_leg"],
-                     "leg_tflops": leg_tflops},
+                     "leg_tflops": leg_tflops,
+                     "peak_measured": peak_measured, "frac_of_measured": grid_tflops / peak_measured},
+        "ceilings_measured": dict(ceil, how="swx_measure_ceilings: 8 independent accumulators per wave, "
+                                  "8 waves per SIMD; HBM passes over 1 GiB; shortest of 5 launches"),
         "finite": main_run["finite"] and pipe_run["finite"],
     }
+    leg_kernel = f"k_leg_mfma_w5<{M}, {main_run['leg_mb']}, false, false, true>"
+    hbm = {}
+    for name, kern, ms in (("grid", grid_kernel, main_run["ms_grid"]), ("legendre", leg_kernel, main_run["ms_leg"])):
+        tb = measured_traffic(kern, B, args.precision)
+        if tb is not None and ms:
+            rate = tb / (ms * 1e-3) / 1e12
+            hbm[name] = {"kernel": kern, "traffic": tb, "tbps": rate, "frac_of_peak": rate / (HBM_PEAK / 1000),
+                         "frac_of_measured_copy": rate / ceil["hbm_copy_tbps"]}
+    out["roofline"]["hbm"] = hbm
+    if grid_kernel.startswith("k_grid_mfma_fold"):
+        ex = fold_grid_mfma_flops(M, J, I, B)
+        ex_tflops = ex / (main_run["ms_grid"] * 1e-3) / 1e12
+        out["roofline"]["executed"] = {"flops_per_launch": ex, "tflops": ex_tflops, "frac_of_peak": ex_tflops / peak,
+                                       "frac_of_measured": ex_tflops / peak_measured}
     if not args.no_extra:
         full = run_batch(M, B, args.precision, max(args.steps // 2, 100), 10, full_phys=True)
         fp64 = run_batch(M, B, "fp64", max(args.steps // 4, 50), 10)

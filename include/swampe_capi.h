@@ -241,6 +241,16 @@ int swx_plan_step_counts(const swx_plan *plan, long long *counts);
  * scripts/benchmark_scan.py:116-141). */
 int swx_time_kernels(swx_plan *plan, int t0, int nsteps, float *ms_leg, float *ms_grid, void *stream);
 
+/* Measurement hook: ceilings of the current GPU measured with the instructions and access
+ * patterns the step kernels use.  out[0..2] = dense matrix-core throughput (TFLOP/s) of
+ * v_mfma_f32_16x16x4_f32, v_mfma_f64_16x16x4_f64 and v_mfma_f32_16x16x16_bf16 (8 independent
+ * accumulators per wave, 8 waves per SIMD); out[3..5] = HBM bandwidth (TB/s) of a device copy
+ * (bytes read + written), a read-only pass and a write-only pass over `bytes` bytes (a multiple of
+ * 16, at least 256 MiB; two such buffers are allocated and freed here).  Shortest of 5 timed
+ * launches after 2 untimed ones, HIP events on `stream`, which is synchronized.  No reference
+ * counterpart: the measured ceilings the step kernels' roofline fractions are quoted against. */
+int swx_measure_ceilings(double *out, size_t bytes, void *stream);
+
 /* ---- Spectral initial state and checkpoints ------------------------------
  * State arrays are device complex128 [B][3][M+1][N+1] (eta, delta, Phi
  * spectral coefficients, n >= m used). */

@@ -44,7 +44,7 @@ EXPORTED_SYMBOLS = (
     "swx_adjoint_step", "swx_adjoint_result", "swx_last_error", "swx_version", "swx_plan_device_bytes",
     "swx_plan_step_counts", "swx_export_state_noalloc", "swx_ensemble_create", "swx_ensemble_run_graph", "swx_ensemble_destroy", "swx_plan_get_option",
     "swx_pipeline_create", "swx_pipeline_run_graph", "swx_pipeline_destroy", "swx_run_tape", "swx_adjoint_sweep",
-    "swx_diagnostic_eta_delta",
+    "swx_diagnostic_eta_delta", "swx_measure_ceilings",
 )
 
 STATE_FIELDS = (
@@ -123,6 +123,7 @@ def load() -> ctypes.CDLL:
     lib.swx_diagnostic_eta_delta.argtypes = ([c_void_p] * 3 + [c_int] * 5 + [c_void_p] * 5 + [ctypes.c_double] +
                                              [c_void_p] * 5)
     lib.swx_time_kernels.argtypes = [c_void_p, c_int, c_int, POINTER(ctypes.c_float), POINTER(ctypes.c_float), c_void_p]
+    lib.swx_measure_ceilings.argtypes = [c_void_p, ctypes.c_size_t, c_void_p]
     lib.swx_plan_device_bytes.argtypes = [c_void_p, POINTER(ctypes.c_size_t)]
     lib.swx_plan_step_counts.argtypes = [c_void_p, POINTER(ctypes.c_longlong)]
     lib.swx_export_state_noalloc.argtypes = lib.swx_export_state.argtypes
@@ -145,7 +146,7 @@ def load() -> ctypes.CDLL:
                  "swx_ensemble_create", "swx_ensemble_run_graph", "swx_plan_get_option", "swx_adjoint_observe",
                  "swx_phase_curve", "swx_phase_curve_vjp", "swx_plan_phase_curve", "swx_set_prev_level",
                  "swx_pipeline_create", "swx_pipeline_run_graph", "swx_run_tape", "swx_adjoint_sweep",
-                 "swx_diagnostic_eta_delta"):
+                 "swx_diagnostic_eta_delta", "swx_measure_ceilings"):
         getattr(lib, name).restype = c_int
     _lib = lib
     return lib

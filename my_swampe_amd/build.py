@@ -20,7 +20,7 @@ LIB_DIR = PKG / "_lib"
 LIB = LIB_DIR / "libswampe_hip.so"
 
 HIP_SOURCES = ["swx_kernels.hip", "swx_mfma.hip", "swx_bf16x3.hip", "swx_adjoint.hip", "swx_adjoint_mfma.hip",
-               "swx_adjoint_leg_mfma.hip", "swx_observe.hip", "swx_transforms.hip"]
+               "swx_adjoint_leg_mfma.hip", "swx_observe.hip", "swx_transforms.hip", "swx_peaks.hip"]
 HOST_SOURCES = ["swx_plan.cpp", "geometry.cpp"]
 
 # Packed FP32 VALU forms are disabled for all device code.

@@ -305,6 +305,9 @@ int grid_mfma_k2p(int M);
 
 // One single-wave launch that spins for `ticks` of the device wall clock (bounded by it).
 hipError_t launch_wait_ticks(unsigned long long ticks, hipStream_t s);
+// Measured ceilings (swx_peaks.hip): out[0..2] matrix-core TFLOP/s (FP32 16x16x4, FP64 16x16x4,
+// BF16 16x16x16), out[3..5] HBM TB/s (copy: bytes read + written; read-only; write-only).
+hipError_t measure_ceilings(double* out, size_t bytes, hipStream_t s);
 // dst <- src, `bytes` a multiple of 16 and both 16-byte aligned (hipErrorInvalidValue otherwise).
 hipError_t launch_copy_words(void* dst, const void* src, size_t bytes, hipStream_t s);
 template <typename R>

@@ -966,6 +966,15 @@ extern "C" {
 
 const char* swx_last_error(void) { return g_err.c_str(); }
 
+int swx_measure_ceilings(double* out, size_t bytes, void* stream) {
+  if (!out) return fail(SWX_ERR_ARG, "null output");
+  if (bytes < ((size_t)256 << 20) || bytes % 16 != 0)
+    return fail(SWX_ERR_ARG, "swx_measure_ceilings: bytes must be a multiple of 16 and at least 256 MiB");
+  const hipError_t e = measure_ceilings(out, bytes, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(SWX_ERR_HIP, std::string("swx_measure_ceilings: ") + hipGetErrorString(e));
+  return SWX_OK;
+}
+
 const char* swx_version(void) { return "swampe-hip 0.1.0 (gfx950)"; }
 
 int swx_grid_size(int M, int* J, int* I) {

@@ -430,3 +430,20 @@ def split_bounds(B: int, parts: int):
     if not 1 <= parts <= min(B, 4):
         raise ValueError("parts must be between 1 and min(members, 4)")
     return [int(x) for x in np.linspace(0, B, parts + 1).round().astype(int)]
+
+
+def measure_ceilings(nbytes: int = 1 << 30, stream=None) -> Dict[str, float]:
+    """Ceilings of the current GPU measured with the step kernels' instructions and access patterns
+    (swx_measure_ceilings): matrix-core TFLOP/s of the FP32 16x16x4, FP64 16x16x4 and BF16 16x16x16
+    MFMA instructions, and HBM TB/s of a device copy (bytes read + written), a read-only and a
+    write-only pass over `nbytes` bytes."""
+    torch = _torch()
+    require_gpu()
+    lib = _capi.load()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    out = (ctypes.c_double * 6)()
+    _capi.check(lib.swx_measure_ceilings(out, ctypes.c_size_t(int(nbytes)), ctypes.c_void_p(s.cuda_stream)),
+                "swx_measure_ceilings")
+    keys = ("mfma_fp32_tflops", "mfma_fp64_tflops", "mfma_bf16_tflops", "hbm_copy_tbps", "hbm_read_tbps",
+            "hbm_write_tbps")
+    return {k: float(v) for k, v in zip(keys, out)}

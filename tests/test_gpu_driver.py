@@ -100,3 +100,18 @@ def test_closing_a_plan_closes_its_pipelines_and_ensembles(torch_gpu):
     with pytest.raises(Exception):
         pipe.run_graph(2, 3, stream=torch.cuda.Stream())
     c.close()
+
+
+def test_measured_ceilings_are_plausible(torch_gpu):
+    """swx_measure_ceilings: positive figures below the datasheet ceilings plus a margin (FP32 matrix
+    157.3, FP64 matrix 78.6, BF16 matrix 2500 TFLOP/s; HBM 8 TB/s), above a tenth of them; the copy
+    counts its bytes twice; sizes below 256 MiB are refused before any device work."""
+    from my_swampe_amd.plan import measure_ceilings
+
+    c = measure_ceilings(1 << 28)
+    print("[gpu-case] ceilings", c, flush=True)
+    for key, sheet in (("mfma_fp32_tflops", 157.3), ("mfma_fp64_tflops", 78.6), ("mfma_bf16_tflops", 2500.0),
+                       ("hbm_copy_tbps", 8.0), ("hbm_read_tbps", 8.0), ("hbm_write_tbps", 8.0)):
+        assert 0.1 * sheet < c[key] < 1.1 * sheet, (key, c[key])
+    with pytest.raises(ValueError, match="256 MiB"):
+        measure_ceilings(1 << 20)
