@@ -175,6 +175,7 @@ struct swx_plan {
   void* adj_stc = nullptr;       // S_t cache of the folded MFMA grid adjoint (T106)
   // captured forward step sequences (swx_run_graph)
   GraphCache graphs;
+  unsigned opt_epoch = 0;        // swx_plan_set_option calls so far (ensembles re-capture on a change)
   size_t dev_bytes = 0;  // device memory held by the plan
 };
 
@@ -837,6 +838,7 @@ struct swx_ensemble {
   bool fuse_grid = true;                // one grid launch for both plans (timed at creation)
   float grid_ms[2] = {0, 0};            // fused, separate
   GraphCache graphs;                    // captured step sequences (swx_ensemble_run_graph)
+  unsigned epoch[2] = {0, 0};           // the plans' opt_epoch when they were captured
 };
 
 namespace {
@@ -1555,6 +1557,12 @@ int swx_ensemble_run_graph(swx_ensemble* e, int t0, int nsteps, int graph_steps,
     ++k;
   }
   if (graph_steps <= 0) return k < nsteps ? steps(k, nsteps - k, true) : SWX_OK;
+  // A plan option changed since the graphs were captured (kernel choices): capture again.
+  if (e->epoch[0] != e->p[0]->opt_epoch || e->epoch[1] != e->p[1]->opt_epoch) {
+    e->graphs.clear();
+    e->epoch[0] = e->p[0]->opt_epoch;
+    e->epoch[1] = e->p[1]->opt_epoch;
+  }
   auto replay = [&](int n, bool save_last) {
     int pk[2], lk[2];
     for (int q = 0; q < 2; ++q) {
@@ -1757,6 +1765,7 @@ int swx_plan_set_option(swx_plan* p, int option, int value) {
       return fail(SWX_ERR_ARG, "unknown option " + std::to_string(option));
   }
   p->graphs.clear();
+  ++p->opt_epoch;
   return SWX_OK;
 }
 

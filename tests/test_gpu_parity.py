@@ -462,6 +462,57 @@ def _noise_spectral(M, B, seed, omega):
     return c
 
 
+def test_ensemble_recaptures_after_plan_option_change(torch_gpu):
+    """An option set on an ensemble's plan after the ensemble captured its step graphs (here the
+    Legendre contractions over all latitudes instead of the folded ones, which round differently)
+    takes effect in the ensemble's next call of the same length: bitwise the states the two plans
+    reach stepped separately with the same option change between the calls."""
+    from my_swampe_amd import _capi
+    from my_swampe_amd.model import param_row
+    from my_swampe_amd.plan import Ensemble, Plan
+
+    torch = torch_gpu
+    _print_case("ensemble-option-change")
+    omega = 3.2e-5
+    specs = []
+    for M, B in ((42, 3), (63, 2)):
+        rng = np.random.default_rng(M + 1)
+        params = np.stack([param_row(dt=120.0, a=8.2e7, omega=omega, g=9.8, Phibar=3e5,
+                                     taurad=10 ** rng.uniform(4, 6), taudrag=10 ** rng.uniform(4, 6),
+                                     DPhieq=rng.uniform(1e5, 5e5), K6=1.24e33, K6Phi=None, alpha=0.01)
+                           for _ in range(B)])
+        specs.append((M, params, _noise_spectral(M, B, M + 1, omega)))
+    stream = torch.cuda.Stream()
+    out = {}
+    for mode in ("separate", "ensemble"):
+        with torch.cuda.stream(stream):
+            plans = []
+            for M, params, c in specs:
+                pl = Plan(M, params, precision="fp32")
+                pl.init_spectral(torch.tensor(c, device="cuda"), stream=stream)
+                plans.append(pl)
+            ens = Ensemble(plans[0], plans[1]) if mode == "ensemble" else None
+            for t0, fold in ((2, 2), (22, 1)):
+                for pl in plans:
+                    pl.set_option(_capi.OPT_LEG_FOLD, fold)
+                if ens is None:
+                    for pl in plans:
+                        pl.run_graph(t0, 20, graph_steps=8, stream=stream)
+                else:
+                    ens.run_graph(t0, 20, graph_steps=8, stream=stream)
+            res = [pl.export(fields=("Phi_curr", "etam_curr"), spec=True, stream=stream) for pl in plans]
+            stream.synchronize()
+            if ens is not None:
+                ens.close()
+            for pl in plans:
+                pl.close()
+        out[mode] = res
+    for q in range(2):
+        for k in ("Phi_curr", "etam_curr", "spec"):
+            a, b = out["separate"][q][k], out["ensemble"][q][k]
+            assert torch.equal(a, b), (q, k)
+
+
 @pytest.mark.parametrize("pair", [(42, 63), (42, 106)])
 def test_mixed_truncation_ensemble_matches_separate_plans(pair, torch_gpu):
     """Two FP32 plans of different truncations stepped together (one Legendre and one grid launch
