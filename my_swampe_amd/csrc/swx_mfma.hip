@@ -274,16 +274,20 @@ __device__ __forceinline__ void grid_mfma_fold_body(const GridArgs<float>& a, co
   double diag[4] = {0.0, DBL_MAX, DBL_MAX, -DBL_MAX};
 
   // Phase 0: Fourier rows -> X^T in the folded k order.  Slot s < K1 / 2 is the row pair
-  // (2s, 2s + 1) = (re, im) of m = 2s (s < KE4 / 2) or m = 2 (s - KE4 / 2) + 1.
+  // (2s, 2s + 1) = (re, im) of m = 2s (s < KE4 / 2) or m = 2 (s - KE4 / 2) + 1.  The steady
+  // variant stages eta, Phi, U, V only: its products do not read delta (field 1), whose grid
+  // field only the physical carry and the per-step outputs of the bookkeeping variant need.
+  constexpr int NF = FAST ? 4 : 5;
   {
-    constexpr int NE = 5 * (K1 / 2) * 8;
+    constexpr int NE = NF * (K1 / 2) * 8;
     constexpr int NQ = (NE + 255) / 256;
     v4f gv[NQ];
 #pragma unroll
     for (int u = 0; u < NQ; ++u) {
       const int e = tid + 256 * u;
       const int jq = e & 7, fs = e >> 3;
-      const int f = fs / (K1 / 2) < 5 ? fs / (K1 / 2) : 4, sl = fs % (K1 / 2);
+      const int fi = fs / (K1 / 2) < NF ? fs / (K1 / 2) : NF - 1, sl = fs % (K1 / 2);
+      const int f = FAST && fi > 0 ? fi + 1 : fi;
       const int m = sl < KE4 / 2 ? 2 * sl : 2 * (sl - KE4 / 2) + 1;
       const v4f v = *reinterpret_cast<const v4f*>(a.G + fidx(b, f, m < Mp ? m : Mp - 1, j0 + 2 * jq, Mp, J));
       gv[u] = (e < NE && m < Mp) ? v : v4f{0.f, 0.f, 0.f, 0.f};
@@ -293,7 +297,8 @@ __device__ __forceinline__ void grid_mfma_fold_body(const GridArgs<float>& a, co
       const int e = tid + 256 * u;
       if (e < NE) {
         const int jq = e & 7, fs = e >> 3;
-        const int f = fs / (K1 / 2), sl = fs % (K1 / 2);
+        const int fi = fs / (K1 / 2), sl = fs % (K1 / 2);
+        const int f = FAST && fi > 0 ? fi + 1 : fi;
         float* xr = sXT + (f * K1 + 2 * sl) * 16 + 2 * jq;
         *reinterpret_cast<float2*>(xr) = make_float2(gv[u][0], gv[u][2]);
         *reinterpret_cast<float2*>(xr + 16) = make_float2(gv[u][1], gv[u][3]);
@@ -331,6 +336,7 @@ __device__ __forceinline__ void grid_mfma_fold_body(const GridArgs<float>& a, co
         const float* xb = sXT + (4 * ks + lg) * 16 + ln;
 #pragma unroll
         for (int f = 0; f < 5; ++f) {
+          if (FAST && f == 1) continue;  // delta: not staged, not used by the steady products
           if (ks < KE4 / 4)
             cE[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(afr[ks / 4][ks % 4], xb[f * K1 * 16], cE[f], 0, 0, 0);
           else
