@@ -325,6 +325,19 @@ __device__ __forceinline__ void grid_mfma_fold_body(const GridArgs<float>& a, co
     for (int f = 0; f < 5; ++f) cE[f] = cO[f] = v4f{0.f, 0.f, 0.f, 0.f};
     const bool active = wave < ntc;
     const int ib = ic + 16 * wave;
+    // Steady variant: the day-side cosines of the lane's 8 longitudes (i, i + H for its 4 points),
+    // loaded now so that they arrive under GEMM 1.
+    float dcp[FAST ? 4 : 1][2];
+    if constexpr (FAST) {
+      if (active) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int i = ic + 16 * wave + 4 * lg + q;
+          dcp[q][0] = (float)geo[gl.dcos + i];
+          dcp[q][1] = (float)geo[gl.dcos + i + H];
+        }
+      }
+    }
     if (active) {
       v4f afr[KQ1 / 4];
       const v4f* Ta = reinterpret_cast<const v4f*>(TinvF + ((long long)(ib / 16) * 64 + lane) * KQ1);
@@ -357,12 +370,11 @@ __device__ __forceinline__ void grid_mfma_fold_body(const GridArgs<float>& a, co
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int il = 16 * wave + 4 * lg + q;
-          const int i = ic + il;
           float pa[8], pb[8];
-          grid_point_step(a, geo, gl, pc, i, cE[0][q] + cO[0][q], cE[2][q] + cO[2][q], cE[3][q] + cO[3][q],
-                          cE[4][q] + cO[4][q], pa);
-          grid_point_step(a, geo, gl, pc, i + H, cE[0][q] - cO[0][q], cE[2][q] - cO[2][q], cE[3][q] - cO[3][q],
-                          cE[4][q] - cO[4][q], pb);
+          grid_products_f32(pc, a.test, dcp[q][0], cE[0][q] + cO[0][q], cE[2][q] + cO[2][q], cE[3][q] + cO[3][q],
+                            cE[4][q] + cO[4][q], pa);
+          grid_products_f32(pc, a.test, dcp[q][1], cE[0][q] - cO[0][q], cE[2][q] - cO[2][q], cE[3][q] - cO[3][q],
+                            cE[4][q] - cO[4][q], pb);
 #pragma unroll
           for (int f = 0; f < 8; ++f) {
             sS[(f * ICH + il) * 16 + n] = pa[f] + pb[f];
