@@ -59,17 +59,18 @@ def flops_per_member_step(M: int, J: int, I: int):
 
 
 def fold_grid_mfma_flops(M: int, J: int, I: int, B: int) -> int:
-    """Matrix-core flops one launch of the half-period folded grid kernel executes (padded tiles
-    included): per 16-row block, the inverse GEMM's K1 / 4 k-steps x 5 fields x H / 16 longitude
-    tiles and the forward GEMM's 4 waves x H / 16 k-groups x 4 k-steps x NT2 output tiles x 2
-    fields, each a 16x16x4 MFMA of 2048 flops (K1 = even-m and odd-m rows each padded to 4,
-    NT2 = even-m and odd-m output tiles of 16 rows, H = I / 2)."""
+    """Matrix-core flops one launch of the steady half-period folded grid kernel executes (padded
+    tiles included): per 16-row block, the inverse GEMM's K1 / 4 k-steps x 4 fields (eta, Phi, U,
+    V: the steady products do not read delta) x H / 16 longitude tiles and the forward GEMM's 4
+    waves x H / 16 k-groups x 4 k-steps x NT2 output tiles x 2 fields, each a 16x16x4 MFMA of 2048
+    flops (K1 = even-m and odd-m rows each padded to 4, NT2 = even-m and odd-m output tiles of 16
+    rows, H = I / 2).  T42: 1120 MFMAs per block, 280 per wave, as SQ_INSTS_MFMA counts them."""
     Mp = M + 1
     ne, no = (Mp + 1) // 2, Mp // 2
     k1 = (2 * ne + 3) // 4 * 4 + (2 * no + 3) // 4 * 4
     nt2 = (2 * ne + 15) // 16 + (2 * no + 15) // 16
     h16 = I // 2 // 16
-    per_block = (k1 // 4) * 5 * h16 + 4 * h16 * 4 * nt2 * 2
+    per_block = (k1 // 4) * 4 * h16 + 4 * h16 * 4 * nt2 * 2
     return B * J // 16 * per_block * 2048
 
 
