@@ -84,14 +84,22 @@ struct GraphCache {
     int key[kKey] = {};
     hipStream_t stream = nullptr;
     unsigned long long used = 0;
+    hipEvent_t done = nullptr;  // recorded after each launch of exec
   };
   Slot slot[kSlots];
   unsigned long long clock = 0;
-  void clear() {
-    for (Slot& sl : slot) {
-      if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
-      sl = Slot{};
+  // A graph is destroyed only once its last launch has completed (its kernel arguments are the
+  // executable graph's own memory).
+  static void release(Slot& sl) {
+    if (sl.exec) {
+      if (sl.done) (void)hipEventSynchronize(sl.done);
+      (void)hipGraphExecDestroy(sl.exec);
     }
+    if (sl.done) (void)hipEventDestroy(sl.done);
+    sl = Slot{};
+  }
+  void clear() {
+    for (Slot& sl : slot) release(sl);
   }
   int live() const {
     int n = 0;
@@ -582,10 +590,7 @@ int graph_replay(GraphCache& gc, const int (&key)[GraphCache::kKey], hipStream_t
     if (!victim || (!sl.exec && victim->exec) || (sl.exec && victim->exec && sl.used < victim->used)) victim = &sl;
   }
   if (!hit) {
-    if (victim->exec) {
-      (void)hipGraphExecDestroy(victim->exec);
-      *victim = GraphCache::Slot{};
-    }
+    GraphCache::release(*victim);
     hipGraph_t graph = nullptr;
     SWX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     const int rc = capture();
@@ -606,6 +611,8 @@ int graph_replay(GraphCache& gc, const int (&key)[GraphCache::kKey], hipStream_t
   }
   hit->used = ++gc.clock;
   SWX_HIP(hipGraphLaunch(hit->exec, s));
+  if (!hit->done) SWX_HIP(hipEventCreateWithFlags(&hit->done, hipEventDisableTiming));
+  SWX_HIP(hipEventRecord(hit->done, s));
   return SWX_OK;
 }
 

@@ -285,6 +285,41 @@ def test_graph_calls_of_many_lengths_match_eager(precision, torch_gpu):
             np.testing.assert_array_equal(a[k], b[k], err_msg=f"call {i} ({lengths[i]} steps): {k}")
 
 
+def test_graph_eviction_without_synchronization_matches_eager(torch_gpu):
+    """Calls of 12 distinct lengths issued back to back with no synchronization (more step-sequence
+    shapes than the plan caches, so graphs are replaced while earlier replays may still run): the
+    final state equals that of eager calls of the same lengths (bitwise)."""
+    from my_swampe_amd.plan import Plan
+    from my_swampe_amd.model import param_row
+
+    torch = torch_gpu
+    p = param_row(dt=120.0, a=8.2e7, omega=3.2e-5, g=9.8, Phibar=3e5, taurad=86400.0, taudrag=86400.0,
+                  DPhieq=3e5, K6=1.24e33, K6Phi=None, alpha=0.01)
+    ics = orc.rest_noise_ic(42, 0, 3.2e-5)
+    lengths = [3, 5, 7, 9, 11, 13, 6, 4, 8, 10, 12, 14, 3, 5, 7]
+    _print_case("graph-eviction")
+    outs = []
+    for use_graph in (False, True):
+        plan = Plan(42, np.stack([p] * 8), precision="fp32")
+        s = torch.cuda.Stream()
+        with torch.cuda.stream(s):
+            plan.init(*ics, stream=s)
+            plan.run(2, 1, stream=s)
+            t = 3
+            for n in lengths:
+                if use_graph:
+                    plan.run_graph(t, n, graph_steps=16, stream=s)
+                else:
+                    plan.run(t, n, stream=s)
+                t += n
+            ex = plan.export(fields=("Phi_curr", "eta_prev", "U_curr"), spec=True, stream=s)
+            s.synchronize()
+        outs.append({k: _np(v) for k, v in ex.items()})
+        plan.close()
+    for k in outs[0]:
+        np.testing.assert_array_equal(outs[0][k], outs[1][k], err_msg=k)
+
+
 def test_fp32_transforms_close_to_fp64(torch_gpu):
     """FP32 transforms with the FP64 state update vs the FP64 oracle, 60 forced steps at T42
     (dt = 120 s, a = 8.2e7, omega = 3.2e-5, Phibar = 3e5, DPhieq = 3e5, rest + noise).
