@@ -134,3 +134,18 @@ def test_product_path_fails_loudly_without_gpu():
         pytest.skip("a GPU is visible")
     with pytest.raises(RuntimeError, match="no CPU fallback"):
         _run(tmax=4)
+
+
+def test_measure_ceilings_refuses_small_or_ragged_sizes():
+    """swx_measure_ceilings checks its arguments before any device work: fewer than 256 MiB, a size
+    that is not a multiple of 16 and a null output are SWX_ERR_ARG with a message."""
+    import ctypes
+
+    from my_swampe_amd import _capi
+
+    lib = _capi.load()
+    out = (ctypes.c_double * 6)()
+    assert lib.swx_measure_ceilings(out, ctypes.c_size_t(1 << 20), None) == _capi.SWX_ERR_ARG
+    assert b"256 MiB" in lib.swx_last_error()
+    assert lib.swx_measure_ceilings(out, ctypes.c_size_t((256 << 20) + 8), None) == _capi.SWX_ERR_ARG
+    assert lib.swx_measure_ceilings(None, ctypes.c_size_t(1 << 30), None) == _capi.SWX_ERR_ARG
