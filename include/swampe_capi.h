@@ -118,6 +118,19 @@ typedef struct swx_state_out {
  * the exactly orthonormal basis everywhere.  SWX_ERR_ARG for an unsupported M. */
 int swx_geometry(int M, int exact_basis, double *mus, double *w, double *lambdas, double *Pmn, double *Hmn);
 
+/* Host-only Gauss-Legendre nodes (ascending) and weights of order J >= 1: replaces
+ * gauss_legendre (src/my_swampe/spectral_transform.py:80-114).  mus[J], w[J]. */
+int swx_gauss_legendre(int J, double *mus, double *w);
+
+/* Host-only longitudes linspace(-pi, pi, I, endpoint=False): replaces build_lambdas
+ * (src/my_swampe/spectral_transform.py:54-77).  lambdas[I], I >= 1. */
+int swx_lambdas(int I, double *lambdas);
+
+/* Host-only normalized associated Legendre basis and its derivative term at J arbitrary nodes
+ * mus (each in [-1, 1]): replaces PmnHmn (src/my_swampe/spectral_transform.py:219-287).
+ * Pmn, Hmn [J][M+1][N+1]; exact_basis as in swx_geometry. */
+int swx_legendre_basis(int J, int M, int N, const double *mus, int exact_basis, double *Pmn, double *Hmn);
+
 /* Grid sizes for truncation M: *J, *I (either may be NULL). */
 int swx_grid_size(int M, int *J, int *I);
 
@@ -327,11 +340,46 @@ int swx_plan_phase_curve(swx_plan *plan, const double *phases, int K, int normal
  * complex; Pmn, Hmn (J, M+1, N+1); w (J); tstepcoeff, mJarray (J, M+1); dt.  Outputs (the
  * reference's return tuple): eta, delta (n, J, I) complex, the grids of invrs_fft(invrs_leg(.))
  * (either may be null), etamn (= zeta_mn + fmn), deltamn (n, M+1, N+1) complex.  Errors:
- * SWX_ERR_ARG for negative n, N < M, I < 2 M + 1 with a grid requested, dt = 0, null inputs. */
+ * SWX_ERR_ARG for negative n or n above 65535, N < M, I < 2 M + 1 or I above 2048 with a grid
+ * requested, dt = 0, null inputs. */
 int swx_diagnostic_eta_delta(const double *Um, const double *Vm, const double *fmn, int n, int I, int J, int M, int N,
                              const double *Pmn, const double *Hmn, const double *w, const double *tstepcoeff,
                              const double *mJarray, double dt, double *eta, double *delta, double *etamn,
                              double *deltamn, void *stream);
+
+/* The reference's standalone transforms on device FP64 arrays (complex = interleaved re, im),
+ * K independent batch rows each (K <= 65535), on `stream`.  Errors: SWX_ERR_ARG for negative
+ * or oversized counts, a longitude count I outside [1, 2048] where a DFT runs, null pointers.
+ *
+ * fwd_fft_trunc / fwd_fft_trunc_batch (src/my_swampe/spectral_transform.py:290-328): data
+ * (K, J, C) real, or complex with is_complex = 1 -> out (K, J, min(M+1, I)) complex,
+ * fft(data / I, n=I) truncated to m = 0..M (rows zero-padded or cut to I columns). */
+int swx_fwd_fft_trunc(const double *data, int is_complex, int K, int J, int C, int I, int M, double *out,
+                      void *stream);
+
+/* invrs_fft (src/my_swampe/spectral_transform.py:331-348): X (K, J, C) complex ->
+ * ifft(I X, n=I) (K, J, I) complex. */
+int swx_invrs_fft(const double *X, int K, int J, int C, int I, double *out, void *stream);
+
+/* fwd_leg (src/my_swampe/spectral_transform.py:405-435) with w (J), or fwd_leg_w /
+ * fwd_leg_w_batch (:370-402) with w = NULL and a preweighted basis: data (K, J, M+1) complex,
+ * basis (J, M+1, N+1) -> out (K, M+1, N+1) complex. */
+int swx_fwd_leg(const double *data, const double *basis, const double *w, int K, int J, int M, int N, double *out,
+                void *stream);
+
+/* invrs_leg (src/my_swampe/spectral_transform.py:438-481): S (K, M+1, N+1) complex ->
+ * (K, J, I) complex, +m at column m and the conjugates at column I - m; I >= M + 1,
+ * M + 1 <= 4096. */
+int swx_invrs_leg(const double *S, const double *basis, int K, int I, int J, int M, int N, double *out,
+                  void *stream);
+
+/* invrsUV / invrsUV_with_coeffs (src/my_swampe/spectral_transform.py:484-596): deltamn, etamn
+ * (K, M+1, N+1) complex, fmn (M+1, N+1) complex, Pmn, Hmn (J, M+1, N+1), tstepcoeffmn,
+ * marray (M+1, N+1) -> U, V (K, J, I) complex and the positive-m coefficients Um, Vm
+ * (K, J, M+1) complex; any output may be NULL (not all four).  Needs N >= M, I >= 2 M + 1. */
+int swx_invrs_uv(const double *deltamn, const double *etamn, const double *fmn, int K, int I, int J, int M, int N,
+                 const double *Pmn, const double *Hmn, const double *tstepcoeffmn, const double *marray, double *U,
+                 double *V, double *Um, double *Vm, void *stream);
 
 /* Plan options (swx_plan_set_option).
  *   SWX_OPT_GRID_KERNEL: 0 = automatic (MFMA grid kernel for FP32 forward

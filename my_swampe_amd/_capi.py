@@ -44,7 +44,8 @@ EXPORTED_SYMBOLS = (
     "swx_adjoint_step", "swx_adjoint_result", "swx_last_error", "swx_version", "swx_plan_device_bytes",
     "swx_plan_step_counts", "swx_export_state_noalloc", "swx_ensemble_create", "swx_ensemble_run_graph", "swx_ensemble_destroy", "swx_plan_get_option",
     "swx_pipeline_create", "swx_pipeline_run_graph", "swx_pipeline_destroy", "swx_run_tape", "swx_adjoint_sweep",
-    "swx_diagnostic_eta_delta", "swx_measure_ceilings",
+    "swx_diagnostic_eta_delta", "swx_measure_ceilings", "swx_gauss_legendre", "swx_lambdas", "swx_legendre_basis",
+    "swx_fwd_fft_trunc", "swx_invrs_fft", "swx_fwd_leg", "swx_invrs_leg", "swx_invrs_uv",
 )
 
 STATE_FIELDS = (
@@ -124,6 +125,14 @@ def load() -> ctypes.CDLL:
                                              [c_void_p] * 5)
     lib.swx_time_kernels.argtypes = [c_void_p, c_int, c_int, POINTER(ctypes.c_float), POINTER(ctypes.c_float), c_void_p]
     lib.swx_measure_ceilings.argtypes = [c_void_p, ctypes.c_size_t, c_void_p]
+    lib.swx_gauss_legendre.argtypes = [c_int, c_void_p, c_void_p]
+    lib.swx_lambdas.argtypes = [c_int, c_void_p]
+    lib.swx_legendre_basis.argtypes = [c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p]
+    lib.swx_fwd_fft_trunc.argtypes = [c_void_p] + [c_int] * 6 + [c_void_p, c_void_p]
+    lib.swx_invrs_fft.argtypes = [c_void_p] + [c_int] * 4 + [c_void_p, c_void_p]
+    lib.swx_fwd_leg.argtypes = [c_void_p] * 3 + [c_int] * 4 + [c_void_p, c_void_p]
+    lib.swx_invrs_leg.argtypes = [c_void_p] * 2 + [c_int] * 5 + [c_void_p, c_void_p]
+    lib.swx_invrs_uv.argtypes = [c_void_p] * 3 + [c_int] * 5 + [c_void_p] * 9
     lib.swx_plan_device_bytes.argtypes = [c_void_p, POINTER(ctypes.c_size_t)]
     lib.swx_plan_step_counts.argtypes = [c_void_p, POINTER(ctypes.c_longlong)]
     lib.swx_export_state_noalloc.argtypes = lib.swx_export_state.argtypes
@@ -146,7 +155,9 @@ def load() -> ctypes.CDLL:
                  "swx_ensemble_create", "swx_ensemble_run_graph", "swx_plan_get_option", "swx_adjoint_observe",
                  "swx_phase_curve", "swx_phase_curve_vjp", "swx_plan_phase_curve", "swx_set_prev_level",
                  "swx_pipeline_create", "swx_pipeline_run_graph", "swx_run_tape", "swx_adjoint_sweep",
-                 "swx_diagnostic_eta_delta", "swx_measure_ceilings"):
+                 "swx_diagnostic_eta_delta", "swx_measure_ceilings", "swx_gauss_legendre", "swx_lambdas",
+                 "swx_legendre_basis", "swx_fwd_fft_trunc", "swx_invrs_fft", "swx_fwd_leg", "swx_invrs_leg",
+                 "swx_invrs_uv"):
         getattr(lib, name).restype = c_int
     _lib = lib
     return lib

@@ -992,6 +992,30 @@ int swx_grid_size(int M, int* J, int* I) {
   return SWX_OK;
 }
 
+int swx_gauss_legendre(int J, double* mus, double* w) {
+  if (J < 1) return fail(SWX_ERR_ARG, "gauss_legendre: J must be >= 1");
+  if (!mus || !w) return fail(SWX_ERR_ARG, "gauss_legendre: null output");
+  gauss_legendre(J, mus, w);
+  return SWX_OK;
+}
+
+int swx_lambdas(int I, double* lambdas_out) {
+  if (I < 1) return fail(SWX_ERR_ARG, "build_lambdas: I must be >= 1");
+  if (!lambdas_out) return fail(SWX_ERR_ARG, "build_lambdas: null output");
+  lambdas(I, lambdas_out);
+  return SWX_OK;
+}
+
+int swx_legendre_basis(int J, int M, int N, const double* mus, int exact_basis, double* Pmn, double* Hmn) {
+  if (J < 1 || M < 0 || N < 0) return fail(SWX_ERR_ARG, "PmnHmn: need J >= 1, M >= 0, N >= 0");
+  if (!mus || !Pmn || !Hmn) return fail(SWX_ERR_ARG, "PmnHmn: null argument");
+  for (int j = 0; j < J; ++j)
+    if (!(std::fabs(mus[j]) <= 1.0))
+      return fail(SWX_ERR_ARG, "PmnHmn: mus[" + std::to_string(j) + "] outside [-1, 1]");
+  legendre_basis(M, N, J, mus, Pmn, Hmn, exact_basis == 0);
+  return SWX_OK;
+}
+
 int swx_geometry(int M, int exact_basis, double* mus, double* w, double* lambdas_out, double* Pmn, double* Hmn) {
   Geometry g;
   if (!build_geometry(M, exact_basis != 0, g))
@@ -2142,6 +2166,26 @@ int scratch_address(int dev, void** out) {
 
 }  // namespace
 
+namespace {
+
+// Shared checks of the standalone transforms: K batch rows of J latitudes; K is the launch's
+// second grid dimension.
+int transform_dims(const char* name, int K, int J) {
+  if (K < 0) return fail(SWX_ERR_ARG, std::string(name) + ": negative batch count");
+  if (K > 65535) return fail(SWX_ERR_ARG, std::string(name) + ": batch count above 65535");
+  if (J < 1) return fail(SWX_ERR_ARG, std::string(name) + ": J must be >= 1");
+  return SWX_OK;
+}
+
+int dft_length(const char* name, int I) {
+  if (I < 1 || I > kMaxDftLength)
+    return fail(SWX_ERR_ARG, std::string(name) + ": I = " + std::to_string(I) + " outside [1, " +
+                                 std::to_string(kMaxDftLength) + "]");
+  return SWX_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int swx_export_state(swx_plan* p, const swx_state_out* o, void* stream) {
@@ -2204,10 +2248,12 @@ int swx_diagnostic_eta_delta(const double* Um, const double* Vm, const double* f
                              const double* Pmn, const double* Hmn, const double* w, const double* tstepcoeff,
                              const double* mJarray, double dt, double* eta, double* delta, double* etamn,
                              double* deltamn, void* stream) {
-  if (n < 0) return fail(SWX_ERR_ARG, "diagnostic_eta_delta: negative member count");
-  if (M < 0 || N < M || J < 1) return fail(SWX_ERR_ARG, "diagnostic_eta_delta: need M >= 0, N >= M, J >= 1");
+  int rc = transform_dims("diagnostic_eta_delta", n, J);
+  if (rc) return rc;
+  if (M < 0 || N < M) return fail(SWX_ERR_ARG, "diagnostic_eta_delta: need M >= 0 and N >= M");
   if ((eta || delta) && I < 2 * M + 1)
     return fail(SWX_ERR_ARG, "diagnostic_eta_delta: I = " + std::to_string(I) + " below 2 M + 1");
+  if ((eta || delta) && (rc = dft_length("diagnostic_eta_delta", I))) return rc;
   if (!(dt != 0.0)) return fail(SWX_ERR_ARG, "diagnostic_eta_delta: dt must be nonzero");
   if (n == 0) return SWX_OK;
   if (!Um || !Vm || !fmn || !Pmn || !Hmn || !w || !tstepcoeff || !mJarray || !etamn || !deltamn)
@@ -2220,6 +2266,67 @@ int swx_diagnostic_eta_delta(const double* Um, const double* Vm, const double* f
   } else if (eta || delta) {
     SWX_HIP(launch_spec_to_grid(eta ? etamn : deltamn, nullptr, n, J, I, M, N, Pmn, eta ? eta : delta, nullptr, s));
   }
+  return SWX_OK;
+}
+
+int swx_fwd_fft_trunc(const double* data, int is_complex, int K, int J, int C, int I, int M, double* out,
+                      void* stream) {
+  int rc = transform_dims("fwd_fft_trunc", K, J);
+  if (rc || (rc = dft_length("fwd_fft_trunc", I))) return rc;
+  if (C < 1 || M < 0) return fail(SWX_ERR_ARG, "fwd_fft_trunc: need C >= 1 columns and M >= 0");
+  if (K == 0) return SWX_OK;
+  if (!data || !out) return fail(SWX_ERR_ARG, "fwd_fft_trunc: null pointer");
+  SWX_HIP(launch_fwd_dft_trunc(data, is_complex != 0, K, J, C, I, std::min(M + 1, I), out, (hipStream_t)stream));
+  return SWX_OK;
+}
+
+int swx_invrs_fft(const double* X, int K, int J, int C, int I, double* out, void* stream) {
+  int rc = transform_dims("invrs_fft", K, J);
+  if (rc || (rc = dft_length("invrs_fft", I))) return rc;
+  if (C < 1) return fail(SWX_ERR_ARG, "invrs_fft: need C >= 1 columns");
+  if (K == 0) return SWX_OK;
+  if (!X || !out) return fail(SWX_ERR_ARG, "invrs_fft: null pointer");
+  SWX_HIP(launch_dft_synth(X, K, J, C, I, out, (hipStream_t)stream));
+  return SWX_OK;
+}
+
+int swx_fwd_leg(const double* data, const double* basis, const double* w, int K, int J, int M, int N, double* out,
+                void* stream) {
+  int rc = transform_dims("fwd_leg", K, J);
+  if (rc) return rc;
+  if (M < 0 || N < 0) return fail(SWX_ERR_ARG, "fwd_leg: need M >= 0 and N >= 0");
+  if (K == 0) return SWX_OK;
+  if (!data || !basis || !out) return fail(SWX_ERR_ARG, "fwd_leg: null pointer");
+  SWX_HIP(launch_fwd_leg(data, basis, w, K, J, M, N, out, (hipStream_t)stream));
+  return SWX_OK;
+}
+
+int swx_invrs_leg(const double* S, const double* basis, int K, int I, int J, int M, int N, double* out,
+                  void* stream) {
+  int rc = transform_dims("invrs_leg", K, J);
+  if (rc) return rc;
+  if (M < 0 || N < 0) return fail(SWX_ERR_ARG, "invrs_leg: need M >= 0 and N >= 0");
+  if (I < M + 1) return fail(SWX_ERR_ARG, "invrs_leg: I = " + std::to_string(I) + " below M + 1");
+  if (M + 1 > 4096) return fail(SWX_ERR_ARG, "invrs_leg: M + 1 above 4096 (the kernel's LDS row)");
+  if (K == 0) return SWX_OK;
+  if (!S || !basis || !out) return fail(SWX_ERR_ARG, "invrs_leg: null pointer");
+  SWX_HIP(launch_invrs_leg(S, basis, K, J, I, M, N, out, (hipStream_t)stream));
+  return SWX_OK;
+}
+
+int swx_invrs_uv(const double* deltamn, const double* etamn, const double* fmn, int K, int I, int J, int M, int N,
+                 const double* Pmn, const double* Hmn, const double* tstepcoeffmn, const double* marray, double* U,
+                 double* V, double* Um, double* Vm, void* stream) {
+  int rc = transform_dims("invrsUV", K, J);
+  if (rc || (rc = dft_length("invrsUV", I))) return rc;
+  if (M < 0 || N < M) return fail(SWX_ERR_ARG, "invrsUV: need M >= 0 and N >= M");
+  if (I < 2 * M + 1) return fail(SWX_ERR_ARG, "invrsUV: I = " + std::to_string(I) + " below 2 M + 1");
+  if (K == 0) return SWX_OK;
+  if (!deltamn || !etamn || !fmn || !Pmn || !Hmn || !tstepcoeffmn || !marray)
+    return fail(SWX_ERR_ARG, "invrsUV: null input");
+  if (!U && !V && !Um && !Vm) return fail(SWX_ERR_ARG, "invrsUV: no output requested");
+  SWX_HIP(launch_invrs_uv(deltamn, etamn, fmn, K, J, I, M, N, Pmn, Hmn, tstepcoeffmn, marray, U, V, Um, Vm,
+                          (hipStream_t)stream));
   return SWX_OK;
 }
 

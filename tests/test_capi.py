@@ -149,3 +149,82 @@ def test_measure_ceilings_refuses_small_or_ragged_sizes():
     assert b"256 MiB" in lib.swx_last_error()
     assert lib.swx_measure_ceilings(out, ctypes.c_size_t((256 << 20) + 8), None) == _capi.SWX_ERR_ARG
     assert lib.swx_measure_ceilings(None, ctypes.c_size_t(1 << 30), None) == _capi.SWX_ERR_ARG
+
+
+def test_spectral_transform_geometry_functions():
+    """The reference's geometry functions from the native library (spectral_transform.py:54-114,
+    :219-287): gauss_legendre against scipy.special.roots_legendre at several orders (nodes to
+    1e-15; weights to 1e-9 relative, the accuracy of scipy's own weights at J = 333, whose error
+    against a long-double Newton solve is 4e-10 there), with weights summing to 2 and integrating
+    x^2 exactly, build_lambdas against numpy.linspace, PmnHmn at the T42 Gauss nodes equal to
+    the plan geometry bit for bit and to the closed forms of test_transform_stack.py:37-48, and at
+    random nodes against the scipy-based oracle."""
+    import scipy.special
+
+    from my_swampe_amd import _capi
+    from my_swampe_amd import spectral_transform as st
+
+    for J in (1, 2, 7, 64, 96, 160, 333):
+        mus, w = st.gauss_legendre(J)
+        x, wx = scipy.special.roots_legendre(J)
+        np.testing.assert_allclose(mus, x, rtol=0, atol=1e-15)
+        np.testing.assert_allclose(w, wx, rtol=1e-9, atol=0)
+        assert abs(np.sum(w) - 2.0) <= 1e-13
+        if J > 1:
+            assert abs(np.sum(w * mus**2) - 2.0 / 3.0) <= 1e-13
+        assert np.all(np.diff(mus) > 0)
+    for I in (1, 128, 192, 320):
+        np.testing.assert_allclose(st.build_lambdas(I), np.linspace(-np.pi, np.pi, I, endpoint=False), rtol=0,
+                                   atol=1e-15)
+    mus, w = st.gauss_legendre(64)
+    P, H = st.PmnHmn(64, 42, 42, mus)
+    m0, w0, l0, P0, H0 = _capi.geometry(42)
+    assert np.array_equal(mus, m0) and np.array_equal(w, w0)
+    assert np.array_equal(P, P0) and np.array_equal(H, H0)
+    np.testing.assert_allclose(P[:, 2, 2], 0.25 * np.sqrt(15.0) * (1.0 - mus**2), atol=1e-10, rtol=1e-10)
+    np.testing.assert_allclose(H[:, 0, 1], 0.5 * np.sqrt(6.0) * (1.0 - mus**2), atol=1e-10, rtol=1e-10)
+    xr = np.sort(np.random.default_rng(3).uniform(-0.99, 0.99, 11))
+    P, H = st.PmnHmn(11, 20, 24, xr)
+    Po, Ho = orc.pmn_hmn(11, 20, 24, xr)
+    np.testing.assert_allclose(P, Po, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(H, Ho, rtol=0, atol=1e-11)
+    Pw = st.weighted_legendre_basis(P0, w0)
+    assert np.array_equal(Pw, w0[:, None, None] * P0)
+    with pytest.raises(ValueError):
+        st.gauss_legendre(0)
+    with pytest.raises(ValueError):
+        st.PmnHmn(2, 3, 3, np.array([0.5, 1.5]))
+    with pytest.raises(ValueError):
+        st.PmnHmn(3, 3, 3, np.array([0.5, 0.2]))
+
+
+def test_standalone_transforms_refuse_bad_arguments():
+    """The device transforms check their arguments before any launch: negative or oversized batch
+    counts, a longitude count outside [1, 2048], I below M + 1 (invrs_leg) or 2 M + 1 (invrsUV), null
+    pointers and no requested output are SWX_ERR_ARG; an empty batch is a no-op."""
+    import ctypes
+
+    from my_swampe_amd import _capi
+
+    lib = _capi.load()
+    E = _capi.SWX_ERR_ARG
+    p = ctypes.c_void_p(8)  # never dereferenced: every call below fails or returns before a launch
+    assert lib.swx_fwd_fft_trunc(p, 0, -1, 4, 8, 8, 2, p, None) == E
+    assert lib.swx_fwd_fft_trunc(p, 0, 70000, 4, 8, 8, 2, p, None) == E
+    assert lib.swx_fwd_fft_trunc(p, 0, 1, 4, 8, 4096, 2, p, None) == E
+    assert b"2048" in lib.swx_last_error()
+    assert lib.swx_fwd_fft_trunc(None, 0, 1, 4, 8, 8, 2, p, None) == E
+    assert lib.swx_fwd_fft_trunc(None, 0, 0, 4, 8, 8, 2, None, None) == _capi.SWX_OK
+    assert lib.swx_invrs_fft(p, 1, 4, 0, 8, p, None) == E
+    assert lib.swx_fwd_leg(p, None, None, 1, 4, 2, 2, p, None) == E
+    assert lib.swx_fwd_leg(p, p, None, 1, 0, 2, 2, p, None) == E
+    assert lib.swx_invrs_leg(p, p, 1, 2, 4, 2, 2, p, None) == E
+    assert b"M + 1" in lib.swx_last_error()
+    args = [p, p, p, 1, 4, 4, 2, 2, p, p, p, p]
+    assert lib.swx_invrs_uv(*args, p, p, None, None, None) == E
+    assert b"2 M + 1" in lib.swx_last_error()
+    args[4] = 5
+    assert lib.swx_invrs_uv(*args, None, None, None, None, None) == E
+    assert b"no output" in lib.swx_last_error()
+    args[7] = 1
+    assert lib.swx_invrs_uv(*args, p, p, None, None, None) == E

@@ -1,6 +1,8 @@
-"""The reference's diagnostic vorticity/divergence transform on the GPU (spectral_transform.py:
-599-650, diagnostic_eta_delta) against the FP64 oracle, and the reference's transform round trips
-(unit_tests/test_transform_stack.py:100-180) with it running on the HIP kernels."""
+"""The reference's standalone spectral transforms on the GPU (spectral_transform.py:290-650):
+fwd_fft_trunc, invrs_fft, fwd_leg(_w)(_batch), invrs_leg, invrsUV(_with_coeffs) and
+diagnostic_eta_delta, against numpy.fft and the FP64 oracle, and the reference's transform tests
+(unit_tests/test_transform_stack.py:37-180) with every transform in the chain running on the HIP
+kernels."""
 
 from __future__ import annotations
 
@@ -11,9 +13,18 @@ from oracle import swampe_oracle as orc
 
 pytestmark = pytest.mark.gpu
 
+# The reference's FP64 tolerances for chained transforms (test_transform_stack.py:21-26).
+RT_ATOL = RT_RTOL = 1e-10
+
 
 def _np(t):
     return t.detach().cpu().numpy()
+
+
+def _rel(got, ref):
+    got = _np(got) if hasattr(got, "detach") else got
+    scale = float(np.max(np.abs(ref)))
+    return float(np.max(np.abs(got - ref))) / (scale if scale else 1.0)
 
 
 def _setup(M, a1, Phibar):
@@ -28,6 +39,164 @@ def _setup(M, a1, Phibar):
 
 def _args(s):
     return (s.fmn, s.I, s.J, s.M, s.N, s.Pmn, s.Hmn, s.w, s.tstepcoeff, s.mJarray, s.dt)
+
+
+def _uv_args(s):
+    return (s.fmn, s.I, s.J, s.M, s.N, s.Pmn, s.Hmn, s.tstepcoeffmn, s.marray)
+
+
+def _random_spectrum(rng, M, nmax, scale):
+    c = np.zeros((M + 1, M + 1), complex)
+    for m in range(nmax + 1):
+        for n in range(m, nmax + 1):
+            c[m, n] = scale * (rng.standard_normal() + (1j * rng.standard_normal() if m else 0.0))
+    return c
+
+
+@pytest.mark.parametrize("M", [42, 106])
+def test_fwd_fft_trunc_matches_numpy_fft(M, torch_gpu):
+    """fwd_fft_trunc of real and complex (J, I) fields, of a short row (zero-padded to I, as
+    fft(n=I) pads) and fwd_fft_trunc_batch of a (2, 3, J, I) stack against numpy.fft.fft(x / I)
+    truncated to m <= M, within 1e-13 of the largest coefficient (direct DFT against FFT order)."""
+    from my_swampe_amd.spectral_transform import fwd_fft_trunc, fwd_fft_trunc_batch
+
+    print(f"[gpu-case] fwd_fft_trunc M={M}", flush=True)
+    J, I = {42: (64, 128), 106: (160, 320)}[M]
+    rng = np.random.default_rng(M)
+    x = rng.standard_normal((J, I))
+    z = x + 1j * rng.standard_normal((J, I))
+
+    def ref(a):
+        return np.fft.fft(a / I, n=I, axis=-1)[..., : M + 1]
+
+    for name, a in (("real", x), ("complex", z), ("padded", x[:, : I - 7])):
+        got = fwd_fft_trunc(a, I, M)
+        assert tuple(got.shape) == (J, M + 1), name
+        assert _rel(got, ref(a)) <= 1e-13, (name, _rel(got, ref(a)))
+    stack = rng.standard_normal((2, 3, J, I))
+    got = fwd_fft_trunc_batch(stack, I, M)
+    assert tuple(got.shape) == (2, 3, J, M + 1)
+    assert _rel(got, ref(stack)) <= 1e-13
+    import torch
+
+    assert torch.equal(got[1, 2], fwd_fft_trunc(stack[1, 2], I, M))
+    with pytest.raises(ValueError):
+        fwd_fft_trunc(stack, I, M)
+
+
+@pytest.mark.parametrize("M", [42, 106])
+def test_invrs_fft_matches_numpy_ifft(M, torch_gpu):
+    """invrs_fft of full (J, I) and truncated (J, M+1) coefficient rows against
+    numpy.fft.ifft(I X, n=I), within 1e-13 of the largest value."""
+    from my_swampe_amd.spectral_transform import invrs_fft
+
+    print(f"[gpu-case] invrs_fft M={M}", flush=True)
+    J, I = {42: (64, 128), 106: (160, 320)}[M]
+    rng = np.random.default_rng(100 + M)
+    for C in (I, M + 1):
+        X = rng.standard_normal((J, C)) + 1j * rng.standard_normal((J, C))
+        got = invrs_fft(X, I)
+        ref = np.fft.ifft(I * X, n=I, axis=1)
+        assert tuple(got.shape) == (J, I)
+        assert _rel(got, ref) <= 1e-13, (C, _rel(got, ref))
+
+
+@pytest.mark.parametrize("M", [42, 106])
+def test_legendre_transforms_vs_oracle(M, torch_gpu):
+    """fwd_leg, fwd_leg_w, fwd_leg_w_batch and invrs_leg against the oracle (1e-13 of the largest
+    value); invrs_leg's layout: +m at column m, the exact conjugate at column I - m, zeros
+    between; a batch of 3 spectra equals three single calls bit for bit."""
+    import torch
+
+    from my_swampe_amd.spectral_transform import (PmnHmn, fwd_leg, fwd_leg_w, fwd_leg_w_batch, gauss_legendre,
+                                                  invrs_leg, weighted_legendre_basis)
+
+    print(f"[gpu-case] Legendre transforms M={M}", flush=True)
+    J, I = {42: (64, 128), 106: (160, 320)}[M]
+    N = M
+    mus, w = gauss_legendre(J)
+    Pmn, Hmn = PmnHmn(J, M, N, mus)
+    rng = np.random.default_rng(200 + M)
+    data = rng.standard_normal((3, J, M + 1)) + 1j * rng.standard_normal((3, J, M + 1))
+    ref = np.stack([orc.fwd_leg(d, w[:, None, None] * Pmn) for d in data])
+    assert _rel(fwd_leg(data[0], J, M, N, Pmn, w), ref[0]) <= 1e-13
+    Hw = weighted_legendre_basis(Hmn, w)
+    refH = orc.fwd_leg(data[1], Hw)
+    assert _rel(fwd_leg_w(data[1], Hw), refH) <= 1e-13
+    batch = fwd_leg_w_batch(data, weighted_legendre_basis(Pmn, w))
+    assert _rel(batch, ref) <= 1e-13
+    S = np.stack([_random_spectrum(rng, M, M, 1.0) for _ in range(3)])
+    out = invrs_leg(S[0], I, J, M, N, Pmn)
+    refS = orc.invrs_leg(S[0], I, J, M, N, Pmn)
+    assert tuple(out.shape) == (J, I)
+    assert _rel(out, refS) <= 1e-13
+    o = _np(out)
+    np.testing.assert_array_equal(o[:, I - M:], np.conj(o[:, 1:M + 1][:, ::-1]))
+    assert not np.any(o[:, M + 1:I - M])
+    outs = invrs_leg(S, I, J, M, N, Pmn)
+    for k in range(3):
+        assert torch.equal(outs[k], invrs_leg(S[k], I, J, M, N, Pmn))
+
+
+@pytest.mark.parametrize("M", [42, 106])
+def test_invrs_uv_vs_oracle(M, torch_gpu):
+    """invrsUV and invrsUV_with_coeffs against the oracle's invrs_uv on random spectra
+    (n <= 20), within 1e-12 of the largest value; both return the same winds bit for bit, and a
+    batch of 2 equals two single calls bit for bit."""
+    import torch
+
+    from my_swampe_amd.spectral_transform import invrsUV, invrsUV_with_coeffs
+
+    print(f"[gpu-case] invrsUV M={M}", flush=True)
+    s, *_ = _setup(M, np.pi / 4, 1.0e3)
+    rng = np.random.default_rng(300 + M)
+    d = np.stack([_random_spectrum(rng, M, 20, 1e-7) for _ in range(2)])
+    e = np.stack([_random_spectrum(rng, M, 20, 1e-6) for _ in range(2)])
+    ref = orc.invrs_uv(d[0], e[0], *_uv_args(s))
+    U, V = invrsUV(d[0], e[0], *_uv_args(s))
+    got4 = invrsUV_with_coeffs(d[0], e[0], *_uv_args(s))
+    for name, g, r in zip(("U", "V"), (U, V), ref[:2]):
+        assert _rel(g, r) <= 1e-12, (name, _rel(g, r))
+    for name, g, r in zip(("U", "V", "Um", "Vm"), got4, ref):
+        assert tuple(g.shape) == r.shape, name
+        assert _rel(g, r) <= 1e-12, (name, _rel(g, r))
+    assert torch.equal(got4[0], U) and torch.equal(got4[1], V)
+    both = invrsUV_with_coeffs(d, e, *_uv_args(s))
+    for k in range(2):
+        one = invrsUV_with_coeffs(d[k], e[k], *_uv_args(s))
+        for a, b in zip(both, one):
+            assert torch.equal(a[k], b)
+
+
+def test_spectral_transform_coriolis_M106(torch_gpu):
+    """f = 2 omega mu through fwd_fft_trunc and fwd_leg (GPU) is omega / sqrt(0.375) at (0, 1)
+    and zero elsewhere (test_transform_stack.py:51-66)."""
+    from my_swampe_amd.spectral_transform import PmnHmn, build_lambdas, fwd_fft_trunc, fwd_leg, gauss_legendre
+
+    print("[gpu-case] Coriolis M=106", flush=True)
+    M, omega = 106, 3.2e-5
+    J, I, N = 160, 320, 106
+    mus, w = gauss_legendre(J)
+    assert build_lambdas(I).shape == (I,)
+    Pmn, _ = PmnHmn(J, M, N, mus)
+    f = (2.0 * omega) * mus[:, None] * np.ones((1, I))
+    fmn = _np(fwd_leg(fwd_fft_trunc(f, I, M), J, M, N, Pmn, w))
+    check = np.zeros((M + 1, N + 1), complex)
+    check[0, 1] = omega / np.sqrt(0.375)
+    np.testing.assert_allclose(fmn, check, atol=RT_ATOL, rtol=RT_RTOL)
+
+
+def test_spectral_transform_forward_inverse_M63(torch_gpu):
+    """fwd_fft_trunc, fwd_leg, invrs_leg, invrs_fft (all GPU) recover the test-1 zonal wind
+    (test_transform_stack.py:69-97, a1 = pi/2, Phibar = 3e3)."""
+    from my_swampe_amd.spectral_transform import fwd_fft_trunc, fwd_leg, invrs_fft, invrs_leg
+
+    print("[gpu-case] forward/inverse round trip M=63", flush=True)
+    s, eta0, delta0, U0, V0 = _setup(63, np.pi / 2, 3.0e3)
+    Um = fwd_fft_trunc(U0, s.I, s.M)
+    Umn = fwd_leg(Um, s.J, s.M, s.N, s.Pmn, s.w)
+    Unew = invrs_fft(invrs_leg(Umn, s.I, s.J, s.M, s.N, s.Pmn), s.I)
+    np.testing.assert_allclose(np.real(_np(Unew)), U0, atol=RT_ATOL, rtol=RT_RTOL)
 
 
 @pytest.mark.parametrize("M", [42, 106])
@@ -50,45 +219,41 @@ def test_diagnostic_eta_delta_vs_oracle(M, torch_gpu):
     ref = orc.diagnostic_eta_delta(Um, Vm, *_args(s))
     got = diagnostic_eta_delta(Um, Vm, *_args(s))
     for name, g, r in zip(("eta", "delta", "etamn", "deltamn"), got, ref):
-        g = _np(g)
-        err = float(np.max(np.abs(g - r)))
-        scale = float(np.max(np.abs(r)))
-        print(name, "max|diff| / max|ref|", err / scale)
-        assert g.shape == r.shape, name
-        assert err <= 1e-12 * scale, (name, err / scale)
+        print(name, "max|diff| / max|ref|", _rel(g, r))
+        assert tuple(g.shape) == r.shape, name
+        assert _rel(g, r) <= 1e-12, (name, _rel(g, r))
 
 
 def test_wind_transform_round_trip_M106(torch_gpu):
-    """diagnostic_eta_delta (GPU) then invrs_uv recovers U, V (test_transform_stack.py:100-136, its
-    FP64 tolerances atol = rtol = 1e-10)."""
-    from my_swampe_amd.spectral_transform import diagnostic_eta_delta
+    """fwd_fft_trunc, diagnostic_eta_delta, invrsUV (all GPU) recover U, V
+    (test_transform_stack.py:100-136, a1 = pi/4, Phibar = 1e3)."""
+    from my_swampe_amd.spectral_transform import diagnostic_eta_delta, fwd_fft_trunc, invrsUV
 
     print("[gpu-case] wind round trip M=106", flush=True)
     s, eta0, delta0, U0, V0 = _setup(106, np.pi / 4, 1.0e3)
-    Um = orc.fwd_fft_trunc(U0, s.I, s.M)
-    Vm = orc.fwd_fft_trunc(V0, s.I, s.M)
+    Um = fwd_fft_trunc(U0, s.I, s.M)
+    Vm = fwd_fft_trunc(V0, s.I, s.M)
     _, _, etamn, deltamn = diagnostic_eta_delta(Um, Vm, *_args(s))
-    U, V, _, _ = orc.invrs_uv(_np(deltamn), _np(etamn), s.fmn, s.I, s.J, s.M, s.N, s.Pmn, s.Hmn, s.tstepcoeffmn,
-                              s.marray)
-    np.testing.assert_allclose(np.real(U), U0, atol=1e-10, rtol=1e-10)
-    np.testing.assert_allclose(np.real(V), V0, atol=1e-10, rtol=1e-10)
+    U, V = invrsUV(deltamn, etamn, *_uv_args(s))
+    np.testing.assert_allclose(np.real(_np(U)), U0, atol=RT_ATOL, rtol=RT_RTOL)
+    np.testing.assert_allclose(np.real(_np(V)), V0, atol=RT_ATOL, rtol=RT_RTOL)
 
 
 def test_vorticity_divergence_round_trip_M63(torch_gpu):
-    """invrs_uv then diagnostic_eta_delta (GPU) recovers eta, delta on the grid
-    (test_transform_stack.py:139-180, atol = rtol = 1e-10)."""
-    from my_swampe_amd.spectral_transform import diagnostic_eta_delta
+    """fwd_fft_trunc, fwd_leg, invrsUV, fwd_fft_trunc of the complex winds, diagnostic_eta_delta
+    (all GPU) recover eta, delta on the grid (test_transform_stack.py:139-180, Phibar = 4e4)."""
+    from my_swampe_amd.spectral_transform import diagnostic_eta_delta, fwd_fft_trunc, fwd_leg, invrsUV
 
     print("[gpu-case] vorticity/divergence round trip M=63", flush=True)
     s, eta0, delta0, U0, V0 = _setup(63, np.pi / 4, 4.0e4)
-    etamn = orc.fwd_leg(orc.fwd_fft_trunc(eta0, s.I, s.M), s.Pmnw)
-    deltamn = orc.fwd_leg(orc.fwd_fft_trunc(delta0, s.I, s.M), s.Pmnw)
-    U, V, _, _ = orc.invrs_uv(deltamn, etamn, s.fmn, s.I, s.J, s.M, s.N, s.Pmn, s.Hmn, s.tstepcoeffmn, s.marray)
-    Um = orc.fwd_fft_trunc(np.real(U), s.I, s.M)
-    Vm = orc.fwd_fft_trunc(np.real(V), s.I, s.M)
+    deltamn = fwd_leg(fwd_fft_trunc(delta0, s.I, s.M), s.J, s.M, s.N, s.Pmn, s.w)
+    etamn = fwd_leg(fwd_fft_trunc(eta0, s.I, s.M), s.J, s.M, s.N, s.Pmn, s.w)
+    U, V = invrsUV(deltamn, etamn, *_uv_args(s))
+    Um = fwd_fft_trunc(U, s.I, s.M)
+    Vm = fwd_fft_trunc(V, s.I, s.M)
     eta, delta, _, _ = diagnostic_eta_delta(Um, Vm, *_args(s))
-    np.testing.assert_allclose(np.real(_np(eta)), eta0, atol=1e-10, rtol=1e-10)
-    np.testing.assert_allclose(np.real(_np(delta)), delta0, atol=1e-10, rtol=1e-10)
+    np.testing.assert_allclose(np.real(_np(eta)), eta0, atol=RT_ATOL, rtol=RT_RTOL)
+    np.testing.assert_allclose(np.real(_np(delta)), delta0, atol=RT_ATOL, rtol=RT_RTOL)
 
 
 def test_diagnostic_eta_delta_batched_and_errors(torch_gpu):
@@ -113,3 +278,25 @@ def test_diagnostic_eta_delta_batched_and_errors(torch_gpu):
     with pytest.raises(ValueError):
         diagnostic_eta_delta(Ums[0], Vms[0], s.fmn, s.I, s.J, s.M, s.N, s.Pmn[:, :, :-1], s.Hmn, s.w, s.tstepcoeff,
                              s.mJarray, s.dt)
+
+
+def test_transform_argument_errors(torch_gpu):
+    """Shape and size errors raise ValueError before any launch: a wrong basis shape, mismatched
+    delta/eta spectra, I below 2 M + 1 for the wind inversion, I above the DFT kernels' 2048."""
+    from my_swampe_amd.spectral_transform import fwd_fft_trunc, fwd_leg, invrs_leg, invrsUV
+
+    s, *_ = _setup(42, np.pi / 4, 1.0e3)
+    rng = np.random.default_rng(9)
+    S = _random_spectrum(rng, 42, 42, 1.0)
+    with pytest.raises(ValueError):
+        fwd_leg(np.zeros((s.J, 43), complex), s.J, 42, 42, s.Pmn[:, :, :-1], s.w)
+    with pytest.raises(ValueError):
+        invrs_leg(S[:, :-1], s.I, s.J, 42, 42, s.Pmn)
+    with pytest.raises(ValueError):
+        invrs_leg(S, 40, s.J, 42, 42, s.Pmn)
+    with pytest.raises(ValueError):
+        invrsUV(S, S[None], *_uv_args(s))
+    with pytest.raises(ValueError):
+        invrsUV(S, S, s.fmn, 84, s.J, s.M, s.N, s.Pmn, s.Hmn, s.tstepcoeffmn, s.marray)
+    with pytest.raises(ValueError):
+        fwd_fft_trunc(np.zeros((2, 4096)), 4096, 42)
