@@ -156,7 +156,8 @@ def test_spectral_transform_geometry_functions():
     :219-287): gauss_legendre against scipy.special.roots_legendre at several orders (nodes to
     1e-15; weights to 1e-9 relative, the accuracy of scipy's own weights at J = 333, whose error
     against a long-double Newton solve is 4e-10 there), with weights summing to 2 and integrating
-    x^2 exactly, build_lambdas against numpy.linspace, PmnHmn at the T42 Gauss nodes equal to
+    x^2 exactly, and by default scipy's values themselves, as the reference returns them;
+    build_lambdas against numpy.linspace, PmnHmn at the T42 Gauss nodes equal to
     the plan geometry bit for bit and to the closed forms of test_transform_stack.py:37-48, and at
     random nodes against the scipy-based oracle."""
     import scipy.special
@@ -165,8 +166,10 @@ def test_spectral_transform_geometry_functions():
     from my_swampe_amd import spectral_transform as st
 
     for J in (1, 2, 7, 64, 96, 160, 333):
-        mus, w = st.gauss_legendre(J)
+        mus, w = st.gauss_legendre(J, exact=True)
         x, wx = scipy.special.roots_legendre(J)
+        mr, wr = st.gauss_legendre(J)
+        assert np.array_equal(mr, x) and np.array_equal(wr, wx)
         np.testing.assert_allclose(mus, x, rtol=0, atol=1e-15)
         np.testing.assert_allclose(w, wx, rtol=1e-9, atol=0)
         assert abs(np.sum(w) - 2.0) <= 1e-13
@@ -176,7 +179,7 @@ def test_spectral_transform_geometry_functions():
     for I in (1, 128, 192, 320):
         np.testing.assert_allclose(st.build_lambdas(I), np.linspace(-np.pi, np.pi, I, endpoint=False), rtol=0,
                                    atol=1e-15)
-    mus, w = st.gauss_legendre(64)
+    mus, w = st.gauss_legendre(64, exact=True)
     P, H = st.PmnHmn(64, 42, 42, mus)
     m0, w0, l0, P0, H0 = _capi.geometry(42)
     assert np.array_equal(mus, m0) and np.array_equal(w, w0)
