@@ -168,16 +168,47 @@ def test_invrs_uv_vs_oracle(M, torch_gpu):
             assert torch.equal(a[k], b)
 
 
+def _reference_case(M, a1, Phibar, dt=30.0):
+    """The setup of the reference's transform tests (test_transform_stack.py:69-180) through this
+    package's own functions: test-1 fields, basis, Coriolis term and coefficient arrays."""
+    from my_swampe_amd import initial_conditions as ic
+    from my_swampe_amd import time_stepping as tstep
+    from my_swampe_amd.spectral_transform import PmnHmn
+
+    omega, a = 7.2921159e-5, 6.37122e6
+    N, I, J, _dt, lambdas, mus, w = ic.spectral_params(M)
+    Pmn, Hmn = PmnHmn(J, M, N, mus)
+    SU0, sina, cosa, etaamp, Phiamp = ic.test1_init(a, omega, a1)
+    eta0, _, delta0, _, Phi0, _ = ic.state_var_init(I, J, mus, lambdas, 1, etaamp, a, sina, cosa, Phibar, Phiamp)
+    U, V = ic.velocity_init(I, J, SU0, cosa, sina, mus, lambdas, 1)
+    return dict(M=M, N=N, I=I, J=J, w=w, Pmn=Pmn, Hmn=Hmn, dt=dt, fmn=ic.coriolismn(M, omega), eta0=eta0,
+                delta0=delta0, U=U, V=V, tstepcoeffmn=tstep.tstepcoeffmn(M, N, a),
+                tstepcoeff=tstep.tstepcoeff(J, M, dt, mus, a), mJarray=tstep.mJarray(J, M), marray=tstep.marray(M, N))
+
+
+def _diag(c, Um, Vm):
+    from my_swampe_amd.spectral_transform import diagnostic_eta_delta
+
+    return diagnostic_eta_delta(Um, Vm, c["fmn"], c["I"], c["J"], c["M"], c["N"], c["Pmn"], c["Hmn"], c["w"],
+                                c["tstepcoeff"], c["mJarray"], c["dt"])
+
+
+def _uv(c, deltamn, etamn):
+    from my_swampe_amd.spectral_transform import invrsUV
+
+    return invrsUV(deltamn, etamn, c["fmn"], c["I"], c["J"], c["M"], c["N"], c["Pmn"], c["Hmn"], c["tstepcoeffmn"],
+                   c["marray"])
+
+
 def test_spectral_transform_coriolis_M106(torch_gpu):
     """f = 2 omega mu through fwd_fft_trunc and fwd_leg (GPU) is omega / sqrt(0.375) at (0, 1)
     and zero elsewhere (test_transform_stack.py:51-66)."""
-    from my_swampe_amd.spectral_transform import PmnHmn, build_lambdas, fwd_fft_trunc, fwd_leg, gauss_legendre
+    from my_swampe_amd import initial_conditions as ic
+    from my_swampe_amd.spectral_transform import PmnHmn, fwd_fft_trunc, fwd_leg
 
     print("[gpu-case] Coriolis M=106", flush=True)
     M, omega = 106, 3.2e-5
-    J, I, N = 160, 320, 106
-    mus, w = gauss_legendre(J)
-    assert build_lambdas(I).shape == (I,)
+    N, I, J, _dt, _lambdas, mus, w = ic.spectral_params(M)
     Pmn, _ = PmnHmn(J, M, N, mus)
     f = (2.0 * omega) * mus[:, None] * np.ones((1, I))
     fmn = _np(fwd_leg(fwd_fft_trunc(f, I, M), J, M, N, Pmn, w))
@@ -192,11 +223,12 @@ def test_spectral_transform_forward_inverse_M63(torch_gpu):
     from my_swampe_amd.spectral_transform import fwd_fft_trunc, fwd_leg, invrs_fft, invrs_leg
 
     print("[gpu-case] forward/inverse round trip M=63", flush=True)
-    s, eta0, delta0, U0, V0 = _setup(63, np.pi / 2, 3.0e3)
-    Um = fwd_fft_trunc(U0, s.I, s.M)
-    Umn = fwd_leg(Um, s.J, s.M, s.N, s.Pmn, s.w)
-    Unew = invrs_fft(invrs_leg(Umn, s.I, s.J, s.M, s.N, s.Pmn), s.I)
-    np.testing.assert_allclose(np.real(_np(Unew)), U0, atol=RT_ATOL, rtol=RT_RTOL)
+    c = _reference_case(63, np.pi / 2, 3.0e3)
+    I, J, M, N = c["I"], c["J"], c["M"], c["N"]
+    Um = fwd_fft_trunc(c["U"], I, M)
+    Umn = fwd_leg(Um, J, M, N, c["Pmn"], c["w"])
+    Unew = invrs_fft(invrs_leg(Umn, I, J, M, N, c["Pmn"]), I)
+    np.testing.assert_allclose(np.real(_np(Unew)), c["U"], atol=RT_ATOL, rtol=RT_RTOL)
 
 
 @pytest.mark.parametrize("M", [42, 106])
@@ -227,33 +259,65 @@ def test_diagnostic_eta_delta_vs_oracle(M, torch_gpu):
 def test_wind_transform_round_trip_M106(torch_gpu):
     """fwd_fft_trunc, diagnostic_eta_delta, invrsUV (all GPU) recover U, V
     (test_transform_stack.py:100-136, a1 = pi/4, Phibar = 1e3)."""
-    from my_swampe_amd.spectral_transform import diagnostic_eta_delta, fwd_fft_trunc, invrsUV
+    from my_swampe_amd.spectral_transform import fwd_fft_trunc
 
     print("[gpu-case] wind round trip M=106", flush=True)
-    s, eta0, delta0, U0, V0 = _setup(106, np.pi / 4, 1.0e3)
-    Um = fwd_fft_trunc(U0, s.I, s.M)
-    Vm = fwd_fft_trunc(V0, s.I, s.M)
-    _, _, etamn, deltamn = diagnostic_eta_delta(Um, Vm, *_args(s))
-    U, V = invrsUV(deltamn, etamn, *_uv_args(s))
-    np.testing.assert_allclose(np.real(_np(U)), U0, atol=RT_ATOL, rtol=RT_RTOL)
-    np.testing.assert_allclose(np.real(_np(V)), V0, atol=RT_ATOL, rtol=RT_RTOL)
+    c = _reference_case(106, np.pi / 4, 1.0e3)
+    Um = fwd_fft_trunc(c["U"], c["I"], c["M"])
+    Vm = fwd_fft_trunc(c["V"], c["I"], c["M"])
+    _, _, etamn, deltamn = _diag(c, Um, Vm)
+    U, V = _uv(c, deltamn, etamn)
+    np.testing.assert_allclose(np.real(_np(U)), c["U"], atol=RT_ATOL, rtol=RT_RTOL)
+    np.testing.assert_allclose(np.real(_np(V)), c["V"], atol=RT_ATOL, rtol=RT_RTOL)
 
 
 def test_vorticity_divergence_round_trip_M63(torch_gpu):
     """fwd_fft_trunc, fwd_leg, invrsUV, fwd_fft_trunc of the complex winds, diagnostic_eta_delta
     (all GPU) recover eta, delta on the grid (test_transform_stack.py:139-180, Phibar = 4e4)."""
-    from my_swampe_amd.spectral_transform import diagnostic_eta_delta, fwd_fft_trunc, fwd_leg, invrsUV
+    from my_swampe_amd.spectral_transform import fwd_fft_trunc, fwd_leg
 
     print("[gpu-case] vorticity/divergence round trip M=63", flush=True)
-    s, eta0, delta0, U0, V0 = _setup(63, np.pi / 4, 4.0e4)
-    deltamn = fwd_leg(fwd_fft_trunc(delta0, s.I, s.M), s.J, s.M, s.N, s.Pmn, s.w)
-    etamn = fwd_leg(fwd_fft_trunc(eta0, s.I, s.M), s.J, s.M, s.N, s.Pmn, s.w)
-    U, V = invrsUV(deltamn, etamn, *_uv_args(s))
-    Um = fwd_fft_trunc(U, s.I, s.M)
-    Vm = fwd_fft_trunc(V, s.I, s.M)
-    eta, delta, _, _ = diagnostic_eta_delta(Um, Vm, *_args(s))
-    np.testing.assert_allclose(np.real(_np(eta)), eta0, atol=RT_ATOL, rtol=RT_RTOL)
-    np.testing.assert_allclose(np.real(_np(delta)), delta0, atol=RT_ATOL, rtol=RT_RTOL)
+    c = _reference_case(63, np.pi / 4, 4.0e4)
+    I, J, M, N = c["I"], c["J"], c["M"], c["N"]
+    deltamn = fwd_leg(fwd_fft_trunc(c["delta0"], I, M), J, M, N, c["Pmn"], c["w"])
+    etamn = fwd_leg(fwd_fft_trunc(c["eta0"], I, M), J, M, N, c["Pmn"], c["w"])
+    U, V = _uv(c, deltamn, etamn)
+    eta, delta, _, _ = _diag(c, fwd_fft_trunc(U, I, M), fwd_fft_trunc(V, I, M))
+    np.testing.assert_allclose(np.real(_np(eta)), c["eta0"], atol=RT_ATOL, rtol=RT_RTOL)
+    np.testing.assert_allclose(np.real(_np(delta)), c["delta0"], atol=RT_ATOL, rtol=RT_RTOL)
+
+
+def test_setup_functions_on_device_tensors(torch_gpu):
+    """state_var_init, velocity_init, ABCDE_init, tstepcoeff and RMS_winds given torch tensors on
+    the GPU compute there and agree with their numpy results within 1e-14 of each field's
+    largest value (the device's transcendental functions against numpy's)."""
+    import torch
+
+    from my_swampe_amd import initial_conditions as ic
+    from my_swampe_amd import time_stepping as ts
+
+    print("[gpu-case] setup functions on device tensors", flush=True)
+    N, I, J, dt, lam, mus, w = ic.spectral_params(63)
+    mus_t, lam_t = torch.tensor(mus, device="cuda"), torch.tensor(lam, device="cuda")
+    a, Phibar = 6.37122e6, 4.0e4
+    SU0, sina, cosa, etaamp, Phiamp = ic.test1_init(a, 7.2921159e-5, np.pi / 4)
+    for test in (None, 1, 2):
+        args = () if test is None else (a, sina, cosa, Phibar, Phiamp)
+        host = ic.state_var_init(I, J, mus, lam, test, etaamp, *args) + ic.velocity_init(I, J, SU0, cosa, sina, mus,
+                                                                                          lam, test)
+        dev = ic.state_var_init(I, J, mus_t, lam_t, test, etaamp, *args) + ic.velocity_init(I, J, SU0, cosa, sina,
+                                                                                             mus_t, lam_t, test)
+        for h, d in zip(host, dev):
+            assert d.is_cuda and tuple(d.shape) == (J, I)
+            assert _rel(d, h) <= 1e-14
+        A = ic.ABCDE_init(dev[6], dev[7], dev[0], dev[4], mus_t, I, J)
+        Ah = ic.ABCDE_init(host[6], host[7], host[0], host[4], mus, I, J)
+        for d, h in zip(A, Ah):
+            assert d.is_cuda and _rel(d, h) <= 1e-14
+        r_dev = float(ts.RMS_winds(a, I, J, lam_t, mus_t, dev[6], dev[7]))
+        r_host = float(ts.RMS_winds(a, I, J, lam, mus, host[6], host[7]))
+        assert abs(r_dev - r_host) <= 1e-13 * max(r_host, 1.0)
+    assert _rel(ts.tstepcoeff(J, N, dt, mus_t, a), ts.tstepcoeff(J, N, dt, mus, a)) <= 1e-15
 
 
 def test_diagnostic_eta_delta_batched_and_errors(torch_gpu):
