@@ -1,7 +1,8 @@
 """The reference's setup functions on the host (no GPU): initial_conditions.py (spectral_params,
-test1_init, state_var_init, velocity_init, ABCDE_init, coriolismn) and the coefficient arrays and
-RMS wind of time_stepping.py:491-576, against the oracle (pinned by the upstream fixtures, whose
-test-1 case starts from these fields) bit for bit, and against what the mathematics fixes."""
+test1_init, state_var_init, velocity_init, ABCDE_init, coriolismn), the coefficient arrays and
+RMS wind of time_stepping.py:491-576, filters.py and forcing.py, against the oracle (pinned by the
+upstream fixtures, whose runs start from and step with these fields) bit for bit, and against
+what the mathematics fixes."""
 
 from __future__ import annotations
 
@@ -93,3 +94,53 @@ def test_time_stepping_coefficients_and_rms():
     r = ts.RMS_winds(6.37122e6, I, J, lam, mus, U, V)
     assert r == orc.rms_winds(6.37122e6, I, J, lam, mus, U, V) and r > 0
     np.testing.assert_allclose(ts.RMS_winds(6.37122e6, I, J, lam, mus, 3.0 * U, 3.0 * V), 3.0 * r, rtol=1e-14)
+
+
+@pytest.mark.parametrize("M,a,dt,K6", [(42, 8.2e7, 120.0, 1.24e33), (106, 6.37122e6, 60.0, 1.0e30)])
+def test_filters(M, a, dt, K6):
+    """The sixth-order factors equal the oracle's bit for bit; every implicit factor is exactly 1 at
+    its neutral degree (n = 1 with the offset, n = 0 without), decreases with n beyond it and lies in
+    (0, 1] there, and diffusion multiplies."""
+    from my_swampe_amd import filters as fl
+
+    for name in ("sigma6", "sigma6Phi", "sigma6_exponential", "sigma6Phi_exponential"):
+        got = getattr(fl, name)(M, M, K6, a, dt)
+        assert got.shape == (M + 1, M + 1) and np.array_equal(got, getattr(orc, name)(M, M, K6, a, dt)), name
+    K4 = K6 / a ** 2
+    for name, neutral, args in (("sigma6", 1, (K6,)), ("sigma6Phi", 0, (K6,)), ("sigma", 1, (K4,)),
+                                ("sigmaPhi", 0, (K4,))):
+        s = getattr(fl, name)(M, M, *args, a, dt)[0]
+        assert s[neutral] == 1.0, name
+        assert np.all(np.diff(s[neutral:]) < 0) and np.all(s[neutral + 1:] > 0), name
+    x = np.arange(6.0).reshape(2, 3)
+    assert np.array_equal(fl.diffusion(x, 0.5), 0.5 * x)
+
+
+def test_forcing():
+    """Phieqfun, Qfun and Rfun equal the oracle's bit for bit; the day side is strict (the nodes at
+    lambda = -pi/2 and pi/2 carry Phibar), negative Q drives no wind forcing, taudrag = -1 turns the
+    drag off (also per member in an array), and a zero Phi + Phibar takes the smallest normal
+    double instead of dividing by zero."""
+    from my_swampe_amd import forcing as fo
+    from my_swampe_amd import initial_conditions as ic
+
+    N, I, J, dt, lam, mus, w = ic.spectral_params(42)
+    Phibar, DPhieq = 3.0e5, 1.0e6
+    pe = fo.Phieqfun(Phibar, DPhieq, lam, mus, I, J, 9.8)
+    assert np.array_equal(pe, orc.phieqfun(Phibar, DPhieq, lam, mus))
+    edge = np.isclose(np.abs(lam), np.pi / 2, rtol=0, atol=1e-14)
+    assert edge.sum() == 2 and np.all(pe[:, edge] == Phibar) and np.all(pe[:, np.abs(lam) > np.pi / 2] == Phibar)
+    rng = np.random.default_rng(4)
+    Phi, U, V = (rng.standard_normal((J, I)) * s for s in (1e4, 10.0, 10.0))
+    Q = fo.Qfun(pe, Phi, Phibar, 36000.0)
+    assert np.array_equal(Q, orc.qfun(pe, Phi, Phibar, 36000.0))
+    for td in (21600.0, -1.0):
+        for g, r in zip(fo.Rfun(U, V, Q, Phi, Phibar, td), orc.rfun(U, V, Q, Phi, Phibar, td)):
+            assert np.array_equal(g, r)
+    F, G = fo.Rfun(U, V, Q, Phi, Phibar, -1.0)
+    assert not np.any(F[Q < 0]) and not np.any(G[Q < 0])
+    tds = np.array([21600.0, -1.0])[:, None, None]
+    Fb, Gb = fo.Rfun(U[None], V[None], Q[None], Phi[None], Phibar, tds)
+    assert np.array_equal(Fb[0], fo.Rfun(U, V, Q, Phi, Phibar, 21600.0)[0]) and np.array_equal(Fb[1], F)
+    Fz, _ = fo.Rfun(np.ones((1, 1)), np.ones((1, 1)), np.ones((1, 1)), np.full((1, 1), -Phibar), Phibar, -1.0)
+    assert Fz[0, 0] == -1.0 / np.finfo(np.float64).tiny
