@@ -173,3 +173,42 @@ def test_pipeline_of_two_truncations(torch_gpu):
     pipe.close()
     for p in plans:
         p.close()
+
+
+def test_headline_arrangement_equals_one_plan_at_full_size(torch_gpu):
+    """The benchmark's arrangement at its size: 256 T42 members in FP32 as two concurrent plans of
+    128 (each plan's kernel variants and Legendre member group left to its own choice, as the
+    benchmark leaves them) export bit for bit what one 256-member plan exports, over graph-replayed
+    calls of 50, 20 and 20 steps (each call ending in its lineage window and row save)."""
+    from my_swampe_amd.plan import Pipeline, Plan, split_plans
+
+    torch = torch_gpu
+    print("[gpu-case] pipeline 2 x 128 vs 256, FP32", flush=True)
+    M, B = 42, 256
+    params, c0 = _params(B), torch.tensor(_ics(M, B), device="cuda")
+    fields = ("Phi_curr", "Phi_prev", "eta_curr", "delta_curr", "U_curr", "V_curr")
+    s = torch.cuda.Stream()
+    one = Plan(M, params, precision="fp32", diagnostics=False)
+    plans = split_plans(M, params, 2, precision="fp32", diagnostics=False)
+    with torch.cuda.stream(s):
+        one.init_spectral(c0, stream=s)
+        b0 = 0
+        for p in plans:
+            p.init_spectral(c0[b0:b0 + p.B].contiguous(), stream=s)
+            b0 += p.B
+        pipe = Pipeline(plans)
+        t = 2
+        for n in (50, 20, 20):
+            one.run_graph(t, n, stream=s)
+            pipe.run_graph(t, n, stream=s)
+            t += n
+        want = one.export(fields=fields, spec=True, stream=s)
+        got = [p.export(fields=fields, spec=True, stream=s) for p in plans]
+    s.synchronize()
+    for name in fields + ("spec",):
+        cat = torch.cat([g[name] for g in got])
+        assert torch.isfinite(cat.real if cat.is_complex() else cat).all(), name
+        assert torch.equal(cat, want[name]), name
+    pipe.close()
+    for p in plans + [one]:
+        p.close()
