@@ -3,12 +3,22 @@
 Public API mirrors my_swampe/__init__.py:64-71 (run_model, run_model_gpu,
 run_model_scan, run_model_scan_final, assert_finite_state) and its submodules
 continuation, spectral_transform, initial_conditions, filters, forcing,
-time_stepping (coefficient arrays) and model, plus the batched ensemble driver
+explicit_tdiff, modEuler_tdiff, time_stepping (coefficient arrays) and model, plus the batched ensemble driver
 (plan) and the differentiable rollouts (autograd).  All time stepping runs in the HIP kernels of
 my_swampe_amd/_lib/libswampe_hip.so.
 """
 
-from . import continuation, filters, forcing, initial_conditions, model, spectral_transform, time_stepping  # noqa: F401
+from . import (  # noqa: F401
+    continuation,
+    explicit_tdiff,
+    filters,
+    forcing,
+    initial_conditions,
+    modEuler_tdiff,
+    model,
+    spectral_transform,
+    time_stepping,
+)
 from .model import (  # noqa: F401
     RunFlags,
     State,
@@ -25,6 +35,6 @@ __version__ = "0.1.0"
 
 __all__ = [
     "RunFlags", "State", "Static", "assert_finite_state", "build_static", "run_model", "run_model_gpu",
-    "run_model_scan", "run_model_scan_final", "__version__", "continuation", "filters", "forcing",
-    "initial_conditions", "model", "spectral_transform", "time_stepping",
+    "run_model_scan", "run_model_scan_final", "__version__", "continuation", "explicit_tdiff", "filters",
+    "forcing", "initial_conditions", "modEuler_tdiff", "model", "spectral_transform", "time_stepping",
 ]
