@@ -1,13 +1,14 @@
-"""Coefficient arrays and the RMS-wind diagnostic of the reference's time_stepping module
-(src/my_swampe/time_stepping.py:491-576), with the same function names, arguments and return
-values.
+"""The reference's time_stepping module (src/my_swampe/time_stepping.py): the one-member step
+``tstepping`` (:16-384), the coefficient arrays (:491-534) and the RMS-wind diagnostic (:537-576),
+with the same function names, arguments and return values.
 
-The step itself (``tstepping``, time_stepping.py:16-384) runs in the HIP kernels: one plan step
-per call of ``my_swampe_amd.model._step_once`` or many per ``Plan.run`` / ``Plan.run_graph``; the
-plans build these coefficients into their own FP64 tables.  The arrays here are for callers that
-use the standalone transforms (``my_swampe_amd.spectral_transform``) the way the reference's
-tests do.  Arrays are float64 numpy; ``tstepcoeff`` and ``RMS_winds`` compute in the array type
-of their field arguments (numpy, or torch tensors where they are).
+The time step of the model runs in the fused HIP kernels: one plan step per call of
+``my_swampe_amd.model._step_once`` or many per ``Plan.run`` / ``Plan.run_graph``, with these
+coefficients in the plans' own FP64 tables.  ``tstepping`` here is the reference's standalone
+step for one member's Fourier fields, composed from the standalone FP64 device updates
+(``modEuler_tdiff`` / ``explicit_tdiff``) and the wind inversion (``invrsUV_with_coeffs``).  The
+coefficient arrays are float64 numpy; ``tstepcoeff`` and ``RMS_winds`` compute in the array type of
+their field arguments (numpy, or torch tensors where they are).
 """
 
 from __future__ import annotations
@@ -81,3 +82,24 @@ def RMS_winds(a: float, I: int, J: int, lambdas, mus, U, V):
     area = (a ** 2) * (xp.sin(phis + math.pi / 2.0) ** 2) * dphi * dlam
     integrand = (U / xp.cos(phis)) ** 2 + (V / xp.cos(phis)) ** 2
     return xp.sqrt(xp.sum(area * integrand / (4.0 * math.pi * a ** 2)))
+
+
+def tstepping(etam0, etam1, deltam0, deltam1, Phim0, Phim1, I, J, M, N, Am, Bm, Cm, Dm, Em, Fm, Gm, Um, Vm, fmn,
+              Pmn, Hmn, Pmnw, Hmnw, tstepcoeff, tstepcoeff2, tstepcoeffmn, marray, mJarray, narray, PhiFm, dt, a,
+              Phibar, taurad, taudrag, forcflag, diffflag, expflag, sigma, sigmaPhi, test, t):
+    """One step of either scheme (time_stepping.py:16-384): the updates of Phi, delta and eta
+    (explicit_tdiff when expflag, else modEuler_tdiff), then the winds of the new delta and eta.
+    Returns (etamn, eta, etam, deltamn, delta, deltam, Phimn, Phi, Phim, U, V, Um, Vm): spectra
+    (M+1, N+1), grids (J, I) and positive-m Fourier coefficients (J, M+1), complex128 on the GPU."""
+    from . import explicit_tdiff, modEuler_tdiff
+    from .spectral_transform import invrsUV_with_coeffs
+
+    scheme = explicit_tdiff if bool(expflag) else modEuler_tdiff
+    args = (etam0, etam1, deltam0, deltam1, Phim0, Phim1, I, J, M, N, Am, Bm, Cm, Dm, Em, Fm, Gm, Um, Vm, Pmn, Pmnw,
+            Hmnw, tstepcoeff, tstepcoeff2, mJarray, narray, PhiFm, dt, a, Phibar, taurad, taudrag, forcflag, diffflag,
+            sigma, sigmaPhi, test, t)
+    Phimn, Phi, Phim = scheme.phi_timestep(*args)
+    deltamn, delta, deltam = scheme.delta_timestep(*args)
+    etamn, eta, etam = scheme.eta_timestep(*args)
+    U, V, Um_new, Vm_new = invrsUV_with_coeffs(deltamn, etamn, fmn, I, J, M, N, Pmn, Hmn, tstepcoeffmn, marray)
+    return etamn, eta, etam, deltamn, delta, deltam, Phimn, Phi, Phim, U, V, Um_new, Vm_new

@@ -2,7 +2,7 @@
 tests of unit_tests/test_parity_quirks.py:85-317 on its small configuration (M = N = 2, J = 4,
 I = 8, seed 7), and all six updates against the oracle's modified-Euler and explicit updates (the
 restatement the fixtures pin end to end) at that size and at T42, with and without forcing and
-diffusion."""
+diffusion; and the one-member step time_stepping.tstepping of both schemes."""
 
 from __future__ import annotations
 
@@ -165,3 +165,38 @@ def test_tendency_argument_errors(torch_gpu):
     args[0] = np.zeros((4, 2))
     with pytest.raises(ValueError):
         mod.phi_timestep(*args)
+
+
+@pytest.mark.parametrize("expflag", [False, True])
+def test_tstepping_vs_oracle(expflag, torch_gpu):
+    """time_stepping.tstepping at T42 (random Fourier fields, forced, diffused): the 13 arrays
+    against the oracle's scheme updates, their inverse transforms and the oracle's wind inversion
+    (spectra within 1e-12, grids and Fourier coefficients within 1e-11 of each array's largest
+    value)."""
+    from my_swampe_amd import initial_conditions as ic
+    from my_swampe_amd import time_stepping as ts
+
+    print(f"[gpu-case] tstepping expflag={expflag}", flush=True)
+    s = _setup(42, 64, 128, seed=13, sigma_K6=1.0e27)
+    a = s["arrays"]
+    M, N, I, J = s["M"], s["N"], s["I"], s["J"]
+    fmn = ic.coriolismn(M, 7.2921159e-5)
+    tcmn, marr = ts.tstepcoeffmn(M, N, A), ts.marray(M, N)
+    got = ts.tstepping(a["etam0"], a["etam1"], a["deltam0"], a["deltam1"], a["Phim0"], a["Phim1"], I, J, M, N, a["Am"],
+                       a["Bm"], a["Cm"], a["Dm"], a["Em"], a["Fm"], a["Gm"], a["Um"], a["Vm"], fmn, s["Pmn"], s["Hmn"],
+                       s["Pmnw"], s["Hmnw"], s["tstepcoeff1"], s["tstepcoeff2"], tcmn, marr, s["mJarray"],
+                       s["narray"], a["PhiFm"], DT, A, PHIBAR, TAURAD, TAUDRAG, True, True, expflag, s["sigma"],
+                       s["sigmaPhi"], None, 2)
+    assert len(got) == 13
+    eta_mn, delta_mn, Phi_mn = _oracle("explicit" if expflag else "modeuler", s, True, True)
+    ref = []
+    for spec in (eta_mn, delta_mn, Phi_mn):
+        Xm = orc.invrs_leg(spec, I, J, M, N, s["Pmn"])
+        ref += [spec, orc.invrs_fft(Xm, I), Xm[:, : M + 1]]
+    ref += list(orc.invrs_uv(delta_mn, eta_mn, fmn, I, J, M, N, s["Pmn"], s["Hmn"], tcmn, marr))
+    names = ("etamn", "eta", "etam", "deltamn", "delta", "deltam", "Phimn", "Phi", "Phim", "U", "V", "Um", "Vm")
+    for name, g, r in zip(names, got, ref):
+        g = _np(g)
+        assert g.shape == r.shape, name
+        tol = 1e-12 if name.endswith("mn") else 1e-11
+        assert float(np.max(np.abs(g - r))) <= tol * float(np.max(np.abs(r))), name
