@@ -231,3 +231,21 @@ def test_standalone_transforms_refuse_bad_arguments():
     assert b"no output" in lib.swx_last_error()
     args[7] = 1
     assert lib.swx_invrs_uv(*args, p, p, None, None, None) == E
+
+
+def test_legacy_main_conventions_on_the_host():
+    """main_function.main refuses test codes beyond None/0/1/2 and the default plotflag=1 with
+    NotImplementedError before any device work (main_function.py:166-176); without a GPU a valid
+    call raises rather than computing."""
+    import torch
+
+    from my_swampe_amd.main_function import main
+
+    args = (42, 1200.0, 6, 3.0e5, 3.2e-5, 8.2e7)
+    with pytest.raises(NotImplementedError, match="test=3"):
+        main(*args, 3, plotflag=0)
+    with pytest.raises(NotImplementedError, match="plotflag"):
+        main(*args, 0)
+    if not torch.cuda.is_available():
+        with pytest.raises(RuntimeError, match="no CPU fallback"):
+            main(*args, 0, plotflag=0, saveflag=0, verbose=False)

@@ -115,3 +115,20 @@ def test_measured_ceilings_are_plausible(torch_gpu):
         assert 0.1 * sheet < c[key] < 1.1 * sheet, (key, c[key])
     with pytest.raises(ValueError, match="256 MiB"):
         measure_ceilings(1 << 20)
+
+
+def test_legacy_main_equals_run_model(torch_gpu):
+    """main_function.main with the legacy conventions (test=0 for forced, 0/1 flags, the unused
+    physics constants) returns what run_model returns for the same run, array for array, on the
+    forced fixture case (unit_tests/fixtures/reference_case_forced_default.npz parameters)."""
+    from my_swampe_amd.main_function import main
+    from my_swampe_amd.model import run_model
+
+    print("[gpu-case] legacy main", flush=True)
+    kw = dict(taurad=36000.0, taudrag=21600.0, DPhieq=1.0e6, verbose=False)
+    got = main(42, 1200.0, 6, 3.0e5, 3.2e-5, 8.2e7, 0, plotflag=0, saveflag=0, diffflag=1, modalflag=1, k1=1.0,
+               sigmaSB=0.0, **kw)
+    want = run_model(42, 1200.0, 6, 3.0e5, 3.2e-5, 8.2e7, None, plotflag=False, saveflag=False, **kw)
+    assert set(got) == set(want)
+    for k in want:
+        assert np.array_equal(np.asarray(got[k]), np.asarray(want[k])), k
