@@ -3,7 +3,7 @@
 Public API mirrors my_swampe/__init__.py:64-71 (run_model, run_model_gpu,
 run_model_scan, run_model_scan_final, assert_finite_state) and its submodules
 continuation, spectral_transform, initial_conditions, filters, forcing,
-explicit_tdiff, modEuler_tdiff, time_stepping, main_function (the legacy main) and
+explicit_tdiff, modEuler_tdiff, semi_implicit_tdiff, time_stepping, main_function (the legacy main) and
 model, plus the batched ensemble driver
 (plan) and the differentiable rollouts (autograd).  All time stepping runs in the HIP kernels of
 my_swampe_amd/_lib/libswampe_hip.so.
@@ -18,6 +18,7 @@ from . import (  # noqa: F401
     main_function,
     modEuler_tdiff,
     model,
+    semi_implicit_tdiff,
     spectral_transform,
     time_stepping,
 )
@@ -38,5 +39,6 @@ __version__ = "0.1.0"
 __all__ = [
     "RunFlags", "State", "Static", "assert_finite_state", "build_static", "run_model", "run_model_gpu",
     "run_model_scan", "run_model_scan_final", "__version__", "continuation", "explicit_tdiff", "filters",
-    "forcing", "initial_conditions", "main_function", "modEuler_tdiff", "model", "spectral_transform", "time_stepping",
+    "forcing", "initial_conditions", "main_function", "modEuler_tdiff", "model", "semi_implicit_tdiff", "spectral_transform",
+    "time_stepping",
 ]

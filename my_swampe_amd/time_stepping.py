@@ -1,5 +1,6 @@
-"""The reference's time_stepping module (src/my_swampe/time_stepping.py): the one-member step
-``tstepping`` (:16-384), the coefficient arrays (:491-534) and the RMS-wind diagnostic (:537-576),
+"""The reference's time_stepping module (src/my_swampe/time_stepping.py): the one-member steps
+``tstepping`` (:16-384) and ``tstepping_semi_implicit`` (:387-488), the coefficient arrays
+(:491-534) and the RMS-wind diagnostic (:537-576),
 with the same function names, arguments and return values.
 
 The time step of the model runs in the fused HIP kernels: one plan step per call of
@@ -103,3 +104,17 @@ def tstepping(etam0, etam1, deltam0, deltam1, Phim0, Phim1, I, J, M, N, Am, Bm, 
     etamn, eta, etam = scheme.eta_timestep(*args)
     U, V, Um_new, Vm_new = invrsUV_with_coeffs(deltamn, etamn, fmn, I, J, M, N, Pmn, Hmn, tstepcoeffmn, marray)
     return etamn, eta, etam, deltamn, delta, deltam, Phimn, Phi, Phim, U, V, Um_new, Vm_new
+
+
+def tstepping_semi_implicit(Rum_lag, Rvm_lag, etam0, etam1, deltam0, deltam1, Phim0, Phim1, I, J, M, N, Am, Bm, Cm,
+                            Dm, Em, Fm, Gm, Um, Vm, fmn, Pmn, Hmn, Pmnw, Hmnw, tstepcoeff, tstepcoeff2, tstepcoeffmn,
+                            marray, mJarray, narray, PhiFm, dt, a, Phibar, taurad, taudrag, forcflag, diffflag,
+                            relax_implicit, sigma_exp, sigmaPhi_exp, si_alpha, test, t):
+    """The semi-implicit leapfrog step with tstepping's 13-array return (time_stepping.py:387-488):
+    semi_implicit_tdiff.si_timestep."""
+    from .semi_implicit_tdiff import si_timestep
+
+    return si_timestep(Rum_lag, Rvm_lag, etam0, etam1, deltam0, deltam1, Phim0, Phim1, I, J, M, N, Am, Bm, Cm, Dm, Em,
+                       Fm, Gm, Um, Vm, fmn, Pmn, Hmn, Pmnw, Hmnw, tstepcoeff, tstepcoeff2, tstepcoeffmn, marray, mJarray,
+                       narray, PhiFm, dt, a, Phibar, taurad, taudrag, forcflag, diffflag, relax_implicit, sigma_exp,
+                       sigmaPhi_exp, si_alpha, test, t)

@@ -2,7 +2,8 @@
 tests of unit_tests/test_parity_quirks.py:85-317 on its small configuration (M = N = 2, J = 4,
 I = 8, seed 7), and all six updates against the oracle's modified-Euler and explicit updates (the
 restatement the fixtures pin end to end) at that size and at T42, with and without forcing and
-diffusion; and the one-member step time_stepping.tstepping of both schemes."""
+diffusion; the one-member step time_stepping.tstepping of both schemes; and the semi-implicit
+step (semi_implicit_tdiff.si_timestep, time_stepping.tstepping_semi_implicit)."""
 
 from __future__ import annotations
 
@@ -189,6 +190,58 @@ def test_tstepping_vs_oracle(expflag, torch_gpu):
                        s["sigmaPhi"], None, 2)
     assert len(got) == 13
     eta_mn, delta_mn, Phi_mn = _oracle("explicit" if expflag else "modeuler", s, True, True)
+    ref = []
+    for spec in (eta_mn, delta_mn, Phi_mn):
+        Xm = orc.invrs_leg(spec, I, J, M, N, s["Pmn"])
+        ref += [spec, orc.invrs_fft(Xm, I), Xm[:, : M + 1]]
+    ref += list(orc.invrs_uv(delta_mn, eta_mn, fmn, I, J, M, N, s["Pmn"], s["Hmn"], tcmn, marr))
+    names = ("etamn", "eta", "etam", "deltamn", "delta", "deltam", "Phimn", "Phi", "Phim", "U", "V", "Um", "Vm")
+    for name, g, r in zip(names, got, ref):
+        g = _np(g)
+        assert g.shape == r.shape, name
+        tol = 1e-12 if name.endswith("mn") else 1e-11
+        assert float(np.max(np.abs(g - r))) <= tol * float(np.max(np.abs(r))), name
+
+
+@pytest.mark.parametrize("relax,forcflag,diffflag", [(True, True, True), (False, True, True), (False, False, False)])
+@pytest.mark.parametrize("size", ["small", "T42"])
+def test_semi_implicit_step_vs_oracle(relax, forcflag, diffflag, size, torch_gpu):
+    """semi_implicit_tdiff.si_timestep (and time_stepping.tstepping_semi_implicit, the same call)
+    against the oracle's semi-implicit update (semi_implicit_tdiff.py:79-264; si_alpha = 0.5,
+    lagged momentum remainders, exponential hyperdiffusion): spectra within 1e-12, grids, Fourier
+    coefficients and winds within 1e-11 of each array's largest value."""
+    import torch
+
+    from my_swampe_amd import filters
+    from my_swampe_amd import initial_conditions as ic
+    from my_swampe_amd import time_stepping as ts
+    from my_swampe_amd.semi_implicit_tdiff import si_timestep
+
+    print(f"[gpu-case] semi-implicit {size} relax={relax} forc={forcflag} diff={diffflag}", flush=True)
+    s = _setup(2, 4, 8) if size == "small" else _setup(42, 64, 128, seed=17)
+    M, N, I, J = s["M"], s["N"], s["I"], s["J"]
+    a = s["arrays"]
+    rng = np.random.default_rng(23)
+    ru, rv = (rng.standard_normal((J, M + 1)) + 1j * rng.standard_normal((J, M + 1)) for _ in range(2))
+    fmn = ic.coriolismn(M, 7.2921159e-5)
+    tcmn, marr = ts.tstepcoeffmn(M, N, A), ts.marray(M, N)
+    sig_e, sigP_e = filters.sigma6_exponential(M, N, 1.0e27, A, DT), filters.sigma6Phi_exponential(M, N, 1.0e27, A, DT)
+    args = (ru, rv, a["etam0"], a["etam1"], a["deltam0"], a["deltam1"], a["Phim0"], a["Phim1"], I, J, M, N, a["Am"],
+            a["Bm"], a["Cm"], a["Dm"], a["Em"], a["Fm"], a["Gm"], a["Um"], a["Vm"], fmn, s["Pmn"], s["Hmn"], s["Pmnw"],
+            s["Hmnw"], s["tstepcoeff1"], s["tstepcoeff2"], tcmn, marr, s["mJarray"], s["narray"], a["PhiFm"], DT, A,
+            PHIBAR, TAURAD, TAUDRAG, forcflag, diffflag, relax, sig_e, sigP_e, 0.5, None, 2)
+    got = si_timestep(*args)
+    again = ts.tstepping_semi_implicit(*args)
+    assert len(got) == 13 and all(torch.equal(x, y) for x, y in zip(got, again))
+    a_ = s["arrays"]
+    state = SimpleNamespace(Am_curr=a_["Am"], Bm_curr=a_["Bm"], Cm_curr=a_["Cm"], Dm_curr=a_["Dm"], Em_curr=a_["Em"],
+                            Fm_curr=a_["Fm"], Gm_curr=a_["Gm"], PhiFm_curr=a_["PhiFm"], Rum_lag=ru, Rvm_lag=rv,
+                            etam_curr=a_["etam1"], deltam_curr=a_["deltam1"], Phim_curr=a_["Phim1"],
+                            etam_prev=a_["etam0"], deltam_prev=a_["deltam0"], Phim_prev=a_["Phim0"])
+    static = SimpleNamespace(Pmnw=s["Pmnw"], Hmnw=s["Hmnw"], mJarray=s["mJarray"], narray=s["narray"], fmn=fmn, dt=DT,
+                             a=A, Phibar=PHIBAR, taurad=TAURAD, taudrag=TAUDRAG, tstepcoeff=s["tstepcoeff1"],
+                             sigma_exp=sig_e, sigmaPhi_exp=sigP_e)
+    eta_mn, delta_mn, Phi_mn = orc._semi_implicit(state, static, forcflag, diffflag, relax, 0.5)
     ref = []
     for spec in (eta_mn, delta_mn, Phi_mn):
         Xm = orc.invrs_leg(spec, I, J, M, N, s["Pmn"])
