@@ -5,8 +5,6 @@ from __future__ import annotations
 
 from types import SimpleNamespace
 
-import numpy as np
-
 from . import spectral_transform as st
 from .plan import require_gpu
 
@@ -35,8 +33,8 @@ def bind(values):
             raise ValueError(f"{k} must have shape ({ns.J}, {ns.M + 1}); got {tuple(getattr(ns, k).shape)}")
     for k in _REAL:
         setattr(ns, k, st._dev(v[k], torch, torch.float64))
-    for k in _SCALARS:
-        setattr(ns, k, float(np.asarray(v[k])))
+    for k in _SCALARS:  # Python, numpy or 0-d torch scalars (float() reads a device tensor too)
+        setattr(ns, k, float(v[k]))
     ns.forcflag, ns.diffflag = bool(v["forcflag"]), bool(v["diffflag"])
     return ns
 

@@ -52,7 +52,10 @@ def Rfun(U, V, Q, Phi, Phibar, taudrag):
             return Ru, Rv
         return Ru - (U / taudrag), Rv - (V / taudrag)
     # an array of drag times (per member, broadcast against the fields): -1 entries drag-free
-    xp = np if not hasattr(taudrag, "detach") else __import__("torch")
+    if hasattr(taudrag, "detach"):
+        import torch as xp
+    else:
+        xp = np
     off = taudrag == -1
     td = xp.where(off, xp.ones_like(taudrag), taudrag)
     return xp.where(off, Ru, Ru - (U / td)), xp.where(off, Rv, Rv - (V / td))
