@@ -10,6 +10,7 @@ my_swampe_amd/_lib/libswampe_hip.so.
 """
 
 from . import (  # noqa: F401
+    autodiff_utils,
     continuation,
     explicit_tdiff,
     filters,
@@ -33,12 +34,13 @@ from .model import (  # noqa: F401
     run_model_scan,
     run_model_scan_final,
 )
+from .autodiff_utils import fwd_grad  # noqa: F401
 
 __version__ = "0.1.0"
 
 __all__ = [
     "RunFlags", "State", "Static", "assert_finite_state", "build_static", "run_model", "run_model_gpu",
-    "run_model_scan", "run_model_scan_final", "__version__", "continuation", "explicit_tdiff", "filters",
+    "run_model_scan", "run_model_scan_final", "__version__", "autodiff_utils", "continuation", "fwd_grad", "explicit_tdiff", "filters",
     "forcing", "initial_conditions", "main_function", "modEuler_tdiff", "model", "semi_implicit_tdiff", "spectral_transform",
     "time_stepping",
 ]

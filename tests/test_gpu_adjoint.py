@@ -93,6 +93,22 @@ def test_parameter_gradients_vs_hip_central_differences(grads):
             assert rel < 1e-6, (name, b, got[name][b], fd, rel)
 
 
+def test_fwd_grad_over_the_rollout_parameters(grads):
+    """fwd_grad of the loss as a function of theta = (taurad, taudrag, DPhieq) of both members
+    equals the adjoint's parameter gradients, which the central differences above pin."""
+    import torch
+
+    from my_swampe_amd import fwd_grad
+    from my_swampe_amd.autograd import rollout_phi
+
+    print("[gpu-case] fwd_grad over 6 rollout parameters", flush=True)
+    theta = torch.tensor(np.concatenate([TAURAD, TAUDRAG, DPHIEQ]), device="cuda")
+    loss = lambda t: _loss(rollout_phi(grads["c"], t[0:2], t[2:4], t[4:6], **CFG), grads["tgt"])  # noqa: E731
+    g = fwd_grad(loss, theta, chunk=2).cpu().numpy()
+    want = np.concatenate([grads["gtr"], grads["gtd"], grads["gdp"]])
+    np.testing.assert_allclose(g, want, rtol=1e-12, atol=0)
+
+
 def test_initial_state_gradient_vs_directional_difference(grads):
     import torch
 
