@@ -311,6 +311,18 @@ int swx_adjoint_step(swx_plan *plan, const double *state_t, void *stream);
 int swx_run_tape(swx_plan *plan, int t0, int nsteps, double *tape, void *stream);
 int swx_adjoint_sweep(swx_plan *plan, int nsteps, const double *tape, void *stream);
 int swx_adjoint_result(swx_plan *plan, double *lam_out, double *pgrad_out, void *stream);
+/* Gradients in the hyperdiffusion coefficients K6 (eta, delta) and K6Phi (Phi), the reverse
+ * mode of the sigma6 / sigma6Phi factors (filters.py:95-151) that jax.grad reaches through the
+ * scan (unit_tests/test_autodiff.py:64-86).  A step ends with S_{t+1} = sigma(K6) * P, so with
+ * the plan holding lam = dL/dS_{t+1}, swx_adjoint_diffusion(S_{t+1}) adds
+ * sum Re(conj(lam) dS_{t+1}/dK6) per member: call it for every state a step produced, before
+ * the reverse step that consumes that lam (swx_adjoint_sweep does so for the states inside its
+ * tape under SWX_OPT_ADJ_DIFFUSION; the caller passes the state past the tape's end, e.g. the
+ * final state after swx_adjoint_terminal).  swx_adjoint_diffusion_result copies the accumulated
+ * device float64 [B][2] (dL/dK6 over eta and delta, dL/dK6Phi over Phi; with K6Phi following
+ * K6, dL/dK6 is their sum).  Zero without diffflag.  State layout as swx_state_get. */
+int swx_adjoint_diffusion(swx_plan *plan, const double *state, void *stream);
+int swx_adjoint_diffusion_result(swx_plan *plan, double *out, void *stream);
 
 /* Phase-curve observable (replaces the projection of
  * unit_tests/test_parity_reference_regression.py:67-82, the retrieval's
@@ -433,6 +445,9 @@ int swx_invrs_uv(const double *deltamn, const double *etamn, const double *fmn, 
  *     the latitudes on T(j) +- T(J-1-j); half the matrix-core work), 1 = over all latitudes.
  *     Results agree to FP32 rounding. */
 #define SWX_OPT_LEG_FOLD 9
+/*   SWX_OPT_ADJ_DIFFUSION: 1 = swx_adjoint_sweep also accumulates the diffusion partials
+ *     (swx_adjoint_diffusion) of the pairs inside its tape; 0 (default) = it does not. */
+#define SWX_OPT_ADJ_DIFFUSION 10
 int swx_plan_set_option(swx_plan *plan, int option, int value);
 /* Current value of an option (for SWX_OPT_LEG_MB and SWX_OPT_GRID_FOLD: the
  * choice in use). */

@@ -29,6 +29,7 @@ OPT_GRID_FOLD = 6
 OPT_CONCURRENT = 7
 OPT_PHYS_LINEAGE = 8
 OPT_LEG_FOLD = 9
+OPT_ADJ_DIFFUSION = 10
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
 
 PARAM_NAMES = ("dt", "a", "omega", "g", "Phibar", "taurad", "taudrag", "DPhieq", "K6", "K6Phi", "alpha", "blowup_rms",
@@ -41,7 +42,8 @@ EXPORTED_SYMBOLS = (
     "swx_init", "swx_run", "swx_export_state", "swx_run_graph", "swx_time_kernels", "swx_plan_set_option",
     "swx_init_spectral", "swx_state_get", "swx_state_set", "swx_adjoint_begin", "swx_adjoint_terminal", "swx_adjoint_observe",
     "swx_phase_curve", "swx_phase_curve_vjp", "swx_plan_phase_curve", "swx_set_prev_level",
-    "swx_adjoint_step", "swx_adjoint_result", "swx_last_error", "swx_version", "swx_plan_device_bytes",
+    "swx_adjoint_step", "swx_adjoint_result", "swx_adjoint_diffusion", "swx_adjoint_diffusion_result",
+    "swx_last_error", "swx_version", "swx_plan_device_bytes",
     "swx_plan_step_counts", "swx_export_state_noalloc", "swx_ensemble_create", "swx_ensemble_run_graph", "swx_ensemble_destroy", "swx_plan_get_option",
     "swx_pipeline_create", "swx_pipeline_run_graph", "swx_pipeline_destroy", "swx_run_tape", "swx_adjoint_sweep",
     "swx_diagnostic_eta_delta", "swx_measure_ceilings", "swx_gauss_legendre", "swx_lambdas", "swx_legendre_basis",
@@ -116,6 +118,8 @@ def load() -> ctypes.CDLL:
     lib.swx_run_tape.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p]
     lib.swx_adjoint_sweep.argtypes = [c_void_p, c_int, c_void_p, c_void_p]
     lib.swx_adjoint_result.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p]
+    lib.swx_adjoint_diffusion.argtypes = [c_void_p, c_void_p, c_void_p]
+    lib.swx_adjoint_diffusion_result.argtypes = [c_void_p, c_void_p, c_void_p]
     lib.swx_phase_curve.argtypes = [c_void_p, ctypes.c_longlong, c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_int,
                                     c_void_p, c_void_p, c_void_p]
     lib.swx_phase_curve_vjp.argtypes = [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_int,
@@ -150,7 +154,8 @@ def load() -> ctypes.CDLL:
     for name in ("swx_geometry", "swx_grid_size", "swx_plan_create", "swx_plan_set_params", "swx_init",
                  "swx_run", "swx_export_state", "swx_run_graph", "swx_time_kernels", "swx_plan_set_option",
                  "swx_init_spectral", "swx_state_get", "swx_state_set", "swx_adjoint_begin",
-                 "swx_adjoint_terminal", "swx_adjoint_step", "swx_adjoint_result", "swx_plan_device_bytes",
+                 "swx_adjoint_terminal", "swx_adjoint_step", "swx_adjoint_result", "swx_adjoint_diffusion",
+                 "swx_adjoint_diffusion_result", "swx_plan_device_bytes",
                  "swx_plan_step_counts", "swx_export_state_noalloc",
                  "swx_ensemble_create", "swx_ensemble_run_graph", "swx_plan_get_option", "swx_adjoint_observe",
                  "swx_phase_curve", "swx_phase_curve_vjp", "swx_plan_phase_curve", "swx_set_prev_level",

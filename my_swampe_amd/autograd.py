@@ -12,8 +12,11 @@ no recompute) when the whole trajectory fits RolloutConfig.checkpoint_budget
 (32 GiB of device memory: 1400 steps of T42 x 256 members), else
 K = ceil(sqrt(nsteps)); checkpoint_every sets K explicitly.
 
-Gradients are returned for c (complex, d/dRe + i d/dIm) and for the
-per-member taurad, taudrag and DPhieq; the other physical constants are fixed.
+Gradients are returned for c (complex, d/dRe + i d/dIm), for the
+per-member taurad, taudrag and DPhieq, and for the hyperdiffusion coefficients
+K6 and K6Phi when rollout_phi is given them as tensors (their partials come
+from the diffusion factor that ends each step); the other physical constants
+are fixed.
 This replaces reverse-mode differentiation of the reference's scan
 (model.py:1281-1342 under jax.grad, unit_tests/test_autodiff.py:53-185).
 """
@@ -68,14 +71,23 @@ class RolloutConfig:
         return max(1, int(math.ceil(math.sqrt(max(self.nsteps, 1)))))
 
 
-def _params(cfg: RolloutConfig, taurad, taudrag, DPhieq) -> np.ndarray:
+def _params(cfg: RolloutConfig, taurad, taudrag, DPhieq, K6=None, K6Phi=None) -> np.ndarray:
+    """Parameter rows; K6 / K6Phi (per member or shared) default to the configuration's, and a
+    K6Phi of None follows K6."""
     tr = np.atleast_1d(np.asarray(taurad, dtype=np.float64))
     td = np.atleast_1d(np.asarray(taudrag, dtype=np.float64))
     dp = np.atleast_1d(np.asarray(DPhieq, dtype=np.float64))
-    B = max(tr.size, td.size, dp.size)
-    tr, td, dp = (np.broadcast_to(x, (B,)) for x in (tr, td, dp))
+    k6 = np.atleast_1d(np.asarray(cfg.K6 if K6 is None else K6, dtype=np.float64))
+    if K6Phi is None and cfg.K6Phi is None:
+        k6p = None
+    else:
+        k6p = np.atleast_1d(np.asarray(cfg.K6Phi if K6Phi is None else K6Phi, dtype=np.float64))
+    B = max(x.size for x in (tr, td, dp, k6) + (() if k6p is None else (k6p,)))
+    tr, td, dp, k6 = (np.broadcast_to(x, (B,)) for x in (tr, td, dp, k6))
+    k6p = None if k6p is None else np.broadcast_to(k6p, (B,))
     return np.stack([param_row(dt=cfg.dt, a=cfg.a, omega=cfg.omega, g=cfg.g, Phibar=cfg.Phibar, taurad=tr[b],
-                               taudrag=td[b], DPhieq=dp[b], K6=cfg.K6, K6Phi=cfg.K6Phi, alpha=cfg.alpha)
+                               taudrag=td[b], DPhieq=dp[b], K6=k6[b], K6Phi=None if k6p is None else k6p[b],
+                               alpha=cfg.alpha)
                      for b in range(B)])
 
 
@@ -135,11 +147,24 @@ def forward_with_checkpoints(plan: Plan, cfg: RolloutConfig, c):
     return ckpts
 
 
-def adjoint_through_checkpoints(plan: Plan, cfg: RolloutConfig, ckpts, gPhi):
-    """Reverse sweep: terminal cotangent gPhi (B, J, I) -> (dL/dc, dL/d(taurad, taudrag, DPhieq))."""
+def adjoint_through_checkpoints(plan: Plan, cfg: RolloutConfig, ckpts, gPhi, final=None):
+    """Reverse sweep: terminal cotangent gPhi (B, J, I) -> (dL/dc, dL/d(taurad, taudrag, DPhieq)).
+    With `final` (the state after the last step) the sweep also accumulates the diffusion partials
+    (dL/dK6, dL/dK6Phi) of every state a step produced, read by plan.adjoint_diffusion_result()."""
     K = cfg.every()
     plan.adjoint_begin()
     plan.adjoint_terminal(g_Phi=gPhi)
+    diffusion = final is not None
+    if diffusion:
+        plan.adjoint_diffusion(final)
+        plan.set_option(_capi.OPT_ADJ_DIFFUSION, 1)
+    try:
+        return _sweep_checkpoints(plan, cfg, ckpts, K, diffusion)
+    finally:
+        plan.set_option(_capi.OPT_ADJ_DIFFUSION, 0)
+
+
+def _sweep_checkpoints(plan: Plan, cfg: RolloutConfig, ckpts, K: int, diffusion: bool):
     if not isinstance(ckpts, list):  # a run_tape trajectory: one native reverse sweep
         plan.adjoint_sweep(ckpts)
         cfg.stats.update(checkpoints=int(ckpts.shape[0]), segment_states=1)
@@ -153,6 +178,8 @@ def adjoint_through_checkpoints(plan: Plan, cfg: RolloutConfig, ckpts, gPhi):
         if n == 1:
             plan.adjoint_step(ckpts[seg])
             peak = max(peak, 1)
+            if diffusion and start > 0:  # the segment's first state came out of a step
+                plan.adjoint_diffusion(ckpts[seg])
             continue
         # Recompute the segment's states from its checkpoint in one native call (the states
         # before its first n - 1 steps), take the last one, and sweep the segment natively.
@@ -163,6 +190,8 @@ def adjoint_through_checkpoints(plan: Plan, cfg: RolloutConfig, ckpts, gPhi):
         seg_tape[n - 1].copy_(plan.state_get())
         peak = max(peak, n)
         plan.adjoint_sweep(seg_tape[:n])
+        if diffusion and start > 0:
+            plan.adjoint_diffusion(ckpts[seg])
     plan.set_option(_capi.OPT_PHYS_STEPS, 1)
     cfg.stats.update(checkpoints=len(ckpts), segment_states=peak)
     return plan.adjoint_result()
@@ -294,28 +323,39 @@ class _Rollout:
 
         class Fn(torch.autograd.Function):
             @staticmethod
-            def forward(ctx, c, taurad, taudrag, DPhieq, cfg: RolloutConfig):
-                params = _params(cfg, taurad.detach().cpu().numpy(), taudrag.detach().cpu().numpy(),
-                                 DPhieq.detach().cpu().numpy())
+            def forward(ctx, c, taurad, taudrag, DPhieq, K6, K6Phi, cfg: RolloutConfig):
+                host = lambda x: None if x is None else x.detach().cpu().numpy()  # noqa: E731
+                params = _params(cfg, host(taurad), host(taudrag), host(DPhieq), host(K6), host(K6Phi))
                 plan = _make_plan(cfg, params)
                 ckpts = forward_with_checkpoints(plan, cfg, c.detach())
                 phi = plan.export(fields=("Phi_curr",))["Phi_curr"]
+                # The diffusion partials need the state after the last step.
+                ctx.final = plan.state_get() if any(ctx.needs_input_grad[4:6]) else None
                 torch.cuda.current_stream(plan.device).synchronize()
                 ctx.plan, ctx.cfg, ctx.ckpts = plan, cfg, ckpts
-                ctx.shapes = (taurad.shape, taudrag.shape, DPhieq.shape)
+                ctx.shapes = (taurad.shape, taudrag.shape, DPhieq.shape, K6.shape,
+                              None if K6Phi is None else K6Phi.shape)
                 return phi
 
             @staticmethod
             def backward(ctx, gPhi):
                 plan, cfg = ctx.plan, ctx.cfg
-                lam, pg = adjoint_through_checkpoints(plan, cfg, ctx.ckpts, gPhi.contiguous())
+                lam, pg = adjoint_through_checkpoints(plan, cfg, ctx.ckpts, gPhi.contiguous(), ctx.final)
+                gk6 = gk6p = None
+                if ctx.final is not None:
+                    dg = plan.adjoint_diffusion_result()
+                    sk, skp = ctx.shapes[3:]
+                    # With K6Phi following K6, K6 also scales the Phi factor.
+                    gk6 = _fold(dg[:, 0] + dg[:, 1] if skp is None else dg[:, 0], sk)
+                    gk6p = None if skp is None else _fold(dg[:, 1], skp)
                 torch.cuda.current_stream(plan.device).synchronize()
                 cfg.stats["plan_device_bytes"] = plan.device_bytes()
                 _release_plan(cfg, plan)
                 ctx.plan = None
                 ctx.ckpts = None
-                sr, sd, sp = ctx.shapes
-                return lam, _fold(pg[:, 0], sr), _fold(pg[:, 1], sd), _fold(pg[:, 2], sp), None
+                ctx.final = None
+                sr, sd, sp = ctx.shapes[:3]
+                return lam, _fold(pg[:, 0], sr), _fold(pg[:, 1], sd), _fold(pg[:, 2], sp), gk6, gk6p, None
 
         cls._cls = Fn
         return Fn
@@ -406,8 +446,9 @@ def rollout_phi(c, taurad, taudrag, DPhieq, *, M: int, nsteps: int, dt: float, a
                 modalflag: bool = True, forcflag: bool = True, stats: Optional[dict] = None):
     """Final Phi grid (B, J, I) of a differentiable rollout (see module docstring).
 
-    c: complex128 tensor (B, 3, M+1, N+1); taurad, taudrag, DPhieq: float64 tensors of
-    shape (B,) or scalars (shared by all members).
+    c: complex128 tensor (B, 3, M+1, N+1); taurad, taudrag, DPhieq, and K6 / K6Phi: float64
+    tensors of shape (B,) or scalars (shared by all members).  K6Phi = None follows K6 (then
+    dL/dK6 includes the Phi factor's share).
     """
     torch = require_gpu()
     dev = torch.device("cuda", torch.cuda.current_device())
@@ -415,7 +456,8 @@ def rollout_phi(c, taurad, taudrag, DPhieq, *, M: int, nsteps: int, dt: float, a
                         alpha=alpha, precision=precision, checkpoint_every=checkpoint_every, diffflag=diffflag,
                         modalflag=modalflag, forcflag=forcflag, stats=stats if stats is not None else {})
     conv = lambda x: x if torch.is_tensor(x) else torch.tensor(x, dtype=torch.float64, device=dev)  # noqa: E731
-    return _Rollout.get().apply(c, conv(taurad), conv(taudrag), conv(DPhieq), cfg)
+    return _Rollout.get().apply(c, conv(taurad), conv(taudrag), conv(DPhieq), conv(K6),
+                                None if K6Phi is None else conv(K6Phi), cfg)
 
 
 def rollout_phi_series(c, taurad, taudrag, DPhieq, *, obs_steps, M: int, dt: float, a: float, omega: float,

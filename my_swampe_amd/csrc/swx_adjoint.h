@@ -57,6 +57,10 @@ hipError_t launch_leg_adj(const LegAdjArgs<R>& a, hipStream_t s);
 template <typename R>
 hipError_t launch_grid_adj(const GridAdjArgs<R>& a, hipStream_t s);
 hipError_t launch_pgrad_reduce(const double* pg, double* out, int B, int J, hipStream_t s);
+// acc[B][2] += the diffusion partials (dL/dK6, dL/dK6Phi) of the pair lam = dL/dS, S (both
+// [B][3][M+1][N+1] complex128) under the coefficient table coef [B][ncoef].
+hipError_t launch_diffusion_partial(const double* Lam, const double* S, const double* coef, int ncoef, int M, int N,
+                                    int J, int B, double* acc, hipStream_t s);
 // MFMA grid adjoint (GADJ_STEP, FP32; the MFMA grid kernel's DFT tables), swx_adjoint_mfma.hip.
 // grid_adj_mfma_lds_bytes is 0 where it is not provided (T106).
 hipError_t launch_grid_adj_mfma(const GridAdjArgs<float>& a, const float* TinvF, const float* TfwdF, hipStream_t s);

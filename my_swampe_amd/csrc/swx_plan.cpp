@@ -181,6 +181,8 @@ struct swx_plan {
   void* Tbar = nullptr;
   void* Gbar = nullptr;
   double* pgrad = nullptr;
+  double* dgrad = nullptr;        // diffusion partials [B][2] (dL/dK6, dL/dK6Phi)
+  bool opt_adj_diffusion = false; // SWX_OPT_ADJ_DIFFUSION
   void* adj_stc = nullptr;       // S_t cache of the folded MFMA grid adjoint (T106)
   // captured forward step sequences (swx_run_graph)
   GraphCache graphs;
@@ -676,7 +678,8 @@ int ensure_adjoint(swx_plan* p) {
   int rc;
   if ((rc = dalloc(p, &p->Lam, (size_t)B * 3 * Mp * Np * 2 * 8)) ||
       (rc = dalloc(p, &p->Tbar, (size_t)B * Mp * kNT * J * 2 * p->rsz)) ||
-      (rc = dalloc(p, &p->Gbar, (size_t)B * kNG * J * Mp * 2 * p->rsz)) || (rc = dalloc(p, &p->pgrad, (size_t)B * J * 3 * 8)))
+      (rc = dalloc(p, &p->Gbar, (size_t)B * kNG * J * Mp * 2 * p->rsz)) || (rc = dalloc(p, &p->pgrad, (size_t)B * J * 3 * 8)) ||
+      (rc = dalloc(p, &p->dgrad, (size_t)B * 2 * 8)))
     return rc;
   const size_t stc = p->use_adj_fold ? grid_adj_mfma_fold_cache_bytes(p->geo.M, B, J) : 0;
   if (stc > 0 && (rc = dalloc(p, &p->adj_stc, stc))) return rc;
@@ -1409,7 +1412,7 @@ int swx_plan_set_params(swx_plan* p, const double* params, void* stream) {
 void swx_plan_destroy(swx_plan* p) {
   if (!p) return;
   p->graphs.clear();
-  void* bufs[] = {p->Lam, p->Tbar, p->Gbar, p->pgrad, p->adj_stc, p->Pf, p->Hf, p->Pi, p->Hi, p->fragP[0], p->fragP[1],
+  void* bufs[] = {p->Lam, p->Tbar, p->Gbar, p->pgrad, p->dgrad, p->adj_stc, p->Pf, p->Hf, p->Pi, p->Hi, p->fragP[0], p->fragP[1],
                   p->fragP[2], p->fragP[3], p->fragF[0], p->fragF[1], p->fragF[2], p->fragF[3], p->tw, p->TinvF, p->TfwdF, p->TinvH, p->TfwdH, p->TinvB, p->TfwdB, p->geo_d, p->f2, p->coef, p->S, p->T, p->G, p->Gprev,
                   p->raw, p->lag, p->g0uv, p->phys, p->uvic, p->part_uv, p->dead, p->R0f, p->R0p, p->resid_c, p->lagx,
                   p->Gsyn};
@@ -1791,6 +1794,10 @@ int swx_plan_set_option(swx_plan* p, int option, int value) {
       if (value == 2 && !p->fragF[0]) return fail(SWX_ERR_ARG, "SWX_OPT_LEG_FOLD: no folded tables for this plan");
       p->leg_fold = value;
       break;
+    case SWX_OPT_ADJ_DIFFUSION:
+      if (value != 0 && value != 1) return fail(SWX_ERR_ARG, "SWX_OPT_ADJ_DIFFUSION must be 0 or 1");
+      p->opt_adj_diffusion = value == 1;
+      break;
     case SWX_OPT_PHYS_STEPS:
       if (value < -1) return fail(SWX_ERR_ARG, "SWX_OPT_PHYS_STEPS must be >= -1");
       p->opt_phys_steps = value;
@@ -1898,6 +1905,7 @@ int swx_plan_get_option(const swx_plan* p, int option, int* value) {
     case SWX_OPT_PHYS_LINEAGE: *value = p->opt_lineage; break;
     case SWX_OPT_LEG_FOLD: *value = p->leg_fold == 2 && p->fragF[0] ? 2 : 1; break;
     case SWX_OPT_LEG_MB: *value = p->leg_mb; break;
+    case SWX_OPT_ADJ_DIFFUSION: *value = p->opt_adj_diffusion ? 1 : 0; break;
     case SWX_OPT_GRID_FOLD: *value = p->grid_fold; break;
     case SWX_OPT_CONCURRENT: *value = p->in_pipeline ? 1 : 0; break;
     default: return fail(SWX_ERR_ARG, "unknown option " + std::to_string(option));
@@ -1964,6 +1972,7 @@ int swx_adjoint_begin(swx_plan* p, void* stream) {
   const int B = p->d.n_members, J = p->geo.J, Mp = p->geo.M + 1, Np = p->geo.N + 1;
   SWX_HIP(hipMemsetAsync(p->Lam, 0, (size_t)B * 3 * Mp * Np * 2 * 8, s));
   SWX_HIP(hipMemsetAsync(p->pgrad, 0, (size_t)B * J * 3 * 8, s));
+  SWX_HIP(hipMemsetAsync(p->dgrad, 0, (size_t)B * 2 * 8, s));
   return SWX_OK;
 }
 
@@ -2016,7 +2025,13 @@ int swx_adjoint_sweep(swx_plan* p, int nsteps, const double* tape, void* stream)
   if (nsteps > 0 && !tape) return fail(SWX_ERR_ARG, "null tape");
   hipStream_t s = (hipStream_t)stream;
   const size_t per = (size_t)p->d.n_members * 3 * (p->geo.M + 1) * (p->geo.N + 1) * 2;
+  const bool diffusion = p->opt_adj_diffusion && p->d.diffflag;
   for (int k = nsteps - 1; k >= 0; --k) {
+    // The pair (dL/dS_{k+1}, S_{k+1}) inside the tape; the caller adds the one past its end
+    // (swx_adjoint_diffusion) before the sweep.
+    if (diffusion && k < nsteps - 1)
+      SWX_HIP(launch_diffusion_partial(p->Lam, tape + (k + 1) * per, p->coef, p->cl.ncoef, p->geo.M, p->geo.N,
+                                       p->geo.J, p->d.n_members, p->dgrad, s));
     rc = p->rsz == 8 ? adjoint_step<double>(p, tape + k * per, s) : adjoint_step<float>(p, tape + k * per, s);
     if (rc) return rc;
   }
@@ -2030,6 +2045,26 @@ int swx_adjoint_step(swx_plan* p, const double* state_t, void* stream) {
   if (!state_t) return fail(SWX_ERR_ARG, "null state");
   hipStream_t s = (hipStream_t)stream;
   return p->rsz == 8 ? adjoint_step<double>(p, state_t, s) : adjoint_step<float>(p, state_t, s);
+}
+
+int swx_adjoint_diffusion(swx_plan* p, const double* state, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!p->Lam) return fail(SWX_ERR_STATE, "swx_adjoint_diffusion before swx_adjoint_begin");
+  if (!state) return fail(SWX_ERR_ARG, "null state");
+  if (!p->d.diffflag) return SWX_OK;  // no diffusion factor: the partials stay zero
+  SWX_HIP(launch_diffusion_partial(p->Lam, state, p->coef, p->cl.ncoef, p->geo.M, p->geo.N, p->geo.J,
+                                   p->d.n_members, p->dgrad, (hipStream_t)stream));
+  return SWX_OK;
+}
+
+int swx_adjoint_diffusion_result(swx_plan* p, double* out, void* stream) {
+  int rc = check_plan(p);
+  if (rc) return rc;
+  if (!p->Lam) return fail(SWX_ERR_STATE, "swx_adjoint_diffusion_result before swx_adjoint_begin");
+  if (!out) return fail(SWX_ERR_ARG, "null output");
+  SWX_HIP(hipMemcpyAsync(out, p->dgrad, (size_t)p->d.n_members * 2 * 8, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return SWX_OK;
 }
 
 int swx_adjoint_result(swx_plan* p, double* lam_out, double* pgrad_out, void* stream) {

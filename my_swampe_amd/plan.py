@@ -264,6 +264,21 @@ class Plan:
                     "swx_adjoint_result")
         return lam, pg
 
+    def adjoint_diffusion(self, state, stream=None) -> None:
+        """Add the diffusion partials of the pair (the held dL/dS, S = state): state is the spectral
+        state whose cotangent the plan holds, produced by a step."""
+        t = self._spec(state)
+        self._keep_diff_state = t
+        _capi.check(self._lib.swx_adjoint_diffusion(self._h, _ptr(t), self._stream(stream)), "swx_adjoint_diffusion")
+
+    def adjoint_diffusion_result(self, stream=None):
+        """Accumulated (dL/dK6 over eta and delta, dL/dK6Phi over Phi) as float64 (B, 2)."""
+        torch = _torch()
+        out = torch.empty((self.B, 2), dtype=torch.float64, device=self.device)
+        _capi.check(self._lib.swx_adjoint_diffusion_result(self._h, _ptr(out), self._stream(stream)),
+                    "swx_adjoint_diffusion_result")
+        return out
+
     def mus_device(self):
         """The plan's Gauss nodes (J,) as a float64 device tensor."""
         if self._mus_d is None:

@@ -93,6 +93,86 @@ def test_parameter_gradients_vs_hip_central_differences(grads):
             assert rel < 1e-6, (name, b, got[name][b], fd, rel)
 
 
+K6S = np.array([1.24e33, 1.24e33])
+K6PHIS = np.array([1.5e33, 0.9e33])
+
+
+def _diffusion_grads(torch, grads, shared_phi, every=None):
+    from my_swampe_amd.autograd import rollout_phi
+
+    dev = torch.device("cuda")
+    c = grads["c"].clone().requires_grad_(True)
+    tr, td, dp = (torch.tensor(x, device=dev, requires_grad=True) for x in (TAURAD, TAUDRAG, DPHIEQ))
+    k6 = torch.tensor(K6S, device=dev, requires_grad=True)
+    k6p = None if shared_phi else torch.tensor(K6PHIS, device=dev, requires_grad=True)
+    L = _loss(rollout_phi(c, tr, td, dp, **CFG, K6=k6, K6Phi=k6p, checkpoint_every=every), grads["tgt"])
+    L.backward()
+    return (c.grad, tr.grad, td.grad, dp.grad, k6.grad.cpu().numpy(),
+            None if shared_phi else k6p.grad.cpu().numpy())
+
+
+@pytest.mark.parametrize("shared_phi", [True, False])
+def test_hyperdiffusion_gradients_vs_hip_central_differences(grads, shared_phi):
+    """dL/dK6 and dL/dK6Phi per member (the reverse mode of the sigma6 / sigma6Phi factors) against
+    fourth-order central differences of the HIP forward, within 1e-6 relative; asking for them
+    leaves the other gradients bit for bit as they were, and every checkpoint layout (one tape,
+    segments of 4 + 2, 5 + 1) gives the same K6 gradients bit for bit."""
+    import torch
+
+    print(f"[gpu-case] K6 gradients shared_phi={shared_phi}", flush=True)
+    dev = torch.device("cuda")
+    gc, gtr, gtd, gdp, gk6, gk6p = _diffusion_grads(torch, grads, shared_phi)
+    if shared_phi:  # K6 = 1.24e33 is the fixture's default: the same rollout
+        np.testing.assert_array_equal(gc.cpu().numpy(), grads["gc"].cpu().numpy())
+        for g, name in ((gtr, "gtr"), (gtd, "gtd"), (gdp, "gdp")):
+            np.testing.assert_array_equal(g.cpu().numpy(), grads[name])
+    for every in (4, 5):
+        _, _, _, _, k6e, k6pe = _diffusion_grads(torch, grads, shared_phi, every)
+        np.testing.assert_array_equal(k6e, gk6)
+        if not shared_phi:
+            np.testing.assert_array_equal(k6pe, gk6p)
+    cases = [("K6", K6S, gk6)] + ([] if shared_phi else [("K6Phi", K6PHIS, gk6p)])
+    for name, base, got in cases:
+        for b in range(2):
+            def f(dh):
+                kw = {"K6": torch.tensor(K6S, device=dev),
+                      "K6Phi": None if shared_phi else torch.tensor(K6PHIS, device=dev)}
+                v = base.copy()
+                v[b] += dh
+                kw[name] = torch.tensor(v, device=dev)
+                phi = _forward(torch, grads["c"], *(torch.tensor(x, device=dev) for x in (TAURAD, TAUDRAG, DPHIEQ)),
+                               **kw)
+                return float(_loss(phi, grads["tgt"]))
+            fd = _richardson(f, 1e-3 * base[b])
+            rel = abs(got[b] - fd) / max(abs(fd), 1e-300)
+            print(name, b, "adjoint", got[b], "fd", fd, "rel", rel)
+            assert got[b] != 0.0 and rel < 1e-6, (name, b, got[b], fd, rel)
+
+
+def test_hyperdiffusion_gradient_vs_oracle_central_differences(grads):
+    """dL/dK6 (K6Phi following K6) against central differences of the FP64 NumPy oracle."""
+    import torch
+
+    N, I, J, lam, mus, w, P, H = orc.cached_geometry(42)
+    c = grads["c"].cpu().numpy()
+    tgt = grads["tgt"].cpu().numpy()
+    got = _diffusion_grads(torch, grads, True)[4]
+    for b in range(2):
+        fields = [np.real(orc.invrs_fft(orc.invrs_leg(c[b, f], I, J, 42, 42, P), I)) for f in range(3)]
+
+        def oracle_loss(dh):
+            out = orc.run_model_scan(M=42, dt=CFG["dt"], tmax=2 + CFG["nsteps"], Phibar=CFG["Phibar"],
+                                     omega=CFG["omega"], a=CFG["a"], taurad=TAURAD[b], taudrag=TAUDRAG[b],
+                                     DPhieq=DPHIEQ[b], K6=K6S[b] + dh, eta0_init=fields[0], delta0_init=fields[1],
+                                     Phi0_init=fields[2], diagnostics=False, return_history=False)
+            return float(np.mean((out["last_state"].Phi_curr - tgt) ** 2))
+
+        fd = _richardson(oracle_loss, 1e-3 * K6S[b])
+        rel = abs(got[b] - fd) / max(abs(fd), 1e-300)
+        print("oracle K6", b, got[b], fd, rel)
+        assert rel < 1e-6, (b, got[b], fd, rel)
+
+
 def test_fwd_grad_over_the_rollout_parameters(grads):
     """fwd_grad of the loss as a function of theta = (taurad, taudrag, DPhieq) of both members
     equals the adjoint's parameter gradients, which the central differences above pin."""
