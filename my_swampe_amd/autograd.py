@@ -223,14 +223,19 @@ def _forward_observed(plan: Plan, cfg: RolloutConfig, c, obs_steps, observe):
     return ckpts
 
 
-def _adjoint_observed(plan: Plan, cfg: RolloutConfig, ckpts, obs_steps, cotangent):
+def _adjoint_observed(plan: Plan, cfg: RolloutConfig, ckpts, obs_steps, cotangent, final=None):
     """Reverse sweep whose observation k adds cotangent(k) (the (B, J, I) cotangent of Phi after
-    obs_steps[k] steps) when the sweep reaches that state."""
+    obs_steps[k] steps) when the sweep reaches that state.  With `final` (the state after the last
+    step) it also accumulates the diffusion partials of every state a step produced, each once its
+    cotangent is whole (after that state's observation)."""
     K = cfg.every()
     where = {int(n): k for k, n in enumerate(obs_steps)}
+    diffusion = final is not None and cfg.nsteps > 0
     plan.adjoint_begin()
     if cfg.nsteps in where:
         plan.adjoint_terminal(g_Phi=cotangent(where[cfg.nsteps]))
+    if diffusion:
+        plan.adjoint_diffusion(final)
     plan.set_option(_capi.OPT_PHYS_STEPS, -1)
     for seg in reversed(range(len(ckpts))):
         start = seg * K
@@ -245,6 +250,8 @@ def _adjoint_observed(plan: Plan, cfg: RolloutConfig, ckpts, obs_steps, cotangen
             plan.adjoint_step(tape[s])
             if start + s in where:
                 plan.adjoint_observe(g_Phi=cotangent(where[start + s]))
+            if diffusion and start + s > 0:
+                plan.adjoint_diffusion(tape[s])
     plan.set_option(_capi.OPT_PHYS_STEPS, 1)
     cfg.stats.update(checkpoints=len(ckpts))
     return plan.adjoint_result()
@@ -259,10 +266,10 @@ def forward_series(plan: Plan, cfg: RolloutConfig, c, obs_steps):
     return ckpts, torch.stack(obs, dim=1)
 
 
-def adjoint_series(plan: Plan, cfg: RolloutConfig, ckpts, obs_steps, gPhi):
+def adjoint_series(plan: Plan, cfg: RolloutConfig, ckpts, obs_steps, gPhi, final=None):
     """Reverse sweep with the cotangents gPhi (B, n_obs, J, I) of Phi after each observed step count
     added when the sweep reaches that state."""
-    return _adjoint_observed(plan, cfg, ckpts, obs_steps, lambda k: gPhi[:, k].contiguous())
+    return _adjoint_observed(plan, cfg, ckpts, obs_steps, lambda k: gPhi[:, k].contiguous(), final)
 
 
 def forward_phase_curve(plan: Plan, cfg: RolloutConfig, c, obs_steps, phases, normalize: bool):
@@ -283,7 +290,8 @@ def forward_phase_curve(plan: Plan, cfg: RolloutConfig, c, obs_steps, phases, no
     return ckpts, flux.permute(1, 0, 2).contiguous(), raw
 
 
-def adjoint_phase_curve(plan: Plan, cfg: RolloutConfig, ckpts, obs_steps, phases, normalize: bool, raw, g_flux):
+def adjoint_phase_curve(plan: Plan, cfg: RolloutConfig, ckpts, obs_steps, phases, normalize: bool, raw, g_flux,
+                        final=None):
     """Reverse sweep of a loss of the phase curves: g_flux (B, n_obs, K); each observation's grid
     cotangent is formed on the GPU (swx_phase_curve_vjp) when the sweep reaches it."""
     from .observables import phase_curve_vjp
@@ -291,7 +299,7 @@ def adjoint_phase_curve(plan: Plan, cfg: RolloutConfig, ckpts, obs_steps, phases
     mus = plan.mus_device()
     return _adjoint_observed(
         plan, cfg, ckpts, obs_steps,
-        lambda k: phase_curve_vjp(g_flux[:, k], raw[k], phases[k], plan.J, plan.I, normalize, mus))
+        lambda k: phase_curve_vjp(g_flux[:, k], raw[k], phases[k], plan.J, plan.I, normalize, mus), final)
 
 
 def _check_steps(obs_steps):
@@ -310,6 +318,27 @@ def _fold(g, shape):
     return g.reshape(shape)
 
 
+def _host(x):
+    return None if x is None else x.detach().cpu().numpy()
+
+
+def _diffusion_result(plan: Plan, ctx):
+    """(dL/dK6, dL/dK6Phi) folded onto the parameters' shapes, or (None, None) when not asked."""
+    if ctx.final is None:
+        return None, None
+    dg = plan.adjoint_diffusion_result()
+    sk, skp = ctx.k6_shapes
+    # With K6Phi following K6, K6 also scales the Phi factor.
+    gk6 = _fold(dg[:, 0] + dg[:, 1] if skp is None else dg[:, 0], sk)
+    return gk6, None if skp is None else _fold(dg[:, 1], skp)
+
+
+def _save_k6(ctx, plan: Plan, K6, K6Phi) -> None:
+    # The diffusion partials need the state after the last step.
+    ctx.final = plan.state_get() if any(ctx.needs_input_grad[4:6]) else None
+    ctx.k6_shapes = (K6.shape, None if K6Phi is None else K6Phi.shape)
+
+
 class _Rollout:
     """torch.autograd.Function factory (torch imported lazily)."""
 
@@ -324,37 +353,28 @@ class _Rollout:
         class Fn(torch.autograd.Function):
             @staticmethod
             def forward(ctx, c, taurad, taudrag, DPhieq, K6, K6Phi, cfg: RolloutConfig):
-                host = lambda x: None if x is None else x.detach().cpu().numpy()  # noqa: E731
-                params = _params(cfg, host(taurad), host(taudrag), host(DPhieq), host(K6), host(K6Phi))
+                params = _params(cfg, _host(taurad), _host(taudrag), _host(DPhieq), _host(K6), _host(K6Phi))
                 plan = _make_plan(cfg, params)
                 ckpts = forward_with_checkpoints(plan, cfg, c.detach())
                 phi = plan.export(fields=("Phi_curr",))["Phi_curr"]
-                # The diffusion partials need the state after the last step.
-                ctx.final = plan.state_get() if any(ctx.needs_input_grad[4:6]) else None
+                _save_k6(ctx, plan, K6, K6Phi)
                 torch.cuda.current_stream(plan.device).synchronize()
                 ctx.plan, ctx.cfg, ctx.ckpts = plan, cfg, ckpts
-                ctx.shapes = (taurad.shape, taudrag.shape, DPhieq.shape, K6.shape,
-                              None if K6Phi is None else K6Phi.shape)
+                ctx.shapes = (taurad.shape, taudrag.shape, DPhieq.shape)
                 return phi
 
             @staticmethod
             def backward(ctx, gPhi):
                 plan, cfg = ctx.plan, ctx.cfg
                 lam, pg = adjoint_through_checkpoints(plan, cfg, ctx.ckpts, gPhi.contiguous(), ctx.final)
-                gk6 = gk6p = None
-                if ctx.final is not None:
-                    dg = plan.adjoint_diffusion_result()
-                    sk, skp = ctx.shapes[3:]
-                    # With K6Phi following K6, K6 also scales the Phi factor.
-                    gk6 = _fold(dg[:, 0] + dg[:, 1] if skp is None else dg[:, 0], sk)
-                    gk6p = None if skp is None else _fold(dg[:, 1], skp)
+                gk6, gk6p = _diffusion_result(plan, ctx)
                 torch.cuda.current_stream(plan.device).synchronize()
                 cfg.stats["plan_device_bytes"] = plan.device_bytes()
                 _release_plan(cfg, plan)
                 ctx.plan = None
                 ctx.ckpts = None
                 ctx.final = None
-                sr, sd, sp = ctx.shapes[:3]
+                sr, sd, sp = ctx.shapes
                 return lam, _fold(pg[:, 0], sr), _fold(pg[:, 1], sd), _fold(pg[:, 2], sp), gk6, gk6p, None
 
         cls._cls = Fn
@@ -374,11 +394,11 @@ class _RolloutSeries:
 
         class Fn(torch.autograd.Function):
             @staticmethod
-            def forward(ctx, c, taurad, taudrag, DPhieq, cfg: RolloutConfig, obs_steps):
-                params = _params(cfg, taurad.detach().cpu().numpy(), taudrag.detach().cpu().numpy(),
-                                 DPhieq.detach().cpu().numpy())
+            def forward(ctx, c, taurad, taudrag, DPhieq, K6, K6Phi, cfg: RolloutConfig, obs_steps):
+                params = _params(cfg, _host(taurad), _host(taudrag), _host(DPhieq), _host(K6), _host(K6Phi))
                 plan = _make_plan(cfg, params)
                 ckpts, phis = forward_series(plan, cfg, c.detach(), obs_steps)
+                _save_k6(ctx, plan, K6, K6Phi)
                 torch.cuda.current_stream(plan.device).synchronize()
                 ctx.plan, ctx.cfg, ctx.ckpts, ctx.obs = plan, cfg, ckpts, obs_steps
                 ctx.shapes = (taurad.shape, taudrag.shape, DPhieq.shape)
@@ -387,13 +407,15 @@ class _RolloutSeries:
             @staticmethod
             def backward(ctx, gPhi):
                 plan, cfg = ctx.plan, ctx.cfg
-                lam, pg = adjoint_series(plan, cfg, ctx.ckpts, ctx.obs, gPhi.contiguous())
+                lam, pg = adjoint_series(plan, cfg, ctx.ckpts, ctx.obs, gPhi.contiguous(), ctx.final)
+                gk6, gk6p = _diffusion_result(plan, ctx)
                 torch.cuda.current_stream(plan.device).synchronize()
                 _release_plan(cfg, plan)
                 ctx.plan = None
                 ctx.ckpts = None
+                ctx.final = None
                 sr, sd, sp = ctx.shapes
-                return lam, _fold(pg[:, 0], sr), _fold(pg[:, 1], sd), _fold(pg[:, 2], sp), None, None
+                return lam, _fold(pg[:, 0], sr), _fold(pg[:, 1], sd), _fold(pg[:, 2], sp), gk6, gk6p, None, None
 
         cls._cls = Fn
         return Fn
@@ -412,11 +434,11 @@ class _RolloutPhaseCurve:
 
         class Fn(torch.autograd.Function):
             @staticmethod
-            def forward(ctx, c, taurad, taudrag, DPhieq, cfg: RolloutConfig, obs_steps, phases, normalize):
-                params = _params(cfg, taurad.detach().cpu().numpy(), taudrag.detach().cpu().numpy(),
-                                 DPhieq.detach().cpu().numpy())
+            def forward(ctx, c, taurad, taudrag, DPhieq, K6, K6Phi, cfg: RolloutConfig, obs_steps, phases, normalize):
+                params = _params(cfg, _host(taurad), _host(taudrag), _host(DPhieq), _host(K6), _host(K6Phi))
                 plan = _make_plan(cfg, params)
                 ckpts, flux, raw = forward_phase_curve(plan, cfg, c.detach(), obs_steps, phases, normalize)
+                _save_k6(ctx, plan, K6, K6Phi)
                 torch.cuda.current_stream(plan.device).synchronize()
                 ctx.plan, ctx.cfg, ctx.ckpts, ctx.obs = plan, cfg, ckpts, obs_steps
                 ctx.phases, ctx.normalize, ctx.raw = phases, normalize, raw
@@ -427,14 +449,16 @@ class _RolloutPhaseCurve:
             def backward(ctx, g_flux):
                 plan, cfg = ctx.plan, ctx.cfg
                 lam, pg = adjoint_phase_curve(plan, cfg, ctx.ckpts, ctx.obs, ctx.phases, ctx.normalize, ctx.raw,
-                                              g_flux.contiguous())
+                                              g_flux.contiguous(), ctx.final)
+                gk6, gk6p = _diffusion_result(plan, ctx)
                 torch.cuda.current_stream(plan.device).synchronize()
                 _release_plan(cfg, plan)
                 ctx.plan = None
                 ctx.ckpts = None
+                ctx.final = None
                 sr, sd, sp = ctx.shapes
-                return (lam, _fold(pg[:, 0], sr), _fold(pg[:, 1], sd), _fold(pg[:, 2], sp), None, None, None,
-                        None)
+                return (lam, _fold(pg[:, 0], sr), _fold(pg[:, 1], sd), _fold(pg[:, 2], sp), gk6, gk6p, None, None,
+                        None, None)
 
         cls._cls = Fn
         return Fn
@@ -467,7 +491,8 @@ def rollout_phi_series(c, taurad, taudrag, DPhieq, *, obs_steps, M: int, dt: flo
     """Phi grids (B, len(obs_steps), J, I) after each step count in obs_steps (ascending, >= 0) of one
     differentiable rollout of max(obs_steps) steps: a loss over the whole time series (a phase-curve
     likelihood) gets its gradient from one reverse sweep whose observation cotangents are added
-    as the sweep passes them (swx_adjoint_observe), with the checkpoints of rollout_phi.
+    as the sweep passes them (swx_adjoint_observe), with the checkpoints and the parameter tensors
+    (K6 / K6Phi included) of rollout_phi.
     """
     torch = require_gpu()
     steps = _check_steps(obs_steps)
@@ -476,7 +501,8 @@ def rollout_phi_series(c, taurad, taudrag, DPhieq, *, obs_steps, M: int, dt: flo
                         alpha=alpha, precision=precision, checkpoint_every=checkpoint_every, diffflag=diffflag,
                         modalflag=modalflag, forcflag=forcflag, stats=stats if stats is not None else {})
     conv = lambda x: x if torch.is_tensor(x) else torch.tensor(x, dtype=torch.float64, device=dev)  # noqa: E731
-    return _RolloutSeries.get().apply(c, conv(taurad), conv(taudrag), conv(DPhieq), cfg, tuple(steps))
+    return _RolloutSeries.get().apply(c, conv(taurad), conv(taudrag), conv(DPhieq), conv(K6),
+                                      None if K6Phi is None else conv(K6Phi), cfg, tuple(steps))
 
 
 def rollout_phase_curve(c, taurad, taudrag, DPhieq, *, obs_steps, phase_angles, normalize: bool = True, M: int,
@@ -504,5 +530,6 @@ def rollout_phase_curve(c, taurad, taudrag, DPhieq, *, obs_steps, phase_angles, 
                         alpha=alpha, precision=precision, checkpoint_every=checkpoint_every, diffflag=diffflag,
                         modalflag=modalflag, forcflag=forcflag, stats=stats if stats is not None else {})
     conv = lambda x: x if torch.is_tensor(x) else torch.tensor(x, dtype=torch.float64, device=dev)  # noqa: E731
-    return _RolloutPhaseCurve.get().apply(c, conv(taurad), conv(taudrag), conv(DPhieq), cfg, tuple(steps),
+    return _RolloutPhaseCurve.get().apply(c, conv(taurad), conv(taudrag), conv(DPhieq), conv(K6),
+                                          None if K6Phi is None else conv(K6Phi), cfg, tuple(steps),
                                           ph.contiguous(), bool(normalize))

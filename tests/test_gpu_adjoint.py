@@ -65,7 +65,7 @@ def grads():
     phi = rollout_phi(c, tr, td, dp, **CFG, stats=stats)
     L = _loss(phi, tgt)
     L.backward()
-    return dict(c=c.detach(), tgt=tgt, L=float(L), gc=c.grad.detach(), gtr=tr.grad.detach().cpu().numpy(),
+    return dict(c=c.detach(), tgt=tgt, L=float(L.detach()), gc=c.grad.detach(), gtr=tr.grad.detach().cpu().numpy(),
                 gtd=td.grad.detach().cpu().numpy(), gdp=dp.grad.detach().cpu().numpy(), stats=stats)
 
 
@@ -312,8 +312,9 @@ def test_mfma_adjoint_matches_plain_fp32(kernel, M, torch_gpu):
 def test_time_series_rollout_gradient_is_the_sum_of_terminal_gradients(torch_gpu):
     """rollout_phi_series (observation cotangents added mid-sweep by swx_adjoint_observe) vs one
     rollout per observed step: the observed fields are identical, and the gradient of a loss
-    summed over the series equals the sum of the single-observation gradients (1e-10 relative,
-    FP64; the terminal gradients are the ones pinned by finite differences above)."""
+    summed over the series equals the sum of the single-observation gradients, K6 and K6Phi
+    included (1e-10 relative, FP64; the terminal gradients are the ones pinned by finite
+    differences above)."""
     from my_swampe_amd.autograd import rollout_phi, rollout_phi_series
 
     torch = torch_gpu
@@ -327,10 +328,10 @@ def test_time_series_rollout_gradient_is_the_sum_of_terminal_gradients(torch_gpu
 
     def leaves():
         return (c0.clone().requires_grad_(True),) + tuple(
-            torch.tensor(x, device=dev, requires_grad=True) for x in (TAURAD, TAUDRAG, DPHIEQ))
+            torch.tensor(x, device=dev, requires_grad=True) for x in (TAURAD, TAUDRAG, DPHIEQ, K6S, K6PHIS))
 
     x = leaves()
-    phis = rollout_phi_series(*x, obs_steps=obs, checkpoint_every=3, **cfg)
+    phis = rollout_phi_series(*x[:4], K6=x[4], K6Phi=x[5], obs_steps=obs, checkpoint_every=3, **cfg)
     sum(_loss(phis[:, k], tgts[k]) for k in range(len(obs))).backward()
     series = [t.grad.detach().clone() for t in x]
 
@@ -338,14 +339,15 @@ def test_time_series_rollout_gradient_is_the_sum_of_terminal_gradients(torch_gpu
     for k, n in enumerate(obs):
         y = leaves()
         if n > 0:
-            phi = rollout_phi(*y, nsteps=n, checkpoint_every=3, **cfg)
+            phi = rollout_phi(*y[:4], K6=y[4], K6Phi=y[5], nsteps=n, checkpoint_every=3, **cfg)
         else:  # the initial field: a zero-step series
-            phi = rollout_phi_series(*y, obs_steps=(0,), **cfg)[:, 0]
+            phi = rollout_phi_series(*y[:4], K6=y[4], K6Phi=y[5], obs_steps=(0,), **cfg)[:, 0]
         assert torch.equal(phi.detach(), phis[:, k].detach()), n
         _loss(phi, tgts[k]).backward()
         for acc, t in zip(total, y):
             if t.grad is not None:
                 acc += t.grad
+    assert np.abs(series[4].cpu().numpy()).min() > 0 and np.abs(series[5].cpu().numpy()).min() > 0
     for i, (a, b) in enumerate(zip(series, total)):
         a, b = a.cpu().numpy(), b.cpu().numpy()
         np.testing.assert_allclose(a, b, rtol=0, atol=1e-10 * np.abs(b).max(), err_msg=str(i))

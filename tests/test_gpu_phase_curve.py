@@ -222,3 +222,32 @@ def test_rollout_phase_curve_parameter_gradients_vs_central_differences(torch_gp
             fd = (8.0 * (L(name, b, h) - L(name, b, -h)) - (L(name, b, 2 * h) - L(name, b, -2 * h))) / (12.0 * h)
             an = grads[name][b]
             assert abs(fd - an) <= 1e-6 * max(abs(an), 1e-30) + 1e-9 * abs(fd), (name, b, fd, an)
+
+
+def test_rollout_phase_curve_hyperdiffusion_gradient_vs_central_differences(torch_gpu):
+    """dL/dK6 of the phase-curve chi^2 for a K6 shared by both members (a 0-d tensor: the members'
+    partials summed; K6Phi follows K6) against fourth-order central differences, within 1e-6."""
+    from my_swampe_amd.autograd import rollout_phase_curve
+
+    torch = torch_gpu
+    print("[gpu-case] rollout phase curve K6 finite differences", flush=True)
+    dev = torch.device("cuda")
+    c = torch.tensor(np.stack([_ic(seed=5), _ic(seed=6)]), device=dev)
+    ph = torch.tensor(_phases(), device=dev)
+    data = torch.tensor(_data(), device=dev)
+    params = [torch.tensor(x, device=dev) for x in (TAURAD, TAUDRAG, DPHIEQ)]
+    K6 = 1.24e33
+    k6 = torch.tensor(K6, dtype=torch.float64, device=dev, requires_grad=True)
+    _chi2(rollout_phase_curve(c, *params, K6=k6, obs_steps=OBS, phase_angles=ph, checkpoint_every=2, **CFG),
+          data).backward()
+    an = float(k6.grad)
+
+    def L(h):
+        with torch.no_grad():
+            f = rollout_phase_curve(c, *params, K6=K6 + h, obs_steps=OBS, phase_angles=ph, **CFG)
+        return float(_chi2(f, data))
+
+    h = 1e-3 * K6
+    fd = (8.0 * (L(h) - L(-h)) - (L(2 * h) - L(-2 * h))) / (12.0 * h)
+    print("phase-curve dchi2/dK6 adjoint", an, "fd", fd)
+    assert an != 0.0 and abs(fd - an) <= 1e-6 * abs(an), (fd, an)
