@@ -457,3 +457,19 @@ def test_explicit_drag_only_forcing():
             np.testing.assert_allclose(e_inc, 0.0, rtol=0, atol=1e-12 * scale)
         if U is z:
             np.testing.assert_allclose(e_inc, -k * zeta_mn, rtol=0, atol=1e-12 * scale)
+
+
+def test_robert_asselin_weight_reaches_only_the_prev_level():
+    """The modified-Euler scan reads the current levels only, so the Robert-Asselin weight alpha
+    (the RA filter of _step_once, model.py:795-1194, on the physical prev level) leaves every current level
+    bit for bit unchanged: the gradient of a loss of Phi_curr in alpha is zero.  The prev level
+    does move."""
+    import numpy as np
+
+    from oracle import swampe_oracle as orc
+
+    runs = [orc.run_model_scan(M=42, dt=600.0, tmax=8, Phibar=3.0e5, omega=7.292e-5, a=6.37122e6, alpha=al,
+                               diagnostics=False, return_history=False)["last_state"] for al in (0.01, 0.3)]
+    for f in ("Phi_curr", "eta_curr", "delta_curr"):
+        assert np.array_equal(getattr(runs[0], f), getattr(runs[1], f)), f
+    assert not np.array_equal(runs[0].Phi_prev, runs[1].Phi_prev)
